@@ -1,0 +1,185 @@
+/* C ABI of liblexhip.so: MI355X (gfx950) kernels for LEX's per-step dynamics.
+ *
+ * Conventions
+ *  - Arrays are FP32 device pointers in the reference's padded C-order layout (x, y, z),
+ *    z contiguous, ghost widths (3, 3, 1):
+ *        scalar (nx+6, ny+6, nz+2)   u (nx+7, ny+6, nz+2)
+ *        v      (nx+6, ny+7, nz+2)   w (nx+6, ny+6, nz+3)
+ *    "interior" arrays are unpadded: scalars (nx, ny, nz), u-points (nx+1, ny, nz),
+ *    v-points (nx, ny+1, nz), w-points (nx, ny, nz+1).
+ *  - The base state is horizontally uniform (setup_lex.py:76-137); entry points take
+ *    its profiles as z-columns of nz+2 values (*_col).
+ *  - Every call is asynchronous on `stream` (a hipStream_t; NULL = default stream).
+ *    The caller owns all memory; nothing is retained after return.
+ *  - Return value: 0 = ok, <0 = argument/shape error, >0 = HIP error code.
+ *    lex_last_error() returns the message of the last failure on this thread.
+ *
+ * Each declaration names the reference routine it replaces (file:line in
+ * MetLab-HKUST/LEX).  The reference is pure JAX with no FFI of its own; the Python
+ * modules in lex_amd/ keep the reference's function signatures and call these.
+ */
+#ifndef LEX_CAPI_H
+#define LEX_CAPI_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lex_grid {
+  int nx, ny, nz;      /* interior cells (namelist_n_constants.py:20-22) */
+  float dx, dy, dz;    /* uniform spacings (namelist_n_constants.py:17-19) */
+} lex_grid_t;
+
+const char* lex_last_error(void);
+int lex_abi_version(void);
+
+/* ---- H1: halo padding --------------------------------------------------------- */
+/* padding3_array / padding3u_array / padding3v_array (one_step_integration.py:950-985):
+ * mode 0 scalar or w (ni, nj, nk) -> (ni+6, nj+6, nk+2); mode 1 u (ni=nx+1);
+ * mode 2 v (nj=ny+1).  pad = 3.  padding_array (pressure_equations.py:111-120): pad = 1. */
+int lex_pad(int mode, int ni, int nj, int nk, int pad, const float* in, float* out, void* stream);
+/* set_w_bc (boundary_conditions.py:104-109), in place */
+int lex_set_w_bc(const lex_grid_t* g, float* w, void* stream);
+
+/* ---- A1: divergences ------------------------------------------------------------ */
+/* get_divergence (advection.py:200-226) -> padded scalar array */
+int lex_div_rho(const lex_grid_t* g, const float* rho0_col, const float* u, const float* v, const float* w,
+                float* out, void* stream);
+/* get_divergence2 (advection.py:229-236) -> padded scalar array */
+int lex_div(const lex_grid_t* g, const float* u, const float* v, const float* w, float* out, void* stream);
+
+/* ---- A2-A4: advection ------------------------------------------------------------ */
+/* advection_scalar (advection.py:9-34); sfc_flux (nx, ny) or NULL -> out interior */
+int lex_adv_scalar(const lex_grid_t* g, const float* rho0_col, const float* s, const float* u, const float* v,
+                   const float* w, float weps, const float* flow_div, const float* sfc_flux, float* out, void* stream);
+/* advection_u (advection.py:37-74); sfc_flux (nx, ny) or NULL -> (nx+1, ny, nz) */
+int lex_adv_u(const lex_grid_t* g, const float* rho0_col, const float* u, const float* v, const float* w, float weps,
+              const float* flow_div, const float* sfc_flux, float* out, void* stream);
+/* advection_v (advection.py:77-114) -> (nx, ny+1, nz) */
+int lex_adv_v(const lex_grid_t* g, const float* rho0_col, const float* u, const float* v, const float* w, float weps,
+              const float* flow_div, const float* sfc_flux, float* out, void* stream);
+/* advection_w (advection.py:117-149) -> (nx, ny, nz+1), zero at surface and lid */
+int lex_adv_w(const lex_grid_t* g, const float* rho0_col, const float* u, const float* v, const float* w, float weps,
+              const float* flow_div, float* out, void* stream);
+/* advection_pi (advection.py:152-197) -> interior */
+int lex_adv_pi(const lex_grid_t* g, const float* rho0_col, const float* pi0_col, const float* pip, const float* u,
+               const float* v, const float* w, float* out, void* stream);
+
+/* ---- B1: buoyancy --------------------------------------------------------------- */
+/* calculate_buoyancy (pressure_equations.py:162-172) -> b interior, b8w (nx, ny, nz+1) */
+int lex_buoyancy(const lex_grid_t* g, const float* theta0_col, const float* qv0_col, const float* theta,
+                 const float* qv, float* b, float* b8w, void* stream);
+/* density potential temperature theta(1+Rv/Rd qv)/(1+qv) (one_step_integration.py:135), n elements */
+int lex_theta_rho(long n, const float* theta, const float* qv, float* out, void* stream);
+
+/* ---- P1/P2: pressure gradient, Coriolis ----------------------------------------- */
+/* pressure_gradient_force (pressure_gradient_coriolis.py:7-37) -> u, v, w interiors */
+int lex_pgf(const lex_grid_t* g, const float* pip, const float* theta_rho, float* pu, float* pv, float* pw,
+            void* stream);
+/* calculate_coriolis_force (pressure_gradient_coriolis.py:80-95) -> fu (nx,ny+1,nz), fv (nx+1,ny,nz) */
+int lex_coriolis(const lex_grid_t* g, const float* u, const float* v, float fcor, float* fu, float* fv, void* stream);
+
+/* ---- P3/P4: pressure equation ---------------------------------------------------- */
+/* calculate_rtt (pressure_equations.py:175-195); buoyancy interior -> padded rtt */
+int lex_rtt(const lex_grid_t* g, const float* rho0_theta0_col, const float* theta, const float* qv,
+            const float* buoyancy, float* rtt, void* stream);
+/* rhs_of_pressure_equation (pressure_equations.py:51-77); fu/fv may be NULL (zero) */
+int lex_p_rhs(const lex_grid_t* g, const float* rho0_theta0_col, const float* adv4u, const float* adv4v,
+              const float* adv4w, const float* fu, const float* fv, const float* buoyancy, float* rhs, void* stream);
+/* laplace_of_pressure (pressure_equations.py:8-48), affine */
+int lex_p_matvec(const lex_grid_t* g, const float* rtt, const float* x, float* out, void* stream);
+/* jax.scipy.sparse.linalg.bicgstab as called by solve_pres_eqn
+ * (one_step_integration.py:468-487): FP64 reductions, device-side loop control.
+ * *k_out receives the iteration count or -10/-11 (breakdown); k_out may be NULL
+ * (then the call does not synchronise the stream at the end). */
+size_t lex_bicgstab_workspace_bytes(const lex_grid_t* g);
+int lex_bicgstab(const lex_grid_t* g, const float* rtt, const float* rhs, const float* x0, float* x, double tol,
+                 double atol, int maxiter, void* workspace, size_t workspace_bytes, int* k_out, void* stream);
+
+/* ---- S1/M1/E1: acoustic substep and Euler updates --------------------------------- */
+/* one pass of the loop in sound_rk_step0/1/2 (one_step_integration.py:528-553):
+ * (u,v,w,pip) -> (u_out,...); rhs_* interior; pip_star, theta_rho padded */
+int lex_acoustic_substep(const lex_grid_t* g, const float* rho0_col, const float* pi0_col, const float* u,
+                         const float* v, const float* w, const float* pip, const float* pip_star, const float* rhs_u,
+                         const float* rhs_v, const float* rhs_w, const float* theta_rho, float dt_sound, float* u_out,
+                         float* v_out, float* w_out, float* pip_out, void* stream);
+/* update_theta_euler / update_qv_euler (one_step_integration.py:450-465) plus the SGS
+ * add-in (:158-162): out = pad(s0 + (adv+heating)*dt) + pad(sgs*dt).  heating, sgs,
+ * sfc_flux and tend_out may be NULL. */
+int lex_scalar_update(const lex_grid_t* g, const float* rho0_col, const float* s0, const float* s, const float* u,
+                      const float* v, const float* w, const float* flow_div, const float* sfc_flux,
+                      const float* heating, const float* sgs, float dt, float weps, float* out, float* tend_out,
+                      void* stream);
+/* update_momentum_eqn_euler (one_step_integration.py:876-891) plus SGS add-in
+ * (:113-118); fu, fv, sgs_* may be NULL; du/dv/dw_out (interior tendencies incl. SGS,
+ * as rk_sub_step0 returns them) may be NULL */
+int lex_momentum_update(const lex_grid_t* g, const float* u0, const float* v0, const float* w0, const float* pip,
+                        const float* theta_rho, const float* adv4u, const float* adv4v, const float* adv4w,
+                        const float* fu, const float* fv, const float* b8w, const float* sgs_u, const float* sgs_v,
+                        const float* sgs_w, float dt, float* u_out, float* v_out, float* w_out, float* du_out,
+                        float* dv_out, float* dw_out, void* stream);
+
+/* ---- G1: Smagorinsky -------------------------------------------------------------- */
+/* compute_deformation (turbulence.py:298-388) -> 7 padded arrays */
+int lex_smag_deformation(const lex_grid_t* g, const float* rho0_col, const float* u, const float* v, const float* w,
+                         float* s11, float* s22, float* s33, float* s12, float* s13, float* s23, float* deform,
+                         void* stream);
+/* compute_nm (turbulence.py:391-427) -> padded n2 */
+int lex_smag_nm(const lex_grid_t* g, const float* theta, const float* qv, float* n2, void* stream);
+/* compute_k (turbulence.py:430-446); cs = 0.18 and prandtl = 1/3 in the reference */
+int lex_smag_k(const lex_grid_t* g, const float* deform, const float* n2, float cs, float prandtl, float* km,
+               float* kh, void* stream);
+/* compute_smag (turbulence.py:9-111): momentum part -> sgs_u, sgs_v, sgs_w */
+int lex_smag_momentum(const lex_grid_t* g, const float* rho0_col, const float* km, const float* s11,
+                      const float* s22, const float* s33, const float* s12, const float* s13, const float* s23,
+                      float* sgs_u, float* sgs_v, float* sgs_w, void* stream);
+/* compute_smag (turbulence.py:80-109): one scalar's SGS tendency (interior) */
+int lex_smag_scalar(const lex_grid_t* g, const float* rho0_col, const float* kh, const float* s, float* out,
+                    void* stream);
+
+/* ---- I1: hydrostatic base state (FP64 on the GPU) ---------------------------------- */
+/* setup_ic_option1 (setup_lex.py:103-137): rh_col (nz+2 doubles, device) ->
+ * out (5*(nz+2)+1 doubles, device): pi0, rho0_theta0, rho0, theta0, qv0, then the
+ * iteration count; *iters (host, may be NULL) receives it after a stream sync. */
+int lex_base_state(int nz, double dz, double theta0, const double* rh_col, double* out, int max_iter, double tol,
+                   int* iters, void* stream);
+
+/* ---- R1/R2: fused WS-RK3 step (solver_opt 2, compressible) ----------------------- */
+typedef struct lex_step_opts {
+  float dt;          /* full step (namelist dt) */
+  int n_sound;       /* acoustic substeps per full step (namelist n_sound) */
+  int turb;          /* 1 = Smagorinsky (turb_opt 1), 0 = off */
+  float cs;          /* Smagorinsky constant (turbulence.py:433) */
+  float prandtl;     /* turbulence.py:432 */
+  int n_passive;     /* passive scalars advected and diffused like qv (qc, qr) */
+} lex_step_opts_t;
+
+typedef struct lex_base_cols {
+  const float* rho0;
+  const float* theta0;
+  const float* qv0;
+  const float* pi0;
+  const float* rho0_theta0;
+} lex_base_cols_t;
+
+/* ws_rk3_big_step (one_sprint_functions.py:10-30) for solver_opt 2: advances the state
+ * in place.  state[] = theta, u, v, w, pip, qv, then n_passive scalars (all padded). */
+size_t lex_rk3_workspace_bytes(const lex_grid_t* g, int n_passive);
+int lex_rk3_step(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b, float* const* state,
+                 void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---- measurement (no reference counterpart: replaces the wall-clock table of
+ * lex_relay.py:65-75 with per-kernel device timing) ------------------------------ */
+/* Bracket every kernel the step driver launches with a HIP event pair (pool of
+ * capacity_pairs, default 8192 when <= 0).  Process-global, single-threaded use. */
+int lex_profile_enable(int on, int capacity_pairs);
+/* Synchronise, sum elapsed ms and launch counts per kernel kind into ms[0..n),
+ * counts[0..n) and reset the pool; returns the number of kernel kinds. */
+int lex_profile_read(double* ms, int* counts, int n);
+const char* lex_profile_kernel_name(int id);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LEX_CAPI_H */

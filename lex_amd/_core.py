@@ -1,0 +1,88 @@
+"""Shared host helpers for the reference-mirroring modules.
+
+Shapes follow the reference's padded layout (ghost widths 3, 3, 1); grid spacings are
+read from the reference's coordinate arrays (x3d, x3d4u, ...), which are uniform on
+every LEX grid (setup_lex.py:12-24).  The base state is horizontally uniform
+(setup_lex.py:76-137): kernels receive its z-profiles, extracted once per tensor.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+
+NGX, NGY, NGZ = 3, 3, 1
+
+
+def dims_from_scalar(a) -> tuple[int, int, int]:
+    """(nx, ny, nz) from a padded scalar-shaped array."""
+    return a.shape[0] - 6, a.shape[1] - 6, a.shape[2] - 2
+
+
+def _axis_spacing(coord, axis) -> float:
+    """Spacing of a coordinate array along `axis` (numpy, broadcast view or tensor)."""
+    idx0 = [0, 0, 0]
+    idx1 = [0, 0, 0]
+    idx1[axis] = 1
+    a = coord[tuple(idx0)]
+    b = coord[tuple(idx1)]
+    return float(b) - float(a)
+
+
+def grid_from(shape_src, x=None, y=None, z=None, dx=None, dy=None, dz=None) -> _lib.Grid:
+    """Grid descriptor from a padded scalar-shaped array and coordinate arrays or spacings."""
+    nx, ny, nz = dims_from_scalar(shape_src)
+    if dx is None:
+        dx = _axis_spacing(x, 0)
+    if dy is None:
+        dy = _axis_spacing(y, 1)
+    if dz is None:
+        dz = _axis_spacing(z, 2)
+    return _lib.grid(nx, ny, nz, dx, dy, dz)
+
+
+_COL_CACHE: dict = {}
+
+
+def column(t: torch.Tensor) -> torch.Tensor:
+    """z-profile (nz+2,) of a horizontally uniform padded scalar field (checked once per tensor version)."""
+    if t.dim() == 1:
+        return t.contiguous()
+    key = (t.data_ptr(), tuple(t.shape), t._version)
+    hit = _COL_CACHE.get(key)
+    if hit is not None:
+        return hit
+    col = t[NGX, NGY, :].contiguous()
+    if not bool(torch.equal(t, col.expand_as(t))):
+        raise ValueError("base-state field is not horizontally uniform (the reference assumes it is: "
+                         "pressure_gradient_coriolis.py:12)")
+    if len(_COL_CACHE) > 64:
+        _COL_CACHE.clear()
+    _COL_CACHE[key] = col
+    return col
+
+
+def zeros(*shape, like: torch.Tensor):
+    return torch.zeros(shape, dtype=torch.float32, device=like.device)
+
+
+def empty(*shape, like: torch.Tensor):
+    return torch.empty(shape, dtype=torch.float32, device=like.device)
+
+
+def as_dev(a, like: torch.Tensor | None = None, device="cuda"):
+    """numpy / tensor -> contiguous float32 device tensor."""
+    if isinstance(a, torch.Tensor):
+        t = a
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(a))
+    dev = like.device if like is not None else device
+    return t.to(device=dev, dtype=torch.float32).contiguous()
+
+
+def check_shape(t: torch.Tensor, shape, name):
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: expected shape {tuple(shape)}, got {tuple(t.shape)}")
+    if t.dtype != torch.float32:
+        raise ValueError(f"{name}: expected float32, got {t.dtype}")

@@ -1,0 +1,169 @@
+"""ctypes binding of liblexhip.so (the C ABI in include/lex_capi.h).
+
+The product path runs only through this library: if it is missing or a call fails,
+an exception is raised.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblexhip.so")
+CSRC = os.path.join(_HERE, "csrc")
+HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+class LexError(RuntimeError):
+    """A liblexhip call returned a non-zero status."""
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("dx", C.c_float), ("dy", C.c_float), ("dz", C.c_float)]
+
+
+class StepOpts(C.Structure):
+    _fields_ = [("dt", C.c_float), ("n_sound", C.c_int), ("turb", C.c_int), ("cs", C.c_float),
+                ("prandtl", C.c_float), ("n_passive", C.c_int)]
+
+
+class BaseCols(C.Structure):
+    _fields_ = [("rho0", C.c_void_p), ("theta0", C.c_void_p), ("qv0", C.c_void_p), ("pi0", C.c_void_p),
+                ("rho0_theta0", C.c_void_p)]
+
+
+def build(verbose: bool = False) -> str:
+    """Compile liblexhip.so for gfx950 in-tree (one hipcc invocation, unity build)."""
+    src = os.path.join(CSRC, "lex_capi.hip")
+    out = LIB_PATH + ".tmp"
+    cmd = ["hipcc", *HIPCC_FLAGS, "-o", out, src]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise LexError("hipcc failed:\n" + r.stdout + r.stderr)
+    os.replace(out, LIB_PATH)
+    if verbose:
+        print(" ".join(cmd))
+    return LIB_PATH
+
+
+_lib = None
+
+_P = C.c_void_p
+_I = C.c_int
+_F = C.c_float
+_D = C.c_double
+_G = C.POINTER(Grid)
+
+# name -> argtypes (all return int unless listed in _RESTYPE)
+_SIGS = {
+    "lex_last_error": [],
+    "lex_abi_version": [],
+    "lex_pad": [_I, _I, _I, _I, _I, _P, _P, _P],
+    "lex_set_w_bc": [_G, _P, _P],
+    "lex_div_rho": [_G, _P, _P, _P, _P, _P, _P],
+    "lex_div": [_G, _P, _P, _P, _P, _P],
+    "lex_adv_scalar": [_G, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P],
+    "lex_adv_u": [_G, _P, _P, _P, _P, _F, _P, _P, _P, _P],
+    "lex_adv_v": [_G, _P, _P, _P, _P, _F, _P, _P, _P, _P],
+    "lex_adv_w": [_G, _P, _P, _P, _P, _F, _P, _P, _P],
+    "lex_adv_pi": [_G, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lex_buoyancy": [_G, _P, _P, _P, _P, _P, _P, _P],
+    "lex_theta_rho": [C.c_long, _P, _P, _P, _P],
+    "lex_pgf": [_G, _P, _P, _P, _P, _P, _P],
+    "lex_coriolis": [_G, _P, _P, _F, _P, _P, _P],
+    "lex_rtt": [_G, _P, _P, _P, _P, _P, _P],
+    "lex_p_rhs": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lex_p_matvec": [_G, _P, _P, _P, _P],
+    "lex_bicgstab_workspace_bytes": [_G],
+    "lex_bicgstab": [_G, _P, _P, _P, _P, _D, _D, _I, _P, C.c_size_t, C.POINTER(_I), _P],
+    "lex_acoustic_substep": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P],
+    "lex_scalar_update": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _F, _P, _P, _P],
+    "lex_momentum_update": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P, _P, _P],
+    "lex_smag_deformation": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lex_smag_nm": [_G, _P, _P, _P, _P],
+    "lex_smag_k": [_G, _P, _P, _F, _F, _P, _P, _P],
+    "lex_smag_momentum": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lex_smag_scalar": [_G, _P, _P, _P, _P, _P],
+    "lex_base_state": [_I, _D, _D, _P, _P, _I, _D, C.POINTER(_I), _P],
+    "lex_rk3_workspace_bytes": [_G, _I],
+    "lex_rk3_step": [_G, C.POINTER(StepOpts), C.POINTER(BaseCols), C.POINTER(_P), _P, C.c_size_t, _P],
+    "lex_profile_enable": [_I, _I],
+    "lex_profile_read": [C.POINTER(_D), C.POINTER(_I), _I],
+    "lex_profile_kernel_name": [_I],
+}
+_RESTYPE = {"lex_last_error": C.c_char_p, "lex_bicgstab_workspace_bytes": C.c_size_t,
+            "lex_rk3_workspace_bytes": C.c_size_t, "lex_profile_kernel_name": C.c_char_p}
+
+
+def profile_enable(on: bool, capacity_pairs: int = 8192):
+    call("lex_profile_enable", int(on), int(capacity_pairs))
+
+
+def profile_read(n: int = 32):
+    """{kernel name: (total ms, launches)} since the last read (synchronises)."""
+    lib = load()
+    ms = (C.c_double * n)()
+    cnt = (C.c_int * n)()
+    k = lib.lex_profile_read(ms, cnt, n)
+    if k < 0:
+        raise LexError("lex_profile_read failed: " + lib.lex_last_error().decode(errors="replace"))
+    out = {}
+    for i in range(min(k, n)):
+        if cnt[i]:
+            out[lib.lex_profile_kernel_name(i).decode()] = (ms[i], cnt[i])
+    return out
+
+
+def load():
+    """Load liblexhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LexError(f"{LIB_PATH} is missing: build it with lex_amd._lib.build() "
+                       "(or __graft_entry__.build()); there is no fallback path")
+    lib = C.CDLL(LIB_PATH)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPE.get(name, C.c_int)
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def call(name, *args):
+    """Call an entry point; raise LexError with lex_last_error() on a non-zero status."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.lex_last_error().decode(errors="replace")
+        raise LexError(f"{name} failed ({rc}): {msg}")
+    return rc
+
+
+def ptr(t: torch.Tensor | None):
+    """Device pointer of a contiguous CUDA/HIP float tensor (None -> NULL)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise LexError("liblexhip needs device tensors (got a CPU tensor)")
+    if not t.is_contiguous():
+        raise LexError("liblexhip needs contiguous tensors")
+    return t.data_ptr()
+
+
+def stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def grid(nx, ny, nz, dx, dy, dz) -> Grid:
+    return Grid(int(nx), int(ny), int(nz), float(dx), float(dy), float(dz))

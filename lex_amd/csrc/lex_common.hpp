@@ -1,0 +1,203 @@
+// Shared device helpers for the LEX dynamics kernels (gfx950, wave64).
+//
+// Layout (the reference's padded C-order, z contiguous; ghost widths 3, 3, 1):
+//   scalar  (nx+6, ny+6, nz+2)   cell (i,j,k) at I=i+3, J=j+3, K=k+1
+//   u       (nx+7, ny+6, nz+2)   face I lies between scalar cells I-1 and I
+//   v       (nx+6, ny+7, nz+2)   face J lies between scalar cells J-1 and J
+//   w       (nx+6, ny+6, nz+3)   level K lies between scalar levels K-1 and K
+// Base-state profiles are horizontally uniform; kernels take them as z-columns of
+// length nz+2 (setup_lex.py:76-137 builds them that way).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lex {
+
+// Physical constants: namelist_n_constants.py:64-75 (rounded to FP32 the way JAX
+// rounds weak Python scalars).
+constexpr double kRd = 287.04;
+constexpr double kRv = 461.5;
+constexpr double kCp = 1004.64;
+constexpr double kCv = kCp - kRd;
+constexpr double kReps = kRv / kRd;
+constexpr double kRepsm1 = kRv / kRd - 1.0;
+constexpr double kG = 9.80616;
+constexpr double kP00 = 100000.0;
+
+struct Dims {
+  int nx, ny, nz;
+  float dx, dy, dz;
+  float idx, idy, idz;  // reciprocals
+  // strides (in elements) for each staggering; z stride is 1
+  __host__ __device__ long sxc() const { return (long)(ny + 6) * (nz + 2); }
+  __host__ __device__ long syc() const { return nz + 2; }
+  __host__ __device__ long sxu() const { return (long)(ny + 6) * (nz + 2); }
+  __host__ __device__ long sxv() const { return (long)(ny + 7) * (nz + 2); }
+  __host__ __device__ long sxw() const { return (long)(ny + 6) * (nz + 3); }
+  __host__ __device__ long syw() const { return nz + 3; }
+  __host__ __device__ long nc() const { return (long)(nx + 6) * (ny + 6) * (nz + 2); }
+  __host__ __device__ long nu() const { return (long)(nx + 7) * (ny + 6) * (nz + 2); }
+  __host__ __device__ long nv() const { return (long)(nx + 6) * (ny + 7) * (nz + 2); }
+  __host__ __device__ long nw() const { return (long)(nx + 6) * (ny + 6) * (nz + 3); }
+};
+
+__device__ __forceinline__ long ic(const Dims& d, int I, int J, int K) { return I * d.sxc() + (long)J * d.syc() + K; }
+__device__ __forceinline__ long iu(const Dims& d, int I, int J, int K) { return I * d.sxu() + (long)J * d.syc() + K; }
+__device__ __forceinline__ long iv(const Dims& d, int I, int J, int K) { return I * d.sxv() + (long)J * d.syc() + K; }
+__device__ __forceinline__ long iw(const Dims& d, int I, int J, int K) { return I * d.sxw() + (long)J * d.syw() + K; }
+
+// Fast reciprocal (v_rcp_f32, 1 ulp).  Used for the WENO weight ratios and the
+// normalisation; results stay within the FP32-vs-FP64 tolerance of the reference.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// WENO5 face value from the upwind-ordered stencil (a, b, c | d, e), c upwind.
+// Jiang & Shu (1996) candidates/indicators, Borges et al. (2008) weights with the
+// squared ratio and the 1e30 cap, as advection.py:311-326.
+__device__ __forceinline__ float weno5(float a, float b, float c, float d, float e, float weps) {
+  const float c1312 = (float)(13.0 / 12.0);
+  float t1 = a - 2.0f * b + c, t2 = a - 4.0f * b + 3.0f * c;
+  float b1 = c1312 * t1 * t1 + 0.25f * t2 * t2;
+  float t3 = b - 2.0f * c + d, t4 = b - d;
+  float b2 = c1312 * t3 * t3 + 0.25f * t4 * t4;
+  float t5 = c - 2.0f * d + e, t6 = 3.0f * c - 4.0f * d + e;
+  float b3 = c1312 * t5 * t5 + 0.25f * t6 * t6;
+  float tau = fabsf(b1 - b3);
+  float r1 = fminf(1.0e30f, tau * frcp(b1 + weps));
+  float r2 = fminf(1.0e30f, tau * frcp(b2 + weps));
+  float r3 = fminf(1.0e30f, tau * frcp(b3 + weps));
+  float w1 = 0.1f * (1.0f + r1 * r1);
+  float w2 = 0.6f * (1.0f + r2 * r2);
+  float w3 = 0.3f * (1.0f + r3 * r3);
+  const float s6 = (float)(1.0 / 6.0);
+  float q1 = (float)(2.0 / 6.0) * a + (float)(-7.0 / 6.0) * b + (float)(11.0 / 6.0) * c;
+  float q2 = -s6 * b + (float)(5.0 / 6.0) * c + (float)(2.0 / 6.0) * d;
+  float q3 = (float)(2.0 / 6.0) * c + (float)(5.0 / 6.0) * d - s6 * e;
+  return (w1 * q1 + w2 * q2 + w3 * q3) * frcp(w1 + w2 + w3);
+}
+
+// WENO3 face value from (a, b | c), b upwind (advection.py:329-340).
+__device__ __forceinline__ float weno3(float a, float b, float c, float weps) {
+  float b1 = (a - b) * (a - b);
+  float b2 = (b - c) * (b - c);
+  float tau = fabsf(b1 - b2);
+  float r1 = fminf(1.0e30f, tau * frcp(b1 + weps));
+  float r2 = fminf(1.0e30f, tau * frcp(b2 + weps));
+  float w1 = (float)(1.0 / 3.0) * (1.0f + r1 * r1);
+  float w2 = (float)(2.0 / 3.0) * (1.0f + r2 * r2);
+  return (w1 * (-0.5f * a + 1.5f * b) + w2 * (0.5f * b + 0.5f * c)) * frcp(w1 + w2);
+}
+
+// Horizontal reconstruction at the flux point between q[m-1] and q[m] from the six
+// values q[m-3..m+2] (advection.py:425-470): upwind side chosen by the mass-flux sign.
+__device__ __forceinline__ float hrecon(float qm3, float qm2, float qm1, float q0, float qp1, float qp2,
+                                        bool pos, float weps) {
+  float a = pos ? qm3 : qp2;
+  float b = pos ? qm2 : qp1;
+  float c = pos ? qm1 : q0;
+  float d = pos ? q0 : qm1;
+  float e = pos ? qp1 : qm2;
+  return weno5(a, b, c, d, e, weps);
+}
+
+// Vertical reconstruction at face f of a padded column p[0..L-1] (face f lies between
+// p[f] and p[f+1]).  The table of advection.py:330-409 (scalars) and its u/v/w copies:
+//   upward:   f=1 centred, f=2 WENO3, f=3..L-4 WENO5, f=L-3 WENO3
+//   downward: f=1 WENO3, f=2..L-5 WENO5, f=L-4 WENO3, f=L-3 centred
+// Faces 0 and L-2 carry zero (boundary-condition slots).  `p` has unit stride.
+__device__ __forceinline__ float vrecon(const float* __restrict__ p, int L, int f, bool pos, float weps) {
+  if (f <= 0 || f >= L - 2) return 0.0f;
+  if (pos) {
+    if (f == 1) return 0.5f * (p[1] + p[2]);
+    if (f == 2) return weno3(p[1], p[2], p[3], weps);
+    if (f == L - 3) return weno3(p[L - 4], p[L - 3], p[L - 2], weps);
+    return weno5(p[f - 2], p[f - 1], p[f], p[f + 1], p[f + 2], weps);
+  } else {
+    if (f == 1) return weno3(p[3], p[2], p[1], weps);
+    if (f == L - 3) return 0.5f * (p[L - 2] + p[L - 3]);
+    if (f == L - 4) return weno3(p[L - 2], p[L - 3], p[L - 4], weps);
+    return weno5(p[f + 3], p[f + 2], p[f + 1], p[f], p[f - 1], weps);
+  }
+}
+
+// select(mass >= 0, mass*up, mass*dn) with only the upwind side evaluated.
+__device__ __forceinline__ bool upwind_pos(float mass) { return mass >= 0.0f; }
+
+// ----------------------------------------------------------------------------------
+// Stores that also fill the periodic / copy ghosts (padding3*_array,
+// one_step_integration.py:950-985).  Each ghost has exactly one interior source.
+// ----------------------------------------------------------------------------------
+__device__ __forceinline__ void ghost_list_x(int i, int n, int* out, int& cnt) {
+  // scalar-type axis: interior i in [0,n); padded P = i+3; images at i+3+n (i<3) and i+3-n (i>=n-3)
+  cnt = 0;
+  out[cnt++] = i + 3;
+  if (i < 3) out[cnt++] = i + 3 + n;
+  if (i >= n - 3) out[cnt++] = i + 3 - n;
+}
+
+__device__ __forceinline__ void ghost_list_face(int f, int n, int* out, int& cnt) {
+  // face-type axis (u along x, v along y): face f in [0,n]; padded f+3; ghosts skip
+  // the duplicate face: padded 0..2 <- faces n-3..n-1, padded n+4..n+6 <- faces 1..3
+  cnt = 0;
+  out[cnt++] = f + 3;
+  if (f >= n - 3 && f <= n - 1) out[cnt++] = f + 3 - n;
+  if (f >= 1 && f <= 3) out[cnt++] = f + 3 + n;
+}
+
+// scalar field: interior (i,j,k)
+__device__ __forceinline__ void store_c(const Dims& d, float* __restrict__ a, int i, int j, int k, float val) {
+  int xs[3], ys[3], zs[2];
+  int nxs, nys, nzs = 1;
+  ghost_list_x(i, d.nx, xs, nxs);
+  ghost_list_x(j, d.ny, ys, nys);
+  zs[0] = k + 1;
+  if (k == 0) zs[nzs++] = 0;
+  if (k == d.nz - 1) zs[nzs++] = d.nz + 1;
+  for (int a1 = 0; a1 < nxs; ++a1)
+    for (int b1 = 0; b1 < nys; ++b1)
+      for (int c1 = 0; c1 < nzs; ++c1) a[ic(d, xs[a1], ys[b1], zs[c1])] = val;
+}
+
+// u field: face f in [0,nx], (j,k) interior
+__device__ __forceinline__ void store_u(const Dims& d, float* __restrict__ a, int f, int j, int k, float val) {
+  int xs[3], ys[3], zs[2];
+  int nxs, nys, nzs = 1;
+  ghost_list_face(f, d.nx, xs, nxs);
+  ghost_list_x(j, d.ny, ys, nys);
+  zs[0] = k + 1;
+  if (k == 0) zs[nzs++] = 0;
+  if (k == d.nz - 1) zs[nzs++] = d.nz + 1;
+  for (int a1 = 0; a1 < nxs; ++a1)
+    for (int b1 = 0; b1 < nys; ++b1)
+      for (int c1 = 0; c1 < nzs; ++c1) a[iu(d, xs[a1], ys[b1], zs[c1])] = val;
+}
+
+// v field: face f in [0,ny], (i,k) interior
+__device__ __forceinline__ void store_v(const Dims& d, float* __restrict__ a, int i, int f, int k, float val) {
+  int xs[3], ys[3], zs[2];
+  int nxs, nys, nzs = 1;
+  ghost_list_x(i, d.nx, xs, nxs);
+  ghost_list_face(f, d.ny, ys, nys);
+  zs[0] = k + 1;
+  if (k == 0) zs[nzs++] = 0;
+  if (k == d.nz - 1) zs[nzs++] = d.nz + 1;
+  for (int a1 = 0; a1 < nxs; ++a1)
+    for (int b1 = 0; b1 < nys; ++b1)
+      for (int c1 = 0; c1 < nzs; ++c1) a[iv(d, xs[a1], ys[b1], zs[c1])] = val;
+}
+
+// w field: interior level l in [0,nz]; level l at K=l+1; z ghosts copy the nearest
+// level (padding3_array on an (nx,ny,nz+1) array).  set_w_bc zeroing is the caller's.
+__device__ __forceinline__ void store_w(const Dims& d, float* __restrict__ a, int i, int j, int l, float val) {
+  int xs[3], ys[3], zs[2];
+  int nxs, nys, nzs = 1;
+  ghost_list_x(i, d.nx, xs, nxs);
+  ghost_list_x(j, d.ny, ys, nys);
+  zs[0] = l + 1;
+  if (l == 0) zs[nzs++] = 0;
+  if (l == d.nz) zs[nzs++] = d.nz + 2;
+  for (int a1 = 0; a1 < nxs; ++a1)
+    for (int b1 = 0; b1 < nys; ++b1)
+      for (int c1 = 0; c1 < nzs; ++c1) a[iw(d, xs[a1], ys[b1], zs[c1])] = val;
+}
+
+}  // namespace lex

@@ -1,0 +1,572 @@
+// Per-operator kernels: one kernel per reference routine on the LEX hot path, one
+// thread per output point.  They are the drop-in entry points behind the Python
+// mirror modules; the fused stage kernels in lex_stage.hip reuse the device functions
+// defined here.
+#include "lex_common.hpp"
+
+namespace lex {
+
+// ----------------------------------------------------------------------------------
+// A1 pieces: mass-weighted face velocities for a horizontally uniform rho0 column.
+// get_divergence (advection.py:200-226): rho at x/y faces is 0.5*(rho+rho) = rho, at
+// w levels 0.5*(rho[K-1]+rho[K]) with zero at the padded end levels.
+// ----------------------------------------------------------------------------------
+__device__ __forceinline__ float rho8w_col(const float* __restrict__ r, int Kw, int nz) {
+  if (Kw <= 0 || Kw >= nz + 2) return 0.0f;
+  return 0.5f * (r[Kw - 1] + r[Kw]);
+}
+
+// flow divergence at padded scalar cell (I,J,K), any cell of the padded array
+__device__ __forceinline__ float flow_div_at(const Dims& d, const float* __restrict__ r, const float* __restrict__ u,
+                                            const float* __restrict__ v, const float* __restrict__ w, int I, int J, int K) {
+  float rk = r[K];
+  float dxp = (rk * u[iu(d, I + 1, J, K)] - rk * u[iu(d, I, J, K)]) * d.idx;
+  float dyp = (rk * v[iv(d, I, J + 1, K)] - rk * v[iv(d, I, J, K)]) * d.idy;
+  float dzp = (rho8w_col(r, K + 1, d.nz) * w[iw(d, I, J, K + 1)] - rho8w_col(r, K, d.nz) * w[iw(d, I, J, K)]) * d.idz;
+  return dxp + dyp + dzp;
+}
+
+__global__ void k_div_rho(Dims d, const float* __restrict__ r, const float* __restrict__ u, const float* __restrict__ v,
+                          const float* __restrict__ w, float* __restrict__ out) {
+  int K = blockIdx.x * blockDim.x + threadIdx.x;
+  int J = blockIdx.y * blockDim.y + threadIdx.y;
+  int I = blockIdx.z;
+  if (K >= d.nz + 2 || J >= d.ny + 6) return;
+  out[ic(d, I, J, K)] = flow_div_at(d, r, u, v, w, I, J, K);
+}
+
+// get_divergence2 (advection.py:229-236)
+__global__ void k_div(Dims d, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                      float* __restrict__ out) {
+  int K = blockIdx.x * blockDim.x + threadIdx.x;
+  int J = blockIdx.y * blockDim.y + threadIdx.y;
+  int I = blockIdx.z;
+  if (K >= d.nz + 2 || J >= d.ny + 6) return;
+  out[ic(d, I, J, K)] = (u[iu(d, I + 1, J, K)] - u[iu(d, I, J, K)]) * d.idx +
+                        (v[iv(d, I, J + 1, K)] - v[iv(d, I, J, K)]) * d.idy +
+                        (w[iw(d, I, J, K + 1)] - w[iw(d, I, J, K)]) * d.idz;
+}
+
+// ----------------------------------------------------------------------------------
+// A2: advection_scalar (advection.py:9-34) with vertical_flux_scalar2 (:305-416) and
+// horizontal_flux_scalar (:419-516).
+// ----------------------------------------------------------------------------------
+__device__ __forceinline__ float scalar_xflux(const Dims& d, float r, const float* __restrict__ s,
+                                              const float* __restrict__ u, int m, int J, int K, float weps) {
+  float mass = r * u[iu(d, m, J, K)];
+  long b = ic(d, m, J, K);
+  long sx = d.sxc();
+  float val = hrecon(s[b - 3 * sx], s[b - 2 * sx], s[b - sx], s[b], s[b + sx], s[b + 2 * sx], upwind_pos(mass), weps);
+  return mass * val;
+}
+
+__device__ __forceinline__ float scalar_yflux(const Dims& d, float r, const float* __restrict__ s,
+                                              const float* __restrict__ v, int I, int m, int K, float weps) {
+  float mass = r * v[iv(d, I, m, K)];
+  long b = ic(d, I, m, K);
+  long sy = d.syc();
+  float val = hrecon(s[b - 3 * sy], s[b - 2 * sy], s[b - sy], s[b], s[b + sy], s[b + 2 * sy], upwind_pos(mass), weps);
+  return mass * val;
+}
+
+// vertical flux at face f (between padded levels f and f+1) of a scalar column
+__device__ __forceinline__ float scalar_zflux(const Dims& d, const float* __restrict__ rc, const float* __restrict__ s,
+                                              const float* __restrict__ w, int I, int J, int f, float weps) {
+  float mass = 0.5f * (rc[f] + rc[f + 1]) * w[iw(d, I, J, f + 1)];
+  float val = vrecon(s + ic(d, I, J, 0), d.nz + 2, f, upwind_pos(mass), weps);
+  return mass * val;
+}
+
+__device__ float adv_scalar_point(const Dims& d, const float* __restrict__ rc, const float* __restrict__ s,
+                                  const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                                  float fdc, float sfc, int i, int j, int k, float weps) {
+  int I = i + 3, J = j + 3, K = k + 1;
+  float r = rc[K];
+  float fx0 = scalar_xflux(d, r, s, u, I, J, K, weps);
+  float fx1 = scalar_xflux(d, r, s, u, I + 1, J, K, weps);
+  float fy0 = scalar_yflux(d, r, s, v, I, J, K, weps);
+  float fy1 = scalar_yflux(d, r, s, v, I, J + 1, K, weps);
+  float fz0 = (k == 0) ? sfc : scalar_zflux(d, rc, s, w, I, J, K - 1, weps);
+  float fz1 = scalar_zflux(d, rc, s, w, I, J, K, weps);
+  float conv = -((fx1 - fx0) * d.idx + (fy1 - fy0) * d.idy + (fz1 - fz0) * d.idz);
+  return (conv + s[ic(d, I, J, K)] * fdc) / r;
+}
+
+__global__ void k_adv_scalar(Dims d, const float* __restrict__ rc, const float* __restrict__ s,
+                             const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                             const float* __restrict__ fd, const float* __restrict__ sfc, float* __restrict__ out,
+                             float weps) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  float sf = sfc ? sfc[(long)i * d.ny + j] : 0.0f;
+  out[((long)i * d.ny + j) * d.nz + k] =
+      adv_scalar_point(d, rc, s, u, v, w, fd[ic(d, i + 3, j + 3, k + 1)], sf, i, j, k, weps);
+}
+
+// ----------------------------------------------------------------------------------
+// A3: advection_u/v/w (advection.py:37-149) with their vertical (:561-675, :816-930,
+// :1070-1183) and horizontal (:678-771, :933-1025, :1186-1281) fluxes.
+// ----------------------------------------------------------------------------------
+// w-level mass flux 0.5*(rho[Kw-1]+rho[Kw])*w at (Ic,Jc,Kw) (vertical_flux_*2: rho0w)
+__device__ __forceinline__ float mz_at(const Dims& d, const float* __restrict__ rc, const float* __restrict__ w,
+                                       int Ic, int Jc, int Kw) {
+  return 0.5f * (rc[Kw - 1] + rc[Kw]) * w[iw(d, Ic, Jc, Kw)];
+}
+
+__device__ float adv_u_point(const Dims& d, const float* __restrict__ rc, const float* __restrict__ u,
+                             const float* __restrict__ v, const float* __restrict__ w, float fdl, float fdr,
+                             float sfc_face, int f, int j, int k, float weps) {
+  int I = f + 3, J = j + 3, K = k + 1;
+  float r = rc[K];
+  long sx = d.sxu(), sy = d.syc();
+  // x flux at scalar cells I-1 and I (flux point m between u[m-1] and u[m])
+  float fx[2], fy[2], fz[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    int m = I + t;
+    long b = iu(d, m, J, K);
+    float mass = 0.5f * (u[b - sx] + u[b]) * r;
+    fx[t] = mass * hrecon(u[b - 3 * sx], u[b - 2 * sx], u[b - sx], u[b], u[b + sx], u[b + 2 * sx], upwind_pos(mass), weps);
+  }
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    int m = J + t;
+    float mass = 0.5f * (r * v[iv(d, I - 1, m, K)] + r * v[iv(d, I, m, K)]);
+    long b = iu(d, I, m, K);
+    fy[t] = mass * hrecon(u[b - 3 * sy], u[b - 2 * sy], u[b - sy], u[b], u[b + sy], u[b + 2 * sy], upwind_pos(mass), weps);
+  }
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    int fz_ = K - 1 + t;  // face between padded levels fz_ and fz_+1 (w level fz_+1)
+    float mass = 0.5f * (mz_at(d, rc, w, I - 1, J, fz_ + 1) + mz_at(d, rc, w, I, J, fz_ + 1));
+    fz[t] = mass * vrecon(u + iu(d, I, J, 0), d.nz + 2, fz_, upwind_pos(mass), weps);
+  }
+  if (k == 0) fz[0] = sfc_face;
+  float conv = -((fx[1] - fx[0]) * d.idx + (fy[1] - fy[0]) * d.idy + (fz[1] - fz[0]) * d.idz);
+  float div8u = 0.5f * (fdl + fdr);
+  float rho8u = 0.5f * (r + r);
+  return (conv + u[iu(d, I, J, K)] * div8u) / rho8u;
+}
+
+__device__ float adv_v_point(const Dims& d, const float* __restrict__ rc, const float* __restrict__ u,
+                             const float* __restrict__ v, const float* __restrict__ w, float fdl, float fdr,
+                             float sfc_face, int i, int f, int k, float weps) {
+  int I = i + 3, J = f + 3, K = k + 1;
+  float r = rc[K];
+  long sx = d.sxv(), sy = d.syc();
+  float fx[2], fy[2], fz[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    int m = I + t;
+    float mass = 0.5f * (r * u[iu(d, m, J - 1, K)] + r * u[iu(d, m, J, K)]);
+    long b = iv(d, m, J, K);
+    fx[t] = mass * hrecon(v[b - 3 * sx], v[b - 2 * sx], v[b - sx], v[b], v[b + sx], v[b + 2 * sx], upwind_pos(mass), weps);
+  }
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    int m = J + t;
+    long b = iv(d, I, m, K);
+    float mass = 0.5f * (v[b - sy] + v[b]) * r;
+    fy[t] = mass * hrecon(v[b - 3 * sy], v[b - 2 * sy], v[b - sy], v[b], v[b + sy], v[b + 2 * sy], upwind_pos(mass), weps);
+  }
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    int fz_ = K - 1 + t;
+    float mass = 0.5f * (mz_at(d, rc, w, I, J - 1, fz_ + 1) + mz_at(d, rc, w, I, J, fz_ + 1));
+    fz[t] = mass * vrecon(v + iv(d, I, J, 0), d.nz + 2, fz_, upwind_pos(mass), weps);
+  }
+  if (k == 0) fz[0] = sfc_face;
+  float conv = -((fx[1] - fx[0]) * d.idx + (fy[1] - fy[0]) * d.idy + (fz[1] - fz[0]) * d.idz);
+  float div8v = 0.5f * (fdl + fdr);
+  float rho8v = 0.5f * (r + r);
+  return (conv + v[iv(d, I, J, K)] * div8v) / rho8v;
+}
+
+// w at interior level l (1..nz-1; the caller zeroes l=0 and l=nz)
+__device__ float adv_w_point(const Dims& d, const float* __restrict__ rc, const float* __restrict__ u,
+                             const float* __restrict__ v, const float* __restrict__ w, float fdb, float fdt,
+                             int i, int j, int l, float weps) {
+  int I = i + 3, J = j + 3, K = l + 1;
+  long sx = d.sxw(), sy = d.syw();
+  float fx[2], fy[2], fz[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    int m = I + t;
+    float mass = 0.5f * (rc[K - 1] * u[iu(d, m, J, K - 1)] + rc[K] * u[iu(d, m, J, K)]);
+    long b = iw(d, m, J, K);
+    fx[t] = mass * hrecon(w[b - 3 * sx], w[b - 2 * sx], w[b - sx], w[b], w[b + sx], w[b + 2 * sx], upwind_pos(mass), weps);
+  }
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    int m = J + t;
+    float mass = 0.5f * (rc[K - 1] * v[iv(d, I, m, K - 1)] + rc[K] * v[iv(d, I, m, K)]);
+    long b = iw(d, I, m, K);
+    fy[t] = mass * hrecon(w[b - 3 * sy], w[b - 2 * sy], w[b - sy], w[b], w[b + sy], w[b + 2 * sy], upwind_pos(mass), weps);
+  }
+  const float* col = w + iw(d, I, J, 0);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    int c = K - 1 + t;  // scalar level c: face between w[c] and w[c+1]
+    float mass = 0.5f * (col[c] + col[c + 1]) * rc[c];
+    fz[t] = mass * vrecon(col, d.nz + 3, c, upwind_pos(mass), weps);
+  }
+  float conv = -((fx[1] - fx[0]) * d.idx + (fy[1] - fy[0]) * d.idy + (fz[1] - fz[0]) * d.idz);
+  float div8w = 0.5f * (fdb + fdt);
+  float rho8w = 0.5f * (rc[K - 1] + rc[K]);
+  return (conv + col[K] * div8w) / rho8w;
+}
+
+__global__ void k_adv_u(Dims d, const float* __restrict__ rc, const float* __restrict__ u, const float* __restrict__ v,
+                        const float* __restrict__ w, const float* __restrict__ fd, const float* __restrict__ sfc,
+                        float* __restrict__ out, float weps) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int f = blockIdx.z;  // 0..nx
+  if (k >= d.nz || j >= d.ny) return;
+  float sface = 0.0f;
+  if (sfc) {
+    int il = (f - 1 + d.nx) % d.nx, ir = f % d.nx;
+    sface = 0.5f * (sfc[(long)il * d.ny + j] + sfc[(long)ir * d.ny + j]);
+  }
+  int I = f + 3, J = j + 3, K = k + 1;
+  out[((long)f * d.ny + j) * d.nz + k] =
+      adv_u_point(d, rc, u, v, w, fd[ic(d, I - 1, J, K)], fd[ic(d, I, J, K)], sface, f, j, k, weps);
+}
+
+__global__ void k_adv_v(Dims d, const float* __restrict__ rc, const float* __restrict__ u, const float* __restrict__ v,
+                        const float* __restrict__ w, const float* __restrict__ fd, const float* __restrict__ sfc,
+                        float* __restrict__ out, float weps) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int f = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
+  int i = blockIdx.z;
+  if (k >= d.nz || f > d.ny) return;
+  float sface = 0.0f;
+  if (sfc) {
+    int jl = (f - 1 + d.ny) % d.ny, jr = f % d.ny;
+    sface = 0.5f * (sfc[(long)i * d.ny + jl] + sfc[(long)i * d.ny + jr]);
+  }
+  int I = i + 3, J = f + 3, K = k + 1;
+  out[((long)i * (d.ny + 1) + f) * d.nz + k] =
+      adv_v_point(d, rc, u, v, w, fd[ic(d, I, J - 1, K)], fd[ic(d, I, J, K)], sface, i, f, k, weps);
+}
+
+__global__ void k_adv_w(Dims d, const float* __restrict__ rc, const float* __restrict__ u, const float* __restrict__ v,
+                        const float* __restrict__ w, const float* __restrict__ fd, float* __restrict__ out, float weps) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (l > d.nz || j >= d.ny) return;
+  float val = 0.0f;
+  if (l > 0 && l < d.nz) {
+    int I = i + 3, J = j + 3, K = l + 1;
+    val = adv_w_point(d, rc, u, v, w, fd[ic(d, I, J, K - 1)], fd[ic(d, I, J, K)], i, j, l, weps);
+  }
+  out[((long)i * d.ny + j) * (d.nz + 1) + l] = val;
+}
+
+// ----------------------------------------------------------------------------------
+// A4: advection_pi (advection.py:152-197), cc1 = cc2 = 1/2 on the uniform grid
+// ----------------------------------------------------------------------------------
+__device__ float adv_pi_point(const Dims& d, const float* __restrict__ rc, const float* __restrict__ pi0c,
+                              const float* __restrict__ ps, const float* __restrict__ u, const float* __restrict__ v,
+                              const float* __restrict__ w, int i, int j, int k) {
+  int I = i + 3, J = j + 3, K = k + 1;
+  long c = ic(d, I, J, K);
+  long sx = d.sxc(), sy = d.syc();
+  float pc = ps[c];
+  float udx0 = u[iu(d, I, J, K)] * ((pc - ps[c - sx]) * d.idx);
+  float udx1 = u[iu(d, I + 1, J, K)] * ((ps[c + sx] - pc) * d.idx);
+  float vdy0 = v[iv(d, I, J, K)] * ((pc - ps[c - sy]) * d.idy);
+  float vdy1 = v[iv(d, I, J + 1, K)] * ((ps[c + sy] - pc) * d.idy);
+  // w levels K (lower) and K+1 (upper); zero at surface (l=0) and lid (l=nz)
+  float dz0 = (pi0c[K] - pi0c[K - 1]) * d.idz + (pc - ps[c - 1]) * d.idz;
+  float dz1 = (pi0c[K + 1] - pi0c[K]) * d.idz + (ps[c + 1] - pc) * d.idz;
+  float wdz0 = (k == 0) ? 0.0f : w[iw(d, I, J, K)] * dz0;
+  float wdz1 = (k == d.nz - 1) ? 0.0f : w[iw(d, I, J, K + 1)] * dz1;
+  float r8w0 = 0.5f * (rc[K - 1] + rc[K]);
+  float r8w1 = 0.5f * (rc[K] + rc[K + 1]);
+  return -(0.5f * (udx1 + udx0) + 0.5f * (vdy1 + vdy0) + (0.5f * wdz0 * r8w0 + 0.5f * wdz1 * r8w1) / rc[K]);
+}
+
+__global__ void k_adv_pi(Dims d, const float* __restrict__ rc, const float* __restrict__ pi0c,
+                         const float* __restrict__ ps, const float* __restrict__ u, const float* __restrict__ v,
+                         const float* __restrict__ w, float* __restrict__ out) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  out[((long)i * d.ny + j) * d.nz + k] = adv_pi_point(d, rc, pi0c, ps, u, v, w, i, j, k);
+}
+
+// ----------------------------------------------------------------------------------
+// B1: calculate_buoyancy (pressure_equations.py:162-172) with interpolate_scalar2w and
+// extrapolate_bottom_top (:123-159)
+// ----------------------------------------------------------------------------------
+__device__ __forceinline__ float theta_rho(float th, float qv) {
+  return th * (1.0f + (float)kReps * qv) / (1.0f + qv);
+}
+
+__device__ __forceinline__ float buoy_at(const float* __restrict__ th0c, const float* __restrict__ qv0c, float th,
+                                         float qv, int K) {
+  float tr0 = th0c[K] * (1.0f + (float)kReps * qv0c[K]) / (1.0f + qv0c[K]);
+  float tr = theta_rho(th, qv);
+  return (float)kG * (tr - tr0) / tr0;
+}
+
+__device__ __forceinline__ float extrap3(float a0, float a1, float a2) {
+  return 1.875f * a0 + (-1.25f) * a1 + 0.375f * a2;
+}
+
+// B at interior w level l (0..nz)
+__device__ float b8w_point(const Dims& d, const float* __restrict__ th0c, const float* __restrict__ qv0c,
+                           const float* __restrict__ th, const float* __restrict__ qv, int I, int J, int l) {
+  long c = ic(d, I, J, 0);
+  auto B = [&](int k) { return buoy_at(th0c, qv0c, th[c + k + 1], qv[c + k + 1], k + 1); };
+  if (l == 0) return extrap3(B(0), B(1), B(2));
+  if (l == d.nz) return extrap3(B(d.nz - 1), B(d.nz - 2), B(d.nz - 3));
+  return 0.5f * (B(l - 1) + B(l));
+}
+
+__global__ void k_buoyancy(Dims d, const float* __restrict__ th0c, const float* __restrict__ qv0c,
+                           const float* __restrict__ th, const float* __restrict__ qv, float* __restrict__ b,
+                           float* __restrict__ b8w) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (l > d.nz || j >= d.ny) return;
+  int I = i + 3, J = j + 3;
+  if (l < d.nz) b[((long)i * d.ny + j) * d.nz + l] = buoy_at(th0c, qv0c, th[ic(d, I, J, l + 1)], qv[ic(d, I, J, l + 1)], l + 1);
+  b8w[((long)i * d.ny + j) * (d.nz + 1) + l] = b8w_point(d, th0c, qv0c, th, qv, I, J, l);
+}
+
+// theta_rho over the whole padded array (one_step_integration.py:135)
+__global__ void k_theta_rho(long n, const float* __restrict__ th, const float* __restrict__ qv, float* __restrict__ out) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = theta_rho(th[t], qv[t]);
+}
+
+// ----------------------------------------------------------------------------------
+// P1: pressure_gradient_force (pressure_gradient_coriolis.py:7-37)
+// ----------------------------------------------------------------------------------
+__device__ __forceinline__ float pgf_x(const Dims& d, const float* __restrict__ p, const float* __restrict__ tr,
+                                       int I, int J, int K) {
+  long c = ic(d, I, J, K);
+  float dp = (p[c] - p[c - d.sxc()]) * d.idx;
+  return (float)(-kCp * 0.5) * (tr[c] + tr[c - d.sxc()]) * dp;
+}
+__device__ __forceinline__ float pgf_y(const Dims& d, const float* __restrict__ p, const float* __restrict__ tr,
+                                       int I, int J, int K) {
+  long c = ic(d, I, J, K);
+  float dp = (p[c] - p[c - d.syc()]) * d.idy;
+  return (float)(-kCp * 0.5) * (tr[c] + tr[c - d.syc()]) * dp;
+}
+// w level Kw (1..nz+1); zero at l=0 and l=nz
+__device__ __forceinline__ float pgf_z(const Dims& d, const float* __restrict__ p, const float* __restrict__ tr,
+                                       int I, int J, int Kw) {
+  if (Kw <= 1 || Kw >= d.nz + 1) return 0.0f;
+  long c = ic(d, I, J, Kw);
+  float dp = (p[c] - p[c - 1]) * d.idz;
+  return (float)(-kCp * 0.5) * (tr[c] + tr[c - 1]) * dp;
+}
+
+__global__ void k_pgf(Dims d, const float* __restrict__ p, const float* __restrict__ tr, float* __restrict__ pu,
+                      float* __restrict__ pv, float* __restrict__ pw) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nz
+  int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
+  int i = blockIdx.z;                             // 0..nx
+  if (k > d.nz || j > d.ny) return;
+  if (k < d.nz && j < d.ny) pu[((long)i * d.ny + j) * d.nz + k] = pgf_x(d, p, tr, i + 3, j + 3, k + 1);
+  if (k < d.nz && i < d.nx) pv[((long)i * (d.ny + 1) + j) * d.nz + k] = pgf_y(d, p, tr, i + 3, j + 3, k + 1);
+  if (i < d.nx && j < d.ny) pw[((long)i * d.ny + j) * (d.nz + 1) + k] = pgf_z(d, p, tr, i + 3, j + 3, k + 1);
+}
+
+// ----------------------------------------------------------------------------------
+// P2: calculate_coriolis_force (pressure_gradient_coriolis.py:80-95)
+// ----------------------------------------------------------------------------------
+__device__ __forceinline__ float cor_fv(const Dims& d, const float* __restrict__ v, int I, int J, int K, float f) {
+  return (f * 0.25f) * (v[iv(d, I - 1, J, K)] + v[iv(d, I - 1, J + 1, K)] + v[iv(d, I, J, K)] + v[iv(d, I, J + 1, K)]);
+}
+__device__ __forceinline__ float cor_fu(const Dims& d, const float* __restrict__ u, int I, int J, int K, float f) {
+  return (f * 0.25f) * (u[iu(d, I, J - 1, K)] + u[iu(d, I, J, K)] + u[iu(d, I + 1, J - 1, K)] + u[iu(d, I + 1, J, K)]);
+}
+
+__global__ void k_coriolis(Dims d, const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ fu,
+                           float* __restrict__ fv, float f) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
+  int i = blockIdx.z;                             // 0..nx
+  if (k >= d.nz || j > d.ny) return;
+  if (j < d.ny) fv[((long)i * d.ny + j) * d.nz + k] = cor_fv(d, v, i + 3, j + 3, k + 1, f);
+  if (i < d.nx) fu[((long)i * (d.ny + 1) + j) * d.nz + k] = cor_fu(d, u, i + 3, j + 3, k + 1, f);
+}
+
+// ----------------------------------------------------------------------------------
+// H1: padding (one_step_integration.py:950-985, pressure_equations.py:111-120)
+// mode 0 = scalar/w (padding3_array), 1 = u (padding3u_array), 2 = v (padding3v_array)
+// Input is an interior array (ni, nj, nk); output padded (ni+6 or +6/+... , nk+2).
+// ----------------------------------------------------------------------------------
+__device__ __forceinline__ int src_index(int P, int n, int g, bool face) {
+  // padded index P along an axis padded by g on each side; returns the interior index
+  if (!face) {
+    int t = P - g;
+    if (t < 0) t += n;
+    if (t >= n) t -= n;
+    return t;
+  }
+  // face axis: interior count n+1 (faces 0..n); ghosts skip the duplicate face
+  int t = P - g;
+  if (t < 0) return t + n;        // padded 0..g-1 -> faces n-g..n-1
+  if (t > n) return t - n;        // padded n+g+1.. -> faces 1..g
+  return t;
+}
+
+__global__ void k_pad(int mode, int ni, int nj, int nk, int g, const float* __restrict__ in, float* __restrict__ out) {
+  // ni, nj, nk: interior extents of `in` (for mode 1 ni = nx+1, for mode 2 nj = ny+1)
+  int K = blockIdx.x * blockDim.x + threadIdx.x;
+  int J = blockIdx.y * blockDim.y + threadIdx.y;
+  int I = blockIdx.z;
+  int PI = ni + 2 * g, PJ = nj + 2 * g, PK = nk + 2;
+  if (K >= PK || J >= PJ || I >= PI) return;
+  int si = (mode == 1) ? src_index(I, ni - 1, g, true) : src_index(I, ni, g, false);
+  int sj = (mode == 2) ? src_index(J, nj - 1, g, true) : src_index(J, nj, g, false);
+  int sk = K - 1;
+  if (sk < 0) sk = 0;
+  if (sk >= nk) sk = nk - 1;
+  out[((long)I * PJ + J) * PK + K] = in[((long)si * nj + sj) * nk + sk];
+}
+
+__global__ void k_set_w_bc(Dims d, float* __restrict__ w) {
+  int J = blockIdx.x * blockDim.x + threadIdx.x;
+  int I = blockIdx.y;
+  if (J >= d.ny + 6) return;
+  long b = iw(d, I, J, 0);
+  w[b] = 0.0f;
+  w[b + 1] = 0.0f;
+  w[b + d.nz + 1] = 0.0f;
+  w[b + d.nz + 2] = 0.0f;
+}
+
+// ----------------------------------------------------------------------------------
+// S1: one forward-backward acoustic substep (one_step_integration.py:528-553).
+// Reads u,v,w,pi' of the current substep, writes the next (halos filled, w BC set).
+// ----------------------------------------------------------------------------------
+__global__ void k_acoustic(Dims d, const float* __restrict__ rc, const float* __restrict__ pi0c,
+                           const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                           const float* __restrict__ p, const float* __restrict__ ps, const float* __restrict__ ru,
+                           const float* __restrict__ rv, const float* __restrict__ rw, const float* __restrict__ tr,
+                           float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
+                           float* __restrict__ po, float dts) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  int I = i + 3, J = j + 3, K = k + 1;
+  // new velocities on the six faces of the cell
+  float un0 = u[iu(d, I, J, K)] + (pgf_x(d, p, tr, I, J, K) + ru[((long)i * d.ny + j) * d.nz + k]) * dts;
+  float un1 = u[iu(d, I + 1, J, K)] + (pgf_x(d, p, tr, I + 1, J, K) + ru[((long)(i + 1) * d.ny + j) * d.nz + k]) * dts;
+  float vn0 = v[iv(d, I, J, K)] + (pgf_y(d, p, tr, I, J, K) + rv[((long)i * (d.ny + 1) + j) * d.nz + k]) * dts;
+  float vn1 = v[iv(d, I, J + 1, K)] + (pgf_y(d, p, tr, I, J + 1, K) + rv[((long)i * (d.ny + 1) + j + 1) * d.nz + k]) * dts;
+  // set_w_bc: surface (l=0) and lid (l=nz) are zero after the update
+  float wn0 = (k == 0) ? 0.0f
+                       : w[iw(d, I, J, K)] + (pgf_z(d, p, tr, I, J, K) + rw[((long)i * d.ny + j) * (d.nz + 1) + k]) * dts;
+  float wn1 = (k == d.nz - 1)
+                  ? 0.0f
+                  : w[iw(d, I, J, K + 1)] + (pgf_z(d, p, tr, I, J, K + 1) + rw[((long)i * d.ny + j) * (d.nz + 1) + k + 1]) * dts;
+  float div = (un1 - un0) * d.idx + (vn1 - vn0) * d.idy + (wn1 - wn0) * d.idz;
+  // advection_pi with the new velocities and the fixed pi*
+  long c = ic(d, I, J, K);
+  long sx = d.sxc(), sy = d.syc();
+  float pc = ps[c];
+  float udx0 = un0 * ((pc - ps[c - sx]) * d.idx);
+  float udx1 = un1 * ((ps[c + sx] - pc) * d.idx);
+  float vdy0 = vn0 * ((pc - ps[c - sy]) * d.idy);
+  float vdy1 = vn1 * ((ps[c + sy] - pc) * d.idy);
+  float dz0 = (pi0c[K] - pi0c[K - 1]) * d.idz + (pc - ps[c - 1]) * d.idz;
+  float dz1 = (pi0c[K + 1] - pi0c[K]) * d.idz + (ps[c + 1] - pc) * d.idz;
+  float wdz0 = (k == 0) ? 0.0f : wn0 * dz0;
+  float wdz1 = (k == d.nz - 1) ? 0.0f : wn1 * dz1;
+  float r8w0 = 0.5f * (rc[K - 1] + rc[K]);
+  float r8w1 = 0.5f * (rc[K] + rc[K + 1]);
+  float advp = -(0.5f * (udx1 + udx0) + 0.5f * (vdy1 + vdy0) + (0.5f * wdz0 * r8w0 + 0.5f * wdz1 * r8w1) / rc[K]);
+  float piadv = pi0c[K] + pc;
+  float dpdt = (float)(-kRd / kCv) * (piadv * div) + advp;
+  float pn = p[c] + 1.1f * dpdt * dts;
+  store_c(d, po, i, j, k, pn);
+  store_u(d, uo, i, j, k, un0);
+  if (i == d.nx - 1) store_u(d, uo, d.nx, j, k, un1);
+  store_v(d, vo, i, j, k, vn0);
+  if (j == d.ny - 1) store_v(d, vo, i, d.ny, k, vn1);
+  store_w(d, wo, i, j, k, wn0);
+  if (k == d.nz - 1) store_w(d, wo, i, j, d.nz, wn1);
+}
+
+// ----------------------------------------------------------------------------------
+// E1 / M1: Euler updates (one_step_integration.py:450-465, :876-891) and SGS add-in
+// (:113-118, :158-162): next = pad(base + tend*dt) [+ pad(sgs*dt)]
+// ----------------------------------------------------------------------------------
+__global__ void k_scalar_update(Dims d, const float* __restrict__ rc, const float* __restrict__ s0,
+                                const float* __restrict__ s, const float* __restrict__ u, const float* __restrict__ v,
+                                const float* __restrict__ w, const float* __restrict__ fd, const float* __restrict__ sfc,
+                                const float* __restrict__ heat, const float* __restrict__ sgs, float* __restrict__ out,
+                                float* __restrict__ tend_out, float dt, float weps) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  int I = i + 3, J = j + 3, K = k + 1;
+  long o = ((long)i * d.ny + j) * d.nz + k;
+  float sf = sfc ? sfc[(long)i * d.ny + j] : 0.0f;
+  float t = adv_scalar_point(d, rc, s, u, v, w, fd[ic(d, I, J, K)], sf, i, j, k, weps);
+  if (heat) t = t + heat[o];
+  float nxt = s0[ic(d, I, J, K)] + t * dt;
+  if (sgs) {
+    nxt = nxt + sgs[o] * dt;
+    t = t + sgs[o];
+  }
+  if (tend_out) tend_out[o] = t;
+  store_c(d, out, i, j, k, nxt);
+}
+
+__global__ void k_momentum_update(Dims d, const float* __restrict__ u0, const float* __restrict__ v0,
+                                  const float* __restrict__ w0, const float* __restrict__ p, const float* __restrict__ tr,
+                                  const float* __restrict__ au, const float* __restrict__ av, const float* __restrict__ aw,
+                                  const float* __restrict__ fu, const float* __restrict__ fv, const float* __restrict__ b8w,
+                                  const float* __restrict__ su, const float* __restrict__ sv, const float* __restrict__ sw,
+                                  float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
+                                  float* __restrict__ duo, float* __restrict__ dvo, float* __restrict__ dwo, float dt) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nz
+  int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
+  int i = blockIdx.z;                             // 0..nx
+  if (k > d.nz || j > d.ny) return;
+  int I = i + 3, J = j + 3, K = k + 1;
+  if (k < d.nz && j < d.ny) {
+    long o = ((long)i * d.ny + j) * d.nz + k;
+    float du = au[o] + pgf_x(d, p, tr, I, J, K) + (fv ? fv[o] : 0.0f);
+    float un = u0[iu(d, I, J, K)] + du * dt;
+    if (su) un = un + su[o] * dt;
+    if (duo) duo[o] = su ? du + su[o] : du;
+    store_u(d, uo, i, j, k, un);
+  }
+  if (k < d.nz && i < d.nx) {
+    long o = ((long)i * (d.ny + 1) + j) * d.nz + k;
+    float dv = av[o] + pgf_y(d, p, tr, I, J, K) - (fu ? fu[o] : 0.0f);
+    float vn = v0[iv(d, I, J, K)] + dv * dt;
+    if (sv) vn = vn + sv[o] * dt;
+    if (dvo) dvo[o] = sv ? dv + sv[o] : dv;
+    store_v(d, vo, i, j, k, vn);
+  }
+  if (i < d.nx && j < d.ny) {
+    long o = ((long)i * d.ny + j) * (d.nz + 1) + k;
+    float dw = aw[o] + pgf_z(d, p, tr, I, J, K) + b8w[o];
+    float wn = w0[iw(d, I, J, K)] + dw * dt;
+    // set_w_bc, then the SGS add (sgs_w is zero at surface and lid)
+    if (k == 0 || k == d.nz) wn = 0.0f;
+    if (sw) wn = wn + sw[o] * dt;
+    if (dwo) dwo[o] = sw ? dw + sw[o] : dw;
+    store_w(d, wo, i, j, k, wn);
+  }
+}
+
+}  // namespace lex

@@ -1,0 +1,215 @@
+// G1: isotropic Smagorinsky-Lilly SGS with the N^2 correction (turbulence.py).
+//   k_smag_deform : compute_deformation (turbulence.py:298-388)  -> rho*S_ij, deformation
+//   k_smag_nm     : compute_nm          (turbulence.py:391-427)  -> N^2
+//   k_smag_k      : compute_k           (turbulence.py:430-446)  -> Km, Kh
+//   k_smag_uvw / k_smag_scalar : compute_smag (turbulence.py:9-111)
+#include "lex_common.hpp"
+
+namespace lex {
+
+__device__ __forceinline__ float ex3(float a0, float a1, float a2) { return 1.875f * a0 + (-1.25f) * a1 + 0.375f * a2; }
+
+// e12 at corner (u-face F, v-face G), scalar level K
+__device__ __forceinline__ float e12_at(const Dims& d, const float* __restrict__ u, const float* __restrict__ v,
+                                        int F, int G, int K) {
+  float dudy = (u[iu(d, F, G, K)] - u[iu(d, F, G - 1, K)]) * d.idy;
+  float dvdx = (v[iv(d, F, G, K)] - v[iv(d, F - 1, G, K)]) * d.idx;
+  return 0.5f * (dudy + dvdx);
+}
+// e13 at (u-face F, row J, interior w level l in 1..nz-1)
+__device__ __forceinline__ float e13_at(const Dims& d, const float* __restrict__ u, const float* __restrict__ w,
+                                        int F, int J, int l) {
+  float dudz = (u[iu(d, F, J, l + 1)] - u[iu(d, F, J, l)]) * d.idz;
+  float dwdx = (w[iw(d, F, J, l + 1)] - w[iw(d, F - 1, J, l + 1)]) * d.idx;
+  return 0.5f * (dudz + dwdx);
+}
+// e23 at (column I, v-face G, interior w level l)
+__device__ __forceinline__ float e23_at(const Dims& d, const float* __restrict__ v, const float* __restrict__ w,
+                                        int I, int G, int l) {
+  float dvdz = (v[iv(d, I, G, l + 1)] - v[iv(d, I, G, l)]) * d.idz;
+  float dwdy = (w[iw(d, I, G, l + 1)] - w[iw(d, I, G - 1, l + 1)]) * d.idy;
+  return 0.5f * (dvdz + dwdy);
+}
+
+__global__ void k_smag_deform(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
+                              const float* __restrict__ v, const float* __restrict__ w, float* __restrict__ s11,
+                              float* __restrict__ s22, float* __restrict__ s33, float* __restrict__ s12,
+                              float* __restrict__ s13, float* __restrict__ s23, float* __restrict__ deform) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
+  float r = rc[K];
+  float a11 = r * (u[iu(d, I + 1, J, K)] - u[iu(d, I, J, K)]) * d.idx;
+  float a22 = r * (v[iv(d, I, J + 1, K)] - v[iv(d, I, J, K)]) * d.idy;
+  float a33 = r * (w[iw(d, I, J, K + 1)] - w[iw(d, I, J, K)]) * d.idz;
+  float a12 = r * (0.25f * (e12_at(d, u, v, I, J, K) + e12_at(d, u, v, I + 1, J + 1, K) + e12_at(d, u, v, I, J + 1, K) +
+                            e12_at(d, u, v, I + 1, J, K)));
+  float a13, a23;
+  if (k == 0) {
+    float b1 = ex3(e13_at(d, u, w, I + 1, J, 1), e13_at(d, u, w, I + 1, J, 2), e13_at(d, u, w, I + 1, J, 3));
+    float b0 = ex3(e13_at(d, u, w, I, J, 1), e13_at(d, u, w, I, J, 2), e13_at(d, u, w, I, J, 3));
+    a13 = 0.5f * (b1 + b0);
+    float c1 = ex3(e23_at(d, v, w, I, J + 1, 1), e23_at(d, v, w, I, J + 1, 2), e23_at(d, v, w, I, J + 1, 3));
+    float c0 = ex3(e23_at(d, v, w, I, J, 1), e23_at(d, v, w, I, J, 2), e23_at(d, v, w, I, J, 3));
+    a23 = 0.5f * (c1 + c0);
+  } else if (k == nz - 1) {
+    float b1 = ex3(e13_at(d, u, w, I + 1, J, nz - 1), e13_at(d, u, w, I + 1, J, nz - 2), e13_at(d, u, w, I + 1, J, nz - 3));
+    float b0 = ex3(e13_at(d, u, w, I, J, nz - 1), e13_at(d, u, w, I, J, nz - 2), e13_at(d, u, w, I, J, nz - 3));
+    a13 = 0.5f * (b1 + b0);
+    float c1 = ex3(e23_at(d, v, w, I, J + 1, nz - 1), e23_at(d, v, w, I, J + 1, nz - 2), e23_at(d, v, w, I, J + 1, nz - 3));
+    float c0 = ex3(e23_at(d, v, w, I, J, nz - 1), e23_at(d, v, w, I, J, nz - 2), e23_at(d, v, w, I, J, nz - 3));
+    a23 = 0.5f * (c1 + c0);
+  } else {
+    a13 = 0.25f * (e13_at(d, u, w, I, J, k) + e13_at(d, u, w, I + 1, J, k + 1) + e13_at(d, u, w, I, J, k + 1) +
+                   e13_at(d, u, w, I + 1, J, k));
+    a23 = 0.25f * (e23_at(d, v, w, I, J, k) + e23_at(d, v, w, I, J + 1, k + 1) + e23_at(d, v, w, I, J, k + 1) +
+                   e23_at(d, v, w, I, J + 1, k));
+  }
+  a13 = r * a13;
+  a23 = r * a23;
+  float df = (2.0f * (a11 * a11 + a22 * a22 + a33 * a33) + 4.0f * (a12 * a12 + a13 * a13 + a23 * a23)) / (r * r);
+  store_c(d, s11, i, j, k, a11);
+  store_c(d, s22, i, j, k, a22);
+  store_c(d, s33, i, j, k, a33);
+  store_c(d, s12, i, j, k, a12);
+  store_c(d, s13, i, j, k, a13);
+  store_c(d, s23, i, j, k, a23);
+  store_c(d, deform, i, j, k, df);
+}
+
+// N^2 at interior w level l (1..nz-1) of column (I,J)
+__device__ __forceinline__ float n28w_at(const Dims& d, const float* __restrict__ th, const float* __restrict__ qv,
+                                         long c, int l) {
+  float a = th[c + l + 1] * (1.0f + (float)kRepsm1 * qv[c + l + 1]);
+  float b = th[c + l] * (1.0f + (float)kRepsm1 * qv[c + l]);
+  return logf(a / b) / d.dz * (float)kG;
+}
+
+__global__ void k_smag_nm(Dims d, const float* __restrict__ th, const float* __restrict__ qv, float* __restrict__ n2) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  long c = ic(d, i + 3, j + 3, 0);
+  int nz = d.nz;
+  float val;
+  if (k == 0)
+    val = ex3(n28w_at(d, th, qv, c, 1), n28w_at(d, th, qv, c, 2), n28w_at(d, th, qv, c, 3));
+  else if (k == nz - 1)
+    val = ex3(n28w_at(d, th, qv, c, nz - 1), n28w_at(d, th, qv, c, nz - 2), n28w_at(d, th, qv, c, nz - 3));
+  else
+    val = 0.5f * (n28w_at(d, th, qv, c, k) + n28w_at(d, th, qv, c, k + 1));
+  store_c(d, n2, i, j, k, val);
+}
+
+// elementwise over the padded array
+__global__ void k_smag_k(long n, const float* __restrict__ s2, const float* __restrict__ n2, float* __restrict__ km,
+                         float* __restrict__ kh, float csgs2, float prandtl) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  float x = s2[t] - n2[t] / prandtl;
+  x = (x > 0.0f) ? x : 0.0f;
+  float m = csgs2 * sqrtf(x);
+  km[t] = m;
+  kh[t] = m / prandtl;
+}
+
+// tau_ij = -2 km s_ij at padded scalar index c
+#define TAU(arr, c) (-2.0f * km[c] * arr[c])
+
+__global__ void k_smag_uvw(Dims d, const float* __restrict__ rc, const float* __restrict__ km,
+                           const float* __restrict__ s11, const float* __restrict__ s22, const float* __restrict__ s33,
+                           const float* __restrict__ s12, const float* __restrict__ s13, const float* __restrict__ s23,
+                           float* __restrict__ su, float* __restrict__ sv, float* __restrict__ sw) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nz
+  int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
+  int i = blockIdx.z;                             // 0..nx
+  if (k > d.nz || j > d.ny) return;
+  const long sx = d.sxc(), sy = d.syc();
+  int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
+  // interpolate_scalar2w_0 of tau at w level l: zero at l=0, nz
+  auto t13w = [&](long col, int l) -> float {
+    if (l <= 0 || l >= nz) return 0.0f;
+    return 0.5f * (TAU(s13, col + l) + TAU(s13, col + l + 1));
+  };
+  auto t23w = [&](long col, int l) -> float {
+    if (l <= 0 || l >= nz) return 0.0f;
+    return 0.5f * (TAU(s23, col + l) + TAU(s23, col + l + 1));
+  };
+  if (k < nz && j < d.ny) {  // sgs_u at u face i (0..nx)
+    long c = ic(d, I, J, K);
+    float r = rc[K];
+    float d11 = (TAU(s11, c) - TAU(s11, c - sx)) * d.idx;
+    auto d12y = [&](long cc) { return (TAU(s12, cc) - TAU(s12, cc - sy)) * d.idy; };
+    float d12 = 0.25f * (d12y(c - sx) + d12y(c + sy) + d12y(c - sx + sy) + d12y(c));
+    auto d13s = [&](long cc) {
+      long col = cc - K;
+      return (t13w(col, k + 1) - t13w(col, k)) * d.idz;
+    };
+    float d13 = 0.5f * (d13s(c - sx) + d13s(c));
+    su[((long)i * d.ny + j) * nz + k] = -(d11 + d12 + d13) / (0.5f * (r + r));
+  }
+  if (k < nz && i < d.nx) {  // sgs_v at v face j (0..ny)
+    long c = ic(d, I, J, K);
+    float r = rc[K];
+    float d22 = (TAU(s22, c) - TAU(s22, c - sy)) * d.idy;
+    auto d12x = [&](long cc) { return (TAU(s12, cc) - TAU(s12, cc - sx)) * d.idx; };
+    float d21 = 0.25f * (d12x(c - sy) + d12x(c + sx) + d12x(c) + d12x(c + sx - sy));
+    auto d23s = [&](long cc) {
+      long col = cc - K;
+      return (t23w(col, k + 1) - t23w(col, k)) * d.idz;
+    };
+    float d23 = 0.5f * (d23s(c - sy) + d23s(c));
+    sv[((long)i * (d.ny + 1) + j) * nz + k] = -(d21 + d22 + d23) / (0.5f * (r + r));
+  }
+  if (i < d.nx && j < d.ny) {  // sgs_w at level k (0..nz)
+    float val = 0.0f;
+    if (k > 0 && k < nz) {
+      int l = k;
+      long cu = ic(d, I, J, l + 1), cl = ic(d, I, J, l);  // cells above and below the level
+      float d33 = (TAU(s33, cu) - TAU(s33, cl)) * d.idz;
+      auto d13x = [&](long cc) { return (TAU(s13, cc + sx) - TAU(s13, cc - sx)) / (2.0f * d.dx); };
+      auto d23y = [&](long cc) { return (TAU(s23, cc + sy) - TAU(s23, cc - sy)) / (2.0f * d.dy); };
+      float a = 0.5f * (d13x(cl) + d13x(cu));
+      float b = 0.5f * (d23y(cl) + d23y(cu));
+      float r8w = 0.5f * (rc[l] + rc[l + 1]);
+      val = -(a + b + d33) / r8w;
+    }
+    sw[((long)i * d.ny + j) * (nz + 1) + k] = val;
+  }
+}
+
+// SGS diffusion of a scalar (turbulence.py:80-109)
+__device__ float smag_scalar_point(const Dims& d, const float* __restrict__ rc, const float* __restrict__ kh,
+                                   const float* __restrict__ s, int i, int j, int k) {
+  const long sx = d.sxc(), sy = d.syc();
+  int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
+  long c = ic(d, I, J, K);
+  float r = rc[K];
+  float r8u = 0.5f * (r + r);
+  auto tx = [&](long cc) { return -r8u * (0.5f * (kh[cc] + kh[cc - sx])) * ((s[cc] - s[cc - sx]) * d.idx); };
+  auto ty = [&](long cc) { return -r8u * (0.5f * (kh[cc] + kh[cc - sy])) * ((s[cc] - s[cc - sy]) * d.idy); };
+  // w level l between padded levels l and l+1; zero at l=0, nz
+  auto tz = [&](int l) -> float {
+    if (l <= 0 || l >= nz) return 0.0f;
+    long cl = c - K + l;
+    float r8w = 0.5f * (rc[l] + rc[l + 1]);
+    return -r8w * (0.5f * (kh[cl] + kh[cl + 1])) * ((s[cl + 1] - s[cl]) * d.idz);
+  };
+  float div = (tx(c + sx) - tx(c)) * d.idx + (ty(c + sy) - ty(c)) * d.idy + (tz(k + 1) - tz(k)) * d.idz;
+  return -div / r;
+}
+
+__global__ void k_smag_scalar(Dims d, const float* __restrict__ rc, const float* __restrict__ kh,
+                              const float* __restrict__ s, float* __restrict__ out) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  out[((long)i * d.ny + j) * d.nz + k] = smag_scalar_point(d, rc, kh, s, i, j, k);
+}
+
+#undef TAU
+}  // namespace lex

@@ -1,0 +1,252 @@
+// R1/R2: the WS-RK3 step for the compressible core (solver_opt 2) on the device.
+// Stage structure follows rk_sub_step0/1/2 (one_step_integration.py:16-447) and
+// ws_rk3_big_step (one_sprint_functions.py:10-30):
+//   stage s (dt_s = dt/3, dt/2, dt) restarts from xi0;
+//   Smagorinsky tendencies are computed in stage 0 and frozen (:57-76, :245);
+//   momentum rhs = advection + buoyancy(w) + SGS; acoustic substeps 1, round(n/2), n;
+//   scalars: s_next = pad(s0 + adv*dt_s) + pad(sgs*dt_s).
+#pragma once
+#include "lex_common.hpp"
+#include "lex_prof.hpp"
+
+namespace lex {
+
+// momentum right-hand sides rhs_u = adv4u + fv + sgs_u, rhs_v = adv4v - fu + sgs_v,
+// rhs_w = adv4w + b8w + sgs_w (one_step_integration.py:126-133); Coriolis is off in
+// every configuration on this path (cor_opt 0), sgs may be NULL.
+__global__ void k_rhs_uv(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
+                         const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ fd,
+                         const float* __restrict__ su, const float* __restrict__ sv, float* __restrict__ ru,
+                         float* __restrict__ rv) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
+  int i = blockIdx.z;                             // 0..nx
+  if (k >= d.nz || j > d.ny) return;
+  const float weps = 1.0e-16f;  // one_step_integration.py:896
+  int I = i + 3, J = j + 3, K = k + 1;
+  if (j < d.ny) {
+    long o = ((long)i * d.ny + j) * d.nz + k;
+    float a = adv_u_point(d, rc, u, v, w, fd[ic(d, I - 1, J, K)], fd[ic(d, I, J, K)], 0.0f, i, j, k, weps);
+    a = a + 0.0f;  // + fv (zero)
+    if (su) a = a + su[o];
+    ru[o] = a;
+  }
+  if (i < d.nx) {
+    long o = ((long)i * (d.ny + 1) + j) * d.nz + k;
+    float a = adv_v_point(d, rc, u, v, w, fd[ic(d, I, J - 1, K)], fd[ic(d, I, J, K)], 0.0f, i, j, k, weps);
+    a = a - 0.0f;  // - fu (zero)
+    if (sv) a = a + sv[o];
+    rv[o] = a;
+  }
+}
+
+__global__ void k_rhs_w(Dims d, const float* __restrict__ rc, const float* __restrict__ th0c,
+                        const float* __restrict__ qv0c, const float* __restrict__ u, const float* __restrict__ v,
+                        const float* __restrict__ w, const float* __restrict__ th, const float* __restrict__ qv,
+                        const float* __restrict__ fd, const float* __restrict__ sw, float* __restrict__ rw) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (l > d.nz || j >= d.ny) return;
+  const float weps = 1.0e-16f;
+  int I = i + 3, J = j + 3, K = l + 1;
+  float a = 0.0f;
+  if (l > 0 && l < d.nz) a = adv_w_point(d, rc, u, v, w, fd[ic(d, I, J, K - 1)], fd[ic(d, I, J, K)], i, j, l, weps);
+  a = a + b8w_point(d, th0c, qv0c, th, qv, I, J, l);
+  long o = ((long)i * d.ny + j) * (d.nz + 1) + l;
+  if (sw) a = a + sw[o];
+  rw[o] = a;
+}
+
+__global__ void k_copy(long n, const float* __restrict__ a, float* __restrict__ b) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = a[t];
+}
+
+struct Rk3Layout {
+  size_t nc, nu, nv, nw;        // padded sizes (elements)
+  size_t iu_, iv_, iw_, ic_;    // interior sizes (u, v, w points, scalar)
+  size_t off_stA, off_stB;      // two stage states (6 + np padded fields each)
+  size_t off_P[2];              // acoustic ping-pong (u, v, w, pip)
+  size_t off_fd, off_tr;        // flow divergence, theta_rho
+  size_t off_ru, off_rv, off_rw;
+  size_t off_su, off_sv, off_sw, off_ss;   // frozen SGS tendencies (ss: 2+np scalars)
+  size_t off_smag;              // 10 padded scratch fields
+  size_t total;
+  int np;
+};
+
+inline size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
+
+inline Rk3Layout rk3_layout(const Dims& d, int np) {
+  Rk3Layout L;
+  L.np = np;
+  L.nc = d.nc();
+  L.nu = d.nu();
+  L.nv = d.nv();
+  L.nw = d.nw();
+  L.iu_ = (size_t)(d.nx + 1) * d.ny * d.nz;
+  L.iv_ = (size_t)d.nx * (d.ny + 1) * d.nz;
+  L.iw_ = (size_t)d.nx * d.ny * (d.nz + 1);
+  L.ic_ = (size_t)d.nx * d.ny * d.nz;
+  size_t state = al256(L.nu * 4) + al256(L.nv * 4) + al256(L.nw * 4) + (3 + np) * al256(L.nc * 4);
+  size_t acou = al256(L.nu * 4) + al256(L.nv * 4) + al256(L.nw * 4) + al256(L.nc * 4);
+  size_t o = 0;
+  L.off_stA = o; o += state;
+  L.off_stB = o; o += state;
+  L.off_P[0] = o; o += acou;
+  L.off_P[1] = o; o += acou;
+  L.off_fd = o; o += al256(L.nc * 4);
+  L.off_tr = o; o += al256(L.nc * 4);
+  L.off_ru = o; o += al256(L.iu_ * 4);
+  L.off_rv = o; o += al256(L.iv_ * 4);
+  L.off_rw = o; o += al256(L.iw_ * 4);
+  L.off_su = o; o += al256(L.iu_ * 4);
+  L.off_sv = o; o += al256(L.iv_ * 4);
+  L.off_sw = o; o += al256(L.iw_ * 4);
+  L.off_ss = o; o += (2 + np) * al256(L.ic_ * 4);
+  L.off_smag = o; o += 10 * al256(L.nc * 4);
+  L.total = o;
+  return L;
+}
+
+// pointers of a state block laid out as theta, u, v, w, pip, qv, passive...
+inline void state_ptrs(char* base, const Rk3Layout& L, float** f) {
+  size_t o = 0;
+  f[1] = (float*)(base + o); o += al256(L.nu * 4);
+  f[2] = (float*)(base + o); o += al256(L.nv * 4);
+  f[3] = (float*)(base + o); o += al256(L.nw * 4);
+  f[0] = (float*)(base + o); o += al256(L.nc * 4);
+  f[4] = (float*)(base + o); o += al256(L.nc * 4);
+  f[5] = (float*)(base + o); o += al256(L.nc * 4);
+  for (int q = 0; q < L.np; ++q) {
+    f[6 + q] = (float*)(base + o);
+    o += al256(L.nc * 4);
+  }
+}
+
+inline void acou_ptrs(char* base, const Rk3Layout& L, float** f) {  // u, v, w, pip
+  size_t o = 0;
+  f[0] = (float*)(base + o); o += al256(L.nu * 4);
+  f[1] = (float*)(base + o); o += al256(L.nv * 4);
+  f[2] = (float*)(base + o); o += al256(L.nw * 4);
+  f[3] = (float*)(base + o);
+}
+
+struct StepOpts {
+  float dt;
+  int n_sound;
+  int turb;
+  float cs;
+  float prandtl;
+  int n_passive;
+};
+
+template <class O, class B>
+int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* ws, const Rk3Layout& L,
+             hipStream_t st) {
+  const int nf = 6 + o.n_passive;
+  float* X0[16];
+  float* XA[16];
+  float* XB[16];
+  for (int q = 0; q < nf; ++q) X0[q] = state[q];
+  state_ptrs(ws + L.off_stA, L, XA);
+  state_ptrs(ws + L.off_stB, L, XB);
+  float* P[2][4];
+  acou_ptrs(ws + L.off_P[0], L, P[0]);
+  acou_ptrs(ws + L.off_P[1], L, P[1]);
+  float* fd = (float*)(ws + L.off_fd);
+  float* tr = (float*)(ws + L.off_tr);
+  float* ru = (float*)(ws + L.off_ru);
+  float* rv = (float*)(ws + L.off_rv);
+  float* rw = (float*)(ws + L.off_rw);
+  float* su = (float*)(ws + L.off_su);
+  float* sv = (float*)(ws + L.off_sv);
+  float* sw = (float*)(ws + L.off_sw);
+  float* ss[10];
+  for (int q = 0; q < 2 + o.n_passive; ++q) ss[q] = (float*)(ws + L.off_ss + q * al256(L.ic_ * 4));
+  float* sm[10];
+  for (int q = 0; q < 10; ++q) sm[q] = (float*)(ws + L.off_smag + q * al256(L.nc * 4));
+  const dim3 bk(64, 4, 1);
+  auto G = [&](int a, int bb, int c) { return dim3((a + 63) / 64, (bb + 3) / 4, c); };
+  const float* rc = b.rho0;
+  const double dtd = (double)o.dt;
+  const double dts_small = dtd / (double)o.n_sound;
+
+  for (int s = 0; s < 3; ++s) {
+    const double dt_s = (s == 0) ? dtd / 3.0 : (s == 1 ? dtd / 2.0 : dtd);
+    float** src = (s == 0) ? X0 : (s == 1 ? XA : XB);
+    float** dst = (s == 0) ? XA : (s == 1 ? XB : X0);
+    const bool turb = o.turb != 0;
+    if (s == 0 && turb) {
+      // compute_deformation -> compute_nm -> compute_k -> compute_smag
+      LEX_LAUNCH(KID_SMAG_DEFORM, k_smag_deform, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, src[1], src[2], src[3], sm[0], sm[1],
+                         sm[2], sm[3], sm[4], sm[5], sm[6]);
+      LEX_LAUNCH(KID_SMAG_NM, k_smag_nm, G(d.nz, d.ny, d.nx), bk, 0, st, d, src[0], src[5], sm[7]);
+      float gs = cbrtf(d.dx * d.dy * d.dz);
+      float cg = o.cs * gs;
+      long n = (long)L.nc;
+      LEX_LAUNCH(KID_SMAG_K, k_smag_k, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, sm[6], sm[7], sm[8], sm[9],
+                         cg * cg, o.prandtl);
+      LEX_LAUNCH(KID_SMAG_UVW, k_smag_uvw, G(d.nz + 1, d.ny + 1, d.nx + 1), bk, 0, st, d, rc, sm[8], sm[0], sm[1], sm[2],
+                         sm[3], sm[4], sm[5], su, sv, sw);
+      LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, sm[9], src[0], ss[0]);
+      LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, sm[9], src[5], ss[1]);
+      for (int q = 0; q < o.n_passive; ++q)
+        LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, sm[9], src[6 + q], ss[2 + q]);
+    }
+    LEX_LAUNCH(KID_DIV_RHO, k_div_rho, G(d.nz + 2, d.ny + 6, d.nx + 6), bk, 0, st, d, rc, src[1], src[2], src[3], fd);
+    LEX_LAUNCH(KID_RHS_UV, k_rhs_uv, G(d.nz, d.ny + 1, d.nx + 1), bk, 0, st, d, rc, src[1], src[2], src[3], fd,
+                       turb ? su : nullptr, turb ? sv : nullptr, ru, rv);
+    LEX_LAUNCH(KID_RHS_W, k_rhs_w, G(d.nz + 1, d.ny, d.nx), bk, 0, st, d, rc, b.theta0, b.qv0, src[1], src[2], src[3],
+                       src[0], src[5], fd, turb ? sw : nullptr, rw);
+    {
+      long n = (long)L.nc;
+      LEX_LAUNCH(KID_THETA_RHO, k_theta_rho, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, src[0], src[5], tr);
+    }
+    // acoustic substeps
+    int nsub;
+    float dts;
+    if (s == 0) {
+      nsub = 1;
+      dts = (float)dt_s;
+    } else {
+      nsub = (int)nearbyint(dt_s / dts_small);
+      dts = (float)dts_small;
+    }
+    float* in[4] = {X0[1], X0[2], X0[3], X0[4]};
+    for (int t = 0; t < nsub; ++t) {
+      float* out[4];
+      bool last = (t == nsub - 1);
+      bool alias = last && (dst == X0) && (in[0] == X0[1]);
+      if (last && !alias) {
+        out[0] = dst[1]; out[1] = dst[2]; out[2] = dst[3]; out[3] = dst[4];
+      } else {
+        int pp = (in[0] == P[0][0]) ? 1 : 0;
+        for (int c = 0; c < 4; ++c) out[c] = P[pp][c];
+      }
+      LEX_LAUNCH(KID_ACOUSTIC, k_acoustic, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, b.pi0, in[0], in[1], in[2], in[3], src[4],
+                         ru, rv, rw, tr, out[0], out[1], out[2], out[3], dts);
+      if (alias) {
+        const size_t ns[4] = {L.nu, L.nv, L.nw, L.nc};
+        for (int c = 0; c < 4; ++c)
+          LEX_LAUNCH(KID_COPY, k_copy, dim3((unsigned)((ns[c] + 255) / 256)), dim3(256), 0, st, (long)ns[c], out[c],
+                             dst[1 + c]);
+      }
+      for (int c = 0; c < 4; ++c) in[c] = out[c];
+    }
+    // scalars: theta (weps 1e-17), qv and passive (1e-18) (one_step_integration.py:905, :913)
+    for (int q = 0; q < 2 + o.n_passive; ++q) {
+      int fi = (q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2));
+      float weps = (q == 0) ? 1.0e-17f : 1.0e-18f;
+      LEX_LAUNCH(KID_SCALAR_UPDATE, k_scalar_update, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, X0[fi], src[fi], src[1], src[2],
+                         src[3], fd, (const float*)nullptr, (const float*)nullptr, turb ? ss[q] : nullptr, dst[fi],
+                         (float*)nullptr, (float)dt_s, weps);
+    }
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return (int)e;
+  return 0;
+}
+
+}  // namespace lex

@@ -1,0 +1,87 @@
+"""WS-RK3 step and sprint with the reference's signatures (one_sprint_functions.py).
+
+For the compressible core (solver_opt 2) the whole step runs in the fused device driver
+`lex_rk3_step` (one C call per step); the pseudo-incompressible core (solver_opt 1)
+composes the stage functions of lex_amd.one_step_integration.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from . import namelist_n_constants as nl
+from . import one_step_integration as one
+from ._core import column, grid_from
+
+_WS: dict = {}
+
+
+def _workspace(g, n_passive, device):
+    key = (g.nx, g.ny, g.nz, n_passive, str(device))
+    ws = _WS.get(key)
+    if ws is None:
+        nbytes = _lib.load().lex_rk3_workspace_bytes(g, n_passive)
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        _WS.clear()
+        _WS[key] = ws
+    return ws
+
+
+def fused_step(phys_state, base_state, grid, model_opt, dt=None, n_sound=None, cs=None, prandtl=None):
+    """Advance a solver_opt-2 state by one WS-RK3 step in place on the device."""
+    solver_opt, cor_opt, turb_opt = one._check_opts(model_opt)
+    if solver_opt != 2 or cor_opt != 0:
+        raise NotImplementedError("the fused step covers solver_opt 2 without Coriolis")
+    rho0_theta0, rho0, theta0, pi0, qv0 = base_state[:5]
+    n_passive = len(phys_state) - 6
+    opts = _lib.StepOpts(float(nl.dt if dt is None else dt), int(nl.n_sound if n_sound is None else n_sound),
+                         int(turb_opt), float(nl.cs if cs is None else cs),
+                         float(nl.prandtl if prandtl is None else prandtl), n_passive)
+    cols = [column(a) for a in (rho0, theta0, qv0, pi0, rho0_theta0)]
+    bc = _lib.BaseCols(*[c.data_ptr() for c in cols])
+    ptrs = (ctypes.c_void_p * len(phys_state))(*[_lib.ptr(a) for a in phys_state])
+    ws = _workspace(grid, n_passive, phys_state[0].device)
+    _lib.call("lex_rk3_step", grid, opts, bc, ptrs, _lib.ptr(ws), ws.numel(), _lib.stream())
+    return phys_state
+
+
+def ws_rk3_big_step(phys_state, base_state, grids, model_opt):
+    """One Wicker-Skamarock RK3 step (one_sprint_functions.py:10-30).
+
+    Returns (phys_state, sfc_others) with new tensors, as the reference does.
+    """
+    solver_opt = model_opt[0]
+    if solver_opt == 2 and model_opt[3] == 0:
+        g = grid_from(base_state[1], *grids[3:6])
+        new = tuple(a.clone() for a in phys_state)
+        fused_step(new, base_state, g, model_opt)
+        return new, one._sfc_etc(-999, new[0])
+    xi0 = phys_state
+    xi1, _, sfc_others, heating, sgs = one.rk_sub_step0(xi0, xi0, base_state, grids, model_opt, nl.dt / 3.0)
+    xi2, _ = one.rk_sub_step1(xi0, xi1, base_state, grids, heating, sfc_others, sgs, model_opt, nl.dt / 2.0)
+    xi3, _ = one.rk_sub_step2(xi0, xi2, base_state, grids, heating, sfc_others, sgs, model_opt, nl.dt)
+    return xi3, sfc_others
+
+
+def ws_rk3_big_step_staged(phys_state, base_state, grids, model_opt):
+    """The same step through the stage functions (used to cross-check the fused driver)."""
+    xi0 = phys_state
+    xi1, _, sfc_others, heating, sgs = one.rk_sub_step0(xi0, xi0, base_state, grids, model_opt, nl.dt / 3.0)
+    xi2, _ = one.rk_sub_step1(xi0, xi1, base_state, grids, heating, sfc_others, sgs, model_opt, nl.dt / 2.0)
+    xi3, _ = one.rk_sub_step2(xi0, xi2, base_state, grids, heating, sfc_others, sgs, model_opt, nl.dt)
+    return xi3, sfc_others
+
+
+def ws_rk3_sprint(phys_state, base_state, grids, model_opt):
+    """sprint_n steps (one_sprint_functions.py:109-115)."""
+    sfc_others = None
+    for _ in range(nl.sprint_n):
+        phys_state, sfc_others = ws_rk3_big_step(phys_state, base_state, grids, model_opt)
+    return phys_state, sfc_others
+
+
+def get_first_step_flux(phys_state, base_state, grids, model_opt):
+    """Stage-0 surface fluxes for output (one_sprint_functions.py:168-174); zero with sfc_opt off."""
+    return one._sfc_etc(-999, phys_state[0])

@@ -1,0 +1,1090 @@
+"""CPU oracle: a NumPy restatement of LEX's per-step dynamics (test infrastructure only).
+
+This module is the checker for the HIP path in ``lex_amd``.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may import
+it; the product path never calls it.
+
+Every function restates one reference routine and cites it as ``file:line`` of the
+reference repository (MetLab-HKUST/LEX).  Arrays use the reference's layout: C-order
+``(x, y, z)`` with z contiguous, ghost widths ``(ngx, ngy, ngz) = (3, 3, 1)``:
+
+* scalars (theta, pi', qv, ...): ``(nx+6, ny+6, nz+2)``
+* u: ``(nx+7, ny+6, nz+2)``  (nx+1 interior faces incl. the periodic duplicate)
+* v: ``(nx+6, ny+7, nz+2)``
+* w: ``(nx+6, ny+6, nz+3)``  (nz+1 interior levels incl. surface and lid)
+
+The dtype of the inputs selects the arithmetic: float64 arrays give the FP64 oracle,
+float32 arrays the FP32 oracle (Python-float constants are weak scalars under NumPy 2,
+as in JAX, so FP32 stays FP32).
+
+Pinning (see tests/test_oracle_*.py): the reference ships no tests or golden vectors
+and JAX is absent, so each function is pinned by closed forms, invariants, brute-force
+per-face loops and library routines (scipy) chosen so that a dropped term, a wrong
+sign/index or a transposed operand fails.  Functions whose only pin would be an import
+of the reference are marked "parity unpinned" in their docstring.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# ---------------------------------------------------------------------------------
+# Physical constants: namelist_n_constants.py:64-75
+# ---------------------------------------------------------------------------------
+Rd = 287.04
+Rv = 461.5
+Cp = 1004.64
+Cv = Cp - Rd
+eps = Rd / Rv
+reps = Rv / Rd
+repsm1 = Rv / Rd - 1.0
+g = 9.80616
+p00 = 100000.0
+fCor = 1.0e-4
+
+NGX, NGY, NGZ = 3, 3, 1   # namelist_n_constants.py:23-25
+
+# WENO small numbers: one_step_integration.py:896 (momentum), :905 (theta), :913 (qv)
+WEPS_MOM = 1.0e-16
+WEPS_THETA = 1.0e-17
+WEPS_QV = 1.0e-18
+
+
+@dataclass(frozen=True)
+class Grid:
+    """Uniform grid (the reference's grids are uniform: setup_lex.py:12-24)."""
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dy: float
+    dz: float
+
+    @property
+    def shape_c(self):
+        return (self.nx + 6, self.ny + 6, self.nz + 2)
+
+    @property
+    def shape_u(self):
+        return (self.nx + 7, self.ny + 6, self.nz + 2)
+
+    @property
+    def shape_v(self):
+        return (self.nx + 6, self.ny + 7, self.nz + 2)
+
+    @property
+    def shape_w(self):
+        return (self.nx + 6, self.ny + 6, self.nz + 3)
+
+
+@dataclass
+class Options:
+    """Mirror of the namelist switches the hot path reads (namelist_n_constants.py:6-50)."""
+    dt: float = 1.0
+    solver_opt: int = 2
+    n_sound: int = 32
+    turb_opt: int = 0
+    cor_opt: int = 0
+    cs: float = 0.18            # turbulence.py:433 (hard-coded there; a runtime scalar here)
+    prandtl: float = 1.0 / 3.0  # turbulence.py:432
+    bicg_tol: float = 1.0e-4    # one_step_integration.py:475
+    bicg_atol: float = 1.0e-8   # one_step_integration.py:476
+    bicg_maxiter: int = 100     # one_step_integration.py:485
+
+
+# ---------------------------------------------------------------------------------
+# small slicing helpers
+# ---------------------------------------------------------------------------------
+def _sl(a, axis, start, stop):
+    idx = [slice(None)] * a.ndim
+    idx[axis] = slice(start, stop)
+    return a[tuple(idx)]
+
+
+def _cat(parts, axis):
+    return np.concatenate(parts, axis=axis)
+
+
+def _interior(a):
+    """[ngx:-ngx, ngy:-ngy, ngz:-ngz] of a scalar-shaped padded array."""
+    return a[NGX:-NGX, NGY:-NGY, NGZ:-NGZ]
+
+
+# ---------------------------------------------------------------------------------
+# H1: halo padding — one_step_integration.py:950-985, pressure_equations.py:111-120
+# ---------------------------------------------------------------------------------
+def _zcopy(a):
+    return _cat([a[:, :, :1], a, a[:, :, -1:]], 2)
+
+
+def padding3_array(arr):
+    """Periodic 3-cell x/y halo, z ghost = nearest level (one_step_integration.py:950-960)."""
+    a = _cat([arr[-3:], arr, arr[:3]], 0)
+    a = _cat([a[:, -3:], a, a[:, :3]], 1)
+    return _zcopy(a)
+
+
+def padding3u_array(arr):
+    """u-point halo: x ghosts skip the duplicate face (one_step_integration.py:963-973)."""
+    a = _cat([arr[-4:-1], arr, arr[1:4]], 0)
+    a = _cat([a[:, -3:], a, a[:, :3]], 1)
+    return _zcopy(a)
+
+
+def padding3v_array(arr):
+    """v-point halo: y ghosts skip the duplicate face (one_step_integration.py:976-985)."""
+    a = _cat([arr[-3:], arr, arr[:3]], 0)
+    a = _cat([a[:, -4:-1], a, a[:, 1:4]], 1)
+    return _zcopy(a)
+
+
+def padding_array(arr):
+    """1-cell periodic x/y halo, z copy (pressure_equations.py:111-120)."""
+    a = _cat([arr[-1:], arr, arr[:1]], 0)
+    a = _cat([a[:, -1:], a, a[:, :1]], 1)
+    return _zcopy(a)
+
+
+def set_w_bc(w):
+    """Zero w on padded levels 0,1 and nz+1,nz+2 (boundary_conditions.py:104-109)."""
+    w = w.copy()
+    w[:, :, 0:2] = 0.0
+    w[:, :, -2:] = 0.0
+    return w
+
+
+# ---------------------------------------------------------------------------------
+# WENO reconstructions (advection.py:305-416 and siblings)
+# Jiang & Shu (1996) candidates and smoothness indicators, Borges et al. (2008)
+# WENO-Z weights with squared ratio and a 1e30 cap.
+# ---------------------------------------------------------------------------------
+def weno5_parts(a, b, c, d, e, weps):
+    """Smoothness indicators, candidate values and weights of one WENO5 face.
+
+    advection.py:311-326: Jiang-Shu beta_k, the three quadratic candidates, and
+    w_k = d_k (1 + min(1e30, |beta1-beta3|/(beta_k+eps))^2) with d = (0.1, 0.6, 0.3).
+    """
+    b1 = (13.0 / 12.0) * (a - 2.0 * b + c) ** 2 + 0.25 * (a - 4.0 * b + 3.0 * c) ** 2
+    b2 = (13.0 / 12.0) * (b - 2.0 * c + d) ** 2 + 0.25 * (b - d) ** 2
+    b3 = (13.0 / 12.0) * (c - 2.0 * d + e) ** 2 + 0.25 * (3.0 * c - 4.0 * d + e) ** 2
+    tau = np.abs(b1 - b3)
+    w1 = 0.1 * (1.0 + np.fmin(1.0e30, tau / (b1 + weps)) ** 2)
+    w2 = 0.6 * (1.0 + np.fmin(1.0e30, tau / (b2 + weps)) ** 2)
+    w3 = 0.3 * (1.0 + np.fmin(1.0e30, tau / (b3 + weps)) ** 2)
+    q1 = (2.0 / 6.0) * a + (-7.0 / 6.0) * b + (11.0 / 6.0) * c
+    q2 = (-1.0 / 6.0) * b + (5.0 / 6.0) * c + (2.0 / 6.0) * d
+    q3 = (2.0 / 6.0) * c + (5.0 / 6.0) * d + (-1.0 / 6.0) * e
+    return (b1, b2, b3), (q1, q2, q3), (w1, w2, w3)
+
+
+def weno5(a, b, c, d, e, weps):
+    """Face value from the upwind-ordered stencil (a, b, c | d, e), c the upwind cell.
+
+    advection.py:311-326 (upward form; the other orientations pass a mirrored stencil).
+    """
+    _, (q1, q2, q3), (w1, w2, w3) = weno5_parts(a, b, c, d, e, weps)
+    return (w1 * q1 + w2 * q2 + w3 * q3) / (w1 + w2 + w3)
+
+
+def weno3(a, b, c, weps):
+    """Face value from (a, b | c), b the upwind cell (advection.py:329-340)."""
+    b1 = (a - b) ** 2
+    b2 = (b - c) ** 2
+    tau = np.abs(b1 - b2)
+    w1 = (1.0 / 3.0) * (1.0 + np.fmin(1.0e30, tau / (b1 + weps)) ** 2)
+    w2 = (2.0 / 3.0) * (1.0 + np.fmin(1.0e30, tau / (b2 + weps)) ** 2)
+    return (w1 * ((-1.0 / 2.0) * a + (3.0 / 2.0) * b) + w2 * ((1.0 / 2.0) * b + (1.0 / 2.0) * c)) / (w1 + w2)
+
+
+def weno5_faces(s, axis, weps):
+    """Both upwind reconstructions at the L-5 faces of a padded axis of length L.
+
+    Face g lies between s[g+2] and s[g+3] (advection.py:425-470 for x, :477-513 for y).
+    Returns (positive-flow value, negative-flow value).
+    """
+    L = s.shape[axis]
+    t = [_sl(s, axis, k, L - 5 + k) for k in range(6)]
+    up = weno5(t[0], t[1], t[2], t[3], t[4], weps)
+    dn = weno5(t[5], t[4], t[3], t[2], t[1], weps)
+    return up, dn
+
+
+def vertical_faces(s, weps):
+    """Vertical reconstruction table on a padded z-column of length L (L-1 faces).
+
+    Face f lies between s[..., f] and s[..., f+1] (advection.py:305-416 for scalars,
+    :561-675 u, :816-930 v, :1070-1183 w).  Faces 0 and L-2 are zero (filled later by
+    boundary conditions).  Upward flow: f=1 centred, f=2 WENO3, f=3..L-4 WENO5,
+    f=L-3 WENO3.  Downward: f=1 WENO3, f=2..L-5 WENO5, f=L-4 WENO3, f=L-3 centred.
+    """
+    L = s.shape[2]
+    p = [s[:, :, k] for k in range(L)]
+    zero = np.zeros_like(p[0])
+    up = [zero] * (L - 1)
+    dn = [zero] * (L - 1)
+    up[1] = 0.5 * (p[1] + p[2])
+    up[2] = weno3(p[1], p[2], p[3], weps)
+    up[L - 3] = weno3(p[L - 4], p[L - 3], p[L - 2], weps)
+    dn[1] = weno3(p[3], p[2], p[1], weps)
+    dn[L - 4] = weno3(p[L - 2], p[L - 3], p[L - 4], weps)
+    dn[L - 3] = 0.5 * (p[L - 2] + p[L - 3])
+    # WENO5 interior faces computed as whole slabs, then spliced in
+    f_up = list(range(3, L - 3))
+    f_dn = list(range(2, L - 4))
+    if f_up:
+        a = np.stack([p[f - 2] for f in f_up], 2)
+        b = np.stack([p[f - 1] for f in f_up], 2)
+        c = np.stack([p[f] for f in f_up], 2)
+        d = np.stack([p[f + 1] for f in f_up], 2)
+        e = np.stack([p[f + 2] for f in f_up], 2)
+        v = weno5(a, b, c, d, e, weps)
+        for n, f in enumerate(f_up):
+            up[f] = v[:, :, n]
+    if f_dn:
+        a = np.stack([p[f + 3] for f in f_dn], 2)
+        b = np.stack([p[f + 2] for f in f_dn], 2)
+        c = np.stack([p[f + 1] for f in f_dn], 2)
+        d = np.stack([p[f] for f in f_dn], 2)
+        e = np.stack([p[f - 1] for f in f_dn], 2)
+        v = weno5(a, b, c, d, e, weps)
+        for n, f in enumerate(f_dn):
+            dn[f] = v[:, :, n]
+    return np.stack(up, 2), np.stack(dn, 2)
+
+
+def _upwind(mass, up, dn):
+    """lax.select(mass >= 0, mass*up, mass*dn) (advection.py:412, :467, :513)."""
+    return np.where(mass >= 0.0, mass * up, mass * dn)
+
+
+# ---------------------------------------------------------------------------------
+# A1: divergences — advection.py:200-258
+# ---------------------------------------------------------------------------------
+def _rho_faces_x(rho):
+    part = 0.5 * (rho[:-1] + rho[1:])
+    return _cat([part[-1:], part, part[:1]], 0)
+
+
+def _rho_faces_y(rho):
+    part = 0.5 * (rho[:, :-1] + rho[:, 1:])
+    return _cat([part[:, -1:], part, part[:, :1]], 1)
+
+
+def _rho_faces_z(rho):
+    part = 0.5 * (rho[:, :, :-1] + rho[:, :, 1:])
+    z = np.zeros_like(part[:, :, :1])
+    return _cat([z, part, z], 2)
+
+
+def get_divergence(rho0, u, v, w, grid):
+    """div(rho0 u) on the whole padded scalar array (advection.py:200-226)."""
+    ru = _rho_faces_x(rho0) * u
+    rv = _rho_faces_y(rho0) * v
+    rw = _rho_faces_z(rho0) * w
+    return ((ru[1:] - ru[:-1]) / grid.dx + (rv[:, 1:] - rv[:, :-1]) / grid.dy
+            + (rw[:, :, 1:] - rw[:, :, :-1]) / grid.dz)
+
+
+def get_divergence2(u, v, w, grid):
+    """div(u) on the padded scalar array (advection.py:229-236)."""
+    return ((u[1:] - u[:-1]) / grid.dx + (v[:, 1:] - v[:, :-1]) / grid.dy
+            + (w[:, :, 1:] - w[:, :, :-1]) / grid.dz)
+
+
+def get_2d_divergence(rho, u, v, grid):
+    """Horizontal div(rho u, rho v) (advection.py:239-258)."""
+    ru = _rho_faces_x(rho) * u
+    rv = _rho_faces_y(rho) * v
+    return (ru[1:] - ru[:-1]) / grid.dx + (rv[:, 1:] - rv[:, :-1]) / grid.dy
+
+
+# ---------------------------------------------------------------------------------
+# A2: scalar advection — advection.py:9-34, :305-416, :419-516
+# ---------------------------------------------------------------------------------
+def vertical_flux_scalar2(weps, rho0, w, s):
+    """(advection.py:305-416) -> (nx, ny, nz+1)."""
+    up, dn = vertical_faces(s, weps)
+    mass = 0.5 * (rho0[:, :, :-1] + rho0[:, :, 1:]) * w[:, :, 1:-1]
+    return _upwind(mass, up, dn)[NGX:-NGX, NGY:-NGY, :]
+
+
+def horizontal_flux_scalar(weps, rho0, u, v, s):
+    """(advection.py:419-516) -> (nx+1, ny, nz), (nx, ny+1, nz)."""
+    up, dn = weno5_faces(s, 0, weps)
+    mass = 0.5 * (rho0[2:-3] + rho0[3:-2]) * u[3:-3]
+    fx = _upwind(mass, up, dn)[:, NGY:-NGY, NGZ:-NGZ]
+    up, dn = weno5_faces(s, 1, weps)
+    mass = 0.5 * (rho0[:, 2:-3] + rho0[:, 3:-2]) * v[:, 3:-3]
+    fy = _upwind(mass, up, dn)[NGX:-NGX, :, NGZ:-NGZ]
+    return fx, fy
+
+
+def _flux_div(fx, fy, fz, grid):
+    return -((fx[1:] - fx[:-1]) / grid.dx + (fy[:, 1:] - fy[:, :-1]) / grid.dy
+             + (fz[:, :, 1:] - fz[:, :, :-1]) / grid.dz)
+
+
+def advection_scalar(rho0, s, u, v, w, weps, flow_divergence, sfc_flux, grid):
+    """[-div F + s div(rho0 u)] / rho0 at interior cells (advection.py:9-34)."""
+    fz = vertical_flux_scalar2(weps, rho0, w, s)
+    fz[:, :, 0] = sfc_flux
+    fx, fy = horizontal_flux_scalar(weps, rho0, u, v, s)
+    conv = _flux_div(fx, fy, fz, grid)
+    return (conv + _interior(s) * _interior(flow_divergence)) / _interior(rho0)
+
+
+# ---------------------------------------------------------------------------------
+# A3: momentum advection — advection.py:37-149, :561-675, :678-771, :816-1025,
+# :1070-1281
+# ---------------------------------------------------------------------------------
+def vertical_flux_u2(weps, rho0, w, u):
+    """(advection.py:561-675) -> (nx+1, ny, nz+1)."""
+    up, dn = vertical_faces(u, weps)
+    m = 0.5 * (rho0[:, :, :-1] + rho0[:, :, 1:]) * w[:, :, 1:-1]
+    m8u = 0.5 * (m[:-1] + m[1:])
+    return _upwind(m8u, up[1:-1], dn[1:-1])[NGX - 1:-(NGX - 1), NGY:-NGY, :]
+
+
+def horizontal_flux_u(weps, rho0, u, v):
+    """(advection.py:678-771) -> (nx+2, ny, nz) x-flux at cells, (nx+1, ny+1, nz) y-flux at corners."""
+    up, dn = weno5_faces(u, 0, weps)
+    mass = 0.5 * (u[2:-3] + u[3:-2]) * rho0[2:-2]
+    fx = _upwind(mass, up, dn)[:, NGY:-NGY, NGZ:-NGZ]
+    up, dn = weno5_faces(u, 1, weps)
+    mv = 0.5 * (rho0[:, 2:-3] + rho0[:, 3:-2]) * v[:, 3:-3]
+    mv8u = 0.5 * (mv[:-1] + mv[1:])
+    fy = _upwind(mv8u, up[1:-1], dn[1:-1])[NGX - 1:-(NGX - 1), :, NGZ:-NGZ]
+    return fx, fy
+
+
+def advection_u(rho0, u, v, w, weps, flow_divergence, u_sfc_flux, grid):
+    """u tendency at the nx+1 interior u faces (advection.py:37-74)."""
+    fz = vertical_flux_u2(weps, rho0, w, u)
+    tx = _cat([u_sfc_flux[-1:], u_sfc_flux, u_sfc_flux[:1]], 0)
+    fz[:, :, 0] = 0.5 * (tx[:-1] + tx[1:])
+    fx, fy = horizontal_flux_u(weps, rho0, u, v)
+    conv = _flux_div(fx, fy, fz, grid)
+    fd = flow_divergence[:, NGY:-NGY, NGZ:-NGZ]
+    div8u = 0.5 * (fd[NGX - 1:-NGX] + fd[NGX:-(NGX - 1)])
+    r = rho0[:, NGY:-NGY, NGZ:-NGZ]
+    rho8u = 0.5 * (r[NGX - 1:-NGX] + r[NGX:-(NGX - 1)])
+    return (conv + u[NGX:-NGX, NGY:-NGY, NGZ:-NGZ] * div8u) / rho8u
+
+
+def vertical_flux_v2(weps, rho0, w, v):
+    """(advection.py:816-930) -> (nx, ny+1, nz+1)."""
+    up, dn = vertical_faces(v, weps)
+    m = 0.5 * (rho0[:, :, :-1] + rho0[:, :, 1:]) * w[:, :, 1:-1]
+    m8v = 0.5 * (m[:, :-1] + m[:, 1:])
+    return _upwind(m8v, up[:, 1:-1], dn[:, 1:-1])[NGX:-NGX, NGY - 1:-(NGY - 1), :]
+
+
+def horizontal_flux_v(weps, rho0, u, v):
+    """(advection.py:933-1025) -> (nx+1, ny+1, nz) x-flux at corners, (nx, ny+2, nz) y-flux at cells."""
+    up, dn = weno5_faces(v, 0, weps)
+    mu = 0.5 * (rho0[2:-3] + rho0[3:-2]) * u[3:-3]
+    mu8v = 0.5 * (mu[:, :-1] + mu[:, 1:])
+    fx = _upwind(mu8v, up[:, 1:-1], dn[:, 1:-1])[:, NGY - 1:-(NGY - 1), NGZ:-NGZ]
+    up, dn = weno5_faces(v, 1, weps)
+    mass = 0.5 * (v[:, 2:-3] + v[:, 3:-2]) * rho0[:, 2:-2]
+    fy = _upwind(mass, up, dn)[NGX:-NGX, :, NGZ:-NGZ]
+    return fx, fy
+
+
+def advection_v(rho0, u, v, w, weps, flow_divergence, v_sfc_flux, grid):
+    """v tendency at the ny+1 interior v faces (advection.py:77-114)."""
+    fz = vertical_flux_v2(weps, rho0, w, v)
+    ty = _cat([v_sfc_flux[:, -1:], v_sfc_flux, v_sfc_flux[:, :1]], 1)
+    fz[:, :, 0] = 0.5 * (ty[:, :-1] + ty[:, 1:])
+    fx, fy = horizontal_flux_v(weps, rho0, u, v)
+    conv = _flux_div(fx, fy, fz, grid)
+    fd = flow_divergence[NGX:-NGX, :, NGZ:-NGZ]
+    div8v = 0.5 * (fd[:, NGY - 1:-NGY] + fd[:, NGY:-(NGY - 1)])
+    r = rho0[NGX:-NGX, :, NGZ:-NGZ]
+    rho8v = 0.5 * (r[:, NGY - 1:-NGY] + r[:, NGY:-(NGY - 1)])
+    return (conv + v[NGX:-NGX, NGY:-NGY, NGZ:-NGZ] * div8v) / rho8v
+
+
+def vertical_flux_w2(weps, rho0, w):
+    """(advection.py:1070-1183) -> (nx, ny, nz+2) at scalar levels incl. ghosts."""
+    up, dn = vertical_faces(w, weps)
+    mass = 0.5 * (w[:, :, :-1] + w[:, :, 1:]) * rho0
+    return _upwind(mass, up, dn)[NGX:-NGX, NGY:-NGY, :]
+
+
+def horizontal_flux_w(weps, rho0, u, v, w):
+    """(advection.py:1186-1281) -> (nx+1, ny, nz+1), (nx, ny+1, nz+1)."""
+    up, dn = weno5_faces(w, 0, weps)
+    mu = 0.5 * (rho0[2:-3] + rho0[3:-2]) * u[3:-3]
+    mu8w = 0.5 * (mu[:, :, :-1] + mu[:, :, 1:])
+    fx = _upwind(mu8w, up[:, :, 1:-1], dn[:, :, 1:-1])[:, NGY:-NGY, :]
+    up, dn = weno5_faces(w, 1, weps)
+    mv = 0.5 * (rho0[:, 2:-3] + rho0[:, 3:-2]) * v[:, 3:-3]
+    mv8w = 0.5 * (mv[:, :, :-1] + mv[:, :, 1:])
+    fy = _upwind(mv8w, up[:, :, 1:-1], dn[:, :, 1:-1])[NGX:-NGX, :, :]
+    return fx, fy
+
+
+def advection_w(rho0, u, v, w, weps, flow_divergence, grid):
+    """w tendency at the nz+1 interior w levels, zero at surface and lid (advection.py:117-149)."""
+    fz = vertical_flux_w2(weps, rho0, w)
+    fx, fy = horizontal_flux_w(weps, rho0, u, v, w)
+    conv = _flux_div(fx, fy, fz, grid)
+    fd = flow_divergence[NGX:-NGX, NGY:-NGY, :]
+    div8w = 0.5 * (fd[:, :, :-1] + fd[:, :, 1:])
+    r = rho0[NGX:-NGX, NGY:-NGY, :]
+    rho8w = 0.5 * (r[:, :, :-1] + r[:, :, 1:])
+    t = (conv + w[NGX:-NGX, NGY:-NGY, NGZ:-NGZ] * div8w) / rho8w
+    t[:, :, 0] = 0.0
+    t[:, :, -1] = 0.0
+    return t
+
+
+# ---------------------------------------------------------------------------------
+# A4: low-order pi' advection — advection.py:152-197
+# ---------------------------------------------------------------------------------
+def advection_pi(rho0, pi0, pip, u, v, w, grid, cc1=0.5, cc2=0.5):
+    """-(u.grad pi) with face products averaged to cells (advection.py:152-197)."""
+    I = slice(NGX, -NGX)
+    J = slice(NGY, -NGY)
+    K = slice(NGZ, -NGZ)
+    dpi_dx = (pip[NGX:-NGX + 1, J, K] - pip[NGX - 1:-NGX, J, K]) / grid.dx
+    dpi_dy = (pip[I, NGY:-NGY + 1, K] - pip[I, NGY - 1:-NGY, K]) / grid.dy
+    dpi0_dz = (pi0[I, J, NGZ:] - pi0[I, J, :-NGZ]) / grid.dz
+    dpip_dz = (pip[I, J, NGZ:] - pip[I, J, :-NGZ]) / grid.dz
+    dpi_dz = dpi0_dz + dpip_dz
+    udx = u[I, J, K] * dpi_dx
+    vdy = v[I, J, K] * dpi_dy
+    wdz = w[I, J, K] * dpi_dz
+    wdz[:, :, 0] = 0.0
+    wdz[:, :, -1] = 0.0
+    r = rho0[I, J, :]
+    rho8w = 0.5 * (r[:, :, :-1] + r[:, :, 1:])
+    return -(0.5 * (udx[1:] + udx[:-1]) + 0.5 * (vdy[:, 1:] + vdy[:, :-1])
+             + (cc1 * wdz[:, :, :-1] * rho8w[:, :, :-1] + cc2 * wdz[:, :, 1:] * rho8w[:, :, 1:])
+             / rho0[I, J, K])
+
+
+# ---------------------------------------------------------------------------------
+# B1 / P3: buoyancy, rtt, pressure equation — pressure_equations.py
+# ---------------------------------------------------------------------------------
+def extrapolate_bottom_top(s):
+    """2nd-order Lagrange extrapolation half a level beyond the end levels (pressure_equations.py:141-159)."""
+    bottom = 1.875 * s[:, :, 0] + (-1.25) * s[:, :, 1] + 0.375 * s[:, :, 2]
+    top = 1.875 * s[:, :, -1] + (-1.25) * s[:, :, -2] + 0.375 * s[:, :, -3]
+    return bottom[:, :, None], top[:, :, None]
+
+
+def interpolate_scalar2w(s):
+    """(pressure_equations.py:123-128) (nx,ny,nz) -> (nx,ny,nz+1)."""
+    b, t = extrapolate_bottom_top(s)
+    return _cat([b, 0.5 * (s[:, :, :-1] + s[:, :, 1:]), t], 2)
+
+
+def interpolate_scalar2w_0(s):
+    """(pressure_equations.py:131-138) zero end levels."""
+    z = np.zeros_like(s[:, :, :1])
+    return _cat([z, 0.5 * (s[:, :, :-1] + s[:, :, 1:]), z], 2)
+
+
+def theta_rho_of(theta, qv):
+    """Density potential temperature theta(1+Rv/Rd qv)/(1+qv) (pressure_equations.py:167)."""
+    return theta * (1.0 + reps * qv) / (1.0 + qv)
+
+
+def calculate_buoyancy(theta0, theta, qv0, qv):
+    """B and B at w levels (pressure_equations.py:162-172)."""
+    tr0 = theta0 * (1.0 + reps * qv0) / (1.0 + qv0)
+    tr = theta * (1.0 + reps * qv) / (1.0 + qv)
+    b = g * (tr - tr0) / tr0
+    bi = _interior(b)
+    return bi, interpolate_scalar2w(bi)
+
+
+def calculate_rtt(rho0_theta0, theta, qv, buoyancy):
+    """rho0 theta0 theta_rho with extrapolated rho0theta0 B/Cp in the z ghosts (pressure_equations.py:175-195)."""
+    part = rho0_theta0[:, :, 1:-1] * theta[:, :, 1:-1] * (1.0 + reps * qv[:, :, 1:-1]) / (1.0 + qv[:, :, 1:-1])
+    b, t = extrapolate_bottom_top(_interior(rho0_theta0) * buoyancy / Cp)
+
+    def pad2(p):
+        p = p[:, :, 0]
+        p = _cat([p[-3:], p, p[:3]], 0)
+        p = _cat([p[:, -3:], p, p[:, :3]], 1)
+        return p[:, :, None]
+    return _cat([pad2(b), part, pad2(t)], 2)
+
+
+def laplace_of_pressure(rtt, pi, grid):
+    """Affine matvec Cp div(rtt_face grad pi) with rtt-ghost boundary z-fluxes (pressure_equations.py:8-48)."""
+    pp = padding_array(pi)
+    dpx = (pp[1:, 1:-1, 1:-1] - pp[:-1, 1:-1, 1:-1]) / grid.dx
+    dpy = (pp[1:-1, 1:, 1:-1] - pp[1:-1, :-1, 1:-1]) / grid.dy
+    dpz = (pp[1:-1, 1:-1, 1:] - pp[1:-1, 1:-1, :-1]) / grid.dz
+    I = slice(NGX, -NGX)
+    J = slice(NGY, -NGY)
+    K = slice(NGZ, -NGZ)
+    px = 0.5 * (rtt[NGX:-(NGX - 1), J, K] + rtt[NGX - 1:-NGX, J, K]) * dpx
+    py = 0.5 * (rtt[I, NGY:-(NGY - 1), K] + rtt[I, NGY - 1:-NGY, K]) * dpy
+    pz = 0.5 * (rtt[I, J, NGZ:] + rtt[I, J, :-NGZ]) * dpz
+    pz[:, :, 0] = rtt[I, J, 0]
+    pz[:, :, -1] = rtt[I, J, -1]
+    return Cp * ((px[1:] - px[:-1]) / grid.dx + (py[:, 1:] - py[:, :-1]) / grid.dy
+                 + (pz[:, :, 1:] - pz[:, :, :-1]) / grid.dz)
+
+
+def rhs_of_pressure_equation(rho0_theta0, adv4u, adv4v, adv4w, fu, fv, buoyancy, grid):
+    """RHS = div(rt adv) + div_h(rt (fv,-fu)) + d/dz(rt B) (pressure_equations.py:51-77)."""
+    rt = _interior(rho0_theta0)
+    # get_divergence on un-padded arrays (pressure_equations.py:54-57)
+    ru = _rho_faces_x(rt) * adv4u
+    rv = _rho_faces_y(rt) * adv4v
+    rw = _rho_faces_z(rt) * adv4w
+    rhs_adv = ((ru[1:] - ru[:-1]) / grid.dx + (rv[:, 1:] - rv[:, :-1]) / grid.dy
+               + (rw[:, :, 1:] - rw[:, :, :-1]) / grid.dz)
+    rhs_cor = get_2d_divergence(rt, fv, -fu, grid)
+    rtb = interpolate_scalar2w(rt * buoyancy)
+    rhs_buoy = (rtb[:, :, 1:] - rtb[:, :, :-1]) / grid.dz
+    return rhs_adv + rhs_cor + rhs_buoy
+
+
+# ---------------------------------------------------------------------------------
+# P4: BiCGSTAB with jax.scipy.sparse.linalg.bicgstab semantics (third-party; the
+# reference calls it at one_step_integration.py:482-485).  Restated from JAX's public
+# _bicgstab_solve: unpreconditioned, atol2 = max(tol^2 |b|^2, atol^2), breakdown codes.
+# The exact iterate trajectory in FP32 is "parity unpinned" (reduction order).
+# ---------------------------------------------------------------------------------
+def _dot(a, b):
+    return float(np.dot(a.ravel().astype(np.float64), b.ravel().astype(np.float64)))
+
+
+def bicgstab(A, b, x0, tol, atol, maxiter):
+    """Returns (x, k) with k the iteration count or -10/-11 on breakdown."""
+    dt = b.dtype
+    bs = _dot(b, b)
+    atol2 = max(tol * tol * bs, atol * atol)
+    x = x0.copy()
+    r = b - A(x)
+    rhat = r.copy()
+    p = r.copy()
+    q = r.copy()
+    rho = alpha = omega = np.float64(1.0)
+    k = 0
+    while _dot(r, r) > atol2 and k < maxiter and k >= 0:
+        with np.errstate(divide="ignore", invalid="ignore", over="ignore"):  # IEEE inf/nan, as in JAX
+            rho_ = np.float64(_dot(rhat, r))
+            beta = rho_ / rho * alpha / omega
+            p = (r + dt.type(beta) * (p - dt.type(omega) * q)).astype(dt)
+            q = A(p)
+            alpha_ = rho_ / np.float64(_dot(rhat, q))
+            s = (r - dt.type(alpha_) * q).astype(dt)
+            early = _dot(s, s) < atol2
+            t = A(s)
+            omega_ = np.float64(_dot(t, s)) / np.float64(_dot(t, t))
+        if early:
+            x = (x + dt.type(alpha_) * p).astype(dt)
+            r = s
+        else:
+            x = (x + (dt.type(alpha_) * p + dt.type(omega_) * s)).astype(dt)
+            r = (s - dt.type(omega_) * t).astype(dt)
+        knew = -11 if (omega_ == 0.0 or alpha_ == 0.0) else k + 1
+        if rho_ == 0.0:
+            knew = -10
+        k = knew
+        rho, alpha, omega = rho_, alpha_, omega_
+    return x, k
+
+
+def solve_pres_eqn(pip_prev, rho0_theta0, rtt, adv4u, adv4v, adv4w, fu, fv, buoyancy, grid, opts):
+    """(one_step_integration.py:468-487) -> interior pi' and the BiCGSTAB exit code."""
+    rhs = rhs_of_pressure_equation(rho0_theta0, adv4u, adv4v, adv4w, fu, fv, buoyancy, grid)
+    A = lambda beta: laplace_of_pressure(rtt, beta, grid)  # noqa: E731
+    return bicgstab(A, rhs, _interior(pip_prev), opts.bicg_tol, opts.bicg_atol, opts.bicg_maxiter)
+
+
+# ---------------------------------------------------------------------------------
+# P1 / P2: pressure gradient and Coriolis — pressure_gradient_coriolis.py:7-95
+# ---------------------------------------------------------------------------------
+def pressure_gradient_force(pip, theta_rho, grid):
+    """-Cp theta_rho_face dpi'/dn on u, v, w interiors; w zero at surface/lid (pressure_gradient_coriolis.py:7-37)."""
+    I = slice(NGX, -NGX)
+    J = slice(NGY, -NGY)
+    K = slice(NGZ, -NGZ)
+    th = theta_rho
+    dx = (pip[NGX:-(NGX - 1), J, K] - pip[NGX - 1:-NGX, J, K]) / grid.dx
+    pu = -Cp * 0.5 * (th[NGX:-(NGX - 1), J, K] + th[NGX - 1:-NGX, J, K]) * dx
+    dy = (pip[I, NGY:-(NGY - 1), K] - pip[I, NGY - 1:-NGY, K]) / grid.dy
+    pv = -Cp * 0.5 * (th[I, NGY:-(NGY - 1), K] + th[I, NGY - 1:-NGY, K]) * dy
+    dz = (pip[I, J, NGZ:] - pip[I, J, :-NGZ]) / grid.dz
+    pw = -Cp * 0.5 * (th[I, J, NGZ:] + th[I, J, :-NGZ]) * dz
+    pw[:, :, 0] = 0.0
+    pw[:, :, -1] = 0.0
+    return pu, pv, pw
+
+
+def calculate_coriolis_force(u, v, f=fCor):
+    """fu at v points, fv at u points (pressure_gradient_coriolis.py:80-95)."""
+    K = slice(NGZ, -NGZ)
+    fv = f * 0.25 * (v[NGX - 1:-NGX, NGY:-(NGY + 1), K] + v[NGX - 1:-NGX, NGY + 1:-NGY, K]
+                     + v[NGX:-(NGX - 1), NGY:-(NGY + 1), K] + v[NGX:-(NGX - 1), NGY + 1:-NGY, K])
+    fu = f * 0.25 * (u[NGX:-(NGX + 1), NGY - 1:-NGY, K] + u[NGX:-(NGX + 1), NGY:-(NGY - 1), K]
+                     + u[NGX + 1:-NGX, NGY - 1:-NGY, K] + u[NGX + 1:-NGX, NGY:-(NGY - 1), K])
+    return fu, fv
+
+
+# ---------------------------------------------------------------------------------
+# G1: Smagorinsky — turbulence.py:9-111, :298-446, :474-504
+# ---------------------------------------------------------------------------------
+def compute_deformation(rho, u, v, w, grid):
+    """rho S_ij at cells and the deformation (turbulence.py:298-388); all padded."""
+    I = slice(NGX, -NGX)
+    J = slice(NGY, -NGY)
+    K = slice(NGZ, -NGZ)
+    r = rho[I, J, K]
+    s11 = r * (u[NGX + 1:-NGX, J, K] - u[NGX:-(NGX + 1), J, K]) / grid.dx
+    s22 = r * (v[I, NGY + 1:-NGY, K] - v[I, NGY:-(NGY + 1), K]) / grid.dy
+    s33 = r * (w[I, J, NGZ + 1:-NGZ] - w[I, J, NGZ:-(NGZ + 1)]) / grid.dz
+    dudy = (u[I, NGY:-(NGY - 1), K] - u[I, NGY - 1:-NGY, K]) / grid.dy
+    dvdx = (v[NGX:-(NGX - 1), NGY:-NGY, K] - v[NGX - 1:-NGX, NGY:-NGY, K]) / grid.dx
+    e12 = 0.5 * (dudy + dvdx)
+    s12 = r * (0.25 * (e12[:-1, :-1] + e12[1:, 1:] + e12[:-1, 1:] + e12[1:, :-1]))
+    dudz = (u[I, J, NGZ + 1:-NGZ] - u[I, J, NGZ:-(NGZ + 1)]) / grid.dz
+    dwdx = (w[NGX:-(NGX - 1), J, NGZ + 1:-(NGZ + 1)] - w[NGX - 1:-NGX, J, NGZ + 1:-(NGZ + 1)]) / grid.dx
+    e13 = 0.5 * (dudz + dwdx)
+    b0, t0 = extrapolate_bottom_top(e13)
+    s13 = _cat([0.5 * (b0[1:] + b0[:-1]),
+                0.25 * (e13[:-1, :, :-1] + e13[1:, :, 1:] + e13[:-1, :, 1:] + e13[1:, :, :-1]),
+                0.5 * (t0[1:] + t0[:-1])], 2)
+    s13 = r * s13
+    dvdz = (v[I, J, NGZ + 1:-NGZ] - v[I, J, NGZ:-(NGZ + 1)]) / grid.dz
+    dwdy = (w[I, NGY:-(NGY - 1), NGZ + 1:-(NGZ + 1)] - w[I, NGY - 1:-NGY, NGZ + 1:-(NGZ + 1)]) / grid.dy
+    e23 = 0.5 * (dvdz + dwdy)
+    b0, t0 = extrapolate_bottom_top(e23)
+    s23 = _cat([0.5 * (b0[:, 1:] + b0[:, :-1]),
+                0.25 * (e23[:, :-1, :-1] + e23[:, 1:, 1:] + e23[:, :-1, 1:] + e23[:, 1:, :-1]),
+                0.5 * (t0[:, 1:] + t0[:, :-1])], 2)
+    s23 = r * s23
+    deform = (2.0 * (s11 ** 2 + s22 ** 2 + s33 ** 2) + 4.0 * (s12 ** 2 + s13 ** 2 + s23 ** 2)) / r ** 2
+    return tuple(padding3_array(a) for a in (s11, s22, s33, s12, s13, s23, deform))
+
+
+def compute_nm(theta, qv, grid):
+    """N^2 = g d ln(theta(1+repsm1 qv))/dz at cells, extrapolated ends (turbulence.py:391-427)."""
+    tr = theta * (1.0 + repsm1 * qv)
+    I = slice(NGX, -NGX)
+    J = slice(NGY, -NGY)
+    n28w = np.log(tr[I, J, NGZ + 1:-NGZ] / tr[I, J, NGZ:-(NGZ + 1)]) / grid.dz * g
+    b, t = extrapolate_bottom_top(n28w)
+    return padding3_array(_cat([b, 0.5 * (n28w[:, :, :-1] + n28w[:, :, 1:]), t], 2))
+
+
+def compute_k(s2, n2, grid, cs=0.18, prandtl=1.0 / 3.0):
+    """Km = (Cs Delta)^2 sqrt(max(S2 - N2/Pr, 0)), Kh = Km/Pr (turbulence.py:430-446)."""
+    gs = np.cbrt(np.asarray(grid.dx * grid.dy * grid.dz, dtype=s2.dtype))
+    s2n2 = s2 - n2 / prandtl
+    s2n2 = np.where(s2n2 > 0.0, s2n2, 0.0)
+    km = (s2.dtype.type(cs) * gs) ** 2 * np.sqrt(s2n2)
+    return km, km / prandtl
+
+
+def stagger_scalar(rho):
+    """(turbulence.py:497-504)."""
+    J = slice(NGY, -NGY)
+    K = slice(NGZ, -NGZ)
+    I = slice(NGX, -NGX)
+    r8u = 0.5 * (rho[NGX:-(NGX - 1), J, K] + rho[NGX - 1:-NGX, J, K])
+    r8v = 0.5 * (rho[I, NGY:-(NGY - 1), K] + rho[I, NGY - 1:-NGY, K])
+    r8w = interpolate_scalar2w(rho[I, J, K])
+    return r8u, r8v, r8w
+
+
+def compute_scalar_gradient(s, grid):
+    """(turbulence.py:474-494)."""
+    I = slice(NGX, -NGX)
+    J = slice(NGY, -NGY)
+    K = slice(NGZ, -NGZ)
+    dx = (s[NGX:-(NGX - 1), J, K] - s[NGX - 1:-NGX, J, K]) / grid.dx
+    dy = (s[I, NGY:-(NGY - 1), K] - s[I, NGY - 1:-NGY, K]) / grid.dy
+    dz = (s[I, J, NGZ:] - s[I, J, :-NGZ]) / grid.dz
+    dz[:, :, 0] = 0.0
+    dz[:, :, -1] = 0.0
+    return dx, dy, dz
+
+
+def _scalar_sgs(rho, kh, s, grid):
+    """SGS diffusion of one scalar (turbulence.py:80-109)."""
+    r8u, r8v, r8w = stagger_scalar(rho)
+    k8u, k8v, k8w = stagger_scalar(kh)
+    dx, dy, dz = compute_scalar_gradient(s, grid)
+    tx = -r8u * k8u * dx
+    ty = -r8v * k8v * dy
+    tz = -r8w * k8w * dz
+    tz[:, :, 0] = 0.0
+    tz[:, :, -1] = 0.0
+    return -((tx[1:] - tx[:-1]) / grid.dx + (ty[:, 1:] - ty[:, :-1]) / grid.dy
+             + (tz[:, :, 1:] - tz[:, :, :-1]) / grid.dz) / _interior(rho)
+
+
+def compute_smag(rho, km, kh, s11, s22, s33, s12, s13, s23, scalars, grid):
+    """SGS tendencies for u, v, w and each scalar in `scalars` (turbulence.py:9-111)."""
+    I = slice(NGX, -NGX)
+    J = slice(NGY, -NGY)
+    K = slice(NGZ, -NGZ)
+    t11 = -2.0 * km * s11
+    t22 = -2.0 * km * s22
+    t33 = -2.0 * km * s33
+    t12 = -2.0 * km * s12
+    t13 = -2.0 * km * s13
+    t23 = -2.0 * km * s23
+    d11 = (t11[NGX:-(NGX - 1), J, K] - t11[NGX - 1:-NGX, J, K]) / grid.dx
+    d12y = (t12[NGX - 1:-(NGX - 1), NGY:-(NGY - 1), K] - t12[NGX - 1:-(NGX - 1), NGY - 1:-NGY, K]) / grid.dy
+    d12 = 0.25 * (d12y[:-1, :-1] + d12y[1:, 1:] + d12y[:-1, 1:] + d12y[1:, :-1])
+    t13w = interpolate_scalar2w_0(t13[:, :, K])
+    d13s = (t13w[:, :, 1:] - t13w[:, :, :-1]) / grid.dz
+    d13 = 0.5 * (d13s[NGX - 1:-NGX, J] + d13s[NGX:-(NGX - 1), J])
+    sgs_u = -(d11 + d12 + d13) / (0.5 * (rho[NGX:-(NGX - 1), J, K] + rho[NGX - 1:-NGX, J, K]))
+
+    d22 = (t22[I, NGY:-(NGY - 1), K] - t22[I, NGY - 1:-NGY, K]) / grid.dy
+    d12x = (t12[NGX:-(NGX - 1), NGY - 1:-(NGY - 1), K] - t12[NGX - 1:-NGX, NGY - 1:-(NGY - 1), K]) / grid.dx
+    d21 = 0.25 * (d12x[:-1, :-1] + d12x[1:, 1:] + d12x[:-1, 1:] + d12x[1:, :-1])
+    t23w = interpolate_scalar2w_0(t23[:, :, K])
+    d23s = (t23w[:, :, 1:] - t23w[:, :, :-1]) / grid.dz
+    d23 = 0.5 * (d23s[I, NGY - 1:-NGY] + d23s[I, NGY:-(NGY - 1)])
+    sgs_v = -(d21 + d22 + d23) / (0.5 * (rho[I, NGY:-(NGY - 1), K] + rho[I, NGY - 1:-NGY, K]))
+
+    d33 = (t33[I, J, NGZ:] - t33[I, J, :-NGZ]) / grid.dz
+    d13x = (t13[NGX + 1:-(NGX - 1), J, K] - t13[NGX - 1:-(NGX + 1), J, K]) / (2.0 * grid.dx)
+    d23y = (t23[I, NGY + 1:-(NGY - 1), K] - t23[I, NGY - 1:-(NGY + 1), K]) / (2.0 * grid.dy)
+    r8w = interpolate_scalar2w(rho[I, J, K])
+    sgs_w = -(interpolate_scalar2w_0(d13x) + interpolate_scalar2w_0(d23y) + d33) / r8w
+    sgs_w[:, :, 0] = 0.0
+    sgs_w[:, :, -1] = 0.0
+    sgs_s = tuple(_scalar_sgs(rho, kh, s, grid) for s in scalars)
+    return (sgs_u, sgs_v, sgs_w) + sgs_s
+
+
+def smagorinsky(rho0, u, v, w, theta, qv, scalars, grid, cs=0.18, prandtl=1.0 / 3.0):
+    """compute_deformation -> compute_nm -> compute_k -> compute_smag (one_step_integration.py:58-74)."""
+    s11, s22, s33, s12, s13, s23, deform = compute_deformation(rho0, u, v, w, grid)
+    n2 = compute_nm(theta, qv, grid)
+    km, kh = compute_k(deform, n2, grid, cs, prandtl)
+    return compute_smag(rho0, km, kh, s11, s22, s33, s12, s13, s23, scalars, grid)
+
+
+# ---------------------------------------------------------------------------------
+# S1 / M1 / E1: acoustic substeps and Euler updates — one_step_integration.py
+# ---------------------------------------------------------------------------------
+def sound_rk_step(u_now, v_now, w_now, pi0, pip_now, pip_star, rhs_u, rhs_v, rhs_w,
+                  theta_rho, rho0, n, dt_sound, grid):
+    """Forward-backward acoustic substeps (one_step_integration.py:519-556, :638-675, :757-794)."""
+    piadv = pi0 + pip_star
+    c = -Rd / Cv
+    for _ in range(n):
+        pu, pv, pw = pressure_gradient_force(pip_now, theta_rho, grid)
+        un = u_now[NGX:-NGX, NGY:-NGY, NGZ:-NGZ] + (pu + rhs_u) * dt_sound
+        vn = v_now[NGX:-NGX, NGY:-NGY, NGZ:-NGZ] + (pv + rhs_v) * dt_sound
+        wn = w_now[NGX:-NGX, NGY:-NGY, NGZ:-NGZ] + (pw + rhs_w) * dt_sound
+        un = padding3u_array(un)
+        vn = padding3v_array(vn)
+        wn = set_w_bc(padding3_array(wn))
+        div = get_divergence2(un, vn, wn, grid)
+        apt = advection_pi(rho0, pi0, pip_star, un, vn, wn, grid)
+        dpdt = c * (_interior(piadv) * _interior(div)) + apt
+        pn = padding3_array(_interior(pip_now) + 1.1 * dpdt * dt_sound)
+        u_now, v_now, w_now, pip_now = un, vn, wn, pn
+    return u_now, v_now, w_now, pip_now
+
+
+def update_momentum_eqn_euler(u0, v0, w0, pip, theta_rho, adv4u, adv4v, adv4w, fu, fv, b8w, dt, grid):
+    """(one_step_integration.py:876-891)."""
+    pu, pv, pw = pressure_gradient_force(pip, theta_rho, grid)
+    du = adv4u + pu + fv
+    dv = adv4v + pv - fu
+    dw = adv4w + pw + b8w
+    un = padding3u_array(u0[NGX:-NGX, NGY:-NGY, NGZ:-NGZ] + du * dt)
+    vn = padding3v_array(v0[NGX:-NGX, NGY:-NGY, NGZ:-NGZ] + dv * dt)
+    wn = set_w_bc(padding3_array(w0[NGX:-NGX, NGY:-NGY, NGZ:-NGZ] + dw * dt))
+    return un, vn, wn, du, dv, dw
+
+
+def update_scalar_euler(rho0, s0, s, u, v, w, flow_divergence, sfc_flux, heating, weps, dt, grid):
+    """update_theta_euler / update_qv_euler (one_step_integration.py:450-465, :903-916)."""
+    tend = advection_scalar(rho0, s, u, v, w, weps, flow_divergence, sfc_flux, grid) + heating
+    return padding3_array(_interior(s0) + tend * dt), tend
+
+
+# ---------------------------------------------------------------------------------
+# R1 / R2: stages and the WS-RK3 step — one_step_integration.py:16-447,
+# one_sprint_functions.py:10-30, :109-115
+# ---------------------------------------------------------------------------------
+def _n_acoustic(stage, dt_stage, opts):
+    """Acoustic substeps per stage: 1 (stage 0, dt/3), round(dt_stage/(dt/n_sound)) after.
+
+    one_step_integration.py:121-122, :253-254, :383-384 (Python round is half-to-even).
+    """
+    if stage == 0:
+        return 1, dt_stage
+    dts = opts.dt / opts.n_sound
+    return int(round(dt_stage / dts)), dts
+
+
+def rk_sub_step(stage, xi0, xi, base, grid, opts, dt_stage, frozen=None):
+    """One WS-RK3 stage (rk_sub_step0/1/2, one_step_integration.py:16-188, :191-317, :320-447).
+
+    State tuples are (theta, u, v, w, pip, qv, *passive) with optional passive scalars
+    (qc, qr) advected and SGS-diffused like qv.  `frozen` carries the stage-0 SGS
+    tendencies (one_step_integration.py:57-76, :245) into stages 1 and 2.
+    Returns (xi_next, frozen, info) with info the BiCGSTAB exit code (solver_opt 1).
+    """
+    rho0_theta0, rho0, theta0, pi0, qv0 = base[:5]
+    th0, u0, v0, w0, pip0, qv0_now = xi0[:6]
+    th, u, v, w, pip, qv = xi[:6]
+    pas0 = tuple(xi0[6:])
+    pas = tuple(xi[6:])
+    dt_ = rho0.dtype.type
+    nx, ny, nz = grid.nx, grid.ny, grid.nz
+    zeros2 = np.zeros((nx, ny), dtype=rho0.dtype)
+    if stage == 0:
+        if opts.turb_opt == 1:
+            frozen = smagorinsky(rho0, u, v, w, th, qv, (th, qv) + pas, grid, opts.cs, opts.prandtl)
+        else:
+            frozen = None
+    sgs = frozen
+    buoy, b8w = calculate_buoyancy(theta0, th, qv0, qv)
+    fdiv = get_divergence(rho0, u, v, w, grid)
+    adv4u = advection_u(rho0, u, v, w, WEPS_MOM, fdiv, zeros2, grid)
+    adv4v = advection_v(rho0, u, v, w, WEPS_MOM, fdiv, zeros2, grid)
+    adv4w = advection_w(rho0, u, v, w, WEPS_MOM, fdiv, grid)
+    if opts.cor_opt == 1:
+        fu, fv = calculate_coriolis_force(u, v)
+    else:
+        fu = np.zeros((nx, ny + 1, nz), rho0.dtype)
+        fv = np.zeros((nx + 1, ny, nz), rho0.dtype)
+    info = None
+    if opts.solver_opt == 1:
+        rtt = calculate_rtt(rho0_theta0, th, qv, buoy)
+        pint, info = solve_pres_eqn(pip0, rho0_theta0, rtt, adv4u, adv4v, adv4w, fu, fv, buoy, grid, opts)
+        pip_n = padding3_array(pint)
+        trho = theta_rho_of(th, qv)
+        un, vn, wn, _, _, _ = update_momentum_eqn_euler(u0, v0, w0, pip_n, trho, adv4u, adv4v, adv4w,
+                                                        fu, fv, b8w, dt_stage, grid)
+        if sgs is not None:
+            un = un + padding3u_array(sgs[0] * dt_stage)
+            vn = vn + padding3v_array(sgs[1] * dt_stage)
+            wn = wn + padding3_array(sgs[2] * dt_stage)
+    else:
+        rhs_u = adv4u + fv
+        rhs_v = adv4v - fu
+        rhs_w = adv4w + b8w
+        if sgs is not None:
+            rhs_u = rhs_u + sgs[0]
+            rhs_v = rhs_v + sgs[1]
+            rhs_w = rhs_w + sgs[2]
+        trho = theta_rho_of(th, qv)
+        n, dts = _n_acoustic(stage, dt_stage, opts)
+        un, vn, wn, pip_n = sound_rk_step(u0, v0, w0, pi0, pip0, pip, rhs_u, rhs_v, rhs_w,
+                                          trho, rho0, n, dts, grid)
+    thn, _ = update_scalar_euler(rho0, th0, th, u, v, w, fdiv, zeros2, 0.0, WEPS_THETA, dt_stage, grid)
+    qvn, _ = update_scalar_euler(rho0, qv0_now, qv, u, v, w, fdiv, zeros2, 0.0, WEPS_QV, dt_stage, grid)
+    pasn = tuple(update_scalar_euler(rho0, a0, a, u, v, w, fdiv, zeros2, 0.0, WEPS_QV, dt_stage, grid)[0]
+                 for a0, a in zip(pas0, pas))
+    if sgs is not None:
+        thn = thn + padding3_array(sgs[3] * dt_stage)
+        qvn = qvn + padding3_array(sgs[4] * dt_stage)
+        pasn = tuple(a + padding3_array(t * dt_stage) for a, t in zip(pasn, sgs[5:]))
+    del dt_
+    return (thn, un, vn, wn, pip_n, qvn) + pasn, frozen, info
+
+
+def ws_rk3_big_step(xi0, base, grid, opts):
+    """Wicker-Skamarock RK3: every stage restarts from xi0 (one_sprint_functions.py:10-30)."""
+    xi1, frozen, i0 = rk_sub_step(0, xi0, xi0, base, grid, opts, opts.dt / 3.0)
+    xi2, _, i1 = rk_sub_step(1, xi0, xi1, base, grid, opts, opts.dt / 2.0, frozen)
+    xi3, _, i2 = rk_sub_step(2, xi0, xi2, base, grid, opts, opts.dt, frozen)
+    return xi3, (i0, i1, i2)
+
+
+def ws_rk3_sprint(xi, base, grid, opts, nsteps):
+    """nsteps WS-RK3 steps (one_sprint_functions.py:109-115 with sprint_n = nsteps)."""
+    info = None
+    for _ in range(nsteps):
+        xi, info = ws_rk3_big_step(xi, base, grid, opts)
+    return xi, info
+
+
+# ---------------------------------------------------------------------------------
+# I1: grid, base state and warm-bubble initial conditions — setup_lex.py:9-167
+# ---------------------------------------------------------------------------------
+def rslf(p, t):
+    """Bolton (1980) saturation mixing ratio over liquid (setup_lex.py:159-167)."""
+    esl = 611.2 * np.exp(17.67 * (t - 273.15) / (t - 29.65))
+    esl = np.where(esl > p * 0.5, p * 0.5, esl)
+    return eps * esl / (p - esl)
+
+
+def hydrostatic_base_state(grid, theta0_value, rh_of_z, tol=1e-9, max_iter=200):
+    """FP64 hydrostatic base state iterated to qv0 convergence (setup_lex.py:103-137).
+
+    Returns 1-D padded columns (nz+2) of pi0, rho0_theta0, rho0, theta0, qv0 and the
+    iteration count.  `rh_of_z(z)` gives the base relative humidity at cell centres.
+    """
+    nz, dz = grid.nz, grid.dz
+    z = (np.arange(nz + 2) - 0.5) * dz     # setup_lex.py:14
+    theta0 = np.full(nz + 2, float(theta0_value))
+    rh = rh_of_z(z)
+
+    def integrate(qv0):
+        dpdz = -g / Cp / (theta0 * (1.0 + reps * qv0) / (1.0 + qv0))
+        part = np.cumsum(dpdz[1:-1]) * dz + 1.0
+        p8w = np.concatenate([[1.0], part])
+        pi0 = np.empty(nz + 2)
+        pi0[1:-1] = 0.5 * (p8w[:-1] + p8w[1:])
+        pi0[0] = pi0[1]
+        pi0[-1] = pi0[-2]
+        return pi0
+
+    pi0 = integrate(np.zeros(nz + 2))
+    qv0 = rslf(pi0 ** (Cp / Rd) * p00, theta0 * pi0) * rh
+    prev = qv0 * 1.5
+    it = 0
+    while np.max(np.abs(qv0 - prev)) > tol and it < max_iter:
+        prev = qv0
+        pi0 = integrate(qv0)
+        qv0 = rslf(pi0 ** (Cp / Rd) * p00, theta0 * pi0) * rh
+        it += 1
+    rho0_theta0 = pi0 ** (Cv / Rd) * p00 / Rd
+    rho0 = rho0_theta0 / theta0 / ((1.0 + reps * qv0) / (1.0 + qv0))
+    # setup_lex.py:64 overwrites rho0_theta0 with rho0 * theta0
+    rho0_theta0 = rho0 * theta0
+    return dict(pi0=pi0, rho0_theta0=rho0_theta0, rho0=rho0, theta0=theta0, qv0=qv0, iters=it, z=z)
+
+
+@dataclass
+class BubbleCase:
+    """Warm-bubble initial-condition recipe (setup_lex.py:74-146, with the workload's sizes)."""
+    grid: Grid
+    amplitude: float = 2.0
+    xr: float = 2000.0
+    yr: float = 2000.0
+    zr: float = 1400.0
+    zc: float = 1400.0
+    xc: float | None = None      # default: domain centre
+    yc: float | None = None
+    rh_surface: float = 0.0      # base RH profile: rh_surface -> rh_top at z_rh_top (linear)
+    rh_top: float = 0.0
+    z_rh_top: float = 10000.0
+    rh_bubble: float = 0.0       # extra RH inside the bubble (times the cos^2 shape)
+    theta_noise: tuple = ("none", 0.0, 2025)   # ("uniform"|"normal"|"none", scale, seed)
+    uvw_noise: tuple = (0.0, 2026)             # (sigma, first seed; u, v, w use seed, +1, +2)
+    passive: int = 0                            # number of passive scalars (qc, qr)
+    extra: dict = field(default_factory=dict)
+
+
+def setup_warm_bubble(case: BubbleCase, dtype=np.float64):
+    """Build (phys_state, base_state) as padded arrays (setup_grid_n_ic, setup_lex.py:9-71)."""
+    gr = case.grid
+    nx, ny, nz = gr.nx, gr.ny, gr.nz
+
+    def rh_of_z(z):
+        zz = np.minimum(np.maximum(z, 0.0), case.z_rh_top)
+        return case.rh_surface + (case.rh_top - case.rh_surface) * zz / case.z_rh_top
+
+    bs = hydrostatic_base_state(gr, 300.0, rh_of_z)
+    col = lambda a: np.broadcast_to(a[None, None, :], gr.shape_c).astype(np.float64)  # noqa: E731
+    pi0, rt0, rho0, th0, qv0 = (col(bs[k]) for k in ("pi0", "rho0_theta0", "rho0", "theta0", "qv0"))
+    x = (np.arange(nx + 6) - 2.5) * gr.dx
+    y = (np.arange(ny + 6) - 2.5) * gr.dy
+    z = (np.arange(nz + 2) - 0.5) * gr.dz
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    xc = case.xc if case.xc is not None else 0.5 * nx * gr.dx
+    yc = case.yc if case.yc is not None else 0.5 * ny * gr.dy
+    r = np.sqrt(((X - xc) / case.xr) ** 2 + ((Y - yc) / case.yr) ** 2 + ((Z - case.zc) / case.zr) ** 2)
+    shape = np.where(r > 1.0, 0.0, np.cos(r * np.pi / 2.0) ** 2)
+    theta_p = case.amplitude * shape
+    kind, scale, seed = case.theta_noise
+    if kind != "none" and scale != 0.0:
+        rng = np.random.default_rng(seed)
+        if kind == "uniform":
+            nz_ = rng.uniform(-scale, scale, size=(nx, ny, nz))
+        else:
+            nz_ = rng.normal(0.0, scale, size=(nx, ny, nz))
+        theta_p = theta_p + padding3_array(nz_)
+    theta = th0 + theta_p
+    # qv from the local RH and the estimated total pi (setup_lex.py:140-144)
+    rh = rh_of_z(Z) + case.rh_bubble * shape
+    pi = (rho0 * theta * (1.0 + reps * qv0) / (1.0 + qv0) * Rd / p00) ** (Rd / Cv)
+    qv = rslf(pi ** (Cp / Rd) * p00, theta * pi) * rh
+    u = np.zeros(gr.shape_u)
+    v = np.zeros(gr.shape_v)
+    w = np.zeros(gr.shape_w)
+    sig, s0 = case.uvw_noise
+    if sig != 0.0:
+        ui = np.random.default_rng(s0).normal(0.0, sig, size=(nx, ny, nz))
+        u = padding3u_array(np.concatenate([ui, ui[:1]], 0))        # face nx = face 0
+        vi = np.random.default_rng(s0 + 1).normal(0.0, sig, size=(nx, ny, nz))
+        v = padding3v_array(np.concatenate([vi, vi[:, :1]], 1))
+        wi = np.random.default_rng(s0 + 2).normal(0.0, sig, size=(nx, ny, nz + 1))
+        w = set_w_bc(padding3_array(wi))
+    pip = np.zeros(gr.shape_c)
+    phys = [theta, u, v, w, pip, qv] + [np.zeros(gr.shape_c) for _ in range(case.passive)]
+    base = (rt0, rho0, th0, pi0, qv0, np.full((nx, ny), 300.0))
+    phys = tuple(np.ascontiguousarray(a, dtype=dtype) for a in phys)
+    base = tuple(np.ascontiguousarray(a, dtype=dtype) for a in base)
+    return phys, base, bs
+
+
+def case_config0():
+    """64x64x40, 600/600/300 m, dt 30 s, pseudo-incompressible, Smagorinsky, dry."""
+    gr = Grid(64, 64, 40, 600.0, 600.0, 300.0)
+    case = BubbleCase(gr, amplitude=2.0, theta_noise=("uniform", 0.01, 2025))
+    opts = Options(dt=30.0, solver_opt=1, n_sound=32, turb_opt=1)
+    return case, opts
+
+
+def case_config1():
+    """256x256x128, 100 m, dt 1 s, compressible, n_sound 6, moist, passive qc/qr."""
+    gr = Grid(256, 256, 128, 100.0, 100.0, 100.0)
+    case = BubbleCase(gr, amplitude=2.0, rh_surface=0.8, rh_top=0.2, z_rh_top=10000.0, rh_bubble=0.1,
+                      theta_noise=("normal", 0.05, 2025), uvw_noise=(0.01, 2026), passive=2)
+    opts = Options(dt=1.0, solver_opt=2, n_sound=6, turb_opt=1)
+    return case, opts
+
+
+def case_config2():
+    """128x128x64, 200 m, dt 2 s, compressible, n_sound 10 (config 2 states no dt)."""
+    gr = Grid(128, 128, 64, 200.0, 200.0, 200.0)
+    case = BubbleCase(gr, amplitude=2.0, rh_surface=0.8, rh_top=0.2, rh_bubble=0.1,
+                      theta_noise=("normal", 0.1, 2025))
+    opts = Options(dt=2.0, solver_opt=2, n_sound=10, turb_opt=1)
+    return case, opts
+
+
+def scaled_case(case: BubbleCase, nx, ny, nz):
+    """Same recipe on a smaller grid of the same spacing, bubble kept centred."""
+    gr = Grid(nx, ny, nz, case.grid.dx, case.grid.dy, case.grid.dz)
+    c = BubbleCase(**{**case.__dict__, "grid": gr})
+    return c
+
+
+# ---------------------------------------------------------------------------------
+# X5: block-mean coarsening (initial_comparison/plot_theta_pert_coarsened.m:7-8)
+# ---------------------------------------------------------------------------------
+def coarsen_block_mean(a, fx, fy, fz):
+    """Mean over non-overlapping fx*fy*fz blocks of an interior (nx, ny, nz) field."""
+    nx, ny, nz = a.shape
+    return a.reshape(nx // fx, fx, ny // fy, fy, nz // fz, fz).mean(axis=(1, 3, 5))
+
+
+def n_acoustic_total(opts):
+    """Acoustic substeps per full step: 1 + round(n/2) + n (solver_opt 2)."""
+    return 1 + _n_acoustic(1, opts.dt / 2.0, opts)[0] + _n_acoustic(2, opts.dt, opts)[0]
+
+
+def python_round(x):
+    """Python's round (half-to-even), kept explicit for the acoustic counts."""
+    return int(round(x))
+
+
+__all__ = [n for n in dir() if not n.startswith("_")] + ["_n_acoustic", "_interior"]
+_ = math
