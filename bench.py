@@ -39,6 +39,11 @@ BYTES_PER_CELL = {
     "k_smag_uvw": 40,
     "k_smag_scalar": 12,
     "k_copy": 8,
+    # fused stage kernels (flow divergence computed inline, never stored)
+    "k_stage_scalars": 80,   # per scalar s, s0, sgs in + s_next out (x4: theta, qv, qc, qr); u, v, w in; theta_rho out
+    "k_mom_u": 20,           # u, v, w, sgs_u in; rhs_u out
+    "k_mom_v": 20,           # u, v, w, sgs_v in; rhs_v out
+    "k_mom_w": 28,           # u, v, w, theta, qv, sgs_w in; rhs_w out
 }
 
 

@@ -153,6 +153,7 @@ typedef struct lex_step_opts {
   float cs;          /* Smagorinsky constant (turbulence.py:433) */
   float prandtl;     /* turbulence.py:432 */
   int n_passive;     /* passive scalars advected and diffused like qv (qc, qr) */
+  int impl;          /* 0 = fused stage kernels (default), 1 = one kernel per reference routine */
 } lex_step_opts_t;
 
 typedef struct lex_base_cols {

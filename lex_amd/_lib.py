@@ -29,7 +29,7 @@ class Grid(C.Structure):
 
 class StepOpts(C.Structure):
     _fields_ = [("dt", C.c_float), ("n_sound", C.c_int), ("turb", C.c_int), ("cs", C.c_float),
-                ("prandtl", C.c_float), ("n_passive", C.c_int)]
+                ("prandtl", C.c_float), ("n_passive", C.c_int), ("impl", C.c_int)]
 
 
 class BaseCols(C.Structure):

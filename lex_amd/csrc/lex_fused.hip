@@ -1,0 +1,449 @@
+// Fused stage kernels for the device WS-RK3 driver.
+//
+// k_stage_scalars: every scalar's WENO advection + Euler update + frozen-SGS add-in of
+// one stage (update_theta_euler / update_qv_euler, one_step_integration.py:450-465,
+// :158-162), plus the flow divergence (advection.py:200-226) and theta_rho
+// (one_step_integration.py:135) inline, in one launch.
+//
+// Work decomposition (each face flux computed once):
+//   block  = full z column (64*ceil(nz/64) lanes) x TY rows of y; marches TX cells in x
+//   x-face : from a 6-value register window along x; the lower face is the previous
+//            step's upper face
+//   y-face : each row computes its lower face from an LDS plane (rows j0-3..j0+TY+2);
+//            the last row also computes the tile's upper face
+//   z-face : each lane its lower face from the LDS column; the lid face is zero
+#pragma once
+#include "lex_common.hpp"
+
+namespace lex {
+
+constexpr int kFTY = 4;    // rows of y per block
+constexpr int kFTX = 16;   // cells marched in x per block
+constexpr int kMaxScalars = 8;
+
+struct ScalarSet {
+  const float* s0[kMaxScalars];   // xi0 fields (padded)
+  const float* s[kMaxScalars];    // stage fields (padded)
+  const float* sgs[kMaxScalars];  // frozen SGS tendencies (interior) or null
+  float* out[kMaxScalars];        // next-stage fields (padded, halos filled)
+  float weps[kMaxScalars];
+  int n;
+};
+
+template <int NS>
+__global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __restrict__ rc,
+                                                         const float* __restrict__ u, const float* __restrict__ v,
+                                                         const float* __restrict__ w, ScalarSet S,
+                                                         float* __restrict__ trho, float dt) {
+  extern __shared__ float lds[];
+  const int nzl = blockDim.x;              // z lanes (multiple of 64, >= nz)
+  const int tz = threadIdx.x, ty = threadIdx.y;
+  const int nt = nzl * kFTY;
+  const int tid = ty * nzl + tz;
+  const int Lz = d.nz + 2;                 // padded scalar column
+  float* P = lds;                          // (kFTY+6) x Lz plane of the current scalar
+  float* FY = P + (kFTY + 6) * Lz;         // (kFTY+1) x nzl lower y-faces (+ tile top)
+  float* FZ = FY + (kFTY + 1) * nzl;       // kFTY x nzl lower z-faces
+
+  const int j0 = blockIdx.y * kFTY;
+  const int i0 = blockIdx.x * kFTX;
+  const int k = tz, j = j0 + ty;
+  const bool act = (k < d.nz) && (j < d.ny);
+  const int K = k + 1, J = j + 3;
+  const int Kc = act ? K : 1, Jc = (j < d.ny) ? J : 3;   // clamped indices for idle lanes
+  const bool last_row = (ty == kFTY - 1) || (j == d.ny - 1);
+  const long sx = d.sxc(), sy = d.syc();
+  const float r = rc[Kc];
+  const float r8lo = (Kc <= 0 || Kc >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc - 1] + rc[Kc]);
+  const float r8hi = (Kc + 1 >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc] + rc[Kc + 1]);
+
+  // x windows q[I-2..I+3] per scalar and the lower x-face flux
+  float win[NS][6];
+  float fxlo[NS];
+  const int iend = min(i0 + kFTX, d.nx);
+  {
+    const int I = i0 + 3;
+    const float mlo = r * u[iu(d, I, Jc, Kc)];
+    const bool pos = upwind_pos(mlo);
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      const float* s = S.s[q];
+      long c = ic(d, I, Jc, Kc);
+      float m3 = s[c - 3 * sx];
+#pragma unroll
+      for (int t = 0; t < 6; ++t) win[q][t] = s[c + (t - 2) * sx];
+      fxlo[q] = mlo * hrecon(m3, win[q][0], win[q][1], win[q][2], win[q][3], win[q][4], pos, S.weps[q]);
+    }
+  }
+  float ulo = u[iu(d, i0 + 3, Jc, Kc)];
+
+  for (int i = i0; i < iend; ++i) {
+    const int I = i + 3;
+    const long c = ic(d, I, Jc, Kc);
+    const float uhi = u[iu(d, I + 1, Jc, Kc)];
+    const float vlo = v[iv(d, I, Jc, Kc)];
+    const float vhi = v[iv(d, I, Jc + 1, Kc)];
+    const float wlo = w[iw(d, I, Jc, Kc)];
+    const float whi = w[iw(d, I, Jc, Kc + 1)];
+    // flow divergence at the cell (get_divergence with uniform rho columns)
+    const float fd = (r * uhi - r * ulo) * d.idx + (r * vhi - r * vlo) * d.idy + (r8hi * whi - r8lo * wlo) * d.idz;
+    const float mx = r * uhi;                      // x-face I+1
+    const float my_lo = r * vlo;                   // y-face J
+    const float my_hi = r * vhi;                   // y-face J+1 (last row only)
+    const float mz_lo = 0.5f * (rc[Kc - 1] + rc[Kc]) * wlo;   // z-face between K-1 and K
+    const bool px = upwind_pos(mx), py0 = upwind_pos(my_lo), py1 = upwind_pos(my_hi), pz = upwind_pos(mz_lo);
+    float thc = 0.f, qvc = 0.f;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      const float* s = S.s[q];
+      const float weps = S.weps[q];
+      // plane of this scalar at x = I: rows J = j0 .. j0+kFTY+5 (padded), all levels
+      for (int e = tid; e < (kFTY + 6) * Lz; e += nt) {
+        int row = e / Lz, col = e - row * Lz;
+        int Jr = min(j0 + row, d.ny + 5);
+        P[e] = s[ic(d, I, Jr, col)];
+      }
+      __syncthreads();
+      const float* Pc = P + (ty + 3) * Lz;   // this row's column
+      const float qc = Pc[Kc];
+      if (q == 0) thc = qc;
+      if (q == 1) qvc = qc;
+      if (act) {
+        float fz = (k == 0) ? 0.0f : mz_lo * vrecon(Pc, Lz, K - 1, pz, weps);
+        FZ[ty * nzl + k] = fz;
+        float fy = my_lo * hrecon(Pc[Kc - 3 * Lz], Pc[Kc - 2 * Lz], Pc[Kc - Lz], Pc[Kc], Pc[Kc + Lz], Pc[Kc + 2 * Lz],
+                                  py0, weps);
+        FY[ty * nzl + k] = fy;
+        if (last_row) {
+          const float* Pn = Pc + Lz;
+          FY[(ty + 1) * nzl + k] =
+              my_hi * hrecon(Pn[Kc - 3 * Lz], Pn[Kc - 2 * Lz], Pn[Kc - Lz], Pn[Kc], Pn[Kc + Lz], Pn[Kc + 2 * Lz], py1, weps);
+        }
+      }
+      // x-face I+1 from the window q[I-2..I+3]
+      float fxhi = mx * hrecon(win[q][0], win[q][1], win[q][2], win[q][3], win[q][4], win[q][5], px, weps);
+      __syncthreads();
+      if (act) {
+        float fzhi = (k == d.nz - 1) ? 0.0f : FZ[ty * nzl + k + 1];
+        float conv = -((fxhi - fxlo[q]) * d.idx + (FY[(ty + 1) * nzl + k] - FY[ty * nzl + k]) * d.idy +
+                       (fzhi - FZ[ty * nzl + k]) * d.idz);
+        float tend = (conv + qc * fd) / r;
+        float nxt = S.s0[q][c] + tend * dt;
+        if (S.sgs[q]) nxt = nxt + S.sgs[q][((long)i * d.ny + j) * d.nz + k] * dt;
+        store_c(d, S.out[q], i, j, k, nxt);
+      }
+      fxlo[q] = fxhi;
+      // advance the window: q[I-1..I+4]
+#pragma unroll
+      for (int t = 0; t < 5; ++t) win[q][t] = win[q][t + 1];
+      win[q][5] = (i + 1 < iend) ? s[c + 4 * sx] : 0.0f;
+    }
+    if (act && trho) store_c(d, trho, i, j, k, theta_rho(thc, qvc));
+    ulo = uhi;
+  }
+}
+
+
+// Flow divergence div(rho0 u) at padded scalar cell (I,J,K) (get_divergence,
+// advection.py:200-226) for a horizontally uniform rho column.
+__device__ __forceinline__ float fd_cell(const Dims& d, const float* __restrict__ rc, const float* __restrict__ u,
+                                         const float* __restrict__ v, const float* __restrict__ w, int I, int J, int K) {
+  float r = rc[K];
+  float r8lo = (K <= 0 || K >= d.nz + 2) ? 0.0f : 0.5f * (rc[K - 1] + rc[K]);
+  float r8hi = (K + 1 >= d.nz + 2) ? 0.0f : 0.5f * (rc[K] + rc[K + 1]);
+  return (r * u[iu(d, I + 1, J, K)] - r * u[iu(d, I, J, K)]) * d.idx +
+         (r * v[iv(d, I, J + 1, K)] - r * v[iv(d, I, J, K)]) * d.idy +
+         (r8hi * w[iw(d, I, J, K + 1)] - r8lo * w[iw(d, I, J, K)]) * d.idz;
+}
+
+// ---------------------------------------------------------------------------------
+// k_mom_u: rhs_u = advection_u + sgs_u at u faces 0..nx-1 (face nx is the periodic
+// duplicate of face 0 and is written from it).  advection.py:37-74, :561-771.
+// ---------------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
+                                                 const float* __restrict__ v, const float* __restrict__ w,
+                                                 const float* __restrict__ sgs, float* __restrict__ ru) {
+  extern __shared__ float lds[];
+  const float weps = 1.0e-16f;
+  const int nzl = blockDim.x, tz = threadIdx.x, ty = threadIdx.y, nt = nzl * kFTY, tid = ty * nzl + tz;
+  const int Lz = d.nz + 2;
+  float* P = lds;
+  float* FY = P + (kFTY + 6) * Lz;
+  float* FZ = FY + (kFTY + 1) * nzl;
+  const int j0 = blockIdx.y * kFTY, i0 = blockIdx.x * kFTX;
+  const int k = tz, j = j0 + ty;
+  const bool act = (k < d.nz) && (j < d.ny);
+  const int K = k + 1;
+  const int Kc = act ? K : 1, Jc = (j < d.ny) ? j + 3 : 3;
+  const bool last_row = (ty == kFTY - 1) || (j == d.ny - 1);
+  const long sx = d.sxu();
+  const float r = rc[Kc];
+  const int iend = min(i0 + kFTX, d.nx);
+  float win[6];
+  float fxlo, fdlo;
+  {
+    const int I = i0 + 3;
+    long b = iu(d, I, Jc, Kc);
+    float m3 = u[b - 3 * sx];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) win[t] = u[b + (t - 2) * sx];
+    float mass = 0.5f * (win[1] + win[2]) * r;   // cell I-1
+    fxlo = mass * hrecon(m3, win[0], win[1], win[2], win[3], win[4], upwind_pos(mass), weps);
+    fdlo = fd_cell(d, rc, u, v, w, I - 1, Jc, Kc);
+  }
+  for (int i = i0; i < iend; ++i) {
+    const int I = i + 3;
+    const float uc = win[2];
+    const float mx = 0.5f * (win[2] + win[3]) * r;   // x-flux at cell I
+    const float fxhi = mx * hrecon(win[0], win[1], win[2], win[3], win[4], win[5], upwind_pos(mx), weps);
+    const float fdhi = fd_cell(d, rc, u, v, w, I, Jc, Kc);
+    for (int e = tid; e < (kFTY + 6) * Lz; e += nt) {
+      int row = e / Lz, col = e - row * Lz;
+      P[e] = u[iu(d, I, min(j0 + row, d.ny + 5), col)];
+    }
+    __syncthreads();
+    const float* Pc = P + (ty + 3) * Lz;
+    if (act) {
+      // z flux at w level K (face K-1): mass averaged over the two columns of the u point
+      float fz = 0.0f;
+      if (k > 0) {
+        float m = 0.5f * (0.5f * (rc[K - 1] + rc[K]) * w[iw(d, I - 1, Jc, K)] + 0.5f * (rc[K - 1] + rc[K]) * w[iw(d, I, Jc, K)]);
+        fz = m * vrecon(Pc, Lz, K - 1, upwind_pos(m), weps);
+      }
+      FZ[ty * nzl + k] = fz;
+      float my = 0.5f * (r * v[iv(d, I - 1, Jc, K)] + r * v[iv(d, I, Jc, K)]);
+      FY[ty * nzl + k] =
+          my * hrecon(Pc[K - 3 * Lz], Pc[K - 2 * Lz], Pc[K - Lz], Pc[K], Pc[K + Lz], Pc[K + 2 * Lz], upwind_pos(my), weps);
+      if (last_row) {
+        const float* Pn = Pc + Lz;
+        float m2 = 0.5f * (r * v[iv(d, I - 1, Jc + 1, K)] + r * v[iv(d, I, Jc + 1, K)]);
+        FY[(ty + 1) * nzl + k] =
+            m2 * hrecon(Pn[K - 3 * Lz], Pn[K - 2 * Lz], Pn[K - Lz], Pn[K], Pn[K + Lz], Pn[K + 2 * Lz], upwind_pos(m2), weps);
+      }
+    }
+    __syncthreads();
+    if (act) {
+      float fzhi = (k == d.nz - 1) ? 0.0f : FZ[ty * nzl + k + 1];
+      float conv = -((fxhi - fxlo) * d.idx + (FY[(ty + 1) * nzl + k] - FY[ty * nzl + k]) * d.idy +
+                     (fzhi - FZ[ty * nzl + k]) * d.idz);
+      float a = (conv + uc * (0.5f * (fdlo + fdhi))) / (0.5f * (r + r));
+      if (sgs) a = a + sgs[((long)i * d.ny + j) * d.nz + k];
+      ru[((long)i * d.ny + j) * d.nz + k] = a;
+      if (i == 0) ru[((long)d.nx * d.ny + j) * d.nz + k] = a;
+    }
+    fxlo = fxhi;
+    fdlo = fdhi;
+#pragma unroll
+    for (int t = 0; t < 5; ++t) win[t] = win[t + 1];
+    win[5] = (i + 1 < iend) ? u[iu(d, I + 4, Jc, Kc)] : 0.0f;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// k_mom_v: rhs_v = advection_v + sgs_v at v faces 0..ny-1 (face ny written from 0).
+// advection.py:77-114, :816-1025.
+// ---------------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
+                                                 const float* __restrict__ v, const float* __restrict__ w,
+                                                 const float* __restrict__ sgs, float* __restrict__ rv) {
+  extern __shared__ float lds[];
+  const float weps = 1.0e-16f;
+  const int nzl = blockDim.x, tz = threadIdx.x, ty = threadIdx.y, nt = nzl * kFTY, tid = ty * nzl + tz;
+  const int Lz = d.nz + 2;
+  float* P = lds;
+  float* FY = P + (kFTY + 6) * Lz;
+  float* FZ = FY + (kFTY + 1) * nzl;
+  const int j0 = blockIdx.y * kFTY, i0 = blockIdx.x * kFTX;
+  const int k = tz, j = j0 + ty;
+  const bool act = (k < d.nz) && (j < d.ny);
+  const int K = k + 1;
+  const int Kc = act ? K : 1, Jc = (j < d.ny) ? j + 3 : 3;
+  const bool last_row = (ty == kFTY - 1) || (j == d.ny - 1);
+  const long sx = d.sxv();
+  const float r = rc[Kc];
+  const int iend = min(i0 + kFTX, d.nx);
+  float win[6];
+  float fxlo;
+  {
+    const int I = i0 + 3;
+    long b = iv(d, I, Jc, Kc);
+    float m3 = v[b - 3 * sx];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) win[t] = v[b + (t - 2) * sx];
+    float mass = 0.5f * (r * u[iu(d, I, Jc - 1, Kc)] + r * u[iu(d, I, Jc, Kc)]);
+    fxlo = mass * hrecon(m3, win[0], win[1], win[2], win[3], win[4], upwind_pos(mass), weps);
+  }
+  for (int i = i0; i < iend; ++i) {
+    const int I = i + 3;
+    const float vc = win[2];
+    const float mx = 0.5f * (r * u[iu(d, I + 1, Jc - 1, Kc)] + r * u[iu(d, I + 1, Jc, Kc)]);
+    const float fxhi = mx * hrecon(win[0], win[1], win[2], win[3], win[4], win[5], upwind_pos(mx), weps);
+    const float fd0 = fd_cell(d, rc, u, v, w, I, Jc - 1, Kc);
+    const float fd1 = fd_cell(d, rc, u, v, w, I, Jc, Kc);
+    for (int e = tid; e < (kFTY + 6) * Lz; e += nt) {
+      int row = e / Lz, col = e - row * Lz;
+      P[e] = v[iv(d, I, min(j0 + row, d.ny + 6), col)];
+    }
+    __syncthreads();
+    const float* Pc = P + (ty + 3) * Lz;
+    if (act) {
+      float fz = 0.0f;
+      if (k > 0) {
+        float m = 0.5f * (0.5f * (rc[K - 1] + rc[K]) * w[iw(d, I, Jc - 1, K)] + 0.5f * (rc[K - 1] + rc[K]) * w[iw(d, I, Jc, K)]);
+        fz = m * vrecon(Pc, Lz, K - 1, upwind_pos(m), weps);
+      }
+      FZ[ty * nzl + k] = fz;
+      float my = 0.5f * (Pc[K - Lz] + Pc[K]) * r;   // y-flux at cell J-1
+      FY[ty * nzl + k] =
+          my * hrecon(Pc[K - 3 * Lz], Pc[K - 2 * Lz], Pc[K - Lz], Pc[K], Pc[K + Lz], Pc[K + 2 * Lz], upwind_pos(my), weps);
+      if (last_row) {
+        const float* Pn = Pc + Lz;
+        float m2 = 0.5f * (Pn[K - Lz] + Pn[K]) * r;
+        FY[(ty + 1) * nzl + k] =
+            m2 * hrecon(Pn[K - 3 * Lz], Pn[K - 2 * Lz], Pn[K - Lz], Pn[K], Pn[K + Lz], Pn[K + 2 * Lz], upwind_pos(m2), weps);
+      }
+    }
+    __syncthreads();
+    if (act) {
+      float fzhi = (k == d.nz - 1) ? 0.0f : FZ[ty * nzl + k + 1];
+      float conv = -((fxhi - fxlo) * d.idx + (FY[(ty + 1) * nzl + k] - FY[ty * nzl + k]) * d.idy +
+                     (fzhi - FZ[ty * nzl + k]) * d.idz);
+      float a = (conv + vc * (0.5f * (fd0 + fd1))) / (0.5f * (r + r));
+      if (sgs) a = a + sgs[((long)i * (d.ny + 1) + j) * d.nz + k];
+      rv[((long)i * (d.ny + 1) + j) * d.nz + k] = a;
+      if (j == 0) rv[((long)i * (d.ny + 1) + d.ny) * d.nz + k] = a;
+    }
+    fxlo = fxhi;
+#pragma unroll
+    for (int t = 0; t < 5; ++t) win[t] = win[t + 1];
+    win[5] = (i + 1 < iend) ? v[iv(d, I + 4, Jc, Kc)] : 0.0f;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// k_mom_w: rhs_w = advection_w + b8w + sgs_w at w levels 0..nz (advection.py:117-149,
+// :1070-1281; pressure_equations.py:162-172).  Lane l owns level l and computes the
+// vertical flux at scalar level l+1 (its upper face); the lid level is written by the
+// top lane.
+// ---------------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_mom_w(Dims d, const float* __restrict__ rc, const float* __restrict__ th0c,
+                                                 const float* __restrict__ qv0c, const float* __restrict__ u,
+                                                 const float* __restrict__ v, const float* __restrict__ w,
+                                                 const float* __restrict__ th, const float* __restrict__ qv,
+                                                 const float* __restrict__ sgs, float* __restrict__ rw) {
+  extern __shared__ float lds[];
+  const float weps = 1.0e-16f;
+  const int nzl = blockDim.x, tz = threadIdx.x, ty = threadIdx.y, nt = nzl * kFTY, tid = ty * nzl + tz;
+  const int Lw = d.nz + 3;
+  float* P = lds;                                   // (kFTY+6) x Lw plane of w
+  float* FY = P + (kFTY + 6) * Lw;                  // (kFTY+1) x nzl
+  float* FZ = FY + (kFTY + 1) * nzl;                // kFTY x nzl: flux at scalar level l+1
+  float* FD = FZ + kFTY * nzl;                      // kFTY x nzl: flow divergence at cell l
+  float* BB = FD + kFTY * nzl;                      // kFTY x nzl: buoyancy at cell l
+  const int j0 = blockIdx.y * kFTY, i0 = blockIdx.x * kFTX;
+  const int l = tz, j = j0 + ty;
+  const bool row_ok = j < d.ny;
+  const bool lane_ok = l < d.nz;                   // owns level l and cell l
+  const bool act = row_ok && lane_ok;
+  const int K = l + 1;                              // w padded level (cell l is scalar K = l+1)
+  const int Kc = act ? K : 1, Jc = row_ok ? j + 3 : 3;
+  const bool last_row = (ty == kFTY - 1) || (j == d.ny - 1);
+  const bool interior = act && l >= 1;              // levels 1..nz-1 carry advection
+  const long sx = d.sxw();
+  const int iend = min(i0 + kFTX, d.nx);
+  float win[6];
+  float fxlo;
+  {
+    const int I = i0 + 3;
+    long b = iw(d, I, Jc, Kc);
+    float m3 = w[b - 3 * sx];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) win[t] = w[b + (t - 2) * sx];
+    float mass = 0.5f * (rc[Kc - 1] * u[iu(d, I, Jc, Kc - 1)] + rc[Kc] * u[iu(d, I, Jc, Kc)]);
+    fxlo = mass * hrecon(m3, win[0], win[1], win[2], win[3], win[4], upwind_pos(mass), weps);
+  }
+  for (int i = i0; i < iend; ++i) {
+    const int I = i + 3;
+    const float wc = win[2];
+    const float mx = 0.5f * (rc[Kc - 1] * u[iu(d, I + 1, Jc, Kc - 1)] + rc[Kc] * u[iu(d, I + 1, Jc, Kc)]);
+    const float fxhi = mx * hrecon(win[0], win[1], win[2], win[3], win[4], win[5], upwind_pos(mx), weps);
+    for (int e = tid; e < (kFTY + 6) * Lw; e += nt) {
+      int row = e / Lw, col = e - row * Lw;
+      P[e] = w[iw(d, I, min(j0 + row, d.ny + 5), col)];
+    }
+    if (act) {
+      FD[ty * nzl + l] = fd_cell(d, rc, u, v, w, I, Jc, K);
+      long c = ic(d, I, Jc, K);
+      BB[ty * nzl + l] = buoy_at(th0c, qv0c, th[c], qv[c], K);
+    }
+    __syncthreads();
+    const float* Pc = P + (ty + 3) * Lw;
+    if (act) {
+      // vertical flux at scalar level c = l+1 (face between w[c] and w[c+1])
+      const int cz = l + 1;
+      float mz = 0.5f * (Pc[cz] + Pc[cz + 1]) * rc[cz];
+      FZ[ty * nzl + l] = mz * vrecon(Pc, Lw, cz, upwind_pos(mz), weps);
+      if (interior) {
+        float my = 0.5f * (rc[K - 1] * v[iv(d, I, Jc, K - 1)] + rc[K] * v[iv(d, I, Jc, K)]);
+        FY[ty * nzl + l] = my * hrecon(Pc[K - 3 * Lw], Pc[K - 2 * Lw], Pc[K - Lw], Pc[K], Pc[K + Lw], Pc[K + 2 * Lw],
+                                       upwind_pos(my), weps);
+        if (last_row) {
+          const float* Pn = Pc + Lw;
+          float m2 = 0.5f * (rc[K - 1] * v[iv(d, I, Jc + 1, K - 1)] + rc[K] * v[iv(d, I, Jc + 1, K)]);
+          FY[(ty + 1) * nzl + l] = m2 * hrecon(Pn[K - 3 * Lw], Pn[K - 2 * Lw], Pn[K - Lw], Pn[K], Pn[K + Lw],
+                                               Pn[K + 2 * Lw], upwind_pos(m2), weps);
+        }
+      }
+    }
+    __syncthreads();
+    if (act) {
+      const float* Bb = BB + ty * nzl;
+      const int nz = d.nz;
+      long o = ((long)i * d.ny + j) * (nz + 1);
+      float b8;
+      float a = 0.0f;
+      if (l == 0) {
+        b8 = 1.875f * Bb[0] + (-1.25f) * Bb[1] + 0.375f * Bb[2];
+      } else {
+        b8 = 0.5f * (Bb[l - 1] + Bb[l]);
+        float conv = -((fxhi - fxlo) * d.idx + (FY[(ty + 1) * nzl + l] - FY[ty * nzl + l]) * d.idy +
+                       (FZ[ty * nzl + l] - FZ[ty * nzl + l - 1]) * d.idz);
+        float div8w = 0.5f * (FD[ty * nzl + l - 1] + FD[ty * nzl + l]);
+        float rho8w = 0.5f * (rc[K - 1] + rc[K]);
+        a = (conv + wc * div8w) / rho8w;
+      }
+      a = a + b8;
+      if (sgs) a = a + sgs[o + l];
+      rw[o + l] = a;
+      if (l == nz - 1) {   // lid level: zero advection, extrapolated buoyancy
+        float bt = 1.875f * Bb[nz - 1] + (-1.25f) * Bb[nz - 2] + 0.375f * Bb[nz - 3];
+        float at = 0.0f + bt;
+        if (sgs) at = at + sgs[o + nz];
+        rw[o + nz] = at;
+      }
+    }
+    fxlo = fxhi;
+#pragma unroll
+    for (int t = 0; t < 5; ++t) win[t] = win[t + 1];
+    win[5] = (i + 1 < iend) ? w[iw(d, I + 4, Jc, Kc)] : 0.0f;
+    __syncthreads();
+  }
+}
+
+inline size_t mom_uv_lds(const Dims& d) {
+  int nzl = 64 * ((d.nz + 63) / 64);
+  return sizeof(float) * ((size_t)(kFTY + 6) * (d.nz + 2) + (size_t)(2 * kFTY + 1) * nzl);
+}
+inline size_t mom_w_lds(const Dims& d) {
+  int nzl = 64 * ((d.nz + 63) / 64);
+  return sizeof(float) * ((size_t)(kFTY + 6) * (d.nz + 3) + (size_t)(4 * kFTY + 1) * nzl);
+}
+
+inline size_t stage_scalars_lds(const Dims& d) {
+  int nzl = 64 * ((d.nz + 63) / 64);
+  return sizeof(float) * ((size_t)(kFTY + 6) * (d.nz + 2) + (size_t)(kFTY + 1) * nzl + (size_t)kFTY * nzl);
+}
+
+}  // namespace lex

@@ -8,6 +8,7 @@
 #pragma once
 #include "lex_common.hpp"
 #include "lex_prof.hpp"
+#include "lex_fused.hip"
 
 namespace lex {
 
@@ -195,12 +196,50 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
       for (int q = 0; q < o.n_passive; ++q)
         LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, sm[9], src[6 + q], ss[2 + q]);
     }
-    LEX_LAUNCH(KID_DIV_RHO, k_div_rho, G(d.nz + 2, d.ny + 6, d.nx + 6), bk, 0, st, d, rc, src[1], src[2], src[3], fd);
-    LEX_LAUNCH(KID_RHS_UV, k_rhs_uv, G(d.nz, d.ny + 1, d.nx + 1), bk, 0, st, d, rc, src[1], src[2], src[3], fd,
-                       turb ? su : nullptr, turb ? sv : nullptr, ru, rv);
-    LEX_LAUNCH(KID_RHS_W, k_rhs_w, G(d.nz + 1, d.ny, d.nx), bk, 0, st, d, rc, b.theta0, b.qv0, src[1], src[2], src[3],
-                       src[0], src[5], fd, turb ? sw : nullptr, rw);
-    {
+    const int nsc = 2 + o.n_passive;
+    const bool fused_ok = (64 * ((d.nz + 63) / 64)) * kFTY <= 1024 && nsc >= 2 && nsc <= 6;
+    if (o.impl == 0 && !fused_ok) return -4;  // the caller falls back to impl 1
+    if (o.impl == 0) {
+      const int nzl = 64 * ((d.nz + 63) / 64);
+      dim3 fb(nzl, kFTY, 1), fg((d.nx + kFTX - 1) / kFTX, (d.ny + kFTY - 1) / kFTY, 1);
+      LEX_LAUNCH(KID_STAGE_MOMENTUM, k_mom_u, fg, fb, mom_uv_lds(d), st, d, rc, src[1], src[2], src[3],
+                 turb ? (const float*)su : nullptr, ru);
+      LEX_LAUNCH(KID_MOM_V, k_mom_v, fg, fb, mom_uv_lds(d), st, d, rc, src[1], src[2], src[3],
+                 turb ? (const float*)sv : nullptr, rv);
+      LEX_LAUNCH(KID_MOM_W, k_mom_w, fg, fb, mom_w_lds(d), st, d, rc, b.theta0, b.qv0, src[1], src[2], src[3],
+                 src[0], src[5], turb ? (const float*)sw : nullptr, rw);
+    } else {
+      LEX_LAUNCH(KID_DIV_RHO, k_div_rho, G(d.nz + 2, d.ny + 6, d.nx + 6), bk, 0, st, d, rc, src[1], src[2], src[3], fd);
+      LEX_LAUNCH(KID_RHS_UV, k_rhs_uv, G(d.nz, d.ny + 1, d.nx + 1), bk, 0, st, d, rc, src[1], src[2], src[3], fd,
+                 turb ? su : nullptr, turb ? sv : nullptr, ru, rv);
+      LEX_LAUNCH(KID_RHS_W, k_rhs_w, G(d.nz + 1, d.ny, d.nx), bk, 0, st, d, rc, b.theta0, b.qv0, src[1], src[2], src[3],
+                 src[0], src[5], fd, turb ? sw : nullptr, rw);
+    }
+    if (o.impl == 0) {
+      // all scalars of the stage + theta_rho in one launch (k_stage_scalars)
+      ScalarSet S;
+      S.n = nsc;
+      for (int q = 0; q < nsc; ++q) {
+        int fi = (q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2));
+        S.s0[q] = X0[fi];
+        S.s[q] = src[fi];
+        S.sgs[q] = turb ? ss[q] : nullptr;
+        S.out[q] = dst[fi];
+        S.weps[q] = (q == 0) ? 1.0e-17f : 1.0e-18f;
+      }
+      const int nzl = 64 * ((d.nz + 63) / 64);
+      dim3 fb(nzl, kFTY, 1), fg((d.nx + kFTX - 1) / kFTX, (d.ny + kFTY - 1) / kFTY, 1);
+      size_t lds = stage_scalars_lds(d);
+      float dts_f = (float)dt_s;
+      switch (nsc) {
+        case 2: LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars<2>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
+        case 3: LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars<3>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
+        case 4: LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars<4>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
+        case 5: LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars<5>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
+        case 6: LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars<6>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
+        default: return -1;
+      }
+    } else {
       long n = (long)L.nc;
       LEX_LAUNCH(KID_THETA_RHO, k_theta_rho, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, src[0], src[5], tr);
     }
@@ -236,12 +275,14 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
       for (int c = 0; c < 4; ++c) in[c] = out[c];
     }
     // scalars: theta (weps 1e-17), qv and passive (1e-18) (one_step_integration.py:905, :913)
-    for (int q = 0; q < 2 + o.n_passive; ++q) {
-      int fi = (q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2));
-      float weps = (q == 0) ? 1.0e-17f : 1.0e-18f;
-      LEX_LAUNCH(KID_SCALAR_UPDATE, k_scalar_update, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, X0[fi], src[fi], src[1], src[2],
-                         src[3], fd, (const float*)nullptr, (const float*)nullptr, turb ? ss[q] : nullptr, dst[fi],
-                         (float*)nullptr, (float)dt_s, weps);
+    if (o.impl != 0) {
+      for (int q = 0; q < nsc; ++q) {
+        int fi = (q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2));
+        float weps = (q == 0) ? 1.0e-17f : 1.0e-18f;
+        LEX_LAUNCH(KID_SCALAR_UPDATE, k_scalar_update, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, X0[fi], src[fi], src[1],
+                   src[2], src[3], fd, (const float*)nullptr, (const float*)nullptr, turb ? ss[q] : nullptr, dst[fi],
+                   (float*)nullptr, (float)dt_s, weps);
+      }
     }
   }
   hipError_t e = hipGetLastError();

@@ -29,8 +29,11 @@ def _workspace(g, n_passive, device):
     return ws
 
 
-def fused_step(phys_state, base_state, grid, model_opt, dt=None, n_sound=None, cs=None, prandtl=None):
-    """Advance a solver_opt-2 state by one WS-RK3 step in place on the device."""
+def fused_step(phys_state, base_state, grid, model_opt, dt=None, n_sound=None, cs=None, prandtl=None, impl=0):
+    """Advance a solver_opt-2 state by one WS-RK3 step in place on the device.
+
+    impl 0 runs the fused stage kernels, impl 1 one kernel per reference routine.
+    """
     solver_opt, cor_opt, turb_opt = one._check_opts(model_opt)
     if solver_opt != 2 or cor_opt != 0:
         raise NotImplementedError("the fused step covers solver_opt 2 without Coriolis")
@@ -38,7 +41,7 @@ def fused_step(phys_state, base_state, grid, model_opt, dt=None, n_sound=None, c
     n_passive = len(phys_state) - 6
     opts = _lib.StepOpts(float(nl.dt if dt is None else dt), int(nl.n_sound if n_sound is None else n_sound),
                          int(turb_opt), float(nl.cs if cs is None else cs),
-                         float(nl.prandtl if prandtl is None else prandtl), n_passive)
+                         float(nl.prandtl if prandtl is None else prandtl), n_passive, int(impl))
     cols = [column(a) for a in (rho0, theta0, qv0, pi0, rho0_theta0)]
     bc = _lib.BaseCols(*[c.data_ptr() for c in cols])
     ptrs = (ctypes.c_void_p * len(phys_state))(*[_lib.ptr(a) for a in phys_state])
