@@ -180,6 +180,120 @@ int lex_profile_enable(int on, int capacity_pairs);
 int lex_profile_read(double* ms, int* counts, int n);
 const char* lex_profile_kernel_name(int id);
 
+/* ---- adjoint: vector-Jacobian products of the step -------------------------------
+ * The reference differentiates implicitly with jax.value_and_grad (lex_dl_train.py:114);
+ * these are the hand-written reverses of the forward entry points above.  Every *_vjp
+ * ACCUMULATES into its adjoint outputs (the caller zeroes them).  Adjoints of padded
+ * inputs are returned per padded position (ghosts included); lex_fold reduces them onto
+ * the interior. */
+
+/* Reverse of padding3_array / padding3u_array / padding3v_array
+ * (one_step_integration.py:950-985): add every ghost's adjoint onto its interior source
+ * and clear the ghost.  mode 0 scalar, 1 u, 2 v, 3 w (ghost z levels of w included). */
+int lex_fold(const lex_grid_t* g, int mode, float* adj, void* stream);
+
+/* advection_scalar (advection.py:9-34) with flow_divergence = get_divergence(u, v, w)
+ * (:200-226) and a prescribed surface flux: lam (nx,ny,nz) -> ls, lu, lv, lw (padded). */
+int lex_adv_scalar_vjp(const lex_grid_t* g, const float* rho0, const float* s, const float* u, const float* v,
+                       const float* w, const float* lam, float weps, float* ls, float* lu, float* lv, float* lw,
+                       void* stream);
+/* advection_u / _v / _w (advection.py:37-149), weps 1e-16, flow divergence recomputed
+ * from (u, v, w): lam on the u, v or w points -> lu, lv, lw (padded). */
+int lex_adv_u_vjp(const lex_grid_t* g, const float* rho0, const float* u, const float* v, const float* w,
+                  const float* lam, float* lu, float* lv, float* lw, void* stream);
+int lex_adv_v_vjp(const lex_grid_t* g, const float* rho0, const float* u, const float* v, const float* w,
+                  const float* lam, float* lu, float* lv, float* lw, void* stream);
+int lex_adv_w_vjp(const lex_grid_t* g, const float* rho0, const float* u, const float* v, const float* w,
+                  const float* lam, float* lu, float* lv, float* lw, void* stream);
+/* b8w of calculate_buoyancy (pressure_equations.py:123-172): lam (nx,ny,nz+1) -> lth, lqv. */
+int lex_buoyancy_vjp(const lex_grid_t* g, const float* theta0, const float* qv0, const float* th, const float* qv,
+                     const float* lam_b8w, float* lth, float* lqv, void* stream);
+/* theta_rho = th (1 + Rv/Rd qv) / (1 + qv) elementwise (one_step_integration.py:135). */
+int lex_theta_rho_vjp(long n, const float* th, const float* qv, const float* ltr, float* lth, float* lqv,
+                      void* stream);
+/* One acoustic substep (one_step_integration.py:528-553): the adjoints of the substep's
+ * outputs luo, lvo, lwo, lpo (padded; folded in place) -> adjoints of its inputs lu, lv,
+ * lw, lp (u, v, w, pi'), of pi* (lps), theta_rho (ltr) and the rhs (lru, lrv, lrw).
+ * lU, lV, lW: caller scratch of the interior u, v, w sizes. */
+int lex_acoustic_substep_vjp(const lex_grid_t* g, const float* rho0, const float* pi0, const float* u,
+                             const float* v, const float* w, const float* p, const float* ps, const float* ru,
+                             const float* rv, const float* rw, const float* tr, float dts, float* luo, float* lvo,
+                             float* lwo, float* lpo, float* lU, float* lV, float* lW, float* lu, float* lv,
+                             float* lw, float* lp, float* lps, float* ltr, float* lru, float* lrv, float* lrw,
+                             void* stream);
+/* Smagorinsky chain compute_deformation -> compute_nm -> compute_k -> compute_smag
+ * (turbulence.py:9-111, :298-446): adjoints of sgs_u, sgs_v, sgs_w and of each scalar's
+ * SGS tendency -> lu, lv, lw, lth, lqv (padded), each scalar's adjoint, and dL/dCs
+ * accumulated into *lcs (device FP64; may be NULL). */
+size_t lex_smag_vjp_workspace_bytes(const lex_grid_t* g);
+int lex_smag_vjp(const lex_grid_t* g, const float* rho0, const float* u, const float* v, const float* w,
+                 const float* th, const float* qv, int n_scalars, const float* const* scalars, float cs,
+                 float prandtl, const float* lsu, const float* lsv, const float* lsw, const float* const* lss,
+                 float* lu, float* lv, float* lw, float* lth, float* lqv, float* const* lscalars, double* lcs,
+                 void* workspace, size_t workspace_bytes, void* stream);
+/* One WS-RK3 step (ws_rk3_big_step, one_sprint_functions.py:10-30; solver_opt 2):
+ * re-runs the step with a tape, then reverses it.  lam_out: adjoint of the output state
+ * (6 + n_passive padded fields; folded in place); lam_in: receives the adjoint of the
+ * input state (folded: ghosts zero); lam_cs: dL/dCs accumulated (device FP64, may be
+ * NULL).  state is not modified. */
+size_t lex_rk3_vjp_workspace_bytes(const lex_grid_t* g, float dt, int n_sound, int n_passive);
+int lex_rk3_step_vjp(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b, float* const* state,
+                     float* const* lam_out, float* const* lam_in, double* lam_cs, void* workspace,
+                     size_t workspace_bytes, void* stream);
+
+/* ---- learned SGS correction net: 3x3x3 'SAME' convolutions on BF16 MFMA -----------
+ * Replaces flax nn.Conv(kernel (3,3,3), strides 1, padding 'SAME', bias) and nn.gelu of
+ * simpleCNN (dl_models.py:19-30), and their jax.grad.  Activations are channels-last
+ * (x, y, z, c) BF16 with the channel count padded to 16 (c <= 16) or 32; weights FP32
+ * W[kx*9 + ky*3 + kz][cin][cout] (flax's (3,3,3,in,out) kernel) packed to BF16 by
+ * lex_conv3d_pack_weights (flip = 1 packs the transposed convolution of the data
+ * gradient).  Accumulation is FP32.  Channels 1..32. */
+size_t lex_conv3d_packed_weight_bytes(int cin, int cout, int flip);
+int lex_conv3d_pack_weights(const float* w, int cin, int cout, int flip, void* wp, void* stream);
+/* y = gelu?(conv(x) + bias): BF16 y (padded cout) or FP32 yf (cout channels); pre
+ * (optional) receives the FP32 pre-activation (padded cout channels). */
+int lex_conv3d_bf16_fwd(int nx, int ny, int nz, int cin, int cout, const void* x, const void* wp, const float* bias,
+                        int gelu, int out_fp32, void* y, float* yf, float* pre, void* stream);
+/* dx (FP32, cin channels) = transposed convolution of g_pre (BF16, padded cout
+ * channels) with the flip-packed weights; zero_bias: padded-cin zeros. */
+int lex_conv3d_bf16_bwd_data(int nx, int ny, int nz, int cin, int cout, const void* g_pre, const void* wp_flip,
+                             const float* zero_bias, float* dx, void* stream);
+/* dw[27][cin_p][cout_p] += sum_p x[p + o] g_pre[p]; db[cout] += sum_p g_pre[p] (db may
+ * be NULL).  x: BF16, padded cin; g_pre: BF16, padded cout. */
+int lex_conv3d_bf16_bwd_weight(int nx, int ny, int nz, int cin, int cout, const void* x, const void* g_pre, float* dw,
+                               float* db, void* stream);
+/* g_pre = g_post * gelu'(pre) (tanh form) or g_post when gelu == 0, as BF16 with cp
+ * channels (channels >= c_real zero). */
+int lex_gelu_bwd(long ncell, int cp, int c_real, int cpre, const float* gpost, int gstride, const float* pre, int gelu,
+                 void* gpre, void* stream);
+/* FP32 channels-last (c_real channels at stride cstride) -> BF16 with cp channels. */
+int lex_to_bf16_channels_last(long ncell, int c_real, int cstride, int cp, const float* in, void* out, void* stream);
+
+/* Net inputs: 11 cell-centred channels (u, v, w at centres, theta - theta0, qv - qv0,
+ * and S11, S22, S33, S12, S13, S23 staggered as compute_deformation,
+ * turbulence.py:298-388, without the rho factor), out = (x - shift[c]) * scale[c] (shift,
+ * scale: host arrays of 11),
+ * channels-last FP32 (nx, ny, nz, 11).  The reference's packing of its own 5 inputs is
+ * phys_state_tuple2array (dl_models.py:125-155). */
+int lex_sgs_features(const lex_grid_t* g, const float* theta0, const float* qv0, const float* u, const float* v,
+                     const float* w, const float* th, const float* qv, const float* shift, const float* scale,
+                     float* out, void* stream);
+/* Net outputs y (nx, ny, nz, 5), times out_scale[c] (host array of 5), -> SGS tendencies
+ * on compute_smag's points (turbulence.py:9-111): sgs_u (nx+1,ny,nz), sgs_v
+ * (nx,ny+1,nz), sgs_w (nx,ny,nz+1; zero at the lids), sgs_theta, sgs_qv (nx,ny,nz). */
+int lex_sgs_apply(const lex_grid_t* g, const float* y, const float* out_scale, float* su, float* sv, float* sw,
+                  float* sth, float* sqv, void* stream);
+/* Block-mean coarsening by (fx, fy, fz) (initial_comparison/plot_theta_pert_coarsened.m:7-8)
+ * of a padded fine field into an interior coarse field: mode 0 cell mean of a scalar,
+ * 1/2/3 face-area means of u/v/w on the coarse faces. */
+int lex_coarsen(const lex_grid_t* g, int mode, int fx, int fy, int fz, const float* a, float* out, void* stream);
+/* lex_rk3_step with caller-supplied SGS tendencies (frozen over the step, as the
+ * stage-0 Smagorinsky tendencies are, one_step_integration.py:57-76): su, sv, sw on the
+ * u, v, w points, ss[] for theta, qv and the passive scalars. */
+int lex_rk3_step_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b, float* const* state,
+                         const float* su, const float* sv, const float* sw, const float* const* ss, void* workspace,
+                         size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

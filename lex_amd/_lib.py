@@ -95,9 +95,39 @@ _SIGS = {
     "lex_profile_enable": [_I, _I],
     "lex_profile_read": [C.POINTER(_D), C.POINTER(_I), _I],
     "lex_profile_kernel_name": [_I],
+    # adjoint (vector-Jacobian products); every *_vjp accumulates into its outputs
+    "lex_fold": [_G, _I, _P, _P],
+    "lex_adv_scalar_vjp": [_G, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P],
+    "lex_adv_u_vjp": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lex_adv_v_vjp": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lex_adv_w_vjp": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lex_buoyancy_vjp": [_G, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lex_theta_rho_vjp": [C.c_long, _P, _P, _P, _P, _P, _P],
+    "lex_acoustic_substep_vjp": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F] + [_P] * 16 + [_P],
+    "lex_smag_vjp_workspace_bytes": [_G],
+    "lex_smag_vjp": [_G, _P, _P, _P, _P, _P, _P, _I, C.POINTER(_P), _F, _F, _P, _P, _P, C.POINTER(_P),
+                     _P, _P, _P, _P, _P, C.POINTER(_P), _P, _P, C.c_size_t, _P],
+    "lex_rk3_vjp_workspace_bytes": [_G, _F, _I, _I],
+    "lex_rk3_step_vjp": [_G, C.POINTER(StepOpts), C.POINTER(BaseCols), C.POINTER(_P), C.POINTER(_P),
+                         C.POINTER(_P), _P, _P, C.c_size_t, _P],
+    # learned SGS net: BF16 MFMA convolutions
+    "lex_conv3d_packed_weight_bytes": [_I, _I, _I],
+    "lex_conv3d_pack_weights": [_P, _I, _I, _I, _P, _P],
+    "lex_conv3d_bf16_fwd": [_I, _I, _I, _I, _I, _P, _P, _P, _I, _I, _P, _P, _P, _P],
+    "lex_conv3d_bf16_bwd_data": [_I, _I, _I, _I, _I, _P, _P, _P, _P, _P],
+    "lex_conv3d_bf16_bwd_weight": [_I, _I, _I, _I, _I, _P, _P, _P, _P, _P],
+    "lex_gelu_bwd": [C.c_long, _I, _I, _I, _P, _I, _P, _I, _P, _P],
+    "lex_to_bf16_channels_last": [C.c_long, _I, _I, _I, _P, _P, _P],
+    "lex_sgs_features": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lex_sgs_apply": [_G, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lex_coarsen": [_G, _I, _I, _I, _I, _P, _P, _P],
+    "lex_rk3_step_ext_sgs": [_G, C.POINTER(StepOpts), C.POINTER(BaseCols), C.POINTER(_P), _P, _P, _P,
+                             C.POINTER(_P), _P, C.c_size_t, _P],
 }
 _RESTYPE = {"lex_last_error": C.c_char_p, "lex_bicgstab_workspace_bytes": C.c_size_t,
-            "lex_rk3_workspace_bytes": C.c_size_t, "lex_profile_kernel_name": C.c_char_p}
+            "lex_rk3_workspace_bytes": C.c_size_t, "lex_profile_kernel_name": C.c_char_p,
+            "lex_smag_vjp_workspace_bytes": C.c_size_t, "lex_rk3_vjp_workspace_bytes": C.c_size_t,
+            "lex_conv3d_packed_weight_bytes": C.c_size_t}
 
 
 def profile_enable(on: bool, capacity_pairs: int = 8192):

@@ -1,0 +1,196 @@
+"""Hand-written adjoint of the compressible WS-RK3 step and a differentiable sprint.
+
+The reference obtains gradients only implicitly, through ``jax.value_and_grad`` of its
+training loss (lex_dl_train.py:114), without rematerialisation.  Here the reverse of one
+step is the device routine ``lex_rk3_step_vjp`` (a taped re-run of the step, then the
+stages and the Smagorinsky chain in reverse), and ``WsRk3Sprint`` wraps N steps in a
+``torch.autograd.Function`` that keeps a checkpoint every k steps and recomputes each
+segment in the backward pass.
+
+Gradients flow to the initial potential-temperature perturbation theta'_0 (interior) and
+to the Smagorinsky constant Cs; the other initial fields are constants.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from ._core import column
+from .one_sprint_functions import fused_step
+
+_WSV: dict = {}
+_WSS: dict = {}
+
+
+def _vjp_workspace(g, dt, n_sound, n_passive, device):
+    key = (g.nx, g.ny, g.nz, float(dt), int(n_sound), n_passive, str(device))
+    ws = _WSV.get(key)
+    if ws is None:
+        nbytes = _lib.load().lex_rk3_vjp_workspace_bytes(g, float(dt), int(n_sound), int(n_passive))
+        if nbytes == 0:
+            raise _lib.LexError("lex_rk3_vjp_workspace_bytes: invalid grid or options")
+        _WSV.clear()
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        _WSV[key] = ws
+    return ws
+
+
+def _base_cols(base_state):
+    rho0_theta0, rho0, theta0, pi0, qv0 = base_state[:5]
+    cols = [column(a) for a in (rho0, theta0, qv0, pi0, rho0_theta0)]
+    return cols, _lib.BaseCols(*[c.data_ptr() for c in cols])
+
+
+def _ptrs(ts):
+    return (ctypes.c_void_p * len(ts))(*[_lib.ptr(t) for t in ts])
+
+
+def step_vjp(phys_state, lam_out, base_state, grid, dt, n_sound, cs, prandtl, turb=1):
+    """Adjoint of one solver_opt-2 WS-RK3 step.
+
+    phys_state: the step's input state (padded tensors, not modified); lam_out: adjoint of
+    the step's output state (padded; not modified).  Returns (lam_in, dcs): the adjoint of
+    the input state (padded, ghosts folded onto the interior) and dL/dCs as a device
+    float64 tensor of shape (1,).
+    """
+    n_passive = len(phys_state) - 6
+    opts = _lib.StepOpts(float(dt), int(n_sound), int(turb), float(cs), float(prandtl), n_passive, 1)
+    cols, bc = _base_cols(base_state)
+    lo = tuple(t.clone() for t in lam_out)
+    li = tuple(torch.empty_like(t) for t in phys_state)
+    dcs = torch.zeros(1, dtype=torch.float64, device=phys_state[0].device)
+    ws = _vjp_workspace(grid, dt, n_sound, n_passive, phys_state[0].device)
+    _lib.call("lex_rk3_step_vjp", grid, opts, bc, _ptrs(phys_state), _ptrs(lo), _ptrs(li), dcs.data_ptr(),
+              _lib.ptr(ws), ws.numel(), _lib.stream())
+    del cols
+    return li, dcs
+
+
+def fold(adj, mode, grid):
+    """Add ghost adjoints onto their interior sources in place (mode 0 c, 1 u, 2 v, 3 w)."""
+    _lib.call("lex_fold", grid, int(mode), _lib.ptr(adj), _lib.stream())
+    return adj
+
+
+# ---------------------------------------------------------------- per-operator VJPs
+def adv_scalar_vjp(grid, rho0c, s, u, v, w, lam, weps, ls, lu, lv, lw):
+    _lib.call("lex_adv_scalar_vjp", grid, _lib.ptr(rho0c), _lib.ptr(s), _lib.ptr(u), _lib.ptr(v), _lib.ptr(w),
+              _lib.ptr(lam), float(weps), _lib.ptr(ls), _lib.ptr(lu), _lib.ptr(lv), _lib.ptr(lw), _lib.stream())
+
+
+def adv_momentum_vjp(which, grid, rho0c, u, v, w, lam, lu, lv, lw):
+    _lib.call(f"lex_adv_{which}_vjp", grid, _lib.ptr(rho0c), _lib.ptr(u), _lib.ptr(v), _lib.ptr(w),
+              _lib.ptr(lam), _lib.ptr(lu), _lib.ptr(lv), _lib.ptr(lw), _lib.stream())
+
+
+def buoyancy_vjp(grid, theta0c, qv0c, th, qv, lam_b8w, lth, lqv):
+    _lib.call("lex_buoyancy_vjp", grid, _lib.ptr(theta0c), _lib.ptr(qv0c), _lib.ptr(th), _lib.ptr(qv),
+              _lib.ptr(lam_b8w), _lib.ptr(lth), _lib.ptr(lqv), _lib.stream())
+
+
+def theta_rho_vjp(th, qv, ltr, lth, lqv):
+    _lib.call("lex_theta_rho_vjp", th.numel(), _lib.ptr(th), _lib.ptr(qv), _lib.ptr(ltr), _lib.ptr(lth),
+              _lib.ptr(lqv), _lib.stream())
+
+
+def acoustic_substep_vjp(grid, rho0c, pi0c, u, v, w, p, ps, ru, rv, rw, tr, dts, luo, lvo, lwo, lpo,
+                         lu, lv, lw, lp, lps, ltr, lru, lrv, lrw):
+    """luo..lpo are folded in place; scratch for the face adjoints is allocated here."""
+    lU, lV, lW = torch.empty_like(ru), torch.empty_like(rv), torch.empty_like(rw)
+    _lib.call("lex_acoustic_substep_vjp", grid, *[_lib.ptr(t) for t in (rho0c, pi0c, u, v, w, p, ps, ru, rv, rw,
+                                                                        tr)],
+              float(dts), *[_lib.ptr(t) for t in (luo, lvo, lwo, lpo, lU, lV, lW, lu, lv, lw, lp, lps, ltr, lru,
+                                                  lrv, lrw)], _lib.stream())
+
+
+def smag_vjp(grid, rho0c, u, v, w, th, qv, scalars, cs, prandtl, lsu, lsv, lsw, lss, lu, lv, lw, lth, lqv,
+             lscalars, lcs=None):
+    key = (grid.nx, grid.ny, grid.nz, str(u.device))
+    ws = _WSS.get(key)
+    if ws is None:
+        ws = torch.empty(_lib.load().lex_smag_vjp_workspace_bytes(grid), dtype=torch.uint8, device=u.device)
+        _WSS.clear()
+        _WSS[key] = ws
+    n = len(scalars)
+    _lib.call("lex_smag_vjp", grid, _lib.ptr(rho0c), _lib.ptr(u), _lib.ptr(v), _lib.ptr(w), _lib.ptr(th),
+              _lib.ptr(qv), n, _ptrs(scalars), float(cs), float(prandtl), _lib.ptr(lsu), _lib.ptr(lsv),
+              _lib.ptr(lsw), _ptrs(lss), _lib.ptr(lu), _lib.ptr(lv), _lib.ptr(lw), _lib.ptr(lth), _lib.ptr(lqv),
+              _ptrs(lscalars), None if lcs is None else lcs.data_ptr(), _lib.ptr(ws), ws.numel(), _lib.stream())
+
+
+# ---------------------------------------------------------------- differentiable sprint
+def _interior(a):
+    return a[3:-3, 3:-3, 1:-1]
+
+
+def _pad_scalar(interior, grid):
+    out = torch.empty((grid.nx + 6, grid.ny + 6, grid.nz + 2), dtype=torch.float32, device=interior.device)
+    _lib.call("lex_pad", 0, grid.nx, grid.ny, grid.nz, 3, _lib.ptr(interior.contiguous()), _lib.ptr(out),
+              _lib.stream())
+    return out
+
+
+class WsRk3Sprint(torch.autograd.Function):
+    """theta'_0 (interior), Cs -> theta' after nsteps WS-RK3 steps (interior).
+
+    forward(theta_p0, cs, state0, base_state, grid, cfg) with cfg = dict(dt, n_sound,
+    prandtl, nsteps, checkpoint_every).  state0 supplies u, v, w, pi', qv and passive
+    scalars (its theta is replaced by theta0 + theta'_0).  Checkpoints of the full state
+    are kept every `checkpoint_every` steps; the backward pass recomputes each segment
+    with the forward step and walks it back with step_vjp.
+    """
+
+    @staticmethod
+    def forward(ctx, theta_p0, cs, state0, base_state, grid, cfg):
+        cs_f = float(cs)
+        th0_int = _interior(base_state[2]).contiguous()
+        th = _pad_scalar(th0_int + theta_p0.detach().to(torch.float32), grid)
+        x = (th,) + tuple(t.clone() for t in state0[1:])
+        k = int(cfg["checkpoint_every"])
+        n = int(cfg["nsteps"])
+        ckpts = {0: tuple(t.clone() for t in x)}
+        with torch.no_grad():
+            for i in range(n):
+                fused_step(x, base_state, grid, (2, False, False, 0, False, False, 1), dt=cfg["dt"],
+                           n_sound=cfg["n_sound"], cs=cs_f, prandtl=cfg["prandtl"])
+                if (i + 1) % k == 0 and i + 1 < n:
+                    ckpts[i + 1] = tuple(t.clone() for t in x)
+        ctx.ckpts = ckpts
+        ctx.base_state, ctx.grid, ctx.cfg, ctx.cs = base_state, grid, cfg, cs_f
+        ctx.cs_is_tensor = torch.is_tensor(cs)
+        return _interior(x[0]) - th0_int
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        cfg, grid, base = ctx.cfg, ctx.grid, ctx.base_state
+        n, k = int(cfg["nsteps"]), int(cfg["checkpoint_every"])
+        x0 = ctx.ckpts[0]
+        lam = tuple(torch.zeros_like(t) for t in x0)
+        _interior(lam[0]).copy_(grad_out.to(torch.float32))
+        dcs = torch.zeros(1, dtype=torch.float64, device=x0[0].device)
+        starts = sorted(ctx.ckpts.keys())
+        for s in reversed(starts):
+            end = min(s + k, n)
+            states = [ctx.ckpts[s]]
+            with torch.no_grad():
+                for _ in range(s, end - 1):
+                    y = tuple(t.clone() for t in states[-1])
+                    fused_step(y, base, grid, (2, False, False, 0, False, False, 1), dt=cfg["dt"],
+                               n_sound=cfg["n_sound"], cs=ctx.cs, prandtl=cfg["prandtl"])
+                    states.append(y)
+            for st in reversed(states):
+                lam, d = step_vjp(st, lam, base, grid, cfg["dt"], cfg["n_sound"], ctx.cs, cfg["prandtl"])
+                dcs += d
+        g_theta = _interior(lam[0]).clone()
+        g_cs = dcs.to(torch.float32).reshape(()) if ctx.cs_is_tensor else None
+        ctx.ckpts = None
+        return g_theta, g_cs, None, None, None, None
+
+
+def sprint_theta(theta_p0, cs, state0, base_state, grid, dt, n_sound, nsteps, checkpoint_every, prandtl=1.0 / 3.0):
+    """Differentiable nsteps-step run: returns theta' (interior) at the last step."""
+    cfg = dict(dt=float(dt), n_sound=int(n_sound), prandtl=float(prandtl), nsteps=int(nsteps),
+               checkpoint_every=int(checkpoint_every))
+    return WsRk3Sprint.apply(theta_p0, cs, state0, base_state, grid, cfg)

@@ -1,0 +1,945 @@
+// Hand-written adjoint (vector-Jacobian products) of the compressible WS-RK3 step.
+//
+// Every forward kernel of the per-operator path (lex_ops.hip, lex_smag.hip) has a reverse
+// kernel here: one thread per forward output point recomputes the local forward
+// quantities and scatters lambda * d(out)/d(in) into the adjoints of the padded inputs
+// with hardware float atomics (global_atomic_add_f32).  All reverse kernels ACCUMULATE
+// into their adjoint outputs.  Ghost images written by the forward stores
+// (padding3*_array) are reversed by k_fold: every ghost's adjoint is added to its
+// interior source and the ghost cleared.
+//
+// Differentiation conventions are JAX's on the reference code: lax.select passes the
+// gradient to the selected branch only; d|x|/dx = sign(x) (0 at 0); fmin(1e30, r) passes
+// it to r only when r < 1e30; where(s2n2 > 0, s2n2, 0) blocks it where false.
+#pragma once
+#include "lex_common.hpp"
+
+namespace lex {
+
+__device__ __forceinline__ void aadd(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+// ---------------------------------------------------------------- WENO gradients
+// R = weno5(a..e) and g[0..4] = dR/d(a, b, c, d, e).  With r_k = tau/(beta_k + eps),
+// w_k = d_k (1 + r_k^2), R = sum w_k q_k / W:
+//   dR/dr_k = (q_k - R) 2 d_k r_k / W,  dr_k/dtau = 1/(beta_k+eps),  dr_k/dbeta_k = -r_k/(beta_k+eps)
+__device__ float weno5_grad(float a, float b, float c, float d, float e, float weps, float* g) {
+  const float c1312 = (float)(13.0 / 12.0);
+  float t1 = a - 2.0f * b + c, t2 = a - 4.0f * b + 3.0f * c;
+  float t3 = b - 2.0f * c + d, t4 = b - d;
+  float t5 = c - 2.0f * d + e, t6 = 3.0f * c - 4.0f * d + e;
+  float b1 = c1312 * t1 * t1 + 0.25f * t2 * t2;
+  float b2 = c1312 * t3 * t3 + 0.25f * t4 * t4;
+  float b3 = c1312 * t5 * t5 + 0.25f * t6 * t6;
+  float df = b1 - b3, tau = fabsf(df);
+  float sg = (df > 0.f) ? 1.f : ((df < 0.f) ? -1.f : 0.f);
+  float e1 = b1 + weps, e2 = b2 + weps, e3 = b3 + weps;
+  float x1 = tau / e1, x2 = tau / e2, x3 = tau / e3;
+  bool c1 = !(x1 < 1.0e30f), c2 = !(x2 < 1.0e30f), c3 = !(x3 < 1.0e30f);
+  float r1 = fminf(1.0e30f, x1), r2 = fminf(1.0e30f, x2), r3 = fminf(1.0e30f, x3);
+  float w1 = 0.1f * (1.0f + r1 * r1), w2 = 0.6f * (1.0f + r2 * r2), w3 = 0.3f * (1.0f + r3 * r3);
+  const float s6 = (float)(1.0 / 6.0);
+  float q1 = (float)(2.0 / 6.0) * a + (float)(-7.0 / 6.0) * b + (float)(11.0 / 6.0) * c;
+  float q2 = -s6 * b + (float)(5.0 / 6.0) * c + (float)(2.0 / 6.0) * d;
+  float q3 = (float)(2.0 / 6.0) * c + (float)(5.0 / 6.0) * d - s6 * e;
+  float W = w1 + w2 + w3;
+  float R = (w1 * q1 + w2 * q2 + w3 * q3) / W;
+  float C1 = c1 ? 0.f : (q1 - R) * 0.2f * r1 / (W * e1);
+  float C2 = c2 ? 0.f : (q2 - R) * 1.2f * r2 / (W * e2);
+  float C3 = c3 ? 0.f : (q3 - R) * 0.6f * r3 / (W * e3);
+  float Cs = C1 + C2 + C3;
+  float E1 = sg * Cs - C1 * r1, E2 = -C2 * r2, E3 = -sg * Cs - C3 * r3;  // dR/dbeta_k
+  float A1 = 2.f * c1312 * t1, B1 = 0.5f * t2, A2 = 2.f * c1312 * t3, B2 = 0.5f * t4;
+  float A3 = 2.f * c1312 * t5, B3 = 0.5f * t6;
+  float W1 = w1 / W, W2 = w2 / W, W3 = w3 / W;
+  g[0] = E1 * (A1 + B1) + W1 * (float)(2.0 / 6.0);
+  g[1] = E1 * (-2.f * A1 - 4.f * B1) + E2 * (A2 + B2) + W1 * (float)(-7.0 / 6.0) - W2 * s6;
+  g[2] = E1 * (A1 + 3.f * B1) + E2 * (-2.f * A2) + E3 * (A3 + 3.f * B3) + W1 * (float)(11.0 / 6.0) +
+         W2 * (float)(5.0 / 6.0) + W3 * (float)(2.0 / 6.0);
+  g[3] = E2 * (A2 - B2) + E3 * (-2.f * A3 - 4.f * B3) + W2 * (float)(2.0 / 6.0) + W3 * (float)(5.0 / 6.0);
+  g[4] = E3 * (A3 + B3) - W3 * s6;
+  return R;
+}
+
+// R = weno3(a, b, c) and g[0..2]
+__device__ float weno3_grad(float a, float b, float c, float weps, float* g) {
+  float b1 = (a - b) * (a - b), b2 = (b - c) * (b - c);
+  float df = b1 - b2, tau = fabsf(df);
+  float sg = (df > 0.f) ? 1.f : ((df < 0.f) ? -1.f : 0.f);
+  float e1 = b1 + weps, e2 = b2 + weps;
+  float x1 = tau / e1, x2 = tau / e2;
+  bool c1 = !(x1 < 1.0e30f), c2 = !(x2 < 1.0e30f);
+  float r1 = fminf(1.0e30f, x1), r2 = fminf(1.0e30f, x2);
+  float w1 = (float)(1.0 / 3.0) * (1.f + r1 * r1), w2 = (float)(2.0 / 3.0) * (1.f + r2 * r2);
+  float q1 = -0.5f * a + 1.5f * b, q2 = 0.5f * b + 0.5f * c;
+  float W = w1 + w2;
+  float R = (w1 * q1 + w2 * q2) / W;
+  float C1 = c1 ? 0.f : (q1 - R) * (float)(2.0 / 3.0) * r1 / (W * e1);
+  float C2 = c2 ? 0.f : (q2 - R) * (float)(4.0 / 3.0) * r2 / (W * e2);
+  float E1 = sg * (C1 + C2) - C1 * r1, E2 = -sg * (C1 + C2) - C2 * r2;
+  float W1 = w1 / W, W2 = w2 / W;
+  g[0] = E1 * 2.f * (a - b) - 0.5f * W1;
+  g[1] = -E1 * 2.f * (a - b) + E2 * 2.f * (b - c) + 1.5f * W1 + 0.5f * W2;
+  g[2] = -E2 * 2.f * (b - c) + 0.5f * W2;
+  return R;
+}
+
+// hrecon (lex_common.hpp) with its gradient w.r.t. q[m-3+t], t = 0..5
+__device__ float hrecon_grad(const float* q6, bool pos, float weps, float* g6) {
+  float g[5];
+  float R;
+  for (int t = 0; t < 6; ++t) g6[t] = 0.f;
+  if (pos) {
+    R = weno5_grad(q6[0], q6[1], q6[2], q6[3], q6[4], weps, g);
+    for (int t = 0; t < 5; ++t) g6[t] = g[t];
+  } else {
+    R = weno5_grad(q6[5], q6[4], q6[3], q6[2], q6[1], weps, g);
+    for (int t = 0; t < 5; ++t) g6[5 - t] = g[t];
+  }
+  return R;
+}
+
+// vrecon (lex_common.hpp) with its gradient w.r.t. p[f-2+t], t = 0..5
+__device__ float vrecon_grad(const float* __restrict__ p, int L, int f, bool pos, float weps, float* g6) {
+  for (int t = 0; t < 6; ++t) g6[t] = 0.f;
+  if (f <= 0 || f >= L - 2) return 0.f;
+  float g[5];
+  float R;
+  if (pos) {
+    if (f == 1) {  // 0.5 (p1 + p2)
+      g6[2] = 0.5f;
+      g6[3] = 0.5f;
+      return 0.5f * (p[1] + p[2]);
+    }
+    if (f == 2 || f == L - 3) {  // weno3(p[f-1], p[f], p[f+1])
+      R = weno3_grad(p[f - 1], p[f], p[f + 1], weps, g);
+      g6[1] = g[0]; g6[2] = g[1]; g6[3] = g[2];
+      return R;
+    }
+    R = weno5_grad(p[f - 2], p[f - 1], p[f], p[f + 1], p[f + 2], weps, g);
+    for (int t = 0; t < 5; ++t) g6[t] = g[t];
+    return R;
+  }
+  if (f == 1) {  // weno3(p3, p2, p1)
+    R = weno3_grad(p[3], p[2], p[1], weps, g);
+    g6[4] = g[0]; g6[3] = g[1]; g6[2] = g[2];
+    return R;
+  }
+  if (f == L - 3) {  // 0.5 (p[f+1] + p[f])
+    g6[3] = 0.5f;
+    g6[2] = 0.5f;
+    return 0.5f * (p[L - 2] + p[L - 3]);
+  }
+  if (f == L - 4) {  // weno3(p[f+2], p[f+1], p[f])
+    R = weno3_grad(p[f + 2], p[f + 1], p[f], weps, g);
+    g6[4] = g[0]; g6[3] = g[1]; g6[2] = g[2];
+    return R;
+  }
+  R = weno5_grad(p[f + 3], p[f + 2], p[f + 1], p[f], p[f - 1], weps, g);
+  g6[5] = g[0]; g6[4] = g[1]; g6[3] = g[2]; g6[2] = g[3]; g6[1] = g[4];
+  return R;
+}
+
+__device__ __forceinline__ void scatter6(float* __restrict__ lq, long base, long stride, const float* g6, float coef) {
+#pragma unroll
+  for (int t = 0; t < 6; ++t)
+    if (g6[t] != 0.f) aadd(lq + base + t * stride, coef * g6[t]);
+}
+
+// vertical variant: unit-stride column, stencil start f-2, column length L
+__device__ __forceinline__ void scatter6z(float* __restrict__ lcol, int f, int L, const float* g6, float coef) {
+#pragma unroll
+  for (int t = 0; t < 6; ++t) {
+    int idx = f - 2 + t;
+    if (g6[t] != 0.f && idx >= 0 && idx < L) aadd(lcol + idx, coef * g6[t]);
+  }
+}
+
+__device__ __forceinline__ float r8w_col(const Dims& d, const float* rc, int Kw) {
+  return (Kw <= 0 || Kw >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kw - 1] + rc[Kw]);
+}
+
+// flow divergence div(rho0 u) at padded cell (I,J,K) (advection.py:200-226)
+__device__ __forceinline__ float fd_at(const Dims& d, const float* __restrict__ rc, const float* __restrict__ u,
+                                       const float* __restrict__ v, const float* __restrict__ w, int I, int J, int K) {
+  float r = rc[K];
+  return (r * u[iu(d, I + 1, J, K)] - r * u[iu(d, I, J, K)]) * d.idx +
+         (r * v[iv(d, I, J + 1, K)] - r * v[iv(d, I, J, K)]) * d.idy +
+         (r8w_col(d, rc, K + 1) * w[iw(d, I, J, K + 1)] - r8w_col(d, rc, K) * w[iw(d, I, J, K)]) * d.idz;
+}
+
+// adjoint of fd_at with weight g
+__device__ __forceinline__ void fd_vjp(const Dims& d, const float* __restrict__ rc, int I, int J, int K, float g,
+                                       float* lu, float* lv, float* lw) {
+  if (g == 0.f) return;
+  float r = rc[K];
+  aadd(lu + iu(d, I + 1, J, K), g * r * d.idx);
+  aadd(lu + iu(d, I, J, K), -g * r * d.idx);
+  aadd(lv + iv(d, I, J + 1, K), g * r * d.idy);
+  aadd(lv + iv(d, I, J, K), -g * r * d.idy);
+  float hi = r8w_col(d, rc, K + 1), lo = r8w_col(d, rc, K);
+  if (hi != 0.f) aadd(lw + iw(d, I, J, K + 1), g * hi * d.idz);
+  if (lo != 0.f) aadd(lw + iw(d, I, J, K), -g * lo * d.idz);
+}
+
+// ------------------------------------------------------------- A2 scalar advection
+// T = [-div F + s div(rho0 u)] / rho0 at interior cells (advection.py:9-34);
+// lam = dL/dT on the interior (nx, ny, nz); surface flux prescribed.
+__global__ void k_adv_scalar_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ s,
+                                 const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                                 const float* __restrict__ lam, float weps, float* __restrict__ ls,
+                                 float* __restrict__ lu, float* __restrict__ lv, float* __restrict__ lw) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  float L = lam[((long)i * d.ny + j) * d.nz + k];
+  if (L == 0.f) return;
+  int I = i + 3, J = j + 3, K = k + 1;
+  float r = rc[K];
+  float mu = L / r;
+  const long sx = d.sxc(), sy = d.syc();
+  long c = ic(d, I, J, K);
+  float q6[6], g6[6];
+  for (int t2 = 0; t2 < 2; ++t2) {  // x faces I (lower) and I+1 (upper)
+    int m = I + t2;
+    float lf = (t2 == 0 ? mu : -mu) * d.idx;
+    float mass = r * u[iu(d, m, J, K)];
+    long base = ic(d, m - 3, J, K);
+    for (int t = 0; t < 6; ++t) q6[t] = s[base + t * sx];
+    float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+    aadd(lu + iu(d, m, J, K), r * lf * R);
+    scatter6(ls, base, sx, g6, lf * mass);
+  }
+  for (int t2 = 0; t2 < 2; ++t2) {
+    int m = J + t2;
+    float lf = (t2 == 0 ? mu : -mu) * d.idy;
+    float mass = r * v[iv(d, I, m, K)];
+    long base = ic(d, I, m - 3, K);
+    for (int t = 0; t < 6; ++t) q6[t] = s[base + t * sy];
+    float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+    aadd(lv + iv(d, I, m, K), r * lf * R);
+    scatter6(ls, base, sy, g6, lf * mass);
+  }
+  const float* col = s + ic(d, I, J, 0);
+  float* lcol = ls + ic(d, I, J, 0);
+  for (int t2 = 0; t2 < 2; ++t2) {
+    int f = K - 1 + t2;    // face between padded levels f and f+1
+    if (f == 0) continue;  // surface flux is prescribed
+    float lf = (t2 == 0 ? mu : -mu) * d.idz;
+    float rf = 0.5f * (rc[f] + rc[f + 1]);
+    float mass = rf * w[iw(d, I, J, f + 1)];
+    float R = vrecon_grad(col, d.nz + 2, f, upwind_pos(mass), weps, g6);
+    if (R != 0.f) aadd(lw + iw(d, I, J, f + 1), rf * lf * R);
+    scatter6z(lcol, f, d.nz + 2, g6, lf * mass);
+  }
+  aadd(ls + c, mu * fd_at(d, rc, u, v, w, I, J, K));
+  fd_vjp(d, rc, I, J, K, mu * s[c], lu, lv, lw);
+}
+
+// ------------------------------------------------------------ A3 momentum advection
+// reverse of advection_u (advection.py:37-74), lam: (nx+1, ny, nz)
+__global__ void k_adv_u_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
+                            const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ lam,
+                            float* __restrict__ lu, float* __restrict__ lv, float* __restrict__ lw) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int f = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  float L = lam[((long)f * d.ny + j) * d.nz + k];
+  if (L == 0.f) return;
+  const float weps = 1.0e-16f;
+  int I = f + 3, J = j + 3, K = k + 1;
+  float r = rc[K];
+  float mu = L / (0.5f * (r + r));
+  const long sx = d.sxu(), sy = d.syc();
+  float q6[6], g6[6];
+  for (int t2 = 0; t2 < 2; ++t2) {  // x flux at scalar cells I-1, I
+    int m = I + t2;
+    float lf = (t2 == 0 ? mu : -mu) * d.idx;
+    long base = iu(d, m - 3, J, K);
+    for (int t = 0; t < 6; ++t) q6[t] = u[base + t * sx];
+    float mass = 0.5f * (q6[2] + q6[3]) * r;
+    float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+    scatter6(lu, base, sx, g6, lf * mass);
+    float gm = lf * R * 0.5f * r;
+    aadd(lu + base + 2 * sx, gm);
+    aadd(lu + base + 3 * sx, gm);
+  }
+  for (int t2 = 0; t2 < 2; ++t2) {  // y flux at corners
+    int m = J + t2;
+    float lf = (t2 == 0 ? mu : -mu) * d.idy;
+    float mass = 0.5f * (r * v[iv(d, I - 1, m, K)] + r * v[iv(d, I, m, K)]);
+    long base = iu(d, I, m - 3, K);
+    for (int t = 0; t < 6; ++t) q6[t] = u[base + t * sy];
+    float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+    scatter6(lu, base, sy, g6, lf * mass);
+    float gm = lf * R * 0.5f * r;
+    aadd(lv + iv(d, I - 1, m, K), gm);
+    aadd(lv + iv(d, I, m, K), gm);
+  }
+  const float* col = u + iu(d, I, J, 0);
+  float* lcol = lu + iu(d, I, J, 0);
+  for (int t2 = 0; t2 < 2; ++t2) {  // z flux at w levels
+    int fz = K - 1 + t2;
+    if (fz == 0) continue;
+    float lf = (t2 == 0 ? mu : -mu) * d.idz;
+    float rf = 0.5f * (rc[fz] + rc[fz + 1]);
+    float mass = 0.5f * (rf * w[iw(d, I - 1, J, fz + 1)] + rf * w[iw(d, I, J, fz + 1)]);
+    float R = vrecon_grad(col, d.nz + 2, fz, upwind_pos(mass), weps, g6);
+    scatter6z(lcol, fz, d.nz + 2, g6, lf * mass);
+    if (R != 0.f) {
+      float gm = lf * R * 0.5f * rf;
+      aadd(lw + iw(d, I - 1, J, fz + 1), gm);
+      aadd(lw + iw(d, I, J, fz + 1), gm);
+    }
+  }
+  float uc = u[iu(d, I, J, K)];
+  aadd(lu + iu(d, I, J, K), mu * 0.5f * (fd_at(d, rc, u, v, w, I - 1, J, K) + fd_at(d, rc, u, v, w, I, J, K)));
+  fd_vjp(d, rc, I - 1, J, K, mu * 0.5f * uc, lu, lv, lw);
+  fd_vjp(d, rc, I, J, K, mu * 0.5f * uc, lu, lv, lw);
+}
+
+// reverse of advection_v (advection.py:77-114), lam: (nx, ny+1, nz)
+__global__ void k_adv_v_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
+                            const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ lam,
+                            float* __restrict__ lu, float* __restrict__ lv, float* __restrict__ lw) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int f = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (k >= d.nz || f > d.ny) return;
+  float L = lam[((long)i * (d.ny + 1) + f) * d.nz + k];
+  if (L == 0.f) return;
+  const float weps = 1.0e-16f;
+  int I = i + 3, J = f + 3, K = k + 1;
+  float r = rc[K];
+  float mu = L / (0.5f * (r + r));
+  const long sx = d.sxv(), sy = d.syc();
+  float q6[6], g6[6];
+  for (int t2 = 0; t2 < 2; ++t2) {  // x flux at corners
+    int m = I + t2;
+    float lf = (t2 == 0 ? mu : -mu) * d.idx;
+    float mass = 0.5f * (r * u[iu(d, m, J - 1, K)] + r * u[iu(d, m, J, K)]);
+    long base = iv(d, m - 3, J, K);
+    for (int t = 0; t < 6; ++t) q6[t] = v[base + t * sx];
+    float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+    scatter6(lv, base, sx, g6, lf * mass);
+    float gm = lf * R * 0.5f * r;
+    aadd(lu + iu(d, m, J - 1, K), gm);
+    aadd(lu + iu(d, m, J, K), gm);
+  }
+  for (int t2 = 0; t2 < 2; ++t2) {  // y flux at cells J-1, J
+    int m = J + t2;
+    float lf = (t2 == 0 ? mu : -mu) * d.idy;
+    long base = iv(d, I, m - 3, K);
+    for (int t = 0; t < 6; ++t) q6[t] = v[base + t * sy];
+    float mass = 0.5f * (q6[2] + q6[3]) * r;
+    float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+    scatter6(lv, base, sy, g6, lf * mass);
+    float gm = lf * R * 0.5f * r;
+    aadd(lv + base + 2 * sy, gm);
+    aadd(lv + base + 3 * sy, gm);
+  }
+  const float* col = v + iv(d, I, J, 0);
+  float* lcol = lv + iv(d, I, J, 0);
+  for (int t2 = 0; t2 < 2; ++t2) {
+    int fz = K - 1 + t2;
+    if (fz == 0) continue;
+    float lf = (t2 == 0 ? mu : -mu) * d.idz;
+    float rf = 0.5f * (rc[fz] + rc[fz + 1]);
+    float mass = 0.5f * (rf * w[iw(d, I, J - 1, fz + 1)] + rf * w[iw(d, I, J, fz + 1)]);
+    float R = vrecon_grad(col, d.nz + 2, fz, upwind_pos(mass), weps, g6);
+    scatter6z(lcol, fz, d.nz + 2, g6, lf * mass);
+    if (R != 0.f) {
+      float gm = lf * R * 0.5f * rf;
+      aadd(lw + iw(d, I, J - 1, fz + 1), gm);
+      aadd(lw + iw(d, I, J, fz + 1), gm);
+    }
+  }
+  float vc = v[iv(d, I, J, K)];
+  aadd(lv + iv(d, I, J, K), mu * 0.5f * (fd_at(d, rc, u, v, w, I, J - 1, K) + fd_at(d, rc, u, v, w, I, J, K)));
+  fd_vjp(d, rc, I, J - 1, K, mu * 0.5f * vc, lu, lv, lw);
+  fd_vjp(d, rc, I, J, K, mu * 0.5f * vc, lu, lv, lw);
+}
+
+// reverse of advection_w (advection.py:117-149), lam: (nx, ny, nz+1); levels 0 and nz
+// carry no advection
+__global__ void k_adv_w_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
+                            const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ lam,
+                            float* __restrict__ lu, float* __restrict__ lv, float* __restrict__ lw) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (l >= d.nz || l < 1 || j >= d.ny) return;
+  float L = lam[((long)i * d.ny + j) * (d.nz + 1) + l];
+  if (L == 0.f) return;
+  const float weps = 1.0e-16f;
+  int I = i + 3, J = j + 3, K = l + 1;
+  float mu = L / (0.5f * (rc[K - 1] + rc[K]));
+  const long sx = d.sxw(), sy = d.syw();
+  float q6[6], g6[6];
+  for (int t2 = 0; t2 < 2; ++t2) {
+    int m = I + t2;
+    float lf = (t2 == 0 ? mu : -mu) * d.idx;
+    float mass = 0.5f * (rc[K - 1] * u[iu(d, m, J, K - 1)] + rc[K] * u[iu(d, m, J, K)]);
+    long base = iw(d, m - 3, J, K);
+    for (int t = 0; t < 6; ++t) q6[t] = w[base + t * sx];
+    float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+    scatter6(lw, base, sx, g6, lf * mass);
+    aadd(lu + iu(d, m, J, K - 1), lf * R * 0.5f * rc[K - 1]);
+    aadd(lu + iu(d, m, J, K), lf * R * 0.5f * rc[K]);
+  }
+  for (int t2 = 0; t2 < 2; ++t2) {
+    int m = J + t2;
+    float lf = (t2 == 0 ? mu : -mu) * d.idy;
+    float mass = 0.5f * (rc[K - 1] * v[iv(d, I, m, K - 1)] + rc[K] * v[iv(d, I, m, K)]);
+    long base = iw(d, I, m - 3, K);
+    for (int t = 0; t < 6; ++t) q6[t] = w[base + t * sy];
+    float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+    scatter6(lw, base, sy, g6, lf * mass);
+    aadd(lv + iv(d, I, m, K - 1), lf * R * 0.5f * rc[K - 1]);
+    aadd(lv + iv(d, I, m, K), lf * R * 0.5f * rc[K]);
+  }
+  const long cb = iw(d, I, J, 0);
+  const float* col = w + cb;
+  float* lcol = lw + cb;
+  for (int t2 = 0; t2 < 2; ++t2) {
+    int cz = K - 1 + t2;  // scalar level: face between w[cz] and w[cz+1]
+    float lf = (t2 == 0 ? mu : -mu) * d.idz;
+    float mass = 0.5f * (col[cz] + col[cz + 1]) * rc[cz];
+    float R = vrecon_grad(col, d.nz + 3, cz, upwind_pos(mass), weps, g6);
+    scatter6z(lcol, cz, d.nz + 3, g6, lf * mass);
+    if (R != 0.f) {
+      float gm = lf * R * 0.5f * rc[cz];
+      aadd(lcol + cz, gm);
+      aadd(lcol + cz + 1, gm);
+    }
+  }
+  float wc = col[K];
+  aadd(lcol + K, mu * 0.5f * (fd_at(d, rc, u, v, w, I, J, K - 1) + fd_at(d, rc, u, v, w, I, J, K)));
+  fd_vjp(d, rc, I, J, K - 1, mu * 0.5f * wc, lu, lv, lw);
+  fd_vjp(d, rc, I, J, K, mu * 0.5f * wc, lu, lv, lw);
+}
+
+// ------------------------------------------------------------------ B1 buoyancy
+__device__ __forceinline__ void theta_rho_grad(float th, float qv, float& dth, float& dqv) {
+  float inv = 1.0f / (1.0f + qv);
+  dth = (1.0f + (float)kReps * qv) * inv;
+  dqv = th * ((float)kReps - 1.0f) * inv * inv;
+}
+
+// theta_rho over the padded array: lth += ltr * d/dth, lqv += ltr * d/dqv (elementwise)
+__global__ void k_theta_rho_vjp(long n, const float* __restrict__ th, const float* __restrict__ qv,
+                                const float* __restrict__ ltr, float* __restrict__ lth, float* __restrict__ lqv) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  float g = ltr[t];
+  if (g == 0.f) return;
+  float a, b;
+  theta_rho_grad(th[t], qv[t], a, b);
+  lth[t] += g * a;
+  lqv[t] += g * b;
+}
+
+// b8w (interpolate_scalar2w + extrapolate_bottom_top of calculate_buoyancy,
+// pressure_equations.py:123-172), lam on (nx, ny, nz+1) w levels -> theta, qv adjoints
+__global__ void k_b8w_vjp(Dims d, const float* __restrict__ th0c, const float* __restrict__ qv0c,
+                          const float* __restrict__ th, const float* __restrict__ qv, const float* __restrict__ lam,
+                          float* __restrict__ lth, float* __restrict__ lqv) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (l > d.nz || j >= d.ny) return;
+  float L = lam[((long)i * d.ny + j) * (d.nz + 1) + l];
+  if (L == 0.f) return;
+  long c = ic(d, i + 3, j + 3, 0);
+  int ks[3];
+  float cf[3];
+  int n;
+  if (l == 0) {
+    ks[0] = 0; ks[1] = 1; ks[2] = 2; cf[0] = 1.875f; cf[1] = -1.25f; cf[2] = 0.375f; n = 3;
+  } else if (l == d.nz) {
+    ks[0] = d.nz - 1; ks[1] = d.nz - 2; ks[2] = d.nz - 3; cf[0] = 1.875f; cf[1] = -1.25f; cf[2] = 0.375f; n = 3;
+  } else {
+    ks[0] = l - 1; ks[1] = l; cf[0] = 0.5f; cf[1] = 0.5f; cf[2] = 0.f; n = 2;
+  }
+  for (int q = 0; q < n; ++q) {
+    int K = ks[q] + 1;
+    float tr0 = th0c[K] * (1.0f + (float)kReps * qv0c[K]) / (1.0f + qv0c[K]);
+    float a, b;
+    theta_rho_grad(th[c + K], qv[c + K], a, b);
+    float gB = L * cf[q] * (float)kG / tr0;
+    aadd(lth + c + K, gB * a);
+    aadd(lqv + c + K, gB * b);
+  }
+}
+
+// ------------------------------------------------------------- S1 acoustic substep
+// Reverse of k_acoustic in two passes.  Pass 1 (per cell) takes the folded adjoint of
+// the new pi' through the pressure tendency (one_step_integration.py:543-546) and
+// scatters it to the six new face velocities (lU, lV, lW, interior face arrays), to pi*
+// and to pi'.
+__global__ void k_acoustic_vjp_cells(Dims d, const float* __restrict__ rc, const float* __restrict__ pi0c,
+                                     const float* __restrict__ u, const float* __restrict__ v,
+                                     const float* __restrict__ w, const float* __restrict__ p,
+                                     const float* __restrict__ ps, const float* __restrict__ ru,
+                                     const float* __restrict__ rv, const float* __restrict__ rw,
+                                     const float* __restrict__ tr, const float* __restrict__ lpo, float dts,
+                                     float* __restrict__ lU, float* __restrict__ lV, float* __restrict__ lW,
+                                     float* __restrict__ lp, float* __restrict__ lps) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  int I = i + 3, J = j + 3, K = k + 1;
+  long c = ic(d, I, J, K);
+  float Lp = lpo[c];
+  if (Lp == 0.f) return;
+  const long sx = d.sxc(), sy = d.syc();
+  float un0 = u[iu(d, I, J, K)] + (pgf_x(d, p, tr, I, J, K) + ru[((long)i * d.ny + j) * d.nz + k]) * dts;
+  float un1 = u[iu(d, I + 1, J, K)] + (pgf_x(d, p, tr, I + 1, J, K) + ru[((long)(i + 1) * d.ny + j) * d.nz + k]) * dts;
+  float vn0 = v[iv(d, I, J, K)] + (pgf_y(d, p, tr, I, J, K) + rv[((long)i * (d.ny + 1) + j) * d.nz + k]) * dts;
+  float vn1 = v[iv(d, I, J + 1, K)] + (pgf_y(d, p, tr, I, J + 1, K) + rv[((long)i * (d.ny + 1) + j + 1) * d.nz + k]) * dts;
+  float wn0 = (k == 0) ? 0.0f : w[iw(d, I, J, K)] + (pgf_z(d, p, tr, I, J, K) + rw[((long)i * d.ny + j) * (d.nz + 1) + k]) * dts;
+  float wn1 = (k == d.nz - 1)
+                  ? 0.0f
+                  : w[iw(d, I, J, K + 1)] + (pgf_z(d, p, tr, I, J, K + 1) + rw[((long)i * d.ny + j) * (d.nz + 1) + k + 1]) * dts;
+  float div = (un1 - un0) * d.idx + (vn1 - vn0) * d.idy + (wn1 - wn0) * d.idz;
+  float pc = ps[c];
+  float piadv = pi0c[K] + pc;
+  const float cr = (float)(-kRd / kCv);
+  aadd(lp + c, Lp);
+  float G = 1.1f * dts * Lp;  // adjoint of the pressure tendency
+  float ldiv = cr * piadv * G;
+  float lpc = cr * div * G;
+  // advp = -(0.5 (udx1 + udx0) + 0.5 (vdy1 + vdy0) + (0.5 wdz0 r8w0 + 0.5 wdz1 r8w1) / rho0)
+  float gx = -0.5f * G * d.idx, gy = -0.5f * G * d.idy;
+  float lun0 = -ldiv * d.idx + gx * (pc - ps[c - sx]);
+  float lun1 = ldiv * d.idx + gx * (ps[c + sx] - pc);
+  lpc += gx * un0 - gx * un1;
+  aadd(lps + c - sx, -gx * un0);
+  aadd(lps + c + sx, gx * un1);
+  float lvn0 = -ldiv * d.idy + gy * (pc - ps[c - sy]);
+  float lvn1 = ldiv * d.idy + gy * (ps[c + sy] - pc);
+  lpc += gy * vn0 - gy * vn1;
+  aadd(lps + c - sy, -gy * vn0);
+  aadd(lps + c + sy, gy * vn1);
+  float lwn0 = 0.f, lwn1 = 0.f;
+  if (k > 0) {
+    float r8w0 = 0.5f * (rc[K - 1] + rc[K]);
+    float gz = -0.5f * r8w0 / rc[K] * G;
+    float dz0 = (pi0c[K] - pi0c[K - 1]) * d.idz + (pc - ps[c - 1]) * d.idz;
+    lwn0 = -ldiv * d.idz + gz * dz0;
+    lpc += gz * wn0 * d.idz;
+    aadd(lps + c - 1, -gz * wn0 * d.idz);
+  }
+  if (k < d.nz - 1) {
+    float r8w1 = 0.5f * (rc[K] + rc[K + 1]);
+    float gz = -0.5f * r8w1 / rc[K] * G;
+    float dz1 = (pi0c[K + 1] - pi0c[K]) * d.idz + (ps[c + 1] - pc) * d.idz;
+    lwn1 = ldiv * d.idz + gz * dz1;
+    lpc -= gz * wn1 * d.idz;
+    aadd(lps + c + 1, gz * wn1 * d.idz);
+  }
+  aadd(lps + c, lpc);
+  aadd(lU + ((long)i * d.ny + j) * d.nz + k, lun0);
+  aadd(lU + ((long)(i + 1) * d.ny + j) * d.nz + k, lun1);
+  aadd(lV + ((long)i * (d.ny + 1) + j) * d.nz + k, lvn0);
+  aadd(lV + ((long)i * (d.ny + 1) + j + 1) * d.nz + k, lvn1);
+  if (k > 0) aadd(lW + ((long)i * d.ny + j) * (d.nz + 1) + k, lwn0);
+  if (k < d.nz - 1) aadd(lW + ((long)i * d.ny + j) * (d.nz + 1) + k + 1, lwn1);
+}
+
+// pgf = -cp/2 (tr[c] + tr[c-sd]) (p[c] - p[c-sd]) id, weight g
+__device__ __forceinline__ void pgf_vjp(float* __restrict__ lp, float* __restrict__ ltr, const float* __restrict__ p,
+                                        const float* __restrict__ tr, long c, long sd, float id, float g) {
+  const float h = (float)(-kCp * 0.5);
+  float dp = (p[c] - p[c - sd]) * id;
+  float ts = tr[c] + tr[c - sd];
+  aadd(ltr + c, g * h * dp);
+  aadd(ltr + c - sd, g * h * dp);
+  aadd(lp + c, g * h * ts * id);
+  aadd(lp + c - sd, -g * h * ts * id);
+}
+
+// Pass 2 (per face): the total adjoint of each new face velocity (its folded output
+// adjoint + pass 1's contribution) goes to the old velocity, the rhs and the PGF inputs.
+// Levels 0 and nz of w are constant zero after set_w_bc and take no adjoint.
+__global__ void k_acoustic_vjp_faces(Dims d, const float* __restrict__ p, const float* __restrict__ tr,
+                                     const float* __restrict__ luo, const float* __restrict__ lvo,
+                                     const float* __restrict__ lwo, const float* __restrict__ lU,
+                                     const float* __restrict__ lV, const float* __restrict__ lW, float dts,
+                                     float* __restrict__ lu, float* __restrict__ lv, float* __restrict__ lw,
+                                     float* __restrict__ lp, float* __restrict__ ltr, float* __restrict__ lru,
+                                     float* __restrict__ lrv, float* __restrict__ lrw) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nz
+  int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
+  int i = blockIdx.z;                             // 0..nx
+  if (k > d.nz || j > d.ny) return;
+  int I = i + 3, J = j + 3, K = k + 1;
+  if (k < d.nz && j < d.ny) {
+    long o = ((long)i * d.ny + j) * d.nz + k;
+    long a = iu(d, I, J, K);
+    float g = lU[o] + luo[a];
+    if (g != 0.f) {
+      aadd(lu + a, g);
+      lru[o] += g * dts;
+      pgf_vjp(lp, ltr, p, tr, ic(d, I, J, K), d.sxc(), d.idx, g * dts);
+    }
+  }
+  if (k < d.nz && i < d.nx) {
+    long o = ((long)i * (d.ny + 1) + j) * d.nz + k;
+    long a = iv(d, I, J, K);
+    float g = lV[o] + lvo[a];
+    if (g != 0.f) {
+      aadd(lv + a, g);
+      lrv[o] += g * dts;
+      pgf_vjp(lp, ltr, p, tr, ic(d, I, J, K), d.syc(), d.idy, g * dts);
+    }
+  }
+  if (i < d.nx && j < d.ny && k > 0 && k < d.nz) {
+    long o = ((long)i * d.ny + j) * (d.nz + 1) + k;
+    long a = iw(d, I, J, K);
+    float g = lW[o] + lwo[a];
+    if (g != 0.f) {
+      aadd(lw + a, g);
+      lrw[o] += g * dts;
+      pgf_vjp(lp, ltr, p, tr, ic(d, I, J, K), 1, d.idz, g * dts);
+    }
+  }
+}
+
+// ------------------------------------------------------------- G1 Smagorinsky
+// SGS scalar diffusion (turbulence.py:80-109): lam (nx, ny, nz) -> kh, s adjoints
+__global__ void k_smag_scalar_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ kh,
+                                  const float* __restrict__ s, const float* __restrict__ lam, float* __restrict__ lkh,
+                                  float* __restrict__ ls) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  float L = lam[((long)i * d.ny + j) * d.nz + k];
+  if (L == 0.f) return;
+  const long sx = d.sxc(), sy = d.syc();
+  int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
+  long c = ic(d, I, J, K);
+  float r = rc[K];
+  float ldiv = -L / r;
+  float r8u = 0.5f * (r + r);
+  // flux t(cc) = -r8 * 0.5 (kh[cc] + kh[cc-sd]) * (s[cc] - s[cc-sd]) * id, weight g
+  auto tb = [&](long cc, long sd, float id, float r8, float g) {
+    float ds = (s[cc] - s[cc - sd]) * id;
+    float kk = 0.5f * (kh[cc] + kh[cc - sd]);
+    float gk = -g * r8 * 0.5f * ds;
+    aadd(lkh + cc, gk);
+    aadd(lkh + cc - sd, gk);
+    float gs = -g * r8 * kk * id;
+    aadd(ls + cc, gs);
+    aadd(ls + cc - sd, -gs);
+  };
+  tb(c + sx, sx, d.idx, r8u, ldiv * d.idx);
+  tb(c, sx, d.idx, r8u, -ldiv * d.idx);
+  tb(c + sy, sy, d.idy, r8u, ldiv * d.idy);
+  tb(c, sy, d.idy, r8u, -ldiv * d.idy);
+  // tz(l), l = w level between padded levels l and l+1 (cells cl+1 over cl), zero at l = 0, nz
+  long col = c - K;
+  if (k + 1 < nz) tb(col + k + 2, 1, d.idz, 0.5f * (rc[k + 1] + rc[k + 2]), ldiv * d.idz);
+  if (k > 0) tb(col + k + 1, 1, d.idz, 0.5f * (rc[k] + rc[k + 1]), -ldiv * d.idz);
+}
+
+// tau = -2 km s at padded index c, weight g
+__device__ __forceinline__ void tau_vjp(const float* __restrict__ km, const float* __restrict__ sij,
+                                        float* __restrict__ lkm, float* __restrict__ lsij, long c, float g) {
+  if (g == 0.f) return;
+  aadd(lkm + c, -2.0f * g * sij[c]);
+  aadd(lsij + c, -2.0f * g * km[c]);
+}
+
+// SGS momentum tendencies (turbulence.py:9-79): lam on u, v, w points -> km, s_ij adjoints
+__global__ void k_smag_uvw_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ km,
+                               const float* __restrict__ s11, const float* __restrict__ s22,
+                               const float* __restrict__ s33, const float* __restrict__ s12,
+                               const float* __restrict__ s13, const float* __restrict__ s23,
+                               const float* __restrict__ lsu, const float* __restrict__ lsv,
+                               const float* __restrict__ lsw, float* __restrict__ lkm, float* __restrict__ l11,
+                               float* __restrict__ l22, float* __restrict__ l33, float* __restrict__ l12,
+                               float* __restrict__ l13, float* __restrict__ l23) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nz
+  int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
+  int i = blockIdx.z;                             // 0..nx
+  if (k > d.nz || j > d.ny) return;
+  const long sx = d.sxc(), sy = d.syc();
+  int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
+  // interpolate_scalar2w_0 of tau at w level l: 0.5 (tau[col+l] + tau[col+l+1]), l in 1..nz-1
+  auto tw_b = [&](long col, int l, float g, const float* sij, float* lsij) {
+    if (l <= 0 || l >= nz) return;
+    tau_vjp(km, sij, lkm, lsij, col + l, 0.5f * g);
+    tau_vjp(km, sij, lkm, lsij, col + l + 1, 0.5f * g);
+  };
+  if (k < nz && j < d.ny) {
+    float L = lsu[((long)i * d.ny + j) * nz + k];
+    if (L != 0.f) {
+      long c = ic(d, I, J, K);
+      float r = rc[K];
+      float g = -L / (0.5f * (r + r));  // adjoint of d11 + d12 + d13
+      tau_vjp(km, s11, lkm, l11, c, g * d.idx);
+      tau_vjp(km, s11, lkm, l11, c - sx, -g * d.idx);
+      const long cs[4] = {c - sx, c + sy, c - sx + sy, c};
+      for (int q = 0; q < 4; ++q) {
+        tau_vjp(km, s12, lkm, l12, cs[q], 0.25f * g * d.idy);
+        tau_vjp(km, s12, lkm, l12, cs[q] - sy, -0.25f * g * d.idy);
+      }
+      const long cz[2] = {c - sx, c};
+      for (int q = 0; q < 2; ++q) {
+        long col = cz[q] - K;
+        tw_b(col, k + 1, 0.5f * g * d.idz, s13, l13);
+        tw_b(col, k, -0.5f * g * d.idz, s13, l13);
+      }
+    }
+  }
+  if (k < nz && i < d.nx) {
+    float L = lsv[((long)i * (d.ny + 1) + j) * nz + k];
+    if (L != 0.f) {
+      long c = ic(d, I, J, K);
+      float r = rc[K];
+      float g = -L / (0.5f * (r + r));
+      tau_vjp(km, s22, lkm, l22, c, g * d.idy);
+      tau_vjp(km, s22, lkm, l22, c - sy, -g * d.idy);
+      const long cs[4] = {c - sy, c + sx, c, c + sx - sy};
+      for (int q = 0; q < 4; ++q) {
+        tau_vjp(km, s12, lkm, l12, cs[q], 0.25f * g * d.idx);
+        tau_vjp(km, s12, lkm, l12, cs[q] - sx, -0.25f * g * d.idx);
+      }
+      const long cz[2] = {c - sy, c};
+      for (int q = 0; q < 2; ++q) {
+        long col = cz[q] - K;
+        tw_b(col, k + 1, 0.5f * g * d.idz, s23, l23);
+        tw_b(col, k, -0.5f * g * d.idz, s23, l23);
+      }
+    }
+  }
+  if (i < d.nx && j < d.ny && k > 0 && k < nz) {
+    float L = lsw[((long)i * d.ny + j) * (nz + 1) + k];
+    if (L != 0.f) {
+      int l = k;
+      long cu = ic(d, I, J, l + 1), cl = ic(d, I, J, l);
+      float g = -L / (0.5f * (rc[l] + rc[l + 1]));
+      tau_vjp(km, s33, lkm, l33, cu, g * d.idz);
+      tau_vjp(km, s33, lkm, l33, cl, -g * d.idz);
+      float gx = 0.5f * g / (2.0f * d.dx), gy = 0.5f * g / (2.0f * d.dy);
+      const long cc[2] = {cl, cu};
+      for (int q = 0; q < 2; ++q) {
+        tau_vjp(km, s13, lkm, l13, cc[q] + sx, gx);
+        tau_vjp(km, s13, lkm, l13, cc[q] - sx, -gx);
+        tau_vjp(km, s23, lkm, l23, cc[q] + sy, gy);
+        tau_vjp(km, s23, lkm, l23, cc[q] - sy, -gy);
+      }
+    }
+  }
+}
+
+// compute_k (turbulence.py:430-446), elementwise over the padded array:
+// km = cg2 sqrt(max(s2 - n2/Pr, 0)), kh = km/Pr.  Writes ls2, ln2 (not accumulated) and
+// accumulates dL/dcg2 = sum(dL/dkm_total * sqrt(x)) into *lcg2 (FP64).
+__global__ void k_smag_k_vjp(long n, const float* __restrict__ s2, const float* __restrict__ n2,
+                             const float* __restrict__ lkm, const float* __restrict__ lkh, float csgs2, float prandtl,
+                             float* __restrict__ ls2, float* __restrict__ ln2, double* __restrict__ lcg2) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  float acc = 0.f;
+  if (t < n) {
+    float x = s2[t] - n2[t] / prandtl;
+    float G = lkm[t] + lkh[t] / prandtl;
+    float a = 0.f, b = 0.f;
+    if (x > 0.0f && G != 0.f) {
+      float sq = sqrtf(x);
+      float gx = G * csgs2 * 0.5f / sq;
+      a = gx;
+      b = -gx / prandtl;
+      acc = G * sq;
+    }
+    ls2[t] = a;
+    ln2[t] = b;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0 && acc != 0.f) unsafeAtomicAdd(lcg2, (double)acc);
+}
+
+// compute_nm (turbulence.py:391-427): folded adjoint at interior cells -> theta, qv
+__global__ void k_smag_nm_vjp(Dims d, const float* __restrict__ th, const float* __restrict__ qv,
+                              const float* __restrict__ ln2, float* __restrict__ lth, float* __restrict__ lqv) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  long c = ic(d, i + 3, j + 3, 0);
+  float L = ln2[c + k + 1];
+  if (L == 0.f) return;
+  int nz = d.nz;
+  int ls[3];
+  float cf[3];
+  int n;
+  if (k == 0) {
+    ls[0] = 1; ls[1] = 2; ls[2] = 3; cf[0] = 1.875f; cf[1] = -1.25f; cf[2] = 0.375f; n = 3;
+  } else if (k == nz - 1) {
+    ls[0] = nz - 1; ls[1] = nz - 2; ls[2] = nz - 3; cf[0] = 1.875f; cf[1] = -1.25f; cf[2] = 0.375f; n = 3;
+  } else {
+    ls[0] = k; ls[1] = k + 1; cf[0] = 0.5f; cf[1] = 0.5f; cf[2] = 0.f; n = 2;
+  }
+  const float em = (float)kRepsm1;
+  for (int q = 0; q < n; ++q) {
+    int l = ls[q];
+    // N^2 at w level l = g/dz * log(a/b), a at padded level l+1, b at level l
+    float g = L * cf[q] * (float)kG / d.dz;
+    long ca = c + l + 1, cb = c + l;
+    float fa = 1.0f + em * qv[ca], fb = 1.0f + em * qv[cb];
+    float ga = g / (th[ca] * fa), gb = -g / (th[cb] * fb);
+    aadd(lth + ca, ga * fa);
+    aadd(lqv + ca, ga * th[ca] * em);
+    aadd(lth + cb, gb * fb);
+    aadd(lqv + cb, gb * th[cb] * em);
+  }
+}
+
+// compute_deformation (turbulence.py:298-388): folded adjoints of rho*S_ij and of the
+// deformation at interior cells -> u, v, w adjoints
+__device__ __forceinline__ void e12_vjp(const Dims& d, float* lu, float* lv, int F, int G, int K, float g) {
+  // e12 = 0.5 ((u[F,G] - u[F,G-1]) idy + (v[F,G] - v[F-1,G]) idx)
+  aadd(lu + iu(d, F, G, K), 0.5f * g * d.idy);
+  aadd(lu + iu(d, F, G - 1, K), -0.5f * g * d.idy);
+  aadd(lv + iv(d, F, G, K), 0.5f * g * d.idx);
+  aadd(lv + iv(d, F - 1, G, K), -0.5f * g * d.idx);
+}
+__device__ __forceinline__ void e13_vjp(const Dims& d, float* lu, float* lw, int F, int J, int l, float g) {
+  // e13 = 0.5 ((u[F,J,l+1] - u[F,J,l]) idz + (w[F,J,l+1] - w[F-1,J,l+1]) idx)
+  aadd(lu + iu(d, F, J, l + 1), 0.5f * g * d.idz);
+  aadd(lu + iu(d, F, J, l), -0.5f * g * d.idz);
+  aadd(lw + iw(d, F, J, l + 1), 0.5f * g * d.idx);
+  aadd(lw + iw(d, F - 1, J, l + 1), -0.5f * g * d.idx);
+}
+__device__ __forceinline__ void e23_vjp(const Dims& d, float* lv, float* lw, int I, int G, int l, float g) {
+  aadd(lv + iv(d, I, G, l + 1), 0.5f * g * d.idz);
+  aadd(lv + iv(d, I, G, l), -0.5f * g * d.idz);
+  aadd(lw + iw(d, I, G, l + 1), 0.5f * g * d.idy);
+  aadd(lw + iw(d, I, G - 1, l + 1), -0.5f * g * d.idy);
+}
+
+__global__ void k_smag_deform_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ s11,
+                                  const float* __restrict__ s22, const float* __restrict__ s33,
+                                  const float* __restrict__ s12, const float* __restrict__ s13,
+                                  const float* __restrict__ s23, const float* __restrict__ l11,
+                                  const float* __restrict__ l22, const float* __restrict__ l33,
+                                  const float* __restrict__ l12, const float* __restrict__ l13,
+                                  const float* __restrict__ l23, const float* __restrict__ ldf,
+                                  float* __restrict__ lu, float* __restrict__ lv, float* __restrict__ lw) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
+  long c = ic(d, I, J, K);
+  float r = rc[K];
+  float q = ldf[c] / (r * r);
+  // deformation = (2 (a11^2 + a22^2 + a33^2) + 4 (a12^2 + a13^2 + a23^2)) / r^2
+  float g11 = l11[c] + 4.0f * q * s11[c];
+  float g22 = l22[c] + 4.0f * q * s22[c];
+  float g33 = l33[c] + 4.0f * q * s33[c];
+  float g12 = l12[c] + 8.0f * q * s12[c];
+  float g13 = l13[c] + 8.0f * q * s13[c];
+  float g23 = l23[c] + 8.0f * q * s23[c];
+  aadd(lu + iu(d, I + 1, J, K), g11 * r * d.idx);
+  aadd(lu + iu(d, I, J, K), -g11 * r * d.idx);
+  aadd(lv + iv(d, I, J + 1, K), g22 * r * d.idy);
+  aadd(lv + iv(d, I, J, K), -g22 * r * d.idy);
+  aadd(lw + iw(d, I, J, K + 1), g33 * r * d.idz);
+  aadd(lw + iw(d, I, J, K), -g33 * r * d.idz);
+  float h12 = 0.25f * r * g12;
+  e12_vjp(d, lu, lv, I, J, K, h12);
+  e12_vjp(d, lu, lv, I + 1, J + 1, K, h12);
+  e12_vjp(d, lu, lv, I, J + 1, K, h12);
+  e12_vjp(d, lu, lv, I + 1, J, K, h12);
+  float h13 = r * g13, h23 = r * g23;
+  if (k == 0 || k == nz - 1) {
+    int l0 = (k == 0) ? 1 : nz - 1, dl = (k == 0) ? 1 : -1;
+    const float cf[3] = {1.875f, -1.25f, 0.375f};
+    for (int t = 0; t < 3; ++t) {
+      int l = l0 + t * dl;
+      e13_vjp(d, lu, lw, I + 1, J, l, 0.5f * cf[t] * h13);
+      e13_vjp(d, lu, lw, I, J, l, 0.5f * cf[t] * h13);
+      e23_vjp(d, lv, lw, I, J + 1, l, 0.5f * cf[t] * h23);
+      e23_vjp(d, lv, lw, I, J, l, 0.5f * cf[t] * h23);
+    }
+  } else {
+    float a = 0.25f * h13, b = 0.25f * h23;
+    e13_vjp(d, lu, lw, I, J, k, a);
+    e13_vjp(d, lu, lw, I + 1, J, k + 1, a);
+    e13_vjp(d, lu, lw, I, J, k + 1, a);
+    e13_vjp(d, lu, lw, I + 1, J, k, a);
+    e23_vjp(d, lv, lw, I, J, k, b);
+    e23_vjp(d, lv, lw, I, J + 1, k + 1, b);
+    e23_vjp(d, lv, lw, I, J, k + 1, b);
+    e23_vjp(d, lv, lw, I, J + 1, k, b);
+  }
+}
+
+// ------------------------------------------------------------------ H1 halo folds
+// Reverse of the padded stores: each ghost's adjoint is added to its interior source,
+// then cleared.  One thread per interior point; every ghost has exactly one source.
+// mode 0 scalar, 1 u, 2 v, 3 w.
+__global__ void k_fold(Dims d, int mode, float* __restrict__ a) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  int ni = d.nx + (mode == 1 ? 1 : 0), nj = d.ny + (mode == 2 ? 1 : 0), nk = d.nz + (mode == 3 ? 1 : 0);
+  if (k >= nk || j >= nj || i >= ni) return;
+  int xs[3], ys[3], zs[3];
+  int nxs, nys, nzs = 1;
+  if (mode == 1) ghost_list_face(i, d.nx, xs, nxs); else ghost_list_x(i, d.nx, xs, nxs);
+  if (mode == 2) ghost_list_face(j, d.ny, ys, nys); else ghost_list_x(j, d.ny, ys, nys);
+  zs[0] = k + 1;
+  if (k == 0) zs[nzs++] = 0;
+  if (k == nk - 1) zs[nzs++] = nk + 1;
+  auto idx = [&](int X, int Y, int Z) -> long {
+    if (mode == 0) return ic(d, X, Y, Z);
+    if (mode == 1) return iu(d, X, Y, Z);
+    if (mode == 2) return iv(d, X, Y, Z);
+    return iw(d, X, Y, Z);
+  };
+  float sum = 0.f;
+  for (int a1 = 0; a1 < nxs; ++a1)
+    for (int b1 = 0; b1 < nys; ++b1)
+      for (int c1 = 0; c1 < nzs; ++c1) {
+        if (a1 == 0 && b1 == 0 && c1 == 0) continue;
+        long g = idx(xs[a1], ys[b1], zs[c1]);
+        sum += a[g];
+        a[g] = 0.f;
+      }
+  a[idx(xs[0], ys[0], zs[0])] += sum;
+}
+
+// ------------------------------------------------------------- E1 scalar update
+// next = pad(s0 + dt T + dt sgs) (one_step_integration.py:450-465, :158-162):
+// ls0 += lnext, lT = dt lnext, lsgs += dt lnext (lnext folded; interior cells).
+__global__ void k_scalar_update_vjp(Dims d, const float* __restrict__ lnext, float dt, float* __restrict__ ls0,
+                                    float* __restrict__ lT, float* __restrict__ lsgs) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  long c = ic(d, i + 3, j + 3, k + 1);
+  long o = ((long)i * d.ny + j) * d.nz + k;
+  float g = lnext[c];
+  ls0[c] += g;
+  lT[o] = g * dt;
+  if (lsgs) lsgs[o] += g * dt;
+}
+
+__global__ void k_add_to(long n, const float* __restrict__ a, float* __restrict__ b) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] += a[t];
+}
+
+__global__ void k_fill(long n, float* __restrict__ a, float v) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
+}
+
+}  // namespace lex

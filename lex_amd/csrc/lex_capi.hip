@@ -12,6 +12,9 @@
 #include "lex_smag.hip"
 #include "lex_pressure.hip"
 #include "lex_stage.hip"
+#include "lex_adjoint_step.hip"
+#include "lex_conv.hip"
+#include "lex_sgs_io.hip"
 #include "../../include/lex_capi.h"
 
 using namespace lex;
@@ -460,6 +463,364 @@ int lex_rk3_step(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_c
   if (rc < 0) return fail(rc, "lex_rk3_step: unsupported configuration%s");
   if (rc > 0) return fail(rc, "lex_rk3_step: HIP error %s", hipGetErrorString((hipError_t)rc));
   return 0;
+}
+
+// ------------------------------------------------------------------ adjoint (VJP)
+int lex_fold(const lex_grid_t* g, int mode, float* a, void* stream) {
+  REQGRID(g);
+  REQ(a);
+  if (mode < 0 || mode > 3) return fail(-1, "lex_fold: mode must be 0..3%s");
+  Dims d = mkdims(g);
+  int ni = d.nx + (mode == 1), nj = d.ny + (mode == 2), nk = d.nz + (mode == 3);
+  hipLaunchKernelGGL(k_fold, grd(nk, nj, ni), blk(), 0, S(stream), d, mode, a);
+  return check_launch("lex_fold");
+}
+
+int lex_adv_scalar_vjp(const lex_grid_t* g, const float* r, const float* s, const float* u, const float* v,
+                       const float* w, const float* lam, float weps, float* ls, float* lu, float* lv, float* lw,
+                       void* stream) {
+  REQGRID(g);
+  REQ(r); REQ(s); REQ(u); REQ(v); REQ(w); REQ(lam); REQ(ls); REQ(lu); REQ(lv); REQ(lw);
+  Dims d = mkdims(g);
+  hipLaunchKernelGGL(k_adv_scalar_vjp, grd(d.nz, d.ny, d.nx), blk(), 0, S(stream), d, r, s, u, v, w, lam, weps, ls,
+                     lu, lv, lw);
+  return check_launch("lex_adv_scalar_vjp");
+}
+
+int lex_adv_u_vjp(const lex_grid_t* g, const float* r, const float* u, const float* v, const float* w,
+                  const float* lam, float* lu, float* lv, float* lw, void* stream) {
+  REQGRID(g);
+  REQ(r); REQ(u); REQ(v); REQ(w); REQ(lam); REQ(lu); REQ(lv); REQ(lw);
+  Dims d = mkdims(g);
+  hipLaunchKernelGGL(k_adv_u_vjp, grd(d.nz, d.ny, d.nx + 1), blk(), 0, S(stream), d, r, u, v, w, lam, lu, lv, lw);
+  return check_launch("lex_adv_u_vjp");
+}
+
+int lex_adv_v_vjp(const lex_grid_t* g, const float* r, const float* u, const float* v, const float* w,
+                  const float* lam, float* lu, float* lv, float* lw, void* stream) {
+  REQGRID(g);
+  REQ(r); REQ(u); REQ(v); REQ(w); REQ(lam); REQ(lu); REQ(lv); REQ(lw);
+  Dims d = mkdims(g);
+  hipLaunchKernelGGL(k_adv_v_vjp, grd(d.nz, d.ny + 1, d.nx), blk(), 0, S(stream), d, r, u, v, w, lam, lu, lv, lw);
+  return check_launch("lex_adv_v_vjp");
+}
+
+int lex_adv_w_vjp(const lex_grid_t* g, const float* r, const float* u, const float* v, const float* w,
+                  const float* lam, float* lu, float* lv, float* lw, void* stream) {
+  REQGRID(g);
+  REQ(r); REQ(u); REQ(v); REQ(w); REQ(lam); REQ(lu); REQ(lv); REQ(lw);
+  Dims d = mkdims(g);
+  hipLaunchKernelGGL(k_adv_w_vjp, grd(d.nz + 1, d.ny, d.nx), blk(), 0, S(stream), d, r, u, v, w, lam, lu, lv, lw);
+  return check_launch("lex_adv_w_vjp");
+}
+
+int lex_buoyancy_vjp(const lex_grid_t* g, const float* th0, const float* qv0, const float* th, const float* qv,
+                     const float* lam_b8w, float* lth, float* lqv, void* stream) {
+  REQGRID(g);
+  REQ(th0); REQ(qv0); REQ(th); REQ(qv); REQ(lam_b8w); REQ(lth); REQ(lqv);
+  Dims d = mkdims(g);
+  hipLaunchKernelGGL(k_b8w_vjp, grd(d.nz + 1, d.ny, d.nx), blk(), 0, S(stream), d, th0, qv0, th, qv, lam_b8w, lth, lqv);
+  return check_launch("lex_buoyancy_vjp");
+}
+
+int lex_theta_rho_vjp(long n, const float* th, const float* qv, const float* ltr, float* lth, float* lqv,
+                      void* stream) {
+  REQ(th); REQ(qv); REQ(ltr); REQ(lth); REQ(lqv);
+  if (n <= 0) return fail(-1, "lex_theta_rho_vjp: n <= 0%s");
+  hipLaunchKernelGGL(k_theta_rho_vjp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), n, th, qv, ltr, lth,
+                     lqv);
+  return check_launch("lex_theta_rho_vjp");
+}
+
+int lex_acoustic_substep_vjp(const lex_grid_t* g, const float* r, const float* pi0, const float* u, const float* v,
+                             const float* w, const float* p, const float* ps, const float* ru, const float* rv,
+                             const float* rw, const float* tr, float dts, float* luo, float* lvo, float* lwo,
+                             float* lpo, float* lU, float* lV, float* lW, float* lu, float* lv, float* lw, float* lp,
+                             float* lps, float* ltr, float* lru, float* lrv, float* lrw, void* stream) {
+  REQGRID(g);
+  REQ(r); REQ(pi0); REQ(u); REQ(v); REQ(w); REQ(p); REQ(ps); REQ(ru); REQ(rv); REQ(rw); REQ(tr);
+  REQ(luo); REQ(lvo); REQ(lwo); REQ(lpo); REQ(lU); REQ(lV); REQ(lW);
+  REQ(lu); REQ(lv); REQ(lw); REQ(lp); REQ(lps); REQ(ltr); REQ(lru); REQ(lrv); REQ(lrw);
+  if (lu == luo || lv == lvo || lw == lwo || lp == lpo)
+    return fail(-1, "lex_acoustic_substep_vjp: input adjoints must not alias output adjoints%s");
+  Dims d = mkdims(g);
+  hipStream_t st = S(stream);
+  hipLaunchKernelGGL(k_fold, grd(d.nz, d.ny, d.nx + 1), blk(), 0, st, d, 1, luo);
+  hipLaunchKernelGGL(k_fold, grd(d.nz, d.ny + 1, d.nx), blk(), 0, st, d, 2, lvo);
+  hipLaunchKernelGGL(k_fold, grd(d.nz + 1, d.ny, d.nx), blk(), 0, st, d, 3, lwo);
+  hipLaunchKernelGGL(k_fold, grd(d.nz, d.ny, d.nx), blk(), 0, st, d, 0, lpo);
+  const size_t nU = (size_t)(d.nx + 1) * d.ny * d.nz, nV = (size_t)d.nx * (d.ny + 1) * d.nz,
+               nW = (size_t)d.nx * d.ny * (d.nz + 1);
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((nU + 255) / 256)), dim3(256), 0, st, (long)nU, lU, 0.0f);
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((nV + 255) / 256)), dim3(256), 0, st, (long)nV, lV, 0.0f);
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((nW + 255) / 256)), dim3(256), 0, st, (long)nW, lW, 0.0f);
+  hipLaunchKernelGGL(k_acoustic_vjp_cells, grd(d.nz, d.ny, d.nx), blk(), 0, st, d, r, pi0, u, v, w, p, ps, ru, rv, rw,
+                     tr, (const float*)lpo, dts, lU, lV, lW, lp, lps);
+  hipLaunchKernelGGL(k_acoustic_vjp_faces, grd(d.nz + 1, d.ny + 1, d.nx + 1), blk(), 0, st, d, p, tr,
+                     (const float*)luo, (const float*)lvo, (const float*)lwo, (const float*)lU, (const float*)lV,
+                     (const float*)lW, dts, lu, lv, lw, lp, ltr, lru, lrv, lrw);
+  return check_launch("lex_acoustic_substep_vjp");
+}
+
+size_t lex_smag_vjp_workspace_bytes(const lex_grid_t* g) {
+  if (!grid_ok(g)) return 0;
+  Dims d = mkdims(g);
+  return 20 * al256((size_t)d.nc() * 4) + 256;
+}
+
+int lex_smag_vjp(const lex_grid_t* g, const float* r, const float* u, const float* v, const float* w, const float* th,
+                 const float* qv, int n_scalars, const float* const* scalars, float cs, float prandtl,
+                 const float* lsu, const float* lsv, const float* lsw, const float* const* lss, float* lu, float* lv,
+                 float* lw, float* lth, float* lqv, float* const* lscalars, double* lcs, void* ws, size_t ws_bytes,
+                 void* stream) {
+  REQGRID(g);
+  REQ(r); REQ(u); REQ(v); REQ(w); REQ(th); REQ(qv); REQ(lsu); REQ(lsv); REQ(lsw); REQ(lu); REQ(lv); REQ(lw);
+  REQ(lth); REQ(lqv); REQ(ws);
+  if (n_scalars < 0 || n_scalars > 10) return fail(-1, "lex_smag_vjp: n_scalars must be 0..10%s");
+  if (n_scalars > 0 && (!scalars || !lss || !lscalars)) return fail(-2, "lex_smag_vjp: null scalar lists%s");
+  if (ws_bytes < lex_smag_vjp_workspace_bytes(g)) return fail(-3, "lex_smag_vjp: workspace too small%s");
+  Dims d = mkdims(g);
+  hipStream_t st = S(stream);
+  const size_t C = al256((size_t)d.nc() * 4);
+  float* sm[10];
+  float* lsm[10];
+  for (int q = 0; q < 10; ++q) {
+    sm[q] = (float*)((char*)ws + q * C);
+    lsm[q] = (float*)((char*)ws + (10 + q) * C);
+  }
+  double* lcg2 = (double*)((char*)ws + 20 * C);
+  long n = (long)d.nc();
+  dim3 e1((unsigned)((n + 255) / 256)), e2(256);
+  hipLaunchKernelGGL(k_smag_deform, grd(d.nz, d.ny, d.nx), blk(), 0, st, d, r, u, v, w, sm[0], sm[1], sm[2], sm[3],
+                     sm[4], sm[5], sm[6]);
+  hipLaunchKernelGGL(k_smag_nm, grd(d.nz, d.ny, d.nx), blk(), 0, st, d, th, qv, sm[7]);
+  float gs = cbrtf(d.dx * d.dy * d.dz);
+  float cg = cs * gs;
+  hipLaunchKernelGGL(k_smag_k, e1, e2, 0, st, n, sm[6], sm[7], sm[8], sm[9], cg * cg, prandtl);
+  for (int q = 0; q < 10; ++q) hipLaunchKernelGGL(k_fill, e1, e2, 0, st, n, lsm[q], 0.0f);
+  hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, st, lcg2);
+  for (int q = 0; q < n_scalars; ++q) {
+    if (!scalars[q] || !lss[q] || !lscalars[q]) return fail(-2, "lex_smag_vjp: null scalar pointer%s");
+    hipLaunchKernelGGL(k_smag_scalar_vjp, grd(d.nz, d.ny, d.nx), blk(), 0, st, d, r, (const float*)sm[9], scalars[q],
+                       lss[q], lsm[9], lscalars[q]);
+  }
+  hipLaunchKernelGGL(k_smag_uvw_vjp, grd(d.nz + 1, d.ny + 1, d.nx + 1), blk(), 0, st, d, r, (const float*)sm[8],
+                     (const float*)sm[0], (const float*)sm[1], (const float*)sm[2], (const float*)sm[3],
+                     (const float*)sm[4], (const float*)sm[5], lsu, lsv, lsw, lsm[8], lsm[0], lsm[1], lsm[2], lsm[3],
+                     lsm[4], lsm[5]);
+  hipLaunchKernelGGL(k_smag_k_vjp, e1, e2, 0, st, n, (const float*)sm[6], (const float*)sm[7], (const float*)lsm[8],
+                     (const float*)lsm[9], cg * cg, prandtl, lsm[6], lsm[7], lcg2);
+  for (int q = 0; q < 8; ++q) hipLaunchKernelGGL(k_fold, grd(d.nz, d.ny, d.nx), blk(), 0, st, d, 0, lsm[q]);
+  hipLaunchKernelGGL(k_smag_nm_vjp, grd(d.nz, d.ny, d.nx), blk(), 0, st, d, th, qv, (const float*)lsm[7], lth, lqv);
+  hipLaunchKernelGGL(k_smag_deform_vjp, grd(d.nz, d.ny, d.nx), blk(), 0, st, d, r, (const float*)sm[0],
+                     (const float*)sm[1], (const float*)sm[2], (const float*)sm[3], (const float*)sm[4],
+                     (const float*)sm[5], (const float*)lsm[0], (const float*)lsm[1], (const float*)lsm[2],
+                     (const float*)lsm[3], (const float*)lsm[4], (const float*)lsm[5], (const float*)lsm[6], lu, lv,
+                     lw);
+  if (lcs)
+    hipLaunchKernelGGL(k_cs_grad, dim3(1), dim3(64), 0, st, (const double*)lcg2,
+                       2.0 * (double)cs * (double)gs * (double)gs, lcs);
+  return check_launch("lex_smag_vjp");
+}
+
+size_t lex_rk3_vjp_workspace_bytes(const lex_grid_t* g, float dt, int n_sound, int n_passive) {
+  if (!grid_ok(g) || n_passive < 0 || n_passive > 8 || n_sound < 1 || !(dt > 0.f)) return 0;
+  Dims d = mkdims(g);
+  return vjp_layout(d, (double)dt, n_sound, n_passive).total;
+}
+
+int lex_rk3_step_vjp(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b, float* const* state,
+                     float* const* lam_out, float* const* lam_in, double* lam_cs, void* ws, size_t ws_bytes,
+                     void* stream) {
+  REQGRID(g);
+  REQ(o); REQ(b); REQ(state); REQ(lam_out); REQ(lam_in); REQ(ws);
+  REQ(b->rho0); REQ(b->theta0); REQ(b->qv0); REQ(b->pi0);
+  if (o->n_passive < 0 || o->n_passive > 8) return fail(-1, "lex_rk3_step_vjp: n_passive must be 0..8%s");
+  if (o->n_sound < 1 || !(o->dt > 0.f)) return fail(-1, "lex_rk3_step_vjp: need n_sound >= 1 and dt > 0%s");
+  for (int q = 0; q < 6 + o->n_passive; ++q) {
+    if (!state[q] || !lam_out[q] || !lam_in[q]) return fail(-2, "lex_rk3_step_vjp: null field pointer%s");
+    if (lam_in[q] == lam_out[q] || lam_in[q] == state[q])
+      return fail(-1, "lex_rk3_step_vjp: lam_in must not alias lam_out or the state%s");
+  }
+  Dims d = mkdims(g);
+  VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
+  if (ws_bytes < L.total) return fail(-3, "lex_rk3_step_vjp: workspace too small%s");
+  int rc = rk3_step_vjp(d, *o, *b, state, lam_out, lam_in, lam_cs, (char*)ws, L, S(stream));
+  if (rc > 0) return fail(rc, "lex_rk3_step_vjp: HIP error %s", hipGetErrorString((hipError_t)rc));
+  return 0;
+}
+
+// ------------------------------------------------------------------ X4 conv3d (BF16 MFMA)
+}  // extern "C"
+namespace {
+inline int padc(int c) { return c <= 16 ? 16 : 32; }
+inline bool conv_ok(int nx, int ny, int nz, int cin, int cout) {
+  return nx > 0 && ny > 0 && nz > 0 && cin >= 1 && cin <= 32 && cout >= 1 && cout <= 32;
+}
+}  // namespace
+extern "C" {
+
+size_t lex_conv3d_packed_weight_bytes(int cin, int cout, int flip) {
+  int ci = flip ? cout : cin, co = flip ? cin : cout;
+  if (ci < 1 || ci > 32 || co < 1 || co > 32) return 0;
+  int cip = padc(ci), cop = padc(co);
+  int tps = 32 / cip, nstep = (27 + tps - 1) / tps;
+  return (size_t)cop * nstep * 32 * 2;
+}
+
+int lex_conv3d_pack_weights(const float* w, int cin, int cout, int flip, void* wp, void* stream) {
+  REQ(w); REQ(wp);
+  if (cin < 1 || cin > 32 || cout < 1 || cout > 32) return fail(-1, "lex_conv3d_pack_weights: channels must be 1..32%s");
+  int ci = flip ? cout : cin, co = flip ? cin : cout;
+  int cip = padc(ci), cop = padc(co);
+  int tps = 32 / cip, nstep = (27 + tps - 1) / tps;
+  long n = (long)cop * nstep * 32;
+  hipLaunchKernelGGL(k_conv3_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), w, cin, cout, cip, cop,
+                     nstep, flip ? 1 : 0, (__bf16*)wp);
+  return check_launch("lex_conv3d_pack_weights");
+}
+
+}  // extern "C"
+namespace {
+template <int CI, int CO>
+void launch_fwd(int nx, int ny, int nz, const void* x, const void* wp, const float* bias, int mode, int cout_real,
+                void* y, float* yf, float* pre, hipStream_t st) {
+  dim3 grid((nz + CTZ - 1) / CTZ, (ny + CTY - 1) / CTY, (nx + CTX - 1) / CTX);
+  hipLaunchKernelGGL((k_conv3_fwd<CI, CO>), grid, dim3(256), 0, st, nx, ny, nz, (const __bf16*)x, (const __bf16*)wp,
+                     bias, mode, cout_real, (__bf16*)y, yf, pre);
+}
+template <int CI, int CO>
+void launch_wgrad(int nx, int ny, int nz, const void* x, const void* g, float* dw, hipStream_t st) {
+  long nt = (long)((nx + CTX - 1) / CTX) * ((ny + CTY - 1) / CTY) * ((nz + CTZ - 1) / CTZ);
+  unsigned nb = (unsigned)(nt < 1024 ? nt : 1024);
+  hipLaunchKernelGGL((k_conv3_wgrad<CI, CO>), dim3(nb), dim3(256), 0, st, nx, ny, nz, (const __bf16*)x,
+                     (const __bf16*)g, dw);
+}
+}  // namespace
+extern "C" {
+
+int lex_conv3d_bf16_fwd(int nx, int ny, int nz, int cin, int cout, const void* x, const void* wp, const float* bias,
+                        int gelu, int out_fp32, void* y, float* yf, float* pre, void* stream) {
+  REQ(x); REQ(wp); REQ(bias);
+  if (!conv_ok(nx, ny, nz, cin, cout)) return fail(-1, "lex_conv3d_bf16_fwd: bad sizes (channels 1..32)%s");
+  if (out_fp32 ? !yf : !y) return fail(-2, "lex_conv3d_bf16_fwd: missing output%s");
+  int cip = padc(cin), cop = padc(cout);
+  int mode = (gelu ? 1 : 0) | (out_fp32 ? 2 : 0);
+  hipStream_t st = S(stream);
+  if (cip == 16 && cop == 16) launch_fwd<16, 16>(nx, ny, nz, x, wp, bias, mode, cout, y, yf, pre, st);
+  else if (cip == 16 && cop == 32) launch_fwd<16, 32>(nx, ny, nz, x, wp, bias, mode, cout, y, yf, pre, st);
+  else if (cip == 32 && cop == 16) launch_fwd<32, 16>(nx, ny, nz, x, wp, bias, mode, cout, y, yf, pre, st);
+  else launch_fwd<32, 32>(nx, ny, nz, x, wp, bias, mode, cout, y, yf, pre, st);
+  return check_launch("lex_conv3d_bf16_fwd");
+}
+
+int lex_conv3d_bf16_bwd_data(int nx, int ny, int nz, int cin, int cout, const void* g_pre, const void* wp_flip,
+                             const float* zero_bias, float* dx, void* stream) {
+  REQ(g_pre); REQ(wp_flip); REQ(zero_bias); REQ(dx);
+  if (!conv_ok(nx, ny, nz, cin, cout)) return fail(-1, "lex_conv3d_bf16_bwd_data: bad sizes (channels 1..32)%s");
+  // transposed convolution: input channels = cout, output channels = cin
+  return lex_conv3d_bf16_fwd(nx, ny, nz, cout, cin, g_pre, wp_flip, zero_bias, 0, 1, nullptr, dx, nullptr, stream);
+}
+
+int lex_conv3d_bf16_bwd_weight(int nx, int ny, int nz, int cin, int cout, const void* x, const void* g_pre, float* dw,
+                               float* db, void* stream) {
+  REQ(x); REQ(g_pre); REQ(dw);
+  if (!conv_ok(nx, ny, nz, cin, cout)) return fail(-1, "lex_conv3d_bf16_bwd_weight: bad sizes (channels 1..32)%s");
+  int cip = padc(cin), cop = padc(cout);
+  hipStream_t st = S(stream);
+  if (cip == 16 && cop == 16) launch_wgrad<16, 16>(nx, ny, nz, x, g_pre, dw, st);
+  else if (cip == 16 && cop == 32) launch_wgrad<16, 32>(nx, ny, nz, x, g_pre, dw, st);
+  else if (cip == 32 && cop == 16) launch_wgrad<32, 16>(nx, ny, nz, x, g_pre, dw, st);
+  else launch_wgrad<32, 32>(nx, ny, nz, x, g_pre, dw, st);
+  if (db) {
+    long ncell = (long)nx * ny * nz;
+    hipLaunchKernelGGL(k_conv3_bgrad, dim3(256, cout), dim3(256), 0, st, ncell, cop, (const __bf16*)g_pre, db);
+  }
+  return check_launch("lex_conv3d_bf16_bwd_weight");
+}
+
+int lex_gelu_bwd(long ncell, int cp, int c_real, int cpre, const float* gpost, int gstride, const float* pre, int gelu,
+                 void* gpre, void* stream) {
+  REQ(gpost); REQ(gpre);
+  if (gelu) REQ(pre);
+  if (ncell <= 0 || cp < c_real || c_real < 1) return fail(-1, "lex_gelu_bwd: bad sizes%s");
+  long n = ncell * cp;
+  hipLaunchKernelGGL(k_gelu_bwd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), ncell, cp, c_real, cpre,
+                     gpost, gstride, pre, gelu, (__bf16*)gpre);
+  return check_launch("lex_gelu_bwd");
+}
+
+int lex_to_bf16_channels_last(long ncell, int c_real, int cstride, int cp, const float* in, void* out, void* stream) {
+  REQ(in); REQ(out);
+  if (ncell <= 0 || cp < c_real || c_real < 1) return fail(-1, "lex_to_bf16_channels_last: bad sizes%s");
+  long n = ncell * cp;
+  hipLaunchKernelGGL(k_to_bf16_cl, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), ncell, c_real, cstride,
+                     cp, in, (__bf16*)out);
+  return check_launch("lex_to_bf16_channels_last");
+}
+
+// ------------------------------------------------------------------ learned-SGS data paths
+int lex_sgs_features(const lex_grid_t* g, const float* theta0, const float* qv0, const float* u, const float* v,
+                     const float* w, const float* th, const float* qv, const float* shift, const float* scale,
+                     float* out, void* stream) {
+  REQGRID(g);
+  REQ(theta0); REQ(qv0); REQ(u); REQ(v); REQ(w); REQ(th); REQ(qv); REQ(shift); REQ(scale); REQ(out);
+  Dims d = mkdims(g);
+  FeatNorm nrm;
+  for (int c = 0; c < 11; ++c) {
+    nrm.shift[c] = shift[c];
+    nrm.scale[c] = scale[c];
+  }
+  hipLaunchKernelGGL(k_sgs_features, grd(d.nz, d.ny, d.nx), blk(), 0, S(stream), d, theta0, qv0, u, v, w, th, qv, nrm,
+                     out);
+  return check_launch("lex_sgs_features");
+}
+
+int lex_sgs_apply(const lex_grid_t* g, const float* y, const float* out_scale, float* su, float* sv, float* sw,
+                  float* sth, float* sqv, void* stream) {
+  REQGRID(g);
+  REQ(y); REQ(out_scale); REQ(su); REQ(sv); REQ(sw); REQ(sth); REQ(sqv);
+  Dims d = mkdims(g);
+  hipLaunchKernelGGL(k_sgs_apply, grd(d.nz + 1, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, y, out_scale[0],
+                     out_scale[1], out_scale[2], out_scale[3], out_scale[4], su, sv, sw, sth, sqv);
+  return check_launch("lex_sgs_apply");
+}
+
+int lex_coarsen(const lex_grid_t* g, int mode, int fx, int fy, int fz, const float* a, float* out, void* stream) {
+  REQGRID(g);
+  REQ(a); REQ(out);
+  if (mode < 0 || mode > 3 || fx < 1 || fy < 1 || fz < 1) return fail(-1, "lex_coarsen: bad mode or factors%s");
+  if (g->nx % fx || g->ny % fy || g->nz % fz) return fail(-1, "lex_coarsen: the grid must divide by the factors%s");
+  Dims d = mkdims(g);
+  int ni = d.nx / fx + (mode == 1), nj = d.ny / fy + (mode == 2), nk = d.nz / fz + (mode == 3);
+  hipLaunchKernelGGL(k_coarsen, grd(nk, nj, ni), blk(), 0, S(stream), d, mode, fx, fy, fz, a, out);
+  return check_launch("lex_coarsen");
+}
+
+int lex_rk3_step_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b, float* const* state,
+                         const float* su, const float* sv, const float* sw, const float* const* ss, void* ws,
+                         size_t ws_bytes, void* stream) {
+  REQGRID(g);
+  REQ(o); REQ(b); REQ(state); REQ(ws); REQ(su); REQ(sv); REQ(sw); REQ(ss);
+  if (o->n_passive < 0 || o->n_passive > 8) return fail(-1, "lex_rk3_step_ext_sgs: n_passive must be 0..8%s");
+  Dims d = mkdims(g);
+  Rk3Layout L = rk3_layout(d, o->n_passive);
+  if (ws_bytes < L.total) return fail(-3, "lex_rk3_step_ext_sgs: workspace too small%s");
+  hipStream_t st = S(stream);
+  char* w = (char*)ws;
+  auto cp = [&](const float* src, size_t off, size_t n) {
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (long)n, src, (float*)(w + off));
+  };
+  cp(su, L.off_su, L.iu_);
+  cp(sv, L.off_sv, L.iv_);
+  cp(sw, L.off_sw, L.iw_);
+  for (int q = 0; q < 2 + o->n_passive; ++q) {
+    if (!ss[q]) return fail(-2, "lex_rk3_step_ext_sgs: null scalar SGS pointer%s");
+    cp(ss[q], L.off_ss + q * al256(L.ic_ * 4), L.ic_);
+  }
+  lex_step_opts_t o2 = *o;
+  o2.turb = 2;
+  return lex_rk3_step(g, &o2, b, state, ws, ws_bytes, stream);
 }
 
 }  // extern "C"

@@ -2,6 +2,7 @@
 // thread per output point.  They are the drop-in entry points behind the Python
 // mirror modules; the fused stage kernels in lex_stage.hip reuse the device functions
 // defined here.
+#pragma once
 #include "lex_common.hpp"
 
 namespace lex {

@@ -7,6 +7,7 @@
 // reduction (per-block FP64 partials, the last block to finish sums them in a fixed
 // order and advances the scalar state).  All kernels check a device-side `done` flag,
 // so the host can enqueue iterations ahead and poll the flag only every few iterations.
+#pragma once
 #include "lex_common.hpp"
 
 namespace lex {

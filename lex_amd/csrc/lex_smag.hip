@@ -3,6 +3,7 @@
 //   k_smag_nm     : compute_nm          (turbulence.py:391-427)  -> N^2
 //   k_smag_k      : compute_k           (turbulence.py:430-446)  -> Km, Kh
 //   k_smag_uvw / k_smag_scalar : compute_smag (turbulence.py:9-111)
+#pragma once
 #include "lex_common.hpp"
 
 namespace lex {

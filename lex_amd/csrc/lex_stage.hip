@@ -178,8 +178,10 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
     const double dt_s = (s == 0) ? dtd / 3.0 : (s == 1 ? dtd / 2.0 : dtd);
     float** src = (s == 0) ? X0 : (s == 1 ? XA : XB);
     float** dst = (s == 0) ? XA : (s == 1 ? XB : X0);
+    // turb 1: Smagorinsky computed here in stage 0; turb 2: SGS tendencies supplied by
+    // the caller in the workspace slots (the learned SGS net); both frozen over the step
     const bool turb = o.turb != 0;
-    if (s == 0 && turb) {
+    if (s == 0 && o.turb == 1) {
       // compute_deformation -> compute_nm -> compute_k -> compute_smag
       LEX_LAUNCH(KID_SMAG_DEFORM, k_smag_deform, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, src[1], src[2], src[3], sm[0], sm[1],
                          sm[2], sm[3], sm[4], sm[5], sm[6]);

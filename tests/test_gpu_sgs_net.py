@@ -1,0 +1,112 @@
+"""BF16 MFMA convolutions of the SGS net against torch's conv3d on the CPU.
+
+Kernel checks use a float64 CPU reference fed with the same BF16-rounded operands, so
+the only difference is FP32 accumulation order: rel-L2 < 1e-5.  The whole net (BF16
+activations and gradients) is checked against plain FP32 torch autograd with the
+tolerance BF16 rounding allows: rel-L2 < 3e-2.  Grid sizes are ragged on purpose
+(not multiples of the 4 x 4 x 16 tile).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (10, 6, 20)
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _torch_w(W):
+    """W[27, cin, cout] -> torch conv3d weight (cout, cin, 3, 3, 3)."""
+    cin, cout = W.shape[1], W.shape[2]
+    return W.reshape(3, 3, 3, cin, cout).permute(4, 3, 0, 1, 2).contiguous()
+
+
+def _ncdhw(x):
+    return x.permute(3, 0, 1, 2).unsqueeze(0).contiguous()
+
+
+def _gelu(x):
+    import torch
+    return torch.nn.functional.gelu(x, approximate="tanh")
+
+
+@pytest.mark.parametrize("cin,cout", [(11, 32), (32, 32), (32, 5), (5, 16)])
+def test_conv_forward(cin, cout):
+    import torch
+    from lex_amd import sgs_net as sn
+    g = torch.Generator().manual_seed(cin * 100 + cout)
+    x = torch.randn(*SHAPE, cin, generator=g)
+    W = torch.randn(27, cin, cout, generator=g) / np.sqrt(27 * cin)
+    b = torch.randn(cout, generator=g) * 0.1
+    xb, Wb = x.bfloat16().double(), W.bfloat16().double()
+    ref_pre = torch.nn.functional.conv3d(_ncdhw(xb), _torch_w(Wb), padding=1)[0].permute(1, 2, 3, 0) + b.double()
+    xd = sn.to_bf16_channels_last(x.cuda())
+    for gelu in (0, 1):
+        ref = _gelu(ref_pre) if gelu else ref_pre
+        yf, pre = sn.conv3d_fwd(xd, SHAPE, W.cuda(), b.cuda(), gelu=gelu, out_fp32=True, want_pre=True)
+        got = yf.cpu().double().reshape(*SHAPE, cout)
+        assert _rel(got, ref) < 1e-5, (gelu, _rel(got, ref))
+        assert _rel(pre.cpu().double()[:, :cout].reshape(*SHAPE, cout), ref_pre) < 1e-5
+        yb, _ = sn.conv3d_fwd(xd, SHAPE, W.cuda(), b.cuda(), gelu=gelu, out_fp32=False)
+        gotb = yb.cpu().double()[:, :cout].reshape(*SHAPE, cout)
+        assert _rel(gotb, ref) < 8e-3  # one BF16 rounding of the output
+        assert not bool(yb.cpu()[:, cout:].float().abs().max() > 0) if yb.shape[1] > cout else True
+
+
+@pytest.mark.parametrize("cin,cout", [(11, 32), (32, 32), (32, 5)])
+def test_conv_backward_kernels(cin, cout):
+    import torch
+    from lex_amd import sgs_net as sn
+    g = torch.Generator().manual_seed(7 + cin + cout)
+    x = torch.randn(*SHAPE, cin, generator=g)
+    W = torch.randn(27, cin, cout, generator=g) / np.sqrt(27 * cin)
+    gy = torch.randn(*SHAPE, cout, generator=g)
+    xb, Wb, gb = x.bfloat16().double(), W.bfloat16().double(), gy.bfloat16().double()
+    # references: input and weight gradients of conv3d (float64)
+    dx_ref = torch.nn.grad.conv3d_input((1, cin) + SHAPE, _torch_w(Wb), _ncdhw(gb), padding=1)[0].permute(1, 2, 3, 0)
+    dw_ref = torch.nn.grad.conv3d_weight(_ncdhw(xb), (cout, cin, 3, 3, 3), _ncdhw(gb), padding=1)
+    dw_ref = dw_ref.permute(2, 3, 4, 1, 0).reshape(27, cin, cout)
+    db_ref = gb.sum(dim=(0, 1, 2))
+    xd = sn.to_bf16_channels_last(x.cuda())
+    gd = sn.to_bf16_channels_last(gy.cuda())
+    dx = sn.conv3d_bwd_data(gd, SHAPE, W.cuda()).cpu().double().reshape(*SHAPE, cin)
+    assert _rel(dx, dx_ref) < 1e-5, _rel(dx, dx_ref)
+    dw, db = sn.conv3d_bwd_weight(xd, gd, SHAPE, cin, cout)
+    assert _rel(dw.cpu(), dw_ref) < 1e-5, _rel(dw.cpu(), dw_ref)
+    assert _rel(db.cpu(), db_ref) < 1e-5, _rel(db.cpu(), db_ref)
+
+
+def test_net_forward_backward_vs_fp32_autograd():
+    import torch
+    from lex_amd.sgs_net import SgsNet
+    net = SgsNet(seed=5)
+    g = torch.Generator().manual_seed(9)
+    x = torch.randn(*SHAPE, 11, generator=g)
+    tgt = torch.randn(*SHAPE, 5, generator=g) * 0.1
+    xd = x.cuda().requires_grad_(True)
+    y = net(xd)
+    loss = 0.5 * ((y - tgt.cuda()) ** 2).sum()
+    loss.backward()
+    # FP32 CPU reference with the same weights
+    ws = [w.detach().cpu().clone().requires_grad_(True) for w in net.weights]
+    bs = [b.detach().cpu().clone().requires_grad_(True) for b in net.biases]
+    xr = x.clone().requires_grad_(True)
+    h = _ncdhw(xr)
+    for i, (w, b) in enumerate(zip(ws, bs)):
+        h = torch.nn.functional.conv3d(h, _torch_w(w), b, padding=1)
+        if i < 3:
+            h = _gelu(h)
+    yr = h[0].permute(1, 2, 3, 0)
+    lr = 0.5 * ((yr - tgt) ** 2).sum()
+    lr.backward()
+    assert _rel(y.detach().cpu(), yr.detach()) < 3e-2
+    for a, r in zip(net.weights, ws):
+        assert _rel(a.grad.cpu(), r.grad) < 3e-2, _rel(a.grad.cpu(), r.grad)
+    for a, r in zip(net.biases, bs):
+        assert _rel(a.grad.cpu(), r.grad) < 3e-2, _rel(a.grad.cpu(), r.grad)
+    assert _rel(xd.grad.cpu(), xr.grad) < 3e-2
