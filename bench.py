@@ -2,9 +2,12 @@
 """Throughput of LEX's WS-RK3 dynamics step on one MI355X.
 
 Workload: the fully compressible moist warm bubble at 256x256x128 cells (dx = dy = dz =
-100 m, dt = 1 s, WS-RK3 with n_sound = 6, i.e. 1 + 3 + 6 acoustic substeps per step,
-WENO5 advection, Smagorinsky Cs = 0.18, scalars theta, qv and passive qc, qr), FP32.
-A "step" is one full WS-RK3 step of the whole grid; inputs are resident in HBM.
+100 m, dt = 1 s, WS-RK3, WENO5 advection, Smagorinsky Cs = 0.18, scalars theta, qv and
+passive qc, qr), FP32.  The specified n_sound = 6 (1 + 3 + 6 acoustic substeps) is
+acoustically unstable on this grid, so the timed run uses n_sound = 7 (1 + 4 + 7) and the
+n_sound = 6 step time is reported beside it.  A "step" is one full WS-RK3 step of the
+whole grid; inputs are resident in HBM.  Also measured: the forward and forward+adjoint
+step times of the differentiable 128x128x64 workload (20 steps, checkpoint every 5).
 
 Prints one JSON line: cell-steps/s, the dominant kernel's roofline (algorithmic bytes per
 launch / its average launch time from HIP events over the timed region, against the
@@ -65,6 +68,63 @@ def cpu_baseline(sample_steps=3, n_sound=7, nx=64, ny=64, nz=128):
                       f"(NumPy FP32 oracle, single-threaded), {dt:.1f} s"}
 
 
+def fwd_adjoint(steps=20, checkpoint_every=5, nx=128, ny=128, nz=64):
+    """Differentiable workload: 128x128x64 at 200 m, dt 2 s, n_sound 10, 20 steps,
+    loss 1/2 ||theta'_20 - target||^2, gradients w.r.t. theta'_0 and Cs, checkpoint every 5
+    steps.  Returns forward and forward+adjoint step times on the same state."""
+    import numpy as np
+    import torch
+    from lex_amd import adjoint as adj, one_sprint_functions as sp
+    from lex_amd._core import grid_from
+    from lex_amd.setup_lex import build_case, config2
+    case, opts = config2()
+    case = case.scaled(nx, ny, nz)
+    phys, base, grids, _ = build_case(case)
+    g = grid_from(base[1], *grids[3:6])
+    dt, ns = opts["dt"], opts["n_sound"]
+    mo = (2, False, False, 0, False, False, 1)
+    th0 = base[2][3:-3, 3:-3, 1:-1]
+    tp0 = (phys[0][3:-3, 3:-3, 1:-1] - th0).contiguous()
+    target = torch.from_numpy(np.random.default_rng(2029).normal(0, 0.5, (nx, ny, nz)).astype(np.float32)).cuda()
+    # forward only
+    st = tuple(a.clone() for a in phys)
+    sp.fused_step(st, base, g, mo, dt=dt, n_sound=ns)
+    st = tuple(a.clone() for a in phys)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        sp.fused_step(st, base, g, mo, dt=dt, n_sound=ns)
+    e1.record()
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1)
+
+    def fwd_bwd():
+        tp = tp0.clone().requires_grad_(True)
+        cs = torch.tensor(0.18, device="cuda", requires_grad=True)
+        th = adj.sprint_theta(tp, cs, phys, base, g, dt, ns, steps, checkpoint_every)
+        loss = 0.5 * ((th - target) ** 2).sum()
+        loss.backward()
+        return tp.grad, cs.grad, loss
+
+    fwd_bwd()
+    torch.cuda.synchronize()
+    e0.record()
+    gt, gc, loss = fwd_bwd()
+    e1.record()
+    torch.cuda.synchronize()
+    tot_ms = e0.elapsed_time(e1)
+    ncell = nx * ny * nz
+    return {"workload": f"differentiable compressible bubble {nx}x{ny}x{nz}, dx 200 m, dt {dt} s, n_sound {ns}, "
+                        f"{steps} steps, checkpoint every {checkpoint_every}",
+            "fwd_ms_per_step": fwd_ms / steps, "fwd_adj_ms_per_step": tot_ms / steps,
+            "ratio_fwd_adj_over_fwd": tot_ms / fwd_ms,
+            "fwd_cell_steps_per_s": ncell * steps / (fwd_ms / 1e3),
+            "fwd_adj_cell_steps_per_s": ncell * steps / (tot_ms / 1e3),
+            "grad_finite": bool(torch.isfinite(gt).all()) and bool(torch.isfinite(gc).all()),
+            "loss": float(loss.detach()), "dL_dCs": float(gc)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +137,10 @@ def main():
                     help="acoustic substeps per full step (the specified 6 is acoustically unstable at "
                          "dx=100 m, dt=1 s; 7 is the smallest stable value)")
     ap.add_argument("--no-as-specified", action="store_true", help="skip the n_sound=6 timing")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint workload")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed region, copy a 256 MiB buffer 3 times (a kernel of known HBM "
+                         "traffic, used to scale rocprof FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -157,6 +221,15 @@ def main():
     if args.profile:
         print(json.dumps(kernels, indent=1), file=sys.stderr)
 
+    if args.copy_probe:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    del state, phys, init
+    fa = None if args.no_adjoint else fwd_adjoint()
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample_steps, n_sound)
     line = {
         "metric": "cell-steps/s (fwd, fwd+adjoint) vs plain at stated error; % HBM roofline",
@@ -183,6 +256,7 @@ def main():
                      "bytes_per_cell": bpc, "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
         "as_specified": as_specified,
+        "fwd_adjoint": fa,
         "kernels": kernels,
         "state_finite": finite,
     }

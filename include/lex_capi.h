@@ -240,6 +240,19 @@ size_t lex_rk3_vjp_workspace_bytes(const lex_grid_t* g, float dt, int n_sound, i
 int lex_rk3_step_vjp(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b, float* const* state,
                      float* const* lam_out, float* const* lam_in, double* lam_cs, void* workspace,
                      size_t workspace_bytes, void* stream);
+/* The same step split in two, so a checkpointed backward pass records each step of a
+ * segment once and reverses it from the record: lex_rk3_step_tape advances state into
+ * state_out (not aliased) recording the tape; lex_rk3_step_vjp_taped reverses that step
+ * from its tape (the tape is only read).  The tape holds the stage states, the frozen
+ * SGS fields, rhs and theta_rho per stage and every acoustic substep input. */
+size_t lex_rk3_tape_bytes(const lex_grid_t* g, float dt, int n_sound, int n_passive);
+size_t lex_rk3_adjoint_bytes(const lex_grid_t* g, float dt, int n_sound, int n_passive);
+int lex_rk3_step_tape(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b, float* const* state,
+                      float* const* state_out, void* tape, size_t tape_bytes, void* stream);
+int lex_rk3_step_vjp_taped(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b,
+                           float* const* state, const void* tape, size_t tape_bytes, float* const* lam_out,
+                           float* const* lam_in, double* lam_cs, void* adjoint_ws, size_t adjoint_bytes,
+                           void* stream);
 
 /* ---- learned SGS correction net: 3x3x3 'SAME' convolutions on BF16 MFMA -----------
  * Replaces flax nn.Conv(kernel (3,3,3), strides 1, padding 'SAME', bias) and nn.gelu of

@@ -110,6 +110,12 @@ _SIGS = {
     "lex_rk3_vjp_workspace_bytes": [_G, _F, _I, _I],
     "lex_rk3_step_vjp": [_G, C.POINTER(StepOpts), C.POINTER(BaseCols), C.POINTER(_P), C.POINTER(_P),
                          C.POINTER(_P), _P, _P, C.c_size_t, _P],
+    "lex_rk3_tape_bytes": [_G, _F, _I, _I],
+    "lex_rk3_adjoint_bytes": [_G, _F, _I, _I],
+    "lex_rk3_step_tape": [_G, C.POINTER(StepOpts), C.POINTER(BaseCols), C.POINTER(_P), C.POINTER(_P), _P,
+                          C.c_size_t, _P],
+    "lex_rk3_step_vjp_taped": [_G, C.POINTER(StepOpts), C.POINTER(BaseCols), C.POINTER(_P), _P, C.c_size_t,
+                               C.POINTER(_P), C.POINTER(_P), _P, _P, C.c_size_t, _P],
     # learned SGS net: BF16 MFMA convolutions
     "lex_conv3d_packed_weight_bytes": [_I, _I, _I],
     "lex_conv3d_pack_weights": [_P, _I, _I, _I, _P, _P],
@@ -127,7 +133,8 @@ _SIGS = {
 _RESTYPE = {"lex_last_error": C.c_char_p, "lex_bicgstab_workspace_bytes": C.c_size_t,
             "lex_rk3_workspace_bytes": C.c_size_t, "lex_profile_kernel_name": C.c_char_p,
             "lex_smag_vjp_workspace_bytes": C.c_size_t, "lex_rk3_vjp_workspace_bytes": C.c_size_t,
-            "lex_conv3d_packed_weight_bytes": C.c_size_t}
+            "lex_conv3d_packed_weight_bytes": C.c_size_t, "lex_rk3_tape_bytes": C.c_size_t,
+            "lex_rk3_adjoint_bytes": C.c_size_t}
 
 
 def profile_enable(on: bool, capacity_pairs: int = 8192):

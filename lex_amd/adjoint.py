@@ -1,11 +1,12 @@
 """Hand-written adjoint of the compressible WS-RK3 step and a differentiable sprint.
 
 The reference obtains gradients only implicitly, through ``jax.value_and_grad`` of its
-training loss (lex_dl_train.py:114), without rematerialisation.  Here the reverse of one
-step is the device routine ``lex_rk3_step_vjp`` (a taped re-run of the step, then the
-stages and the Smagorinsky chain in reverse), and ``WsRk3Sprint`` wraps N steps in a
-``torch.autograd.Function`` that keeps a checkpoint every k steps and recomputes each
-segment in the backward pass.
+training loss (lex_dl_train.py:114), without rematerialisation.  Here one step is
+recorded by ``lex_rk3_step_tape`` (stage states, frozen SGS fields, rhs, theta_rho and every
+acoustic substep input) and reversed by ``lex_rk3_step_vjp_taped`` (the stages backwards,
+then the Smagorinsky chain); ``lex_rk3_step_vjp`` does both in one call.  ``WsRk3Sprint``
+wraps N steps in a ``torch.autograd.Function`` that keeps the full state every k steps; its
+backward pass records each segment once from its checkpoint and reverses it.
 
 Gradients flow to the initial potential-temperature perturbation theta'_0 (interior) and
 to the Smagorinsky constant Cs; the other initial fields are constants.
@@ -64,6 +65,50 @@ def step_vjp(phys_state, lam_out, base_state, grid, dt, n_sound, cs, prandtl, tu
     ws = _vjp_workspace(grid, dt, n_sound, n_passive, phys_state[0].device)
     _lib.call("lex_rk3_step_vjp", grid, opts, bc, _ptrs(phys_state), _ptrs(lo), _ptrs(li), dcs.data_ptr(),
               _lib.ptr(ws), ws.numel(), _lib.stream())
+    del cols
+    return li, dcs
+
+
+def _bytes(fn, g, dt, n_sound, n_passive):
+    nb = getattr(_lib.load(), fn)(g, float(dt), int(n_sound), int(n_passive))
+    if nb == 0:
+        raise _lib.LexError(f"{fn}: invalid grid or options")
+    return nb
+
+
+def step_tape(phys_state, base_state, grid, dt, n_sound, cs, prandtl, tape=None, turb=1):
+    """One step recorded for the reverse pass: returns (tape, output state); phys_state is
+    not modified.  `tape` may be a preallocated uint8 tensor of lex_rk3_tape_bytes."""
+    n_passive = len(phys_state) - 6
+    if tape is None:
+        tape = torch.empty(_bytes("lex_rk3_tape_bytes", grid, dt, n_sound, n_passive), dtype=torch.uint8,
+                           device=phys_state[0].device)
+    out = tuple(torch.empty_like(t) for t in phys_state)
+    opts = _lib.StepOpts(float(dt), int(n_sound), int(turb), float(cs), float(prandtl), n_passive, 1)
+    cols, bc = _base_cols(base_state)
+    _lib.call("lex_rk3_step_tape", grid, opts, bc, _ptrs(phys_state), _ptrs(out), _lib.ptr(tape), tape.numel(),
+              _lib.stream())
+    del cols
+    return tape, out
+
+
+def step_vjp_taped(phys_state, tape, lam_out, base_state, grid, dt, n_sound, cs, prandtl, turb=1):
+    """Reverse of the step recorded by step_tape (same inputs); returns (lam_in, dcs)."""
+    n_passive = len(phys_state) - 6
+    opts = _lib.StepOpts(float(dt), int(n_sound), int(turb), float(cs), float(prandtl), n_passive, 1)
+    cols, bc = _base_cols(base_state)
+    lo = tuple(t.clone() for t in lam_out)
+    li = tuple(torch.empty_like(t) for t in phys_state)
+    dcs = torch.zeros(1, dtype=torch.float64, device=phys_state[0].device)
+    key = ("adj", grid.nx, grid.ny, grid.nz, float(dt), int(n_sound), n_passive, str(phys_state[0].device))
+    ws = _WSV.get(key)
+    if ws is None:
+        _WSV.clear()
+        ws = torch.empty(_bytes("lex_rk3_adjoint_bytes", grid, dt, n_sound, n_passive), dtype=torch.uint8,
+                         device=phys_state[0].device)
+        _WSV[key] = ws
+    _lib.call("lex_rk3_step_vjp_taped", grid, opts, bc, _ptrs(phys_state), _lib.ptr(tape), tape.numel(), _ptrs(lo),
+              _ptrs(li), dcs.data_ptr(), _lib.ptr(ws), ws.numel(), _lib.stream())
     del cols
     return li, dcs
 
@@ -171,17 +216,20 @@ class WsRk3Sprint(torch.autograd.Function):
         _interior(lam[0]).copy_(grad_out.to(torch.float32))
         dcs = torch.zeros(1, dtype=torch.float64, device=x0[0].device)
         starts = sorted(ctx.ckpts.keys())
+        tapes = []
         for s in reversed(starts):
             end = min(s + k, n)
+            # record every step of the segment once, then reverse it from the records
             states = [ctx.ckpts[s]]
-            with torch.no_grad():
-                for _ in range(s, end - 1):
-                    y = tuple(t.clone() for t in states[-1])
-                    fused_step(y, base, grid, (2, False, False, 0, False, False, 1), dt=cfg["dt"],
-                               n_sound=cfg["n_sound"], cs=ctx.cs, prandtl=cfg["prandtl"])
-                    states.append(y)
-            for st in reversed(states):
-                lam, d = step_vjp(st, lam, base, grid, cfg["dt"], cfg["n_sound"], ctx.cs, cfg["prandtl"])
+            for t in range(end - s):
+                tp = tapes[t] if t < len(tapes) else None
+                tp, y = step_tape(states[-1], base, grid, cfg["dt"], cfg["n_sound"], ctx.cs, cfg["prandtl"], tape=tp)
+                if t >= len(tapes):
+                    tapes.append(tp)
+                states.append(y)
+            for t in reversed(range(end - s)):
+                lam, d = step_vjp_taped(states[t], tapes[t], lam, base, grid, cfg["dt"], cfg["n_sound"], ctx.cs,
+                                        cfg["prandtl"])
                 dcs += d
         g_theta = _interior(lam[0]).clone()
         g_cs = dcs.to(torch.float32).reshape(()) if ctx.cs_is_tensor else None

@@ -123,7 +123,7 @@ inline int field_mode(int q) { return q == 1 ? 1 : (q == 2 ? 2 : (q == 3 ? 3 : 0
 // Forward pass with tape.  X0: the step's input state (not modified).
 template <class O, class B>
 void rk3_forward_tape(const Dims& d, const O& o, const B& b, float* const* X0, char* ws, const VjpLayout& L,
-                      hipStream_t st) {
+                      hipStream_t st, float* const* out = nullptr) {
   VjpCtx cx{d, st, dim3(64, 4, 1)};
   const int nf = 6 + o.n_passive, nsc = 2 + o.n_passive;
   Rk3Layout RL;
@@ -148,7 +148,7 @@ void rk3_forward_tape(const Dims& d, const O& o, const B& b, float* const* X0, c
   for (int s = 0; s < 3; ++s) {
     const double dt_s = (s == 0) ? dtd / 3.0 : (s == 1 ? dtd / 2.0 : dtd);
     float* const* src = (s == 0) ? X0 : (s == 1 ? XA : XB);
-    float** dst = (s == 0) ? XA : (s == 1 ? XB : Xn);
+    float* const* dst = (s == 0) ? XA : (s == 1 ? XB : (out ? out : (float* const*)Xn));
     float* tr = (float*)(ws + L.off_tr[s]);
     float* ru = (float*)(ws + L.off_ru[s]);
     float* rv = (float*)(ws + L.off_rv[s]);
@@ -205,40 +205,43 @@ void rk3_forward_tape(const Dims& d, const O& o, const B& b, float* const* X0, c
 // lam_cs: accumulates dL/dCs.  X0 is the step's input state (not modified).
 template <class O, class B>
 int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float* const* lam_out,
-                 float* const* lam_in, double* lam_cs, char* ws, const VjpLayout& L, hipStream_t st) {
+                 float* const* lam_in, double* lam_cs, char* ws, char* adj, const VjpLayout& L, hipStream_t st,
+                 bool run_tape = true) {
   VjpCtx cx{d, st, dim3(64, 4, 1)};
   const int nf = 6 + o.n_passive, nsc = 2 + o.n_passive;
-  rk3_forward_tape(d, o, b, X0, ws, L, st);
+  if (run_tape) rk3_forward_tape(d, o, b, X0, ws, L, st);
+  // adjoint buffers live in `adj` (offsets from off_lXA on); the tape in `ws`
+  auto A = [&](size_t off) { return adj + (off - L.off_lXA); };
   Rk3Layout RL;
   RL.np = o.n_passive; RL.nc = L.nc; RL.nu = L.nu; RL.nv = L.nv; RL.nw = L.nw;
   float *XA[16], *XB[16], *lXA[16], *lXB[16];
   state_ptrs(ws + L.off_XA, RL, XA);
   state_ptrs(ws + L.off_XB, RL, XB);
-  state_ptrs(ws + L.off_lXA, RL, lXA);
-  state_ptrs(ws + L.off_lXB, RL, lXB);
+  state_ptrs(A(L.off_lXA), RL, lXA);
+  state_ptrs(A(L.off_lXB), RL, lXB);
   float* lP[2][4];
-  acou_ptrs(ws + L.off_lP[0], RL, lP[0]);
-  acou_ptrs(ws + L.off_lP[1], RL, lP[1]);
+  acou_ptrs(A(L.off_lP[0]), RL, lP[0]);
+  acou_ptrs(A(L.off_lP[1]), RL, lP[1]);
   float* sm[10];
   float* lsm[10];
   for (int q = 0; q < 10; ++q) {
     sm[q] = (float*)(ws + L.off_sm + q * al256(L.nc * 4));
-    lsm[q] = (float*)(ws + L.off_lsm + q * al256(L.nc * 4));
+    lsm[q] = (float*)(A(L.off_lsm) + q * al256(L.nc * 4));
   }
-  float* lU = (float*)(ws + L.off_lU);
-  float* lV = (float*)(ws + L.off_lV);
-  float* lW = (float*)(ws + L.off_lW);
-  float* lru = (float*)(ws + L.off_lru);
-  float* lrv = (float*)(ws + L.off_lrv);
-  float* lrw = (float*)(ws + L.off_lrw);
-  float* ltr = (float*)(ws + L.off_ltr);
-  float* lsu = (float*)(ws + L.off_lsu);
-  float* lsv = (float*)(ws + L.off_lsv);
-  float* lsw = (float*)(ws + L.off_lsw);
+  float* lU = (float*)(A(L.off_lU));
+  float* lV = (float*)(A(L.off_lV));
+  float* lW = (float*)(A(L.off_lW));
+  float* lru = (float*)(A(L.off_lru));
+  float* lrv = (float*)(A(L.off_lrv));
+  float* lrw = (float*)(A(L.off_lrw));
+  float* ltr = (float*)(A(L.off_ltr));
+  float* lsu = (float*)(A(L.off_lsu));
+  float* lsv = (float*)(A(L.off_lsv));
+  float* lsw = (float*)(A(L.off_lsw));
   float* lss[10];
-  for (int q = 0; q < nsc; ++q) lss[q] = (float*)(ws + L.off_lss + q * al256(L.ic_ * 4));
-  float* lT = (float*)(ws + L.off_lT);
-  double* lcg2 = (double*)(ws + L.off_lcg2);
+  for (int q = 0; q < nsc; ++q) lss[q] = (float*)(A(L.off_lss) + q * al256(L.ic_ * 4));
+  float* lT = (float*)(A(L.off_lT));
+  double* lcg2 = (double*)(A(L.off_lcg2));
   const float* rc = b.rho0;
   const bool turb = o.turb != 0;
   const double dtd = (double)o.dt;

@@ -645,8 +645,61 @@ int lex_rk3_step_vjp(const lex_grid_t* g, const lex_step_opts_t* o, const lex_ba
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (ws_bytes < L.total) return fail(-3, "lex_rk3_step_vjp: workspace too small%s");
-  int rc = rk3_step_vjp(d, *o, *b, state, lam_out, lam_in, lam_cs, (char*)ws, L, S(stream));
+  int rc = rk3_step_vjp(d, *o, *b, state, lam_out, lam_in, lam_cs, (char*)ws, (char*)ws + L.off_lXA, L, S(stream));
   if (rc > 0) return fail(rc, "lex_rk3_step_vjp: HIP error %s", hipGetErrorString((hipError_t)rc));
+  return 0;
+}
+
+size_t lex_rk3_tape_bytes(const lex_grid_t* g, float dt, int n_sound, int n_passive) {
+  if (!grid_ok(g) || n_passive < 0 || n_passive > 8 || n_sound < 1 || !(dt > 0.f)) return 0;
+  Dims d = mkdims(g);
+  return vjp_layout(d, (double)dt, n_sound, n_passive).off_lXA;
+}
+
+size_t lex_rk3_adjoint_bytes(const lex_grid_t* g, float dt, int n_sound, int n_passive) {
+  if (!grid_ok(g) || n_passive < 0 || n_passive > 8 || n_sound < 1 || !(dt > 0.f)) return 0;
+  Dims d = mkdims(g);
+  VjpLayout L = vjp_layout(d, (double)dt, n_sound, n_passive);
+  return L.total - L.off_lXA;
+}
+
+int lex_rk3_step_tape(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b, float* const* state,
+                      float* const* state_out, void* tape, size_t tape_bytes, void* stream) {
+  REQGRID(g);
+  REQ(o); REQ(b); REQ(state); REQ(state_out); REQ(tape);
+  REQ(b->rho0); REQ(b->theta0); REQ(b->qv0); REQ(b->pi0);
+  if (o->n_passive < 0 || o->n_passive > 8) return fail(-1, "lex_rk3_step_tape: n_passive must be 0..8%s");
+  if (o->n_sound < 1 || !(o->dt > 0.f)) return fail(-1, "lex_rk3_step_tape: need n_sound >= 1 and dt > 0%s");
+  for (int q = 0; q < 6 + o->n_passive; ++q) {
+    if (!state[q] || !state_out[q]) return fail(-2, "lex_rk3_step_tape: null field pointer%s");
+    if (state[q] == state_out[q]) return fail(-1, "lex_rk3_step_tape: state_out must not alias state%s");
+  }
+  Dims d = mkdims(g);
+  VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
+  if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_tape: tape too small%s");
+  rk3_forward_tape(d, *o, *b, state, (char*)tape, L, S(stream), state_out);
+  return check_launch("lex_rk3_step_tape");
+}
+
+int lex_rk3_step_vjp_taped(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b,
+                           float* const* state, const void* tape, size_t tape_bytes, float* const* lam_out,
+                           float* const* lam_in, double* lam_cs, void* adj, size_t adj_bytes, void* stream) {
+  REQGRID(g);
+  REQ(o); REQ(b); REQ(state); REQ(tape); REQ(lam_out); REQ(lam_in); REQ(adj);
+  REQ(b->rho0); REQ(b->theta0); REQ(b->qv0); REQ(b->pi0);
+  if (o->n_passive < 0 || o->n_passive > 8) return fail(-1, "lex_rk3_step_vjp_taped: n_passive must be 0..8%s");
+  if (o->n_sound < 1 || !(o->dt > 0.f)) return fail(-1, "lex_rk3_step_vjp_taped: need n_sound >= 1 and dt > 0%s");
+  for (int q = 0; q < 6 + o->n_passive; ++q) {
+    if (!state[q] || !lam_out[q] || !lam_in[q]) return fail(-2, "lex_rk3_step_vjp_taped: null field pointer%s");
+    if (lam_in[q] == lam_out[q] || lam_in[q] == state[q])
+      return fail(-1, "lex_rk3_step_vjp_taped: lam_in must not alias lam_out or the state%s");
+  }
+  Dims d = mkdims(g);
+  VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
+  if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_vjp_taped: tape too small%s");
+  if (adj_bytes < L.total - L.off_lXA) return fail(-3, "lex_rk3_step_vjp_taped: adjoint workspace too small%s");
+  int rc = rk3_step_vjp(d, *o, *b, state, lam_out, lam_in, lam_cs, (char*)tape, (char*)adj, L, S(stream), false);
+  if (rc > 0) return fail(rc, "lex_rk3_step_vjp_taped: HIP error %s", hipGetErrorString((hipError_t)rc));
   return 0;
 }
 

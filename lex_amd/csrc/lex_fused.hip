@@ -20,6 +20,34 @@ namespace lex {
 constexpr int kFTY = 4;    // rows of y per block
 constexpr int kFTX = 16;   // cells marched in x per block
 constexpr int kMaxScalars = 8;
+constexpr int kSlots = 3;  // ceil((kFTY + 6) * L / (64 * ceil(nz / 64) * kFTY)) <= 3 for L <= nz + 3
+
+// Per-thread plane-load slots of a (kFTY + 6) x L plane (rows j0-3.. of the padded
+// field, clamped to the last padded row), fixed for the whole x-march: offsets within
+// one x-plane of a field with row stride `rs`.
+struct PlaneSlots {
+  int off[kSlots];
+  int dst[kSlots];
+  int n;
+};
+__device__ __forceinline__ PlaneSlots plane_slots(int tid, int nt, int L, int j0, int row_max, int rs) {
+  PlaneSlots p;
+  p.n = 0;
+#pragma unroll
+  for (int q = 0; q < kSlots; ++q) {
+    int e = tid + q * nt;
+    int row = e / L, col = e - row * L;
+    p.off[q] = min(j0 + row, row_max) * rs + col;
+    p.dst[q] = e;
+    if (e < (kFTY + 6) * L) p.n = q + 1;
+  }
+  return p;
+}
+__device__ __forceinline__ void plane_load(float* __restrict__ P, const float* __restrict__ plane, const PlaneSlots& ps) {
+#pragma unroll
+  for (int q = 0; q < kSlots; ++q)
+    if (q < ps.n) P[ps.dst[q]] = plane[ps.off[q]];
+}
 
 struct ScalarSet {
   const float* s0[kMaxScalars];   // xi0 fields (padded)
@@ -57,6 +85,7 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
   const float r8lo = (Kc <= 0 || Kc >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc - 1] + rc[Kc]);
   const float r8hi = (Kc + 1 >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc] + rc[Kc + 1]);
 
+  const PlaneSlots psl = plane_slots(tid, nt, Lz, j0, d.ny + 5, (int)sy);
   // x windows q[I-2..I+3] per scalar and the lower x-face flux
   float win[NS][6];
   float fxlo[NS];
@@ -98,11 +127,7 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
       const float* s = S.s[q];
       const float weps = S.weps[q];
       // plane of this scalar at x = I: rows J = j0 .. j0+kFTY+5 (padded), all levels
-      for (int e = tid; e < (kFTY + 6) * Lz; e += nt) {
-        int row = e / Lz, col = e - row * Lz;
-        int Jr = min(j0 + row, d.ny + 5);
-        P[e] = s[ic(d, I, Jr, col)];
-      }
+      plane_load(P, s + (long)I * sx, psl);
       __syncthreads();
       const float* Pc = P + (ty + 3) * Lz;   // this row's column
       const float qc = Pc[Kc];
@@ -130,7 +155,7 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
         float tend = (conv + qc * fd) / r;
         float nxt = S.s0[q][c] + tend * dt;
         if (S.sgs[q]) nxt = nxt + S.sgs[q][((long)i * d.ny + j) * d.nz + k] * dt;
-        store_c(d, S.out[q], i, j, k, nxt);
+        store_c_fast(d, S.out[q], i, j, k, nxt);
       }
       fxlo[q] = fxhi;
       // advance the window: q[I-1..I+4]
@@ -138,7 +163,7 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
       for (int t = 0; t < 5; ++t) win[q][t] = win[q][t + 1];
       win[q][5] = (i + 1 < iend) ? s[c + 4 * sx] : 0.0f;
     }
-    if (act && trho) store_c(d, trho, i, j, k, theta_rho(thc, qvc));
+    if (act && trho) store_c_fast(d, trho, i, j, k, theta_rho(thc, qvc));
     ulo = uhi;
   }
 }
@@ -179,6 +204,7 @@ __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict_
   const long sx = d.sxu();
   const float r = rc[Kc];
   const int iend = min(i0 + kFTX, d.nx);
+  const PlaneSlots psl = plane_slots(tid, nt, Lz, j0, d.ny + 5, (int)d.syc());
   float win[6];
   float fxlo, fdlo;
   {
@@ -197,10 +223,7 @@ __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict_
     const float mx = 0.5f * (win[2] + win[3]) * r;   // x-flux at cell I
     const float fxhi = mx * hrecon(win[0], win[1], win[2], win[3], win[4], win[5], upwind_pos(mx), weps);
     const float fdhi = fd_cell(d, rc, u, v, w, I, Jc, Kc);
-    for (int e = tid; e < (kFTY + 6) * Lz; e += nt) {
-      int row = e / Lz, col = e - row * Lz;
-      P[e] = u[iu(d, I, min(j0 + row, d.ny + 5), col)];
-    }
+    plane_load(P, u + (long)I * sx, psl);
     __syncthreads();
     const float* Pc = P + (ty + 3) * Lz;
     if (act) {
@@ -263,6 +286,7 @@ __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict_
   const long sx = d.sxv();
   const float r = rc[Kc];
   const int iend = min(i0 + kFTX, d.nx);
+  const PlaneSlots psl = plane_slots(tid, nt, Lz, j0, d.ny + 6, (int)d.syc());
   float win[6];
   float fxlo;
   {
@@ -281,10 +305,7 @@ __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict_
     const float fxhi = mx * hrecon(win[0], win[1], win[2], win[3], win[4], win[5], upwind_pos(mx), weps);
     const float fd0 = fd_cell(d, rc, u, v, w, I, Jc - 1, Kc);
     const float fd1 = fd_cell(d, rc, u, v, w, I, Jc, Kc);
-    for (int e = tid; e < (kFTY + 6) * Lz; e += nt) {
-      int row = e / Lz, col = e - row * Lz;
-      P[e] = v[iv(d, I, min(j0 + row, d.ny + 6), col)];
-    }
+    plane_load(P, v + (long)I * sx, psl);
     __syncthreads();
     const float* Pc = P + (ty + 3) * Lz;
     if (act) {
@@ -353,6 +374,7 @@ __global__ void __launch_bounds__(1024) k_mom_w(Dims d, const float* __restrict_
   const bool interior = act && l >= 1;              // levels 1..nz-1 carry advection
   const long sx = d.sxw();
   const int iend = min(i0 + kFTX, d.nx);
+  const PlaneSlots psl = plane_slots(tid, nt, Lw, j0, d.ny + 5, (int)d.syw());
   float win[6];
   float fxlo;
   {
@@ -369,10 +391,7 @@ __global__ void __launch_bounds__(1024) k_mom_w(Dims d, const float* __restrict_
     const float wc = win[2];
     const float mx = 0.5f * (rc[Kc - 1] * u[iu(d, I + 1, Jc, Kc - 1)] + rc[Kc] * u[iu(d, I + 1, Jc, Kc)]);
     const float fxhi = mx * hrecon(win[0], win[1], win[2], win[3], win[4], win[5], upwind_pos(mx), weps);
-    for (int e = tid; e < (kFTY + 6) * Lw; e += nt) {
-      int row = e / Lw, col = e - row * Lw;
-      P[e] = w[iw(d, I, min(j0 + row, d.ny + 5), col)];
-    }
+    plane_load(P, w + (long)I * sx, psl);
     if (act) {
       FD[ty * nzl + l] = fd_cell(d, rc, u, v, w, I, Jc, K);
       long c = ic(d, I, Jc, K);

@@ -494,12 +494,12 @@ __global__ void k_acoustic(Dims d, const float* __restrict__ rc, const float* __
   float piadv = pi0c[K] + pc;
   float dpdt = (float)(-kRd / kCv) * (piadv * div) + advp;
   float pn = p[c] + 1.1f * dpdt * dts;
-  store_c(d, po, i, j, k, pn);
-  store_u(d, uo, i, j, k, un0);
+  store_c_fast(d, po, i, j, k, pn);
+  store_u_fast(d, uo, i, j, k, un0);
   if (i == d.nx - 1) store_u(d, uo, d.nx, j, k, un1);
-  store_v(d, vo, i, j, k, vn0);
+  store_v_fast(d, vo, i, j, k, vn0);
   if (j == d.ny - 1) store_v(d, vo, i, d.ny, k, vn1);
-  store_w(d, wo, i, j, k, wn0);
+  store_w_fast(d, wo, i, j, k, wn0);
   if (k == d.nz - 1) store_w(d, wo, i, j, d.nz, wn1);
 }
 
@@ -527,7 +527,7 @@ __global__ void k_scalar_update(Dims d, const float* __restrict__ rc, const floa
     t = t + sgs[o];
   }
   if (tend_out) tend_out[o] = t;
-  store_c(d, out, i, j, k, nxt);
+  store_c_fast(d, out, i, j, k, nxt);
 }
 
 __global__ void k_momentum_update(Dims d, const float* __restrict__ u0, const float* __restrict__ v0,

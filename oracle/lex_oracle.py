@@ -1076,6 +1076,29 @@ def coarsen_block_mean(a, fx, fy, fz):
     return a.reshape(nx // fx, fx, ny // fy, fy, nz // fz, fz).mean(axis=(1, 3, 5))
 
 
+def coarsen_face_mean(a, axis, fx, fy, fz):
+    """Face-area mean of a staggered interior velocity onto the coarse faces: u (nx+1, ny, nz)
+    with axis 0, v (nx, ny+1, nz) with axis 1, w (nx, ny, nz+1) with axis 2.  The coarse
+    face n is fine face n*f of the staggered axis, averaged over the f_other1*f_other2
+    fine faces it covers.  (No reference routine: the reference coarsens theta only,
+    initial_comparison/plot_theta_pert_coarsened.m:7-8; this restriction keeps the
+    block-mean of the divergence, see tests.)"""
+    f = (fx, fy, fz)
+    sl = [slice(None)] * 3
+    sl[axis] = slice(0, None, f[axis])
+    b = a[tuple(sl)]
+    shp = list(b.shape)
+    red = []
+    new = []
+    for ax in range(3):
+        if ax == axis:
+            new.append(shp[ax])
+        else:
+            new += [shp[ax] // f[ax], f[ax]]
+            red.append(len(new) - 1)
+    return b.reshape(new).mean(axis=tuple(red))
+
+
 def n_acoustic_total(opts):
     """Acoustic substeps per full step: 1 + round(n/2) + n (solver_opt 2)."""
     return 1 + _n_acoustic(1, opts.dt / 2.0, opts)[0] + _n_acoustic(2, opts.dt, opts)[0]

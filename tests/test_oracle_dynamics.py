@@ -237,6 +237,26 @@ def test_block_mean_coarsening():
                 assert abs(c[I, J, K] - a[6 * I:6 * I + 6, 3 * J:3 * J + 3, 3 * K:3 * K + 3].mean()) < 1e-14
 
 
+def test_face_mean_coarsening_preserves_block_mean_divergence():
+    """div_coarse(face-mean(u, v, w)) == block-mean(div_fine(u, v, w)) exactly (telescoping
+    sums of the fine fluxes across each coarse block), for a non-periodic w with lids."""
+    rng = np.random.default_rng(6)
+    nx, ny, nz, f = 12, 18, 9, (6, 3, 3)
+    dx, dy, dz = 100.0, 120.0, 80.0
+    u = rng.normal(size=(nx + 1, ny, nz))
+    v = rng.normal(size=(nx, ny + 1, nz))
+    w = rng.normal(size=(nx, ny, nz + 1))
+    div_f = np.diff(u, axis=0) / dx + np.diff(v, axis=1) / dy + np.diff(w, axis=2) / dz
+    uc = o.coarsen_face_mean(u, 0, *f)
+    vc = o.coarsen_face_mean(v, 1, *f)
+    wc = o.coarsen_face_mean(w, 2, *f)
+    assert uc.shape == (nx // 6 + 1, ny // 3, nz // 3) and wc.shape == (nx // 6, ny // 3, nz // 3 + 1)
+    div_c = np.diff(uc, axis=0) / (6 * dx) + np.diff(vc, axis=1) / (3 * dy) + np.diff(wc, axis=2) / (3 * dz)
+    np.testing.assert_allclose(div_c, o.coarsen_block_mean(div_f, *f), rtol=0, atol=1e-12)
+    # the coarse face value is the plain mean of the fine faces it covers
+    assert abs(uc[1, 2, 1] - u[6, 6:9, 3:6].mean()) < 1e-14
+
+
 @pytest.mark.slow
 def test_config1_n_sound_6_is_unstable_and_7_is_not():
     """The acoustic Courant number of config 1 (dx=100 m, dt=1 s) at n_sound=6 is ~1.0:
