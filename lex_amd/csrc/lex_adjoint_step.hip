@@ -169,13 +169,47 @@ void rk3_forward_tape(const Dims& d, const O& o, const B& b, float* const* X0, c
         hipLaunchKernelGGL(k_smag_scalar, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, rc, sm[9], src[fi], ss[q]);
       }
     }
-    hipLaunchKernelGGL(k_div_rho, cx.G(d.nz + 2, d.ny + 6, d.nx + 6), cx.bk, 0, st, d, rc, src[1], src[2], src[3], fd);
-    hipLaunchKernelGGL(k_rhs_uv, cx.G(d.nz, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, src[1], src[2], src[3], fd,
-                       turb ? (const float*)su : nullptr, turb ? (const float*)sv : nullptr, ru, rv);
-    hipLaunchKernelGGL(k_rhs_w, cx.G(d.nz + 1, d.ny, d.nx), cx.bk, 0, st, d, rc, b.theta0, b.qv0, src[1], src[2],
-                       src[3], src[0], src[5], fd, turb ? (const float*)sw : nullptr, rw);
-    long n = (long)L.nc;
-    hipLaunchKernelGGL(k_theta_rho, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, src[0], src[5], tr);
+    // the fused stage kernels of the forward step where they apply (same trajectory as
+    // lex_rk3_step impl 0), else one kernel per reference routine
+    const int nzl = 64 * ((d.nz + 63) / 64);
+    const bool fused = nzl * kFTY <= 1024 && nsc >= 2 && nsc <= 6;
+    if (fused) {
+      dim3 fb(nzl, kFTY, 1), fg((d.nx + kFTX - 1) / kFTX, (d.ny + kFTY - 1) / kFTY, 1);
+      hipLaunchKernelGGL(k_mom_u, fg, fb, mom_uv_lds(d), st, d, rc, src[1], src[2], src[3],
+                         turb ? (const float*)su : nullptr, ru);
+      hipLaunchKernelGGL(k_mom_v, fg, fb, mom_uv_lds(d), st, d, rc, src[1], src[2], src[3],
+                         turb ? (const float*)sv : nullptr, rv);
+      hipLaunchKernelGGL(k_mom_w, fg, fb, mom_w_lds(d), st, d, rc, b.theta0, b.qv0, src[1], src[2], src[3], src[0],
+                         src[5], turb ? (const float*)sw : nullptr, rw);
+      ScalarSet S;
+      S.n = nsc;
+      for (int q = 0; q < nsc; ++q) {
+        int fi = (q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2));
+        S.s0[q] = X0[fi];
+        S.s[q] = src[fi];
+        S.sgs[q] = turb ? ss[q] : nullptr;
+        S.out[q] = dst[fi];
+        S.weps[q] = (q == 0) ? 1.0e-17f : 1.0e-18f;
+      }
+      size_t lds = stage_scalars_lds(d);
+      float dts_f = (float)dt_s;
+      switch (nsc) {
+        case 2: hipLaunchKernelGGL(k_stage_scalars<2>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
+        case 3: hipLaunchKernelGGL(k_stage_scalars<3>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
+        case 4: hipLaunchKernelGGL(k_stage_scalars<4>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
+        case 5: hipLaunchKernelGGL(k_stage_scalars<5>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
+        default: hipLaunchKernelGGL(k_stage_scalars<6>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
+      }
+    } else {
+      hipLaunchKernelGGL(k_div_rho, cx.G(d.nz + 2, d.ny + 6, d.nx + 6), cx.bk, 0, st, d, rc, src[1], src[2], src[3],
+                         fd);
+      hipLaunchKernelGGL(k_rhs_uv, cx.G(d.nz, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, src[1], src[2], src[3], fd,
+                         turb ? (const float*)su : nullptr, turb ? (const float*)sv : nullptr, ru, rv);
+      hipLaunchKernelGGL(k_rhs_w, cx.G(d.nz + 1, d.ny, d.nx), cx.bk, 0, st, d, rc, b.theta0, b.qv0, src[1], src[2],
+                         src[3], src[0], src[5], fd, turb ? (const float*)sw : nullptr, rw);
+      long n = (long)L.nc;
+      hipLaunchKernelGGL(k_theta_rho, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, src[0], src[5], tr);
+    }
     const int nsub = L.nsub[s];
     const float dts = (s == 0) ? (float)dt_s : (float)dts_small;
     float* in[4] = {X0[1], X0[2], X0[3], X0[4]};
@@ -190,12 +224,14 @@ void rk3_forward_tape(const Dims& d, const O& o, const B& b, float* const* X0, c
                          src[4], ru, rv, rw, tr, out[0], out[1], out[2], out[3], dts);
       for (int c = 0; c < 4; ++c) in[c] = out[c];
     }
-    for (int q = 0; q < nsc; ++q) {
-      int fi = (q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2));
-      float weps = (q == 0) ? 1.0e-17f : 1.0e-18f;
-      hipLaunchKernelGGL(k_scalar_update, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, rc, X0[fi], src[fi], src[1],
-                         src[2], src[3], fd, (const float*)nullptr, (const float*)nullptr,
-                         turb ? (const float*)ss[q] : nullptr, dst[fi], (float*)nullptr, (float)dt_s, weps);
+    if (!fused) {
+      for (int q = 0; q < nsc; ++q) {
+        int fi = (q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2));
+        float weps = (q == 0) ? 1.0e-17f : 1.0e-18f;
+        hipLaunchKernelGGL(k_scalar_update, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, rc, X0[fi], src[fi], src[1],
+                           src[2], src[3], fd, (const float*)nullptr, (const float*)nullptr,
+                           turb ? (const float*)ss[q] : nullptr, dst[fi], (float*)nullptr, (float)dt_s, weps);
+      }
     }
   }
 }

@@ -456,8 +456,8 @@ int lex_rk3_step(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_c
   Dims d = mkdims(g);
   Rk3Layout L = rk3_layout(d, o->n_passive);
   if (ws_bytes < L.total) return fail(-3, "lex_rk3_step: workspace too small%s");
-  if (o->impl != 0 && o->impl != 1) return fail(-1, "lex_rk3_step: impl must be 0 or 1%s");
-  if (o->impl == 0 && (64 * ((g->nz + 63) / 64)) * kFTY > 1024)
+  if (o->impl < 0 || o->impl > 2) return fail(-1, "lex_rk3_step: impl must be 0, 1 or 2%s");
+  if (o->impl != 1 && (64 * ((g->nz + 63) / 64)) * kFTY > 1024)
     return fail(-1, "lex_rk3_step: the fused kernels need nz <= 256 (use impl 1)%s");
   int rc = rk3_step(d, *o, *b, state, (char*)ws, L, S(stream));
   if (rc < 0) return fail(rc, "lex_rk3_step: unsupported configuration%s");

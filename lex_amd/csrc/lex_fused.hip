@@ -25,20 +25,23 @@ constexpr int kSlots = 3;  // ceil((kFTY + 6) * L / (64 * ceil(nz / 64) * kFTY))
 // Per-thread plane-load slots of a (kFTY + 6) x L plane (rows j0-3.. of the padded
 // field, clamped to the last padded row), fixed for the whole x-march: offsets within
 // one x-plane of a field with row stride `rs`.
+// Slot q of thread tid is plane element tid + q * nt.
 struct PlaneSlots {
   int off[kSlots];
-  int dst[kSlots];
   int n;
+  int tid, nt;
+  __device__ __forceinline__ int dst(int q) const { return tid + q * nt; }
 };
 __device__ __forceinline__ PlaneSlots plane_slots(int tid, int nt, int L, int j0, int row_max, int rs) {
   PlaneSlots p;
   p.n = 0;
+  p.tid = tid;
+  p.nt = nt;
 #pragma unroll
   for (int q = 0; q < kSlots; ++q) {
     int e = tid + q * nt;
     int row = e / L, col = e - row * L;
     p.off[q] = min(j0 + row, row_max) * rs + col;
-    p.dst[q] = e;
     if (e < (kFTY + 6) * L) p.n = q + 1;
   }
   return p;
@@ -46,7 +49,7 @@ __device__ __forceinline__ PlaneSlots plane_slots(int tid, int nt, int L, int j0
 __device__ __forceinline__ void plane_load(float* __restrict__ P, const float* __restrict__ plane, const PlaneSlots& ps) {
 #pragma unroll
   for (int q = 0; q < kSlots; ++q)
-    if (q < ps.n) P[ps.dst[q]] = plane[ps.off[q]];
+    if (q < ps.n) P[ps.dst(q)] = plane[ps.off[q]];
 }
 
 struct ScalarSet {
@@ -463,6 +466,147 @@ inline size_t mom_w_lds(const Dims& d) {
 inline size_t stage_scalars_lds(const Dims& d) {
   int nzl = 64 * ((d.nz + 63) / 64);
   return sizeof(float) * ((size_t)(kFTY + 6) * (d.nz + 2) + (size_t)(kFTY + 1) * nzl + (size_t)kFTY * nzl);
+}
+
+// ---------------------------------------------------------------------------------
+// k_stage_scalars_db: the same arithmetic as k_stage_scalars, reorganised for latency:
+// the planes of all NS scalars at x = I+1 are fetched into registers at the start of
+// step I and written to the second of two LDS plane buffers after the step's face
+// fluxes, so one x-step takes two barriers for all scalars (k_stage_scalars: two per
+// scalar) and the global loads overlap the WENO arithmetic.
+// LDS: 2 x NS planes + NS x (2 kFTY + 1) face rows.
+// ---------------------------------------------------------------------------------
+inline size_t stage_scalars_db_lds(const Dims& d, int ns) {
+  int nzl = 64 * ((d.nz + 63) / 64);
+  return sizeof(float) * ((size_t)2 * ns * (kFTY + 6) * (d.nz + 2) + (size_t)ns * (2 * kFTY + 1) * nzl);
+}
+
+template <int NS>
+__global__ void __launch_bounds__(512) k_stage_scalars_db(Dims d, const float* __restrict__ rc,
+                                                            const float* __restrict__ u, const float* __restrict__ v,
+                                                            const float* __restrict__ w, ScalarSet S,
+                                                            float* __restrict__ trho, float dt) {
+  extern __shared__ float lds[];
+  const int nzl = blockDim.x;
+  const int tz = threadIdx.x, ty = threadIdx.y;
+  const int nt = nzl * kFTY;
+  const int tid = ty * nzl + tz;
+  const int Lz = d.nz + 2;
+  const int PL = (kFTY + 6) * Lz;
+  float* Pb = lds;                              // [2][NS][PL]
+  float* FYb = Pb + 2 * NS * PL;                // [NS][(kFTY+1) * nzl]
+  float* FZb = FYb + NS * (kFTY + 1) * nzl;     // [NS][kFTY * nzl]
+
+  const int j0 = blockIdx.y * kFTY;
+  const int i0 = blockIdx.x * kFTX;
+  const int k = tz, j = j0 + ty;
+  const bool act = (k < d.nz) && (j < d.ny);
+  const int K = k + 1;
+  const int Kc = act ? K : 1, Jc = (j < d.ny) ? j + 3 : 3;
+  const bool last_row = (ty == kFTY - 1) || (j == d.ny - 1);
+  const long sx = d.sxc(), sy = d.syc();
+  const float r = rc[Kc];
+  const float r8lo = (Kc <= 0 || Kc >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc - 1] + rc[Kc]);
+  const float r8hi = (Kc + 1 >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc] + rc[Kc + 1]);
+  const float rz = 0.5f * (rc[Kc - 1] + rc[Kc]);
+  const PlaneSlots psl = plane_slots(tid, nt, Lz, j0, d.ny + 5, (int)sy);
+  const int iend = min(i0 + kFTX, d.nx);
+
+  float win[NS][6];
+  float fxlo[NS];
+  {
+    const int I = i0 + 3;
+    const float mlo = r * u[iu(d, I, Jc, Kc)];
+    const bool pos = upwind_pos(mlo);
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      const float* s = S.s[q];
+      long c = ic(d, I, Jc, Kc);
+      float m3 = s[c - 3 * sx];
+#pragma unroll
+      for (int t = 0; t < 6; ++t) win[q][t] = s[c + (t - 2) * sx];
+      fxlo[q] = mlo * hrecon(m3, win[q][0], win[q][1], win[q][2], win[q][3], win[q][4], pos, S.weps[q]);
+      plane_load(Pb + q * PL, s + (long)I * sx, psl);
+    }
+  }
+  __syncthreads();
+  float ulo = u[iu(d, i0 + 3, Jc, Kc)];
+  int b = 0;
+  for (int i = i0; i < iend; ++i) {
+    const int I = i + 3;
+    const long c = ic(d, I, Jc, Kc);
+    const bool more = (i + 1 < iend);
+    // prefetch the next x-plane of every scalar
+    float pre[NS][kSlots];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      const float* nxt = S.s[q] + (long)(I + 1) * sx;
+#pragma unroll
+      for (int t = 0; t < kSlots; ++t) pre[q][t] = (more && t < psl.n) ? nxt[psl.off[t]] : 0.0f;
+    }
+    const float uhi = u[iu(d, I + 1, Jc, Kc)];
+    const float vlo = v[iv(d, I, Jc, Kc)];
+    const float vhi = v[iv(d, I, Jc + 1, Kc)];
+    const float wlo = w[iw(d, I, Jc, Kc)];
+    const float whi = w[iw(d, I, Jc, Kc + 1)];
+    const float fd = (r * uhi - r * ulo) * d.idx + (r * vhi - r * vlo) * d.idy + (r8hi * whi - r8lo * wlo) * d.idz;
+    const float mx = r * uhi, my_lo = r * vlo, my_hi = r * vhi, mz_lo = rz * wlo;
+    const bool px = upwind_pos(mx), py0 = upwind_pos(my_lo), py1 = upwind_pos(my_hi), pz = upwind_pos(mz_lo);
+    float qcs[NS], fxhi[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      const float weps = S.weps[q];
+      const float* Pc = Pb + (b * NS + q) * PL + (ty + 3) * Lz;
+      qcs[q] = Pc[Kc];
+      if (act) {
+        FZb[(q * kFTY + ty) * nzl + k] = (k == 0) ? 0.0f : mz_lo * vrecon(Pc, Lz, K - 1, pz, weps);
+        FYb[(q * (kFTY + 1) + ty) * nzl + k] =
+            my_lo * hrecon(Pc[Kc - 3 * Lz], Pc[Kc - 2 * Lz], Pc[Kc - Lz], Pc[Kc], Pc[Kc + Lz], Pc[Kc + 2 * Lz], py0,
+                           weps);
+        if (last_row) {
+          const float* Pn = Pc + Lz;
+          FYb[(q * (kFTY + 1) + ty + 1) * nzl + k] =
+              my_hi * hrecon(Pn[Kc - 3 * Lz], Pn[Kc - 2 * Lz], Pn[Kc - Lz], Pn[Kc], Pn[Kc + Lz], Pn[Kc + 2 * Lz], py1,
+                             weps);
+        }
+      }
+      fxhi[q] = mx * hrecon(win[q][0], win[q][1], win[q][2], win[q][3], win[q][4], win[q][5], px, weps);
+    }
+    // next planes into the other buffer (its last readers finished before the previous barrier)
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      float* P = Pb + ((b ^ 1) * NS + q) * PL;
+#pragma unroll
+      for (int t = 0; t < kSlots; ++t)
+        if (more && t < psl.n) P[psl.dst(t)] = pre[q][t];
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        const float* FY = FYb + q * (kFTY + 1) * nzl;
+        const float* FZ = FZb + q * kFTY * nzl;
+        float fzhi = (k == d.nz - 1) ? 0.0f : FZ[ty * nzl + k + 1];
+        float conv = -((fxhi[q] - fxlo[q]) * d.idx + (FY[(ty + 1) * nzl + k] - FY[ty * nzl + k]) * d.idy +
+                       (fzhi - FZ[ty * nzl + k]) * d.idz);
+        float tend = (conv + qcs[q] * fd) / r;
+        float nxt = S.s0[q][c] + tend * dt;
+        if (S.sgs[q]) nxt = nxt + S.sgs[q][((long)i * d.ny + j) * d.nz + k] * dt;
+        store_c_fast(d, S.out[q], i, j, k, nxt);
+      }
+      if (trho) store_c_fast(d, trho, i, j, k, theta_rho(qcs[0], qcs[1]));
+    }
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      fxlo[q] = fxhi[q];
+#pragma unroll
+      for (int t = 0; t < 5; ++t) win[q][t] = win[q][t + 1];
+      win[q][5] = more ? S.s[q][c + 4 * sx] : 0.0f;
+    }
+    ulo = uhi;
+    b ^= 1;
+    __syncthreads();   // the face rows are rewritten in the next step
+  }
 }
 
 }  // namespace lex

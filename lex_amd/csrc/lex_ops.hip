@@ -503,6 +503,106 @@ __global__ void k_acoustic(Dims d, const float* __restrict__ rc, const float* __
   if (k == d.nz - 1) store_w(d, wo, i, j, d.nz, wn1);
 }
 
+// k_acoustic_m: the same substep with each thread marching kAX cells along x: the x-face
+// velocity, pi', theta_rho and pi* of the upper neighbour are carried to the next cell
+// (one x-face update per cell instead of two), and the z neighbours of pi', theta_rho
+// and pi* come from the adjacent lanes of the wave.  Block 64 (z) x 4 (y).
+constexpr int kAX = 8;
+
+__global__ void __launch_bounds__(256) k_acoustic_m(Dims d, const float* __restrict__ rc, const float* __restrict__ pi0c,
+                                                    const float* __restrict__ u, const float* __restrict__ v,
+                                                    const float* __restrict__ w, const float* __restrict__ p,
+                                                    const float* __restrict__ ps, const float* __restrict__ ru,
+                                                    const float* __restrict__ rv, const float* __restrict__ rw,
+                                                    const float* __restrict__ tr, float* __restrict__ uo,
+                                                    float* __restrict__ vo, float* __restrict__ wo,
+                                                    float* __restrict__ po, float dts) {
+  const int lane = threadIdx.x;
+  const int k = blockIdx.z * 64 + lane;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i0 = blockIdx.x * kAX;
+  const bool act = (k < d.nz) && (j < d.ny);
+  const int kc = min(k, d.nz - 1), jc = min(j, d.ny - 1);   // clamped (idle lanes load valid data)
+  const int J = jc + 3, K = kc + 1;
+  const long sx = d.sxc(), sy = d.syc();
+  const float h = (float)(-kCp * 0.5);
+  const float cr = (float)(-kRd / kCv);
+  const float r = rc[K];
+  const float r8w0 = 0.5f * (rc[K - 1] + rc[K]);
+  const float r8w1 = 0.5f * (rc[K] + rc[K + 1]);
+  const float dpi0lo = (pi0c[K] - pi0c[K - 1]) * d.idz;
+  const float dpi0hi = (pi0c[K + 1] - pi0c[K]) * d.idz;
+  const float pi0k = pi0c[K];
+  const int iend = min(i0 + kAX, d.nx);
+  // carried x-direction values at cell I (and ps at I-1) and the new velocity of face I
+  long c = ic(d, i0 + 3, J, K);
+  float p_c = p[c], tr_c = tr[c], ps_c = ps[c], ps_m = ps[c - sx];
+  float un0 = u[iu(d, i0 + 3, J, K)] +
+              (h * (tr_c + tr[c - sx]) * ((p_c - p[c - sx]) * d.idx) + ru[((long)i0 * d.ny + jc) * d.nz + kc]) * dts;
+  for (int i = i0; i < iend; ++i) {
+    const int I = i + 3;
+    c = ic(d, I, J, K);
+    // x: upper face I+1
+    const float p_p = p[c + sx], tr_p = tr[c + sx], ps_p = ps[c + sx];
+    const float un1 = u[iu(d, I + 1, J, K)] +
+                      (h * (tr_p + tr_c) * ((p_p - p_c) * d.idx) + ru[((long)(i + 1) * d.ny + jc) * d.nz + kc]) * dts;
+    // y: faces J and J+1
+    const float p_s = p[c - sy], p_n = p[c + sy], tr_s = tr[c - sy], tr_n = tr[c + sy];
+    const float ps_s = ps[c - sy], ps_n = ps[c + sy];
+    const float vn0 = v[iv(d, I, J, K)] +
+                      (h * (tr_c + tr_s) * ((p_c - p_s) * d.idy) + rv[((long)i * (d.ny + 1) + jc) * d.nz + kc]) * dts;
+    const float vn1 = v[iv(d, I, J + 1, K)] +
+                      (h * (tr_n + tr_c) * ((p_n - p_c) * d.idy) + rv[((long)i * (d.ny + 1) + jc + 1) * d.nz + kc]) * dts;
+    // z: neighbours from the adjacent lanes (lane 0 / 63 of the wave load their own)
+    float p_b = __shfl_up(p_c, 1, 64), tr_b = __shfl_up(tr_c, 1, 64), ps_b = __shfl_up(ps_c, 1, 64);
+    float p_t = __shfl_down(p_c, 1, 64), tr_t = __shfl_down(tr_c, 1, 64), ps_t = __shfl_down(ps_c, 1, 64);
+    if (lane == 0) {
+      p_b = p[c - 1];
+      tr_b = tr[c - 1];
+      ps_b = ps[c - 1];
+    }
+    if (lane == 63 || k >= d.nz - 1) {
+      p_t = p[c + 1];
+      tr_t = tr[c + 1];
+      ps_t = ps[c + 1];
+    }
+    // pgf_z is zero at the surface and lid levels, and set_w_bc zeroes the new w there
+    const float wn0 = (kc == 0) ? 0.0f
+                                : w[iw(d, I, J, K)] + (h * (tr_c + tr_b) * ((p_c - p_b) * d.idz) +
+                                                       rw[((long)i * d.ny + jc) * (d.nz + 1) + kc]) * dts;
+    const float wn1 = (kc == d.nz - 1)
+                          ? 0.0f
+                          : w[iw(d, I, J, K + 1)] + (h * (tr_t + tr_c) * ((p_t - p_c) * d.idz) +
+                                                     rw[((long)i * d.ny + jc) * (d.nz + 1) + kc + 1]) * dts;
+    const float div = (un1 - un0) * d.idx + (vn1 - vn0) * d.idy + (wn1 - wn0) * d.idz;
+    const float udx0 = un0 * ((ps_c - ps_m) * d.idx);
+    const float udx1 = un1 * ((ps_p - ps_c) * d.idx);
+    const float vdy0 = vn0 * ((ps_c - ps_s) * d.idy);
+    const float vdy1 = vn1 * ((ps_n - ps_c) * d.idy);
+    const float dz0 = dpi0lo + (ps_c - ps_b) * d.idz;
+    const float dz1 = dpi0hi + (ps_t - ps_c) * d.idz;
+    const float wdz0 = (kc == 0) ? 0.0f : wn0 * dz0;
+    const float wdz1 = (kc == d.nz - 1) ? 0.0f : wn1 * dz1;
+    const float advp = -(0.5f * (udx1 + udx0) + 0.5f * (vdy1 + vdy0) + (0.5f * wdz0 * r8w0 + 0.5f * wdz1 * r8w1) / r);
+    const float dpdt = cr * ((pi0k + ps_c) * div) + advp;
+    const float pn = p_c + 1.1f * dpdt * dts;
+    if (act) {
+      store_c_fast(d, po, i, j, k, pn);
+      store_u_fast(d, uo, i, j, k, un0);
+      if (i == d.nx - 1) store_u(d, uo, d.nx, j, k, un1);
+      store_v_fast(d, vo, i, j, k, vn0);
+      if (j == d.ny - 1) store_v(d, vo, i, d.ny, k, vn1);
+      store_w_fast(d, wo, i, j, k, wn0);
+      if (k == d.nz - 1) store_w(d, wo, i, j, d.nz, wn1);
+    }
+    un0 = un1;
+    ps_m = ps_c;
+    p_c = p_p;
+    tr_c = tr_p;
+    ps_c = ps_p;
+  }
+}
+
 // ----------------------------------------------------------------------------------
 // E1 / M1: Euler updates (one_step_integration.py:450-465, :876-891) and SGS add-in
 // (:113-118, :158-162): next = pad(base + tend*dt) [+ pad(sgs*dt)]

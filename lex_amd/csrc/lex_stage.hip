@@ -200,8 +200,12 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
     }
     const int nsc = 2 + o.n_passive;
     const bool fused_ok = (64 * ((d.nz + 63) / 64)) * kFTY <= 1024 && nsc >= 2 && nsc <= 6;
-    if (o.impl == 0 && !fused_ok) return -4;  // the caller falls back to impl 1
-    if (o.impl == 0) {
+    // impl 0: fused stage kernels; impl 2: the same with the double-buffered scalar kernel;
+    // impl 1: one kernel per reference routine
+    const bool fused = (o.impl == 0 || o.impl == 2);
+    if (fused && !fused_ok) return -4;  // the caller falls back to impl 1
+    if (o.impl == 2 && (stage_scalars_db_lds(d, nsc) > 65536 || 64 * ((d.nz + 63) / 64) * kFTY > 512)) return -4;
+    if (fused) {
       const int nzl = 64 * ((d.nz + 63) / 64);
       dim3 fb(nzl, kFTY, 1), fg((d.nx + kFTX - 1) / kFTX, (d.ny + kFTY - 1) / kFTY, 1);
       LEX_LAUNCH(KID_STAGE_MOMENTUM, k_mom_u, fg, fb, mom_uv_lds(d), st, d, rc, src[1], src[2], src[3],
@@ -217,7 +221,7 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
       LEX_LAUNCH(KID_RHS_W, k_rhs_w, G(d.nz + 1, d.ny, d.nx), bk, 0, st, d, rc, b.theta0, b.qv0, src[1], src[2], src[3],
                  src[0], src[5], fd, turb ? sw : nullptr, rw);
     }
-    if (o.impl == 0) {
+    if (fused) {
       // all scalars of the stage + theta_rho in one launch (k_stage_scalars)
       ScalarSet S;
       S.n = nsc;
@@ -233,7 +237,15 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
       dim3 fb(nzl, kFTY, 1), fg((d.nx + kFTX - 1) / kFTX, (d.ny + kFTY - 1) / kFTY, 1);
       size_t lds = stage_scalars_lds(d);
       float dts_f = (float)dt_s;
-      switch (nsc) {
+      if (o.impl == 2) {
+        size_t ldb = stage_scalars_db_lds(d, nsc);
+        switch (nsc) {
+          case 2: LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars_db<2>, fg, fb, ldb, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
+          case 3: LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars_db<3>, fg, fb, ldb, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
+          case 4: LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars_db<4>, fg, fb, ldb, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
+          default: return -4;
+        }
+      } else switch (nsc) {
         case 2: LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars<2>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
         case 3: LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars<3>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
         case 4: LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars<4>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
@@ -266,8 +278,14 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
         int pp = (in[0] == P[0][0]) ? 1 : 0;
         for (int c = 0; c < 4; ++c) out[c] = P[pp][c];
       }
-      LEX_LAUNCH(KID_ACOUSTIC, k_acoustic, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, b.pi0, in[0], in[1], in[2], in[3], src[4],
-                         ru, rv, rw, tr, out[0], out[1], out[2], out[3], dts);
+      if (o.impl == 2) {
+        dim3 ag((d.nx + kAX - 1) / kAX, (d.ny + 3) / 4, (d.nz + 63) / 64);
+        LEX_LAUNCH(KID_ACOUSTIC, k_acoustic_m, ag, dim3(64, 4, 1), 0, st, d, rc, b.pi0, in[0], in[1], in[2], in[3],
+                   src[4], ru, rv, rw, tr, out[0], out[1], out[2], out[3], dts);
+      } else {
+        LEX_LAUNCH(KID_ACOUSTIC, k_acoustic, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, b.pi0, in[0], in[1], in[2], in[3],
+                   src[4], ru, rv, rw, tr, out[0], out[1], out[2], out[3], dts);
+      }
       if (alias) {
         const size_t ns[4] = {L.nu, L.nv, L.nw, L.nc};
         for (int c = 0; c < 4; ++c)
@@ -277,7 +295,7 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
       for (int c = 0; c < 4; ++c) in[c] = out[c];
     }
     // scalars: theta (weps 1e-17), qv and passive (1e-18) (one_step_integration.py:905, :913)
-    if (o.impl != 0) {
+    if (!fused) {
       for (int q = 0; q < nsc; ++q) {
         int fi = (q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2));
         float weps = (q == 0) ? 1.0e-17f : 1.0e-18f;

@@ -80,10 +80,12 @@ def test_fused_equals_staged():
             assert rel_l2(dx, dy) < 1e-5, n
 
 
+@pytest.mark.parametrize("impl", [0, 2])
 @pytest.mark.parametrize("shape", [(24, 20, 24), (37, 23, 70), (16, 16, 128)], ids=lambda s: "x".join(map(str, s)))
-def test_fused_kernels_equal_per_operator_kernels(shape):
-    """impl 0 (fused stage kernels) vs impl 1 (one kernel per reference routine), incl.
-    ragged tiles (nx % 16, ny % 4, nz % 64 != 0), three steps."""
+def test_fused_kernels_equal_per_operator_kernels(shape, impl):
+    """impl 0 / 2 (fused stage kernels; 2 with the double-buffered scalar kernel) vs impl 1
+    (one kernel per reference routine), incl. ragged tiles (nx % 16, ny % 4, nz % 64 != 0),
+    three steps."""
     from lex_amd import one_sprint_functions as sp, _lib
     from lex_amd._core import grid_from
     phys, base, grids, mo, og, oo = _setup(1, *shape)
@@ -91,7 +93,7 @@ def test_fused_kernels_equal_per_operator_kernels(shape):
     a = tuple(t.clone() for t in phys)
     b = tuple(t.clone() for t in phys)
     for _ in range(3):
-        sp.fused_step(a, base, g, mo, impl=0)
+        sp.fused_step(a, base, g, mo, impl=impl)
         sp.fused_step(b, base, g, mo, impl=1)
     for n, x, y, x0 in zip(NAMES, a, b, phys):
         xa, ya = from_dev(x), from_dev(y)
