@@ -22,11 +22,14 @@ __device__ __forceinline__ void aadd(float* p, float v) { unsafeAtomicAdd(p, v);
 // R = weno5(a..e) and g[0..4] = dR/d(a, b, c, d, e).  With r_k = tau/(beta_k + eps),
 // w_k = d_k (1 + r_k^2), R = sum w_k q_k / W:
 //   dR/dr_k = (q_k - R) 2 d_k r_k / W,  dr_k/dtau = 1/(beta_k+eps),  dr_k/dbeta_k = -r_k/(beta_k+eps)
+// Evaluated on differences of neighbouring values (exact in FP32 for close values), so
+// fields with a large offset (theta ~ 300 K) keep their gradient accuracy.
 __device__ float weno5_grad(float a, float b, float c, float d, float e, float weps, float* g) {
   const float c1312 = (float)(13.0 / 12.0);
-  float t1 = a - 2.0f * b + c, t2 = a - 4.0f * b + 3.0f * c;
-  float t3 = b - 2.0f * c + d, t4 = b - d;
-  float t5 = c - 2.0f * d + e, t6 = 3.0f * c - 4.0f * d + e;
+  const float dab = a - b, dbc = b - c, dcd = c - d, dde = d - e;
+  float t1 = dab - dbc, t2 = dab - 3.0f * dbc;
+  float t3 = dbc - dcd, t4 = dbc + dcd;
+  float t5 = dcd - dde, t6 = 3.0f * dcd - dde;
   float b1 = c1312 * t1 * t1 + 0.25f * t2 * t2;
   float b2 = c1312 * t3 * t3 + 0.25f * t4 * t4;
   float b3 = c1312 * t5 * t5 + 0.25f * t6 * t6;
@@ -38,14 +41,17 @@ __device__ float weno5_grad(float a, float b, float c, float d, float e, float w
   float r1 = fminf(1.0e30f, x1), r2 = fminf(1.0e30f, x2), r3 = fminf(1.0e30f, x3);
   float w1 = 0.1f * (1.0f + r1 * r1), w2 = 0.6f * (1.0f + r2 * r2), w3 = 0.3f * (1.0f + r3 * r3);
   const float s6 = (float)(1.0 / 6.0);
-  float q1 = (float)(2.0 / 6.0) * a + (float)(-7.0 / 6.0) * b + (float)(11.0 / 6.0) * c;
-  float q2 = -s6 * b + (float)(5.0 / 6.0) * c + (float)(2.0 / 6.0) * d;
-  float q3 = (float)(2.0 / 6.0) * c + (float)(5.0 / 6.0) * d - s6 * e;
+  // candidates relative to the upwind value c: h_k = q_k - c
+  const float da = a - c, db = b - c, dd = d - c, de = e - c;
+  float h1 = (float)(2.0 / 6.0) * da + (float)(-7.0 / 6.0) * db;
+  float h2 = -s6 * db + (float)(2.0 / 6.0) * dd;
+  float h3 = (float)(5.0 / 6.0) * dd - s6 * de;
   float W = w1 + w2 + w3;
-  float R = (w1 * q1 + w2 * q2 + w3 * q3) / W;
-  float C1 = c1 ? 0.f : (q1 - R) * 0.2f * r1 / (W * e1);
-  float C2 = c2 ? 0.f : (q2 - R) * 1.2f * r2 / (W * e2);
-  float C3 = c3 ? 0.f : (q3 - R) * 0.6f * r3 / (W * e3);
+  float Rd = (w1 * h1 + w2 * h2 + w3 * h3) / W;
+  float R = c + Rd;
+  float C1 = c1 ? 0.f : (h1 - Rd) * 0.2f * r1 / (W * e1);
+  float C2 = c2 ? 0.f : (h2 - Rd) * 1.2f * r2 / (W * e2);
+  float C3 = c3 ? 0.f : (h3 - Rd) * 0.6f * r3 / (W * e3);
   float Cs = C1 + C2 + C3;
   float E1 = sg * Cs - C1 * r1, E2 = -C2 * r2, E3 = -sg * Cs - C3 * r3;  // dR/dbeta_k
   float A1 = 2.f * c1312 * t1, B1 = 0.5f * t2, A2 = 2.f * c1312 * t3, B2 = 0.5f * t4;
@@ -62,7 +68,8 @@ __device__ float weno5_grad(float a, float b, float c, float d, float e, float w
 
 // R = weno3(a, b, c) and g[0..2]
 __device__ float weno3_grad(float a, float b, float c, float weps, float* g) {
-  float b1 = (a - b) * (a - b), b2 = (b - c) * (b - c);
+  const float dab = a - b, dbc = b - c;
+  float b1 = dab * dab, b2 = dbc * dbc;
   float df = b1 - b2, tau = fabsf(df);
   float sg = (df > 0.f) ? 1.f : ((df < 0.f) ? -1.f : 0.f);
   float e1 = b1 + weps, e2 = b2 + weps;
@@ -70,16 +77,18 @@ __device__ float weno3_grad(float a, float b, float c, float weps, float* g) {
   bool c1 = !(x1 < 1.0e30f), c2 = !(x2 < 1.0e30f);
   float r1 = fminf(1.0e30f, x1), r2 = fminf(1.0e30f, x2);
   float w1 = (float)(1.0 / 3.0) * (1.f + r1 * r1), w2 = (float)(2.0 / 3.0) * (1.f + r2 * r2);
-  float q1 = -0.5f * a + 1.5f * b, q2 = 0.5f * b + 0.5f * c;
+  // candidates relative to the upwind value b: h1 = q1 - b, h2 = q2 - b
+  float h1 = -0.5f * dab, h2 = -0.5f * dbc;
   float W = w1 + w2;
-  float R = (w1 * q1 + w2 * q2) / W;
-  float C1 = c1 ? 0.f : (q1 - R) * (float)(2.0 / 3.0) * r1 / (W * e1);
-  float C2 = c2 ? 0.f : (q2 - R) * (float)(4.0 / 3.0) * r2 / (W * e2);
+  float Rd = (w1 * h1 + w2 * h2) / W;
+  float R = b + Rd;
+  float C1 = c1 ? 0.f : (h1 - Rd) * (float)(2.0 / 3.0) * r1 / (W * e1);
+  float C2 = c2 ? 0.f : (h2 - Rd) * (float)(4.0 / 3.0) * r2 / (W * e2);
   float E1 = sg * (C1 + C2) - C1 * r1, E2 = -sg * (C1 + C2) - C2 * r2;
   float W1 = w1 / W, W2 = w2 / W;
-  g[0] = E1 * 2.f * (a - b) - 0.5f * W1;
-  g[1] = -E1 * 2.f * (a - b) + E2 * 2.f * (b - c) + 1.5f * W1 + 0.5f * W2;
-  g[2] = -E2 * 2.f * (b - c) + 0.5f * W2;
+  g[0] = E1 * 2.f * dab - 0.5f * W1;
+  g[1] = -E1 * 2.f * dab + E2 * 2.f * dbc + 1.5f * W1 + 0.5f * W2;
+  g[2] = -E2 * 2.f * dbc + 0.5f * W2;
   return R;
 }
 

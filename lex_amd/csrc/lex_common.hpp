@@ -200,26 +200,51 @@ __device__ __forceinline__ void store_w(const Dims& d, float* __restrict__ a, in
       for (int c1 = 0; c1 < nzs; ++c1) a[iw(d, xs[a1], ys[b1], zs[c1])] = val;
 }
 
-// Fast paths: a cell away from every periodic image and z ghost has exactly one
-// destination; only the boundary cells take the general ghost lists above.
-__device__ __forceinline__ bool near_c(const Dims& d, int i, int j, int k) {
-  return i < 3 || i >= d.nx - 3 || j < 3 || j >= d.ny - 3 || k == 0 || k == d.nz - 1;
+// Fast paths: away from the periodic x/y images a cell has its own position plus, at the
+// first and last level, one z ghost.  The x/y test is uniform across a wave whose lanes
+// run along z, so only boundary columns take the general ghost lists above.
+__device__ __forceinline__ bool near_xy(const Dims& d, int i, int j) {
+  return i < 3 || i >= d.nx - 3 || j < 3 || j >= d.ny - 3;
 }
 __device__ __forceinline__ void store_c_fast(const Dims& d, float* __restrict__ a, int i, int j, int k, float val) {
-  if (near_c(d, i, j, k)) store_c(d, a, i, j, k, val);
-  else a[ic(d, i + 3, j + 3, k + 1)] = val;
+  if (near_xy(d, i, j)) {
+    store_c(d, a, i, j, k, val);
+    return;
+  }
+  const long b = ic(d, i + 3, j + 3, 0);
+  a[b + k + 1] = val;
+  if (k == 0) a[b] = val;
+  if (k == d.nz - 1) a[b + d.nz + 1] = val;
 }
 __device__ __forceinline__ void store_u_fast(const Dims& d, float* __restrict__ a, int f, int j, int k, float val) {
-  if (f < 4 || f >= d.nx - 3 || j < 3 || j >= d.ny - 3 || k == 0 || k == d.nz - 1) store_u(d, a, f, j, k, val);
-  else a[iu(d, f + 3, j + 3, k + 1)] = val;
+  if (f < 4 || f >= d.nx - 3 || j < 3 || j >= d.ny - 3) {
+    store_u(d, a, f, j, k, val);
+    return;
+  }
+  const long b = iu(d, f + 3, j + 3, 0);
+  a[b + k + 1] = val;
+  if (k == 0) a[b] = val;
+  if (k == d.nz - 1) a[b + d.nz + 1] = val;
 }
 __device__ __forceinline__ void store_v_fast(const Dims& d, float* __restrict__ a, int i, int f, int k, float val) {
-  if (i < 3 || i >= d.nx - 3 || f < 4 || f >= d.ny - 3 || k == 0 || k == d.nz - 1) store_v(d, a, i, f, k, val);
-  else a[iv(d, i + 3, f + 3, k + 1)] = val;
+  if (i < 3 || i >= d.nx - 3 || f < 4 || f >= d.ny - 3) {
+    store_v(d, a, i, f, k, val);
+    return;
+  }
+  const long b = iv(d, i + 3, f + 3, 0);
+  a[b + k + 1] = val;
+  if (k == 0) a[b] = val;
+  if (k == d.nz - 1) a[b + d.nz + 1] = val;
 }
 __device__ __forceinline__ void store_w_fast(const Dims& d, float* __restrict__ a, int i, int j, int l, float val) {
-  if (i < 3 || i >= d.nx - 3 || j < 3 || j >= d.ny - 3 || l == 0 || l == d.nz) store_w(d, a, i, j, l, val);
-  else a[iw(d, i + 3, j + 3, l + 1)] = val;
+  if (near_xy(d, i, j)) {
+    store_w(d, a, i, j, l, val);
+    return;
+  }
+  const long b = iw(d, i + 3, j + 3, 0);
+  a[b + l + 1] = val;
+  if (l == 0) a[b] = val;
+  if (l == d.nz) a[b + d.nz + 2] = val;
 }
 
 }  // namespace lex

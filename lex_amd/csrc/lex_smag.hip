@@ -212,5 +212,42 @@ __global__ void k_smag_scalar(Dims d, const float* __restrict__ rc, const float*
   out[((long)i * d.ny + j) * d.nz + k] = smag_scalar_point(d, rc, kh, s, i, j, k);
 }
 
+// All scalars' SGS diffusion in one pass: the diffusivity faces (kh averaged to the six
+// faces of the cell, times rho at the face) are computed once and applied to every scalar.
+struct SgsScalars {
+  const float* s[10];
+  float* out[10];
+  int n;
+};
+
+__global__ void k_smag_scalars(Dims d, const float* __restrict__ rc, const float* __restrict__ kh, SgsScalars S) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  const long sx = d.sxc(), sy = d.syc();
+  const int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
+  const long c = ic(d, I, J, K);
+  const float r = rc[K];
+  const float r8u = 0.5f * (r + r);
+  // -rho_face * kh_face per face (same association as smag_scalar_point)
+  const float kx0 = -r8u * (0.5f * (kh[c] + kh[c - sx])), kx1 = -r8u * (0.5f * (kh[c + sx] + kh[c]));
+  const float ky0 = -r8u * (0.5f * (kh[c] + kh[c - sy])), ky1 = -r8u * (0.5f * (kh[c + sy] + kh[c]));
+  const bool zlo = k > 0, zhi = k + 1 < nz;
+  const float kz0 = zlo ? -(0.5f * (rc[K - 1] + rc[K])) * (0.5f * (kh[c - 1] + kh[c])) : 0.0f;
+  const float kz1 = zhi ? -(0.5f * (rc[K] + rc[K + 1])) * (0.5f * (kh[c] + kh[c + 1])) : 0.0f;
+  const long o = ((long)i * d.ny + j) * d.nz + k;
+  for (int q = 0; q < S.n; ++q) {
+    const float* sc = S.s[q];
+    const float s0 = sc[c];
+    float tx1 = kx1 * ((sc[c + sx] - s0) * d.idx), tx0 = kx0 * ((s0 - sc[c - sx]) * d.idx);
+    float ty1 = ky1 * ((sc[c + sy] - s0) * d.idy), ty0 = ky0 * ((s0 - sc[c - sy]) * d.idy);
+    float tz1 = zhi ? kz1 * ((sc[c + 1] - s0) * d.idz) : 0.0f;
+    float tz0 = zlo ? kz0 * ((s0 - sc[c - 1]) * d.idz) : 0.0f;
+    float div = (tx1 - tx0) * d.idx + (ty1 - ty0) * d.idy + (tz1 - tz0) * d.idz;
+    S.out[q][o] = -div / r;
+  }
+}
+
 #undef TAU
 }  // namespace lex

@@ -193,10 +193,21 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
                          cg * cg, o.prandtl);
       LEX_LAUNCH(KID_SMAG_UVW, k_smag_uvw, G(d.nz + 1, d.ny + 1, d.nx + 1), bk, 0, st, d, rc, sm[8], sm[0], sm[1], sm[2],
                          sm[3], sm[4], sm[5], su, sv, sw);
-      LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, sm[9], src[0], ss[0]);
-      LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, sm[9], src[5], ss[1]);
-      for (int q = 0; q < o.n_passive; ++q)
-        LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, sm[9], src[6 + q], ss[2 + q]);
+      if (o.impl == 2) {  // every scalar in one pass
+        SgsScalars SS;
+        SS.n = 2 + o.n_passive;
+        for (int q = 0; q < SS.n; ++q) {
+          SS.s[q] = src[(q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2))];
+          SS.out[q] = ss[q];
+        }
+        LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalars, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, sm[9], SS);
+      } else {
+        LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, sm[9], src[0], ss[0]);
+        LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, sm[9], src[5], ss[1]);
+        for (int q = 0; q < o.n_passive; ++q)
+          LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, sm[9], src[6 + q],
+                     ss[2 + q]);
+      }
     }
     const int nsc = 2 + o.n_passive;
     const bool fused_ok = (64 * ((d.nz + 63) / 64)) * kFTY <= 1024 && nsc >= 2 && nsc <= 6;
