@@ -50,6 +50,22 @@ BYTES_PER_CELL = {
 }
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rk3_step_256x256x128_hbm_traffic.json")
+
+
+def measured_traffic(kernel, nx, ny, nz, n_sound):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    measurement (scaled by a copy of known size) when it was taken on this workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if (nx, ny, nz, n_sound) != (256, 256, 128, 7) or kernel not in t.get("kernels", {}):
+        return None
+    return t["kernels"][kernel]["traffic_bytes"]
+
+
 def cpu_baseline(sample_steps=3, n_sound=7, nx=64, ny=64, nz=128):
     """NumPy FP32 oracle on a sub-grid of the same workload (one core)."""
     import numpy as np
@@ -252,7 +268,10 @@ def main():
                    "advection": "WENO5 (WENO3/centred near lid and surface)", "sgs": "Smagorinsky Cs=0.18",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dom, case.nx, case.ny, case.nz, n_sound),
+                     "traffic_source": "profiles/rk3_step_256x256x128_hbm_traffic.md (rocprofv3 --pmc FETCH_SIZE, "
+                                       "WRITE_SIZE; copy-kernel scaled)",
                      "bytes_per_cell": bpc, "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
         "as_specified": as_specified,
