@@ -141,6 +141,39 @@ def fwd_adjoint(steps=20, checkpoint_every=5, nx=128, ny=128, nz=64):
             "loss": float(loss.detach()), "dL_dCs": float(gc)}
 
 
+def sgs_net_timing(nx=96, ny=96, nz=48, reps=10):
+    """The learned-SGS net (11 -> 32 -> 32 -> 32 -> 5, 3x3x3, BF16 MFMA) on the coarse grid:
+    forward and forward+backward times and the MFMA throughput of the forward."""
+    import torch
+    from lex_amd.sgs_net import SgsNet
+    net = SgsNet(seed=1)
+    g = torch.Generator(device="cuda").manual_seed(2031)
+    x = torch.randn(nx, ny, nz, 11, device="cuda", generator=g)
+    t = torch.randn(nx, ny, nz, 5, device="cuda", generator=g)
+    for _ in range(2):
+        ((net(x) - t) ** 2).mean().backward()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    with torch.no_grad():
+        for _ in range(reps):
+            net(x)
+    e1.record()
+    torch.cuda.synchronize()
+    fwd = e0.elapsed_time(e1) / reps
+    e0.record()
+    for _ in range(reps):
+        ((net(x) - t) ** 2).mean().backward()
+    e1.record()
+    torch.cuda.synchronize()
+    fb = e0.elapsed_time(e1) / reps
+    flop = 2.0 * 27 * (11 * 32 + 32 * 32 + 32 * 32 + 32 * 5) * nx * ny * nz
+    tf = flop / (fwd / 1e3) / 1e12
+    return {"grid": [nx, ny, nz], "fwd_ms": fwd, "fwd_bwd_ms": fb,
+            "roofline": {"bound": "mfma", "achieved": tf, "peak": 2500.0, "unit": "TFLOP/s", "frac": tf / 2500.0,
+                         "note": "useful flops of the four convolutions (unpadded channels), BF16 dense peak"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,6 +279,7 @@ def main():
         del src, dst
     del state, phys, init
     fa = None if args.no_adjoint else fwd_adjoint()
+    sn = None if args.no_adjoint else sgs_net_timing()
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample_steps, n_sound)
     line = {
         "metric": "cell-steps/s (fwd, fwd+adjoint) vs plain at stated error; % HBM roofline",
@@ -276,6 +310,7 @@ def main():
         "cpu_baseline": cpu,
         "as_specified": as_specified,
         "fwd_adjoint": fa,
+        "sgs_net": sn,
         "kernels": kernels,
         "state_finite": finite,
     }

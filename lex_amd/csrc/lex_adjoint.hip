@@ -617,6 +617,116 @@ __global__ void k_acoustic_vjp_faces(Dims d, const float* __restrict__ p, const 
   }
 }
 
+// Gather form of the same reverse, used by the step driver.
+// k_acoustic_vjp_cells2: the new pi' (folded adjoint lpo) of each cell -> pi' (identity)
+// and pi* (its seven advection_pi points); the face-velocity terms are gathered by
+// k_acoustic_vjp_faces2 instead of scattered.
+__global__ void k_acoustic_vjp_cells2(Dims d, const float* __restrict__ rc, const float* __restrict__ pi0c,
+                                      const float* __restrict__ u, const float* __restrict__ v,
+                                      const float* __restrict__ w, const float* __restrict__ p,
+                                      const float* __restrict__ ps, const float* __restrict__ ru,
+                                      const float* __restrict__ rv, const float* __restrict__ rw,
+                                      const float* __restrict__ tr, const float* __restrict__ lpo, float dts,
+                                      float* __restrict__ lp, float* __restrict__ lps) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  int I = i + 3, J = j + 3, K = k + 1;
+  long c = ic(d, I, J, K);
+  float Lp = lpo[c];
+  if (Lp == 0.f) return;
+  const long sx = d.sxc(), sy = d.syc();
+  float un0 = u[iu(d, I, J, K)] + (pgf_x(d, p, tr, I, J, K) + ru[((long)i * d.ny + j) * d.nz + k]) * dts;
+  float un1 = u[iu(d, I + 1, J, K)] + (pgf_x(d, p, tr, I + 1, J, K) + ru[((long)(i + 1) * d.ny + j) * d.nz + k]) * dts;
+  float vn0 = v[iv(d, I, J, K)] + (pgf_y(d, p, tr, I, J, K) + rv[((long)i * (d.ny + 1) + j) * d.nz + k]) * dts;
+  float vn1 = v[iv(d, I, J + 1, K)] + (pgf_y(d, p, tr, I, J + 1, K) + rv[((long)i * (d.ny + 1) + j + 1) * d.nz + k]) * dts;
+  float wn0 = (k == 0) ? 0.0f : w[iw(d, I, J, K)] + (pgf_z(d, p, tr, I, J, K) + rw[((long)i * d.ny + j) * (d.nz + 1) + k]) * dts;
+  float wn1 = (k == d.nz - 1)
+                  ? 0.0f
+                  : w[iw(d, I, J, K + 1)] + (pgf_z(d, p, tr, I, J, K + 1) + rw[((long)i * d.ny + j) * (d.nz + 1) + k + 1]) * dts;
+  float div = (un1 - un0) * d.idx + (vn1 - vn0) * d.idy + (wn1 - wn0) * d.idz;
+  const float cr = (float)(-kRd / kCv);
+  aadd(lp + c, Lp);
+  float G = 1.1f * dts * Lp;
+  float lpc = cr * div * G;
+  float gx = -0.5f * G * d.idx, gy = -0.5f * G * d.idy;
+  lpc += gx * un0 - gx * un1 + gy * vn0 - gy * vn1;
+  aadd(lps + c - sx, -gx * un0);
+  aadd(lps + c + sx, gx * un1);
+  aadd(lps + c - sy, -gy * vn0);
+  aadd(lps + c + sy, gy * vn1);
+  if (k > 0) {
+    float gz = -0.5f * (0.5f * (rc[K - 1] + rc[K])) / rc[K] * G;
+    lpc += gz * wn0 * d.idz;
+    aadd(lps + c - 1, -gz * wn0 * d.idz);
+  }
+  if (k < d.nz - 1) {
+    float gz = -0.5f * (0.5f * (rc[K] + rc[K + 1])) / rc[K] * G;
+    lpc -= gz * wn1 * d.idz;
+    aadd(lps + c + 1, gz * wn1 * d.idz);
+  }
+  aadd(lps + c, lpc);
+}
+
+// k_acoustic_vjp_faces2: per face, the total adjoint of the new velocity = its folded
+// output adjoint + the terms of the (one or two) cells whose pressure tendency reads it,
+// gathered here; then old velocity (+= when acc, else =), rhs (+=, one owner per face) and
+// the PGF inputs (atomics).
+__global__ void k_acoustic_vjp_faces2(Dims d, const float* __restrict__ rc, const float* __restrict__ pi0c,
+                                      const float* __restrict__ p, const float* __restrict__ ps,
+                                      const float* __restrict__ tr, const float* __restrict__ lpo,
+                                      const float* __restrict__ luo, const float* __restrict__ lvo,
+                                      const float* __restrict__ lwo, float dts, int acc, float* __restrict__ lu,
+                                      float* __restrict__ lv, float* __restrict__ lw, float* __restrict__ lp,
+                                      float* __restrict__ ltr, float* __restrict__ lru, float* __restrict__ lrv,
+                                      float* __restrict__ lrw) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nz
+  int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
+  int i = blockIdx.z;                             // 0..nx
+  if (k > d.nz || j > d.ny) return;
+  const float cr = (float)(-kRd / kCv);
+  const float g11 = 1.1f * dts;
+  int I = i + 3, J = j + 3, K = k + 1;
+  if (k < d.nz && j < d.ny) {  // u face i
+    long o = ((long)i * d.ny + j) * d.nz + k;
+    long a = iu(d, I, J, K);
+    long cR = ic(d, I, J, K), cL = cR - d.sxc();
+    float dps = (ps[cR] - ps[cL]) * d.idx;
+    float g = luo[a];
+    if (i < d.nx) g += g11 * lpo[cR] * (-cr * (pi0c[K] + ps[cR]) * d.idx - 0.5f * dps);
+    if (i > 0) g += g11 * lpo[cL] * (cr * (pi0c[K] + ps[cL]) * d.idx - 0.5f * dps);
+    lu[a] = acc ? lu[a] + g : g;
+    lru[o] += g * dts;
+    if (g != 0.f) pgf_vjp(lp, ltr, p, tr, cR, d.sxc(), d.idx, g * dts);
+  }
+  if (k < d.nz && i < d.nx) {  // v face j
+    long o = ((long)i * (d.ny + 1) + j) * d.nz + k;
+    long a = iv(d, I, J, K);
+    long cR = ic(d, I, J, K), cL = cR - d.syc();
+    float dps = (ps[cR] - ps[cL]) * d.idy;
+    float g = lvo[a];
+    if (j < d.ny) g += g11 * lpo[cR] * (-cr * (pi0c[K] + ps[cR]) * d.idy - 0.5f * dps);
+    if (j > 0) g += g11 * lpo[cL] * (cr * (pi0c[K] + ps[cL]) * d.idy - 0.5f * dps);
+    lv[a] = acc ? lv[a] + g : g;
+    lrv[o] += g * dts;
+    if (g != 0.f) pgf_vjp(lp, ltr, p, tr, cR, d.syc(), d.idy, g * dts);
+  }
+  if (i < d.nx && j < d.ny && k > 0 && k < d.nz) {  // w level k: cell above (K) and below (K-1)
+    long o = ((long)i * d.ny + j) * (d.nz + 1) + k;
+    long a = iw(d, I, J, K);
+    long ca = ic(d, I, J, K), cb = ca - 1;
+    float r8 = 0.5f * (rc[K - 1] + rc[K]);
+    float dzz = (pi0c[K] - pi0c[K - 1]) * d.idz + (ps[ca] - ps[cb]) * d.idz;
+    float g = lwo[a];
+    g += g11 * lpo[ca] * (-cr * (pi0c[K] + ps[ca]) * d.idz + (-0.5f * r8 / rc[K]) * dzz);
+    g += g11 * lpo[cb] * (cr * (pi0c[K - 1] + ps[cb]) * d.idz + (-0.5f * r8 / rc[K - 1]) * dzz);
+    lw[a] = acc ? lw[a] + g : g;
+    lrw[o] += g * dts;
+    if (g != 0.f) pgf_vjp(lp, ltr, p, tr, ca, 1, d.idz, g * dts);
+  }
+}
+
 // ------------------------------------------------------------- G1 Smagorinsky
 // SGS scalar diffusion (turbulence.py:80-109): lam (nx, ny, nz) -> kh, s adjoints
 __global__ void k_smag_scalar_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ kh,

@@ -264,9 +264,6 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
     sm[q] = (float*)(ws + L.off_sm + q * al256(L.nc * 4));
     lsm[q] = (float*)(A(L.off_lsm) + q * al256(L.nc * 4));
   }
-  float* lU = (float*)(A(L.off_lU));
-  float* lV = (float*)(A(L.off_lV));
-  float* lW = (float*)(A(L.off_lW));
   float* lru = (float*)(A(L.off_lru));
   float* lrv = (float*)(A(L.off_lrv));
   float* lrw = (float*)(A(L.off_lrw));
@@ -325,32 +322,24 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
     for (int t = nsub - 1; t >= 0; --t) {
       float* in[4];
       float* li[4];
+      const int acc = (t == 0);   // the step input's adjoint accumulates; ping-pong buffers are overwritten
       if (t == 0) {
         for (int c = 0; c < 4; ++c) { in[c] = X0[1 + c]; li[c] = lam_in[1 + c]; }
       } else {
         acou_ptrs(ws + ac_off(L, s, t), RL, in);
         for (int c = 0; c < 4; ++c) li[c] = lP[t & 1][c];
-        cx.zero(li[0], L.nu);
-        cx.zero(li[1], L.nv);
-        cx.zero(li[2], L.nw);
         cx.zero(li[3], L.nc);
       }
-      cx.zero(lU, L.iu_);
-      cx.zero(lV, L.iv_);
-      cx.zero(lW, L.iw_);
-      hipLaunchKernelGGL(k_acoustic_vjp_cells, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, rc, b.pi0,
+      hipLaunchKernelGGL(k_acoustic_vjp_cells2, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, rc, b.pi0,
                          (const float*)in[0], (const float*)in[1], (const float*)in[2], (const float*)in[3],
-                         (const float*)src[4], ru, rv, rw, tr, (const float*)lo[3], dts, lU, lV, lW, li[3], lsrc[4]);
-      hipLaunchKernelGGL(k_acoustic_vjp_faces, cx.G(d.nz + 1, d.ny + 1, d.nx + 1), cx.bk, 0, st, d,
-                         (const float*)in[3], tr, (const float*)lo[0], (const float*)lo[1], (const float*)lo[2],
-                         (const float*)lU, (const float*)lV, (const float*)lW, dts, li[0], li[1], li[2], li[3], ltr,
-                         lru, lrv, lrw);
-      if (t > 0) {
-        cx.fold(1, li[0]);
-        cx.fold(2, li[1]);
-        cx.fold(3, li[2]);
-        cx.fold(0, li[3]);
-      }
+                         (const float*)src[4], ru, rv, rw, tr, (const float*)lo[3], dts, li[3], lsrc[4]);
+      hipLaunchKernelGGL(k_acoustic_vjp_faces2, cx.G(d.nz + 1, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, b.pi0,
+                         (const float*)in[3], (const float*)src[4], tr, (const float*)lo[3], (const float*)lo[0],
+                         (const float*)lo[1], (const float*)lo[2], dts, acc, li[0], li[1], li[2], li[3], ltr, lru, lrv,
+                         lrw);
+      // the velocity adjoints are written at interior points only (their ghosts are never
+      // read); the pi' adjoint also has ghost contributions from the PGF stencil
+      if (t > 0) cx.fold(0, li[3]);
       for (int c = 0; c < 4; ++c) lo[c] = li[c];
     }
     // momentum rhs = advection (+ b8w for w) + frozen SGS
