@@ -23,8 +23,9 @@ __device__ __forceinline__ void aadd(float* p, float v) { unsafeAtomicAdd(p, v);
 // w_k = d_k (1 + r_k^2), R = sum w_k q_k / W:
 //   dR/dr_k = (q_k - R) 2 d_k r_k / W,  dr_k/dtau = 1/(beta_k+eps),  dr_k/dbeta_k = -r_k/(beta_k+eps)
 // Evaluated on differences of neighbouring values (exact in FP32 for close values), so
-// fields with a large offset (theta ~ 300 K) keep their gradient accuracy.
-__device__ float weno5_grad(float a, float b, float c, float d, float e, float weps, float* g) {
+// fields with a large offset (theta ~ 300 K) keep their gradient accuracy.  Returns
+// R - ref, formed as (c - ref) + (R - c) so a nearby ref leaves no cancellation error.
+__device__ float weno5_grad(float a, float b, float c, float d, float e, float weps, float* g, float ref = 0.f) {
   const float c1312 = (float)(13.0 / 12.0);
   const float dab = a - b, dbc = b - c, dcd = c - d, dde = d - e;
   float t1 = dab - dbc, t2 = dab - 3.0f * dbc;
@@ -36,7 +37,8 @@ __device__ float weno5_grad(float a, float b, float c, float d, float e, float w
   float df = b1 - b3, tau = fabsf(df);
   float sg = (df > 0.f) ? 1.f : ((df < 0.f) ? -1.f : 0.f);
   float e1 = b1 + weps, e2 = b2 + weps, e3 = b3 + weps;
-  float x1 = tau / e1, x2 = tau / e2, x3 = tau / e3;
+  const float ie1 = frcp(e1), ie2 = frcp(e2), ie3 = frcp(e3);
+  float x1 = tau * ie1, x2 = tau * ie2, x3 = tau * ie3;
   bool c1 = !(x1 < 1.0e30f), c2 = !(x2 < 1.0e30f), c3 = !(x3 < 1.0e30f);
   float r1 = fminf(1.0e30f, x1), r2 = fminf(1.0e30f, x2), r3 = fminf(1.0e30f, x3);
   float w1 = 0.1f * (1.0f + r1 * r1), w2 = 0.6f * (1.0f + r2 * r2), w3 = 0.3f * (1.0f + r3 * r3);
@@ -47,16 +49,17 @@ __device__ float weno5_grad(float a, float b, float c, float d, float e, float w
   float h2 = -s6 * db + (float)(2.0 / 6.0) * dd;
   float h3 = (float)(5.0 / 6.0) * dd - s6 * de;
   float W = w1 + w2 + w3;
-  float Rd = (w1 * h1 + w2 * h2 + w3 * h3) / W;
-  float R = c + Rd;
-  float C1 = c1 ? 0.f : (h1 - Rd) * 0.2f * r1 / (W * e1);
-  float C2 = c2 ? 0.f : (h2 - Rd) * 1.2f * r2 / (W * e2);
-  float C3 = c3 ? 0.f : (h3 - Rd) * 0.6f * r3 / (W * e3);
+  const float iW = frcp(W);
+  float Rd = (w1 * h1 + w2 * h2 + w3 * h3) * iW;
+  float R = (c - ref) + Rd;
+  float C1 = c1 ? 0.f : (h1 - Rd) * 0.2f * r1 * iW * ie1;
+  float C2 = c2 ? 0.f : (h2 - Rd) * 1.2f * r2 * iW * ie2;
+  float C3 = c3 ? 0.f : (h3 - Rd) * 0.6f * r3 * iW * ie3;
   float Cs = C1 + C2 + C3;
   float E1 = sg * Cs - C1 * r1, E2 = -C2 * r2, E3 = -sg * Cs - C3 * r3;  // dR/dbeta_k
   float A1 = 2.f * c1312 * t1, B1 = 0.5f * t2, A2 = 2.f * c1312 * t3, B2 = 0.5f * t4;
   float A3 = 2.f * c1312 * t5, B3 = 0.5f * t6;
-  float W1 = w1 / W, W2 = w2 / W, W3 = w3 / W;
+  float W1 = w1 * iW, W2 = w2 * iW, W3 = w3 * iW;
   g[0] = E1 * (A1 + B1) + W1 * (float)(2.0 / 6.0);
   g[1] = E1 * (-2.f * A1 - 4.f * B1) + E2 * (A2 + B2) + W1 * (float)(-7.0 / 6.0) - W2 * s6;
   g[2] = E1 * (A1 + 3.f * B1) + E2 * (-2.f * A2) + E3 * (A3 + 3.f * B3) + W1 * (float)(11.0 / 6.0) +
@@ -66,84 +69,87 @@ __device__ float weno5_grad(float a, float b, float c, float d, float e, float w
   return R;
 }
 
-// R = weno3(a, b, c) and g[0..2]
-__device__ float weno3_grad(float a, float b, float c, float weps, float* g) {
+// R = weno3(a, b, c) - ref and g[0..2]
+__device__ float weno3_grad(float a, float b, float c, float weps, float* g, float ref = 0.f) {
   const float dab = a - b, dbc = b - c;
   float b1 = dab * dab, b2 = dbc * dbc;
   float df = b1 - b2, tau = fabsf(df);
   float sg = (df > 0.f) ? 1.f : ((df < 0.f) ? -1.f : 0.f);
   float e1 = b1 + weps, e2 = b2 + weps;
-  float x1 = tau / e1, x2 = tau / e2;
+  const float ie1 = frcp(e1), ie2 = frcp(e2);
+  float x1 = tau * ie1, x2 = tau * ie2;
   bool c1 = !(x1 < 1.0e30f), c2 = !(x2 < 1.0e30f);
   float r1 = fminf(1.0e30f, x1), r2 = fminf(1.0e30f, x2);
   float w1 = (float)(1.0 / 3.0) * (1.f + r1 * r1), w2 = (float)(2.0 / 3.0) * (1.f + r2 * r2);
   // candidates relative to the upwind value b: h1 = q1 - b, h2 = q2 - b
   float h1 = -0.5f * dab, h2 = -0.5f * dbc;
   float W = w1 + w2;
-  float Rd = (w1 * h1 + w2 * h2) / W;
-  float R = b + Rd;
-  float C1 = c1 ? 0.f : (h1 - Rd) * (float)(2.0 / 3.0) * r1 / (W * e1);
-  float C2 = c2 ? 0.f : (h2 - Rd) * (float)(4.0 / 3.0) * r2 / (W * e2);
+  const float iW = frcp(W);
+  float Rd = (w1 * h1 + w2 * h2) * iW;
+  float R = (b - ref) + Rd;
+  float C1 = c1 ? 0.f : (h1 - Rd) * (float)(2.0 / 3.0) * r1 * iW * ie1;
+  float C2 = c2 ? 0.f : (h2 - Rd) * (float)(4.0 / 3.0) * r2 * iW * ie2;
   float E1 = sg * (C1 + C2) - C1 * r1, E2 = -sg * (C1 + C2) - C2 * r2;
-  float W1 = w1 / W, W2 = w2 / W;
+  float W1 = w1 * iW, W2 = w2 * iW;
   g[0] = E1 * 2.f * dab - 0.5f * W1;
   g[1] = -E1 * 2.f * dab + E2 * 2.f * dbc + 1.5f * W1 + 0.5f * W2;
   g[2] = -E2 * 2.f * dbc + 0.5f * W2;
   return R;
 }
 
-// hrecon (lex_common.hpp) with its gradient w.r.t. q[m-3+t], t = 0..5
-__device__ float hrecon_grad(const float* q6, bool pos, float weps, float* g6) {
+// hrecon (lex_common.hpp) minus ref, with its gradient w.r.t. q[m-3+t], t = 0..5
+__device__ float hrecon_grad(const float* q6, bool pos, float weps, float* g6, float ref = 0.f) {
   float g[5];
   float R;
   for (int t = 0; t < 6; ++t) g6[t] = 0.f;
   if (pos) {
-    R = weno5_grad(q6[0], q6[1], q6[2], q6[3], q6[4], weps, g);
+    R = weno5_grad(q6[0], q6[1], q6[2], q6[3], q6[4], weps, g, ref);
     for (int t = 0; t < 5; ++t) g6[t] = g[t];
   } else {
-    R = weno5_grad(q6[5], q6[4], q6[3], q6[2], q6[1], weps, g);
+    R = weno5_grad(q6[5], q6[4], q6[3], q6[2], q6[1], weps, g, ref);
     for (int t = 0; t < 5; ++t) g6[5 - t] = g[t];
   }
   return R;
 }
 
-// vrecon (lex_common.hpp) with its gradient w.r.t. p[f-2+t], t = 0..5
-__device__ float vrecon_grad(const float* __restrict__ p, int L, int f, bool pos, float weps, float* g6) {
+// vrecon (lex_common.hpp) minus ref, with its gradient w.r.t. p[f-2+t], t = 0..5
+__device__ float vrecon_grad(const float* __restrict__ p, int L, int f, bool pos, float weps, float* g6,
+                            float ref = 0.f) {
   for (int t = 0; t < 6; ++t) g6[t] = 0.f;
-  if (f <= 0 || f >= L - 2) return 0.f;
+  if (f <= 0 || f >= L - 2) return -ref;
   float g[5];
   float R;
   if (pos) {
     if (f == 1) {  // 0.5 (p1 + p2)
       g6[2] = 0.5f;
       g6[3] = 0.5f;
-      return 0.5f * (p[1] + p[2]);
+      return 0.5f * ((p[1] - ref) + (p[2] - ref));
     }
     if (f == 2 || f == L - 3) {  // weno3(p[f-1], p[f], p[f+1])
-      R = weno3_grad(p[f - 1], p[f], p[f + 1], weps, g);
+      R = weno3_grad(p[f - 1], p[f], p[f + 1], weps, g, ref);
       g6[1] = g[0]; g6[2] = g[1]; g6[3] = g[2];
       return R;
     }
-    R = weno5_grad(p[f - 2], p[f - 1], p[f], p[f + 1], p[f + 2], weps, g);
+    R = weno5_grad(p[f - 2], p[f - 1], p[f], p[f + 1], p[f + 2], weps, g, ref);
     for (int t = 0; t < 5; ++t) g6[t] = g[t];
     return R;
   }
   if (f == 1) {  // weno3(p3, p2, p1)
-    R = weno3_grad(p[3], p[2], p[1], weps, g);
+    R = weno3_grad(p[3], p[2], p[1], weps, g, ref);
     g6[4] = g[0]; g6[3] = g[1]; g6[2] = g[2];
     return R;
   }
   if (f == L - 3) {  // 0.5 (p[f+1] + p[f])
     g6[3] = 0.5f;
     g6[2] = 0.5f;
-    return 0.5f * (p[L - 2] + p[L - 3]);
+    return 0.5f * ((p[L - 2] - ref) + (p[L - 3] - ref));
   }
   if (f == L - 4) {  // weno3(p[f+2], p[f+1], p[f])
-    R = weno3_grad(p[f + 2], p[f + 1], p[f], weps, g);
+    R = weno3_grad(p[f + 2], p[f + 1], p[f], weps, g, ref);
     g6[4] = g[0]; g6[3] = g[1]; g6[2] = g[2];
     return R;
   }
-  R = weno5_grad(p[f + 3], p[f + 2], p[f + 1], p[f], p[f - 1], weps, g);
+  R = weno5_grad(p[f + 3], p[f + 2], p[f + 1], p[f], p[f - 1], weps, g, ref);
   g6[5] = g[0]; g6[4] = g[1]; g6[3] = g[2]; g6[2] = g[3]; g6[1] = g[4];
   return R;
 }
@@ -193,6 +199,19 @@ __device__ __forceinline__ void fd_vjp(const Dims& d, const float* __restrict__ 
 // ------------------------------------------------------------- A2 scalar advection
 // T = [-div F + s div(rho0 u)] / rho0 at interior cells (advection.py:9-34);
 // lam = dL/dT on the interior (nx, ny, nz); surface flux prescribed.
+//
+// Face-centric reverse.  Thread (i, j, k), i in [0, nx], j in [0, ny], k in [0, nz), owns
+// the x face at padded I = i+3 (j < ny), the y face at J = j+3 (i < nx), the w level
+// K = k+1 and, for k = nz-1, the lid level (i < nx, j < ny), and cell (I, J, K) when it
+// is interior.  A face flux's adjoint is the difference of its two cells' lam/rho0 (zero
+// outside the interior), so every WENO gradient is evaluated once per face, and the
+// div(rho0 u) terms of both cells are collected at the face: each face-velocity adjoint
+// (lu, lv, lw) has exactly one writer and is updated without atomics.
+__device__ __forceinline__ float mu_cell(const Dims& d, const float* __restrict__ lam, const float* __restrict__ rc,
+                                         int i, int j, int k) {
+  return lam[((long)i * d.ny + j) * d.nz + k] / rc[k + 1];
+}
+
 __global__ void k_adv_scalar_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ s,
                                  const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
                                  const float* __restrict__ lam, float weps, float* __restrict__ ls,
@@ -200,233 +219,316 @@ __global__ void k_adv_scalar_vjp(Dims d, const float* __restrict__ rc, const flo
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   int i = blockIdx.z;
-  if (k >= d.nz || j >= d.ny) return;
-  float L = lam[((long)i * d.ny + j) * d.nz + k];
-  if (L == 0.f) return;
-  int I = i + 3, J = j + 3, K = k + 1;
-  float r = rc[K];
-  float mu = L / r;
+  if (k >= d.nz || j > d.ny) return;
+  const int I = i + 3, J = j + 3, K = k + 1;
+  const bool cell = i < d.nx && j < d.ny;
+  const float r = rc[K];
+  const float mc = cell ? mu_cell(d, lam, rc, i, j, k) : 0.f;
   const long sx = d.sxc(), sy = d.syc();
-  long c = ic(d, I, J, K);
+  const long c = ic(d, I, J, K);
+  const float sc = s[c];
   float q6[6], g6[6];
-  for (int t2 = 0; t2 < 2; ++t2) {  // x faces I (lower) and I+1 (upper)
-    int m = I + t2;
-    float lf = (t2 == 0 ? mu : -mu) * d.idx;
-    float mass = r * u[iu(d, m, J, K)];
-    long base = ic(d, m - 3, J, K);
-    for (int t = 0; t < 6; ++t) q6[t] = s[base + t * sx];
-    float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
-    aadd(lu + iu(d, m, J, K), r * lf * R);
-    scatter6(ls, base, sx, g6, lf * mass);
+  if (j < d.ny) {  // x face I: lower face of cell I, upper face of cell I-1
+    const float ml = i > 0 ? mu_cell(d, lam, rc, i - 1, j, k) : 0.f;
+    const float gF = (mc - ml) * d.idx;
+    const long a = iu(d, I, J, K);
+    const float mass = r * u[a];
+    // d(flux)/du collects mc (R - s_I) - ml (R - s_{I-1}) = gF (R - s_I) - ml (s_I - s_{I-1}) / dx:
+    // every factor is a small difference, not a difference of theta-sized products
+    float g = -ml * d.idx * (sc - s[c - sx]);
+    if (gF != 0.f) {
+      const long base = c - 3 * sx;
+      for (int t = 0; t < 6; ++t) q6[t] = s[base + t * sx];
+      float Rc = hrecon_grad(q6, upwind_pos(mass), weps, g6, sc);
+      scatter6(ls, base, sx, g6, gF * mass);
+      g += gF * Rc;
+    }
+    if (g != 0.f) lu[a] += r * g;
   }
-  for (int t2 = 0; t2 < 2; ++t2) {
-    int m = J + t2;
-    float lf = (t2 == 0 ? mu : -mu) * d.idy;
-    float mass = r * v[iv(d, I, m, K)];
-    long base = ic(d, I, m - 3, K);
-    for (int t = 0; t < 6; ++t) q6[t] = s[base + t * sy];
-    float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
-    aadd(lv + iv(d, I, m, K), r * lf * R);
-    scatter6(ls, base, sy, g6, lf * mass);
+  if (i < d.nx) {  // y face J
+    const float ml = j > 0 ? mu_cell(d, lam, rc, i, j - 1, k) : 0.f;
+    const float gF = (mc - ml) * d.idy;
+    const long a = iv(d, I, J, K);
+    const float mass = r * v[a];
+    float g = -ml * d.idy * (sc - s[c - sy]);
+    if (gF != 0.f) {
+      const long base = c - 3 * sy;
+      for (int t = 0; t < 6; ++t) q6[t] = s[base + t * sy];
+      float Rc = hrecon_grad(q6, upwind_pos(mass), weps, g6, sc);
+      scatter6(ls, base, sy, g6, gF * mass);
+      g += gF * Rc;
+    }
+    if (g != 0.f) lv[a] += r * g;
   }
+  if (!cell) return;
   const float* col = s + ic(d, I, J, 0);
   float* lcol = ls + ic(d, I, J, 0);
-  for (int t2 = 0; t2 < 2; ++t2) {
-    int f = K - 1 + t2;    // face between padded levels f and f+1
-    if (f == 0) continue;  // surface flux is prescribed
-    float lf = (t2 == 0 ? mu : -mu) * d.idz;
-    float rf = 0.5f * (rc[f] + rc[f + 1]);
-    float mass = rf * w[iw(d, I, J, f + 1)];
-    float R = vrecon_grad(col, d.nz + 2, f, upwind_pos(mass), weps, g6);
-    if (R != 0.f) aadd(lw + iw(d, I, J, f + 1), rf * lf * R);
-    scatter6z(lcol, f, d.nz + 2, g6, lf * mass);
+  const long wb = iw(d, I, J, 0);
+  const float mb = k > 0 ? mu_cell(d, lam, rc, i, j, k - 1) : 0.f;  // cell below
+  const float r8 = 0.5f * (rc[K - 1] + rc[K]);
+  float g;
+  if (k > 0) {  // flux face between levels k and k+1 (the surface face k = 0 is prescribed)
+    const float gF = (mc - mb) * d.idz;
+    g = -mb * d.idz * (sc - col[K - 1]);
+    if (gF != 0.f) {
+      const float mass = r8 * w[wb + K];
+      float Rc = vrecon_grad(col, d.nz + 2, k, upwind_pos(mass), weps, g6, sc);
+      scatter6z(lcol, k, d.nz + 2, g6, gF * mass);
+      g += gF * Rc;
+    }
+  } else {
+    g = -mc * sc * d.idz;
   }
-  aadd(ls + c, mu * fd_at(d, rc, u, v, w, I, J, K));
-  fd_vjp(d, rc, I, J, K, mu * s[c], lu, lv, lw);
+  if (g != 0.f) lw[wb + K] += r8 * g;
+  if (k == d.nz - 1) {  // lid level: div(rho0 u) of the top cell only
+    const float gt = mc * sc * (0.5f * (rc[K] + rc[K + 1])) * d.idz;
+    if (gt != 0.f) lw[wb + K + 1] += gt;
+  }
+  aadd(ls + c, mc * fd_at(d, rc, u, v, w, I, J, K));
 }
 
 // ------------------------------------------------------------ A3 momentum advection
-// reverse of advection_u (advection.py:37-74), lam: (nx+1, ny, nz)
+// Flux-point-centric reverses: every flux point is shared by the two momentum points it
+// separates, so its adjoint is the difference of their lam/rho0 and its WENO gradient is
+// evaluated once.  The div(rho0 u) product term is reversed per scalar cell, collecting
+// both momentum points that average it.
+__device__ __forceinline__ float mu_uface(const Dims& d, const float* __restrict__ lam, const float* __restrict__ rc,
+                                          int f, int j, int k) {
+  return lam[((long)f * d.ny + j) * d.nz + k] / rc[k + 1];
+}
+__device__ __forceinline__ float mu_vface(const Dims& d, const float* __restrict__ lam, const float* __restrict__ rc,
+                                          int i, int f, int k) {
+  return lam[((long)i * (d.ny + 1) + f) * d.nz + k] / rc[k + 1];
+}
+__device__ __forceinline__ float mu_wlev(const Dims& d, const float* __restrict__ lam, const float* __restrict__ rc,
+                                         int i, int j, int l) {
+  return lam[((long)i * d.ny + j) * (d.nz + 1) + l] / (0.5f * (rc[l] + rc[l + 1]));
+}
+
+// reverse of advection_u (advection.py:37-74), lam: (nx+1, ny, nz).  Thread (a, b, k),
+// a in [0, nx+1], b in [0, ny]: x flux at scalar cell a+3 (b < ny), y flux at corner
+// (a+3, b+3) (a <= nx), z flux at the w level below u point (a+3, b+3, k+1) and that
+// point's own term (a <= nx, b < ny), div(rho0 u) of cell (a+2, b+3, k+1) (b < ny).
 __global__ void k_adv_u_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
                             const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ lam,
                             float* __restrict__ lu, float* __restrict__ lv, float* __restrict__ lw) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int f = blockIdx.z;
-  if (k >= d.nz || j >= d.ny) return;
-  float L = lam[((long)f * d.ny + j) * d.nz + k];
-  if (L == 0.f) return;
+  int b = blockIdx.y * blockDim.y + threadIdx.y;
+  int a = blockIdx.z;
+  if (k >= d.nz || b > d.ny) return;
   const float weps = 1.0e-16f;
-  int I = f + 3, J = j + 3, K = k + 1;
-  float r = rc[K];
-  float mu = L / (0.5f * (r + r));
+  const int J = b + 3, K = k + 1;
+  const float r = rc[K];
+  const bool upt = a <= d.nx && b < d.ny;
+  const float mc = upt ? mu_uface(d, lam, rc, a, b, k) : 0.f;
   const long sx = d.sxu(), sy = d.syc();
   float q6[6], g6[6];
-  for (int t2 = 0; t2 < 2; ++t2) {  // x flux at scalar cells I-1, I
-    int m = I + t2;
-    float lf = (t2 == 0 ? mu : -mu) * d.idx;
-    long base = iu(d, m - 3, J, K);
-    for (int t = 0; t < 6; ++t) q6[t] = u[base + t * sx];
-    float mass = 0.5f * (q6[2] + q6[3]) * r;
-    float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
-    scatter6(lu, base, sx, g6, lf * mass);
-    float gm = lf * R * 0.5f * r;
-    aadd(lu + base + 2 * sx, gm);
-    aadd(lu + base + 3 * sx, gm);
+  if (b < d.ny) {
+    // x flux at scalar cell a+3: lower flux of u face a, upper flux of u face a-1
+    const float ml = a > 0 ? mu_uface(d, lam, rc, a - 1, b, k) : 0.f;
+    const float gF = (mc - ml) * d.idx;
+    const long base = iu(d, a, J, K);
+    if (gF != 0.f) {
+      for (int t = 0; t < 6; ++t) q6[t] = u[base + t * sx];
+      const float mass = 0.5f * (q6[2] + q6[3]) * r;
+      float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+      scatter6(lu, base, sx, g6, gF * mass);
+      const float gm = gF * R * 0.5f * r;
+      aadd(lu + base + 2 * sx, gm);
+      aadd(lu + base + 3 * sx, gm);
+    }
+    // div(rho0 u) of cell a+2, averaged by u faces a-1 (its lower) and a (its upper)
+    const float gl = ml * u[base + 2 * sx], gr = mc * u[base + 3 * sx];
+    fd_vjp(d, rc, a + 2, J, K, 0.5f * (gl + gr), lu, lv, lw);
   }
-  for (int t2 = 0; t2 < 2; ++t2) {  // y flux at corners
-    int m = J + t2;
-    float lf = (t2 == 0 ? mu : -mu) * d.idy;
-    float mass = 0.5f * (r * v[iv(d, I - 1, m, K)] + r * v[iv(d, I, m, K)]);
-    long base = iu(d, I, m - 3, K);
-    for (int t = 0; t < 6; ++t) q6[t] = u[base + t * sy];
-    float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
-    scatter6(lu, base, sy, g6, lf * mass);
-    float gm = lf * R * 0.5f * r;
-    aadd(lv + iv(d, I - 1, m, K), gm);
-    aadd(lv + iv(d, I, m, K), gm);
-  }
-  const float* col = u + iu(d, I, J, 0);
-  float* lcol = lu + iu(d, I, J, 0);
-  for (int t2 = 0; t2 < 2; ++t2) {  // z flux at w levels
-    int fz = K - 1 + t2;
-    if (fz == 0) continue;
-    float lf = (t2 == 0 ? mu : -mu) * d.idz;
-    float rf = 0.5f * (rc[fz] + rc[fz + 1]);
-    float mass = 0.5f * (rf * w[iw(d, I - 1, J, fz + 1)] + rf * w[iw(d, I, J, fz + 1)]);
-    float R = vrecon_grad(col, d.nz + 2, fz, upwind_pos(mass), weps, g6);
-    scatter6z(lcol, fz, d.nz + 2, g6, lf * mass);
-    if (R != 0.f) {
-      float gm = lf * R * 0.5f * rf;
-      aadd(lw + iw(d, I - 1, J, fz + 1), gm);
-      aadd(lw + iw(d, I, J, fz + 1), gm);
+  if (a > d.nx) return;
+  const int I = a + 3;
+  {  // y flux at corner (I, J): lower flux of u (a, b), upper flux of u (a, b-1)
+    const float ml = b > 0 ? mu_uface(d, lam, rc, a, b - 1, k) : 0.f;
+    const float gF = (mc - ml) * d.idy;
+    if (gF != 0.f) {
+      const long va = iv(d, I - 1, J, K), vb = iv(d, I, J, K);
+      const float mass = 0.5f * (r * v[va] + r * v[vb]);
+      const long base = iu(d, I, b, K);
+      for (int t = 0; t < 6; ++t) q6[t] = u[base + t * sy];
+      float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+      scatter6(lu, base, sy, g6, gF * mass);
+      const float gm = gF * R * 0.5f * r;
+      aadd(lv + va, gm);
+      aadd(lv + vb, gm);
     }
   }
-  float uc = u[iu(d, I, J, K)];
-  aadd(lu + iu(d, I, J, K), mu * 0.5f * (fd_at(d, rc, u, v, w, I - 1, J, K) + fd_at(d, rc, u, v, w, I, J, K)));
-  fd_vjp(d, rc, I - 1, J, K, mu * 0.5f * uc, lu, lv, lw);
-  fd_vjp(d, rc, I, J, K, mu * 0.5f * uc, lu, lv, lw);
+  if (!upt) return;
+  const long ub = iu(d, I, J, 0);
+  const float* col = u + ub;
+  float* lcol = lu + ub;
+  if (k > 0) {  // z flux at w level K between u levels K-1 and K
+    const float gF = (mc - mu_uface(d, lam, rc, a, b, k - 1)) * d.idz;
+    if (gF != 0.f) {
+      const float rf = 0.5f * (rc[k] + rc[k + 1]);
+      const long wa = iw(d, I - 1, J, K), wb = iw(d, I, J, K);
+      const float mass = 0.5f * (rf * w[wa] + rf * w[wb]);
+      float R = vrecon_grad(col, d.nz + 2, k, upwind_pos(mass), weps, g6);
+      scatter6z(lcol, k, d.nz + 2, g6, gF * mass);
+      if (R != 0.f) {
+        const float gm = gF * R * 0.5f * rf;
+        aadd(lw + wa, gm);
+        aadd(lw + wb, gm);
+      }
+    }
+  }
+  aadd(lcol + K, mc * 0.5f * (fd_at(d, rc, u, v, w, I - 1, J, K) + fd_at(d, rc, u, v, w, I, J, K)));
 }
 
-// reverse of advection_v (advection.py:77-114), lam: (nx, ny+1, nz)
+// reverse of advection_v (advection.py:77-114), lam: (nx, ny+1, nz).  Thread (a, b, k),
+// a in [0, nx], b in [0, ny+1]: x flux at corner (a+3, b+3) (b <= ny), y flux at scalar
+// cell b+3 (a < nx), z flux and own term of v point (a+3, b+3, k+1) (a < nx, b <= ny),
+// div(rho0 u) of cell (a+3, b+2, k+1) (a < nx).
 __global__ void k_adv_v_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
                             const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ lam,
                             float* __restrict__ lu, float* __restrict__ lv, float* __restrict__ lw) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int f = blockIdx.y * blockDim.y + threadIdx.y;
-  int i = blockIdx.z;
-  if (k >= d.nz || f > d.ny) return;
-  float L = lam[((long)i * (d.ny + 1) + f) * d.nz + k];
-  if (L == 0.f) return;
+  int b = blockIdx.y * blockDim.y + threadIdx.y;
+  int a = blockIdx.z;
+  if (k >= d.nz || b > d.ny + 1) return;
   const float weps = 1.0e-16f;
-  int I = i + 3, J = f + 3, K = k + 1;
-  float r = rc[K];
-  float mu = L / (0.5f * (r + r));
+  const int I = a + 3, K = k + 1;
+  const float r = rc[K];
+  const bool vpt = a < d.nx && b <= d.ny;
+  const float mc = vpt ? mu_vface(d, lam, rc, a, b, k) : 0.f;
   const long sx = d.sxv(), sy = d.syc();
   float q6[6], g6[6];
-  for (int t2 = 0; t2 < 2; ++t2) {  // x flux at corners
-    int m = I + t2;
-    float lf = (t2 == 0 ? mu : -mu) * d.idx;
-    float mass = 0.5f * (r * u[iu(d, m, J - 1, K)] + r * u[iu(d, m, J, K)]);
-    long base = iv(d, m - 3, J, K);
-    for (int t = 0; t < 6; ++t) q6[t] = v[base + t * sx];
-    float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
-    scatter6(lv, base, sx, g6, lf * mass);
-    float gm = lf * R * 0.5f * r;
-    aadd(lu + iu(d, m, J - 1, K), gm);
-    aadd(lu + iu(d, m, J, K), gm);
-  }
-  for (int t2 = 0; t2 < 2; ++t2) {  // y flux at cells J-1, J
-    int m = J + t2;
-    float lf = (t2 == 0 ? mu : -mu) * d.idy;
-    long base = iv(d, I, m - 3, K);
-    for (int t = 0; t < 6; ++t) q6[t] = v[base + t * sy];
-    float mass = 0.5f * (q6[2] + q6[3]) * r;
-    float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
-    scatter6(lv, base, sy, g6, lf * mass);
-    float gm = lf * R * 0.5f * r;
-    aadd(lv + base + 2 * sy, gm);
-    aadd(lv + base + 3 * sy, gm);
-  }
-  const float* col = v + iv(d, I, J, 0);
-  float* lcol = lv + iv(d, I, J, 0);
-  for (int t2 = 0; t2 < 2; ++t2) {
-    int fz = K - 1 + t2;
-    if (fz == 0) continue;
-    float lf = (t2 == 0 ? mu : -mu) * d.idz;
-    float rf = 0.5f * (rc[fz] + rc[fz + 1]);
-    float mass = 0.5f * (rf * w[iw(d, I, J - 1, fz + 1)] + rf * w[iw(d, I, J, fz + 1)]);
-    float R = vrecon_grad(col, d.nz + 2, fz, upwind_pos(mass), weps, g6);
-    scatter6z(lcol, fz, d.nz + 2, g6, lf * mass);
-    if (R != 0.f) {
-      float gm = lf * R * 0.5f * rf;
-      aadd(lw + iw(d, I, J - 1, fz + 1), gm);
-      aadd(lw + iw(d, I, J, fz + 1), gm);
+  if (b <= d.ny) {  // x flux at corner (I, J): lower flux of v (a, b), upper flux of v (a-1, b)
+    const int J = b + 3;
+    const float ml = a > 0 ? mu_vface(d, lam, rc, a - 1, b, k) : 0.f;
+    const float gF = (mc - ml) * d.idx;
+    if (gF != 0.f) {
+      const long ua = iu(d, I, J - 1, K), ub = iu(d, I, J, K);
+      const float mass = 0.5f * (r * u[ua] + r * u[ub]);
+      const long base = iv(d, a, J, K);
+      for (int t = 0; t < 6; ++t) q6[t] = v[base + t * sx];
+      float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+      scatter6(lv, base, sx, g6, gF * mass);
+      const float gm = gF * R * 0.5f * r;
+      aadd(lu + ua, gm);
+      aadd(lu + ub, gm);
     }
   }
-  float vc = v[iv(d, I, J, K)];
-  aadd(lv + iv(d, I, J, K), mu * 0.5f * (fd_at(d, rc, u, v, w, I, J - 1, K) + fd_at(d, rc, u, v, w, I, J, K)));
-  fd_vjp(d, rc, I, J - 1, K, mu * 0.5f * vc, lu, lv, lw);
-  fd_vjp(d, rc, I, J, K, mu * 0.5f * vc, lu, lv, lw);
+  if (a >= d.nx) return;
+  {
+    // y flux at scalar cell b+3: lower flux of v face b, upper flux of v face b-1
+    const float ml = b > 0 ? mu_vface(d, lam, rc, a, b - 1, k) : 0.f;
+    const float gF = (mc - ml) * d.idy;
+    const long base = iv(d, I, b, K);
+    if (gF != 0.f) {
+      for (int t = 0; t < 6; ++t) q6[t] = v[base + t * sy];
+      const float mass = 0.5f * (q6[2] + q6[3]) * r;
+      float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+      scatter6(lv, base, sy, g6, gF * mass);
+      const float gm = gF * R * 0.5f * r;
+      aadd(lv + base + 2 * sy, gm);
+      aadd(lv + base + 3 * sy, gm);
+    }
+    // div(rho0 u) of cell (I, b+2), averaged by v faces b-1 and b
+    const float gl = ml * v[base + 2 * sy], gr = mc * v[base + 3 * sy];
+    fd_vjp(d, rc, I, b + 2, K, 0.5f * (gl + gr), lu, lv, lw);
+  }
+  if (!vpt) return;
+  const int J = b + 3;
+  const long vb = iv(d, I, J, 0);
+  const float* col = v + vb;
+  float* lcol = lv + vb;
+  if (k > 0) {
+    const float gF = (mc - mu_vface(d, lam, rc, a, b, k - 1)) * d.idz;
+    if (gF != 0.f) {
+      const float rf = 0.5f * (rc[k] + rc[k + 1]);
+      const long wa = iw(d, I, J - 1, K), wb = iw(d, I, J, K);
+      const float mass = 0.5f * (rf * w[wa] + rf * w[wb]);
+      float R = vrecon_grad(col, d.nz + 2, k, upwind_pos(mass), weps, g6);
+      scatter6z(lcol, k, d.nz + 2, g6, gF * mass);
+      if (R != 0.f) {
+        const float gm = gF * R * 0.5f * rf;
+        aadd(lw + wa, gm);
+        aadd(lw + wb, gm);
+      }
+    }
+  }
+  aadd(lcol + K, mc * 0.5f * (fd_at(d, rc, u, v, w, I, J - 1, K) + fd_at(d, rc, u, v, w, I, J, K)));
 }
 
 // reverse of advection_w (advection.py:117-149), lam: (nx, ny, nz+1); levels 0 and nz
-// carry no advection
+// carry no advection.  Thread (a, b, t), a in [0, nx], b in [0, ny], t in [0, nz]: x flux
+// at (a+3, b+3) on w level t+1 (b < ny), y flux there (a < nx), z flux at scalar level t,
+// own term of w level t and div(rho0 u) of cell level t+1 (a < nx, b < ny).
 __global__ void k_adv_w_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
                             const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ lam,
                             float* __restrict__ lu, float* __restrict__ lv, float* __restrict__ lw) {
-  int l = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int i = blockIdx.z;
-  if (l >= d.nz || l < 1 || j >= d.ny) return;
-  float L = lam[((long)i * d.ny + j) * (d.nz + 1) + l];
-  if (L == 0.f) return;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y * blockDim.y + threadIdx.y;
+  int a = blockIdx.z;
+  if (t > d.nz || b > d.ny) return;
   const float weps = 1.0e-16f;
-  int I = i + 3, J = j + 3, K = l + 1;
-  float mu = L / (0.5f * (rc[K - 1] + rc[K]));
+  const int I = a + 3, J = b + 3, K = t + 1;
+  const bool lev = t >= 1 && t <= d.nz - 1;  // advected w level
+  const bool col_in = a < d.nx && b < d.ny;
+  const float mc = (lev && col_in) ? mu_wlev(d, lam, rc, a, b, t) : 0.f;
   const long sx = d.sxw(), sy = d.syw();
   float q6[6], g6[6];
-  for (int t2 = 0; t2 < 2; ++t2) {
-    int m = I + t2;
-    float lf = (t2 == 0 ? mu : -mu) * d.idx;
-    float mass = 0.5f * (rc[K - 1] * u[iu(d, m, J, K - 1)] + rc[K] * u[iu(d, m, J, K)]);
-    long base = iw(d, m - 3, J, K);
-    for (int t = 0; t < 6; ++t) q6[t] = w[base + t * sx];
-    float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
-    scatter6(lw, base, sx, g6, lf * mass);
-    aadd(lu + iu(d, m, J, K - 1), lf * R * 0.5f * rc[K - 1]);
-    aadd(lu + iu(d, m, J, K), lf * R * 0.5f * rc[K]);
+  if (lev && b < d.ny) {  // x flux: lower flux of w column a, upper flux of column a-1
+    const float ml = a > 0 ? mu_wlev(d, lam, rc, a - 1, b, t) : 0.f;
+    const float gF = (mc - ml) * d.idx;
+    if (gF != 0.f) {
+      const long u0 = iu(d, I, J, K - 1), u1 = iu(d, I, J, K);
+      const float mass = 0.5f * (rc[K - 1] * u[u0] + rc[K] * u[u1]);
+      const long base = iw(d, a, J, K);
+      for (int q = 0; q < 6; ++q) q6[q] = w[base + q * sx];
+      float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+      scatter6(lw, base, sx, g6, gF * mass);
+      aadd(lu + u0, gF * R * 0.5f * rc[K - 1]);
+      aadd(lu + u1, gF * R * 0.5f * rc[K]);
+    }
   }
-  for (int t2 = 0; t2 < 2; ++t2) {
-    int m = J + t2;
-    float lf = (t2 == 0 ? mu : -mu) * d.idy;
-    float mass = 0.5f * (rc[K - 1] * v[iv(d, I, m, K - 1)] + rc[K] * v[iv(d, I, m, K)]);
-    long base = iw(d, I, m - 3, K);
-    for (int t = 0; t < 6; ++t) q6[t] = w[base + t * sy];
-    float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
-    scatter6(lw, base, sy, g6, lf * mass);
-    aadd(lv + iv(d, I, m, K - 1), lf * R * 0.5f * rc[K - 1]);
-    aadd(lv + iv(d, I, m, K), lf * R * 0.5f * rc[K]);
+  if (lev && a < d.nx) {  // y flux: lower flux of w row b, upper flux of row b-1
+    const float ml = b > 0 ? mu_wlev(d, lam, rc, a, b - 1, t) : 0.f;
+    const float gF = (mc - ml) * d.idy;
+    if (gF != 0.f) {
+      const long v0 = iv(d, I, J, K - 1), v1 = iv(d, I, J, K);
+      const float mass = 0.5f * (rc[K - 1] * v[v0] + rc[K] * v[v1]);
+      const long base = iw(d, I, b, K);
+      for (int q = 0; q < 6; ++q) q6[q] = w[base + q * sy];
+      float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+      scatter6(lw, base, sy, g6, gF * mass);
+      aadd(lv + v0, gF * R * 0.5f * rc[K - 1]);
+      aadd(lv + v1, gF * R * 0.5f * rc[K]);
+    }
   }
+  if (!col_in) return;
   const long cb = iw(d, I, J, 0);
   const float* col = w + cb;
   float* lcol = lw + cb;
-  for (int t2 = 0; t2 < 2; ++t2) {
-    int cz = K - 1 + t2;  // scalar level: face between w[cz] and w[cz+1]
-    float lf = (t2 == 0 ? mu : -mu) * d.idz;
-    float mass = 0.5f * (col[cz] + col[cz + 1]) * rc[cz];
-    float R = vrecon_grad(col, d.nz + 3, cz, upwind_pos(mass), weps, g6);
-    scatter6z(lcol, cz, d.nz + 3, g6, lf * mass);
-    if (R != 0.f) {
-      float gm = lf * R * 0.5f * rc[cz];
-      aadd(lcol + cz, gm);
-      aadd(lcol + cz + 1, gm);
+  // mu of w level t-1 (advected for t-1 >= 1)
+  const float mlo = (t >= 2 && t - 1 <= d.nz - 1) ? mu_wlev(d, lam, rc, a, b, t - 1) : 0.f;
+  if (t >= 1) {  // z flux at scalar level t: lower flux of w level t, upper flux of level t-1
+    const float gF = (mc - mlo) * d.idz;
+    if (gF != 0.f) {
+      const int cz = t;
+      const float mass = 0.5f * (col[cz] + col[cz + 1]) * rc[cz];
+      float R = vrecon_grad(col, d.nz + 3, cz, upwind_pos(mass), weps, g6);
+      scatter6z(lcol, cz, d.nz + 3, g6, gF * mass);
+      if (R != 0.f) {
+        const float gm = gF * R * 0.5f * rc[cz];
+        aadd(lcol + cz, gm);
+        aadd(lcol + cz + 1, gm);
+      }
     }
   }
-  float wc = col[K];
-  aadd(lcol + K, mu * 0.5f * (fd_at(d, rc, u, v, w, I, J, K - 1) + fd_at(d, rc, u, v, w, I, J, K)));
-  fd_vjp(d, rc, I, J, K - 1, mu * 0.5f * wc, lu, lv, lw);
-  fd_vjp(d, rc, I, J, K, mu * 0.5f * wc, lu, lv, lw);
+  if (lev) aadd(lcol + K, mc * 0.5f * (fd_at(d, rc, u, v, w, I, J, K - 1) + fd_at(d, rc, u, v, w, I, J, K)));
+  if (t < d.nz) {
+    // div(rho0 u) of cell level K = t+1, averaged by w levels t (its lower) and t+1 (its upper)
+    const float mup = (t + 1 <= d.nz - 1) ? mu_wlev(d, lam, rc, a, b, t + 1) : 0.f;
+    const float gsum = mc * col[K] + mup * col[K + 1];
+    fd_vjp(d, rc, I, J, K, 0.5f * gsum, lu, lv, lw);
+  }
 }
 
 // ------------------------------------------------------------------ B1 buoyancy
@@ -1016,6 +1118,7 @@ __global__ void k_fold(Dims d, int mode, float* __restrict__ a) {
   zs[0] = k + 1;
   if (k == 0) zs[nzs++] = 0;
   if (k == nk - 1) zs[nzs++] = nk + 1;
+  if (nxs == 1 && nys == 1 && nzs == 1) return;  // no ghost image: nothing to move
   auto idx = [&](int X, int Y, int Z) -> long {
     if (mode == 0) return ic(d, X, Y, Z);
     if (mode == 1) return iu(d, X, Y, Z);

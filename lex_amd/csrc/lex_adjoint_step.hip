@@ -311,7 +311,7 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
       float weps = (q == 0) ? 1.0e-17f : 1.0e-18f;
       hipLaunchKernelGGL(k_scalar_update_vjp, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, (const float*)ldst[fi],
                          (float)dt_s, lam_in[fi], lT, turb ? lss[q] : nullptr);
-      hipLaunchKernelGGL(k_adv_scalar_vjp, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, rc, (const float*)src[fi],
+      hipLaunchKernelGGL(k_adv_scalar_vjp, cx.G(d.nz, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, (const float*)src[fi],
                          (const float*)src[1], (const float*)src[2], (const float*)src[3], (const float*)lT, weps,
                          lsrc[fi], lsrc[1], lsrc[2], lsrc[3]);
     }
@@ -351,11 +351,11 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
       hipLaunchKernelGGL(k_add_to, dim3((unsigned)((L.iw_ + 255) / 256)), dim3(256), 0, st, (long)L.iw_,
                          (const float*)lrw, lsw);
     }
-    hipLaunchKernelGGL(k_adv_u_vjp, cx.G(d.nz, d.ny, d.nx + 1), cx.bk, 0, st, d, rc, (const float*)src[1],
+    hipLaunchKernelGGL(k_adv_u_vjp, cx.G(d.nz, d.ny + 1, d.nx + 2), cx.bk, 0, st, d, rc, (const float*)src[1],
                        (const float*)src[2], (const float*)src[3], (const float*)lru, lsrc[1], lsrc[2], lsrc[3]);
-    hipLaunchKernelGGL(k_adv_v_vjp, cx.G(d.nz, d.ny + 1, d.nx), cx.bk, 0, st, d, rc, (const float*)src[1],
+    hipLaunchKernelGGL(k_adv_v_vjp, cx.G(d.nz, d.ny + 2, d.nx + 1), cx.bk, 0, st, d, rc, (const float*)src[1],
                        (const float*)src[2], (const float*)src[3], (const float*)lrv, lsrc[1], lsrc[2], lsrc[3]);
-    hipLaunchKernelGGL(k_adv_w_vjp, cx.G(d.nz + 1, d.ny, d.nx), cx.bk, 0, st, d, rc, (const float*)src[1],
+    hipLaunchKernelGGL(k_adv_w_vjp, cx.G(d.nz + 1, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, (const float*)src[1],
                        (const float*)src[2], (const float*)src[3], (const float*)lrw, lsrc[1], lsrc[2], lsrc[3]);
     hipLaunchKernelGGL(k_b8w_vjp, cx.G(d.nz + 1, d.ny, d.nx), cx.bk, 0, st, d, b.theta0, b.qv0,
                        (const float*)src[0], (const float*)src[5], (const float*)lrw, lsrc[0], lsrc[5]);

@@ -482,7 +482,7 @@ int lex_adv_scalar_vjp(const lex_grid_t* g, const float* r, const float* s, cons
   REQGRID(g);
   REQ(r); REQ(s); REQ(u); REQ(v); REQ(w); REQ(lam); REQ(ls); REQ(lu); REQ(lv); REQ(lw);
   Dims d = mkdims(g);
-  hipLaunchKernelGGL(k_adv_scalar_vjp, grd(d.nz, d.ny, d.nx), blk(), 0, S(stream), d, r, s, u, v, w, lam, weps, ls,
+  hipLaunchKernelGGL(k_adv_scalar_vjp, grd(d.nz, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, r, s, u, v, w, lam, weps, ls,
                      lu, lv, lw);
   return check_launch("lex_adv_scalar_vjp");
 }
@@ -492,7 +492,7 @@ int lex_adv_u_vjp(const lex_grid_t* g, const float* r, const float* u, const flo
   REQGRID(g);
   REQ(r); REQ(u); REQ(v); REQ(w); REQ(lam); REQ(lu); REQ(lv); REQ(lw);
   Dims d = mkdims(g);
-  hipLaunchKernelGGL(k_adv_u_vjp, grd(d.nz, d.ny, d.nx + 1), blk(), 0, S(stream), d, r, u, v, w, lam, lu, lv, lw);
+  hipLaunchKernelGGL(k_adv_u_vjp, grd(d.nz, d.ny + 1, d.nx + 2), blk(), 0, S(stream), d, r, u, v, w, lam, lu, lv, lw);
   return check_launch("lex_adv_u_vjp");
 }
 
@@ -501,7 +501,7 @@ int lex_adv_v_vjp(const lex_grid_t* g, const float* r, const float* u, const flo
   REQGRID(g);
   REQ(r); REQ(u); REQ(v); REQ(w); REQ(lam); REQ(lu); REQ(lv); REQ(lw);
   Dims d = mkdims(g);
-  hipLaunchKernelGGL(k_adv_v_vjp, grd(d.nz, d.ny + 1, d.nx), blk(), 0, S(stream), d, r, u, v, w, lam, lu, lv, lw);
+  hipLaunchKernelGGL(k_adv_v_vjp, grd(d.nz, d.ny + 2, d.nx + 1), blk(), 0, S(stream), d, r, u, v, w, lam, lu, lv, lw);
   return check_launch("lex_adv_v_vjp");
 }
 
@@ -510,7 +510,7 @@ int lex_adv_w_vjp(const lex_grid_t* g, const float* r, const float* u, const flo
   REQGRID(g);
   REQ(r); REQ(u); REQ(v); REQ(w); REQ(lam); REQ(lu); REQ(lv); REQ(lw);
   Dims d = mkdims(g);
-  hipLaunchKernelGGL(k_adv_w_vjp, grd(d.nz + 1, d.ny, d.nx), blk(), 0, S(stream), d, r, u, v, w, lam, lu, lv, lw);
+  hipLaunchKernelGGL(k_adv_w_vjp, grd(d.nz + 1, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, r, u, v, w, lam, lu, lv, lw);
   return check_launch("lex_adv_w_vjp");
 }
 
