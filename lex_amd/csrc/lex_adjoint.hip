@@ -829,53 +829,263 @@ __global__ void k_acoustic_vjp_faces2(Dims d, const float* __restrict__ rc, cons
   }
 }
 
+// Atomic-free reverse used by the step driver, in two passes.
+// k_acoustic_vjp_faces3: per face, the total adjoint g of the new velocity = its folded
+// output adjoint + the terms of the (one or two) cells whose pressure tendency reads it;
+// then old velocity (+= when acc, else =), rhs (+= g dts, one owner per face), and
+// gdt = g dts into gu/gv/gw (interior face layout) for the cell pass.
+__global__ void k_acoustic_vjp_faces3(Dims d, const float* __restrict__ rc, const float* __restrict__ pi0c,
+                                      const float* __restrict__ ps, const float* __restrict__ lpo,
+                                      const float* __restrict__ luo, const float* __restrict__ lvo,
+                                      const float* __restrict__ lwo, float dts, int acc, float* __restrict__ lu,
+                                      float* __restrict__ lv, float* __restrict__ lw, float* __restrict__ lru,
+                                      float* __restrict__ lrv, float* __restrict__ lrw, float* __restrict__ gu,
+                                      float* __restrict__ gv, float* __restrict__ gw) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nz
+  int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
+  int i = blockIdx.z;                             // 0..nx
+  if (k > d.nz || j > d.ny) return;
+  const float cr = (float)(-kRd / kCv);
+  const float g11 = 1.1f * dts;
+  int I = i + 3, J = j + 3, K = k + 1;
+  if (k < d.nz && j < d.ny) {  // u face i
+    long o = ((long)i * d.ny + j) * d.nz + k;
+    long a = iu(d, I, J, K);
+    long cR = ic(d, I, J, K), cL = cR - d.sxc();
+    float dps = (ps[cR] - ps[cL]) * d.idx;
+    float g = luo[a];
+    if (i < d.nx) g += g11 * lpo[cR] * (-cr * (pi0c[K] + ps[cR]) * d.idx - 0.5f * dps);
+    if (i > 0) g += g11 * lpo[cL] * (cr * (pi0c[K] + ps[cL]) * d.idx - 0.5f * dps);
+    lu[a] = acc ? lu[a] + g : g;
+    const float gd = g * dts;
+    lru[o] += gd;
+    gu[o] = gd;
+  }
+  if (k < d.nz && i < d.nx) {  // v face j
+    long o = ((long)i * (d.ny + 1) + j) * d.nz + k;
+    long a = iv(d, I, J, K);
+    long cR = ic(d, I, J, K), cL = cR - d.syc();
+    float dps = (ps[cR] - ps[cL]) * d.idy;
+    float g = lvo[a];
+    if (j < d.ny) g += g11 * lpo[cR] * (-cr * (pi0c[K] + ps[cR]) * d.idy - 0.5f * dps);
+    if (j > 0) g += g11 * lpo[cL] * (cr * (pi0c[K] + ps[cL]) * d.idy - 0.5f * dps);
+    lv[a] = acc ? lv[a] + g : g;
+    const float gd = g * dts;
+    lrv[o] += gd;
+    gv[o] = gd;
+  }
+  if (i < d.nx && j < d.ny && k > 0 && k < d.nz) {  // w level k: cell above (K) and below (K-1)
+    long o = ((long)i * d.ny + j) * (d.nz + 1) + k;
+    long a = iw(d, I, J, K);
+    long ca = ic(d, I, J, K), cb = ca - 1;
+    float r8 = 0.5f * (rc[K - 1] + rc[K]);
+    float dzz = (pi0c[K] - pi0c[K - 1]) * d.idz + (ps[ca] - ps[cb]) * d.idz;
+    float g = lwo[a];
+    g += g11 * lpo[ca] * (-cr * (pi0c[K] + ps[ca]) * d.idz + (-0.5f * r8 / rc[K]) * dzz);
+    g += g11 * lpo[cb] * (cr * (pi0c[K - 1] + ps[cb]) * d.idz + (-0.5f * r8 / rc[K - 1]) * dzz);
+    lw[a] = acc ? lw[a] + g : g;
+    const float gd = g * dts;
+    lrw[o] += gd;
+    gw[o] = gd;
+  }
+}
+
+// k_acoustic_vjp_cells3: thread per padded cell of the interior and its one-cell x/y
+// ring (levels 1..nz).  Gathers (a) the new pi' adjoint (identity) and the PGF terms of
+// the cell's six faces (weights gdt from the face pass) into lp and ltr, and (b) the
+// advection_pi terms of the cell and of its six neighbours into lps; the new face
+// velocities are recomputed from the substep's inputs as in the forward.
+// lp and lps may be the same array (the first substep of the first stage).
+__global__ void k_acoustic_vjp_cells3(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
+                                      const float* __restrict__ v, const float* __restrict__ w,
+                                      const float* __restrict__ p, const float* __restrict__ ru,
+                                      const float* __restrict__ rv, const float* __restrict__ rw,
+                                      const float* __restrict__ tr, const float* __restrict__ lpo,
+                                      const float* __restrict__ gu, const float* __restrict__ gv,
+                                      const float* __restrict__ gw, float dts, float* lp, float* __restrict__ ltr,
+                                      float* lps) {
+  const int Z = blockIdx.x * blockDim.x + threadIdx.x + 1;  // 1..nz
+  const int Y = blockIdx.y * blockDim.y + threadIdx.y + 2;  // 2..ny+3
+  const int X = blockIdx.z + 2;                             // 2..nx+3
+  if (Z > d.nz || Y > d.ny + 3) return;
+  const long sx = d.sxc(), sy = d.syc();
+  const long c = ic(d, X, Y, Z);
+  const int i = X - 3, j = Y - 3, k = Z - 1;
+  const bool in_x = i >= 0 && i < d.nx, in_y = j >= 0 && j < d.ny, inner = in_x && in_y;
+  const float h = (float)(-kCp * 0.5);
+  const float pc = p[c], tc = tr[c];
+  float U0 = 0.f, U1 = 0.f, V0 = 0.f, V1 = 0.f, W0 = 0.f, W1 = 0.f;
+  float alp = 0.f, atr = 0.f;  // lp and ltr sums
+  // u faces X (this cell on its + side) and X+1 (- side)
+  if (in_y && i >= 0 && i <= d.nx) {
+    const long o = ((long)i * d.ny + j) * d.nz + k;
+    const float dp = (pc - p[c - sx]) * d.idx, ts = tc + tr[c - sx];
+    U0 = u[iu(d, X, Y, Z)] + (h * ts * dp + ru[o]) * dts;
+    const float g = gu[o];
+    atr += g * h * dp;
+    alp += g * h * ts * d.idx;
+  }
+  if (in_y && i + 1 >= 0 && i + 1 <= d.nx) {
+    const long o = ((long)(i + 1) * d.ny + j) * d.nz + k;
+    const float dp = (p[c + sx] - pc) * d.idx, ts = tr[c + sx] + tc;
+    U1 = u[iu(d, X + 1, Y, Z)] + (h * ts * dp + ru[o]) * dts;
+    const float g = gu[o];
+    atr += g * h * dp;
+    alp -= g * h * ts * d.idx;
+  }
+  if (in_x && j >= 0 && j <= d.ny) {
+    const long o = ((long)i * (d.ny + 1) + j) * d.nz + k;
+    const float dp = (pc - p[c - sy]) * d.idy, ts = tc + tr[c - sy];
+    V0 = v[iv(d, X, Y, Z)] + (h * ts * dp + rv[o]) * dts;
+    const float g = gv[o];
+    atr += g * h * dp;
+    alp += g * h * ts * d.idy;
+  }
+  if (in_x && j + 1 >= 0 && j + 1 <= d.ny) {
+    const long o = ((long)i * (d.ny + 1) + j + 1) * d.nz + k;
+    const float dp = (p[c + sy] - pc) * d.idy, ts = tr[c + sy] + tc;
+    V1 = v[iv(d, X, Y + 1, Z)] + (h * ts * dp + rv[o]) * dts;
+    const float g = gv[o];
+    atr += g * h * dp;
+    alp -= g * h * ts * d.idy;
+  }
+  // w levels Z (below the cell) and Z+1 (above); zero at the surface and the lid
+  const long ow = ((long)i * d.ny + j) * (d.nz + 1) + k;
+  if (inner && k >= 1) {
+    const float dp = (pc - p[c - 1]) * d.idz, ts = tc + tr[c - 1];
+    W0 = w[iw(d, X, Y, Z)] + (h * ts * dp + rw[ow]) * dts;
+    const float g = gw[ow];
+    atr += g * h * dp;
+    alp += g * h * ts * d.idz;
+  }
+  if (inner && k + 1 <= d.nz - 1) {
+    const float dp = (p[c + 1] - pc) * d.idz, ts = tr[c + 1] + tc;
+    W1 = w[iw(d, X, Y, Z + 1)] + (h * ts * dp + rw[ow + 1]) * dts;
+    const float g = gw[ow + 1];
+    atr += g * h * dp;
+    alp -= g * h * ts * d.idz;
+  }
+  // advection_pi reverse (one_step_integration.py:543-546): G = 1.1 dts lam(new pi')
+  const float g11 = 1.1f * dts;
+  const float cr = (float)(-kRd / kCv);
+  float aps = 0.f;
+  auto lpo_in = [&](int x, int y, int z) -> float {
+    return (x >= 3 && x < d.nx + 3 && y >= 3 && y < d.ny + 3 && z >= 1 && z <= d.nz) ? lpo[ic(d, x, y, z)] : 0.f;
+  };
+  if (inner) {
+    const float Lc = lpo[c];
+    alp += Lc;
+    const float G = g11 * Lc;
+    const float div = (U1 - U0) * d.idx + (V1 - V0) * d.idy + (W1 - W0) * d.idz;
+    const float gx = -0.5f * G * d.idx, gy = -0.5f * G * d.idy;
+    aps += cr * div * G + (gx * U0 - gx * U1) + (gy * V0 - gy * V1);
+    if (k > 0) aps += (-0.5f * (0.5f * (rc[Z - 1] + rc[Z])) / rc[Z] * G) * W0 * d.idz;
+    if (k < d.nz - 1) aps -= (-0.5f * (0.5f * (rc[Z] + rc[Z + 1])) / rc[Z] * G) * W1 * d.idz;
+    // the cells above and below read the shared w level
+    if (k + 1 <= d.nz - 1) {
+      const float Ga = g11 * lpo[c + 1];
+      aps -= (-0.5f * (0.5f * (rc[Z] + rc[Z + 1])) / rc[Z + 1] * Ga) * W1 * d.idz;
+    }
+    if (k >= 1) {
+      const float Gb = g11 * lpo[c - 1];
+      aps += (-0.5f * (0.5f * (rc[Z - 1] + rc[Z])) / rc[Z - 1] * Gb) * W0 * d.idz;
+    }
+  }
+  // x and y neighbours read the shared face velocities
+  aps += 0.5f * g11 * d.idx * (lpo_in(X + 1, Y, Z) * U1 - lpo_in(X - 1, Y, Z) * U0);
+  aps += 0.5f * g11 * d.idy * (lpo_in(X, Y + 1, Z) * V1 - lpo_in(X, Y - 1, Z) * V0);
+  if (lp == lps) {
+    const float t = alp + aps;
+    if (t != 0.f) lp[c] += t;
+  } else {
+    if (alp != 0.f) lp[c] += alp;
+    if (aps != 0.f) lps[c] += aps;
+  }
+  if (atr != 0.f) ltr[c] += atr;
+}
+
 // ------------------------------------------------------------- G1 Smagorinsky
-// SGS scalar diffusion (turbulence.py:80-109): lam (nx, ny, nz) -> kh, s adjoints
+// Gather-form reverses: one thread per padded cell (or momentum point) sums every
+// contribution of the flux and stress stencils that read it, so each adjoint element has
+// a single writer and no atomics are needed.  Ghost cells receive the contributions of
+// the boundary stencils; the caller folds them as for the other padded adjoints.
+
+// -lam/rho0 of an interior cell, 0 elsewhere (padded indices)
+__device__ __forceinline__ float sgs_ldiv(const Dims& d, const float* __restrict__ lam, const float* __restrict__ rc,
+                                          int X, int Y, int Z) {
+  const int i = X - 3, j = Y - 3, k = Z - 1;
+  if (i < 0 || i >= d.nx || j < 0 || j >= d.ny || k < 0 || k >= d.nz) return 0.f;
+  return -lam[((long)i * d.ny + j) * d.nz + k] / rc[Z];
+}
+
+// SGS scalar diffusion (turbulence.py:80-109): lam (nx, ny, nz) -> kh, s adjoints.  The
+// flux through the face below padded cell cc along axis sd,
+//   t = -r8 0.5 (kh[cc] + kh[cc-sd]) (s[cc] - s[cc-sd]) id,
+// enters the two cells' tendencies with total weight g = (ldiv[cc-sd] - ldiv[cc]) id.
+// Thread per padded cell X in [2, nx+3], Y in [2, ny+3], Z in [1, nz].
 __global__ void k_smag_scalar_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ kh,
                                   const float* __restrict__ s, const float* __restrict__ lam, float* __restrict__ lkh,
                                   float* __restrict__ ls) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int i = blockIdx.z;
-  if (k >= d.nz || j >= d.ny) return;
-  float L = lam[((long)i * d.ny + j) * d.nz + k];
-  if (L == 0.f) return;
+  const int Z = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int X = blockIdx.z;
+  if (Z < 1 || Z > d.nz || Y < 2 || Y > d.ny + 3 || X < 2 || X > d.nx + 3) return;
   const long sx = d.sxc(), sy = d.syc();
-  int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
-  long c = ic(d, I, J, K);
-  float r = rc[K];
-  float ldiv = -L / r;
-  float r8u = 0.5f * (r + r);
-  // flux t(cc) = -r8 * 0.5 (kh[cc] + kh[cc-sd]) * (s[cc] - s[cc-sd]) * id, weight g
-  auto tb = [&](long cc, long sd, float id, float r8, float g) {
-    float ds = (s[cc] - s[cc - sd]) * id;
-    float kk = 0.5f * (kh[cc] + kh[cc - sd]);
-    float gk = -g * r8 * 0.5f * ds;
-    aadd(lkh + cc, gk);
-    aadd(lkh + cc - sd, gk);
-    float gs = -g * r8 * kk * id;
-    aadd(ls + cc, gs);
-    aadd(ls + cc - sd, -gs);
+  const long c = ic(d, X, Y, Z);
+  const float lc = sgs_ldiv(d, lam, rc, X, Y, Z);
+  const float r = rc[Z];
+  float gkh = 0.f, gs = 0.f;
+  // this cell is cc (side +1) or cc - sd (side -1) of the face
+  auto face = [&](long cc, long sd, float id, float r8, float g, float side) {
+    if (g == 0.f) return;
+    const float ds = (s[cc] - s[cc - sd]) * id;
+    const float kk = 0.5f * (kh[cc] + kh[cc - sd]);
+    gkh += -g * r8 * 0.5f * ds;
+    gs += side * (-g * r8 * kk * id);
   };
-  tb(c + sx, sx, d.idx, r8u, ldiv * d.idx);
-  tb(c, sx, d.idx, r8u, -ldiv * d.idx);
-  tb(c + sy, sy, d.idy, r8u, ldiv * d.idy);
-  tb(c, sy, d.idy, r8u, -ldiv * d.idy);
-  // tz(l), l = w level between padded levels l and l+1 (cells cl+1 over cl), zero at l = 0, nz
-  long col = c - K;
-  if (k + 1 < nz) tb(col + k + 2, 1, d.idz, 0.5f * (rc[k + 1] + rc[k + 2]), ldiv * d.idz);
-  if (k > 0) tb(col + k + 1, 1, d.idz, 0.5f * (rc[k] + rc[k + 1]), -ldiv * d.idz);
+  face(c, sx, d.idx, r, (sgs_ldiv(d, lam, rc, X - 1, Y, Z) - lc) * d.idx, 1.f);
+  face(c + sx, sx, d.idx, r, (lc - sgs_ldiv(d, lam, rc, X + 1, Y, Z)) * d.idx, -1.f);
+  face(c, sy, d.idy, r, (sgs_ldiv(d, lam, rc, X, Y - 1, Z) - lc) * d.idy, 1.f);
+  face(c + sy, sy, d.idy, r, (lc - sgs_ldiv(d, lam, rc, X, Y + 1, Z)) * d.idy, -1.f);
+  // vertical faces between levels 1..nz only (zero flux at the surface and the lid)
+  if (Z >= 2) face(c, 1, d.idz, 0.5f * (rc[Z - 1] + rc[Z]), (sgs_ldiv(d, lam, rc, X, Y, Z - 1) - lc) * d.idz, 1.f);
+  if (Z + 1 <= d.nz)
+    face(c + 1, 1, d.idz, 0.5f * (rc[Z] + rc[Z + 1]), (lc - sgs_ldiv(d, lam, rc, X, Y, Z + 1)) * d.idz, -1.f);
+  if (gkh != 0.f) lkh[c] += gkh;
+  if (gs != 0.f) ls[c] += gs;
 }
 
-// tau = -2 km s at padded index c, weight g
-__device__ __forceinline__ void tau_vjp(const float* __restrict__ km, const float* __restrict__ sij,
-                                        float* __restrict__ lkm, float* __restrict__ lsij, long c, float g) {
-  if (g == 0.f) return;
-  aadd(lkm + c, -2.0f * g * sij[c]);
-  aadd(lsij + c, -2.0f * g * km[c]);
+// -lam/rho0 weights of the SGS momentum tendencies at padded u, v points and w levels
+__device__ __forceinline__ float sgs_gu(const Dims& d, const float* __restrict__ lsu, const float* __restrict__ rc,
+                                        int X, int Y, int Z) {
+  const int f = X - 3, j = Y - 3, k = Z - 1;
+  if (f < 0 || f > d.nx || j < 0 || j >= d.ny || k < 0 || k >= d.nz) return 0.f;
+  const float r = rc[Z];
+  return -lsu[((long)f * d.ny + j) * d.nz + k] / (0.5f * (r + r));
+}
+__device__ __forceinline__ float sgs_gv(const Dims& d, const float* __restrict__ lsv, const float* __restrict__ rc,
+                                        int X, int Y, int Z) {
+  const int i = X - 3, f = Y - 3, k = Z - 1;
+  if (i < 0 || i >= d.nx || f < 0 || f > d.ny || k < 0 || k >= d.nz) return 0.f;
+  const float r = rc[Z];
+  return -lsv[((long)i * (d.ny + 1) + f) * d.nz + k] / (0.5f * (r + r));
+}
+__device__ __forceinline__ float sgs_gw(const Dims& d, const float* __restrict__ lsw, const float* __restrict__ rc,
+                                        int X, int Y, int l) {
+  const int i = X - 3, j = Y - 3;
+  if (i < 0 || i >= d.nx || j < 0 || j >= d.ny || l < 1 || l > d.nz - 1) return 0.f;
+  return -lsw[((long)i * d.ny + j) * (d.nz + 1) + l] / (0.5f * (rc[l] + rc[l + 1]));
 }
 
-// SGS momentum tendencies (turbulence.py:9-79): lam on u, v, w points -> km, s_ij adjoints
+// SGS momentum tendencies (turbulence.py:9-79): lam on u, v, w points -> km, s_ij
+// adjoints.  tau_ij = -2 km s_ij, so with G_ij = dL/dtau_ij at a cell,
+// lkm += -2 sum G_ij s_ij and ls_ij += -2 G_ij km.  G_ij gathers:
+//   d11 (u): -d(tau11)/dx at faces;  d22 (v), d33 (w) likewise;
+//   d12: tau12 averaged to the four cells around an edge, differenced across it (the two
+//        cells shared by both edges cancel);
+//   d13, d23 at u/v points: tau interpolated to w levels 1..nz-1, differenced vertically;
+//   at w levels: centred differences over two columns of both adjacent levels.
+// Thread per padded cell X in [2, nx+3], Y in [2, ny+3], Z in [1, nz].
 __global__ void k_smag_uvw_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ km,
                                const float* __restrict__ s11, const float* __restrict__ s22,
                                const float* __restrict__ s33, const float* __restrict__ s12,
@@ -884,89 +1094,69 @@ __global__ void k_smag_uvw_vjp(Dims d, const float* __restrict__ rc, const float
                                const float* __restrict__ lsw, float* __restrict__ lkm, float* __restrict__ l11,
                                float* __restrict__ l22, float* __restrict__ l33, float* __restrict__ l12,
                                float* __restrict__ l13, float* __restrict__ l23) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nz
-  int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
-  int i = blockIdx.z;                             // 0..nx
-  if (k > d.nz || j > d.ny) return;
-  const long sx = d.sxc(), sy = d.syc();
-  int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
-  // interpolate_scalar2w_0 of tau at w level l: 0.5 (tau[col+l] + tau[col+l+1]), l in 1..nz-1
-  auto tw_b = [&](long col, int l, float g, const float* sij, float* lsij) {
-    if (l <= 0 || l >= nz) return;
-    tau_vjp(km, sij, lkm, lsij, col + l, 0.5f * g);
-    tau_vjp(km, sij, lkm, lsij, col + l + 1, 0.5f * g);
-  };
-  if (k < nz && j < d.ny) {
-    float L = lsu[((long)i * d.ny + j) * nz + k];
-    if (L != 0.f) {
-      long c = ic(d, I, J, K);
-      float r = rc[K];
-      float g = -L / (0.5f * (r + r));  // adjoint of d11 + d12 + d13
-      tau_vjp(km, s11, lkm, l11, c, g * d.idx);
-      tau_vjp(km, s11, lkm, l11, c - sx, -g * d.idx);
-      const long cs[4] = {c - sx, c + sy, c - sx + sy, c};
-      for (int q = 0; q < 4; ++q) {
-        tau_vjp(km, s12, lkm, l12, cs[q], 0.25f * g * d.idy);
-        tau_vjp(km, s12, lkm, l12, cs[q] - sy, -0.25f * g * d.idy);
-      }
-      const long cz[2] = {c - sx, c};
-      for (int q = 0; q < 2; ++q) {
-        long col = cz[q] - K;
-        tw_b(col, k + 1, 0.5f * g * d.idz, s13, l13);
-        tw_b(col, k, -0.5f * g * d.idz, s13, l13);
-      }
-    }
-  }
-  if (k < nz && i < d.nx) {
-    float L = lsv[((long)i * (d.ny + 1) + j) * nz + k];
-    if (L != 0.f) {
-      long c = ic(d, I, J, K);
-      float r = rc[K];
-      float g = -L / (0.5f * (r + r));
-      tau_vjp(km, s22, lkm, l22, c, g * d.idy);
-      tau_vjp(km, s22, lkm, l22, c - sy, -g * d.idy);
-      const long cs[4] = {c - sy, c + sx, c, c + sx - sy};
-      for (int q = 0; q < 4; ++q) {
-        tau_vjp(km, s12, lkm, l12, cs[q], 0.25f * g * d.idx);
-        tau_vjp(km, s12, lkm, l12, cs[q] - sx, -0.25f * g * d.idx);
-      }
-      const long cz[2] = {c - sy, c};
-      for (int q = 0; q < 2; ++q) {
-        long col = cz[q] - K;
-        tw_b(col, k + 1, 0.5f * g * d.idz, s23, l23);
-        tw_b(col, k, -0.5f * g * d.idz, s23, l23);
-      }
-    }
-  }
-  if (i < d.nx && j < d.ny && k > 0 && k < nz) {
-    float L = lsw[((long)i * d.ny + j) * (nz + 1) + k];
-    if (L != 0.f) {
-      int l = k;
-      long cu = ic(d, I, J, l + 1), cl = ic(d, I, J, l);
-      float g = -L / (0.5f * (rc[l] + rc[l + 1]));
-      tau_vjp(km, s33, lkm, l33, cu, g * d.idz);
-      tau_vjp(km, s33, lkm, l33, cl, -g * d.idz);
-      float gx = 0.5f * g / (2.0f * d.dx), gy = 0.5f * g / (2.0f * d.dy);
-      const long cc[2] = {cl, cu};
-      for (int q = 0; q < 2; ++q) {
-        tau_vjp(km, s13, lkm, l13, cc[q] + sx, gx);
-        tau_vjp(km, s13, lkm, l13, cc[q] - sx, -gx);
-        tau_vjp(km, s23, lkm, l23, cc[q] + sy, gy);
-        tau_vjp(km, s23, lkm, l23, cc[q] - sy, -gy);
-      }
-    }
-  }
+  const int Z = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int X = blockIdx.z;
+  if (Z < 1 || Z > d.nz || Y < 2 || Y > d.ny + 3 || X < 2 || X > d.nx + 3) return;
+  const int nz = d.nz;
+  auto U = [&](int x, int y, int z) { return sgs_gu(d, lsu, rc, x, y, z); };
+  auto V = [&](int x, int y, int z) { return sgs_gv(d, lsv, rc, x, y, z); };
+  auto W = [&](int x, int y, int l) { return sgs_gw(d, lsw, rc, x, y, l); };
+  const float G11 = d.idx * (U(X, Y, Z) - U(X + 1, Y, Z));
+  const float G22 = d.idy * (V(X, Y, Z) - V(X, Y + 1, Z));
+  const float G33 = d.idz * (W(X, Y, Z - 1) - W(X, Y, Z));
+  const float G12 = 0.25f * d.idy * ((U(X + 1, Y - 1, Z) + U(X, Y - 1, Z)) - (U(X + 1, Y + 1, Z) + U(X, Y + 1, Z))) +
+                    0.25f * d.idx * ((V(X - 1, Y, Z) + V(X - 1, Y + 1, Z)) - (V(X + 1, Y + 1, Z) + V(X + 1, Y, Z)));
+  // vertical pattern of a u (v) column: level Z gathers +g(Z-1) (Z <= nz), -g(Z+1), and
+  // g(Z) where only one of the two w levels around Z is inside 1..nz-1
+  const float c0 = (Z <= nz - 1 ? 1.f : 0.f) - (Z >= 2 ? 1.f : 0.f);
+  const float cm = Z <= nz ? 1.f : 0.f;
+  auto zp = [&](float gm, float g0, float gp) { return c0 * g0 + cm * gm - gp; };
+  float G13 = 0.25f * d.idz *
+              (zp(U(X, Y, Z - 1), U(X, Y, Z), U(X, Y, Z + 1)) + zp(U(X + 1, Y, Z - 1), U(X + 1, Y, Z), U(X + 1, Y, Z + 1)));
+  float G23 = 0.25f * d.idz *
+              (zp(V(X, Y, Z - 1), V(X, Y, Z), V(X, Y, Z + 1)) + zp(V(X, Y + 1, Z - 1), V(X, Y + 1, Z), V(X, Y + 1, Z + 1)));
+  const float hx = 0.5f / (2.0f * d.dx), hy = 0.5f / (2.0f * d.dy);
+  G13 += hx * ((W(X - 1, Y, Z - 1) + W(X - 1, Y, Z)) - (W(X + 1, Y, Z - 1) + W(X + 1, Y, Z)));
+  G23 += hy * ((W(X, Y - 1, Z - 1) + W(X, Y - 1, Z)) - (W(X, Y + 1, Z - 1) + W(X, Y + 1, Z)));
+  const long c = ic(d, X, Y, Z);
+  const float k2 = -2.0f * km[c];
+  float a = 0.f;
+  if (G11 != 0.f) { a += G11 * s11[c]; l11[c] += k2 * G11; }
+  if (G22 != 0.f) { a += G22 * s22[c]; l22[c] += k2 * G22; }
+  if (G33 != 0.f) { a += G33 * s33[c]; l33[c] += k2 * G33; }
+  if (G12 != 0.f) { a += G12 * s12[c]; l12[c] += k2 * G12; }
+  if (G13 != 0.f) { a += G13 * s13[c]; l13[c] += k2 * G13; }
+  if (G23 != 0.f) { a += G23 * s23[c]; l23[c] += k2 * G23; }
+  if (a != 0.f) lkm[c] += -2.0f * a;
 }
 
 // compute_k (turbulence.py:430-446), elementwise over the padded array:
-// km = cg2 sqrt(max(s2 - n2/Pr, 0)), kh = km/Pr.  Writes ls2, ln2 (not accumulated) and
-// accumulates dL/dcg2 = sum(dL/dkm_total * sqrt(x)) into *lcg2 (FP64).
-__global__ void k_smag_k_vjp(long n, const float* __restrict__ s2, const float* __restrict__ n2,
-                             const float* __restrict__ lkm, const float* __restrict__ lkh, float csgs2, float prandtl,
-                             float* __restrict__ ls2, float* __restrict__ ln2, double* __restrict__ lcg2) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  float acc = 0.f;
-  if (t < n) {
+// km = cg2 sqrt(max(s2 - n2/Pr, 0)), kh = km/Pr.  Writes ls2, ln2 (not accumulated).
+// dL/dcg2 = sum(dL/dkm_total * sqrt(x)) is reduced in FP64 without atomics: a grid of
+// at most kSmagKBlocks blocks strides over the array, block b writes part[b], and
+// k_smag_k_sum adds the partials in a fixed order to *lcg2 (deterministic).
+constexpr int kSmagKBlocks = 512;
+
+inline unsigned smag_k_blocks(long n) {
+  const long b = (n + 255) / 256;
+  return (unsigned)(b < kSmagKBlocks ? b : kSmagKBlocks);
+}
+
+__device__ __forceinline__ double block_sum_256(double v) {
+  __shared__ double wsum[4];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+}
+
+__global__ void __launch_bounds__(256) k_smag_k_vjp(long n, const float* __restrict__ s2, const float* __restrict__ n2,
+                                                    const float* __restrict__ lkm, const float* __restrict__ lkh,
+                                                    float csgs2, float prandtl, float* __restrict__ ls2,
+                                                    float* __restrict__ ln2, double* __restrict__ part) {
+  double acc = 0.0;
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
     float x = s2[t] - n2[t] / prandtl;
     float G = lkm[t] + lkh[t] / prandtl;
     float a = 0.f, b = 0.f;
@@ -975,13 +1165,21 @@ __global__ void k_smag_k_vjp(long n, const float* __restrict__ s2, const float* 
       float gx = G * csgs2 * 0.5f / sq;
       a = gx;
       b = -gx / prandtl;
-      acc = G * sq;
+      acc += (double)(G * sq);
     }
     ls2[t] = a;
     ln2[t] = b;
   }
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if ((threadIdx.x & 63) == 0 && acc != 0.f) unsafeAtomicAdd(lcg2, (double)acc);
+  const double tot = block_sum_256(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// one block of 256 threads: *lcg2 += sum of part[0..nb)
+__global__ void __launch_bounds__(256) k_smag_k_sum(const double* __restrict__ part, int nb, double* __restrict__ lcg2) {
+  double a = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) a += part[b];
+  const double tot = block_sum_256(a);
+  if (threadIdx.x == 0) lcg2[0] += tot;
 }
 
 // compute_nm (turbulence.py:391-427): folded adjoint at interior cells -> theta, qv
@@ -1020,97 +1218,112 @@ __global__ void k_smag_nm_vjp(Dims d, const float* __restrict__ th, const float*
   }
 }
 
-// compute_deformation (turbulence.py:298-388): folded adjoints of rho*S_ij and of the
-// deformation at interior cells -> u, v, w adjoints
-__device__ __forceinline__ void e12_vjp(const Dims& d, float* lu, float* lv, int F, int G, int K, float g) {
-  // e12 = 0.5 ((u[F,G] - u[F,G-1]) idy + (v[F,G] - v[F-1,G]) idx)
-  aadd(lu + iu(d, F, G, K), 0.5f * g * d.idy);
-  aadd(lu + iu(d, F, G - 1, K), -0.5f * g * d.idy);
-  aadd(lv + iv(d, F, G, K), 0.5f * g * d.idx);
-  aadd(lv + iv(d, F - 1, G, K), -0.5f * g * d.idx);
-}
-__device__ __forceinline__ void e13_vjp(const Dims& d, float* lu, float* lw, int F, int J, int l, float g) {
-  // e13 = 0.5 ((u[F,J,l+1] - u[F,J,l]) idz + (w[F,J,l+1] - w[F-1,J,l+1]) idx)
-  aadd(lu + iu(d, F, J, l + 1), 0.5f * g * d.idz);
-  aadd(lu + iu(d, F, J, l), -0.5f * g * d.idz);
-  aadd(lw + iw(d, F, J, l + 1), 0.5f * g * d.idx);
-  aadd(lw + iw(d, F - 1, J, l + 1), -0.5f * g * d.idx);
-}
-__device__ __forceinline__ void e23_vjp(const Dims& d, float* lv, float* lw, int I, int G, int l, float g) {
-  aadd(lv + iv(d, I, G, l + 1), 0.5f * g * d.idz);
-  aadd(lv + iv(d, I, G, l), -0.5f * g * d.idz);
-  aadd(lw + iw(d, I, G, l + 1), 0.5f * g * d.idy);
-  aadd(lw + iw(d, I, G - 1, l + 1), -0.5f * g * d.idy);
+// compute_deformation (turbulence.py:298-388), reverse in two passes.  Pass 1 (interior
+// cells, in place): the folded adjoints of rho*S_ij (l_ij) and of the deformation (ldf)
+// become the weights H of the velocity differences each cell's strain reads:
+//   H11 = r g11, H22, H33 likewise; H12 = r g12 / 4; H13 = r g13, H23 = r g23, with
+//   g_ii = l_ii + 4 q s_ii, g_ij = l_ij + 8 q s_ij, q = ldf / r^2
+// (deformation = (2 (a11^2 + a22^2 + a33^2) + 4 (a12^2 + a13^2 + a23^2)) / r^2).
+__global__ void k_smag_deform_h(Dims d, const float* __restrict__ rc, const float* __restrict__ s11,
+                                const float* __restrict__ s22, const float* __restrict__ s33,
+                                const float* __restrict__ s12, const float* __restrict__ s13,
+                                const float* __restrict__ s23, const float* __restrict__ ldf, float* __restrict__ h11,
+                                float* __restrict__ h22, float* __restrict__ h33, float* __restrict__ h12,
+                                float* __restrict__ h13, float* __restrict__ h23) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  const int K = k + 1;
+  const long c = ic(d, i + 3, j + 3, K);
+  const float r = rc[K];
+  const float q = ldf[c] / (r * r);
+  h11[c] = (h11[c] + 4.0f * q * s11[c]) * r;
+  h22[c] = (h22[c] + 4.0f * q * s22[c]) * r;
+  h33[c] = (h33[c] + 4.0f * q * s33[c]) * r;
+  h12[c] = 0.25f * r * (h12[c] + 8.0f * q * s12[c]);
+  h13[c] = r * (h13[c] + 8.0f * q * s13[c]);
+  h23[c] = r * (h23[c] + 8.0f * q * s23[c]);
 }
 
-__global__ void k_smag_deform_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ s11,
-                                  const float* __restrict__ s22, const float* __restrict__ s33,
-                                  const float* __restrict__ s12, const float* __restrict__ s13,
-                                  const float* __restrict__ s23, const float* __restrict__ l11,
-                                  const float* __restrict__ l22, const float* __restrict__ l33,
-                                  const float* __restrict__ l12, const float* __restrict__ l13,
-                                  const float* __restrict__ l23, const float* __restrict__ ldf,
+// H of an interior cell, 0 elsewhere (padded indices)
+__device__ __forceinline__ float hcell(const Dims& d, const float* __restrict__ h, int X, int Y, int Z) {
+  if (X < 3 || X >= d.nx + 3 || Y < 3 || Y >= d.ny + 3 || Z < 1 || Z > d.nz) return 0.f;
+  return h[ic(d, X, Y, Z)];
+}
+
+// Weight of the xz (yz) edge at horizontal face position F between columns F-1 and F (of
+// the other axis' column Y), w level l+1: the strain a13 of an interior cell K in
+// [2, nz-1] averages the four edges around it (1/4 each); the bottom (top) cell
+// extrapolates from the three edges above (below) it with weights 1.875, -1.25, 0.375
+// (extrapolate_bottom_top), each halved by the two-face average.
+template <int AX>
+__device__ __forceinline__ float strain_edge(const Dims& d, const float* __restrict__ h, int F, int Y, int l) {
+  auto H2 = [&](int Z) {
+    return AX == 0 ? hcell(d, h, F, Y, Z) + hcell(d, h, F - 1, Y, Z) : hcell(d, h, Y, F, Z) + hcell(d, h, Y, F - 1, Z);
+  };
+  const int nz = d.nz;
+  float e = 0.f;
+  if (l + 1 >= 2 && l + 1 <= nz - 1) e += 0.25f * H2(l + 1);
+  if (l >= 2 && l <= nz - 1) e += 0.25f * H2(l);
+  if (l >= 1 && l <= 3) e += 0.5f * (l == 1 ? 1.875f : (l == 2 ? -1.25f : 0.375f)) * H2(1);
+  const int t = nz - 1 - l;
+  if (t >= 0 && t <= 2) e += 0.5f * (t == 0 ? 1.875f : (t == 1 ? -1.25f : 0.375f)) * H2(nz);
+  return e;
+}
+
+// weight of the xy edge (corner) at (F, G): the four cells around it
+__device__ __forceinline__ float strain_corner(const Dims& d, const float* __restrict__ h12, int F, int G, int Z) {
+  return (hcell(d, h12, F, G, Z) + hcell(d, h12, F - 1, G - 1, Z)) +
+         (hcell(d, h12, F, G - 1, Z) + hcell(d, h12, F - 1, G, Z));
+}
+
+// Pass 2: thread per padded velocity point (u, v, w arrays share the thread grid
+// (nx+7, ny+7, nz+3)); every velocity adjoint gathers its strain stencils.
+__global__ void k_smag_deform_vjp(Dims d, const float* __restrict__ h11, const float* __restrict__ h22,
+                                  const float* __restrict__ h33, const float* __restrict__ h12,
+                                  const float* __restrict__ h13, const float* __restrict__ h23,
                                   float* __restrict__ lu, float* __restrict__ lv, float* __restrict__ lw) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int i = blockIdx.z;
-  if (k >= d.nz || j >= d.ny) return;
-  int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
-  long c = ic(d, I, J, K);
-  float r = rc[K];
-  float q = ldf[c] / (r * r);
-  // deformation = (2 (a11^2 + a22^2 + a33^2) + 4 (a12^2 + a13^2 + a23^2)) / r^2
-  float g11 = l11[c] + 4.0f * q * s11[c];
-  float g22 = l22[c] + 4.0f * q * s22[c];
-  float g33 = l33[c] + 4.0f * q * s33[c];
-  float g12 = l12[c] + 8.0f * q * s12[c];
-  float g13 = l13[c] + 8.0f * q * s13[c];
-  float g23 = l23[c] + 8.0f * q * s23[c];
-  aadd(lu + iu(d, I + 1, J, K), g11 * r * d.idx);
-  aadd(lu + iu(d, I, J, K), -g11 * r * d.idx);
-  aadd(lv + iv(d, I, J + 1, K), g22 * r * d.idy);
-  aadd(lv + iv(d, I, J, K), -g22 * r * d.idy);
-  aadd(lw + iw(d, I, J, K + 1), g33 * r * d.idz);
-  aadd(lw + iw(d, I, J, K), -g33 * r * d.idz);
-  float h12 = 0.25f * r * g12;
-  e12_vjp(d, lu, lv, I, J, K, h12);
-  e12_vjp(d, lu, lv, I + 1, J + 1, K, h12);
-  e12_vjp(d, lu, lv, I, J + 1, K, h12);
-  e12_vjp(d, lu, lv, I + 1, J, K, h12);
-  float h13 = r * g13, h23 = r * g23;
-  if (k == 0 || k == nz - 1) {
-    int l0 = (k == 0) ? 1 : nz - 1, dl = (k == 0) ? 1 : -1;
-    const float cf[3] = {1.875f, -1.25f, 0.375f};
-    for (int t = 0; t < 3; ++t) {
-      int l = l0 + t * dl;
-      e13_vjp(d, lu, lw, I + 1, J, l, 0.5f * cf[t] * h13);
-      e13_vjp(d, lu, lw, I, J, l, 0.5f * cf[t] * h13);
-      e23_vjp(d, lv, lw, I, J + 1, l, 0.5f * cf[t] * h23);
-      e23_vjp(d, lv, lw, I, J, l, 0.5f * cf[t] * h23);
-    }
-  } else {
-    float a = 0.25f * h13, b = 0.25f * h23;
-    e13_vjp(d, lu, lw, I, J, k, a);
-    e13_vjp(d, lu, lw, I + 1, J, k + 1, a);
-    e13_vjp(d, lu, lw, I, J, k + 1, a);
-    e13_vjp(d, lu, lw, I + 1, J, k, a);
-    e23_vjp(d, lv, lw, I, J, k, b);
-    e23_vjp(d, lv, lw, I, J + 1, k + 1, b);
-    e23_vjp(d, lv, lw, I, J, k + 1, b);
-    e23_vjp(d, lv, lw, I, J + 1, k, b);
+  const int Z = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int X = blockIdx.z;
+  if (Z > d.nz + 2 || Y > d.ny + 6) return;
+  if (X < d.nx + 7 && Y < d.ny + 6 && Z < d.nz + 2) {  // u point (X, Y, Z)
+    float g = d.idx * (hcell(d, h11, X - 1, Y, Z) - hcell(d, h11, X, Y, Z)) +
+              0.5f * d.idy * (strain_corner(d, h12, X, Y, Z) - strain_corner(d, h12, X, Y + 1, Z)) +
+              0.5f * d.idz * (strain_edge<0>(d, h13, X, Y, Z - 1) - strain_edge<0>(d, h13, X, Y, Z));
+    if (g != 0.f) lu[iu(d, X, Y, Z)] += g;
+  }
+  if (X < d.nx + 6 && Y < d.ny + 7 && Z < d.nz + 2) {  // v point (X, Y, Z)
+    float g = d.idy * (hcell(d, h22, X, Y - 1, Z) - hcell(d, h22, X, Y, Z)) +
+              0.5f * d.idx * (strain_corner(d, h12, X, Y, Z) - strain_corner(d, h12, X + 1, Y, Z)) +
+              0.5f * d.idz * (strain_edge<1>(d, h23, Y, X, Z - 1) - strain_edge<1>(d, h23, Y, X, Z));
+    if (g != 0.f) lv[iv(d, X, Y, Z)] += g;
+  }
+  if (X < d.nx + 6 && Y < d.ny + 6) {  // w point (X, Y, Z), Z < nz+3
+    float g = d.idz * (hcell(d, h33, X, Y, Z - 1) - hcell(d, h33, X, Y, Z)) +
+              0.5f * d.idx * (strain_edge<0>(d, h13, X, Y, Z - 1) - strain_edge<0>(d, h13, X + 1, Y, Z - 1)) +
+              0.5f * d.idy * (strain_edge<1>(d, h23, Y, X, Z - 1) - strain_edge<1>(d, h23, Y + 1, X, Z - 1));
+    if (g != 0.f) lw[iw(d, X, Y, Z)] += g;
   }
 }
 
 // ------------------------------------------------------------------ H1 halo folds
 // Reverse of the padded stores: each ghost's adjoint is added to its interior source,
-// then cleared.  One thread per interior point; every ghost has exactly one source.
-// mode 0 scalar, 1 u, 2 v, 3 w.
-__global__ void k_fold(Dims d, int mode, float* __restrict__ a) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int i = blockIdx.z;
-  int ni = d.nx + (mode == 1 ? 1 : 0), nj = d.ny + (mode == 2 ? 1 : 0), nk = d.nz + (mode == 3 ? 1 : 0);
-  if (k >= nk || j >= nj || i >= ni) return;
+// then cleared.  Work is per interior point (every ghost has exactly one source), mode 0
+// scalar, 1 u, 2 v, 3 w.  Only points with ghost images do anything: those within four
+// points of an x or y edge (all levels) and the bottom and top levels of every column.
+__device__ __forceinline__ long fold_idx(const Dims& d, int mode, int X, int Y, int Z) {
+  if (mode == 0) return ic(d, X, Y, Z);
+  if (mode == 1) return iu(d, X, Y, Z);
+  if (mode == 2) return iv(d, X, Y, Z);
+  return iw(d, X, Y, Z);
+}
+
+// All images are loaded before anything is stored, so a point costs one memory round
+// trip (the loops have fixed bounds and stay in registers).
+__device__ __forceinline__ void fold_point(const Dims& d, int mode, float* __restrict__ a, int i, int j, int k,
+                                           int nk) {
   int xs[3], ys[3], zs[3];
   int nxs, nys, nzs = 1;
   if (mode == 1) ghost_list_face(i, d.nx, xs, nxs); else ghost_list_x(i, d.nx, xs, nxs);
@@ -1119,22 +1332,96 @@ __global__ void k_fold(Dims d, int mode, float* __restrict__ a) {
   if (k == 0) zs[nzs++] = 0;
   if (k == nk - 1) zs[nzs++] = nk + 1;
   if (nxs == 1 && nys == 1 && nzs == 1) return;  // no ghost image: nothing to move
-  auto idx = [&](int X, int Y, int Z) -> long {
-    if (mode == 0) return ic(d, X, Y, Z);
-    if (mode == 1) return iu(d, X, Y, Z);
-    if (mode == 2) return iv(d, X, Y, Z);
-    return iw(d, X, Y, Z);
-  };
+  float val[3][3][3];
+#pragma unroll
+  for (int a1 = 0; a1 < 3; ++a1)
+#pragma unroll
+    for (int b1 = 0; b1 < 3; ++b1)
+#pragma unroll
+      for (int c1 = 0; c1 < 3; ++c1)
+        val[a1][b1][c1] = (a1 < nxs && b1 < nys && c1 < nzs) ? a[fold_idx(d, mode, xs[a1], ys[b1], zs[c1])] : 0.f;
   float sum = 0.f;
-  for (int a1 = 0; a1 < nxs; ++a1)
-    for (int b1 = 0; b1 < nys; ++b1)
-      for (int c1 = 0; c1 < nzs; ++c1) {
-        if (a1 == 0 && b1 == 0 && c1 == 0) continue;
-        long g = idx(xs[a1], ys[b1], zs[c1]);
-        sum += a[g];
-        a[g] = 0.f;
-      }
-  a[idx(xs[0], ys[0], zs[0])] += sum;
+#pragma unroll
+  for (int a1 = 0; a1 < 3; ++a1)
+#pragma unroll
+    for (int b1 = 0; b1 < 3; ++b1)
+#pragma unroll
+      for (int c1 = 0; c1 < 3; ++c1)
+        if ((a1 | b1 | c1) != 0) sum += val[a1][b1][c1];
+#pragma unroll
+  for (int a1 = 0; a1 < 3; ++a1)
+#pragma unroll
+    for (int b1 = 0; b1 < 3; ++b1)
+#pragma unroll
+      for (int c1 = 0; c1 < 3; ++c1)
+        if ((a1 | b1 | c1) != 0 && a1 < nxs && b1 < nys && c1 < nzs) a[fold_idx(d, mode, xs[a1], ys[b1], zs[c1])] = 0.f;
+  a[fold_idx(d, mode, xs[0], ys[0], zs[0])] = val[0][0][0] + sum;
+}
+
+struct FoldExt {
+  int ni, nj, nk;
+  __host__ __device__ FoldExt(const Dims& d, int mode)
+      : ni(d.nx + (mode == 1 ? 1 : 0)), nj(d.ny + (mode == 2 ? 1 : 0)), nk(d.nz + (mode == 3 ? 1 : 0)) {}
+  // columns within 4 of an x or y edge: 8 full rows of nj, then 8 per remaining row
+  __host__ __device__ int nband() const { return 8 * nj + 8 * (ni - 8); }
+  __host__ __device__ int ninner() const { return (ni - 8) * (nj - 8); }
+  __host__ __device__ long nwork() const { return (long)nband() * nk + ninner(); }
+};
+
+// every interior point (small grids)
+__global__ void k_fold(Dims d, int mode, float* __restrict__ a) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i = blockIdx.z;
+  const FoldExt e(d, mode);
+  if (k >= e.nk || j >= e.nj || i >= e.ni) return;
+  fold_point(d, mode, a, i, j, k, e.nk);
+}
+
+// One launch: work items [0, nband*nk) are the points of the edge-band columns (level
+// fastest), the rest are the inner columns, whose only images are the vertical ghosts
+// below the bottom level and above the top level.
+__global__ void k_fold_edges(Dims d, int mode, float* __restrict__ a) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const FoldExt e(d, mode);
+  const long nb = (long)e.nband() * e.nk;
+  if (t < nb) {
+    const int b = (int)(t / e.nk), k = (int)(t - (long)b * e.nk);
+    int i, j;
+    if (b < 8 * e.nj) {
+      const int r = b / e.nj;
+      i = r < 4 ? r : e.ni - 8 + r;
+      j = b - r * e.nj;
+    } else {
+      const int q = b - 8 * e.nj, r = q >> 3, c = q & 7;
+      i = 4 + r;
+      j = c < 4 ? c : e.nj - 8 + c;
+    }
+    fold_point(d, mode, a, i, j, k, e.nk);
+    return;
+  }
+  const long q = t - nb;
+  if (q >= e.ninner()) return;
+  const int i = 4 + (int)(q / (e.nj - 8)), j = 4 + (int)(q % (e.nj - 8));
+  float* col = a + fold_idx(d, mode, i + 3, j + 3, 0);
+  const float g0 = col[0], s1 = col[1], sn = col[e.nk], gn = col[e.nk + 1];
+  col[0] = 0.f;
+  col[e.nk + 1] = 0.f;
+  if (e.nk == 1) {
+    col[1] = s1 + (g0 + gn);
+  } else {
+    col[1] = s1 + g0;
+    col[e.nk] = sn + gn;
+  }
+}
+
+inline void launch_fold(const Dims& d, int mode, float* a, hipStream_t st) {
+  const FoldExt e(d, mode);
+  if (e.ni < 9 || e.nj < 9) {
+    hipLaunchKernelGGL(k_fold, dim3((e.nk + 63) / 64, (e.nj + 3) / 4, e.ni), dim3(64, 4, 1), 0, st, d, mode, a);
+    return;
+  }
+  hipLaunchKernelGGL(k_fold_edges, dim3((unsigned)((e.nwork() + 255) / 256)), dim3(256), 0, st, d, mode, a);
 }
 
 // ------------------------------------------------------------- E1 scalar update

@@ -80,7 +80,7 @@ inline VjpLayout vjp_layout(const Dims& d, double dt, int n_sound, int np) {
   L.off_lss = o; o += (2 + np) * S;
   L.off_lT = o; o += S;
   L.off_lsm = o; o += 10 * C;
-  L.off_lcg2 = o; o += 256;
+  L.off_lcg2 = o; o += 256 + al256((size_t)kSmagKBlocks * sizeof(double));  // + reduction partials
   L.total = o;
   return L;
 }
@@ -108,10 +108,7 @@ struct VjpCtx {
   void zero(float* p, size_t n) const {
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (long)n, p, 0.0f);
   }
-  void fold(int mode, float* a) const {
-    int ni = d.nx + (mode == 1), nj = d.ny + (mode == 2), nk = d.nz + (mode == 3);
-    hipLaunchKernelGGL(k_fold, G(nk, nj, ni), bk, 0, st, d, mode, a);
-  }
+  void fold(int mode, float* a) const { launch_fold(d, mode, a, st); }
 };
 
 // padded field sizes of a state block in state_ptrs order (theta, u, v, w, pip, qv, ...)
@@ -264,6 +261,10 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
     sm[q] = (float*)(ws + L.off_sm + q * al256(L.nc * 4));
     lsm[q] = (float*)(A(L.off_lsm) + q * al256(L.nc * 4));
   }
+  // face weights g dts of the acoustic reverse, interior face layouts
+  float* gU = (float*)(A(L.off_lU));
+  float* gV = (float*)(A(L.off_lV));
+  float* gW = (float*)(A(L.off_lW));
   float* lru = (float*)(A(L.off_lru));
   float* lrv = (float*)(A(L.off_lrv));
   float* lrw = (float*)(A(L.off_lrw));
@@ -280,15 +281,10 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
   const double dtd = (double)o.dt;
   const double dts_small = dtd / (double)o.n_sound;
 
-  for (int q = 0; q < nf; ++q) {
-    cx.zero(lam_in[q], field_n(L, q));
-    cx.zero(lXA[q], field_n(L, q));
-    cx.zero(lXB[q], field_n(L, q));
-  }
-  cx.zero(lsu, L.iu_);
-  cx.zero(lsv, L.iv_);
-  cx.zero(lsw, L.iw_);
-  for (int q = 0; q < nsc; ++q) cx.zero(lss[q], L.ic_);
+  for (int q = 0; q < nf; ++q) cx.zero(lam_in[q], field_n(L, q));
+  // the whole adjoint scratch region (stage adjoints, rhs, SGS and Smagorinsky adjoints)
+  // is contiguous: one fill
+  cx.zero((float*)A(L.off_lXA), (L.off_lcg2 - L.off_lXA) / sizeof(float));
   hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, st, lcg2);
 
   for (int s = 2; s >= 0; --s) {
@@ -301,10 +297,7 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
     const float* rv = (const float*)(ws + L.off_rv[s]);
     const float* rw = (const float*)(ws + L.off_rw[s]);
     for (int q = 0; q < nf; ++q) cx.fold(field_mode(q), ldst[q]);
-    cx.zero(lru, L.iu_);
-    cx.zero(lrv, L.iv_);
-    cx.zero(lrw, L.iw_);
-    cx.zero(ltr, L.nc);
+    cx.zero(lru, (L.off_lsu - L.off_lru) / sizeof(float));  // lru, lrv, lrw, ltr are contiguous
     // scalars: next = pad(s0 + dt_s adv(s; u, v, w) + dt_s sgs)
     for (int q = 0; q < nsc; ++q) {
       int fi = (q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2));
@@ -330,13 +323,13 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
         for (int c = 0; c < 4; ++c) li[c] = lP[t & 1][c];
         cx.zero(li[3], L.nc);
       }
-      hipLaunchKernelGGL(k_acoustic_vjp_cells2, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, rc, b.pi0,
-                         (const float*)in[0], (const float*)in[1], (const float*)in[2], (const float*)in[3],
-                         (const float*)src[4], ru, rv, rw, tr, (const float*)lo[3], dts, li[3], lsrc[4]);
-      hipLaunchKernelGGL(k_acoustic_vjp_faces2, cx.G(d.nz + 1, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, b.pi0,
-                         (const float*)in[3], (const float*)src[4], tr, (const float*)lo[3], (const float*)lo[0],
-                         (const float*)lo[1], (const float*)lo[2], dts, acc, li[0], li[1], li[2], li[3], ltr, lru, lrv,
-                         lrw);
+      hipLaunchKernelGGL(k_acoustic_vjp_faces3, cx.G(d.nz + 1, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, b.pi0,
+                         (const float*)src[4], (const float*)lo[3], (const float*)lo[0], (const float*)lo[1],
+                         (const float*)lo[2], dts, acc, li[0], li[1], li[2], lru, lrv, lrw, gU, gV, gW);
+      hipLaunchKernelGGL(k_acoustic_vjp_cells3, cx.G(d.nz, d.ny + 2, d.nx + 2), cx.bk, 0, st, d, rc,
+                         (const float*)in[0], (const float*)in[1], (const float*)in[2], (const float*)in[3], ru, rv, rw,
+                         tr, (const float*)lo[3], (const float*)gU, (const float*)gV, (const float*)gW, dts, li[3], ltr,
+                         lsrc[4]);
       // the velocity adjoints are written at interior points only (their ghosts are never
       // read); the pi' adjoint also has ghost contributions from the PGF stencil
       if (t > 0) cx.fold(0, li[3]);
@@ -365,30 +358,33 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
   }
 
   if (turb) {
-    for (int q = 0; q < 10; ++q) cx.zero(lsm[q], L.nc);
+    // lsm was cleared with the scratch region and is not written before this point
     for (int q = 0; q < nsc; ++q) {
       int fi = (q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2));
-      hipLaunchKernelGGL(k_smag_scalar_vjp, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, rc, (const float*)sm[9],
+      hipLaunchKernelGGL(k_smag_scalar_vjp, cx.G(d.nz + 2, d.ny + 6, d.nx + 6), cx.bk, 0, st, d, rc, (const float*)sm[9],
                          (const float*)X0[fi], (const float*)lss[q], lsm[9], lam_in[fi]);
     }
-    hipLaunchKernelGGL(k_smag_uvw_vjp, cx.G(d.nz + 1, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, (const float*)sm[8],
+    hipLaunchKernelGGL(k_smag_uvw_vjp, cx.G(d.nz + 2, d.ny + 6, d.nx + 6), cx.bk, 0, st, d, rc, (const float*)sm[8],
                        (const float*)sm[0], (const float*)sm[1], (const float*)sm[2], (const float*)sm[3],
                        (const float*)sm[4], (const float*)sm[5], (const float*)lsu, (const float*)lsv,
                        (const float*)lsw, lsm[8], lsm[0], lsm[1], lsm[2], lsm[3], lsm[4], lsm[5]);
     float gs = cbrtf(d.dx * d.dy * d.dz);
     float cg = o.cs * gs;
     long n = (long)L.nc;
-    hipLaunchKernelGGL(k_smag_k_vjp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, (const float*)sm[6],
-                       (const float*)sm[7], (const float*)lsm[8], (const float*)lsm[9], cg * cg, o.prandtl, lsm[6],
-                       lsm[7], lcg2);
+    const unsigned nbk = smag_k_blocks(n);
+    double* kpart = (double*)(A(L.off_lcg2 + 256));
+    hipLaunchKernelGGL(k_smag_k_vjp, dim3(nbk), dim3(256), 0, st, n, (const float*)sm[6], (const float*)sm[7],
+                       (const float*)lsm[8], (const float*)lsm[9], cg * cg, o.prandtl, lsm[6], lsm[7], kpart);
+    hipLaunchKernelGGL(k_smag_k_sum, dim3(1), dim3(256), 0, st, (const double*)kpart, (int)nbk, lcg2);
     for (int q = 0; q < 8; ++q) cx.fold(0, lsm[q]);
     hipLaunchKernelGGL(k_smag_nm_vjp, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, (const float*)X0[0],
                        (const float*)X0[5], (const float*)lsm[7], lam_in[0], lam_in[5]);
-    hipLaunchKernelGGL(k_smag_deform_vjp, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, rc, (const float*)sm[0],
+    hipLaunchKernelGGL(k_smag_deform_h, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, rc, (const float*)sm[0],
                        (const float*)sm[1], (const float*)sm[2], (const float*)sm[3], (const float*)sm[4],
-                       (const float*)sm[5], (const float*)lsm[0], (const float*)lsm[1], (const float*)lsm[2],
-                       (const float*)lsm[3], (const float*)lsm[4], (const float*)lsm[5], (const float*)lsm[6],
-                       lam_in[1], lam_in[2], lam_in[3]);
+                       (const float*)sm[5], (const float*)lsm[6], lsm[0], lsm[1], lsm[2], lsm[3], lsm[4], lsm[5]);
+    hipLaunchKernelGGL(k_smag_deform_vjp, cx.G(d.nz + 3, d.ny + 7, d.nx + 7), cx.bk, 0, st, d, (const float*)lsm[0],
+                       (const float*)lsm[1], (const float*)lsm[2], (const float*)lsm[3], (const float*)lsm[4],
+                       (const float*)lsm[5], lam_in[1], lam_in[2], lam_in[3]);
     // d(cg^2)/dCs = 2 Cs gs^2
     if (lam_cs) {
       double factor = 2.0 * (double)o.cs * (double)gs * (double)gs;

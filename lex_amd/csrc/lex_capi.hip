@@ -471,8 +471,7 @@ int lex_fold(const lex_grid_t* g, int mode, float* a, void* stream) {
   REQ(a);
   if (mode < 0 || mode > 3) return fail(-1, "lex_fold: mode must be 0..3%s");
   Dims d = mkdims(g);
-  int ni = d.nx + (mode == 1), nj = d.ny + (mode == 2), nk = d.nz + (mode == 3);
-  hipLaunchKernelGGL(k_fold, grd(nk, nj, ni), blk(), 0, S(stream), d, mode, a);
+  launch_fold(d, mode, a, S(stream));
   return check_launch("lex_fold");
 }
 
@@ -545,10 +544,10 @@ int lex_acoustic_substep_vjp(const lex_grid_t* g, const float* r, const float* p
     return fail(-1, "lex_acoustic_substep_vjp: input adjoints must not alias output adjoints%s");
   Dims d = mkdims(g);
   hipStream_t st = S(stream);
-  hipLaunchKernelGGL(k_fold, grd(d.nz, d.ny, d.nx + 1), blk(), 0, st, d, 1, luo);
-  hipLaunchKernelGGL(k_fold, grd(d.nz, d.ny + 1, d.nx), blk(), 0, st, d, 2, lvo);
-  hipLaunchKernelGGL(k_fold, grd(d.nz + 1, d.ny, d.nx), blk(), 0, st, d, 3, lwo);
-  hipLaunchKernelGGL(k_fold, grd(d.nz, d.ny, d.nx), blk(), 0, st, d, 0, lpo);
+  launch_fold(d, 1, luo, st);
+  launch_fold(d, 2, lvo, st);
+  launch_fold(d, 3, lwo, st);
+  launch_fold(d, 0, lpo, st);
   const size_t nU = (size_t)(d.nx + 1) * d.ny * d.nz, nV = (size_t)d.nx * (d.ny + 1) * d.nz,
                nW = (size_t)d.nx * d.ny * (d.nz + 1);
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((nU + 255) / 256)), dim3(256), 0, st, (long)nU, lU, 0.0f);
@@ -565,7 +564,7 @@ int lex_acoustic_substep_vjp(const lex_grid_t* g, const float* r, const float* p
 size_t lex_smag_vjp_workspace_bytes(const lex_grid_t* g) {
   if (!grid_ok(g)) return 0;
   Dims d = mkdims(g);
-  return 20 * al256((size_t)d.nc() * 4) + 256;
+  return 20 * al256((size_t)d.nc() * 4) + 256 + al256((size_t)kSmagKBlocks * sizeof(double));
 }
 
 int lex_smag_vjp(const lex_grid_t* g, const float* r, const float* u, const float* v, const float* w, const float* th,
@@ -601,22 +600,26 @@ int lex_smag_vjp(const lex_grid_t* g, const float* r, const float* u, const floa
   hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, st, lcg2);
   for (int q = 0; q < n_scalars; ++q) {
     if (!scalars[q] || !lss[q] || !lscalars[q]) return fail(-2, "lex_smag_vjp: null scalar pointer%s");
-    hipLaunchKernelGGL(k_smag_scalar_vjp, grd(d.nz, d.ny, d.nx), blk(), 0, st, d, r, (const float*)sm[9], scalars[q],
+    hipLaunchKernelGGL(k_smag_scalar_vjp, grd(d.nz + 2, d.ny + 6, d.nx + 6), blk(), 0, st, d, r, (const float*)sm[9], scalars[q],
                        lss[q], lsm[9], lscalars[q]);
   }
-  hipLaunchKernelGGL(k_smag_uvw_vjp, grd(d.nz + 1, d.ny + 1, d.nx + 1), blk(), 0, st, d, r, (const float*)sm[8],
+  hipLaunchKernelGGL(k_smag_uvw_vjp, grd(d.nz + 2, d.ny + 6, d.nx + 6), blk(), 0, st, d, r, (const float*)sm[8],
                      (const float*)sm[0], (const float*)sm[1], (const float*)sm[2], (const float*)sm[3],
                      (const float*)sm[4], (const float*)sm[5], lsu, lsv, lsw, lsm[8], lsm[0], lsm[1], lsm[2], lsm[3],
                      lsm[4], lsm[5]);
-  hipLaunchKernelGGL(k_smag_k_vjp, e1, e2, 0, st, n, (const float*)sm[6], (const float*)sm[7], (const float*)lsm[8],
-                     (const float*)lsm[9], cg * cg, prandtl, lsm[6], lsm[7], lcg2);
-  for (int q = 0; q < 8; ++q) hipLaunchKernelGGL(k_fold, grd(d.nz, d.ny, d.nx), blk(), 0, st, d, 0, lsm[q]);
+  const unsigned nbk = smag_k_blocks(n);
+  double* kpart = (double*)((char*)ws + 20 * C + 256);
+  hipLaunchKernelGGL(k_smag_k_vjp, dim3(nbk), dim3(256), 0, st, n, (const float*)sm[6], (const float*)sm[7],
+                     (const float*)lsm[8], (const float*)lsm[9], cg * cg, prandtl, lsm[6], lsm[7], kpart);
+  hipLaunchKernelGGL(k_smag_k_sum, dim3(1), dim3(256), 0, st, (const double*)kpart, (int)nbk, lcg2);
+  for (int q = 0; q < 8; ++q) launch_fold(d, 0, lsm[q], st);
   hipLaunchKernelGGL(k_smag_nm_vjp, grd(d.nz, d.ny, d.nx), blk(), 0, st, d, th, qv, (const float*)lsm[7], lth, lqv);
-  hipLaunchKernelGGL(k_smag_deform_vjp, grd(d.nz, d.ny, d.nx), blk(), 0, st, d, r, (const float*)sm[0],
+  hipLaunchKernelGGL(k_smag_deform_h, grd(d.nz, d.ny, d.nx), blk(), 0, st, d, r, (const float*)sm[0],
                      (const float*)sm[1], (const float*)sm[2], (const float*)sm[3], (const float*)sm[4],
-                     (const float*)sm[5], (const float*)lsm[0], (const float*)lsm[1], (const float*)lsm[2],
-                     (const float*)lsm[3], (const float*)lsm[4], (const float*)lsm[5], (const float*)lsm[6], lu, lv,
-                     lw);
+                     (const float*)sm[5], (const float*)lsm[6], lsm[0], lsm[1], lsm[2], lsm[3], lsm[4], lsm[5]);
+  hipLaunchKernelGGL(k_smag_deform_vjp, grd(d.nz + 3, d.ny + 7, d.nx + 7), blk(), 0, st, d, (const float*)lsm[0],
+                     (const float*)lsm[1], (const float*)lsm[2], (const float*)lsm[3], (const float*)lsm[4],
+                     (const float*)lsm[5], lu, lv, lw);
   if (lcs)
     hipLaunchKernelGGL(k_cs_grad, dim3(1), dim3(64), 0, st, (const double*)lcg2,
                        2.0 * (double)cs * (double)gs * (double)gs, lcs);
