@@ -184,7 +184,9 @@ class WsRk3Sprint(torch.autograd.Function):
     prandtl, nsteps, checkpoint_every).  state0 supplies u, v, w, pi', qv and passive
     scalars (its theta is replaced by theta0 + theta'_0).  Checkpoints of the full state
     are kept every `checkpoint_every` steps; the backward pass recomputes each segment
-    with the forward step and walks it back with step_vjp.
+    with the recording step (step_tape) and walks it back with step_vjp_taped.  The last
+    segment is recorded during the forward pass itself (the backward pass needs one
+    segment's records at a time anyway), so it is never recomputed.
     """
 
     @staticmethod
@@ -196,13 +198,22 @@ class WsRk3Sprint(torch.autograd.Function):
         k = int(cfg["checkpoint_every"])
         n = int(cfg["nsteps"])
         ckpts = {0: tuple(t.clone() for t in x)}
+        s_last = ((n - 1) // k) * k if n > 0 else 0
+        seg_states, seg_tapes = [], []
         with torch.no_grad():
             for i in range(n):
-                fused_step(x, base_state, grid, (2, False, False, 0, False, False, 1), dt=cfg["dt"],
-                           n_sound=cfg["n_sound"], cs=cs_f, prandtl=cfg["prandtl"])
-                if (i + 1) % k == 0 and i + 1 < n:
-                    ckpts[i + 1] = tuple(t.clone() for t in x)
+                if i < s_last:
+                    fused_step(x, base_state, grid, (2, False, False, 0, False, False, 1), dt=cfg["dt"],
+                               n_sound=cfg["n_sound"], cs=cs_f, prandtl=cfg["prandtl"])
+                    if (i + 1) % k == 0:
+                        ckpts[i + 1] = tuple(t.clone() for t in x)
+                else:
+                    tp, y = step_tape(x, base_state, grid, cfg["dt"], cfg["n_sound"], cs_f, cfg["prandtl"])
+                    seg_states.append(x)
+                    seg_tapes.append(tp)
+                    x = y
         ctx.ckpts = ckpts
+        ctx.seg = (s_last, seg_states, seg_tapes)
         ctx.base_state, ctx.grid, ctx.cfg, ctx.cs = base_state, grid, cfg, cs_f
         ctx.cs_is_tensor = torch.is_tensor(cs)
         return _interior(x[0]) - th0_int
@@ -216,17 +227,22 @@ class WsRk3Sprint(torch.autograd.Function):
         _interior(lam[0]).copy_(grad_out.to(torch.float32))
         dcs = torch.zeros(1, dtype=torch.float64, device=x0[0].device)
         starts = sorted(ctx.ckpts.keys())
-        tapes = []
+        s_last, seg_states, tapes = ctx.seg
         for s in reversed(starts):
             end = min(s + k, n)
-            # record every step of the segment once, then reverse it from the records
-            states = [ctx.ckpts[s]]
-            for t in range(end - s):
-                tp = tapes[t] if t < len(tapes) else None
-                tp, y = step_tape(states[-1], base, grid, cfg["dt"], cfg["n_sound"], ctx.cs, cfg["prandtl"], tape=tp)
-                if t >= len(tapes):
-                    tapes.append(tp)
-                states.append(y)
+            if s == s_last:
+                states = seg_states  # recorded by the forward pass
+            else:
+                # record every step of the segment once (reusing the record buffers), then
+                # reverse it from the records
+                states = [ctx.ckpts[s]]
+                for t in range(end - s):
+                    tp = tapes[t] if t < len(tapes) else None
+                    tp, y = step_tape(states[-1], base, grid, cfg["dt"], cfg["n_sound"], ctx.cs, cfg["prandtl"],
+                                      tape=tp)
+                    if t >= len(tapes):
+                        tapes.append(tp)
+                    states.append(y)
             for t in reversed(range(end - s)):
                 lam, d = step_vjp_taped(states[t], tapes[t], lam, base, grid, cfg["dt"], cfg["n_sound"], ctx.cs,
                                         cfg["prandtl"])
@@ -234,6 +250,7 @@ class WsRk3Sprint(torch.autograd.Function):
         g_theta = _interior(lam[0]).clone()
         g_cs = dcs.to(torch.float32).reshape(()) if ctx.cs_is_tensor else None
         ctx.ckpts = None
+        ctx.seg = None
         return g_theta, g_cs, None, None, None, None
 
 
