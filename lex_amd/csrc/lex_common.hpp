@@ -200,6 +200,28 @@ __device__ __forceinline__ void store_w(const Dims& d, float* __restrict__ a, in
       for (int c1 = 0; c1 < nzs; ++c1) a[iw(d, xs[a1], ys[b1], zs[c1])] = val;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2: hardware
+// block h runs on XCD h % 8.  xcd_tile renumbers blocks so that each XCD receives one
+// contiguous range of the logical grid (x fastest, then y, then z), keeping the halo
+// lines shared by neighbouring tiles in one L2.  Bijective: the first 8*(n/8) blocks are
+// permuted, the remainder keep their numbers.
+struct TileId {
+  int x, y, z;
+};
+__device__ __forceinline__ TileId xcd_tile() {
+  const unsigned gx = gridDim.x, gy = gridDim.y, n = gx * gy * gridDim.z;
+  const unsigned h = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned per = n >> 3, full = per << 3;
+  const unsigned lid = h < full ? (h & 7u) * per + (h >> 3) : h;
+  const unsigned gxy = gx * gy;
+  TileId t;
+  t.z = (int)(lid / gxy);
+  const unsigned r = lid - (unsigned)t.z * gxy;
+  t.y = (int)(r / gx);
+  t.x = (int)(r - (unsigned)t.y * gx);
+  return t;
+}
+
 // Fast paths: away from the periodic x/y images a cell has its own position plus, at the
 // first and last level, one z ghost.  The x/y test is uniform across a wave whose lanes
 // run along z, so only boundary columns take the general ghost lists above.

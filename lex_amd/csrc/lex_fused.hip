@@ -76,8 +76,9 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
   float* FY = P + (kFTY + 6) * Lz;         // (kFTY+1) x nzl lower y-faces (+ tile top)
   float* FZ = FY + (kFTY + 1) * nzl;       // kFTY x nzl lower z-faces
 
-  const int j0 = blockIdx.y * kFTY;
-  const int i0 = blockIdx.x * kFTX;
+  const TileId tb = xcd_tile();
+  const int j0 = tb.y * kFTY;
+  const int i0 = tb.x * kFTX;
   const int k = tz, j = j0 + ty;
   const bool act = (k < d.nz) && (j < d.ny);
   const int K = k + 1, J = j + 3;
@@ -198,7 +199,8 @@ __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict_
   float* P = lds;
   float* FY = P + (kFTY + 6) * Lz;
   float* FZ = FY + (kFTY + 1) * nzl;
-  const int j0 = blockIdx.y * kFTY, i0 = blockIdx.x * kFTX;
+  const TileId tb = xcd_tile();
+  const int j0 = tb.y * kFTY, i0 = tb.x * kFTX;
   const int k = tz, j = j0 + ty;
   const bool act = (k < d.nz) && (j < d.ny);
   const int K = k + 1;
@@ -280,7 +282,8 @@ __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict_
   float* P = lds;
   float* FY = P + (kFTY + 6) * Lz;
   float* FZ = FY + (kFTY + 1) * nzl;
-  const int j0 = blockIdx.y * kFTY, i0 = blockIdx.x * kFTX;
+  const TileId tb = xcd_tile();
+  const int j0 = tb.y * kFTY, i0 = tb.x * kFTX;
   const int k = tz, j = j0 + ty;
   const bool act = (k < d.nz) && (j < d.ny);
   const int K = k + 1;
@@ -366,7 +369,8 @@ __global__ void __launch_bounds__(1024) k_mom_w(Dims d, const float* __restrict_
   float* FZ = FY + (kFTY + 1) * nzl;                // kFTY x nzl: flux at scalar level l+1
   float* FD = FZ + kFTY * nzl;                      // kFTY x nzl: flow divergence at cell l
   float* BB = FD + kFTY * nzl;                      // kFTY x nzl: buoyancy at cell l
-  const int j0 = blockIdx.y * kFTY, i0 = blockIdx.x * kFTX;
+  const TileId tb = xcd_tile();
+  const int j0 = tb.y * kFTY, i0 = tb.x * kFTX;
   const int l = tz, j = j0 + ty;
   const bool row_ok = j < d.ny;
   const bool lane_ok = l < d.nz;                   // owns level l and cell l
@@ -497,8 +501,9 @@ __global__ void __launch_bounds__(512) k_stage_scalars_db(Dims d, const float* _
   float* FYb = Pb + 2 * NS * PL;                // [NS][(kFTY+1) * nzl]
   float* FZb = FYb + NS * (kFTY + 1) * nzl;     // [NS][kFTY * nzl]
 
-  const int j0 = blockIdx.y * kFTY;
-  const int i0 = blockIdx.x * kFTX;
+  const TileId tb = xcd_tile();
+  const int j0 = tb.y * kFTY;
+  const int i0 = tb.x * kFTX;
   const int k = tz, j = j0 + ty;
   const bool act = (k < d.nz) && (j < d.ny);
   const int K = k + 1;

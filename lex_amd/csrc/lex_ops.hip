@@ -459,41 +459,59 @@ __global__ void k_acoustic(Dims d, const float* __restrict__ rc, const float* __
                            const float* __restrict__ rv, const float* __restrict__ rw, const float* __restrict__ tr,
                            float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
                            float* __restrict__ po, float dts) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int i = blockIdx.z;
+  const TileId tb = xcd_tile();
+  int k = tb.x * blockDim.x + threadIdx.x;
+  int j = tb.y * blockDim.y + threadIdx.y;
+  int i = tb.z;
   if (k >= d.nz || j >= d.ny) return;
   int I = i + 3, J = j + 3, K = k + 1;
-  // new velocities on the six faces of the cell
-  float un0 = u[iu(d, I, J, K)] + (pgf_x(d, p, tr, I, J, K) + ru[((long)i * d.ny + j) * d.nz + k]) * dts;
-  float un1 = u[iu(d, I + 1, J, K)] + (pgf_x(d, p, tr, I + 1, J, K) + ru[((long)(i + 1) * d.ny + j) * d.nz + k]) * dts;
-  float vn0 = v[iv(d, I, J, K)] + (pgf_y(d, p, tr, I, J, K) + rv[((long)i * (d.ny + 1) + j) * d.nz + k]) * dts;
-  float vn1 = v[iv(d, I, J + 1, K)] + (pgf_y(d, p, tr, I, J + 1, K) + rv[((long)i * (d.ny + 1) + j + 1) * d.nz + k]) * dts;
-  // set_w_bc: surface (l=0) and lid (l=nz) are zero after the update
-  float wn0 = (k == 0) ? 0.0f
-                       : w[iw(d, I, J, K)] + (pgf_z(d, p, tr, I, J, K) + rw[((long)i * d.ny + j) * (d.nz + 1) + k]) * dts;
-  float wn1 = (k == d.nz - 1)
-                  ? 0.0f
-                  : w[iw(d, I, J, K + 1)] + (pgf_z(d, p, tr, I, J, K + 1) + rw[((long)i * d.ny + j) * (d.nz + 1) + k + 1]) * dts;
+  const long c = ic(d, I, J, K);
+  const long sx = d.sxc(), sy = d.syc();
+  // Every operand is loaded unconditionally (all indices are inside the padded arrays,
+  // the w levels K and K+1 exist for K in 1..nz) so the loads issue back to back; the
+  // surface and lid cases are selected afterwards.
+  const float rcm = rc[K - 1], rcc = rc[K], rcp = rc[K + 1];
+  const float pim = pi0c[K - 1], pic = pi0c[K], pip = pi0c[K + 1];
+  const long cu = iu(d, I, J, K), cv = iv(d, I, J, K), cw = iw(d, I, J, K);
+  const long ou = ((long)i * d.ny + j) * d.nz + k, ov = ((long)i * (d.ny + 1) + j) * d.nz + k;
+  const long ow = ((long)i * d.ny + j) * (d.nz + 1) + k;
+  const float u0 = u[cu], u1 = u[cu + d.sxu()], v0 = v[cv], v1 = v[cv + d.syc()], w0 = w[cw], w1 = w[cw + 1];
+  const float ru0 = ru[ou], ru1 = ru[ou + (long)d.ny * d.nz], rv0 = rv[ov], rv1 = rv[ov + d.nz];
+  const float rw0 = rw[ow], rw1 = rw[ow + 1];
+  const float pc = p[c], pxm = p[c - sx], pxp = p[c + sx], pym = p[c - sy], pyp = p[c + sy], pzm = p[c - 1],
+              pzp = p[c + 1];
+  const float tc = tr[c], txm = tr[c - sx], txp = tr[c + sx], tym = tr[c - sy], typ = tr[c + sy],
+              tzm = tr[c - 1], tzp = tr[c + 1];
+  const float sc = ps[c], sxm = ps[c - sx], sxp = ps[c + sx], sym = ps[c - sy], syp = ps[c + sy],
+              szm = ps[c - 1], szp = ps[c + 1];
+  const float h = (float)(-kCp * 0.5);
+  // new velocities on the six faces of the cell (pgf_x/y/z with the same operation order)
+  float un0 = u0 + (h * (tc + txm) * ((pc - pxm) * d.idx) + ru0) * dts;
+  float un1 = u1 + (h * (txp + tc) * ((pxp - pc) * d.idx) + ru1) * dts;
+  float vn0 = v0 + (h * (tc + tym) * ((pc - pym) * d.idy) + rv0) * dts;
+  float vn1 = v1 + (h * (typ + tc) * ((pyp - pc) * d.idy) + rv1) * dts;
+  // set_w_bc: surface (l=0) and lid (l=nz) are zero after the update; pgf_z vanishes at
+  // levels 1 and nz+1
+  const float pz0 = (K <= 1) ? 0.0f : h * (tc + tzm) * ((pc - pzm) * d.idz);
+  const float pz1 = (K + 1 >= d.nz + 1) ? 0.0f : h * (tzp + tc) * ((pzp - pc) * d.idz);
+  float wn0 = (k == 0) ? 0.0f : w0 + (pz0 + rw0) * dts;
+  float wn1 = (k == d.nz - 1) ? 0.0f : w1 + (pz1 + rw1) * dts;
   float div = (un1 - un0) * d.idx + (vn1 - vn0) * d.idy + (wn1 - wn0) * d.idz;
   // advection_pi with the new velocities and the fixed pi*
-  long c = ic(d, I, J, K);
-  long sx = d.sxc(), sy = d.syc();
-  float pc = ps[c];
-  float udx0 = un0 * ((pc - ps[c - sx]) * d.idx);
-  float udx1 = un1 * ((ps[c + sx] - pc) * d.idx);
-  float vdy0 = vn0 * ((pc - ps[c - sy]) * d.idy);
-  float vdy1 = vn1 * ((ps[c + sy] - pc) * d.idy);
-  float dz0 = (pi0c[K] - pi0c[K - 1]) * d.idz + (pc - ps[c - 1]) * d.idz;
-  float dz1 = (pi0c[K + 1] - pi0c[K]) * d.idz + (ps[c + 1] - pc) * d.idz;
+  float udx0 = un0 * ((sc - sxm) * d.idx);
+  float udx1 = un1 * ((sxp - sc) * d.idx);
+  float vdy0 = vn0 * ((sc - sym) * d.idy);
+  float vdy1 = vn1 * ((syp - sc) * d.idy);
+  float dz0 = (pic - pim) * d.idz + (sc - szm) * d.idz;
+  float dz1 = (pip - pic) * d.idz + (szp - sc) * d.idz;
   float wdz0 = (k == 0) ? 0.0f : wn0 * dz0;
   float wdz1 = (k == d.nz - 1) ? 0.0f : wn1 * dz1;
-  float r8w0 = 0.5f * (rc[K - 1] + rc[K]);
-  float r8w1 = 0.5f * (rc[K] + rc[K + 1]);
-  float advp = -(0.5f * (udx1 + udx0) + 0.5f * (vdy1 + vdy0) + (0.5f * wdz0 * r8w0 + 0.5f * wdz1 * r8w1) / rc[K]);
-  float piadv = pi0c[K] + pc;
+  float r8w0 = 0.5f * (rcm + rcc);
+  float r8w1 = 0.5f * (rcc + rcp);
+  float advp = -(0.5f * (udx1 + udx0) + 0.5f * (vdy1 + vdy0) + (0.5f * wdz0 * r8w0 + 0.5f * wdz1 * r8w1) / rcc);
+  float piadv = pic + sc;
   float dpdt = (float)(-kRd / kCv) * (piadv * div) + advp;
-  float pn = p[c] + 1.1f * dpdt * dts;
+  float pn = pc + 1.1f * dpdt * dts;
   store_c_fast(d, po, i, j, k, pn);
   store_u_fast(d, uo, i, j, k, un0);
   if (i == d.nx - 1) store_u(d, uo, d.nx, j, k, un1);

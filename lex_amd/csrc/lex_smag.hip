@@ -36,9 +36,10 @@ __global__ void k_smag_deform(Dims d, const float* __restrict__ rc, const float*
                               const float* __restrict__ v, const float* __restrict__ w, float* __restrict__ s11,
                               float* __restrict__ s22, float* __restrict__ s33, float* __restrict__ s12,
                               float* __restrict__ s13, float* __restrict__ s23, float* __restrict__ deform) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int i = blockIdx.z;
+  const TileId tb = xcd_tile();
+  int k = tb.x * blockDim.x + threadIdx.x;
+  int j = tb.y * blockDim.y + threadIdx.y;
+  int i = tb.z;
   if (k >= d.nz || j >= d.ny) return;
   int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
   float r = rc[K];
@@ -89,9 +90,10 @@ __device__ __forceinline__ float n28w_at(const Dims& d, const float* __restrict_
 }
 
 __global__ void k_smag_nm(Dims d, const float* __restrict__ th, const float* __restrict__ qv, float* __restrict__ n2) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int i = blockIdx.z;
+  const TileId tb = xcd_tile();
+  int k = tb.x * blockDim.x + threadIdx.x;
+  int j = tb.y * blockDim.y + threadIdx.y;
+  int i = tb.z;
   if (k >= d.nz || j >= d.ny) return;
   long c = ic(d, i + 3, j + 3, 0);
   int nz = d.nz;
@@ -124,9 +126,10 @@ __global__ void k_smag_uvw(Dims d, const float* __restrict__ rc, const float* __
                            const float* __restrict__ s11, const float* __restrict__ s22, const float* __restrict__ s33,
                            const float* __restrict__ s12, const float* __restrict__ s13, const float* __restrict__ s23,
                            float* __restrict__ su, float* __restrict__ sv, float* __restrict__ sw) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nz
-  int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
-  int i = blockIdx.z;                             // 0..nx
+  const TileId tb = xcd_tile();
+  int k = tb.x * blockDim.x + threadIdx.x;  // 0..nz
+  int j = tb.y * blockDim.y + threadIdx.y;  // 0..ny
+  int i = tb.z;                             // 0..nx
   if (k > d.nz || j > d.ny) return;
   const long sx = d.sxc(), sy = d.syc();
   int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
@@ -205,9 +208,10 @@ __device__ float smag_scalar_point(const Dims& d, const float* __restrict__ rc, 
 
 __global__ void k_smag_scalar(Dims d, const float* __restrict__ rc, const float* __restrict__ kh,
                               const float* __restrict__ s, float* __restrict__ out) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int i = blockIdx.z;
+  const TileId tb = xcd_tile();
+  int k = tb.x * blockDim.x + threadIdx.x;
+  int j = tb.y * blockDim.y + threadIdx.y;
+  int i = tb.z;
   if (k >= d.nz || j >= d.ny) return;
   out[((long)i * d.ny + j) * d.nz + k] = smag_scalar_point(d, rc, kh, s, i, j, k);
 }
@@ -221,9 +225,10 @@ struct SgsScalars {
 };
 
 __global__ void k_smag_scalars(Dims d, const float* __restrict__ rc, const float* __restrict__ kh, SgsScalars S) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int i = blockIdx.z;
+  const TileId tb = xcd_tile();
+  int k = tb.x * blockDim.x + threadIdx.x;
+  int j = tb.y * blockDim.y + threadIdx.y;
+  int i = tb.z;
   if (k >= d.nz || j >= d.ny) return;
   const long sx = d.sxc(), sy = d.syc();
   const int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
