@@ -207,9 +207,10 @@ __device__ __forceinline__ void fd_vjp(const Dims& d, const float* __restrict__ 
 // outside the interior), so every WENO gradient is evaluated once per face, and the
 // div(rho0 u) terms of both cells are collected at the face: each face-velocity adjoint
 // (lu, lv, lw) has exactly one writer and is updated without atomics.
+// lam/rho0 at interior cell (i, j, k) when ok, else 0 (branch-free load)
 __device__ __forceinline__ float mu_cell(const Dims& d, const float* __restrict__ lam, const float* __restrict__ rc,
-                                         int i, int j, int k) {
-  return lam[((long)i * d.ny + j) * d.nz + k] / rc[k + 1];
+                                         int i, int j, int k, bool ok) {
+  return ld_or0(lam, ((long)i * d.ny + j) * d.nz + k, ok) / rc[ok ? k + 1 : 1];
 }
 
 __global__ void k_adv_scalar_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ s,
@@ -223,13 +224,16 @@ __global__ void k_adv_scalar_vjp(Dims d, const float* __restrict__ rc, const flo
   const int I = i + 3, J = j + 3, K = k + 1;
   const bool cell = i < d.nx && j < d.ny;
   const float r = rc[K];
-  const float mc = cell ? mu_cell(d, lam, rc, i, j, k) : 0.f;
+  const float mc = mu_cell(d, lam, rc, i, j, k, cell);
+  const float mlx = mu_cell(d, lam, rc, i - 1, j, k, i > 0 && j < d.ny);
+  const float mly = mu_cell(d, lam, rc, i, j - 1, k, j > 0 && i < d.nx);
+  const float mb = mu_cell(d, lam, rc, i, j, k - 1, cell && k > 0);  // cell below
   const long sx = d.sxc(), sy = d.syc();
   const long c = ic(d, I, J, K);
   const float sc = s[c];
   float q6[6], g6[6];
   if (j < d.ny) {  // x face I: lower face of cell I, upper face of cell I-1
-    const float ml = i > 0 ? mu_cell(d, lam, rc, i - 1, j, k) : 0.f;
+    const float ml = mlx;
     const float gF = (mc - ml) * d.idx;
     const long a = iu(d, I, J, K);
     const float mass = r * u[a];
@@ -246,7 +250,7 @@ __global__ void k_adv_scalar_vjp(Dims d, const float* __restrict__ rc, const flo
     if (g != 0.f) lu[a] += r * g;
   }
   if (i < d.nx) {  // y face J
-    const float ml = j > 0 ? mu_cell(d, lam, rc, i, j - 1, k) : 0.f;
+    const float ml = mly;
     const float gF = (mc - ml) * d.idy;
     const long a = iv(d, I, J, K);
     const float mass = r * v[a];
@@ -264,7 +268,6 @@ __global__ void k_adv_scalar_vjp(Dims d, const float* __restrict__ rc, const flo
   const float* col = s + ic(d, I, J, 0);
   float* lcol = ls + ic(d, I, J, 0);
   const long wb = iw(d, I, J, 0);
-  const float mb = k > 0 ? mu_cell(d, lam, rc, i, j, k - 1) : 0.f;  // cell below
   const float r8 = 0.5f * (rc[K - 1] + rc[K]);
   float g;
   if (k > 0) {  // flux face between levels k and k+1 (the surface face k = 0 is prescribed)
@@ -292,17 +295,19 @@ __global__ void k_adv_scalar_vjp(Dims d, const float* __restrict__ rc, const flo
 // separates, so its adjoint is the difference of their lam/rho0 and its WENO gradient is
 // evaluated once.  The div(rho0 u) product term is reversed per scalar cell, collecting
 // both momentum points that average it.
+// lam/rho0 at a momentum point when ok, else 0 (branch-free loads)
 __device__ __forceinline__ float mu_uface(const Dims& d, const float* __restrict__ lam, const float* __restrict__ rc,
-                                          int f, int j, int k) {
-  return lam[((long)f * d.ny + j) * d.nz + k] / rc[k + 1];
+                                          int f, int j, int k, bool ok) {
+  return ld_or0(lam, ((long)f * d.ny + j) * d.nz + k, ok) / rc[ok ? k + 1 : 1];
 }
 __device__ __forceinline__ float mu_vface(const Dims& d, const float* __restrict__ lam, const float* __restrict__ rc,
-                                          int i, int f, int k) {
-  return lam[((long)i * (d.ny + 1) + f) * d.nz + k] / rc[k + 1];
+                                          int i, int f, int k, bool ok) {
+  return ld_or0(lam, ((long)i * (d.ny + 1) + f) * d.nz + k, ok) / rc[ok ? k + 1 : 1];
 }
 __device__ __forceinline__ float mu_wlev(const Dims& d, const float* __restrict__ lam, const float* __restrict__ rc,
-                                         int i, int j, int l) {
-  return lam[((long)i * d.ny + j) * (d.nz + 1) + l] / (0.5f * (rc[l] + rc[l + 1]));
+                                         int i, int j, int l, bool ok) {
+  const int lc = ok ? l : 1;
+  return ld_or0(lam, ((long)i * d.ny + j) * (d.nz + 1) + l, ok) / (0.5f * (rc[lc] + rc[lc + 1]));
 }
 
 // reverse of advection_u (advection.py:37-74), lam: (nx+1, ny, nz).  Thread (a, b, k),
@@ -320,12 +325,15 @@ __global__ void k_adv_u_vjp(Dims d, const float* __restrict__ rc, const float* _
   const int J = b + 3, K = k + 1;
   const float r = rc[K];
   const bool upt = a <= d.nx && b < d.ny;
-  const float mc = upt ? mu_uface(d, lam, rc, a, b, k) : 0.f;
+  const float mc = mu_uface(d, lam, rc, a, b, k, upt);
+  const float mlx = mu_uface(d, lam, rc, a - 1, b, k, a > 0 && b < d.ny);
+  const float mly = mu_uface(d, lam, rc, a, b - 1, k, b > 0 && a <= d.nx);
+  const float mlz = mu_uface(d, lam, rc, a, b, k - 1, upt && k > 0);
   const long sx = d.sxu(), sy = d.syc();
   float q6[6], g6[6];
   if (b < d.ny) {
     // x flux at scalar cell a+3: lower flux of u face a, upper flux of u face a-1
-    const float ml = a > 0 ? mu_uface(d, lam, rc, a - 1, b, k) : 0.f;
+    const float ml = mlx;
     const float gF = (mc - ml) * d.idx;
     const long base = iu(d, a, J, K);
     if (gF != 0.f) {
@@ -344,7 +352,7 @@ __global__ void k_adv_u_vjp(Dims d, const float* __restrict__ rc, const float* _
   if (a > d.nx) return;
   const int I = a + 3;
   {  // y flux at corner (I, J): lower flux of u (a, b), upper flux of u (a, b-1)
-    const float ml = b > 0 ? mu_uface(d, lam, rc, a, b - 1, k) : 0.f;
+    const float ml = mly;
     const float gF = (mc - ml) * d.idy;
     if (gF != 0.f) {
       const long va = iv(d, I - 1, J, K), vb = iv(d, I, J, K);
@@ -363,7 +371,7 @@ __global__ void k_adv_u_vjp(Dims d, const float* __restrict__ rc, const float* _
   const float* col = u + ub;
   float* lcol = lu + ub;
   if (k > 0) {  // z flux at w level K between u levels K-1 and K
-    const float gF = (mc - mu_uface(d, lam, rc, a, b, k - 1)) * d.idz;
+    const float gF = (mc - mlz) * d.idz;
     if (gF != 0.f) {
       const float rf = 0.5f * (rc[k] + rc[k + 1]);
       const long wa = iw(d, I - 1, J, K), wb = iw(d, I, J, K);
@@ -395,12 +403,15 @@ __global__ void k_adv_v_vjp(Dims d, const float* __restrict__ rc, const float* _
   const int I = a + 3, K = k + 1;
   const float r = rc[K];
   const bool vpt = a < d.nx && b <= d.ny;
-  const float mc = vpt ? mu_vface(d, lam, rc, a, b, k) : 0.f;
+  const float mc = mu_vface(d, lam, rc, a, b, k, vpt);
+  const float mlx = mu_vface(d, lam, rc, a - 1, b, k, a > 0 && b <= d.ny);
+  const float mly = mu_vface(d, lam, rc, a, b - 1, k, b > 0 && a < d.nx);
+  const float mlz = mu_vface(d, lam, rc, a, b, k - 1, vpt && k > 0);
   const long sx = d.sxv(), sy = d.syc();
   float q6[6], g6[6];
   if (b <= d.ny) {  // x flux at corner (I, J): lower flux of v (a, b), upper flux of v (a-1, b)
     const int J = b + 3;
-    const float ml = a > 0 ? mu_vface(d, lam, rc, a - 1, b, k) : 0.f;
+    const float ml = mlx;
     const float gF = (mc - ml) * d.idx;
     if (gF != 0.f) {
       const long ua = iu(d, I, J - 1, K), ub = iu(d, I, J, K);
@@ -417,7 +428,7 @@ __global__ void k_adv_v_vjp(Dims d, const float* __restrict__ rc, const float* _
   if (a >= d.nx) return;
   {
     // y flux at scalar cell b+3: lower flux of v face b, upper flux of v face b-1
-    const float ml = b > 0 ? mu_vface(d, lam, rc, a, b - 1, k) : 0.f;
+    const float ml = mly;
     const float gF = (mc - ml) * d.idy;
     const long base = iv(d, I, b, K);
     if (gF != 0.f) {
@@ -439,7 +450,7 @@ __global__ void k_adv_v_vjp(Dims d, const float* __restrict__ rc, const float* _
   const float* col = v + vb;
   float* lcol = lv + vb;
   if (k > 0) {
-    const float gF = (mc - mu_vface(d, lam, rc, a, b, k - 1)) * d.idz;
+    const float gF = (mc - mlz) * d.idz;
     if (gF != 0.f) {
       const float rf = 0.5f * (rc[k] + rc[k + 1]);
       const long wa = iw(d, I, J - 1, K), wb = iw(d, I, J, K);
@@ -471,11 +482,16 @@ __global__ void k_adv_w_vjp(Dims d, const float* __restrict__ rc, const float* _
   const int I = a + 3, J = b + 3, K = t + 1;
   const bool lev = t >= 1 && t <= d.nz - 1;  // advected w level
   const bool col_in = a < d.nx && b < d.ny;
-  const float mc = (lev && col_in) ? mu_wlev(d, lam, rc, a, b, t) : 0.f;
+  const float mc = mu_wlev(d, lam, rc, a, b, t, lev && col_in);
+  const float mlx = mu_wlev(d, lam, rc, a - 1, b, t, lev && a > 0 && b < d.ny);
+  const float mly = mu_wlev(d, lam, rc, a, b - 1, t, lev && b > 0 && a < d.nx);
+  // w levels t-1 and t+1 of this column (advected for 1..nz-1)
+  const float mlo = mu_wlev(d, lam, rc, a, b, t - 1, col_in && t >= 2 && t - 1 <= d.nz - 1);
+  const float mup = mu_wlev(d, lam, rc, a, b, t + 1, col_in && t + 1 <= d.nz - 1);
   const long sx = d.sxw(), sy = d.syw();
   float q6[6], g6[6];
   if (lev && b < d.ny) {  // x flux: lower flux of w column a, upper flux of column a-1
-    const float ml = a > 0 ? mu_wlev(d, lam, rc, a - 1, b, t) : 0.f;
+    const float ml = mlx;
     const float gF = (mc - ml) * d.idx;
     if (gF != 0.f) {
       const long u0 = iu(d, I, J, K - 1), u1 = iu(d, I, J, K);
@@ -489,7 +505,7 @@ __global__ void k_adv_w_vjp(Dims d, const float* __restrict__ rc, const float* _
     }
   }
   if (lev && a < d.nx) {  // y flux: lower flux of w row b, upper flux of row b-1
-    const float ml = b > 0 ? mu_wlev(d, lam, rc, a, b - 1, t) : 0.f;
+    const float ml = mly;
     const float gF = (mc - ml) * d.idy;
     if (gF != 0.f) {
       const long v0 = iv(d, I, J, K - 1), v1 = iv(d, I, J, K);
@@ -506,8 +522,6 @@ __global__ void k_adv_w_vjp(Dims d, const float* __restrict__ rc, const float* _
   const long cb = iw(d, I, J, 0);
   const float* col = w + cb;
   float* lcol = lw + cb;
-  // mu of w level t-1 (advected for t-1 >= 1)
-  const float mlo = (t >= 2 && t - 1 <= d.nz - 1) ? mu_wlev(d, lam, rc, a, b, t - 1) : 0.f;
   if (t >= 1) {  // z flux at scalar level t: lower flux of w level t, upper flux of level t-1
     const float gF = (mc - mlo) * d.idz;
     if (gF != 0.f) {
@@ -525,7 +539,6 @@ __global__ void k_adv_w_vjp(Dims d, const float* __restrict__ rc, const float* _
   if (lev) aadd(lcol + K, mc * 0.5f * (fd_at(d, rc, u, v, w, I, J, K - 1) + fd_at(d, rc, u, v, w, I, J, K)));
   if (t < d.nz) {
     // div(rho0 u) of cell level K = t+1, averaged by w levels t (its lower) and t+1 (its upper)
-    const float mup = (t + 1 <= d.nz - 1) ? mu_wlev(d, lam, rc, a, b, t + 1) : 0.f;
     const float gsum = mc * col[K] + mup * col[K + 1];
     fd_vjp(d, rc, I, J, K, 0.5f * gsum, lu, lv, lw);
   }
@@ -847,46 +860,63 @@ __global__ void k_acoustic_vjp_faces3(Dims d, const float* __restrict__ rc, cons
   if (k > d.nz || j > d.ny) return;
   const float cr = (float)(-kRd / kCv);
   const float g11 = 1.1f * dts;
-  int I = i + 3, J = j + 3, K = k + 1;
-  if (k < d.nz && j < d.ny) {  // u face i
-    long o = ((long)i * d.ny + j) * d.nz + k;
-    long a = iu(d, I, J, K);
-    long cR = ic(d, I, J, K), cL = cR - d.sxc();
-    float dps = (ps[cR] - ps[cL]) * d.idx;
-    float g = luo[a];
-    if (i < d.nx) g += g11 * lpo[cR] * (-cr * (pi0c[K] + ps[cR]) * d.idx - 0.5f * dps);
-    if (i > 0) g += g11 * lpo[cL] * (cr * (pi0c[K] + ps[cL]) * d.idx - 0.5f * dps);
-    lu[a] = acc ? lu[a] + g : g;
+  const bool du = k < d.nz && j < d.ny;                      // u face (i, j, k)
+  const bool dv = k < d.nz && i < d.nx;                      // v face (i, j, k)
+  const bool dw = i < d.nx && j < d.ny && k > 0 && k < d.nz;  // w level k of column (i, j)
+  // clamped indices: every load below is in bounds and issues before the first use
+  const int kc = min(k, d.nz - 1), jc = min(j, d.ny - 1), icl = min(i, d.nx - 1);
+  const int kw = min(max(k, 1), d.nz - 1);
+  const long sxc = d.sxc(), syc = d.syc();
+  // u face (i+3, jc+3, kc+1)
+  const int Ku = kc + 1;
+  const long ou = ((long)i * d.ny + jc) * d.nz + kc, au = iu(d, i + 3, jc + 3, Ku);
+  const long cRu = ic(d, i + 3, jc + 3, Ku), cLu = cRu - sxc;
+  const float psRu = ps[cRu], psLu = ps[cLu];
+  const float lpRu = ld_or0(lpo, cRu, i < d.nx), lpLu = ld_or0(lpo, cLu, i > 0);
+  const float gou = luo[au], lu0 = acc ? lu[au] : 0.f, lru0 = lru[ou], piu = pi0c[Ku];
+  // v face (icl+3, j+3, kc+1)
+  const long ov = ((long)icl * (d.ny + 1) + j) * d.nz + kc, av = iv(d, icl + 3, j + 3, Ku);
+  const long cRv = ic(d, icl + 3, j + 3, Ku), cLv = cRv - syc;
+  const float psRv = ps[cRv], psLv = ps[cLv];
+  const float lpRv = ld_or0(lpo, cRv, j < d.ny), lpLv = ld_or0(lpo, cLv, j > 0);
+  const float gov = lvo[av], lv0 = acc ? lv[av] : 0.f, lrv0 = lrv[ov];
+  // w level kw of column (icl, jc): cells above (Kw) and below (Kw-1)
+  const int Kw = kw + 1;
+  const long ow = ((long)icl * d.ny + jc) * (d.nz + 1) + kw, aw = iw(d, icl + 3, jc + 3, Kw);
+  const long ca = ic(d, icl + 3, jc + 3, Kw), cb = ca - 1;
+  const float psa = ps[ca], psb = ps[cb], lpa = lpo[ca], lpb = lpo[cb];
+  const float gow = lwo[aw], lw0 = acc ? lw[aw] : 0.f, lrw0 = lrw[ow];
+  const float rwa = rc[Kw], rwb = rc[Kw - 1], pia = pi0c[Kw], pib = pi0c[Kw - 1];
+  if (du) {
+    const float dps = (psRu - psLu) * d.idx;
+    float g = gou;
+    if (i < d.nx) g += g11 * lpRu * (-cr * (piu + psRu) * d.idx - 0.5f * dps);
+    if (i > 0) g += g11 * lpLu * (cr * (piu + psLu) * d.idx - 0.5f * dps);
+    lu[au] = acc ? lu0 + g : g;
     const float gd = g * dts;
-    lru[o] += gd;
-    gu[o] = gd;
+    lru[ou] = lru0 + gd;
+    gu[ou] = gd;
   }
-  if (k < d.nz && i < d.nx) {  // v face j
-    long o = ((long)i * (d.ny + 1) + j) * d.nz + k;
-    long a = iv(d, I, J, K);
-    long cR = ic(d, I, J, K), cL = cR - d.syc();
-    float dps = (ps[cR] - ps[cL]) * d.idy;
-    float g = lvo[a];
-    if (j < d.ny) g += g11 * lpo[cR] * (-cr * (pi0c[K] + ps[cR]) * d.idy - 0.5f * dps);
-    if (j > 0) g += g11 * lpo[cL] * (cr * (pi0c[K] + ps[cL]) * d.idy - 0.5f * dps);
-    lv[a] = acc ? lv[a] + g : g;
+  if (dv) {
+    const float dps = (psRv - psLv) * d.idy;
+    float g = gov;
+    if (j < d.ny) g += g11 * lpRv * (-cr * (piu + psRv) * d.idy - 0.5f * dps);
+    if (j > 0) g += g11 * lpLv * (cr * (piu + psLv) * d.idy - 0.5f * dps);
+    lv[av] = acc ? lv0 + g : g;
     const float gd = g * dts;
-    lrv[o] += gd;
-    gv[o] = gd;
+    lrv[ov] = lrv0 + gd;
+    gv[ov] = gd;
   }
-  if (i < d.nx && j < d.ny && k > 0 && k < d.nz) {  // w level k: cell above (K) and below (K-1)
-    long o = ((long)i * d.ny + j) * (d.nz + 1) + k;
-    long a = iw(d, I, J, K);
-    long ca = ic(d, I, J, K), cb = ca - 1;
-    float r8 = 0.5f * (rc[K - 1] + rc[K]);
-    float dzz = (pi0c[K] - pi0c[K - 1]) * d.idz + (ps[ca] - ps[cb]) * d.idz;
-    float g = lwo[a];
-    g += g11 * lpo[ca] * (-cr * (pi0c[K] + ps[ca]) * d.idz + (-0.5f * r8 / rc[K]) * dzz);
-    g += g11 * lpo[cb] * (cr * (pi0c[K - 1] + ps[cb]) * d.idz + (-0.5f * r8 / rc[K - 1]) * dzz);
-    lw[a] = acc ? lw[a] + g : g;
+  if (dw) {
+    const float r8 = 0.5f * (rwb + rwa);
+    const float dzz = (pia - pib) * d.idz + (psa - psb) * d.idz;
+    float g = gow;
+    g += g11 * lpa * (-cr * (pia + psa) * d.idz + (-0.5f * r8 / rwa) * dzz);
+    g += g11 * lpb * (cr * (pib + psb) * d.idz + (-0.5f * r8 / rwb) * dzz);
+    lw[aw] = acc ? lw0 + g : g;
     const float gd = g * dts;
-    lrw[o] += gd;
-    gw[o] = gd;
+    lrw[ow] = lrw0 + gd;
+    gw[ow] = gd;
   }
 }
 
@@ -912,88 +942,93 @@ __global__ void k_acoustic_vjp_cells3(Dims d, const float* __restrict__ rc, cons
   const long c = ic(d, X, Y, Z);
   const int i = X - 3, j = Y - 3, k = Z - 1;
   const bool in_x = i >= 0 && i < d.nx, in_y = j >= 0 && j < d.ny, inner = in_x && in_y;
+  // which of the six faces exist (u faces 0..nx, v faces 0..ny, w levels 1..nz-1)
+  const bool fu0 = in_y && i >= 0 && i <= d.nx, fu1 = in_y && i + 1 >= 0 && i + 1 <= d.nx;
+  const bool fv0 = in_x && j >= 0 && j <= d.ny, fv1 = in_x && j + 1 >= 0 && j + 1 <= d.ny;
+  const bool fw0 = inner && k >= 1, fw1 = inner && k + 1 <= d.nz - 1;
+  const long ou0 = fu0 ? ((long)i * d.ny + j) * d.nz + k : 0, ou1 = fu1 ? ((long)(i + 1) * d.ny + j) * d.nz + k : 0;
+  const long ov0 = fv0 ? ((long)i * (d.ny + 1) + j) * d.nz + k : 0;
+  const long ov1 = fv1 ? ((long)i * (d.ny + 1) + j + 1) * d.nz + k : 0;
+  const long ow0 = inner ? ((long)i * d.ny + j) * (d.nz + 1) + k : 0;
+  // every load issues up front (all indices in bounds)
+  const float pc = p[c], pxm = p[c - sx], pxp = p[c + sx], pym = p[c - sy], pyp = p[c + sy], pzm = p[c - 1],
+              pzp = p[c + 1];
+  const float tc = tr[c], txm = tr[c - sx], txp = tr[c + sx], tym = tr[c - sy], typ = tr[c + sy],
+              tzm = tr[c - 1], tzp = tr[c + 1];
+  const long cu = iu(d, X, Y, Z), cv = iv(d, X, Y, Z), cw = iw(d, X, Y, Z);
+  const float u0 = u[cu], u1 = u[cu + d.sxu()], v0 = v[cv], v1 = v[cv + d.syc()], w0 = w[cw], w1 = w[cw + 1];
+  const float ru0 = ld_or0(ru, ou0, fu0), ru1 = ld_or0(ru, ou1, fu1), rv0 = ld_or0(rv, ov0, fv0),
+              rv1 = ld_or0(rv, ov1, fv1), rw0 = ld_or0(rw, ow0, fw0), rw1 = ld_or0(rw, ow0 + 1, fw1);
+  const float gu0 = ld_or0(gu, ou0, fu0), gu1 = ld_or0(gu, ou1, fu1), gv0 = ld_or0(gv, ov0, fv0),
+              gv1 = ld_or0(gv, ov1, fv1), gw0 = ld_or0(gw, ow0, fw0), gw1 = ld_or0(gw, ow0 + 1, fw1);
+  const float Lc = ld_or0(lpo, c, inner);
+  const float Lxp = ld_or0(lpo, c + sx, i + 1 >= 0 && i + 1 < d.nx && in_y);
+  const float Lxm = ld_or0(lpo, c - sx, i - 1 >= 0 && i - 1 < d.nx && in_y);
+  const float Lyp = ld_or0(lpo, c + sy, j + 1 >= 0 && j + 1 < d.ny && in_x);
+  const float Lym = ld_or0(lpo, c - sy, j - 1 >= 0 && j - 1 < d.ny && in_x);
+  const float Lzp = ld_or0(lpo, c + 1, inner && k + 1 <= d.nz - 1);
+  const float Lzm = ld_or0(lpo, c - 1, inner && k >= 1);
+  const float rcm = rc[Z - 1], rcc = rc[Z], rcp = rc[Z + 1];
   const float h = (float)(-kCp * 0.5);
-  const float pc = p[c], tc = tr[c];
   float U0 = 0.f, U1 = 0.f, V0 = 0.f, V1 = 0.f, W0 = 0.f, W1 = 0.f;
   float alp = 0.f, atr = 0.f;  // lp and ltr sums
-  // u faces X (this cell on its + side) and X+1 (- side)
-  if (in_y && i >= 0 && i <= d.nx) {
-    const long o = ((long)i * d.ny + j) * d.nz + k;
-    const float dp = (pc - p[c - sx]) * d.idx, ts = tc + tr[c - sx];
-    U0 = u[iu(d, X, Y, Z)] + (h * ts * dp + ru[o]) * dts;
-    const float g = gu[o];
-    atr += g * h * dp;
-    alp += g * h * ts * d.idx;
+  // faces on this cell's - side (X, Y, level Z) and + side (X+1, Y+1, level Z+1)
+  if (fu0) {
+    const float dp = (pc - pxm) * d.idx, ts = tc + txm;
+    U0 = u0 + (h * ts * dp + ru0) * dts;
+    atr += gu0 * h * dp;
+    alp += gu0 * h * ts * d.idx;
   }
-  if (in_y && i + 1 >= 0 && i + 1 <= d.nx) {
-    const long o = ((long)(i + 1) * d.ny + j) * d.nz + k;
-    const float dp = (p[c + sx] - pc) * d.idx, ts = tr[c + sx] + tc;
-    U1 = u[iu(d, X + 1, Y, Z)] + (h * ts * dp + ru[o]) * dts;
-    const float g = gu[o];
-    atr += g * h * dp;
-    alp -= g * h * ts * d.idx;
+  if (fu1) {
+    const float dp = (pxp - pc) * d.idx, ts = txp + tc;
+    U1 = u1 + (h * ts * dp + ru1) * dts;
+    atr += gu1 * h * dp;
+    alp -= gu1 * h * ts * d.idx;
   }
-  if (in_x && j >= 0 && j <= d.ny) {
-    const long o = ((long)i * (d.ny + 1) + j) * d.nz + k;
-    const float dp = (pc - p[c - sy]) * d.idy, ts = tc + tr[c - sy];
-    V0 = v[iv(d, X, Y, Z)] + (h * ts * dp + rv[o]) * dts;
-    const float g = gv[o];
-    atr += g * h * dp;
-    alp += g * h * ts * d.idy;
+  if (fv0) {
+    const float dp = (pc - pym) * d.idy, ts = tc + tym;
+    V0 = v0 + (h * ts * dp + rv0) * dts;
+    atr += gv0 * h * dp;
+    alp += gv0 * h * ts * d.idy;
   }
-  if (in_x && j + 1 >= 0 && j + 1 <= d.ny) {
-    const long o = ((long)i * (d.ny + 1) + j + 1) * d.nz + k;
-    const float dp = (p[c + sy] - pc) * d.idy, ts = tr[c + sy] + tc;
-    V1 = v[iv(d, X, Y + 1, Z)] + (h * ts * dp + rv[o]) * dts;
-    const float g = gv[o];
-    atr += g * h * dp;
-    alp -= g * h * ts * d.idy;
+  if (fv1) {
+    const float dp = (pyp - pc) * d.idy, ts = typ + tc;
+    V1 = v1 + (h * ts * dp + rv1) * dts;
+    atr += gv1 * h * dp;
+    alp -= gv1 * h * ts * d.idy;
   }
   // w levels Z (below the cell) and Z+1 (above); zero at the surface and the lid
-  const long ow = ((long)i * d.ny + j) * (d.nz + 1) + k;
-  if (inner && k >= 1) {
-    const float dp = (pc - p[c - 1]) * d.idz, ts = tc + tr[c - 1];
-    W0 = w[iw(d, X, Y, Z)] + (h * ts * dp + rw[ow]) * dts;
-    const float g = gw[ow];
-    atr += g * h * dp;
-    alp += g * h * ts * d.idz;
+  if (fw0) {
+    const float dp = (pc - pzm) * d.idz, ts = tc + tzm;
+    W0 = w0 + (h * ts * dp + rw0) * dts;
+    atr += gw0 * h * dp;
+    alp += gw0 * h * ts * d.idz;
   }
-  if (inner && k + 1 <= d.nz - 1) {
-    const float dp = (p[c + 1] - pc) * d.idz, ts = tr[c + 1] + tc;
-    W1 = w[iw(d, X, Y, Z + 1)] + (h * ts * dp + rw[ow + 1]) * dts;
-    const float g = gw[ow + 1];
-    atr += g * h * dp;
-    alp -= g * h * ts * d.idz;
+  if (fw1) {
+    const float dp = (pzp - pc) * d.idz, ts = tzp + tc;
+    W1 = w1 + (h * ts * dp + rw1) * dts;
+    atr += gw1 * h * dp;
+    alp -= gw1 * h * ts * d.idz;
   }
   // advection_pi reverse (one_step_integration.py:543-546): G = 1.1 dts lam(new pi')
   const float g11 = 1.1f * dts;
   const float cr = (float)(-kRd / kCv);
   float aps = 0.f;
-  auto lpo_in = [&](int x, int y, int z) -> float {
-    return (x >= 3 && x < d.nx + 3 && y >= 3 && y < d.ny + 3 && z >= 1 && z <= d.nz) ? lpo[ic(d, x, y, z)] : 0.f;
-  };
   if (inner) {
-    const float Lc = lpo[c];
     alp += Lc;
     const float G = g11 * Lc;
     const float div = (U1 - U0) * d.idx + (V1 - V0) * d.idy + (W1 - W0) * d.idz;
     const float gx = -0.5f * G * d.idx, gy = -0.5f * G * d.idy;
     aps += cr * div * G + (gx * U0 - gx * U1) + (gy * V0 - gy * V1);
-    if (k > 0) aps += (-0.5f * (0.5f * (rc[Z - 1] + rc[Z])) / rc[Z] * G) * W0 * d.idz;
-    if (k < d.nz - 1) aps -= (-0.5f * (0.5f * (rc[Z] + rc[Z + 1])) / rc[Z] * G) * W1 * d.idz;
+    if (k > 0) aps += (-0.5f * (0.5f * (rcm + rcc)) / rcc * G) * W0 * d.idz;
+    if (k < d.nz - 1) aps -= (-0.5f * (0.5f * (rcc + rcp)) / rcc * G) * W1 * d.idz;
     // the cells above and below read the shared w level
-    if (k + 1 <= d.nz - 1) {
-      const float Ga = g11 * lpo[c + 1];
-      aps -= (-0.5f * (0.5f * (rc[Z] + rc[Z + 1])) / rc[Z + 1] * Ga) * W1 * d.idz;
-    }
-    if (k >= 1) {
-      const float Gb = g11 * lpo[c - 1];
-      aps += (-0.5f * (0.5f * (rc[Z - 1] + rc[Z])) / rc[Z - 1] * Gb) * W0 * d.idz;
-    }
+    if (k + 1 <= d.nz - 1) aps -= (-0.5f * (0.5f * (rcc + rcp)) / rcp * (g11 * Lzp)) * W1 * d.idz;
+    if (k >= 1) aps += (-0.5f * (0.5f * (rcm + rcc)) / rcm * (g11 * Lzm)) * W0 * d.idz;
   }
   // x and y neighbours read the shared face velocities
-  aps += 0.5f * g11 * d.idx * (lpo_in(X + 1, Y, Z) * U1 - lpo_in(X - 1, Y, Z) * U0);
-  aps += 0.5f * g11 * d.idy * (lpo_in(X, Y + 1, Z) * V1 - lpo_in(X, Y - 1, Z) * V0);
+  aps += 0.5f * g11 * d.idx * (Lxp * U1 - Lxm * U0);
+  aps += 0.5f * g11 * d.idy * (Lyp * V1 - Lym * V0);
   if (lp == lps) {
     const float t = alp + aps;
     if (t != 0.f) lp[c] += t;
@@ -1014,8 +1049,9 @@ __global__ void k_acoustic_vjp_cells3(Dims d, const float* __restrict__ rc, cons
 __device__ __forceinline__ float sgs_ldiv(const Dims& d, const float* __restrict__ lam, const float* __restrict__ rc,
                                           int X, int Y, int Z) {
   const int i = X - 3, j = Y - 3, k = Z - 1;
-  if (i < 0 || i >= d.nx || j < 0 || j >= d.ny || k < 0 || k >= d.nz) return 0.f;
-  return -lam[((long)i * d.ny + j) * d.nz + k] / rc[Z];
+  const bool ok = i >= 0 && i < d.nx && j >= 0 && j < d.ny && k >= 0 && k < d.nz;
+  const float L = ld_or0(lam, ((long)i * d.ny + j) * d.nz + k, ok);
+  return ok ? -L / rc[ok ? Z : 1] : 0.f;
 }
 
 // SGS scalar diffusion (turbulence.py:80-109): lam (nx, ny, nz) -> kh, s adjoints.  The
@@ -1059,22 +1095,26 @@ __global__ void k_smag_scalar_vjp(Dims d, const float* __restrict__ rc, const fl
 __device__ __forceinline__ float sgs_gu(const Dims& d, const float* __restrict__ lsu, const float* __restrict__ rc,
                                         int X, int Y, int Z) {
   const int f = X - 3, j = Y - 3, k = Z - 1;
-  if (f < 0 || f > d.nx || j < 0 || j >= d.ny || k < 0 || k >= d.nz) return 0.f;
-  const float r = rc[Z];
-  return -lsu[((long)f * d.ny + j) * d.nz + k] / (0.5f * (r + r));
+  const bool ok = f >= 0 && f <= d.nx && j >= 0 && j < d.ny && k >= 0 && k < d.nz;
+  const float L = ld_or0(lsu, ((long)f * d.ny + j) * d.nz + k, ok);
+  const float r = rc[ok ? Z : 1];
+  return ok ? -L / (0.5f * (r + r)) : 0.f;
 }
 __device__ __forceinline__ float sgs_gv(const Dims& d, const float* __restrict__ lsv, const float* __restrict__ rc,
                                         int X, int Y, int Z) {
   const int i = X - 3, f = Y - 3, k = Z - 1;
-  if (i < 0 || i >= d.nx || f < 0 || f > d.ny || k < 0 || k >= d.nz) return 0.f;
-  const float r = rc[Z];
-  return -lsv[((long)i * (d.ny + 1) + f) * d.nz + k] / (0.5f * (r + r));
+  const bool ok = i >= 0 && i < d.nx && f >= 0 && f <= d.ny && k >= 0 && k < d.nz;
+  const float L = ld_or0(lsv, ((long)i * (d.ny + 1) + f) * d.nz + k, ok);
+  const float r = rc[ok ? Z : 1];
+  return ok ? -L / (0.5f * (r + r)) : 0.f;
 }
 __device__ __forceinline__ float sgs_gw(const Dims& d, const float* __restrict__ lsw, const float* __restrict__ rc,
                                         int X, int Y, int l) {
   const int i = X - 3, j = Y - 3;
-  if (i < 0 || i >= d.nx || j < 0 || j >= d.ny || l < 1 || l > d.nz - 1) return 0.f;
-  return -lsw[((long)i * d.ny + j) * (d.nz + 1) + l] / (0.5f * (rc[l] + rc[l + 1]));
+  const bool ok = i >= 0 && i < d.nx && j >= 0 && j < d.ny && l >= 1 && l <= d.nz - 1;
+  const float L = ld_or0(lsw, ((long)i * d.ny + j) * (d.nz + 1) + l, ok);
+  const int lc = ok ? l : 1;
+  return ok ? -L / (0.5f * (rc[lc] + rc[lc + 1])) : 0.f;
 }
 
 // SGS momentum tendencies (turbulence.py:9-79): lam on u, v, w points -> km, s_ij
@@ -1248,8 +1288,8 @@ __global__ void k_smag_deform_h(Dims d, const float* __restrict__ rc, const floa
 
 // H of an interior cell, 0 elsewhere (padded indices)
 __device__ __forceinline__ float hcell(const Dims& d, const float* __restrict__ h, int X, int Y, int Z) {
-  if (X < 3 || X >= d.nx + 3 || Y < 3 || Y >= d.ny + 3 || Z < 1 || Z > d.nz) return 0.f;
-  return h[ic(d, X, Y, Z)];
+  const bool ok = X >= 3 && X < d.nx + 3 && Y >= 3 && Y < d.ny + 3 && Z >= 1 && Z <= d.nz;
+  return ld_or0(h, ic(d, X, Y, Z), ok);
 }
 
 // Weight of the xz (yz) edge at horizontal face position F between columns F-1 and F (of

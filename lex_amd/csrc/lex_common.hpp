@@ -200,6 +200,13 @@ __device__ __forceinline__ void store_w(const Dims& d, float* __restrict__ a, in
       for (int c1 = 0; c1 < nzs; ++c1) a[iw(d, xs[a1], ys[b1], zs[c1])] = val;
 }
 
+// a[idx] when ok, else 0, without a branch: the load always issues (at index 0 when
+// !ok), so it is batched with the thread's other loads instead of waiting behind a branch.
+__device__ __forceinline__ float ld_or0(const float* __restrict__ a, long idx, bool ok) {
+  const float v = a[ok ? idx : 0];
+  return ok ? v : 0.f;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2: hardware
 // block h runs on XCD h % 8.  xcd_tile renumbers blocks so that each XCD receives one
 // contiguous range of the logical grid (x fastest, then y, then z), keeping the halo

@@ -126,6 +126,14 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
     const float mz_lo = 0.5f * (rc[Kc - 1] + rc[Kc]) * wlo;   // z-face between K-1 and K
     const bool px = upwind_pos(mx), py0 = upwind_pos(my_lo), py1 = upwind_pos(my_hi), pz = upwind_pos(mz_lo);
     float thc = 0.f, qvc = 0.f;
+    // the update's xi0 values and frozen SGS tendencies, issued before the first barrier
+    float s0v[NS], sgv[NS];
+    const long oint = ((long)i * d.ny + (Jc - 3)) * d.nz + (Kc - 1);
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      s0v[q] = S.s0[q][c];
+      sgv[q] = S.sgs[q] ? S.sgs[q][oint] : 0.0f;
+    }
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
       const float* s = S.s[q];
@@ -157,8 +165,8 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
         float conv = -((fxhi - fxlo[q]) * d.idx + (FY[(ty + 1) * nzl + k] - FY[ty * nzl + k]) * d.idy +
                        (fzhi - FZ[ty * nzl + k]) * d.idz);
         float tend = (conv + qc * fd) / r;
-        float nxt = S.s0[q][c] + tend * dt;
-        if (S.sgs[q]) nxt = nxt + S.sgs[q][((long)i * d.ny + j) * d.nz + k] * dt;
+        float nxt = s0v[q] + tend * dt;
+        if (S.sgs[q]) nxt = nxt + sgv[q] * dt;
         store_c_fast(d, S.out[q], i, j, k, nxt);
       }
       fxlo[q] = fxhi;
@@ -228,6 +236,11 @@ __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict_
     const float mx = 0.5f * (win[2] + win[3]) * r;   // x-flux at cell I
     const float fxhi = mx * hrecon(win[0], win[1], win[2], win[3], win[4], win[5], upwind_pos(mx), weps);
     const float fdhi = fd_cell(d, rc, u, v, w, I, Jc, Kc);
+    // operands of the y and z mass fluxes and the SGS add-in, issued before the barrier
+    const float wA = w[iw(d, I - 1, Jc, Kc)], wB = w[iw(d, I, Jc, Kc)];
+    const float vA = v[iv(d, I - 1, Jc, Kc)], vB = v[iv(d, I, Jc, Kc)];
+    const float vC = v[iv(d, I - 1, Jc + 1, Kc)], vD = v[iv(d, I, Jc + 1, Kc)];
+    const float sgv = sgs ? sgs[((long)i * d.ny + (Jc - 3)) * d.nz + (Kc - 1)] : 0.0f;
     plane_load(P, u + (long)I * sx, psl);
     __syncthreads();
     const float* Pc = P + (ty + 3) * Lz;
@@ -235,16 +248,16 @@ __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict_
       // z flux at w level K (face K-1): mass averaged over the two columns of the u point
       float fz = 0.0f;
       if (k > 0) {
-        float m = 0.5f * (0.5f * (rc[K - 1] + rc[K]) * w[iw(d, I - 1, Jc, K)] + 0.5f * (rc[K - 1] + rc[K]) * w[iw(d, I, Jc, K)]);
+        float m = 0.5f * (0.5f * (rc[K - 1] + rc[K]) * wA + 0.5f * (rc[K - 1] + rc[K]) * wB);
         fz = m * vrecon(Pc, Lz, K - 1, upwind_pos(m), weps);
       }
       FZ[ty * nzl + k] = fz;
-      float my = 0.5f * (r * v[iv(d, I - 1, Jc, K)] + r * v[iv(d, I, Jc, K)]);
+      float my = 0.5f * (r * vA + r * vB);
       FY[ty * nzl + k] =
           my * hrecon(Pc[K - 3 * Lz], Pc[K - 2 * Lz], Pc[K - Lz], Pc[K], Pc[K + Lz], Pc[K + 2 * Lz], upwind_pos(my), weps);
       if (last_row) {
         const float* Pn = Pc + Lz;
-        float m2 = 0.5f * (r * v[iv(d, I - 1, Jc + 1, K)] + r * v[iv(d, I, Jc + 1, K)]);
+        float m2 = 0.5f * (r * vC + r * vD);
         FY[(ty + 1) * nzl + k] =
             m2 * hrecon(Pn[K - 3 * Lz], Pn[K - 2 * Lz], Pn[K - Lz], Pn[K], Pn[K + Lz], Pn[K + 2 * Lz], upwind_pos(m2), weps);
       }
@@ -255,7 +268,7 @@ __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict_
       float conv = -((fxhi - fxlo) * d.idx + (FY[(ty + 1) * nzl + k] - FY[ty * nzl + k]) * d.idy +
                      (fzhi - FZ[ty * nzl + k]) * d.idz);
       float a = (conv + uc * (0.5f * (fdlo + fdhi))) / (0.5f * (r + r));
-      if (sgs) a = a + sgs[((long)i * d.ny + j) * d.nz + k];
+      if (sgs) a = a + sgv;
       ru[((long)i * d.ny + j) * d.nz + k] = a;
       if (i == 0) ru[((long)d.nx * d.ny + j) * d.nz + k] = a;
     }
@@ -311,13 +324,16 @@ __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict_
     const float fxhi = mx * hrecon(win[0], win[1], win[2], win[3], win[4], win[5], upwind_pos(mx), weps);
     const float fd0 = fd_cell(d, rc, u, v, w, I, Jc - 1, Kc);
     const float fd1 = fd_cell(d, rc, u, v, w, I, Jc, Kc);
+    // operands of the z mass flux and the SGS add-in, issued before the barrier
+    const float wA = w[iw(d, I, Jc - 1, Kc)], wB = w[iw(d, I, Jc, Kc)];
+    const float sgv = sgs ? sgs[((long)i * (d.ny + 1) + (Jc - 3)) * d.nz + (Kc - 1)] : 0.0f;
     plane_load(P, v + (long)I * sx, psl);
     __syncthreads();
     const float* Pc = P + (ty + 3) * Lz;
     if (act) {
       float fz = 0.0f;
       if (k > 0) {
-        float m = 0.5f * (0.5f * (rc[K - 1] + rc[K]) * w[iw(d, I, Jc - 1, K)] + 0.5f * (rc[K - 1] + rc[K]) * w[iw(d, I, Jc, K)]);
+        float m = 0.5f * (0.5f * (rc[K - 1] + rc[K]) * wA + 0.5f * (rc[K - 1] + rc[K]) * wB);
         fz = m * vrecon(Pc, Lz, K - 1, upwind_pos(m), weps);
       }
       FZ[ty * nzl + k] = fz;
@@ -337,7 +353,7 @@ __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict_
       float conv = -((fxhi - fxlo) * d.idx + (FY[(ty + 1) * nzl + k] - FY[ty * nzl + k]) * d.idy +
                      (fzhi - FZ[ty * nzl + k]) * d.idz);
       float a = (conv + vc * (0.5f * (fd0 + fd1))) / (0.5f * (r + r));
-      if (sgs) a = a + sgs[((long)i * (d.ny + 1) + j) * d.nz + k];
+      if (sgs) a = a + sgv;
       rv[((long)i * (d.ny + 1) + j) * d.nz + k] = a;
       if (j == 0) rv[((long)i * (d.ny + 1) + d.ny) * d.nz + k] = a;
     }
@@ -398,6 +414,11 @@ __global__ void __launch_bounds__(1024) k_mom_w(Dims d, const float* __restrict_
     const float wc = win[2];
     const float mx = 0.5f * (rc[Kc - 1] * u[iu(d, I + 1, Jc, Kc - 1)] + rc[Kc] * u[iu(d, I + 1, Jc, Kc)]);
     const float fxhi = mx * hrecon(win[0], win[1], win[2], win[3], win[4], win[5], upwind_pos(mx), weps);
+    // operands of the y mass fluxes and the SGS add-in, issued before the barrier
+    const float vA = v[iv(d, I, Jc, Kc - 1)], vB = v[iv(d, I, Jc, Kc)];
+    const float vC = v[iv(d, I, Jc + 1, Kc - 1)], vD = v[iv(d, I, Jc + 1, Kc)];
+    const long ow = ((long)i * d.ny + (Jc - 3)) * (d.nz + 1);
+    const float sgl = sgs ? sgs[ow + (Kc - 1)] : 0.0f, sgt = sgs ? sgs[ow + d.nz] : 0.0f;
     plane_load(P, w + (long)I * sx, psl);
     if (act) {
       FD[ty * nzl + l] = fd_cell(d, rc, u, v, w, I, Jc, K);
@@ -412,12 +433,12 @@ __global__ void __launch_bounds__(1024) k_mom_w(Dims d, const float* __restrict_
       float mz = 0.5f * (Pc[cz] + Pc[cz + 1]) * rc[cz];
       FZ[ty * nzl + l] = mz * vrecon(Pc, Lw, cz, upwind_pos(mz), weps);
       if (interior) {
-        float my = 0.5f * (rc[K - 1] * v[iv(d, I, Jc, K - 1)] + rc[K] * v[iv(d, I, Jc, K)]);
+        float my = 0.5f * (rc[K - 1] * vA + rc[K] * vB);
         FY[ty * nzl + l] = my * hrecon(Pc[K - 3 * Lw], Pc[K - 2 * Lw], Pc[K - Lw], Pc[K], Pc[K + Lw], Pc[K + 2 * Lw],
                                        upwind_pos(my), weps);
         if (last_row) {
           const float* Pn = Pc + Lw;
-          float m2 = 0.5f * (rc[K - 1] * v[iv(d, I, Jc + 1, K - 1)] + rc[K] * v[iv(d, I, Jc + 1, K)]);
+          float m2 = 0.5f * (rc[K - 1] * vC + rc[K] * vD);
           FY[(ty + 1) * nzl + l] = m2 * hrecon(Pn[K - 3 * Lw], Pn[K - 2 * Lw], Pn[K - Lw], Pn[K], Pn[K + Lw],
                                                Pn[K + 2 * Lw], upwind_pos(m2), weps);
         }
@@ -441,12 +462,12 @@ __global__ void __launch_bounds__(1024) k_mom_w(Dims d, const float* __restrict_
         a = (conv + wc * div8w) / rho8w;
       }
       a = a + b8;
-      if (sgs) a = a + sgs[o + l];
+      if (sgs) a = a + sgl;
       rw[o + l] = a;
       if (l == nz - 1) {   // lid level: zero advection, extrapolated buoyancy
         float bt = 1.875f * Bb[nz - 1] + (-1.25f) * Bb[nz - 2] + 0.375f * Bb[nz - 3];
         float at = 0.0f + bt;
-        if (sgs) at = at + sgs[o + nz];
+        if (sgs) at = at + sgt;
         rw[o + nz] = at;
       }
     }
