@@ -51,6 +51,20 @@ __device__ __forceinline__ void plane_load(float* __restrict__ P, const float* _
   for (int q = 0; q < kSlots; ++q)
     if (q < ps.n) P[ps.dst(q)] = plane[ps.off[q]];
 }
+// Split form for software pipelining: fetch the next plane into registers while the
+// current one is being used, store it to LDS at the top of the next step.
+struct PlaneRegs {
+  float v[kSlots];
+};
+__device__ __forceinline__ void plane_fetch(PlaneRegs& r, const float* __restrict__ plane, const PlaneSlots& ps) {
+#pragma unroll
+  for (int q = 0; q < kSlots; ++q) r.v[q] = (q < ps.n) ? plane[ps.off[q]] : 0.0f;
+}
+__device__ __forceinline__ void plane_store(float* __restrict__ P, const PlaneRegs& r, const PlaneSlots& ps) {
+#pragma unroll
+  for (int q = 0; q < kSlots; ++q)
+    if (q < ps.n) P[ps.dst(q)] = r.v[q];
+}
 
 struct ScalarSet {
   const float* s0[kMaxScalars];   // xi0 fields (padded)
@@ -109,6 +123,8 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
     }
   }
   float ulo = u[iu(d, i0 + 3, Jc, Kc)];
+  PlaneRegs pr;
+  plane_fetch(pr, S.s[0] + (long)(i0 + 3) * sx, psl);
 
   for (int i = i0; i < iend; ++i) {
     const int I = i + 3;
@@ -138,8 +154,11 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
     for (int q = 0; q < NS; ++q) {
       const float* s = S.s[q];
       const float weps = S.weps[q];
-      // plane of this scalar at x = I: rows J = j0 .. j0+kFTY+5 (padded), all levels
-      plane_load(P, s + (long)I * sx, psl);
+      // plane of this scalar at x = I: rows J = j0 .. j0+kFTY+5 (padded), all levels,
+      // fetched one plane ahead; then fetch the next (scalar q+1 here, or scalar 0 at I+1)
+      plane_store(P, pr, psl);
+      if (q + 1 < NS) plane_fetch(pr, S.s[(q + 1) % NS] + (long)I * sx, psl);
+      else if (i + 1 < iend) plane_fetch(pr, S.s[0] + (long)(I + 1) * sx, psl);
       __syncthreads();
       const float* Pc = P + (ty + 3) * Lz;   // this row's column
       const float qc = Pc[Kc];
@@ -230,6 +249,8 @@ __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict_
     fxlo = mass * hrecon(m3, win[0], win[1], win[2], win[3], win[4], upwind_pos(mass), weps);
     fdlo = fd_cell(d, rc, u, v, w, I - 1, Jc, Kc);
   }
+  PlaneRegs pr;
+  plane_fetch(pr, u + (long)(i0 + 3) * sx, psl);
   for (int i = i0; i < iend; ++i) {
     const int I = i + 3;
     const float uc = win[2];
@@ -241,7 +262,8 @@ __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict_
     const float vA = v[iv(d, I - 1, Jc, Kc)], vB = v[iv(d, I, Jc, Kc)];
     const float vC = v[iv(d, I - 1, Jc + 1, Kc)], vD = v[iv(d, I, Jc + 1, Kc)];
     const float sgv = sgs ? sgs[((long)i * d.ny + (Jc - 3)) * d.nz + (Kc - 1)] : 0.0f;
-    plane_load(P, u + (long)I * sx, psl);
+    plane_store(P, pr, psl);  // fetched one plane ahead
+    if (i + 1 < iend) plane_fetch(pr, u + (long)(I + 1) * sx, psl);
     __syncthreads();
     const float* Pc = P + (ty + 3) * Lz;
     if (act) {
@@ -317,6 +339,8 @@ __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict_
     float mass = 0.5f * (r * u[iu(d, I, Jc - 1, Kc)] + r * u[iu(d, I, Jc, Kc)]);
     fxlo = mass * hrecon(m3, win[0], win[1], win[2], win[3], win[4], upwind_pos(mass), weps);
   }
+  PlaneRegs pr;
+  plane_fetch(pr, v + (long)(i0 + 3) * sx, psl);
   for (int i = i0; i < iend; ++i) {
     const int I = i + 3;
     const float vc = win[2];
@@ -327,7 +351,8 @@ __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict_
     // operands of the z mass flux and the SGS add-in, issued before the barrier
     const float wA = w[iw(d, I, Jc - 1, Kc)], wB = w[iw(d, I, Jc, Kc)];
     const float sgv = sgs ? sgs[((long)i * (d.ny + 1) + (Jc - 3)) * d.nz + (Kc - 1)] : 0.0f;
-    plane_load(P, v + (long)I * sx, psl);
+    plane_store(P, pr, psl);  // fetched one plane ahead
+    if (i + 1 < iend) plane_fetch(pr, v + (long)(I + 1) * sx, psl);
     __syncthreads();
     const float* Pc = P + (ty + 3) * Lz;
     if (act) {
@@ -409,6 +434,8 @@ __global__ void __launch_bounds__(1024) k_mom_w(Dims d, const float* __restrict_
     float mass = 0.5f * (rc[Kc - 1] * u[iu(d, I, Jc, Kc - 1)] + rc[Kc] * u[iu(d, I, Jc, Kc)]);
     fxlo = mass * hrecon(m3, win[0], win[1], win[2], win[3], win[4], upwind_pos(mass), weps);
   }
+  PlaneRegs pr;
+  plane_fetch(pr, w + (long)(i0 + 3) * sx, psl);
   for (int i = i0; i < iend; ++i) {
     const int I = i + 3;
     const float wc = win[2];
@@ -419,7 +446,8 @@ __global__ void __launch_bounds__(1024) k_mom_w(Dims d, const float* __restrict_
     const float vC = v[iv(d, I, Jc + 1, Kc - 1)], vD = v[iv(d, I, Jc + 1, Kc)];
     const long ow = ((long)i * d.ny + (Jc - 3)) * (d.nz + 1);
     const float sgl = sgs ? sgs[ow + (Kc - 1)] : 0.0f, sgt = sgs ? sgs[ow + d.nz] : 0.0f;
-    plane_load(P, w + (long)I * sx, psl);
+    plane_store(P, pr, psl);  // fetched one plane ahead
+    if (i + 1 < iend) plane_fetch(pr, w + (long)(I + 1) * sx, psl);
     if (act) {
       FD[ty * nzl + l] = fd_cell(d, rc, u, v, w, I, Jc, K);
       long c = ic(d, I, Jc, K);
