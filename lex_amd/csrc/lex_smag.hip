@@ -32,6 +32,30 @@ __device__ __forceinline__ float e23_at(const Dims& d, const float* __restrict__
   return 0.5f * (dvdz + dwdy);
 }
 
+// Vertical stencil of N^2 at a cell: an interior cell averages the w levels k and k+1;
+// the bottom (top) cell extrapolates from levels 1, 2, 3 (nz-1, nz-2, nz-3) with 1.875,
+// -1.25, 0.375 (extrapolate_bottom_top).  Written as three weighted levels for every
+// lane, so the boundary lanes take no separate path.
+struct ZStencil {
+  int l[3];
+  float c[3];
+};
+__device__ __forceinline__ ZStencil z_stencil(int k, int nz) {
+  ZStencil z;
+  if (k == 0) {
+    z.l[0] = 1; z.l[1] = 2; z.l[2] = 3;
+  } else if (k == nz - 1) {
+    z.l[0] = nz - 1; z.l[1] = nz - 2; z.l[2] = nz - 3;
+  } else {
+    z.l[0] = k; z.l[1] = k + 1; z.l[2] = k;
+  }
+  const bool edge = (k == 0) || (k == nz - 1);
+  z.c[0] = edge ? 1.875f : 0.5f;
+  z.c[1] = edge ? -1.25f : 0.5f;
+  z.c[2] = edge ? 0.375f : 0.0f;
+  return z;
+}
+
 __global__ void k_smag_deform(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
                               const float* __restrict__ v, const float* __restrict__ w, float* __restrict__ s11,
                               float* __restrict__ s22, float* __restrict__ s33, float* __restrict__ s12,
@@ -72,13 +96,13 @@ __global__ void k_smag_deform(Dims d, const float* __restrict__ rc, const float*
   a13 = r * a13;
   a23 = r * a23;
   float df = (2.0f * (a11 * a11 + a22 * a22 + a33 * a33) + 4.0f * (a12 * a12 + a13 * a13 + a23 * a23)) / (r * r);
-  store_c(d, s11, i, j, k, a11);
-  store_c(d, s22, i, j, k, a22);
-  store_c(d, s33, i, j, k, a33);
-  store_c(d, s12, i, j, k, a12);
-  store_c(d, s13, i, j, k, a13);
-  store_c(d, s23, i, j, k, a23);
-  store_c(d, deform, i, j, k, df);
+  store_c_fast(d, s11, i, j, k, a11);
+  store_c_fast(d, s22, i, j, k, a22);
+  store_c_fast(d, s33, i, j, k, a33);
+  store_c_fast(d, s12, i, j, k, a12);
+  store_c_fast(d, s13, i, j, k, a13);
+  store_c_fast(d, s23, i, j, k, a23);
+  store_c_fast(d, deform, i, j, k, df);
 }
 
 // N^2 at interior w level l (1..nz-1) of column (I,J)
@@ -96,15 +120,11 @@ __global__ void k_smag_nm(Dims d, const float* __restrict__ th, const float* __r
   int i = tb.z;
   if (k >= d.nz || j >= d.ny) return;
   long c = ic(d, i + 3, j + 3, 0);
-  int nz = d.nz;
-  float val;
-  if (k == 0)
-    val = ex3(n28w_at(d, th, qv, c, 1), n28w_at(d, th, qv, c, 2), n28w_at(d, th, qv, c, 3));
-  else if (k == nz - 1)
-    val = ex3(n28w_at(d, th, qv, c, nz - 1), n28w_at(d, th, qv, c, nz - 2), n28w_at(d, th, qv, c, nz - 3));
-  else
-    val = 0.5f * (n28w_at(d, th, qv, c, k) + n28w_at(d, th, qv, c, k + 1));
-  store_c(d, n2, i, j, k, val);
+  const ZStencil z = z_stencil(k, d.nz);
+  float val = 0.f;
+#pragma unroll
+  for (int t = 0; t < 3; ++t) val += z.c[t] * n28w_at(d, th, qv, c, z.l[t]);
+  store_c_fast(d, n2, i, j, k, val);
 }
 
 // elementwise over the padded array
@@ -238,17 +258,20 @@ __global__ void k_smag_scalars(Dims d, const float* __restrict__ rc, const float
   // -rho_face * kh_face per face (same association as smag_scalar_point)
   const float kx0 = -r8u * (0.5f * (kh[c] + kh[c - sx])), kx1 = -r8u * (0.5f * (kh[c + sx] + kh[c]));
   const float ky0 = -r8u * (0.5f * (kh[c] + kh[c - sy])), ky1 = -r8u * (0.5f * (kh[c + sy] + kh[c]));
+  // vertical faces: zero flux at the surface and the lid (levels c-1 and c+1 exist in the
+  // padded column, so the loads need no branch)
   const bool zlo = k > 0, zhi = k + 1 < nz;
   const float kz0 = zlo ? -(0.5f * (rc[K - 1] + rc[K])) * (0.5f * (kh[c - 1] + kh[c])) : 0.0f;
   const float kz1 = zhi ? -(0.5f * (rc[K] + rc[K + 1])) * (0.5f * (kh[c] + kh[c + 1])) : 0.0f;
   const long o = ((long)i * d.ny + j) * d.nz + k;
   for (int q = 0; q < S.n; ++q) {
     const float* sc = S.s[q];
-    const float s0 = sc[c];
-    float tx1 = kx1 * ((sc[c + sx] - s0) * d.idx), tx0 = kx0 * ((s0 - sc[c - sx]) * d.idx);
-    float ty1 = ky1 * ((sc[c + sy] - s0) * d.idy), ty0 = ky0 * ((s0 - sc[c - sy]) * d.idy);
-    float tz1 = zhi ? kz1 * ((sc[c + 1] - s0) * d.idz) : 0.0f;
-    float tz0 = zlo ? kz0 * ((s0 - sc[c - 1]) * d.idz) : 0.0f;
+    const float s0 = sc[c], sxp = sc[c + sx], sxm = sc[c - sx], syp = sc[c + sy], sym = sc[c - sy];
+    const float szp = sc[c + 1], szm = sc[c - 1];
+    float tx1 = kx1 * ((sxp - s0) * d.idx), tx0 = kx0 * ((s0 - sxm) * d.idx);
+    float ty1 = ky1 * ((syp - s0) * d.idy), ty0 = ky0 * ((s0 - sym) * d.idy);
+    float tz1 = zhi ? kz1 * ((szp - s0) * d.idz) : 0.0f;
+    float tz0 = zlo ? kz0 * ((s0 - szm) * d.idz) : 0.0f;
     float div = (tx1 - tx0) * d.idx + (ty1 - ty0) * d.idy + (tz1 - tz0) * d.idz;
     S.out[q][o] = -div / r;
   }

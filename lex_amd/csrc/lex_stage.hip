@@ -193,7 +193,7 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
                          cg * cg, o.prandtl);
       LEX_LAUNCH(KID_SMAG_UVW, k_smag_uvw, G(d.nz + 1, d.ny + 1, d.nx + 1), bk, 0, st, d, rc, sm[8], sm[0], sm[1], sm[2],
                          sm[3], sm[4], sm[5], su, sv, sw);
-      if (o.impl == 2) {  // every scalar in one pass
+      if (o.impl != 1) {  // every scalar in one pass (impl 1: one call per scalar, as the reference)
         SgsScalars SS;
         SS.n = 2 + o.n_passive;
         for (int q = 0; q < SS.n; ++q) {
