@@ -154,6 +154,48 @@ __device__ float vrecon_grad(const float* __restrict__ p, int L, int f, bool pos
   return R;
 }
 
+// vrecon_grad on preloaded values p6[t] = p[f-2+t] (entries outside the column unused), so
+// the caller can issue the column loads together with its other loads
+__device__ float vrecon_grad6(const float* p6, int L, int f, bool pos, float weps, float* g6, float ref = 0.f) {
+  for (int t = 0; t < 6; ++t) g6[t] = 0.f;
+  if (f <= 0 || f >= L - 2) return -ref;
+  float g[5];
+  float R;
+  if (pos) {
+    if (f == 1) {  // 0.5 (p1 + p2)
+      g6[2] = 0.5f;
+      g6[3] = 0.5f;
+      return 0.5f * ((p6[2] - ref) + (p6[3] - ref));
+    }
+    if (f == 2 || f == L - 3) {  // weno3(p[f-1], p[f], p[f+1])
+      R = weno3_grad(p6[1], p6[2], p6[3], weps, g, ref);
+      g6[1] = g[0]; g6[2] = g[1]; g6[3] = g[2];
+      return R;
+    }
+    R = weno5_grad(p6[0], p6[1], p6[2], p6[3], p6[4], weps, g, ref);
+    for (int t = 0; t < 5; ++t) g6[t] = g[t];
+    return R;
+  }
+  if (f == 1) {  // weno3(p3, p2, p1)
+    R = weno3_grad(p6[4], p6[3], p6[2], weps, g, ref);
+    g6[4] = g[0]; g6[3] = g[1]; g6[2] = g[2];
+    return R;
+  }
+  if (f == L - 3) {  // 0.5 (p[f+1] + p[f])
+    g6[3] = 0.5f;
+    g6[2] = 0.5f;
+    return 0.5f * ((p6[3] - ref) + (p6[2] - ref));
+  }
+  if (f == L - 4) {  // weno3(p[f+2], p[f+1], p[f])
+    R = weno3_grad(p6[4], p6[3], p6[2], weps, g, ref);
+    g6[4] = g[0]; g6[3] = g[1]; g6[2] = g[2];
+    return R;
+  }
+  R = weno5_grad(p6[5], p6[4], p6[3], p6[2], p6[1], weps, g, ref);
+  g6[5] = g[0]; g6[4] = g[1]; g6[3] = g[2]; g6[2] = g[3]; g6[1] = g[4];
+  return R;
+}
+
 __device__ __forceinline__ void scatter6(float* __restrict__ lq, long base, long stride, const float* g6, float coef) {
 #pragma unroll
   for (int t = 0; t < 6; ++t)
@@ -223,71 +265,63 @@ __global__ void k_adv_scalar_vjp(Dims d, const float* __restrict__ rc, const flo
   if (k >= d.nz || j > d.ny) return;
   const int I = i + 3, J = j + 3, K = k + 1;
   const bool cell = i < d.nx && j < d.ny;
-  const float r = rc[K];
-  const float mc = mu_cell(d, lam, rc, i, j, k, cell);
-  const float mlx = mu_cell(d, lam, rc, i - 1, j, k, i > 0 && j < d.ny);
-  const float mly = mu_cell(d, lam, rc, i, j - 1, k, j > 0 && i < d.nx);
-  const float mb = mu_cell(d, lam, rc, i, j, k - 1, cell && k > 0);  // cell below
+  const bool fx = j < d.ny, fy = i < d.nx, fz = cell && k > 0;
   const long sx = d.sxc(), sy = d.syc();
   const long c = ic(d, I, J, K);
-  const float sc = s[c];
-  float q6[6], g6[6];
-  if (j < d.ny) {  // x face I: lower face of cell I, upper face of cell I-1
-    const float ml = mlx;
-    const float gF = (mc - ml) * d.idx;
-    const long a = iu(d, I, J, K);
-    const float mass = r * u[a];
+  // Every operand is loaded up front (the x face of row ny and the y face of column nx
+  // read ghost lines, which exist), the three reconstructions are evaluated for every
+  // thread, and only the adjoint updates depend on which faces the thread owns.
+  const float mc = mu_cell(d, lam, rc, i, j, k, cell);
+  const float mlx = mu_cell(d, lam, rc, i - 1, j, k, i > 0 && fx);
+  const float mly = mu_cell(d, lam, rc, i, j - 1, k, j > 0 && fy);
+  const float mb = mu_cell(d, lam, rc, i, j, k - 1, fz);  // cell below
+  float qx[6], qy[6], qz[6];
+#pragma unroll
+  for (int t = 0; t < 6; ++t) {
+    qx[t] = s[c + (t - 3) * sx];
+    qy[t] = s[c + (t - 3) * sy];
+    qz[t] = s[c - K + min(max(k - 2 + t, 0), d.nz + 1)];
+  }
+  const float sc = qx[3];
+  const long au = iu(d, I, J, K), av = iv(d, I, J, K), wb = iw(d, I, J, 0);
+  const float ua = u[au], va = v[av], wa = w[wb + K];
+  const float rcm = rc[K - 1], r = rc[K], rcp = rc[K + 1];
+  const float fd = fd_at(d, rc, u, v, w, I, J, K);
+  const float r8 = 0.5f * (rcm + r);
+  float gx6[6], gy6[6], gz6[6];
+  const float massx = r * ua, massy = r * va, massz = r8 * wa;
+  const float Rcx = hrecon_grad(qx, upwind_pos(massx), weps, gx6, sc);
+  const float Rcy = hrecon_grad(qy, upwind_pos(massy), weps, gy6, sc);
+  const float Rcz = vrecon_grad6(qz, d.nz + 2, k, upwind_pos(massz), weps, gz6, sc);
+  if (fx) {  // x face I: lower face of cell I, upper face of cell I-1
+    const float gF = (mc - mlx) * d.idx;
     // d(flux)/du collects mc (R - s_I) - ml (R - s_{I-1}) = gF (R - s_I) - ml (s_I - s_{I-1}) / dx:
     // every factor is a small difference, not a difference of theta-sized products
-    float g = -ml * d.idx * (sc - s[c - sx]);
-    if (gF != 0.f) {
-      const long base = c - 3 * sx;
-      for (int t = 0; t < 6; ++t) q6[t] = s[base + t * sx];
-      float Rc = hrecon_grad(q6, upwind_pos(mass), weps, g6, sc);
-      scatter6(ls, base, sx, g6, gF * mass);
-      g += gF * Rc;
-    }
-    if (g != 0.f) lu[a] += r * g;
+    const float g = -mlx * d.idx * (sc - qx[2]) + gF * Rcx;
+    scatter6(ls, c - 3 * sx, sx, gx6, gF * massx);
+    if (g != 0.f) lu[au] += r * g;
   }
-  if (i < d.nx) {  // y face J
-    const float ml = mly;
-    const float gF = (mc - ml) * d.idy;
-    const long a = iv(d, I, J, K);
-    const float mass = r * v[a];
-    float g = -ml * d.idy * (sc - s[c - sy]);
-    if (gF != 0.f) {
-      const long base = c - 3 * sy;
-      for (int t = 0; t < 6; ++t) q6[t] = s[base + t * sy];
-      float Rc = hrecon_grad(q6, upwind_pos(mass), weps, g6, sc);
-      scatter6(ls, base, sy, g6, gF * mass);
-      g += gF * Rc;
-    }
-    if (g != 0.f) lv[a] += r * g;
+  if (fy) {  // y face J
+    const float gF = (mc - mly) * d.idy;
+    const float g = -mly * d.idy * (sc - qy[2]) + gF * Rcy;
+    scatter6(ls, c - 3 * sy, sy, gy6, gF * massy);
+    if (g != 0.f) lv[av] += r * g;
   }
   if (!cell) return;
-  const float* col = s + ic(d, I, J, 0);
-  float* lcol = ls + ic(d, I, J, 0);
-  const long wb = iw(d, I, J, 0);
-  const float r8 = 0.5f * (rc[K - 1] + rc[K]);
   float g;
   if (k > 0) {  // flux face between levels k and k+1 (the surface face k = 0 is prescribed)
     const float gF = (mc - mb) * d.idz;
-    g = -mb * d.idz * (sc - col[K - 1]);
-    if (gF != 0.f) {
-      const float mass = r8 * w[wb + K];
-      float Rc = vrecon_grad(col, d.nz + 2, k, upwind_pos(mass), weps, g6, sc);
-      scatter6z(lcol, k, d.nz + 2, g6, gF * mass);
-      g += gF * Rc;
-    }
+    g = -mb * d.idz * (sc - qz[2]) + gF * Rcz;
+    scatter6z(ls + (c - K), k, d.nz + 2, gz6, gF * massz);
   } else {
     g = -mc * sc * d.idz;
   }
   if (g != 0.f) lw[wb + K] += r8 * g;
   if (k == d.nz - 1) {  // lid level: div(rho0 u) of the top cell only
-    const float gt = mc * sc * (0.5f * (rc[K] + rc[K + 1])) * d.idz;
+    const float gt = mc * sc * (0.5f * (r + rcp)) * d.idz;
     if (gt != 0.f) lw[wb + K + 1] += gt;
   }
-  aadd(ls + c, mc * fd_at(d, rc, u, v, w, I, J, K));
+  aadd(ls + c, mc * fd);
 }
 
 // ------------------------------------------------------------ A3 momentum advection
