@@ -307,6 +307,28 @@ int lex_rk3_step_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o, const le
                          const float* su, const float* sv, const float* sw, const float* const* ss, void* workspace,
                          size_t workspace_bytes, void* stream);
 
+/* ---- training the net through the solver ---------------------------------------------
+ * Replace jax.value_and_grad of the loss through ws_rk3_sprint_dl_train in train_step
+ * (lex_dl_train.py:111-131; the sprint is one_sprint_functions.py:118-165) for the
+ * build's in-stage SGS net.  Reverses of lex_sgs_apply (ly written) and of
+ * lex_sgs_features (padded adjoints of u, v, w, theta, qv accumulated, ghost lines
+ * included: fold with lex_fold afterwards; scale as passed to lex_sgs_features). */
+int lex_sgs_apply_vjp(const lex_grid_t* g, const float* lsu, const float* lsv, const float* lsw, const float* lsth,
+                      const float* lsqv, const float* out_scale, float* ly, void* stream);
+int lex_sgs_features_vjp(const lex_grid_t* g, const float* lfeat, const float* scale, float* lu, float* lv, float* lw,
+                         float* lth, float* lqv, void* stream);
+/* lex_rk3_step_tape / lex_rk3_step_vjp_taped with caller-supplied SGS tendencies (as
+ * lex_rk3_step_ext_sgs): the reverse writes the adjoints of su, sv, sw and ss[] (same
+ * shapes) instead of taking them through the Smagorinsky reverse. */
+int lex_rk3_step_tape_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b,
+                              float* const* state, const float* su, const float* sv, const float* sw,
+                              const float* const* ss, float* const* state_out, void* tape, size_t tape_bytes,
+                              void* stream);
+int lex_rk3_step_vjp_taped_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b,
+                                   float* const* state, const void* tape, size_t tape_bytes, float* const* lam_out,
+                                   float* const* lam_in, float* lsu, float* lsv, float* lsw, float* const* lss,
+                                   void* adjoint_ws, size_t adjoint_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

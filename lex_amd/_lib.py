@@ -129,6 +129,14 @@ _SIGS = {
     "lex_coarsen": [_G, _I, _I, _I, _I, _P, _P, _P],
     "lex_rk3_step_ext_sgs": [_G, C.POINTER(StepOpts), C.POINTER(BaseCols), C.POINTER(_P), _P, _P, _P,
                              C.POINTER(_P), _P, C.c_size_t, _P],
+    # training the net through the solver
+    "lex_sgs_apply_vjp": [_G, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lex_sgs_features_vjp": [_G, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lex_rk3_step_tape_ext_sgs": [_G, C.POINTER(StepOpts), C.POINTER(BaseCols), C.POINTER(_P), _P, _P, _P,
+                                  C.POINTER(_P), C.POINTER(_P), _P, C.c_size_t, _P],
+    "lex_rk3_step_vjp_taped_ext_sgs": [_G, C.POINTER(StepOpts), C.POINTER(BaseCols), C.POINTER(_P), _P,
+                                       C.c_size_t, C.POINTER(_P), C.POINTER(_P), _P, _P, _P, C.POINTER(_P), _P,
+                                       C.c_size_t, _P],
 }
 _RESTYPE = {"lex_last_error": C.c_char_p, "lex_bicgstab_workspace_bytes": C.c_size_t,
             "lex_rk3_workspace_bytes": C.c_size_t, "lex_profile_kernel_name": C.c_char_p,

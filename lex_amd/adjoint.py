@@ -113,6 +113,74 @@ def step_vjp_taped(phys_state, tape, lam_out, base_state, grid, dt, n_sound, cs,
     return li, dcs
 
 
+def _adj_workspace(grid, dt, n_sound, n_passive, device):
+    key = ("adj", grid.nx, grid.ny, grid.nz, float(dt), int(n_sound), n_passive, str(device))
+    ws = _WSV.get(key)
+    if ws is None:
+        _WSV.clear()
+        ws = torch.empty(_bytes("lex_rk3_adjoint_bytes", grid, dt, n_sound, n_passive), dtype=torch.uint8,
+                         device=device)
+        _WSV[key] = ws
+    return ws
+
+
+def step_tape_ext_sgs(phys_state, sgs, base_state, grid, dt, n_sound, tape=None):
+    """step_tape with caller-supplied SGS tendencies sgs = (su, sv, sw, s_theta, s_qv, ...)
+    frozen over the step (the learned net's); returns (tape, output state)."""
+    n_passive = len(phys_state) - 6
+    if tape is None:
+        tape = torch.empty(_bytes("lex_rk3_tape_bytes", grid, dt, n_sound, n_passive), dtype=torch.uint8,
+                           device=phys_state[0].device)
+    out = tuple(torch.empty_like(t) for t in phys_state)
+    opts = _lib.StepOpts(float(dt), int(n_sound), 2, 0.18, 1.0 / 3.0, n_passive, 1)
+    cols, bc = _base_cols(base_state)
+    _lib.call("lex_rk3_step_tape_ext_sgs", grid, opts, bc, _ptrs(phys_state), _lib.ptr(sgs[0]), _lib.ptr(sgs[1]),
+              _lib.ptr(sgs[2]), _ptrs(sgs[3:]), _ptrs(out), _lib.ptr(tape), tape.numel(), _lib.stream())
+    del cols
+    return tape, out
+
+
+def step_vjp_taped_ext_sgs(phys_state, tape, lam_out, base_state, grid, dt, n_sound):
+    """Reverse of step_tape_ext_sgs: returns (lam_in, lam_sgs) with lam_sgs the adjoints of
+    (su, sv, sw, s_theta, s_qv, ...)."""
+    n_passive = len(phys_state) - 6
+    g = grid
+    dev = phys_state[0].device
+    opts = _lib.StepOpts(float(dt), int(n_sound), 2, 0.18, 1.0 / 3.0, n_passive, 1)
+    cols, bc = _base_cols(base_state)
+    lo = tuple(t.clone() for t in lam_out)
+    li = tuple(torch.empty_like(t) for t in phys_state)
+    lsu = torch.empty((g.nx + 1, g.ny, g.nz), dtype=torch.float32, device=dev)
+    lsv = torch.empty((g.nx, g.ny + 1, g.nz), dtype=torch.float32, device=dev)
+    lsw = torch.empty((g.nx, g.ny, g.nz + 1), dtype=torch.float32, device=dev)
+    lss = tuple(torch.empty((g.nx, g.ny, g.nz), dtype=torch.float32, device=dev) for _ in range(2 + n_passive))
+    ws = _adj_workspace(grid, dt, n_sound, n_passive, dev)
+    _lib.call("lex_rk3_step_vjp_taped_ext_sgs", grid, opts, bc, _ptrs(phys_state), _lib.ptr(tape), tape.numel(),
+              _ptrs(lo), _ptrs(li), _lib.ptr(lsu), _lib.ptr(lsv), _lib.ptr(lsw), _ptrs(lss), _lib.ptr(ws), ws.numel(),
+              _lib.stream())
+    del cols
+    return li, (lsu, lsv, lsw) + lss
+
+
+def sgs_apply_vjp(grid, lsgs, out_scale):
+    """Adjoint of lex_sgs_apply: (lsu, lsv, lsw, ls_theta, ls_qv) -> ly (nx, ny, nz, 5)."""
+    g = grid
+    ly = torch.empty((g.nx, g.ny, g.nz, 5), dtype=torch.float32, device=lsgs[0].device)
+    _lib.call("lex_sgs_apply_vjp", g, _lib.ptr(lsgs[0]), _lib.ptr(lsgs[1]), _lib.ptr(lsgs[2]), _lib.ptr(lsgs[3]),
+              _lib.ptr(lsgs[4]), out_scale, _lib.ptr(ly), _lib.stream())
+    return ly
+
+
+def sgs_features_vjp(grid, lfeat, scale, lam):
+    """Adds the adjoint of lex_sgs_features (normalised features' adjoint lfeat) into the
+    padded state adjoint lam = (theta, u, v, w, pip, qv, ...) and folds the ghost lines."""
+    _lib.call("lex_sgs_features_vjp", grid, _lib.ptr(lfeat.contiguous()), scale, _lib.ptr(lam[1]), _lib.ptr(lam[2]),
+              _lib.ptr(lam[3]), _lib.ptr(lam[0]), _lib.ptr(lam[5]), _lib.stream())
+    for t, mode in ((lam[0], 0), (lam[1], 1), (lam[2], 2), (lam[3], 3), (lam[5], 0)):
+        fold(t, mode, grid)
+    return lam
+
+
 def fold(adj, mode, grid):
     """Add ghost adjoints onto their interior sources in place (mode 0 c, 1 u, 2 v, 3 w)."""
     _lib.call("lex_fold", grid, int(mode), _lib.ptr(adj), _lib.stream())

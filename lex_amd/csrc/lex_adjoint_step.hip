@@ -150,7 +150,9 @@ void rk3_forward_tape(const Dims& d, const O& o, const B& b, float* const* X0, c
     float* ru = (float*)(ws + L.off_ru[s]);
     float* rv = (float*)(ws + L.off_rv[s]);
     float* rw = (float*)(ws + L.off_rw[s]);
-    if (s == 0 && turb) {
+    // turb 1: Smagorinsky recorded here; turb 2: the caller's SGS tendencies are already in
+    // the tape's SGS slots
+    if (s == 0 && o.turb == 1) {
       hipLaunchKernelGGL(k_smag_deform, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, rc, src[1], src[2], src[3], sm[0],
                          sm[1], sm[2], sm[3], sm[4], sm[5], sm[6]);
       hipLaunchKernelGGL(k_smag_nm, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, src[0], src[5], sm[7]);
@@ -357,7 +359,9 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
                        (const float*)src[0], (const float*)src[5], (const float*)ltr, lsrc[0], lsrc[5]);
   }
 
-  if (turb) {
+  // turb 2: the adjoints of the caller's SGS tendencies (lsu, lsv, lsw, lss in the adjoint
+  // workspace) are the result; turb 1 takes them through the Smagorinsky reverse
+  if (o.turb == 1) {
     // lsm was cleared with the scratch region and is not written before this point
     for (int q = 0; q < nsc; ++q) {
       int fi = (q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2));

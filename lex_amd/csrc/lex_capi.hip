@@ -706,6 +706,79 @@ int lex_rk3_step_vjp_taped(const lex_grid_t* g, const lex_step_opts_t* o, const 
   return 0;
 }
 
+// recorded step and reverse with caller-supplied SGS tendencies (turb 2: the learned net)
+int lex_rk3_step_tape_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b,
+                              float* const* state, const float* su, const float* sv, const float* sw,
+                              const float* const* ss, float* const* state_out, void* tape, size_t tape_bytes,
+                              void* stream) {
+  REQGRID(g);
+  REQ(o); REQ(b); REQ(state); REQ(state_out); REQ(tape); REQ(su); REQ(sv); REQ(sw); REQ(ss);
+  REQ(b->rho0); REQ(b->theta0); REQ(b->qv0); REQ(b->pi0);
+  if (o->n_passive < 0 || o->n_passive > 8) return fail(-1, "lex_rk3_step_tape_ext_sgs: n_passive must be 0..8%s");
+  if (o->n_sound < 1 || !(o->dt > 0.f)) return fail(-1, "lex_rk3_step_tape_ext_sgs: need n_sound >= 1 and dt > 0%s");
+  for (int q = 0; q < 6 + o->n_passive; ++q) {
+    if (!state[q] || !state_out[q]) return fail(-2, "lex_rk3_step_tape_ext_sgs: null field pointer%s");
+    if (state[q] == state_out[q]) return fail(-1, "lex_rk3_step_tape_ext_sgs: state_out must not alias state%s");
+  }
+  for (int q = 0; q < 2 + o->n_passive; ++q)
+    if (!ss[q]) return fail(-2, "lex_rk3_step_tape_ext_sgs: null scalar SGS pointer%s");
+  Dims d = mkdims(g);
+  VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
+  if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_tape_ext_sgs: tape too small%s");
+  hipStream_t st = S(stream);
+  char* t = (char*)tape;
+  auto cp = [&](const float* src, size_t off, size_t n) {
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (long)n, src, (float*)(t + off));
+  };
+  cp(su, L.off_su, L.iu_);
+  cp(sv, L.off_sv, L.iv_);
+  cp(sw, L.off_sw, L.iw_);
+  for (int q = 0; q < 2 + o->n_passive; ++q) cp(ss[q], L.off_ss + q * al256(L.ic_ * 4), L.ic_);
+  lex_step_opts_t o2 = *o;
+  o2.turb = 2;
+  rk3_forward_tape(d, o2, *b, state, t, L, st, state_out);
+  return check_launch("lex_rk3_step_tape_ext_sgs");
+}
+
+int lex_rk3_step_vjp_taped_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b,
+                                   float* const* state, const void* tape, size_t tape_bytes, float* const* lam_out,
+                                   float* const* lam_in, float* lsu, float* lsv, float* lsw, float* const* lss,
+                                   void* adj, size_t adj_bytes, void* stream) {
+  REQGRID(g);
+  REQ(o); REQ(b); REQ(state); REQ(tape); REQ(lam_out); REQ(lam_in); REQ(adj); REQ(lsu); REQ(lsv); REQ(lsw); REQ(lss);
+  REQ(b->rho0); REQ(b->theta0); REQ(b->qv0); REQ(b->pi0);
+  if (o->n_passive < 0 || o->n_passive > 8) return fail(-1, "lex_rk3_step_vjp_taped_ext_sgs: n_passive must be 0..8%s");
+  if (o->n_sound < 1 || !(o->dt > 0.f))
+    return fail(-1, "lex_rk3_step_vjp_taped_ext_sgs: need n_sound >= 1 and dt > 0%s");
+  for (int q = 0; q < 6 + o->n_passive; ++q) {
+    if (!state[q] || !lam_out[q] || !lam_in[q]) return fail(-2, "lex_rk3_step_vjp_taped_ext_sgs: null field pointer%s");
+    if (lam_in[q] == lam_out[q] || lam_in[q] == state[q])
+      return fail(-1, "lex_rk3_step_vjp_taped_ext_sgs: lam_in must not alias lam_out or the state%s");
+  }
+  for (int q = 0; q < 2 + o->n_passive; ++q)
+    if (!lss[q]) return fail(-2, "lex_rk3_step_vjp_taped_ext_sgs: null scalar SGS adjoint pointer%s");
+  Dims d = mkdims(g);
+  VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
+  if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_vjp_taped_ext_sgs: tape too small%s");
+  if (adj_bytes < L.total - L.off_lXA) return fail(-3, "lex_rk3_step_vjp_taped_ext_sgs: adjoint workspace too small%s");
+  lex_step_opts_t o2 = *o;
+  o2.turb = 2;
+  hipStream_t st = S(stream);
+  int rc = rk3_step_vjp(d, o2, *b, state, lam_out, lam_in, nullptr, (char*)tape, (char*)adj, L, st, false);
+  if (rc > 0) return fail(rc, "lex_rk3_step_vjp_taped_ext_sgs: HIP error %s", hipGetErrorString((hipError_t)rc));
+  // the adjoints of the frozen SGS tendencies sit in the adjoint workspace
+  const char* a = (const char*)adj;
+  auto cp = [&](size_t off, float* dst, size_t n) {
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (long)n,
+                       (const float*)(a + (off - L.off_lXA)), dst);
+  };
+  cp(L.off_lsu, lsu, L.iu_);
+  cp(L.off_lsv, lsv, L.iv_);
+  cp(L.off_lsw, lsw, L.iw_);
+  for (int q = 0; q < 2 + o->n_passive; ++q) cp(L.off_lss + q * al256(L.ic_ * 4), lss[q], L.ic_);
+  return check_launch("lex_rk3_step_vjp_taped_ext_sgs");
+}
+
 // ------------------------------------------------------------------ X4 conv3d (BF16 MFMA)
 }  // extern "C"
 namespace {
@@ -840,6 +913,31 @@ int lex_sgs_apply(const lex_grid_t* g, const float* y, const float* out_scale, f
   hipLaunchKernelGGL(k_sgs_apply, grd(d.nz + 1, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, y, out_scale[0],
                      out_scale[1], out_scale[2], out_scale[3], out_scale[4], su, sv, sw, sth, sqv);
   return check_launch("lex_sgs_apply");
+}
+
+int lex_sgs_apply_vjp(const lex_grid_t* g, const float* lsu, const float* lsv, const float* lsw, const float* lsth,
+                      const float* lsqv, const float* out_scale, float* ly, void* stream) {
+  REQGRID(g);
+  REQ(lsu); REQ(lsv); REQ(lsw); REQ(lsth); REQ(lsqv); REQ(out_scale); REQ(ly);
+  Dims d = mkdims(g);
+  hipLaunchKernelGGL(k_sgs_apply_vjp, grd(d.nz, d.ny, d.nx), blk(), 0, S(stream), d, lsu, lsv, lsw, lsth, lsqv,
+                     out_scale[0], out_scale[1], out_scale[2], out_scale[3], out_scale[4], ly);
+  return check_launch("lex_sgs_apply_vjp");
+}
+
+int lex_sgs_features_vjp(const lex_grid_t* g, const float* lfeat, const float* scale, float* lu, float* lv, float* lw,
+                         float* lth, float* lqv, void* stream) {
+  REQGRID(g);
+  REQ(lfeat); REQ(scale); REQ(lu); REQ(lv); REQ(lw); REQ(lth); REQ(lqv);
+  Dims d = mkdims(g);
+  FeatNorm nrm;
+  for (int c = 0; c < 11; ++c) {
+    nrm.shift[c] = 0.f;
+    nrm.scale[c] = scale[c];
+  }
+  hipLaunchKernelGGL(k_sgs_features_vjp, grd(d.nz, d.ny, d.nx), blk(), 0, S(stream), d, lfeat, nrm, lu, lv, lw, lth,
+                     lqv);
+  return check_launch("lex_sgs_features_vjp");
 }
 
 int lex_coarsen(const lex_grid_t* g, int mode, int fx, int fy, int fz, const float* a, float* out, void* stream) {

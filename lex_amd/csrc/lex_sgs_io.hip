@@ -88,6 +88,112 @@ __global__ void k_sgs_apply(Dims d, const float* __restrict__ y, float os0, floa
   }
 }
 
+// ---------------------------------------------------------------- reverses (training
+// the net through the solver)
+// k_sgs_apply_vjp: adjoints of sgs_u, sgs_v, sgs_w, sgs_theta, sgs_qv -> adjoint of the
+// net output ly (nx, ny, nz, 5), written (not accumulated).  Gather per cell: cell i is
+// read by u faces i and i+1, and (periodic) by face 0 when i = nx-1 and face nx when
+// i = 0; cell k by w levels k and k+1 inside 1..nz-1.
+__global__ void k_sgs_apply_vjp(Dims d, const float* __restrict__ lsu, const float* __restrict__ lsv,
+                                const float* __restrict__ lsw, const float* __restrict__ lsth,
+                                const float* __restrict__ lsqv, float os0, float os1, float os2, float os3, float os4,
+                                float* __restrict__ ly) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  auto U = [&](int f) { return lsu[((long)f * ny + j) * nz + k]; };
+  auto V = [&](int f) { return lsv[((long)i * (ny + 1) + f) * nz + k]; };
+  float gu = U(i) + U(i + 1);
+  if (i == nx - 1) gu += U(0);
+  if (i == 0) gu += U(nx);
+  float gv = V(j) + V(j + 1);
+  if (j == ny - 1) gv += V(0);
+  if (j == 0) gv += V(ny);
+  const long ow = ((long)i * ny + j) * (nz + 1);
+  float gw = 0.f;
+  if (k >= 1) gw += lsw[ow + k];
+  if (k + 1 <= nz - 1) gw += lsw[ow + k + 1];
+  const long o = ((long)i * ny + j) * nz + k;
+  float* out = ly + o * 5;
+  out[0] = os0 * 0.5f * gu;
+  out[1] = os1 * 0.5f * gv;
+  out[2] = os2 * 0.5f * gw;
+  out[3] = os3 * lsth[o];
+  out[4] = os4 * lsqv[o];
+}
+
+// reverses of the strain-rate stencils of k_sgs_features (scatter with atomics)
+__device__ __forceinline__ void sgs_e12_vjp(const Dims& d, float* lu, float* lv, int F, int G, int K, float g) {
+  unsafeAtomicAdd(lu + iu(d, F, G, K), 0.5f * g * d.idy);
+  unsafeAtomicAdd(lu + iu(d, F, G - 1, K), -0.5f * g * d.idy);
+  unsafeAtomicAdd(lv + iv(d, F, G, K), 0.5f * g * d.idx);
+  unsafeAtomicAdd(lv + iv(d, F - 1, G, K), -0.5f * g * d.idx);
+}
+__device__ __forceinline__ void sgs_e13_vjp(const Dims& d, float* lu, float* lw, int F, int J, int l, float g) {
+  unsafeAtomicAdd(lu + iu(d, F, J, l + 1), 0.5f * g * d.idz);
+  unsafeAtomicAdd(lu + iu(d, F, J, l), -0.5f * g * d.idz);
+  unsafeAtomicAdd(lw + iw(d, F, J, l + 1), 0.5f * g * d.idx);
+  unsafeAtomicAdd(lw + iw(d, F - 1, J, l + 1), -0.5f * g * d.idx);
+}
+__device__ __forceinline__ void sgs_e23_vjp(const Dims& d, float* lv, float* lw, int I, int G, int l, float g) {
+  unsafeAtomicAdd(lv + iv(d, I, G, l + 1), 0.5f * g * d.idz);
+  unsafeAtomicAdd(lv + iv(d, I, G, l), -0.5f * g * d.idz);
+  unsafeAtomicAdd(lw + iw(d, I, G, l + 1), 0.5f * g * d.idy);
+  unsafeAtomicAdd(lw + iw(d, I, G - 1, l + 1), -0.5f * g * d.idy);
+}
+
+// k_sgs_features_vjp: adjoint of the normalised features lf (nx, ny, nz, 11) -> padded
+// adjoints of u, v, w, theta, qv (accumulated, ghost lines included: fold afterwards).
+__global__ void k_sgs_features_vjp(Dims d, const float* __restrict__ lf, FeatNorm nrm, float* __restrict__ lu,
+                                   float* __restrict__ lv, float* __restrict__ lw, float* __restrict__ lth,
+                                   float* __restrict__ lqv) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  const int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
+  const long o = (((long)i * d.ny + j) * d.nz + k) * 11;
+  float g[11];
+#pragma unroll
+  for (int c = 0; c < 11; ++c) g[c] = lf[o + c] * nrm.scale[c];
+  unsafeAtomicAdd(lu + iu(d, I, J, K), 0.5f * g[0] - g[5] * d.idx);
+  unsafeAtomicAdd(lu + iu(d, I + 1, J, K), 0.5f * g[0] + g[5] * d.idx);
+  unsafeAtomicAdd(lv + iv(d, I, J, K), 0.5f * g[1] - g[6] * d.idy);
+  unsafeAtomicAdd(lv + iv(d, I, J + 1, K), 0.5f * g[1] + g[6] * d.idy);
+  unsafeAtomicAdd(lw + iw(d, I, J, K), 0.5f * g[2] - g[7] * d.idz);
+  unsafeAtomicAdd(lw + iw(d, I, J, K + 1), 0.5f * g[2] + g[7] * d.idz);
+  unsafeAtomicAdd(lth + ic(d, I, J, K), g[3]);
+  unsafeAtomicAdd(lqv + ic(d, I, J, K), g[4]);
+  const float h12 = 0.25f * g[8];
+  sgs_e12_vjp(d, lu, lv, I, J, K, h12);
+  sgs_e12_vjp(d, lu, lv, I + 1, J + 1, K, h12);
+  sgs_e12_vjp(d, lu, lv, I, J + 1, K, h12);
+  sgs_e12_vjp(d, lu, lv, I + 1, J, K, h12);
+  if (k == 0 || k == nz - 1) {
+    const int l0 = (k == 0) ? 1 : nz - 1, dl = (k == 0) ? 1 : -1;
+    const float cf[3] = {1.875f, -1.25f, 0.375f};
+    for (int t = 0; t < 3; ++t) {
+      const int l = l0 + t * dl;
+      sgs_e13_vjp(d, lu, lw, I + 1, J, l, 0.5f * cf[t] * g[9]);
+      sgs_e13_vjp(d, lu, lw, I, J, l, 0.5f * cf[t] * g[9]);
+      sgs_e23_vjp(d, lv, lw, I, J + 1, l, 0.5f * cf[t] * g[10]);
+      sgs_e23_vjp(d, lv, lw, I, J, l, 0.5f * cf[t] * g[10]);
+    }
+  } else {
+    const float a = 0.25f * g[9], b = 0.25f * g[10];
+    sgs_e13_vjp(d, lu, lw, I, J, k, a);
+    sgs_e13_vjp(d, lu, lw, I + 1, J, k + 1, a);
+    sgs_e13_vjp(d, lu, lw, I, J, k + 1, a);
+    sgs_e13_vjp(d, lu, lw, I + 1, J, k, a);
+    sgs_e23_vjp(d, lv, lw, I, J, k, b);
+    sgs_e23_vjp(d, lv, lw, I, J + 1, k + 1, b);
+    sgs_e23_vjp(d, lv, lw, I, J, k + 1, b);
+    sgs_e23_vjp(d, lv, lw, I, J + 1, k, b);
+  }
+}
+
 // Coarsening of a padded fine field by (fx, fy, fz) into an interior coarse field.
 // mode 0: cell block mean of a scalar -> (nxc, nyc, nzc);
 // mode 1: u, mean over the fy*fz fine faces of each coarse face -> (nxc+1, nyc, nzc);

@@ -845,11 +845,9 @@ def rk_sub_step(stage, xi0, xi, base, grid, opts, dt_stage, frozen=None):
     dt_ = rho0.dtype.type
     nx, ny, nz = grid.nx, grid.ny, grid.nz
     zeros2 = np.zeros((nx, ny), dtype=rho0.dtype)
-    if stage == 0:
+    if stage == 0 and frozen is None:
         if opts.turb_opt == 1:
             frozen = smagorinsky(rho0, u, v, w, th, qv, (th, qv) + pas, grid, opts.cs, opts.prandtl)
-        else:
-            frozen = None
     sgs = frozen
     buoy, b8w = calculate_buoyancy(theta0, th, qv0, qv)
     fdiv = get_divergence(rho0, u, v, w, grid)
@@ -902,6 +900,16 @@ def ws_rk3_big_step(xi0, base, grid, opts):
     xi1, frozen, i0 = rk_sub_step(0, xi0, xi0, base, grid, opts, opts.dt / 3.0)
     xi2, _, i1 = rk_sub_step(1, xi0, xi1, base, grid, opts, opts.dt / 2.0, frozen)
     xi3, _, i2 = rk_sub_step(2, xi0, xi2, base, grid, opts, opts.dt, frozen)
+    return xi3, (i0, i1, i2)
+
+
+def ws_rk3_big_step_ext_sgs(xi0, base, grid, opts, sgs):
+    """ws_rk3_big_step with caller-supplied SGS tendencies (sgs_u, sgs_v, sgs_w, then one
+    per scalar) frozen over the step in place of the stage-0 Smagorinsky ones: the path
+    of the learned SGS net (its outputs replace compute_smag's, one_step_integration.py:57-76)."""
+    xi1, _, i0 = rk_sub_step(0, xi0, xi0, base, grid, opts, opts.dt / 3.0, tuple(sgs))
+    xi2, _, i1 = rk_sub_step(1, xi0, xi1, base, grid, opts, opts.dt / 2.0, tuple(sgs))
+    xi3, _, i2 = rk_sub_step(2, xi0, xi2, base, grid, opts, opts.dt, tuple(sgs))
     return xi3, (i0, i1, i2)
 
 
