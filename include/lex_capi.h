@@ -104,6 +104,14 @@ int lex_acoustic_substep(const lex_grid_t* g, const float* rho0_col, const float
                          const float* v, const float* w, const float* pip, const float* pip_star, const float* rhs_u,
                          const float* rhs_v, const float* rhs_w, const float* theta_rho, float dt_sound, float* u_out,
                          float* v_out, float* w_out, float* pip_out, void* stream);
+/* one pass of the loop in sound_implicit_rk_step0/1/2 (one_step_integration.py:590-633,
+ * solver_opt 3): the same arguments plus the off-centring alpha (namelist alpha = 0.6);
+ * w and pi' implicit in the vertical, one tridiagonal solve per column. */
+int lex_acoustic_substep_implicit(const lex_grid_t* g, const float* rho0_col, const float* pi0_col, const float* u,
+                                  const float* v, const float* w, const float* pip, const float* pip_star,
+                                  const float* rhs_u, const float* rhs_v, const float* rhs_w, const float* theta_rho,
+                                  float dt_sound, float alpha, float* u_out, float* v_out, float* w_out,
+                                  float* pip_out, void* stream);
 /* update_theta_euler / update_qv_euler (one_step_integration.py:450-465) plus the SGS
  * add-in (:158-162): out = pad(s0 + (adv+heating)*dt) + pad(sgs*dt).  heating, sgs,
  * sfc_flux and tend_out may be NULL. */
@@ -154,6 +162,8 @@ typedef struct lex_step_opts {
   float prandtl;     /* turbulence.py:432 */
   int n_passive;     /* passive scalars advected and diffused like qv (qc, qr) */
   int impl;          /* 0 = fused stage kernels (default), 1 = one kernel per reference routine */
+  int solver;        /* acoustic substeps: 0 or 2 = forward-backward (solver_opt 2), 3 = vertically
+                        implicit w / pi' (solver_opt 3, one_step_integration.py:559-873) */
 } lex_step_opts_t;
 
 typedef struct lex_base_cols {

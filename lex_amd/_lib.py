@@ -29,7 +29,7 @@ class Grid(C.Structure):
 
 class StepOpts(C.Structure):
     _fields_ = [("dt", C.c_float), ("n_sound", C.c_int), ("turb", C.c_int), ("cs", C.c_float),
-                ("prandtl", C.c_float), ("n_passive", C.c_int), ("impl", C.c_int)]
+                ("prandtl", C.c_float), ("n_passive", C.c_int), ("impl", C.c_int), ("solver", C.c_int)]
 
 
 class BaseCols(C.Structure):
@@ -129,6 +129,7 @@ _SIGS = {
     "lex_coarsen": [_G, _I, _I, _I, _I, _P, _P, _P],
     "lex_rk3_step_ext_sgs": [_G, C.POINTER(StepOpts), C.POINTER(BaseCols), C.POINTER(_P), _P, _P, _P,
                              C.POINTER(_P), _P, C.c_size_t, _P],
+    "lex_acoustic_substep_implicit": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _F, _P, _P, _P, _P, _P],
     # training the net through the solver
     "lex_sgs_apply_vjp": [_G, _P, _P, _P, _P, _P, _P, _P, _P],
     "lex_sgs_features_vjp": [_G, _P, _P, _P, _P, _P, _P, _P, _P],

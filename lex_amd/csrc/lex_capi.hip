@@ -295,6 +295,22 @@ int lex_acoustic_substep(const lex_grid_t* g, const float* r, const float* pi0, 
   return check_launch("lex_acoustic_substep");
 }
 
+int lex_acoustic_substep_implicit(const lex_grid_t* g, const float* r, const float* pi0, const float* u,
+                                  const float* v, const float* w, const float* p, const float* ps, const float* ru,
+                                  const float* rv, const float* rw, const float* tr, float dts, float alpha, float* uo,
+                                  float* vo, float* wo, float* po, void* stream) {
+  REQGRID(g);
+  REQ(r); REQ(pi0); REQ(u); REQ(v); REQ(w); REQ(p); REQ(ps); REQ(ru); REQ(rv); REQ(rw); REQ(tr);
+  REQ(uo); REQ(vo); REQ(wo); REQ(po);
+  if (uo == u || vo == v || wo == w || po == p)
+    return fail(-1, "lex_acoustic_substep_implicit: outputs must not alias inputs%s");
+  if (g->nz < 2 || g->nz > 256) return fail(-1, "lex_acoustic_substep_implicit: nz must be 2..256%s");
+  Dims d = mkdims(g);
+  hipLaunchKernelGGL(k_acoustic_vi, dim3(d.ny, d.nx, 1), dim3(acoustic_vi_lanes(d)), acoustic_vi_lds(d), S(stream),
+                     d, r, pi0, u, v, w, p, ps, ru, rv, rw, tr, uo, vo, wo, po, dts, alpha);
+  return check_launch("lex_acoustic_substep_implicit");
+}
+
 int lex_scalar_update(const lex_grid_t* g, const float* r, const float* s0, const float* s, const float* u,
                       const float* v, const float* w, const float* fd, const float* sfc, const float* heat,
                       const float* sgs, float dt, float weps, float* out, float* tend, void* stream) {
@@ -457,6 +473,8 @@ int lex_rk3_step(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_c
   Rk3Layout L = rk3_layout(d, o->n_passive);
   if (ws_bytes < L.total) return fail(-3, "lex_rk3_step: workspace too small%s");
   if (o->impl < 0 || o->impl > 2) return fail(-1, "lex_rk3_step: impl must be 0, 1 or 2%s");
+  if (o->solver != 0 && o->solver != 2 && o->solver != 3) return fail(-1, "lex_rk3_step: solver must be 0, 2 or 3%s");
+  if (o->solver == 3 && (g->nz < 2 || g->nz > 256)) return fail(-1, "lex_rk3_step: solver 3 needs nz 2..256%s");
   if (o->impl != 1 && (64 * ((g->nz + 63) / 64)) * kFTY > 1024)
     return fail(-1, "lex_rk3_step: the fused kernels need nz <= 256 (use impl 1)%s");
   int rc = rk3_step(d, *o, *b, state, (char*)ws, L, S(stream));
@@ -645,6 +663,7 @@ int lex_rk3_step_vjp(const lex_grid_t* g, const lex_step_opts_t* o, const lex_ba
     if (lam_in[q] == lam_out[q] || lam_in[q] == state[q])
       return fail(-1, "lex_rk3_step_vjp: lam_in must not alias lam_out or the state%s");
   }
+  if (o->solver == 3) return fail(-1, "lex_rk3_step_vjp: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (ws_bytes < L.total) return fail(-3, "lex_rk3_step_vjp: workspace too small%s");
@@ -677,6 +696,7 @@ int lex_rk3_step_tape(const lex_grid_t* g, const lex_step_opts_t* o, const lex_b
     if (!state[q] || !state_out[q]) return fail(-2, "lex_rk3_step_tape: null field pointer%s");
     if (state[q] == state_out[q]) return fail(-1, "lex_rk3_step_tape: state_out must not alias state%s");
   }
+  if (o->solver == 3) return fail(-1, "lex_rk3_step_tape: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_tape: tape too small%s");
@@ -697,6 +717,7 @@ int lex_rk3_step_vjp_taped(const lex_grid_t* g, const lex_step_opts_t* o, const 
     if (lam_in[q] == lam_out[q] || lam_in[q] == state[q])
       return fail(-1, "lex_rk3_step_vjp_taped: lam_in must not alias lam_out or the state%s");
   }
+  if (o->solver == 3) return fail(-1, "lex_rk3_step_vjp_taped: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_vjp_taped: tape too small%s");
@@ -722,6 +743,7 @@ int lex_rk3_step_tape_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o, con
   }
   for (int q = 0; q < 2 + o->n_passive; ++q)
     if (!ss[q]) return fail(-2, "lex_rk3_step_tape_ext_sgs: null scalar SGS pointer%s");
+  if (o->solver == 3) return fail(-1, "lex_rk3_step_tape_ext_sgs: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_tape_ext_sgs: tape too small%s");
@@ -757,6 +779,7 @@ int lex_rk3_step_vjp_taped_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o
   }
   for (int q = 0; q < 2 + o->n_passive; ++q)
     if (!lss[q]) return fail(-2, "lex_rk3_step_vjp_taped_ext_sgs: null scalar SGS adjoint pointer%s");
+  if (o->solver == 3) return fail(-1, "lex_rk3_step_vjp_taped_ext_sgs: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_vjp_taped_ext_sgs: tape too small%s");

@@ -9,6 +9,7 @@
 #include "lex_common.hpp"
 #include "lex_prof.hpp"
 #include "lex_fused.hip"
+#include "lex_acoustic_vi.hip"
 
 namespace lex {
 
@@ -289,7 +290,11 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
         int pp = (in[0] == P[0][0]) ? 1 : 0;
         for (int c = 0; c < 4; ++c) out[c] = P[pp][c];
       }
-      if (o.impl == 2) {
+      if (o.solver == 3) {
+        LEX_LAUNCH(KID_ACOUSTIC_VI, k_acoustic_vi, dim3(d.ny, d.nx, 1), dim3(acoustic_vi_lanes(d)), acoustic_vi_lds(d),
+                   st, d, rc, b.pi0, in[0], in[1], in[2], in[3], src[4], ru, rv, rw, tr, out[0], out[1], out[2], out[3],
+                   dts, kAlphaImplicit);
+      } else if (o.impl == 2) {
         dim3 ag((d.nx + kAX - 1) / kAX, (d.ny + 3) / 4, (d.nz + 63) / 64);
         LEX_LAUNCH(KID_ACOUSTIC, k_acoustic_m, ag, dim3(64, 4, 1), 0, st, d, rc, b.pi0, in[0], in[1], in[2], in[3],
                    src[4], ru, rv, rw, tr, out[0], out[1], out[2], out[3], dts);

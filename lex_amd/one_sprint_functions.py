@@ -30,18 +30,18 @@ def _workspace(g, n_passive, device):
 
 
 def fused_step(phys_state, base_state, grid, model_opt, dt=None, n_sound=None, cs=None, prandtl=None, impl=0):
-    """Advance a solver_opt-2 state by one WS-RK3 step in place on the device.
-
-    impl 0 runs the fused stage kernels, impl 1 one kernel per reference routine.
+    """Advance a compressible (solver_opt 2 or 3) state by one WS-RK3 step in place on the
+    device.  impl 0 runs the fused stage kernels, impl 1 one kernel per reference routine;
+    solver_opt 3 takes the vertically implicit acoustic substeps.
     """
     solver_opt, cor_opt, turb_opt = one._check_opts(model_opt)
-    if solver_opt != 2 or cor_opt != 0:
-        raise NotImplementedError("the fused step covers solver_opt 2 without Coriolis")
+    if solver_opt not in (2, 3) or cor_opt != 0:
+        raise NotImplementedError("the fused step covers solver_opt 2 and 3 without Coriolis")
     rho0_theta0, rho0, theta0, pi0, qv0 = base_state[:5]
     n_passive = len(phys_state) - 6
     opts = _lib.StepOpts(float(nl.dt if dt is None else dt), int(nl.n_sound if n_sound is None else n_sound),
                          int(turb_opt), float(nl.cs if cs is None else cs),
-                         float(nl.prandtl if prandtl is None else prandtl), n_passive, int(impl))
+                         float(nl.prandtl if prandtl is None else prandtl), n_passive, int(impl), int(solver_opt))
     cols = [column(a) for a in (rho0, theta0, qv0, pi0, rho0_theta0)]
     bc = _lib.BaseCols(*[c.data_ptr() for c in cols])
     ptrs = (ctypes.c_void_p * len(phys_state))(*[_lib.ptr(a) for a in phys_state])
@@ -56,7 +56,7 @@ def ws_rk3_big_step(phys_state, base_state, grids, model_opt):
     Returns (phys_state, sfc_others) with new tensors, as the reference does.
     """
     solver_opt = model_opt[0]
-    if solver_opt == 2 and model_opt[3] == 0:
+    if solver_opt in (2, 3) and model_opt[3] == 0:
         g = grid_from(base_state[1], *grids[3:6])
         new = tuple(a.clone() for a in phys_state)
         fused_step(new, base_state, g, model_opt)

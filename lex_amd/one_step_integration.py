@@ -4,8 +4,8 @@
 State tuples are (theta, u, v, w, pip, qv) as in setup_lex.py:66, optionally followed by
 passive scalars (qc, qr) that are advected and SGS-diffused like qv.  Options that are
 off in every LEX configuration on this path (surface fluxes, Rayleigh damping,
-Newtonian heating, the pi' constant correction, solver_opt 3) raise
-NotImplementedError instead of being silently ignored.
+Newtonian heating, the pi' constant correction) raise NotImplementedError instead of
+being silently ignored.
 """
 from __future__ import annotations
 
@@ -158,17 +158,23 @@ def solve_pres_eqn(pip_prev, rho0_theta0, pi0, rtt, adv4u, adv4v, adv4w, fu, fv,
 
 
 # --------------------------------------------------------------- S1 acoustic
+ALPHA = 0.6  # off-centring of the vertically implicit acoustic step (namelist_n_constants.py:14)
+
+
 def _sound(u_now, v_now, w_now, pi0, pip_now, pip_star, rhs_u, rhs_v, rhs_w, rhs_pi, theta_rho, rho0, rk_n_sound,
-           dt_sound, g):
+           dt_sound, g, implicit=False):
     if not (isinstance(rhs_pi, (int, float)) and rhs_pi == 0.0):
         raise NotImplementedError("rhs_pi is a zero placeholder in the reference (one_step_integration.py:134)")
     rc, p0c = column(rho0), column(pi0)
     cur = (u_now, v_now, w_now, pip_now)
     for _ in range(int(rk_n_sound)):
         nxt = tuple(torch.empty_like(a) for a in cur)
-        _lib.call("lex_acoustic_substep", g, _lib.ptr(rc), _lib.ptr(p0c), *[_lib.ptr(a) for a in cur],
-                  _lib.ptr(pip_star), _lib.ptr(rhs_u), _lib.ptr(rhs_v), _lib.ptr(rhs_w), _lib.ptr(theta_rho),
-                  float(dt_sound), *[_lib.ptr(a) for a in nxt], _lib.stream())
+        args = [g, _lib.ptr(rc), _lib.ptr(p0c), *[_lib.ptr(a) for a in cur], _lib.ptr(pip_star), _lib.ptr(rhs_u),
+                _lib.ptr(rhs_v), _lib.ptr(rhs_w), _lib.ptr(theta_rho), float(dt_sound)]
+        if implicit:
+            _lib.call("lex_acoustic_substep_implicit", *args, float(ALPHA), *[_lib.ptr(a) for a in nxt], _lib.stream())
+        else:
+            _lib.call("lex_acoustic_substep", *args, *[_lib.ptr(a) for a in nxt], _lib.stream())
         cur = nxt
     return cur
 
@@ -185,6 +191,18 @@ sound_rk_step1 = sound_rk_step0   # identical bodies in the reference (:638-675)
 sound_rk_step2 = sound_rk_step0   # (:757-794)
 
 
+def sound_implicit_rk_step0(u_now, v_now, w_now, pi0, pip_now, pip_star, rhs_u, rhs_v, rhs_w, rhs_pi, theta_rho, rho0,
+                            rk_n_sound, dt_sound, x3d, y3d, z3d, x3d4u, y3d4v, z3d4w, cc1=None, cc2=None):
+    """Vertically implicit acoustic substeps, solver_opt 3 (one_step_integration.py:559-635)."""
+    g = grid_from(pip_now, x3d4u, y3d4v, z3d4w)
+    return _sound(u_now, v_now, w_now, pi0, pip_now, pip_star, rhs_u, rhs_v, rhs_w, rhs_pi, theta_rho, rho0,
+                  rk_n_sound, dt_sound, g, implicit=True)
+
+
+sound_implicit_rk_step1 = sound_implicit_rk_step0   # identical bodies in the reference (:678-754)
+sound_implicit_rk_step2 = sound_implicit_rk_step0   # (:797-873)
+
+
 def _theta_rho(theta, qv):
     out = torch.empty_like(theta)
     _lib.call("lex_theta_rho", theta.numel(), _lib.ptr(theta), _lib.ptr(qv), _lib.ptr(out), _lib.stream())
@@ -194,8 +212,8 @@ def _theta_rho(theta, qv):
 # ------------------------------------------------------------------ R1 stages
 def _check_opts(model_opt):
     solver_opt, damp_opt, rad_opt, cor_opt, sfc_opt, pic_opt, turb_opt = model_opt
-    if solver_opt not in (1, 2):
-        raise NotImplementedError("solver_opt 3 (vertically implicit acoustics) is not on this path")
+    if solver_opt not in (1, 2, 3):
+        raise NotImplementedError(f"solver_opt {solver_opt} does not exist in the reference")
     for name, flag in (("damp_opt", damp_opt), ("rad_opt", rad_opt), ("sfc_opt", sfc_opt), ("pic_opt", pic_opt)):
         if flag:
             raise NotImplementedError(f"{name} is off in every configuration on this path")
@@ -265,7 +283,7 @@ def _stage(stage, phys_state_now, phys_state, base_state, grids, model_opt, dt, 
             rhs_w = rhs_w + sgs_tend[2]
         n, dts = _n_sound(stage, dt)
         un, vn, wn, pip_n = _sound(u0, v0, w0, pi0, pip0, pip, rhs_u.contiguous(), rhs_v.contiguous(),
-                                   rhs_w.contiguous(), 0.0, trho, rho0, n, dts, g)
+                                   rhs_w.contiguous(), 0.0, trho, rho0, n, dts, g, implicit=solver_opt == 3)
     sg_th = sgs_tend[3] if sgs_tend is not None else None
     sg_qv = sgs_tend[4] if sgs_tend is not None else None
     thn, dth = _scalar_update(g, rho0, th0, th, u, v, w, fd, None, None, sg_th, dt, WEPS_THETA)

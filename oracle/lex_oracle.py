@@ -796,6 +796,79 @@ def sound_rk_step(u_now, v_now, w_now, pi0, pip_now, pip_star, rhs_u, rhs_v, rhs
     return u_now, v_now, w_now, pip_now
 
 
+ALPHA_IMPLICIT = 0.6  # off-centring of the vertically implicit acoustic step (namelist_n_constants.py:14)
+
+
+def tridiagonal_solve(a, b, c, d):
+    """Solve a_i x_{i-1} + b_i x_i + c_i x_{i+1} = d_i along the last axis (a_0, c_{n-1}
+    unused): the Thomas algorithm, no pivoting.  Restates jax.lax.linalg.tridiagonal_solve
+    as called at one_step_integration.py:614, :733, :852 (third-party; the systems there are
+    diagonally dominant)."""
+    n = d.shape[-1]
+    cp = np.empty_like(d)
+    dp = np.empty_like(d)
+    cp[..., 0] = c[..., 0] / b[..., 0]
+    dp[..., 0] = d[..., 0] / b[..., 0]
+    for i in range(1, n):
+        m = b[..., i] - a[..., i] * cp[..., i - 1]
+        cp[..., i] = c[..., i] / m
+        dp[..., i] = (d[..., i] - a[..., i] * dp[..., i - 1]) / m
+    x = np.empty_like(d)
+    x[..., -1] = dp[..., -1]
+    for i in range(n - 2, -1, -1):
+        x[..., i] = dp[..., i] - cp[..., i] * x[..., i + 1]
+    return x
+
+
+def implicit_w_coefficients(pi0, pip_star, theta_rho, dt_sound, grid, alpha=ALPHA_IMPLICIT):
+    """Tridiagonal coefficients (Ak, Bk, Ck) of the w equation at levels 1..nz-1 and the
+    rhs prefactor (one_step_integration.py:570-588), uniform vertical spacing."""
+    I, J = slice(NGX, -NGX), slice(NGY, -NGY)
+    dz = grid.dz
+    pa = (pi0 + pip_star)[I, J, :]
+    thr8w = 0.5 * (theta_rho[I, J, 2:-1] + theta_rho[I, J, 1:-2])
+    lhs = (alpha * dt_sound) ** 2 * Cp * thr8w / dz
+    dpdz = (pa[:, :, 1:] - pa[:, :, :-1]) / dz
+    ak = -lhs * (Rd / Cv * pa[:, :, 1:-2] / dz - dpdz[:, :, 0:-2] / 2.0)
+    ck = -lhs * (Rd / Cv * pa[:, :, 2:-1] / dz + dpdz[:, :, 2:] / 2.0)
+    bk = 1.0 + lhs * Rd / Cv * (pa[:, :, 2:-1] / dz + pa[:, :, 1:-2] / dz)
+    ak[:, :, 0] = 0.0
+    ck[:, :, -1] = 0.0
+    return ak, bk, ck, alpha * dt_sound ** 2 * Cp * thr8w
+
+
+def sound_implicit_rk_step(u_now, v_now, w_now, pi0, pip_now, pip_star, rhs_u, rhs_v, rhs_w,
+                           theta_rho, rho0, n, dt_sound, grid, alpha=ALPHA_IMPLICIT):
+    """Vertically implicit acoustic substeps, solver_opt 3 (sound_implicit_rk_step0/1/2,
+    one_step_integration.py:559-635, :678-754, :797-873): u, v forward; w and pi' coupled
+    implicitly in the vertical with off-centring alpha (one tridiagonal solve per column);
+    rhs_pi = 0 as in the reference stages (:133)."""
+    I, J = slice(NGX, -NGX), slice(NGY, -NGY)
+    piadv = pi0 + pip_star
+    ak, bk, ck, rhs3 = implicit_w_coefficients(pi0, pip_star, theta_rho, dt_sound, grid, alpha)
+    beta = 1.0 - alpha
+    c = -Rd / Cv
+    for _ in range(n):
+        pu, pv, pw = pressure_gradient_force(pip_now, theta_rho, grid)
+        un = padding3u_array(u_now[I, J, NGZ:-NGZ] + (pu + rhs_u) * dt_sound)
+        vn = padding3v_array(v_now[I, J, NGZ:-NGZ] + (pv + rhs_v) * dt_sound)
+        wb = beta * w_now
+        div = get_divergence2(un, vn, wb, grid)
+        ptm = c * (_interior(piadv) * _interior(div)) + advection_pi(rho0, pi0, pip_star, un, vn, wb, grid)
+        dptm = (ptm[:, :, 1:] - ptm[:, :, :-1]) / grid.dz
+        dk = w_now[I, J, 2:-2] + dt_sound * (rhs_w[:, :, 1:-1] + pw[:, :, 1:-1]) - rhs3 * dptm
+        wi = tridiagonal_solve(ak, bk, ck, dk)
+        z0 = np.zeros(wi.shape[:2] + (1,), dtype=wi.dtype)
+        wn = padding3_array(np.concatenate([z0, wi, z0], axis=2))
+        wmix = alpha * wn + beta * w_now
+        div = get_divergence2(un, vn, wmix, grid)
+        apt = advection_pi(rho0, pi0, pip_star, un, vn, wmix, grid)
+        dpdt = c * (_interior(piadv) * _interior(div)) + apt
+        pn = padding3_array(_interior(pip_now) + 1.1 * dpdt * dt_sound)
+        u_now, v_now, w_now, pip_now = un, vn, wn, pn
+    return u_now, v_now, w_now, pip_now
+
+
 def update_momentum_eqn_euler(u0, v0, w0, pip, theta_rho, adv4u, adv4v, adv4w, fu, fv, b8w, dt, grid):
     """(one_step_integration.py:876-891)."""
     pu, pv, pw = pressure_gradient_force(pip, theta_rho, grid)
@@ -881,8 +954,8 @@ def rk_sub_step(stage, xi0, xi, base, grid, opts, dt_stage, frozen=None):
             rhs_w = rhs_w + sgs[2]
         trho = theta_rho_of(th, qv)
         n, dts = _n_acoustic(stage, dt_stage, opts)
-        un, vn, wn, pip_n = sound_rk_step(u0, v0, w0, pi0, pip0, pip, rhs_u, rhs_v, rhs_w,
-                                          trho, rho0, n, dts, grid)
+        step = sound_implicit_rk_step if opts.solver_opt == 3 else sound_rk_step
+        un, vn, wn, pip_n = step(u0, v0, w0, pi0, pip0, pip, rhs_u, rhs_v, rhs_w, trho, rho0, n, dts, grid)
     thn, _ = update_scalar_euler(rho0, th0, th, u, v, w, fdiv, zeros2, 0.0, WEPS_THETA, dt_stage, grid)
     qvn, _ = update_scalar_euler(rho0, qv0_now, qv, u, v, w, fdiv, zeros2, 0.0, WEPS_QV, dt_stage, grid)
     pasn = tuple(update_scalar_euler(rho0, a0, a, u, v, w, fdiv, zeros2, 0.0, WEPS_QV, dt_stage, grid)[0]

@@ -220,6 +220,29 @@ def test_acoustic_substeps(case):
         check_fp32(from_dev(a) - a0, b - a0, c - a0, f"acoustic {name}")
 
 
+def test_implicit_acoustic_substeps(case):
+    """solver_opt 3: vertically implicit substeps (PCR tridiagonal solve per column on the
+    GPU, Thomas in the oracle) against the FP64/FP32 oracle."""
+    from lex_amd import one_step_integration as one
+    gr, g = case["gr"], case["grids"]
+    p32, p64, b32, b64, dv, bd = case["p32"], case["p64"], case["b32"], case["b64"], case["dev"], case["bdev"]
+    rng = np.random.default_rng(12)
+    ru = f32(rng.normal(0, 1e-2, (gr.nx + 1, gr.ny, gr.nz)))
+    rv = f32(rng.normal(0, 1e-2, (gr.nx, gr.ny + 1, gr.nz)))
+    rw = f32(rng.normal(0, 1e-2, (gr.nx, gr.ny, gr.nz + 1)))
+    tr = f32(o.theta_rho_of(p32[0], p32[5]))
+    pstar = f32(p32[4] * 0.5)
+    n, dts = 3, 0.2
+    got = one.sound_implicit_rk_step1(dv[1], dv[2], dv[3], bd[3], dv[4], to_dev(pstar), to_dev(ru), to_dev(rv),
+                                      to_dev(rw), 0.0, to_dev(tr), bd[1], n, dts, *g[:8])
+    r64 = o.sound_implicit_rk_step(p64[1], p64[2], p64[3], b64[3], p64[4], f64(pstar), f64(ru), f64(rv), f64(rw),
+                                   f64(tr), b64[1], n, dts, gr)
+    r32 = o.sound_implicit_rk_step(p32[1], p32[2], p32[3], b32[3], p32[4], pstar, ru, rv, rw, tr, b32[1], n, dts, gr)
+    base = (p64[1], p64[2], p64[3], p64[4])
+    for name, a, b, c, a0 in zip(["u", "v", "w", "pip"], got, r64, r32, base):
+        check_fp32(from_dev(a) - a0, b - a0, c - a0, f"implicit acoustic {name}")
+
+
 def test_scalar_and_momentum_updates(case):
     from lex_amd import one_step_integration as one, advection as adv
     gr, g = case["gr"], case["grids"]

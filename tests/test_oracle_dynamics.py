@@ -273,3 +273,90 @@ def test_config1_n_sound_6_is_unstable_and_7_is_not():
                 xi, _ = o.ws_rk3_big_step(xi, base, c.grid, opts)
                 wmax = float(np.nanmax(np.abs(xi[3]))) if np.isfinite(xi[3]).all() else np.inf
             assert (wmax > 100.0) == expect_blowup, (ns, wmax)
+
+
+# ---------------------------------------------------------------- solver_opt 3
+def test_tridiagonal_solve_matches_banded_solver():
+    """Thomas restatement of jax.lax.linalg.tridiagonal_solve vs scipy's banded LU."""
+    from scipy.linalg import solve_banded
+    rng = np.random.default_rng(8)
+    n = 37
+    a = rng.normal(size=(3, 4, n))
+    c = rng.normal(size=(3, 4, n))
+    b = 3.0 + np.abs(a) + np.abs(c) + rng.uniform(size=(3, 4, n))
+    d = rng.normal(size=(3, 4, n))
+    a[..., 0] = 0.0
+    c[..., -1] = 0.0
+    x = o.tridiagonal_solve(a, b, c, d)
+    for i in range(3):
+        for j in range(4):
+            ab = np.zeros((3, n))
+            ab[0, 1:] = c[i, j, :-1]
+            ab[1] = b[i, j]
+            ab[2, :-1] = a[i, j, 1:]
+            ref = solve_banded((1, 1), ab, d[i, j])
+            assert np.allclose(x[i, j], ref, rtol=1e-12, atol=1e-12)
+
+
+def _acoustic_inputs(nx=8, ny=6, nz=12, seed=3):
+    gr = o.Grid(nx, ny, nz, 200.0, 200.0, 200.0)
+    rng = np.random.default_rng(seed)
+    z = (np.arange(nz + 2) - 0.5) * gr.dz
+    th0 = np.broadcast_to(300.0 + 0.003 * z, (nx + 6, ny + 6, nz + 2)).copy()
+    pi0 = np.broadcast_to(1.0 - 9.81 * z / (1004.64 * 300.0), (nx + 6, ny + 6, nz + 2)).copy()
+    rho0 = np.broadcast_to(1.1 - 1e-4 * z, (nx + 6, ny + 6, nz + 2)).copy()
+    u = o.padding3u_array(rng.normal(0, 1.0, (nx + 1, ny, nz)))
+    v = o.padding3v_array(rng.normal(0, 1.0, (nx, ny + 1, nz)))
+    wi = rng.normal(0, 0.5, (nx, ny, nz + 1))
+    wi[:, :, 0] = wi[:, :, -1] = 0.0
+    w = o.padding3_array(wi)
+    pip = o.padding3_array(rng.normal(0, 1e-4, (nx, ny, nz)))
+    pstar = o.padding3_array(rng.normal(0, 1e-4, (nx, ny, nz)))
+    ru = rng.normal(0, 1e-2, (nx + 1, ny, nz))
+    rv = rng.normal(0, 1e-2, (nx, ny + 1, nz))
+    rw = rng.normal(0, 1e-2, (nx, ny, nz + 1))
+    trho = th0 + o.padding3_array(rng.normal(0, 0.5, (nx, ny, nz)))
+    return gr, (u, v, w, pi0, pip, pstar, ru, rv, rw, trho, rho0)
+
+
+def test_implicit_acoustic_rest_state_stays_at_rest():
+    gr, (u, v, w, pi0, pip, pstar, ru, rv, rw, trho, rho0) = _acoustic_inputs()
+    z = [np.zeros_like(a) for a in (u, v, w, pip, ru, rv, rw)]
+    out = o.sound_implicit_rk_step(z[0], z[1], z[2], pi0, z[3], z[3], z[4], z[5], z[6], trho, rho0, 3, 0.2, gr)
+    assert all(np.max(np.abs(a)) == 0.0 for a in out)
+
+
+def test_implicit_acoustic_substep_is_consistent_with_the_explicit_one():
+    """One substep of the vertically implicit scheme differs from the forward-backward one
+    by O(dts^2) (the off-centred implicit terms): halving dts divides the difference by ~4.
+    A dropped or misplaced O(dts) term (wrong slice, missing rhs) breaks the ratio."""
+    gr, (u, v, w, pi0, pip, pstar, ru, rv, rw, trho, rho0) = _acoustic_inputs()
+    diffs = []
+    for dts in (0.02, 0.01, 0.005):
+        ei = o.sound_implicit_rk_step(u, v, w, pi0, pip, pstar, ru, rv, rw, trho, rho0, 1, dts, gr)
+        ee = o.sound_rk_step(u, v, w, pi0, pip, pstar, ru, rv, rw, trho, rho0, 1, dts, gr)
+        diffs.append((np.max(np.abs(ei[2] - ee[2])), np.max(np.abs(ei[3] - ee[3]))))
+        # the explicit parts (u, v) are identical
+        assert np.array_equal(ei[0], ee[0]) and np.array_equal(ei[1], ee[1])
+    # w: ratio 4; pi' carries the O(dts^2) w difference one substep further, with O(dts^3)
+    # terms still visible at these steps (ratio 4.5-5.1): bounds (3.5, 6)
+    for q, hi in ((0, 4.5), (1, 6.0)):
+        r1 = diffs[0][q] / diffs[1][q]
+        r2 = diffs[1][q] / diffs[2][q]
+        assert 3.5 < r1 < hi and 3.5 < r2 < hi, (q, diffs)
+
+
+def test_config1_n_sound_6_is_stable_with_the_implicit_solver():
+    """The vertically implicit acoustic step (solver_opt 3) removes the vertical acoustic
+    Courant limit that makes config 1 (dx = dz = 100 m, dt = 1 s) unstable at n_sound 6
+    with solver_opt 2 (same reduced column set as the solver_opt 2 check above)."""
+    case, opts = o.case_config1()
+    c = o.scaled_case(case, 8, 8, 64)
+    phys, base, _ = o.setup_warm_bubble(c)
+    opts.n_sound = 6
+    opts.solver_opt = 3
+    xi = phys
+    with np.errstate(all="ignore"):
+        for _ in range(6):
+            xi, _ = o.ws_rk3_big_step(xi, base, c.grid, opts)
+    assert np.isfinite(xi[3]).all() and float(np.max(np.abs(xi[3]))) < 100.0
