@@ -6,8 +6,10 @@ Workload: the fully compressible moist warm bubble at 256x256x128 cells (dx = dy
 passive qc, qr), FP32.  The specified n_sound = 6 (1 + 3 + 6 acoustic substeps) is
 acoustically unstable on this grid, so the timed run uses n_sound = 7 (1 + 4 + 7) and the
 n_sound = 6 step time is reported beside it.  A "step" is one full WS-RK3 step of the
-whole grid; inputs are resident in HBM.  Also measured: the forward and forward+adjoint
-step times of the differentiable 128x128x64 workload (20 steps, checkpoint every 5).
+whole grid; inputs are resident in HBM.  Also measured: the same workload with the
+vertically implicit acoustic substeps (solver_opt 3) at the specified n_sound = 6, which is
+stable, and the forward and forward+adjoint step times of the differentiable 128x128x64
+workload (20 steps, checkpoint every 5).
 
 Prints one JSON line: cell-steps/s, the dominant kernel's roofline (algorithmic bytes per
 launch / its average launch time from HIP events over the timed region, against the
@@ -82,6 +84,40 @@ def cpu_baseline(sample_steps=3, n_sound=7, nx=64, ny=64, nz=128):
     return {"value": nx * ny * nz * sample_steps / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
             "sample": f"{sample_steps} WS-RK3 steps of the same recipe on a {nx}x{ny}x{nz} sub-grid "
                       f"(NumPy FP32 oracle, single-threaded), {dt:.1f} s"}
+
+
+def implicit_solver(init, base, g, ncell, n_sound, steps=20, warmup=2):
+    """The same workload with solver_opt 3 (vertically implicit acoustic substeps) at the
+    specified n_sound: the implicit vertical coupling removes the vertical sound-wave limit
+    that makes the explicit substeps unstable there."""
+    import torch
+    from lex_amd import _lib, one_sprint_functions as sp
+    mo = (3, False, False, 0, False, False, 1)
+    st = tuple(a.clone() for a in init)
+    for _ in range(warmup):
+        sp.fused_step(st, base, g, mo, n_sound=n_sound)
+    torch.cuda.synchronize()
+    _lib.profile_enable(True, 16384)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        sp.fused_step(st, base, g, mo, n_sound=n_sound)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    prof = _lib.profile_read()
+    _lib.profile_enable(False)
+    kvi = prof.get("k_acoustic_vi")
+    out = {"solver_opt": 3, "n_sound": n_sound, "steps": steps, "warmup": warmup,
+           "value": ncell * steps / (ms / 1e3), "unit": "cell-steps/s", "ms_per_step": ms / steps,
+           "state_finite": all(bool(torch.isfinite(a).all()) for a in st)}
+    if kvi:
+        avg = kvi[0] / kvi[1]
+        gbs = BYTES_PER_CELL["k_acoustic"] * ncell / (avg / 1e3) / 1e9
+        out["k_acoustic_vi"] = {"avg_launch_ms": avg, "launches_per_step": kvi[1] / steps,
+                                "share": kvi[0] / ms, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
+                                "bytes_per_cell": BYTES_PER_CELL["k_acoustic"]}
+    return out
 
 
 def fwd_adjoint(steps=20, checkpoint_every=5, nx=128, ny=128, nz=64):
@@ -187,9 +223,14 @@ def main():
                          "dx=100 m, dt=1 s; 7 is the smallest stable value)")
     ap.add_argument("--no-as-specified", action="store_true", help="skip the n_sound=6 timing")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint workload")
+    ap.add_argument("--no-implicit", action="store_true",
+                    help="skip the solver_opt 3 (vertically implicit) timing at the specified n_sound")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, copy a 256 MiB buffer 3 times (a kernel of known HBM "
                          "traffic, used to scale rocprof FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--only", choices=("adjoint", "sgs-net"), default=None,
+                    help="run only the forward+adjoint workload or only the SGS-net timing and print its "
+                         "JSON (for per-kernel profiles of that workload)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -201,6 +242,9 @@ def main():
     from lex_amd.setup_lex import build_case, config1
 
     _lib.load()
+    if args.only is not None:
+        print(json.dumps(fwd_adjoint() if args.only == "adjoint" else sgs_net_timing()))
+        return
     case, opts = config1()
     phys, base, grids, _ = build_case(case)
     n_sound = args.n_sound
@@ -277,7 +321,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    del state, phys, init
+    del state, phys
+    impl = None if args.no_implicit else implicit_solver(init, base, g, ncell, opts["n_sound"])
+    del init
     fa = None if args.no_adjoint else fwd_adjoint()
     sn = None if args.no_adjoint else sgs_net_timing()
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample_steps, n_sound)
@@ -309,6 +355,7 @@ def main():
                      "bytes_per_cell": bpc, "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
         "as_specified": as_specified,
+        "implicit_as_specified": impl,
         "fwd_adjoint": fa,
         "sgs_net": sn,
         "kernels": kernels,

@@ -844,7 +844,7 @@ void launch_fwd(int nx, int ny, int nz, const void* x, const void* wp, const flo
 template <int CI, int CO>
 void launch_wgrad(int nx, int ny, int nz, const void* x, const void* g, float* dw, hipStream_t st) {
   long nt = (long)((nx + CTX - 1) / CTX) * ((ny + CTY - 1) / CTY) * ((nz + CTZ - 1) / CTZ);
-  unsigned nb = (unsigned)(nt < 1024 ? nt : 1024);
+  unsigned nb = (unsigned)(nt < 256 ? nt : 256);  // one workgroup per CU (the accumulators fill the register file)
   hipLaunchKernelGGL((k_conv3_wgrad<CI, CO>), dim3(nb), dim3(256), 0, st, nx, ny, nz, (const __bf16*)x,
                      (const __bf16*)g, dw);
 }
@@ -886,7 +886,7 @@ int lex_conv3d_bf16_bwd_weight(int nx, int ny, int nz, int cin, int cout, const 
   else launch_wgrad<32, 32>(nx, ny, nz, x, g_pre, dw, st);
   if (db) {
     long ncell = (long)nx * ny * nz;
-    hipLaunchKernelGGL(k_conv3_bgrad, dim3(256, cout), dim3(256), 0, st, ncell, cop, (const __bf16*)g_pre, db);
+    hipLaunchKernelGGL(k_conv3_bgrad, dim3(256), dim3(256), 0, st, ncell, cop, cout, (const __bf16*)g_pre, db);
   }
   return check_launch("lex_conv3d_bf16_bwd_weight");
 }

@@ -120,12 +120,45 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(int nx, int ny, int nz, const
 // Weight gradient: dW[tap][c][n] += sum_p X[p + o(tap)][c] * G[p][n] over the cells of
 // the grid (G: BF16 gradient of the pre-activation, COUT channels).  Workgroups loop
 // over output tiles and flush their FP32 partial sums with one atomic add per element.
+//
+// The reduction runs over cells, so an MFMA operand lane needs 8 consecutive cells of
+// one channel: both tiles are transposed into channel rows in LDS, the X rows as z-runs
+// of XR = 24 elements per (c, hx, hy): the CTZ interior cells at 8..8+CTZ-1, the lower
+// halo at 7 and the upper halo at 0.  A tap's z offset dz then shifts the 8-cell window
+// by one element, which is rebuilt from one aligned 16-byte read and one neighbouring
+// dword with byte-align (the upper halo read for dz = +1 at the end of the run wraps to
+// offset 0, where it is stored).
+constexpr int XR = 24;
+
+__device__ __forceinline__ bf16x8 xrun_window(const __bf16* __restrict__ row, int hz, int dz) {
+  typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4 c = *(const u32x4*)(row + 8 + hz);
+  if (dz == 0) return __builtin_bit_cast(bf16x8, c);
+  u32x4 r;
+  if (dz < 0) {
+    const unsigned lo = *(const unsigned*)(row + 6 + hz);  // elements 6+hz, 7+hz
+    r[0] = __builtin_amdgcn_alignbyte(c[0], lo, 2);
+    r[1] = __builtin_amdgcn_alignbyte(c[1], c[0], 2);
+    r[2] = __builtin_amdgcn_alignbyte(c[2], c[1], 2);
+    r[3] = __builtin_amdgcn_alignbyte(c[3], c[2], 2);
+  } else {
+    const unsigned hi = *(const unsigned*)(row + (16 + hz) % XR);  // element 16+hz (or the upper halo)
+    r[0] = __builtin_amdgcn_alignbyte(c[1], c[0], 2);
+    r[1] = __builtin_amdgcn_alignbyte(c[2], c[1], 2);
+    r[2] = __builtin_amdgcn_alignbyte(c[3], c[2], 2);
+    r[3] = __builtin_amdgcn_alignbyte(hi, c[3], 2);
+  }
+  return __builtin_bit_cast(bf16x8, r);
+}
+
 template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, const __bf16* __restrict__ X,
                                                      const __bf16* __restrict__ G, float* __restrict__ dW) {
+  static_assert(CTZ == 16 && XR == 24, "z-run layout assumes a 16-cell z tile");
   constexpr int MT = COUT / 16, NT = CIN / 16, TPW = 7;  // taps per wave: 7, 7, 7, 6
-  __shared__ __attribute__((aligned(16))) __bf16 xt[CIN * HX * HY * HZ];        // [c][hx][hy][hz]
-  __shared__ __attribute__((aligned(16))) __bf16 gt[COUT * CTX * CTY * CTZ];    // [n][cell]
+  constexpr int NCELL = CTX * CTY * CTZ;
+  __shared__ __attribute__((aligned(16))) __bf16 xt[CIN * HX * HY * XR];  // [c][hx][hy][z-run]
+  __shared__ __attribute__((aligned(16))) __bf16 gt[COUT * NCELL];        // [n][cell]
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, m = l & 15, g = l >> 4;
   f32x4 acc[TPW][MT][NT];
 #pragma unroll
@@ -137,21 +170,36 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, con
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int z0 = (tile % tz) * CTZ, y0 = ((tile / tz) % ty) * CTY, x0 = (tile / (tz * ty)) * CTX;
     __syncthreads();
-    for (int e = threadIdx.x; e < HX * HY * HZ * CIN; e += 256) {
-      int c = e % CIN, cell = e / CIN;
-      int hz = cell % HZ, hy = (cell / HZ) % HY, hx = cell / (HZ * HY);
-      int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-      __bf16 v = (__bf16)0.0f;
-      if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) v = X[(((long)gx * ny + gy) * nz + gz) * CIN + c];
-      xt[c * (HX * HY * HZ) + cell] = v;
+    // X halo tile: one 16-byte load = 8 channels of one cell, scattered to 8 channel rows
+    // (neighbouring lanes take neighbouring cells, so their 2-byte stores share banks
+    // only pairwise)
+    for (int e = threadIdx.x; e < HX * HY * HZ * (CIN / 8); e += 256) {
+      const int cell = e % (HX * HY * HZ), cg = e / (HX * HY * HZ);
+      const int hz = cell % HZ, hy = (cell / HZ) % HY, hx = cell / (HZ * HY);
+      const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+      bf16x8 v;
+      if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
+        v = *(const bf16x8*)(X + (((long)gx * ny + gy) * nz + gz) * CIN + 8 * cg);
+      } else {
+        for (int j = 0; j < 8; ++j) v[j] = (__bf16)0.0f;
+      }
+      __bf16* dst = xt + ((8 * cg) * (HX * HY) + hx * HY + hy) * XR + (hz == HZ - 1 ? 0 : 7 + hz);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) dst[j * (HX * HY * XR)] = v[j];
     }
-    for (int e = threadIdx.x; e < CTX * CTY * CTZ * COUT; e += 256) {
-      int n = e % COUT, cell = e / COUT;
-      int hz = cell % CTZ, hy = (cell / CTZ) % CTY, hx = cell / (CTZ * CTY);
-      int gx = x0 + hx, gy = y0 + hy, gz = z0 + hz;
-      __bf16 v = (__bf16)0.0f;
-      if (gx < nx && gy < ny && gz < nz) v = G[(((long)gx * ny + gy) * nz + gz) * COUT + n];
-      gt[n * (CTX * CTY * CTZ) + cell] = v;
+    // G tile: 8 channels of one cell per load, scattered to 8 channel rows
+    for (int e = threadIdx.x; e < NCELL * (COUT / 8); e += 256) {
+      const int cell = e % NCELL, ng = e / NCELL;
+      const int hz = cell % CTZ, hy = (cell / CTZ) % CTY, hx = cell / (CTZ * CTY);
+      const int gx = x0 + hx, gy = y0 + hy, gz = z0 + hz;
+      bf16x8 v;
+      if (gx < nx && gy < ny && gz < nz) {
+        v = *(const bf16x8*)(G + (((long)gx * ny + gy) * nz + gz) * COUT + 8 * ng);
+      } else {
+        for (int j = 0; j < 8; ++j) v[j] = (__bf16)0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) gt[(8 * ng + j) * NCELL + cell] = v[j];
     }
     __syncthreads();
 #pragma unroll
@@ -159,19 +207,15 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, con
       const int tap = w + 4 * q;
       if (tap > 26) continue;
       const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
-      for (int ks = 0; ks < (CTX * CTY * CTZ) / 32; ++ks) {
+      for (int ks = 0; ks < NCELL / 32; ++ks) {
         const int k0 = ks * 32 + 8 * g;  // 8 consecutive cells along z of one (hx, hy) line
         const int hx = k0 / (CTY * CTZ), hy = (k0 / CTZ) % CTY, hz = k0 % CTZ;
         bf16x8 a[MT], b[NT];
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) a[mt] = *(const bf16x8*)(gt + (mt * 16 + m) * (CTX * CTY * CTZ) + k0);
-        const int hb = ((hx + 1 + dx) * HY + (hy + 1 + dy)) * HZ + (hz + 1 + dz);
+        for (int mt = 0; mt < MT; ++mt) a[mt] = *(const bf16x8*)(gt + (mt * 16 + m) * NCELL + k0);
+        const int rb = (hx + 1 + dx) * HY + (hy + 1 + dy);
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-          const __bf16* src = xt + (nt * 16 + m) * (HX * HY * HZ) + hb;
-#pragma unroll
-          for (int j = 0; j < 8; ++j) b[nt][j] = src[j];
-        }
+        for (int nt = 0; nt < NT; ++nt) b[nt] = xrun_window(xt + ((nt * 16 + m) * (HX * HY) + rb) * XR, hz, dz);
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -194,14 +238,30 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, con
   }
 }
 
-// bias gradient: db[n] += sum_p G[p][n]
-__global__ void k_conv3_bgrad(long ncell, int cout, const __bf16* __restrict__ G, float* __restrict__ db) {
-  const int n = blockIdx.y;
-  float s = 0.f;
-  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < ncell; p += (long)gridDim.x * blockDim.x)
-    s += (float)G[p * cout + n];
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(db + n, s);
+// bias gradient: db[n] += sum_p G[p][n] for n < cout_real.  A thread reads 8 channels of
+// one cell per 16-byte load; partial sums are reduced across the wave and flushed with
+// one atomic add per channel per wave.
+__global__ __launch_bounds__(256) void k_conv3_bgrad(long ncell, int cp, int cout_real,
+                                                     const __bf16* __restrict__ G, float* __restrict__ db) {
+  const int nch = cp / 8;  // 16-byte chunks per cell (2 or 4; divides 64)
+  const int cg = threadIdx.x % nch;
+  float s[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s[j] = 0.f;
+  const long step = (long)gridDim.x * (blockDim.x / nch);
+  for (long p = ((long)blockIdx.x * blockDim.x + threadIdx.x) / nch; p < ncell; p += step) {
+    const bf16x8 v = *(const bf16x8*)(G + p * cp + 8 * cg);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] += (float)v[j];
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    for (int off = 32; off >= nch; off >>= 1) s[j] += __shfl_down(s[j], off, 64);
+  if ((threadIdx.x & 63) < nch) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (8 * cg + j < cout_real) unsafeAtomicAdd(db + 8 * cg + j, s[j]);
+  }
 }
 
 // g_pre = g_post * gelu'(pre) (or g_post when gelu is 0) -> BF16, cp channels (the
