@@ -33,6 +33,7 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, 8 TB/s
 # fields read or written at the boundary of the reference routine(s) it implements
 BYTES_PER_CELL = {
     "k_acoustic": 52,        # u,v,w,pi',rhs x3, theta_rho, pi* in; u,v,w,pi' out
+    "k_acoustic2": 104,      # two substeps per launch: 2 x 52
     "k_scalar_update": 32,   # s, s0, u, v, w, flow_div, sgs in; s_next out
     "k_rhs_uv": 32,          # u, v, w, flow_div, sgs_u, sgs_v in; rhs_u, rhs_v out
     "k_rhs_w": 32,           # u, v, w, theta, qv, flow_div, sgs_w in; rhs_w out
@@ -118,6 +119,24 @@ def implicit_solver(init, base, g, ncell, n_sound, steps=20, warmup=2):
                                 "share": kvi[0] / ms, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
                                 "bytes_per_cell": BYTES_PER_CELL["k_acoustic"]}
     return out
+
+
+def copy_bandwidth(mib=1024, reps=5):
+    """Device-to-device copy of `mib` MiB (read + write), the practical HBM ceiling."""
+    import torch
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * src.numel() * 4 * reps / (e0.elapsed_time(e1) / 1e3) / 1e9
+    del src, dst
+    return {"bytes_per_copy": 2 * (mib << 20), "GBs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS}
 
 
 def fwd_adjoint(steps=20, checkpoint_every=5, nx=128, ny=128, nz=64):
@@ -324,6 +343,7 @@ def main():
     del state, phys
     impl = None if args.no_implicit else implicit_solver(init, base, g, ncell, opts["n_sound"])
     del init
+    cb = copy_bandwidth()
     fa = None if args.no_adjoint else fwd_adjoint()
     sn = None if args.no_adjoint else sgs_net_timing()
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample_steps, n_sound)
@@ -352,7 +372,8 @@ def main():
                      "traffic": measured_traffic(dom, case.nx, case.ny, case.nz, n_sound),
                      "traffic_source": "profiles/rk3_step_256x256x128_hbm_traffic.md (rocprofv3 --pmc FETCH_SIZE, "
                                        "WRITE_SIZE; copy-kernel scaled)",
-                     "bytes_per_cell": bpc, "avg_launch_ms": avg_ms},
+                     "bytes_per_cell": bpc, "avg_launch_ms": avg_ms,
+                     "copy_GBs": cb["GBs"], "frac_of_copy": achieved / cb["GBs"]},
         "cpu_baseline": cpu,
         "as_specified": as_specified,
         "implicit_as_specified": impl,

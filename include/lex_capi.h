@@ -104,6 +104,12 @@ int lex_acoustic_substep(const lex_grid_t* g, const float* rho0_col, const float
                          const float* v, const float* w, const float* pip, const float* pip_star, const float* rhs_u,
                          const float* rhs_v, const float* rhs_w, const float* theta_rho, float dt_sound, float* u_out,
                          float* v_out, float* w_out, float* pip_out, void* stream);
+/* two consecutive passes of that loop (one_step_integration.py:528-553) in one launch,
+ * the same arguments; results equal two lex_acoustic_substep calls.  nz <= 256. */
+int lex_acoustic_substep_pair(const lex_grid_t* g, const float* rho0_col, const float* pi0_col, const float* u,
+                              const float* v, const float* w, const float* pip, const float* pip_star,
+                              const float* rhs_u, const float* rhs_v, const float* rhs_w, const float* theta_rho,
+                              float dt_sound, float* u_out, float* v_out, float* w_out, float* pip_out, void* stream);
 /* one pass of the loop in sound_implicit_rk_step0/1/2 (one_step_integration.py:590-633,
  * solver_opt 3): the same arguments plus the off-centring alpha (namelist alpha = 0.6);
  * w and pi' implicit in the vertical, one tridiagonal solve per column. */

@@ -295,6 +295,22 @@ int lex_acoustic_substep(const lex_grid_t* g, const float* r, const float* pi0, 
   return check_launch("lex_acoustic_substep");
 }
 
+int lex_acoustic_substep_pair(const lex_grid_t* g, const float* r, const float* pi0, const float* u, const float* v,
+                              const float* w, const float* p, const float* ps, const float* ru, const float* rv,
+                              const float* rw, const float* tr, float dts, float* uo, float* vo, float* wo, float* po,
+                              void* stream) {
+  REQGRID(g);
+  REQ(r); REQ(pi0); REQ(u); REQ(v); REQ(w); REQ(p); REQ(ps); REQ(ru); REQ(rv); REQ(rw); REQ(tr);
+  REQ(uo); REQ(vo); REQ(wo); REQ(po);
+  if (uo == u || vo == v || wo == w || po == p)
+    return fail(-1, "lex_acoustic_substep_pair: outputs must not alias inputs%s");
+  Dims d = mkdims(g);
+  if (!acoustic2_ok(d)) return fail(-1, "lex_acoustic_substep_pair: nz must be <= 256%s");
+  hipLaunchKernelGGL(k_acoustic2, acoustic2_grid(d), dim3(acoustic2_lanes(d), acoustic2_rows(d), 1), acoustic2_lds(d),
+                     S(stream), d, r, pi0, u, v, w, p, ps, ru, rv, rw, tr, uo, vo, wo, po, dts);
+  return check_launch("lex_acoustic_substep_pair");
+}
+
 int lex_acoustic_substep_implicit(const lex_grid_t* g, const float* r, const float* pi0, const float* u,
                                   const float* v, const float* w, const float* p, const float* ps, const float* ru,
                                   const float* rv, const float* rw, const float* tr, float dts, float alpha, float* uo,

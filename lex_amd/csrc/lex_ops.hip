@@ -450,6 +450,67 @@ __global__ void k_set_w_bc(Dims d, float* __restrict__ w) {
 }
 
 // ----------------------------------------------------------------------------------
+// S1 arithmetic of one cell (one_step_integration.py:528-553): the six face velocities
+// after pressure_gradient_force + rhs, set_w_bc at the surface and lid, the divergence,
+// advection_pi with the fixed pi* and the damped pi' update.  Operands: the cell's faces
+// (u0/u1 lower/upper x face, ...), the rhs on those faces, the 7-point stencils of pi',
+// theta_rho and pi*, and the rho0 / pi0 column values at levels K-1, K, K+1.
+// ----------------------------------------------------------------------------------
+struct AcousticOut {
+  float un0, un1, vn0, vn1, wn0, wn1, pn;
+};
+
+__device__ __forceinline__ AcousticOut acoustic_cell(
+    const Dims& d, int k, float u0, float u1, float v0, float v1, float w0, float w1, float ru0, float ru1, float rv0,
+    float rv1, float rw0, float rw1, float pc, float pxm, float pxp, float pym, float pyp, float pzm, float pzp, float tc,
+    float txm, float txp, float tym, float typ, float tzm, float tzp, float sc, float sxm, float sxp, float sym,
+    float syp, float szm, float szp, float rcm, float rcc, float rcp, float pim, float pic, float pip, float dts) {
+  // Contraction is spelled out (fmaf) and otherwise off, so the value of every output
+  // does not depend on the surrounding kernel's code generation: k_acoustic and both
+  // substeps of k_acoustic2 round identically.
+#pragma clang fp contract(off)
+  const int K = k + 1;
+  const float h = (float)(-kCp * 0.5);
+  // new velocities on the six faces of the cell (pgf_x/y/z with the same operation order)
+  const float un0 = fmaf(fmaf(h * (tc + txm), (pc - pxm) * d.idx, ru0), dts, u0);
+  const float un1 = fmaf(fmaf(h * (txp + tc), (pxp - pc) * d.idx, ru1), dts, u1);
+  const float vn0 = fmaf(fmaf(h * (tc + tym), (pc - pym) * d.idy, rv0), dts, v0);
+  const float vn1 = fmaf(fmaf(h * (typ + tc), (pyp - pc) * d.idy, rv1), dts, v1);
+  // set_w_bc: surface (l=0) and lid (l=nz) are zero after the update; pgf_z vanishes at
+  // levels 1 and nz+1
+  const float pz0 = (K <= 1) ? 0.0f : h * (tc + tzm) * ((pc - pzm) * d.idz);
+  const float pz1 = (K + 1 >= d.nz + 1) ? 0.0f : h * (tzp + tc) * ((pzp - pc) * d.idz);
+  const float wn0 = (k == 0) ? 0.0f : fmaf(pz0 + rw0, dts, w0);
+  const float wn1 = (k == d.nz - 1) ? 0.0f : fmaf(pz1 + rw1, dts, w1);
+  const float div = fmaf(wn1 - wn0, d.idz, fmaf(vn1 - vn0, d.idy, (un1 - un0) * d.idx));
+  // advection_pi with the new velocities and the fixed pi*
+  const float udx0 = un0 * ((sc - sxm) * d.idx);
+  const float udx1 = un1 * ((sxp - sc) * d.idx);
+  const float vdy0 = vn0 * ((sc - sym) * d.idy);
+  const float vdy1 = vn1 * ((syp - sc) * d.idy);
+  const float dz0 = fmaf(sc - szm, d.idz, (pic - pim) * d.idz);
+  const float dz1 = fmaf(szp - sc, d.idz, (pip - pic) * d.idz);
+  const float wdz0 = (k == 0) ? 0.0f : wn0 * dz0;
+  const float wdz1 = (k == d.nz - 1) ? 0.0f : wn1 * dz1;
+  const float r8w0 = 0.5f * (rcm + rcc);
+  const float r8w1 = 0.5f * (rcc + rcp);
+  const float advp =
+      -(fmaf(0.5f, udx1 + udx0, 0.5f * (vdy1 + vdy0)) + fmaf(0.5f * wdz0, r8w0, 0.5f * wdz1 * r8w1) / rcc);
+  const float piadv = pic + sc;
+  const float dpdt = fmaf((float)(-kRd / kCv), piadv * div, advp);
+  const float pn = fmaf(1.1f * dpdt, dts, pc);
+  AcousticOut o;
+  o.un0 = un0;
+  o.un1 = un1;
+  o.vn0 = vn0;
+  o.vn1 = vn1;
+  o.wn0 = wn0;
+  o.wn1 = wn1;
+  o.pn = pn;
+  return o;
+}
+
+// ----------------------------------------------------------------------------------
 // S1: one forward-backward acoustic substep (one_step_integration.py:528-553).
 // Reads u,v,w,pi' of the current substep, writes the next (halos filled, w BC set).
 // ----------------------------------------------------------------------------------
@@ -484,34 +545,10 @@ __global__ void k_acoustic(Dims d, const float* __restrict__ rc, const float* __
               tzm = tr[c - 1], tzp = tr[c + 1];
   const float sc = ps[c], sxm = ps[c - sx], sxp = ps[c + sx], sym = ps[c - sy], syp = ps[c + sy],
               szm = ps[c - 1], szp = ps[c + 1];
-  const float h = (float)(-kCp * 0.5);
-  // new velocities on the six faces of the cell (pgf_x/y/z with the same operation order)
-  float un0 = u0 + (h * (tc + txm) * ((pc - pxm) * d.idx) + ru0) * dts;
-  float un1 = u1 + (h * (txp + tc) * ((pxp - pc) * d.idx) + ru1) * dts;
-  float vn0 = v0 + (h * (tc + tym) * ((pc - pym) * d.idy) + rv0) * dts;
-  float vn1 = v1 + (h * (typ + tc) * ((pyp - pc) * d.idy) + rv1) * dts;
-  // set_w_bc: surface (l=0) and lid (l=nz) are zero after the update; pgf_z vanishes at
-  // levels 1 and nz+1
-  const float pz0 = (K <= 1) ? 0.0f : h * (tc + tzm) * ((pc - pzm) * d.idz);
-  const float pz1 = (K + 1 >= d.nz + 1) ? 0.0f : h * (tzp + tc) * ((pzp - pc) * d.idz);
-  float wn0 = (k == 0) ? 0.0f : w0 + (pz0 + rw0) * dts;
-  float wn1 = (k == d.nz - 1) ? 0.0f : w1 + (pz1 + rw1) * dts;
-  float div = (un1 - un0) * d.idx + (vn1 - vn0) * d.idy + (wn1 - wn0) * d.idz;
-  // advection_pi with the new velocities and the fixed pi*
-  float udx0 = un0 * ((sc - sxm) * d.idx);
-  float udx1 = un1 * ((sxp - sc) * d.idx);
-  float vdy0 = vn0 * ((sc - sym) * d.idy);
-  float vdy1 = vn1 * ((syp - sc) * d.idy);
-  float dz0 = (pic - pim) * d.idz + (sc - szm) * d.idz;
-  float dz1 = (pip - pic) * d.idz + (szp - sc) * d.idz;
-  float wdz0 = (k == 0) ? 0.0f : wn0 * dz0;
-  float wdz1 = (k == d.nz - 1) ? 0.0f : wn1 * dz1;
-  float r8w0 = 0.5f * (rcm + rcc);
-  float r8w1 = 0.5f * (rcc + rcp);
-  float advp = -(0.5f * (udx1 + udx0) + 0.5f * (vdy1 + vdy0) + (0.5f * wdz0 * r8w0 + 0.5f * wdz1 * r8w1) / rcc);
-  float piadv = pic + sc;
-  float dpdt = (float)(-kRd / kCv) * (piadv * div) + advp;
-  float pn = pc + 1.1f * dpdt * dts;
+  const AcousticOut o = acoustic_cell(d, k, u0, u1, v0, v1, w0, w1, ru0, ru1, rv0, rv1, rw0, rw1, pc, pxm, pxp, pym,
+                                      pyp, pzm, pzp, tc, txm, txp, tym, typ, tzm, tzp, sc, sxm, sxp, sym, syp, szm, szp,
+                                      rcm, rcc, rcp, pim, pic, pip, dts);
+  const float pn = o.pn, un0 = o.un0, un1 = o.un1, vn0 = o.vn0, vn1 = o.vn1, wn0 = o.wn0, wn1 = o.wn1;
   store_c_fast(d, po, i, j, k, pn);
   store_u_fast(d, uo, i, j, k, un0);
   if (i == d.nx - 1) store_u(d, uo, d.nx, j, k, un1);
@@ -519,6 +556,118 @@ __global__ void k_acoustic(Dims d, const float* __restrict__ rc, const float* __
   if (j == d.ny - 1) store_v(d, vo, i, d.ny, k, vn1);
   store_w_fast(d, wo, i, j, k, wn0);
   if (k == d.nz - 1) store_w(d, wo, i, j, d.nz, wn1);
+}
+
+// ----------------------------------------------------------------------------------
+// k_acoustic2: two consecutive S1 substeps in one launch (the same arithmetic as two
+// k_acoustic launches, acoustic_cell for each).  A workgroup owns whole z-columns of a
+// strip of R-2 rows in y and marches kA2X cells along x.  Each thread takes one
+// (row, level) of the strip plus one halo row on each side and, at x position s,
+//   substep A: advances cell s from the launch's input (global loads, as k_acoustic);
+//   its new pi' goes to a three-slab LDS ring;
+//   substep B: advances cell s-1 from the substep-A values: pi' of s-2, s-1, s from its
+//   own registers, the y and z neighbours from the ring, its faces' velocities and the
+//   operands of the rhs, theta_rho and pi* terms carried from its substep-A step of s-1.
+// Halo rows and the x positions x0-1 and x1 are computed at their periodic images
+// (wrapped indices), so every value substep B reads equals what the second of two
+// k_acoustic launches would read through the ghost cells.  Only substep B writes.
+// ----------------------------------------------------------------------------------
+#ifndef LEX_A2X
+#define LEX_A2X 16
+#endif
+#ifndef LEX_A2_ROWS
+#define LEX_A2_ROWS 8
+#endif
+constexpr int kA2X = LEX_A2X;        // cells along x per workgroup
+constexpr int kA2Rows = LEX_A2_ROWS;  // rows per workgroup (two of them halo rows), at most
+
+inline int acoustic2_lanes(const Dims& d) { return 64 * ((d.nz + 63) / 64); }
+inline int acoustic2_rows(const Dims& d) {
+  const int r = 1024 / acoustic2_lanes(d);
+  return r > kA2Rows ? kA2Rows : r;
+}
+inline bool acoustic2_ok(const Dims& d) { return acoustic2_rows(d) >= 4; }
+inline size_t acoustic2_lds(const Dims& d) {
+  return (size_t)3 * acoustic2_rows(d) * (acoustic2_lanes(d) + 2) * sizeof(float);
+}
+inline dim3 acoustic2_grid(const Dims& d) {
+  const int ty = acoustic2_rows(d) - 2;
+  return dim3((d.ny + ty - 1) / ty, (d.nx + kA2X - 1) / kA2X, 1);
+}
+
+__device__ __forceinline__ int wrap_cell(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+__global__ void __launch_bounds__(1024) k_acoustic2(Dims d, const float* __restrict__ rc,
+                                                    const float* __restrict__ pi0c, const float* __restrict__ u,
+                                                    const float* __restrict__ v, const float* __restrict__ w,
+                                                    const float* __restrict__ p, const float* __restrict__ ps,
+                                                    const float* __restrict__ ru, const float* __restrict__ rv,
+                                                    const float* __restrict__ rw, const float* __restrict__ tr,
+                                                    float* __restrict__ uo, float* __restrict__ vo,
+                                                    float* __restrict__ wo, float* __restrict__ po, float dts) {
+  extern __shared__ float ring[];  // [3 slabs][R rows][NZL + 2 levels]
+  const int NZL = blockDim.x, R = blockDim.y, LR = NZL + 2;
+  const int k = threadIdx.x, r = threadIdx.y;
+  const TileId tb = xcd_tile();
+  const int j = tb.x * (R - 2) - 1 + r;  // -1 .. ny (rows past ny+... are clamped, never stored)
+  const int jw = wrap_cell(min(j, d.ny), d.ny);
+  const int x0 = tb.y * kA2X, x1 = min(x0 + kA2X, d.nx);
+  const bool act = k < d.nz;
+  const int kc = act ? k : d.nz - 1;
+  const bool bcell = act && r >= 1 && r <= R - 2 && j < d.ny;
+  const int J = jw + 3, K = kc + 1;
+  const long sx = d.sxc(), sy = d.syc();
+  const float rcm = rc[K - 1], rcc = rc[K], rcp = rc[K + 1];
+  const float pim = pi0c[K - 1], pic = pi0c[K], pip = pi0c[K + 1];
+  // substep-A results and operands of the cell that substep B advances next
+  float bu0 = 0.f, bu1 = 0.f, bv0 = 0.f, bv1 = 0.f, bw0 = 0.f, bw1 = 0.f, bp = 0.f, bpm = 0.f;
+  float bru0 = 0.f, bru1 = 0.f, brv0 = 0.f, brv1 = 0.f, brw0 = 0.f, brw1 = 0.f;
+  float btc = 0.f, btxm = 0.f, btxp = 0.f, btym = 0.f, btyp = 0.f, btzm = 0.f, btzp = 0.f;
+  float bsc = 0.f, bsxm = 0.f, bsxp = 0.f, bsym = 0.f, bsyp = 0.f, bszm = 0.f, bszp = 0.f;
+  for (int s = x0 - 1; s <= x1; ++s) {
+    // ---- substep A at cell (s, j, k), taken at its periodic image (sw, jw, kc)
+    const int sw = wrap_cell(s, d.nx), I = sw + 3;
+    const long c = ic(d, I, J, K);
+    const long cu = iu(d, I, J, K), cv = iv(d, I, J, K), cw = iw(d, I, J, K);
+    const long ou = ((long)sw * d.ny + jw) * d.nz + kc, ov = ((long)sw * (d.ny + 1) + jw) * d.nz + kc;
+    const long ow = ((long)sw * d.ny + jw) * (d.nz + 1) + kc;
+    const float u0 = u[cu], u1 = u[cu + d.sxu()], v0 = v[cv], v1 = v[cv + d.syc()], w0 = w[cw], w1 = w[cw + 1];
+    const float ru0 = ru[ou], ru1 = ru[ou + (long)d.ny * d.nz], rv0 = rv[ov], rv1 = rv[ov + d.nz];
+    const float rw0 = rw[ow], rw1 = rw[ow + 1];
+    const float pc = p[c], pxm = p[c - sx], pxp = p[c + sx], pym = p[c - sy], pyp = p[c + sy], pzm = p[c - 1],
+                pzp = p[c + 1];
+    const float tc = tr[c], txm = tr[c - sx], txp = tr[c + sx], tym = tr[c - sy], typ = tr[c + sy],
+                tzm = tr[c - 1], tzp = tr[c + 1];
+    const float sc = ps[c], sxm = ps[c - sx], sxp = ps[c + sx], sym = ps[c - sy], syp = ps[c + sy],
+                szm = ps[c - 1], szp = ps[c + 1];
+    const AcousticOut a = acoustic_cell(d, kc, u0, u1, v0, v1, w0, w1, ru0, ru1, rv0, rv1, rw0, rw1, pc, pxm, pxp,
+                                        pym, pyp, pzm, pzp, tc, txm, txp, tym, typ, tzm, tzp, sc, sxm, sxp, sym, syp,
+                                        szm, szp, rcm, rcc, rcp, pim, pic, pip, dts);
+    ring[((s - x0 + 3) % 3) * R * LR + r * LR + k + 1] = a.pn;
+    __syncthreads();
+    // ---- substep B at cell (s-1, j, k); the ring slab of s-1 was completed one step ago
+    if (s > x0 && bcell) {
+      const int i = s - 1;
+      const float* L = ring + ((i - x0 + 3) % 3) * R * LR + r * LR + k + 1;
+      const AcousticOut b = acoustic_cell(d, kc, bu0, bu1, bv0, bv1, bw0, bw1, bru0, bru1, brv0, brv1, brw0, brw1, bp,
+                                          bpm, a.pn, L[-LR], L[LR], L[-1], L[1], btc, btxm, btxp, btym, btyp, btzm,
+                                          btzp, bsc, bsxm, bsxp, bsym, bsyp, bszm, bszp, rcm, rcc, rcp, pim, pic, pip,
+                                          dts);
+      store_c_fast(d, po, i, j, k, b.pn);
+      store_u_fast(d, uo, i, j, k, b.un0);
+      if (i == d.nx - 1) store_u(d, uo, d.nx, j, k, b.un1);
+      store_v_fast(d, vo, i, j, k, b.vn0);
+      if (j == d.ny - 1) store_v(d, vo, i, d.ny, k, b.vn1);
+      store_w_fast(d, wo, i, j, k, b.wn0);
+      if (k == d.nz - 1) store_w(d, wo, i, j, d.nz, b.wn1);
+    }
+    bpm = bp;
+    bp = a.pn;
+    bu0 = a.un0; bu1 = a.un1; bv0 = a.vn0; bv1 = a.vn1; bw0 = a.wn0; bw1 = a.wn1;
+    bru0 = ru0; bru1 = ru1; brv0 = rv0; brv1 = rv1; brw0 = rw0; brw1 = rw1;
+    btc = tc; btxm = txm; btxp = txp; btym = tym; btyp = typ; btzm = tzm; btzp = tzp;
+    bsc = sc; bsxm = sxm; bsxp = sxp; bsym = sym; bsyp = syp; bszm = szm; bszp = szp;
+  }
 }
 
 // k_acoustic_m: the same substep with each thread marching kAX cells along x: the x-face

@@ -280,9 +280,12 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
       dts = (float)dts_small;
     }
     float* in[4] = {X0[1], X0[2], X0[3], X0[4]};
-    for (int t = 0; t < nsub; ++t) {
+    // the fused path advances two substeps per launch while two remain
+    const bool pairs = (o.solver != 3) && (o.impl == 0) && acoustic2_ok(d);
+    for (int t = 0; t < nsub;) {
+      const int tn = t + ((pairs && t + 1 < nsub) ? 2 : 1);
       float* out[4];
-      bool last = (t == nsub - 1);
+      bool last = (tn == nsub);
       bool alias = last && (dst == X0) && (in[0] == X0[1]);
       if (last && !alias) {
         out[0] = dst[1]; out[1] = dst[2]; out[2] = dst[3]; out[3] = dst[4];
@@ -298,6 +301,10 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
         dim3 ag((d.nx + kAX - 1) / kAX, (d.ny + 3) / 4, (d.nz + 63) / 64);
         LEX_LAUNCH(KID_ACOUSTIC, k_acoustic_m, ag, dim3(64, 4, 1), 0, st, d, rc, b.pi0, in[0], in[1], in[2], in[3],
                    src[4], ru, rv, rw, tr, out[0], out[1], out[2], out[3], dts);
+      } else if (tn == t + 2) {
+        LEX_LAUNCH(KID_ACOUSTIC2, k_acoustic2, acoustic2_grid(d), dim3(acoustic2_lanes(d), acoustic2_rows(d), 1),
+                   acoustic2_lds(d), st, d, rc, b.pi0, in[0], in[1], in[2], in[3], src[4], ru, rv, rw, tr, out[0],
+                   out[1], out[2], out[3], dts);
       } else {
         LEX_LAUNCH(KID_ACOUSTIC, k_acoustic, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, b.pi0, in[0], in[1], in[2], in[3],
                    src[4], ru, rv, rw, tr, out[0], out[1], out[2], out[3], dts);
@@ -309,6 +316,7 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
                              dst[1 + c]);
       }
       for (int c = 0; c < 4; ++c) in[c] = out[c];
+      t = tn;
     }
     // scalars: theta (weps 1e-17), qv and passive (1e-18) (one_step_integration.py:905, :913)
     if (!fused) {

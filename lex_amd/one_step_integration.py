@@ -167,14 +167,21 @@ def _sound(u_now, v_now, w_now, pi0, pip_now, pip_star, rhs_u, rhs_v, rhs_w, rhs
         raise NotImplementedError("rhs_pi is a zero placeholder in the reference (one_step_integration.py:134)")
     rc, p0c = column(rho0), column(pi0)
     cur = (u_now, v_now, w_now, pip_now)
-    for _ in range(int(rk_n_sound)):
+    left = int(rk_n_sound)
+    while left > 0:
         nxt = tuple(torch.empty_like(a) for a in cur)
         args = [g, _lib.ptr(rc), _lib.ptr(p0c), *[_lib.ptr(a) for a in cur], _lib.ptr(pip_star), _lib.ptr(rhs_u),
                 _lib.ptr(rhs_v), _lib.ptr(rhs_w), _lib.ptr(theta_rho), float(dt_sound)]
         if implicit:
             _lib.call("lex_acoustic_substep_implicit", *args, float(ALPHA), *[_lib.ptr(a) for a in nxt], _lib.stream())
+            left -= 1
+        elif left >= 2 and g.nz <= 256:
+            # two passes of the loop in one launch
+            _lib.call("lex_acoustic_substep_pair", *args, *[_lib.ptr(a) for a in nxt], _lib.stream())
+            left -= 2
         else:
             _lib.call("lex_acoustic_substep", *args, *[_lib.ptr(a) for a in nxt], _lib.stream())
+            left -= 1
         cur = nxt
     return cur
 
