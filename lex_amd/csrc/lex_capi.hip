@@ -307,7 +307,7 @@ int lex_acoustic_substep_pair(const lex_grid_t* g, const float* r, const float* 
   Dims d = mkdims(g);
   if (!acoustic2_ok(d)) return fail(-1, "lex_acoustic_substep_pair: nz must be <= 256%s");
   hipLaunchKernelGGL(k_acoustic2, acoustic2_grid(d), dim3(acoustic2_lanes(d), acoustic2_rows(d), 1), acoustic2_lds(d),
-                     S(stream), d, r, pi0, u, v, w, p, ps, ru, rv, rw, tr, uo, vo, wo, po, dts);
+                     S(stream), d, r, pi0, u, v, w, p, ps, ru, rv, rw, tr, uo, vo, wo, po, dts, acoustic2_ax(d));
   return check_launch("lex_acoustic_substep_pair");
 }
 

@@ -561,7 +561,7 @@ __global__ void k_acoustic(Dims d, const float* __restrict__ rc, const float* __
 // ----------------------------------------------------------------------------------
 // k_acoustic2: two consecutive S1 substeps in one launch (the same arithmetic as two
 // k_acoustic launches, acoustic_cell for each).  A workgroup owns whole z-columns of a
-// strip of R-2 rows in y and marches kA2X cells along x.  Each thread takes one
+// strip of R-2 rows in y and marches `ax` cells along x.  Each thread takes one
 // (row, level) of the strip plus one halo row on each side and, at x position s,
 //   substep A: advances cell s from the launch's input (global loads, as k_acoustic);
 //   its new pi' goes to a three-slab LDS ring;
@@ -572,14 +572,7 @@ __global__ void k_acoustic(Dims d, const float* __restrict__ rc, const float* __
 // (wrapped indices), so every value substep B reads equals what the second of two
 // k_acoustic launches would read through the ghost cells.  Only substep B writes.
 // ----------------------------------------------------------------------------------
-#ifndef LEX_A2X
-#define LEX_A2X 16
-#endif
-#ifndef LEX_A2_ROWS
-#define LEX_A2_ROWS 8
-#endif
-constexpr int kA2X = LEX_A2X;        // cells along x per workgroup
-constexpr int kA2Rows = LEX_A2_ROWS;  // rows per workgroup (two of them halo rows), at most
+constexpr int kA2Rows = 8;  // rows per workgroup (two of them halo rows), at most
 
 inline int acoustic2_lanes(const Dims& d) { return 64 * ((d.nz + 63) / 64); }
 inline int acoustic2_rows(const Dims& d) {
@@ -590,9 +583,17 @@ inline bool acoustic2_ok(const Dims& d) { return acoustic2_rows(d) >= 4; }
 inline size_t acoustic2_lds(const Dims& d) {
   return (size_t)3 * acoustic2_rows(d) * (acoustic2_lanes(d) + 2) * sizeof(float);
 }
+// cells along x per workgroup: 16 (2 halo positions per 16), fewer when that would leave
+// the 256 CUs with under two workgroups each
+inline int acoustic2_ax(const Dims& d) {
+  const int ty = acoustic2_rows(d) - 2, strips = (d.ny + ty - 1) / ty;
+  int ax = 16;
+  while (ax > 4 && (long)strips * ((d.nx + ax - 1) / ax) < 512) ax >>= 1;
+  return ax;
+}
 inline dim3 acoustic2_grid(const Dims& d) {
-  const int ty = acoustic2_rows(d) - 2;
-  return dim3((d.ny + ty - 1) / ty, (d.nx + kA2X - 1) / kA2X, 1);
+  const int ty = acoustic2_rows(d) - 2, ax = acoustic2_ax(d);
+  return dim3((d.ny + ty - 1) / ty, (d.nx + ax - 1) / ax, 1);
 }
 
 __device__ __forceinline__ int wrap_cell(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
@@ -604,14 +605,15 @@ __global__ void __launch_bounds__(1024) k_acoustic2(Dims d, const float* __restr
                                                     const float* __restrict__ ru, const float* __restrict__ rv,
                                                     const float* __restrict__ rw, const float* __restrict__ tr,
                                                     float* __restrict__ uo, float* __restrict__ vo,
-                                                    float* __restrict__ wo, float* __restrict__ po, float dts) {
+                                                    float* __restrict__ wo, float* __restrict__ po, float dts,
+                                                    int ax) {
   extern __shared__ float ring[];  // [3 slabs][R rows][NZL + 2 levels]
   const int NZL = blockDim.x, R = blockDim.y, LR = NZL + 2;
   const int k = threadIdx.x, r = threadIdx.y;
   const TileId tb = xcd_tile();
   const int j = tb.x * (R - 2) - 1 + r;  // -1 .. ny (rows past ny+... are clamped, never stored)
   const int jw = wrap_cell(min(j, d.ny), d.ny);
-  const int x0 = tb.y * kA2X, x1 = min(x0 + kA2X, d.nx);
+  const int x0 = tb.y * ax, x1 = min(x0 + ax, d.nx);
   const bool act = k < d.nz;
   const int kc = act ? k : d.nz - 1;
   const bool bcell = act && r >= 1 && r <= R - 2 && j < d.ny;

@@ -304,7 +304,7 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
       } else if (tn == t + 2) {
         LEX_LAUNCH(KID_ACOUSTIC2, k_acoustic2, acoustic2_grid(d), dim3(acoustic2_lanes(d), acoustic2_rows(d), 1),
                    acoustic2_lds(d), st, d, rc, b.pi0, in[0], in[1], in[2], in[3], src[4], ru, rv, rw, tr, out[0],
-                   out[1], out[2], out[3], dts);
+                   out[1], out[2], out[3], dts, acoustic2_ax(d));
       } else {
         LEX_LAUNCH(KID_ACOUSTIC, k_acoustic, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, b.pi0, in[0], in[1], in[2], in[3],
                    src[4], ru, rv, rw, tr, out[0], out[1], out[2], out[3], dts);
