@@ -902,7 +902,8 @@ int lex_conv3d_bf16_bwd_weight(int nx, int ny, int nz, int cin, int cout, const 
   else launch_wgrad<32, 32>(nx, ny, nz, x, g_pre, dw, st);
   if (db) {
     long ncell = (long)nx * ny * nz;
-    hipLaunchKernelGGL(k_conv3_bgrad, dim3(256), dim3(256), 0, st, ncell, cop, cout, (const __bf16*)g_pre, db);
+    hipLaunchKernelGGL(k_conv3_bgrad, dim3(kBgradBlocks), dim3(256), 0, st, ncell, cop, cout, (const __bf16*)g_pre,
+                       db);
   }
   return check_launch("lex_conv3d_bf16_bwd_weight");
 }

@@ -239,10 +239,14 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, con
 }
 
 // bias gradient: db[n] += sum_p G[p][n] for n < cout_real.  A thread reads 8 channels of
-// one cell per 16-byte load; partial sums are reduced across the wave and flushed with
-// one atomic add per channel per wave.
+// one cell per 16-byte load; partial sums are reduced across the wave, then across the
+// workgroup's 4 waves in LDS, and flushed with one atomic add per channel per workgroup
+// (all workgroups add into the same cp addresses, so their number is kept small).
+constexpr int kBgradBlocks = 128;
+
 __global__ __launch_bounds__(256) void k_conv3_bgrad(long ncell, int cp, int cout_real,
                                                      const __bf16* __restrict__ G, float* __restrict__ db) {
+  __shared__ float red[4][32];
   const int nch = cp / 8;  // 16-byte chunks per cell (2 or 4; divides 64)
   const int cg = threadIdx.x % nch;
   float s[8];
@@ -257,11 +261,14 @@ __global__ __launch_bounds__(256) void k_conv3_bgrad(long ncell, int cp, int cou
 #pragma unroll
   for (int j = 0; j < 8; ++j)
     for (int off = 32; off >= nch; off >>= 1) s[j] += __shfl_down(s[j], off, 64);
-  if ((threadIdx.x & 63) < nch) {
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l < nch) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (8 * cg + j < cout_real) unsafeAtomicAdd(db + 8 * cg + j, s[j]);
+    for (int j = 0; j < 8; ++j) red[wv][8 * l + j] = s[j];
   }
+  __syncthreads();
+  const int n = threadIdx.x;
+  if (n < cp && n < cout_real) unsafeAtomicAdd(db + n, red[0][n] + red[1][n] + red[2][n] + red[3][n]);
 }
 
 // g_pre = g_post * gelu'(pre) (or g_post when gelu is 0) -> BF16, cp channels (the

@@ -312,3 +312,41 @@ def test_sprint_gradients_with_checkpoints():
     e = 1e-5
     fdc = (loss_oracle(tp0, 0.18 + e) - loss_oracle(tp0, 0.18 - e)) / (2 * e)
     _check(gcs, fdc, 5e-3, "sprint d/dCs")
+
+
+def test_sprint_gradient_along_16_seeded_directions():
+    """The differentiable workload's check protocol at a reduced size: a 6-step run with
+    a checkpoint every 2 steps, loss 1/2||theta'_6 - target||^2 (target from rng 2029);
+    the adjoint's d loss/d theta'_0 against FP64 central differences of the oracle along
+    16 seeded random directions (rng 2030)."""
+    import torch
+    from lex_amd import adjoint as adj
+    from lex_amd._core import grid_from
+    case, opts, og, oo, st, base, grids = _step_setup(nx=12, ny=12, nz=12, seed=6)
+    nsteps, every = 6, 2
+    bs = tuple(from_dev(a) for a in base[:5]) + (None,)
+    th0 = bs[2]
+    target = np.random.default_rng(2029).normal(0, 0.5, (og.nx, og.ny, og.nz))
+    tp0 = st[0][3:-3, 3:-3, 1:-1] - th0[3:-3, 3:-3, 1:-1]
+
+    def loss_oracle(tp):
+        x = (o.padding3_array(th0[3:-3, 3:-3, 1:-1] + tp),) + tuple(st[1:])
+        for _ in range(nsteps):
+            x = o.ws_rk3_big_step(x, bs, og, oo)[0]
+        d = x[0][3:-3, 3:-3, 1:-1] - th0[3:-3, 3:-3, 1:-1] - target
+        return 0.5 * float(np.sum(d * d))
+
+    g = grid_from(base[1], *grids[3:6])
+    tp = to_dev(tp0).requires_grad_(True)
+    cs = torch.tensor(0.18, dtype=torch.float32, device=tp.device, requires_grad=True)
+    th = adj.sprint_theta(tp, cs, tuple(to_dev(a) for a in st), base, g, opts["dt"], opts["n_sound"], nsteps, every)
+    (0.5 * ((th - to_dev(target)) ** 2).sum()).backward()
+    gth = from_dev(tp.grad)
+    rng = np.random.default_rng(2030)
+    errs = []
+    for d in range(16):
+        v = rng.normal(0, 1.0, tp0.shape)
+        e = 1e-5
+        fd = (loss_oracle(tp0 + e * v) - loss_oracle(tp0 - e * v)) / (2 * e)
+        errs.append(_check(_dot(gth, v), fd, 2e-3, f"direction {d}"))
+    print(f"16 directions: max rel err {max(errs):.2e}")
