@@ -121,6 +121,30 @@ def implicit_solver(init, base, g, ncell, n_sound, steps=20, warmup=2):
     return out
 
 
+def output_timing(reps=3):
+    """save2nc of the 256x256x128 state (device tensors -> a netCDF file in the
+    reference's layout): seconds per file and MB/s of field data."""
+    import tempfile
+    import torch
+    from lex_amd import write_netcdf as wn
+    from lex_amd.one_step_integration import _sfc_etc
+    from lex_amd.setup_lex import build_case, config1
+    case, _ = config1()
+    phys, base, grids, _ = build_case(case)
+    torch.cuda.synchronize()
+    nbytes = sum((a.shape[0] - 6) * (a.shape[1] - 6) * (a.shape[2] - 2) * 4 for a in phys[:6])
+    with tempfile.TemporaryDirectory() as td:
+        times = []
+        for r in range(reps):
+            t0 = time.perf_counter()
+            fn = wn.save2nc(phys, _sfc_etc(0, phys[0]), grids, r, float(r), filename=os.path.join(td, f"o{r}.nc"))
+            times.append(time.perf_counter() - t0)
+            fsize = os.path.getsize(fn)
+    t = min(times)
+    return {"workload": f"save2nc of the {case.nx}x{case.ny}x{case.nz} state (6 fields, device -> netCDF-3 file)",
+            "s_per_file": t, "field_MB": nbytes / 1e6, "file_MB": fsize / 1e6, "MB_per_s": nbytes / 1e6 / t}
+
+
 def copy_bandwidth(mib=1024, reps=5):
     """Device-to-device copy of `mib` MiB (read + write), the practical HBM ceiling."""
     import torch
@@ -247,9 +271,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, copy a 256 MiB buffer 3 times (a kernel of known HBM "
                          "traffic, used to scale rocprof FETCH_SIZE/WRITE_SIZE counters)")
-    ap.add_argument("--only", choices=("adjoint", "sgs-net"), default=None,
-                    help="run only the forward+adjoint workload or only the SGS-net timing and print its "
-                         "JSON (for per-kernel profiles of that workload)")
+    ap.add_argument("--only", choices=("adjoint", "sgs-net", "output"), default=None,
+                    help="run only the forward+adjoint workload, the SGS-net timing or the output-file "
+                         "timing and print its JSON (for per-kernel profiles of that workload)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -262,7 +286,8 @@ def main():
 
     _lib.load()
     if args.only is not None:
-        print(json.dumps(fwd_adjoint() if args.only == "adjoint" else sgs_net_timing()))
+        fn = {"adjoint": fwd_adjoint, "sgs-net": sgs_net_timing, "output": output_timing}[args.only]
+        print(json.dumps(fn()))
         return
     case, opts = config1()
     phys, base, grids, _ = build_case(case)

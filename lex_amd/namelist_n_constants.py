@@ -25,6 +25,10 @@ ngx = 3
 ngy = 3
 ngz = 1
 
+# Output files (namelist_n_constants.py:29-30)
+file_name_format = "experiments/lex_out_%0.4i.nc"
+base_file_name = "experiments/lex_reference_state.nc"
+
 ic_option = 1
 rand_opt = False
 cor_opt = 0
