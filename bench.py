@@ -145,6 +145,51 @@ def output_timing(reps=3):
             "s_per_file": t, "field_MB": nbytes / 1e6, "file_MB": fsize / 1e6, "MB_per_s": nbytes / 1e6 / t}
 
 
+def dl_sprint_timing(sprint_n=10):
+    """ws_rk3_sprint_dl on the 256x256x128 workload with simpleCNN (5 -> 32 -> 32 -> 32 -> 5,
+    BF16 MFMA) correcting the state after every step, against the same steps without it."""
+    import torch
+    from lex_amd import _lib, dl_models as dlm, namelist_n_constants as nl, one_sprint_functions as sp
+    from lex_amd.setup_lex import build_case, config1
+    case, opts = config1()
+    phys, base, grids, _ = build_case(case)
+    nl.dt, nl.n_sound, nl.solver_opt, nl.turb_opt, nl.sprint_n = opts["dt"], 7, 2, 1, sprint_n
+    mo = (2, False, False, 0, False, False, 1)
+    g = _lib.grid(case.nx, case.ny, case.nz, case.dx, case.dy, case.dz)
+    sc = []
+    for a, q in zip((phys[0], phys[1], phys[2], phys[3], phys[5]), range(5)):
+        inner = a[3:-3, 3:-3, 1:-1]
+        spread = float(inner.std()) + 1e-3
+        lo = (inner - spread).unsqueeze(0).unsqueeze(0).contiguous()
+        hi = (inner + spread).unsqueeze(0).unsqueeze(0).contiguous()
+        sc += [lo, hi]
+    scaling = tuple(sc) + (None,) * 4
+    net = dlm.simpleCNN(seed=7)
+    with torch.no_grad():
+        net.weights[-1].mul_(1e-3)
+    st = tuple(a.clone() for a in phys)
+    sp.ws_rk3_sprint_dl(st, base, grids, mo, net, scaling)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    out, _ = sp.ws_rk3_sprint_dl(st, base, grids, mo, net, scaling)
+    e1.record()
+    torch.cuda.synchronize()
+    t_dl = e0.elapsed_time(e1) / sprint_n
+    st2 = tuple(a.clone() for a in phys)
+    e0.record()
+    for _ in range(sprint_n):
+        sp.fused_step(st2, base, g, mo)
+    e1.record()
+    torch.cuda.synchronize()
+    t_plain = e0.elapsed_time(e1) / sprint_n
+    ncell = case.nx * case.ny * case.nz
+    return {"workload": f"ws_rk3_sprint_dl, {case.nx}x{case.ny}x{case.nz}, n_sound 7, sprint_n {sprint_n}, "
+                        "simpleCNN post-step correction",
+            "ms_per_step": t_dl, "ms_per_step_without_correction": t_plain, "correction_ms": t_dl - t_plain,
+            "cell_steps_per_s": ncell / (t_dl / 1e3), "state_finite": all(bool(torch.isfinite(a).all()) for a in out)}
+
+
 def copy_bandwidth(mib=1024, reps=5):
     """Device-to-device copy of `mib` MiB (read + write), the practical HBM ceiling."""
     import torch
@@ -271,9 +316,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, copy a 256 MiB buffer 3 times (a kernel of known HBM "
                          "traffic, used to scale rocprof FETCH_SIZE/WRITE_SIZE counters)")
-    ap.add_argument("--only", choices=("adjoint", "sgs-net", "output"), default=None,
-                    help="run only the forward+adjoint workload, the SGS-net timing or the output-file "
-                         "timing and print its JSON (for per-kernel profiles of that workload)")
+    ap.add_argument("--only", choices=("adjoint", "sgs-net", "output", "dl-sprint"), default=None,
+                    help="run only the forward+adjoint workload, the SGS-net timing, the output-file "
+                         "timing or the post-step-correction sprint and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -286,7 +331,8 @@ def main():
 
     _lib.load()
     if args.only is not None:
-        fn = {"adjoint": fwd_adjoint, "sgs-net": sgs_net_timing, "output": output_timing}[args.only]
+        fn = {"adjoint": fwd_adjoint, "sgs-net": sgs_net_timing, "output": output_timing,
+              "dl-sprint": dl_sprint_timing}[args.only]
         print(json.dumps(fn()))
         return
     case, opts = config1()

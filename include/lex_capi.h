@@ -345,6 +345,27 @@ int lex_rk3_step_vjp_taped_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o
                                    float* const* lam_in, float* lsu, float* lsv, float* lsw, float* const* lss,
                                    void* adjoint_ws, size_t adjoint_bytes, void* stream);
 
+/* ---- the reference's post-step deep-learning correction ------------------------------
+ * The pieces of ws_rk3_sprint_dl (one_sprint_functions.py:118-165) around its network:
+ * phys_state_tuple2array (dl_models.py:125-155): padded theta, u, v, w, qv -> out
+ * (nx, ny, nz, 5) channels-last FP32, each field normalised per level as (x - lo) / den
+ * (lo: min over x, y of the scaling minimum; den: max over x, y of the scaling maximum
+ * - lo + 0.01, or + 1e-5 for qv), u, v, w averaged to centres.  lo and den concatenate
+ * the per-level values of theta, u, v (nz each), w (nz+1) and qv (nz): 5 nz + 1 floats. */
+int lex_dl_pack(const lex_grid_t* g, const float* theta, const float* u, const float* v, const float* w,
+                const float* qv, const float* lo, const float* den, float* out, void* stream);
+/* correct_array2tuple with stagger_uvw (dl_models.py:158-182): the net's (nx, ny, nz, 5)
+ * centred output -> corrections on theta (nx,ny,nz), u (nx+1,ny,nz), v (nx,ny+1,nz),
+ * w (nx,ny,nz+1), qv points, with the reference's end faces. */
+int lex_dl_stagger(const lex_grid_t* g, const float* c, float* theta_c, float* u_c, float* v_c, float* w_c,
+                   float* qv_c, void* stream);
+/* The correction step of ws_rk3_sprint_dl (one_sprint_functions.py:137-160): padded
+ * theta, u, v, w, qv += staggered correction x range (r_*: the scaling maximum - minimum
+ * on each field's points, interior shapes), in place, ghosts refilled; pi' untouched. */
+int lex_dl_apply(const lex_grid_t* g, const float* c, const float* r_theta, const float* r_u, const float* r_v,
+                 const float* r_w, const float* r_qv, float* theta, float* u, float* v, float* w, float* qv,
+                 void* stream);
+
 #ifdef __cplusplus
 }
 #endif

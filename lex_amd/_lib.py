@@ -83,6 +83,9 @@ _SIGS = {
     "lex_bicgstab": [_G, _P, _P, _P, _P, _D, _D, _I, _P, C.c_size_t, C.POINTER(_I), _P],
     "lex_acoustic_substep": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P],
     "lex_acoustic_substep_pair": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P],
+    "lex_dl_pack": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lex_dl_stagger": [_G, _P, _P, _P, _P, _P, _P, _P],
+    "lex_dl_apply": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "lex_scalar_update": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _F, _P, _P, _P],
     "lex_momentum_update": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P, _P, _P],
     "lex_smag_deformation": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],

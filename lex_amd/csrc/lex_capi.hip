@@ -15,6 +15,7 @@
 #include "lex_adjoint_step.hip"
 #include "lex_conv.hip"
 #include "lex_sgs_io.hip"
+#include "lex_dl_post.hip"
 #include "../../include/lex_capi.h"
 
 using namespace lex;
@@ -943,6 +944,46 @@ int lex_sgs_features(const lex_grid_t* g, const float* theta0, const float* qv0,
   hipLaunchKernelGGL(k_sgs_features, grd(d.nz, d.ny, d.nx), blk(), 0, S(stream), d, theta0, qv0, u, v, w, th, qv, nrm,
                      out);
   return check_launch("lex_sgs_features");
+}
+
+// ------------------------------------------------- post-step deep-learning correction
+int lex_dl_pack(const lex_grid_t* g, const float* th, const float* u, const float* v, const float* w, const float* qv,
+                const float* lo, const float* den, float* out, void* stream) {
+  REQGRID(g);
+  REQ(th); REQ(u); REQ(v); REQ(w); REQ(qv); REQ(lo); REQ(den); REQ(out);
+  Dims d = mkdims(g);
+  DlNorm nrm;
+  nrm.lo = lo;
+  nrm.den = den;
+  hipLaunchKernelGGL(k_dl_pack, grd(d.nz, d.ny, d.nx), blk(), 0, S(stream), d, th, u, v, w, qv, nrm, out);
+  return check_launch("lex_dl_pack");
+}
+
+int lex_dl_stagger(const lex_grid_t* g, const float* c, float* tc, float* uc, float* vc, float* wc, float* qc,
+                   void* stream) {
+  REQGRID(g);
+  REQ(c); REQ(tc); REQ(uc); REQ(vc); REQ(wc); REQ(qc);
+  Dims d = mkdims(g);
+  if (d.nx < 2 || d.ny < 2 || d.nz < 2) return fail(-1, "lex_dl_stagger: nx, ny, nz must be >= 2%s");
+  hipLaunchKernelGGL(k_dl_stagger, grd(d.nz + 1, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, c, tc, uc, vc, wc, qc);
+  return check_launch("lex_dl_stagger");
+}
+
+int lex_dl_apply(const lex_grid_t* g, const float* c, const float* r_th, const float* r_u, const float* r_v,
+                 const float* r_w, const float* r_qv, float* th, float* u, float* v, float* w, float* qv,
+                 void* stream) {
+  REQGRID(g);
+  REQ(c); REQ(r_th); REQ(r_u); REQ(r_v); REQ(r_w); REQ(r_qv); REQ(th); REQ(u); REQ(v); REQ(w); REQ(qv);
+  Dims d = mkdims(g);
+  if (d.nx < 2 || d.ny < 2 || d.nz < 2) return fail(-1, "lex_dl_apply: nx, ny, nz must be >= 2%s");
+  DlRanges rg;
+  rg.th = r_th;
+  rg.u = r_u;
+  rg.v = r_v;
+  rg.w = r_w;
+  rg.qv = r_qv;
+  hipLaunchKernelGGL(k_dl_apply, grd(d.nz, d.ny, d.nx), blk(), 0, S(stream), d, c, rg, th, u, v, w, qv);
+  return check_launch("lex_dl_apply");
 }
 
 int lex_sgs_apply(const lex_grid_t* g, const float* y, const float* out_scale, float* su, float* sv, float* sw,

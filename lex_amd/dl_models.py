@@ -1,0 +1,93 @@
+"""The reference's post-step deep-learning correction: its network and data maps.
+
+Mirrors dl_models.py of the reference for the pieces ws_rk3_sprint_dl
+(one_sprint_functions.py:118-165) calls around the step:
+
+* ``simpleCNN`` (dl_models.py:19-30): four 3x3x3 'SAME' convolutions with bias,
+  5 -> 32 -> 32 -> 32 -> 5 channels, tanh-GELU between them, on the BF16 MFMA kernels
+  of liblexhip (``lex_amd.sgs_net.SgsNet`` with five input channels).
+* ``phys_state_tuple2array`` (:125-155) and ``correct_array2tuple`` (:158-163, with
+  ``stagger_uvw``): ``lex_dl_pack`` and ``lex_dl_stagger``.
+
+The flax parameters of the reference become the net object itself (``dl_params``).
+Zcnn and AutoEncoder (:33-122) are not built.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from .sgs_net import SgsNet
+
+
+def simpleCNN(seed=2031, device="cuda"):  # noqa: N802 - the reference's class name
+    """The reference's simpleCNN shape on the MFMA kernels: 5 -> 32 -> 32 -> 32 -> 5."""
+    return SgsNet(channels=(5, 32, 32, 32, 5), seed=seed, device=device)
+
+
+def level_norm(scaling_params):
+    """(lo, den), each (5 nz + 1,) FP32: per-level offsets and denominators of the
+    normalisation in phys_state_tuple2array (dl_models.py:127-151), concatenated for
+    theta, u, v (nz levels), w (nz + 1) and qv (nz)."""
+    los, dens = [], []
+    for q in range(5):
+        mn, mx = scaling_params[2 * q], scaling_params[2 * q + 1]
+        mn, mx = torch.as_tensor(mn).float().squeeze(), torch.as_tensor(mx).float().squeeze()
+        lo = torch.amin(mn, dim=(0, 1))
+        den = torch.amax(mx, dim=(0, 1)) - lo + (1.0e-5 if q == 4 else 0.01)
+        los.append(lo)
+        dens.append(den)
+    return torch.cat(los).contiguous(), torch.cat(dens).contiguous()
+
+
+def scaling_ranges(scaling_params, device):
+    """The scaling maximum minus minimum of theta, u, v, w, qv on their own points."""
+    out = []
+    for q in range(5):
+        mn, mx = scaling_params[2 * q], scaling_params[2 * q + 1]
+        r = (torch.as_tensor(mx).float() - torch.as_tensor(mn).float()).squeeze()
+        out.append(r.to(device).contiguous())
+    return tuple(out)
+
+
+def _grid(phys_state):
+    th = phys_state[0]
+    return _lib.grid(th.shape[0] - 6, th.shape[1] - 6, th.shape[2] - 2, 1.0, 1.0, 1.0)
+
+
+def phys_state_tuple2array(phys_state, scaling_params, norm=None):
+    """(1, nx, ny, nz, 5) FP32 network input (dl_models.py:125-155)."""
+    th, u, v, w, _, qv = phys_state[:6]
+    g = _grid(phys_state)
+    lo, den = norm if norm is not None else level_norm(scaling_params)
+    lo, den = lo.to(th.device), den.to(th.device)
+    out = torch.empty((g.nx, g.ny, g.nz, 5), dtype=torch.float32, device=th.device)
+    _lib.call("lex_dl_pack", g, *[_lib.ptr(a) for a in (th, u, v, w, qv, lo, den, out)], _lib.stream())
+    return out.view(1, g.nx, g.ny, g.nz, 5)
+
+
+def correct_array2tuple(correct):
+    """(nx, ny, nz, 5) correction -> (theta, u, v, w, qv) corrections on their own points
+    (dl_models.py:158-163 with stagger_uvw, :174-182)."""
+    c = correct.reshape(correct.shape[-4:]).contiguous().float()
+    nx, ny, nz = c.shape[:3]
+    g = _lib.grid(nx, ny, nz, 1.0, 1.0, 1.0)
+    dev = c.device
+    tc = torch.empty((nx, ny, nz), dtype=torch.float32, device=dev)
+    uc = torch.empty((nx + 1, ny, nz), dtype=torch.float32, device=dev)
+    vc = torch.empty((nx, ny + 1, nz), dtype=torch.float32, device=dev)
+    wc = torch.empty((nx, ny, nz + 1), dtype=torch.float32, device=dev)
+    qc = torch.empty_like(tc)
+    _lib.call("lex_dl_stagger", g, *[_lib.ptr(a) for a in (c, tc, uc, vc, wc, qc)], _lib.stream())
+    return tc, uc, vc, wc, qc
+
+
+def apply_correction(phys_state, correct, ranges):
+    """In place: theta, u, v, w, qv += staggered correction x scaling range, ghosts
+    refilled (one_sprint_functions.py:137-160)."""
+    th, u, v, w, _, qv = phys_state[:6]
+    g = _grid(phys_state)
+    c = correct.reshape(g.nx, g.ny, g.nz, 5).contiguous().float()
+    _lib.call("lex_dl_apply", g, _lib.ptr(c), *[_lib.ptr(r) for r in ranges],
+              *[_lib.ptr(a) for a in (th, u, v, w, qv)], _lib.stream())
+    return phys_state

@@ -85,6 +85,30 @@ def ws_rk3_sprint(phys_state, base_state, grids, model_opt):
     return phys_state, sfc_others
 
 
+def ws_rk3_sprint_dl(phys_state, base_state, grids, model_opt, dl_params, scaling_params):
+    """sprint_n steps, each followed by the network's post-step correction
+    (one_sprint_functions.py:118-165).  dl_params: the network (a module mapping the
+    (nx, ny, nz, 5) input to the (nx, ny, nz, 5) correction, e.g. dl_models.simpleCNN());
+    scaling_params: the reference's 14-tuple of scaling extremes (theta_min, theta_max,
+    u_min, ..., qv_la_max).  Returns (phys_state, sfc_others) as new tensors."""
+    from . import dl_models as dlm
+    state = tuple(a.clone() for a in phys_state)
+    norm = dlm.level_norm(scaling_params)
+    ranges = dlm.scaling_ranges(scaling_params, state[0].device)
+    sfc_others = None
+    with torch.no_grad():
+        for _ in range(nl.sprint_n):
+            if model_opt[0] in (2, 3) and model_opt[3] == 0:
+                fused_step(state, base_state, grid_from(base_state[1], *grids[3:6]), model_opt)
+                sfc_others = one._sfc_etc(-999, state[0])
+            else:
+                state, sfc_others = ws_rk3_big_step(state, base_state, grids, model_opt)
+            x = dlm.phys_state_tuple2array(state, scaling_params, norm)
+            corr = dl_params(x[0])
+            dlm.apply_correction(state, corr, ranges)
+    return state, sfc_others
+
+
 def get_first_step_flux(phys_state, base_state, grids, model_opt):
     """Stage-0 surface fluxes for output (one_sprint_functions.py:168-174); zero with sfc_opt off."""
     return one._sfc_etc(-999, phys_state[0])

@@ -1180,6 +1180,94 @@ def coarsen_face_mean(a, axis, fx, fy, fz):
     return b.reshape(new).mean(axis=tuple(red))
 
 
+# ---------------------------------------------------------------------------------
+# The reference's post-step deep-learning correction (one_sprint_functions.py:118-165,
+# dl_models.py:125-182).  The network itself is a parameter (net_fn), so these restate
+# only the data maps and the correction loop around it.
+# ---------------------------------------------------------------------------------
+def dl_level_norm(scaling_params):
+    """Per-level offsets and denominators of phys_state_tuple2array (dl_models.py:127-151):
+    for theta, u, v, w, qv: lo = min over x, y of the squeezed minimum field, den = max
+    over x, y of the squeezed maximum field - lo + 0.01 (1e-5 for qv)."""
+    mins = scaling_params[0:10:2]
+    maxs = scaling_params[1:10:2]
+    out = []
+    for q, (mn, mx) in enumerate(zip(mins, maxs)):
+        lo = np.min(np.squeeze(mn), axis=(0, 1))
+        c = 1.0e-5 if q == 4 else 0.01
+        den = np.max(np.squeeze(mx), axis=(0, 1)) - lo + c
+        out.append((lo, den))
+    return out
+
+
+def unstagger_uvw(u, v, w):
+    """Face values to cell centres (dl_models.py:166-171)."""
+    return 0.5 * (u[:-1] + u[1:]), 0.5 * (v[:, :-1] + v[:, 1:]), 0.5 * (w[:, :, :-1] + w[:, :, 1:])
+
+
+def stagger_uvw(ui, vi, wi):
+    """Centres back to faces (dl_models.py:174-182): interior faces average their two
+    centres; the first x (y) face takes the last interior face's value and the last face
+    the first's; w is zero at the surface and the lid."""
+    uc = 0.5 * (ui[:-1] + ui[1:])
+    vc = 0.5 * (vi[:, :-1] + vi[:, 1:])
+    wc = 0.5 * (wi[:, :, :-1] + wi[:, :, 1:])
+    u = np.concatenate([uc[-1:], uc, uc[:1]], 0)
+    v = np.concatenate([vc[:, -1:], vc, vc[:, :1]], 1)
+    z = np.zeros(wi.shape[:2] + (1,), wi.dtype)
+    w = np.concatenate([z, wc, z], 2)
+    return u, v, w
+
+
+def phys_state_tuple2array(phys_state, scaling_params):
+    """Normalised, centred (nx, ny, nz, 5) network input (dl_models.py:125-155)."""
+    th, u, v, w, _, qv = phys_state[:6]
+    (lt, dt_), (lu, du), (lv, dv), (lw, dw), (lq, dq) = dl_level_norm(scaling_params)
+    thn = (_interior(th) - lt) / dt_
+    un = (u[NGX:-NGX, NGY:-NGY, NGZ:-NGZ] - lu) / du
+    vn = (v[NGX:-NGX, NGY:-NGY, NGZ:-NGZ] - lv) / dv
+    wn = (w[NGX:-NGX, NGY:-NGY, NGZ:-NGZ] - lw) / dw
+    qn = (_interior(qv) - lq) / dq
+    ui, vi, wi = unstagger_uvw(un, vn, wn)
+    return np.stack([thn, ui, vi, wi, qn], axis=3)
+
+
+def correct_array2tuple(correct):
+    """(nx, ny, nz, 5) -> staggered corrections of theta, u, v, w, qv (dl_models.py:158-163)."""
+    tc, uic, vic, wic, qc = (correct[..., q] for q in range(5))
+    uc, vc, wc = stagger_uvw(uic, vic, wic)
+    return tc, uc, vc, wc, qc
+
+
+def dl_apply_correction(phys_state, correct, scaling_params):
+    """The correction step of ws_rk3_sprint_dl (one_sprint_functions.py:137-160): each
+    field's interior plus its correction times the scaling range, padded again; pi'
+    unchanged."""
+    th, u, v, w, pip, qv = phys_state[:6]
+    tc, uc, vc, wc, qc = correct_array2tuple(correct)
+    mins = scaling_params[0:10:2]
+    maxs = scaling_params[1:10:2]
+    rng = [np.squeeze(mx - mn) for mn, mx in zip(mins, maxs)]
+    thn = _interior(th) + tc * rng[0]
+    un = u[NGX:-NGX, NGY:-NGY, NGZ:-NGZ] + uc * rng[1]
+    vn = v[NGX:-NGX, NGY:-NGY, NGZ:-NGZ] + vc * rng[2]
+    wn = w[NGX:-NGX, NGY:-NGY, NGZ:-NGZ] + wc * rng[3]
+    qn = _interior(qv) + qc * rng[4]
+    return (padding3_array(thn), padding3u_array(un), padding3v_array(vn), padding3_array(wn), pip,
+            padding3_array(qn)) + tuple(phys_state[6:])
+
+
+def ws_rk3_sprint_dl(xi, base, grid, opts, net_fn, scaling_params, sprint_n):
+    """ws_rk3_sprint_dl (one_sprint_functions.py:118-165): every step followed by the
+    network's correction; net_fn maps the (nx, ny, nz, 5) input to the (nx, ny, nz, 5)
+    correction."""
+    for _ in range(sprint_n):
+        xi, _ = ws_rk3_big_step(xi, base, grid, opts)
+        corr = net_fn(phys_state_tuple2array(xi, scaling_params))
+        xi = dl_apply_correction(xi, corr, scaling_params)
+    return xi
+
+
 def n_acoustic_total(opts):
     """Acoustic substeps per full step: 1 + round(n/2) + n (solver_opt 2)."""
     return 1 + _n_acoustic(1, opts.dt / 2.0, opts)[0] + _n_acoustic(2, opts.dt, opts)[0]
