@@ -190,6 +190,32 @@ def dl_sprint_timing(sprint_n=10):
             "cell_steps_per_s": ncell / (t_dl / 1e3), "state_finite": all(bool(torch.isfinite(a).all()) for a in out)}
 
 
+def physics_timing(steps=10):
+    """The 256x256x128 step with the upper-level Rayleigh damping (layer above 9.6 km of
+    the 12.8 km column) and the Newtonian heating switched on, against the plain step."""
+    import torch
+    from lex_amd import _lib, namelist_n_constants as nl, one_sprint_functions as sp
+    from lex_amd.setup_lex import build_case, config1
+    case, opts = config1()
+    phys, base, grids, _ = build_case(case)
+    nl.dt, nl.n_sound, nl.solver_opt, nl.turb_opt = opts["dt"], 7, 2, 1
+    g = _lib.grid(case.nx, case.ny, case.nz, case.dx, case.dy, case.dz)
+    out = {}
+    for name, mo in (("plain", (2, False, False, 0, False, False, 1)), ("damp_rad", (2, True, True, 0, False, False, 1))):
+        st = tuple(a.clone() for a in phys)
+        sp.fused_step(st, base, g, mo)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(steps):
+            sp.fused_step(st, base, g, mo)
+        e1.record()
+        torch.cuda.synchronize()
+        out[name + "_ms_per_step"] = e0.elapsed_time(e1) / steps
+    out["workload"] = f"{case.nx}x{case.ny}x{case.nz}, n_sound 7, z_damping {nl.z_damping} m, rd_alpha {nl.rd_alpha:.5f}"
+    return out
+
+
 def copy_bandwidth(mib=1024, reps=5):
     """Device-to-device copy of `mib` MiB (read + write), the practical HBM ceiling."""
     import torch
@@ -316,9 +342,10 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, copy a 256 MiB buffer 3 times (a kernel of known HBM "
                          "traffic, used to scale rocprof FETCH_SIZE/WRITE_SIZE counters)")
-    ap.add_argument("--only", choices=("adjoint", "sgs-net", "output", "dl-sprint"), default=None,
+    ap.add_argument("--only", choices=("adjoint", "sgs-net", "output", "dl-sprint", "physics"), default=None,
                     help="run only the forward+adjoint workload, the SGS-net timing, the output-file "
-                         "timing or the post-step-correction sprint and print its JSON")
+                         "timing, the post-step-correction sprint or the damping/heating step and print its "
+                         "JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -332,7 +359,7 @@ def main():
     _lib.load()
     if args.only is not None:
         fn = {"adjoint": fwd_adjoint, "sgs-net": sgs_net_timing, "output": output_timing,
-              "dl-sprint": dl_sprint_timing}[args.only]
+              "dl-sprint": dl_sprint_timing, "physics": physics_timing}[args.only]
         print(json.dumps(fn()))
         return
     case, opts = config1()

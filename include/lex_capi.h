@@ -170,6 +170,12 @@ typedef struct lex_step_opts {
   int impl;          /* 0 = fused stage kernels (default), 1 = one kernel per reference routine */
   int solver;        /* acoustic substeps: 0 or 2 = forward-backward (solver_opt 2), 3 = vertically
                         implicit w / pi' (solver_opt 3, one_step_integration.py:559-873) */
+  int damp;          /* 1 = upper-level Rayleigh damping after every stage (damp_opt,
+                        boundary_conditions.py:119-141, one_step_integration.py:164-175) */
+  int rad;           /* 1 = Newtonian heating of theta (rad_opt, radiation.py:7-16), computed from
+                        the stage-0 theta and kept for the step (one_step_integration.py:31-35) */
+  float z_damping;   /* height (m) above which the damping acts (namelist z_damping, 9600) */
+  float rd_alpha;    /* inverse e-folding time (1/s) of the damping (namelist rd_alpha, 1/120) */
 } lex_step_opts_t;
 
 typedef struct lex_base_cols {
@@ -180,8 +186,9 @@ typedef struct lex_base_cols {
   const float* rho0_theta0;
 } lex_base_cols_t;
 
-/* ws_rk3_big_step (one_sprint_functions.py:10-30) for solver_opt 2: advances the state
- * in place.  state[] = theta, u, v, w, pip, qv, then n_passive scalars (all padded). */
+/* ws_rk3_big_step (one_sprint_functions.py:10-30) for solver_opt 2 and 3: advances the
+ * state in place.  state[] = theta, u, v, w, pip, qv, then n_passive scalars (all
+ * padded).  The adjoint entry points below reject damp and rad. */
 size_t lex_rk3_workspace_bytes(const lex_grid_t* g, int n_passive);
 int lex_rk3_step(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b, float* const* state,
                  void* workspace, size_t workspace_bytes, void* stream);

@@ -29,7 +29,8 @@ class Grid(C.Structure):
 
 class StepOpts(C.Structure):
     _fields_ = [("dt", C.c_float), ("n_sound", C.c_int), ("turb", C.c_int), ("cs", C.c_float),
-                ("prandtl", C.c_float), ("n_passive", C.c_int), ("impl", C.c_int), ("solver", C.c_int)]
+                ("prandtl", C.c_float), ("n_passive", C.c_int), ("impl", C.c_int), ("solver", C.c_int),
+                ("damp", C.c_int), ("rad", C.c_int), ("z_damping", C.c_float), ("rd_alpha", C.c_float)]
 
 
 class BaseCols(C.Structure):

@@ -181,6 +181,7 @@ void rk3_forward_tape(const Dims& d, const O& o, const B& b, float* const* X0, c
       hipLaunchKernelGGL(k_mom_w, fg, fb, mom_w_lds(d), st, d, rc, b.theta0, b.qv0, src[1], src[2], src[3], src[0],
                          src[5], turb ? (const float*)sw : nullptr, rw);
       ScalarSet S;
+      S.heat = nullptr;
       S.n = nsc;
       for (int q = 0; q < nsc; ++q) {
         int fi = (q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2));

@@ -492,6 +492,7 @@ int lex_rk3_step(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_c
   if (o->impl < 0 || o->impl > 2) return fail(-1, "lex_rk3_step: impl must be 0, 1 or 2%s");
   if (o->solver != 0 && o->solver != 2 && o->solver != 3) return fail(-1, "lex_rk3_step: solver must be 0, 2 or 3%s");
   if (o->solver == 3 && (g->nz < 2 || g->nz > 256)) return fail(-1, "lex_rk3_step: solver 3 needs nz 2..256%s");
+  if (o->damp && g->nz > 256) return fail(-1, "lex_rk3_step: damping needs nz <= 256%s");
   if (o->impl != 1 && (64 * ((g->nz + 63) / 64)) * kFTY > 1024)
     return fail(-1, "lex_rk3_step: the fused kernels need nz <= 256 (use impl 1)%s");
   int rc = rk3_step(d, *o, *b, state, (char*)ws, L, S(stream));
@@ -681,6 +682,7 @@ int lex_rk3_step_vjp(const lex_grid_t* g, const lex_step_opts_t* o, const lex_ba
       return fail(-1, "lex_rk3_step_vjp: lam_in must not alias lam_out or the state%s");
   }
   if (o->solver == 3) return fail(-1, "lex_rk3_step_vjp: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
+  if (o->damp || o->rad) return fail(-1, "lex_rk3_step_vjp: the adjoint covers damp_opt and rad_opt off%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (ws_bytes < L.total) return fail(-3, "lex_rk3_step_vjp: workspace too small%s");
@@ -714,6 +716,7 @@ int lex_rk3_step_tape(const lex_grid_t* g, const lex_step_opts_t* o, const lex_b
     if (state[q] == state_out[q]) return fail(-1, "lex_rk3_step_tape: state_out must not alias state%s");
   }
   if (o->solver == 3) return fail(-1, "lex_rk3_step_tape: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
+  if (o->damp || o->rad) return fail(-1, "lex_rk3_step_tape: the adjoint covers damp_opt and rad_opt off%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_tape: tape too small%s");
@@ -735,6 +738,7 @@ int lex_rk3_step_vjp_taped(const lex_grid_t* g, const lex_step_opts_t* o, const 
       return fail(-1, "lex_rk3_step_vjp_taped: lam_in must not alias lam_out or the state%s");
   }
   if (o->solver == 3) return fail(-1, "lex_rk3_step_vjp_taped: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
+  if (o->damp || o->rad) return fail(-1, "lex_rk3_step_vjp_taped: the adjoint covers damp_opt and rad_opt off%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_vjp_taped: tape too small%s");
@@ -761,6 +765,7 @@ int lex_rk3_step_tape_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o, con
   for (int q = 0; q < 2 + o->n_passive; ++q)
     if (!ss[q]) return fail(-2, "lex_rk3_step_tape_ext_sgs: null scalar SGS pointer%s");
   if (o->solver == 3) return fail(-1, "lex_rk3_step_tape_ext_sgs: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
+  if (o->damp || o->rad) return fail(-1, "lex_rk3_step_tape_ext_sgs: the adjoint covers damp_opt and rad_opt off%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_tape_ext_sgs: tape too small%s");
@@ -797,6 +802,7 @@ int lex_rk3_step_vjp_taped_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o
   for (int q = 0; q < 2 + o->n_passive; ++q)
     if (!lss[q]) return fail(-2, "lex_rk3_step_vjp_taped_ext_sgs: null scalar SGS adjoint pointer%s");
   if (o->solver == 3) return fail(-1, "lex_rk3_step_vjp_taped_ext_sgs: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
+  if (o->damp || o->rad) return fail(-1, "lex_rk3_step_vjp_taped_ext_sgs: the adjoint covers damp_opt and rad_opt off%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_vjp_taped_ext_sgs: tape too small%s");

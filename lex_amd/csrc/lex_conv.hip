@@ -72,23 +72,32 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(int nx, int ny, int nz, const
   f32x4 acc[CTY][NT];
   for (int a = 0; a < CTY; ++a)
     for (int b = 0; b < NT; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const int kl = 8 * g;  // this lane's first k within a step
+  // the weight fragments of step s+1 are fetched while step s runs
+  bf16x8 bw[NT], bwn[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) bw[nt] = *(const bf16x8*)(Wp + (long)(nt * 16 + m) * (NSTEP * 32) + kl);
   for (int s = 0; s < NSTEP; ++s) {
-    const int kl = 8 * g;                // this lane's first k within the step
     int tap = s * TPS + kl / CIN;
     const int ch0 = kl % CIN;
     if (tap > 26) tap = 26;              // padded K-step: weights are zero there
     const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
-    bf16x8 bw[NT];
+    const int sn = (s + 1 < NSTEP) ? s + 1 : s;
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt)
-      bw[nt] = *(const bf16x8*)(Wp + (long)(nt * 16 + m) * (NSTEP * 32) + s * 32 + kl);
+      bwn[nt] = *(const bf16x8*)(Wp + (long)(nt * 16 + m) * (NSTEP * 32) + sn * 32 + kl);
+    bf16x8 a[CTY];
 #pragma unroll
     for (int mt = 0; mt < CTY; ++mt) {
       const int cell = ((w + 1 + dx) * HY + (mt + 1 + dy)) * HZ + (m + 1 + dz);
-      bf16x8 a = *(const bf16x8*)(tile + cell * CIN + ch0);
-#pragma unroll
-      for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma16(a, bw[nt], acc[mt][nt]);
+      a[mt] = *(const bf16x8*)(tile + cell * CIN + ch0);
     }
+#pragma unroll
+    for (int mt = 0; mt < CTY; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma16(a[mt], bw[nt], acc[mt][nt]);
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) bw[nt] = bwn[nt];
   }
   const int xc = x0 + w;
 #pragma unroll

@@ -72,6 +72,7 @@ struct ScalarSet {
   const float* sgs[kMaxScalars];  // frozen SGS tendencies (interior) or null
   float* out[kMaxScalars];        // next-stage fields (padded, halos filled)
   float weps[kMaxScalars];
+  const float* heat = nullptr;    // diabatic heating added to scalar 0's tendency (interior) or null
   int n;
 };
 
@@ -150,6 +151,7 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
       s0v[q] = S.s0[q][c];
       sgv[q] = S.sgs[q] ? S.sgs[q][oint] : 0.0f;
     }
+    const float hv = S.heat ? S.heat[oint] : 0.0f;
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
       const float* s = S.s[q];
@@ -184,6 +186,7 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
         float conv = -((fxhi - fxlo[q]) * d.idx + (FY[(ty + 1) * nzl + k] - FY[ty * nzl + k]) * d.idy +
                        (fzhi - FZ[ty * nzl + k]) * d.idz);
         float tend = (conv + qc * fd) / r;
+        if (q == 0 && S.heat) tend = tend + hv;
         float nxt = s0v[q] + tend * dt;
         if (S.sgs[q]) nxt = nxt + sgv[q] * dt;
         store_c_fast(d, S.out[q], i, j, k, nxt);
@@ -644,6 +647,7 @@ __global__ void __launch_bounds__(512) k_stage_scalars_db(Dims d, const float* _
         float conv = -((fxhi[q] - fxlo[q]) * d.idx + (FY[(ty + 1) * nzl + k] - FY[ty * nzl + k]) * d.idy +
                        (fzhi - FZ[ty * nzl + k]) * d.idz);
         float tend = (conv + qcs[q] * fd) / r;
+        if (q == 0 && S.heat) tend = tend + S.heat[((long)i * d.ny + j) * d.nz + k];
         float nxt = S.s0[q][c] + tend * dt;
         if (S.sgs[q]) nxt = nxt + S.sgs[q][((long)i * d.ny + j) * d.nz + k] * dt;
         store_c_fast(d, S.out[q], i, j, k, nxt);

@@ -65,6 +65,101 @@ __global__ void k_copy(long n, const float* __restrict__ a, float* __restrict__ 
   if (t < n) b[t] = a[t];
 }
 
+// Newtonian heating (newton_radiation, radiation.py:7-16): relaxation of theta to the
+// base-state theta0 over 12 h, the tendency capped at +-1 K / 12 h beyond +-1 K.
+__global__ void k_newton_heating(Dims d, const float* __restrict__ th, const float* __restrict__ th0c,
+                                 float* __restrict__ heat) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = blockIdx.z;
+  if (k >= d.nz || j >= d.ny) return;
+  const float tau = (float)(12.0 * 3600.0);
+  const float dth = th[ic(d, i + 3, j + 3, k + 1)] - th0c[k + 1];
+  float t = -dth / tau;
+  t = (dth > 1.0f) ? -1.0f / tau : t;
+  t = (dth < -1.0f) ? 1.0f / tau : t;
+  heat[((long)i * d.ny + j) * d.nz + k] = t;
+}
+
+// Horizontal mean of theta per level (rayleigh_damping's theta_avg,
+// boundary_conditions.py:136): block i sums its x-plane over y for every level (lanes
+// along z, 4 rows of threads taking every 4th y, FP64, combined in a fixed order), one
+// block then adds the nx partials per level in order.
+constexpr int kMeanRows = 4;
+__global__ void __launch_bounds__(1024) k_level_partial(Dims d, const float* __restrict__ th,
+                                                        double* __restrict__ part) {
+  __shared__ double red[kMeanRows][256];
+  const int k = threadIdx.x, r = threadIdx.y;
+  const int i = blockIdx.x;
+  const int kc = k < d.nz ? k : d.nz - 1;
+  double s0 = 0.0, s1 = 0.0;
+  int j = r;
+  for (; j + kMeanRows < d.ny; j += 2 * kMeanRows) {
+    s0 += (double)th[ic(d, i + 3, j + 3, kc + 1)];
+    s1 += (double)th[ic(d, i + 3, j + kMeanRows + 3, kc + 1)];
+  }
+  if (j < d.ny) s0 += (double)th[ic(d, i + 3, j + 3, kc + 1)];
+  red[r][k] = s0 + s1;
+  __syncthreads();
+  if (r == 0 && k < d.nz) {
+    double sum = red[0][k];
+    for (int q = 1; q < kMeanRows; ++q) sum += red[q][k];
+    part[(long)i * d.nz + k] = sum;
+  }
+}
+__global__ void k_level_mean(Dims d, const double* __restrict__ part, float* __restrict__ mean) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.nz) return;
+  double sum = 0.0;
+  for (int i = 0; i < d.nx; ++i) sum += part[(long)i * d.nz + k];
+  mean[k] = (float)(sum / ((double)d.nx * (double)d.ny));
+}
+
+// Ramp of the damping layer at height z (setup_damping_tau, setup_lex.py:149-156): zero
+// up to z_damping, 0.5 (1 - cos(pi (z - z_damping) / depth)) above, depth measured to the
+// top w level of the padded grid, (nz + 1) dz.
+__device__ __forceinline__ float damp_tau(double z, double zd, double depth) {
+  return (z <= zd) ? 0.0f : (float)(0.5 * (1.0 - cos(M_PI * (z - zd) / depth)));
+}
+
+// Upper-level Rayleigh damping (rayleigh_damping, boundary_conditions.py:119-141) applied
+// to the stage result in place (one_step_integration.py:164-175): u, v, w relax to zero and
+// theta to its horizontal mean with rate alpha x ramp, ghosts refilled.
+// Levels below k0 have a zero ramp, where the update leaves every value unchanged, so the
+// launch covers levels k0..nz-1 only.
+__global__ void k_rayleigh(Dims d, int k0, double zd, float alpha, float dt, const float* __restrict__ mean,
+                           float* __restrict__ th, float* __restrict__ u, float* __restrict__ v,
+                           float* __restrict__ w) {
+  const TileId tb = xcd_tile();
+  const int k = k0 + tb.x * blockDim.x + threadIdx.x;
+  const int j = tb.y * blockDim.y + threadIdx.y;
+  const int i = tb.z;
+  if (k >= d.nz || j >= d.ny) return;
+  const int I = i + 3, J = j + 3, K = k + 1;
+  const double dz = (double)d.dz, depth = (d.nz + 1) * dz - zd;
+  const float th_k = damp_tau((K - 0.5) * dz, zd, depth);   // cell centre, padded level K
+  const float tf_k = damp_tau((double)k * dz, zd, depth);   // w level k
+  const float tf_top = damp_tau((double)d.nz * dz, zd, depth);
+  const float hu = 0.5f * (th_k + th_k);
+  const float un = u[iu(d, I, J, K)], vn = v[iv(d, I, J, K)], wn = w[iw(d, I, J, K)], tn = th[ic(d, I, J, K)];
+  store_u_fast(d, u, i, j, k, un + (-alpha * (hu * un)) * dt);
+  if (i == d.nx - 1) {
+    const float ue = u[iu(d, d.nx + 3, J, K)];
+    store_u(d, u, d.nx, j, k, ue + (-alpha * (hu * ue)) * dt);
+  }
+  store_v_fast(d, v, i, j, k, vn + (-alpha * (hu * vn)) * dt);
+  if (j == d.ny - 1) {
+    const float ve = v[iv(d, I, d.ny + 3, K)];
+    store_v(d, v, i, d.ny, k, ve + (-alpha * (hu * ve)) * dt);
+  }
+  store_w_fast(d, w, i, j, k, wn + (-alpha * (tf_k * wn)) * dt);
+  if (k == d.nz - 1) {
+    const float wt = w[iw(d, I, J, d.nz + 1)];
+    store_w(d, w, i, j, d.nz, wt + (-alpha * (tf_top * wt)) * dt);
+  }
+  store_c_fast(d, th, i, j, k, tn + (-alpha * th_k * (tn - mean[k])) * dt);
+}
+
 struct Rk3Layout {
   size_t nc, nu, nv, nw;        // padded sizes (elements)
   size_t iu_, iv_, iw_, ic_;    // interior sizes (u, v, w points, scalar)
@@ -74,6 +169,8 @@ struct Rk3Layout {
   size_t off_ru, off_rv, off_rw;
   size_t off_su, off_sv, off_sw, off_ss;   // frozen SGS tendencies (ss: 2+np scalars)
   size_t off_smag;              // 10 padded scratch fields
+  size_t off_heat;              // Newtonian heating of theta (interior), rad
+  size_t off_lmean;             // per-level partial sums (nx x nz doubles) and means (nz floats), damp
   size_t total;
   int np;
 };
@@ -108,6 +205,8 @@ inline Rk3Layout rk3_layout(const Dims& d, int np) {
   L.off_sw = o; o += al256(L.iw_ * 4);
   L.off_ss = o; o += (2 + np) * al256(L.ic_ * 4);
   L.off_smag = o; o += 10 * al256(L.nc * 4);
+  L.off_heat = o; o += al256(L.ic_ * 4);
+  L.off_lmean = o; o += al256((size_t)d.nx * d.nz * 8) + al256((size_t)d.nz * 4);
   L.total = o;
   return L;
 }
@@ -142,6 +241,10 @@ struct StepOpts {
   float cs;
   float prandtl;
   int n_passive;
+  int damp = 0;
+  int rad = 0;
+  float z_damping = 9600.0f;
+  float rd_alpha = 1.0f / 120.0f;
 };
 
 template <class O, class B>
@@ -169,6 +272,9 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
   for (int q = 0; q < 2 + o.n_passive; ++q) ss[q] = (float*)(ws + L.off_ss + q * al256(L.ic_ * 4));
   float* sm[10];
   for (int q = 0; q < 10; ++q) sm[q] = (float*)(ws + L.off_smag + q * al256(L.nc * 4));
+  float* heat = o.rad ? (float*)(ws + L.off_heat) : nullptr;
+  double* lpart = (double*)(ws + L.off_lmean);
+  float* lmean = (float*)(ws + L.off_lmean + al256((size_t)d.nx * d.nz * 8));
   const dim3 bk(64, 4, 1);
   auto G = [&](int a, int bb, int c) { return dim3((a + 63) / 64, (bb + 3) / 4, c); };
   const float* rc = b.rho0;
@@ -179,6 +285,9 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
     const double dt_s = (s == 0) ? dtd / 3.0 : (s == 1 ? dtd / 2.0 : dtd);
     float** src = (s == 0) ? X0 : (s == 1 ? XA : XB);
     float** dst = (s == 0) ? XA : (s == 1 ? XB : X0);
+    // diabatic heating from the stage-0 theta, kept for the step (one_step_integration.py:31-35)
+    if (s == 0 && o.rad)
+      LEX_LAUNCH(KID_HEATING, k_newton_heating, G(d.nz, d.ny, d.nx), bk, 0, st, d, X0[0], b.theta0, heat);
     // turb 1: Smagorinsky computed here in stage 0; turb 2: SGS tendencies supplied by
     // the caller in the workspace slots (the learned SGS net); both frozen over the step
     const bool turb = o.turb != 0;
@@ -245,6 +354,7 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
         S.out[q] = dst[fi];
         S.weps[q] = (q == 0) ? 1.0e-17f : 1.0e-18f;
       }
+      S.heat = heat;
       const int nzl = 64 * ((d.nz + 63) / 64);
       dim3 fb(nzl, kFTY, 1), fg((d.nx + kFTX - 1) / kFTX, (d.ny + kFTY - 1) / kFTY, 1);
       size_t lds = stage_scalars_lds(d);
@@ -324,9 +434,20 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
         int fi = (q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2));
         float weps = (q == 0) ? 1.0e-17f : 1.0e-18f;
         LEX_LAUNCH(KID_SCALAR_UPDATE, k_scalar_update, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, X0[fi], src[fi], src[1],
-                   src[2], src[3], fd, (const float*)nullptr, (const float*)nullptr, turb ? ss[q] : nullptr, dst[fi],
-                   (float*)nullptr, (float)dt_s, weps);
+                   src[2], src[3], fd, (const float*)nullptr, (q == 0) ? (const float*)heat : nullptr,
+                   turb ? ss[q] : nullptr, dst[fi], (float*)nullptr, (float)dt_s, weps);
       }
+    }
+    // upper-level Rayleigh damping of the stage result (one_step_integration.py:164-175)
+    if (o.damp) {
+      LEX_LAUNCH(KID_DAMPING, k_level_partial, dim3(d.nx), dim3(64 * ((d.nz + 63) / 64), kMeanRows), 0, st, d,
+                 (const float*)dst[0], lpart);
+      LEX_LAUNCH(KID_DAMPING, k_level_mean, dim3((d.nz + 63) / 64), dim3(64), 0, st, d, (const double*)lpart, lmean);
+      // first level whose cell centre or lower w level can lie above z_damping
+      int k0 = (int)floor((double)o.z_damping / (double)d.dz - 0.5);
+      k0 = k0 < 0 ? 0 : (k0 > d.nz - 1 ? d.nz - 1 : k0);
+      LEX_LAUNCH(KID_DAMPING, k_rayleigh, G(d.nz - k0, d.ny, d.nx), bk, 0, st, d, k0, (double)o.z_damping,
+                 o.rd_alpha, (float)dt_s, (const float*)lmean, dst[0], dst[1], dst[2], dst[3]);
     }
   }
   hipError_t e = hipGetLastError();

@@ -39,6 +39,10 @@ damp_opt = False
 rad_opt = False
 pic_opt = False
 
+# Rayleigh damping layer (namelist_n_constants.py:54-55)
+z_damping = 9600.0
+rd_alpha = 1.0 / 120.0
+
 # Smagorinsky constants (turbulence.py:432-433 hard-codes them; runtime values here)
 cs = 0.18
 prandtl = 1.0 / 3.0

@@ -34,14 +34,15 @@ def fused_step(phys_state, base_state, grid, model_opt, dt=None, n_sound=None, c
     device.  impl 0 runs the fused stage kernels, impl 1 one kernel per reference routine;
     solver_opt 3 takes the vertically implicit acoustic substeps.
     """
-    solver_opt, cor_opt, turb_opt = one._check_opts(model_opt)
+    solver_opt, cor_opt, turb_opt = one._check_opts(model_opt, physics=True)
     if solver_opt not in (2, 3) or cor_opt != 0:
         raise NotImplementedError("the fused step covers solver_opt 2 and 3 without Coriolis")
     rho0_theta0, rho0, theta0, pi0, qv0 = base_state[:5]
     n_passive = len(phys_state) - 6
     opts = _lib.StepOpts(float(nl.dt if dt is None else dt), int(nl.n_sound if n_sound is None else n_sound),
                          int(turb_opt), float(nl.cs if cs is None else cs),
-                         float(nl.prandtl if prandtl is None else prandtl), n_passive, int(impl), int(solver_opt))
+                         float(nl.prandtl if prandtl is None else prandtl), n_passive, int(impl), int(solver_opt),
+                         int(bool(model_opt[1])), int(bool(model_opt[2])), float(nl.z_damping), float(nl.rd_alpha))
     cols = [column(a) for a in (rho0, theta0, qv0, pi0, rho0_theta0)]
     bc = _lib.BaseCols(*[c.data_ptr() for c in cols])
     ptrs = (ctypes.c_void_p * len(phys_state))(*[_lib.ptr(a) for a in phys_state])

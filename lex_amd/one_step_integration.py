@@ -217,13 +217,19 @@ def _theta_rho(theta, qv):
 
 
 # ------------------------------------------------------------------ R1 stages
-def _check_opts(model_opt):
+def _check_opts(model_opt, physics=False):
+    """Validates model_opt; physics=True (the device step driver) also accepts damp_opt
+    and rad_opt, which the per-routine stage functions here do not implement."""
     solver_opt, damp_opt, rad_opt, cor_opt, sfc_opt, pic_opt, turb_opt = model_opt
     if solver_opt not in (1, 2, 3):
         raise NotImplementedError(f"solver_opt {solver_opt} does not exist in the reference")
-    for name, flag in (("damp_opt", damp_opt), ("rad_opt", rad_opt), ("sfc_opt", sfc_opt), ("pic_opt", pic_opt)):
+    flags = (("sfc_opt", sfc_opt), ("pic_opt", pic_opt))
+    if not physics:
+        flags = (("damp_opt", damp_opt), ("rad_opt", rad_opt)) + flags
+    for name, flag in flags:
         if flag:
-            raise NotImplementedError(f"{name} is off in every configuration on this path")
+            raise NotImplementedError(f"{name} is not built on this path"
+                                      + ("" if physics else " (damp_opt and rad_opt: use the fused step)"))
     if turb_opt not in (0, 1):
         raise NotImplementedError("turb_opt 2 (anisotropic Smagorinsky) is not on this path")
     return solver_opt, cor_opt, turb_opt

@@ -92,6 +92,10 @@ class Options:
     bicg_tol: float = 1.0e-4    # one_step_integration.py:475
     bicg_atol: float = 1.0e-8   # one_step_integration.py:476
     bicg_maxiter: int = 100     # one_step_integration.py:485
+    damp_opt: bool = False      # upper-level Rayleigh damping (boundary_conditions.py:119-141)
+    rad_opt: bool = False       # Newtonian heating (radiation.py:7-16)
+    z_damping: float = 9600.0   # namelist_n_constants.py:54
+    rd_alpha: float = 1.0 / 120.0  # namelist_n_constants.py:55
 
 
 # ---------------------------------------------------------------------------------
@@ -902,7 +906,45 @@ def _n_acoustic(stage, dt_stage, opts):
     return int(round(dt_stage / dts)), dts
 
 
-def rk_sub_step(stage, xi0, xi, base, grid, opts, dt_stage, frozen=None):
+def setup_damping_tau(grid, z_damping):
+    """Ramp of the damping layer (setup_lex.py:149-156), FP64, horizontally uniform:
+    tauh on the padded cell levels (nz+2), tauf on the padded w levels (nz+3).  The depth
+    runs to the top padded w level, (nz+1) dz."""
+    nz, dz = grid.nz, grid.dz
+    z3d = np.linspace(-0.5 * dz, (nz + 0.5) * dz, nz + 2)
+    z3d4w = np.linspace(-dz, (nz + 1) * dz, nz + 3)
+    depth = z3d4w[-1] - z_damping
+    tauh = 0.5 * (1.0 - np.cos(np.pi * (z3d - z_damping) / depth))
+    tauf = 0.5 * (1.0 - np.cos(np.pi * (z3d4w - z_damping) / depth))
+    tauh = np.where(z3d <= z_damping, 0.0, tauh)
+    tauf = np.where(z3d4w <= z_damping, 0.0, tauf)
+    sh = (grid.nx + 6, grid.ny + 6)
+    return (np.broadcast_to(tauh, sh + (nz + 2,)), np.broadcast_to(tauf, sh + (nz + 3,)))
+
+
+def rayleigh_damping(tauh, tauf, u, v, w, theta, rd_alpha):
+    """Damping tendencies of u, v, w (towards zero) and theta (towards its horizontal
+    mean per level) (boundary_conditions.py:119-141)."""
+    I, J, K = slice(NGX, -NGX), slice(NGY, -NGY), slice(NGZ, -NGZ)
+    u_t = -rd_alpha * (0.5 * (tauh[NGX - 1:-NGX, J, K] + tauh[NGX:-(NGX - 1), J, K]) * u[I, J, K])
+    v_t = -rd_alpha * (0.5 * (tauh[I, NGY - 1:-NGY, K] + tauh[I, NGY:-(NGY - 1), K]) * v[I, J, K])
+    w_t = -rd_alpha * (tauf[I, J, K] * w[I, J, K])
+    th_i = theta[I, J, K]
+    avg = np.mean(th_i, axis=(0, 1), keepdims=True)
+    th_t = -rd_alpha * tauh[I, J, K] * (th_i - avg)
+    return u_t, v_t, w_t, th_t
+
+
+def newton_radiation(theta, theta0_ic):
+    """Newtonian relaxation of theta to theta0 over 12 h, capped beyond +-1 K
+    (radiation.py:7-16); interior."""
+    t = -(theta - theta0_ic) / (12.0 * 3600.0)
+    t = np.where(theta - theta0_ic > 1.0, -1.0 / (12.0 * 3600.0), t)
+    t = np.where(theta - theta0_ic < -1.0, 1.0 / (12.0 * 3600.0), t)
+    return t[NGX:-NGX, NGY:-NGY, NGZ:-NGZ]
+
+
+def rk_sub_step(stage, xi0, xi, base, grid, opts, dt_stage, frozen=None, heating=None):
     """One WS-RK3 stage (rk_sub_step0/1/2, one_step_integration.py:16-188, :191-317, :320-447).
 
     State tuples are (theta, u, v, w, pip, qv, *passive) with optional passive scalars
@@ -956,7 +998,8 @@ def rk_sub_step(stage, xi0, xi, base, grid, opts, dt_stage, frozen=None):
         n, dts = _n_acoustic(stage, dt_stage, opts)
         step = sound_implicit_rk_step if opts.solver_opt == 3 else sound_rk_step
         un, vn, wn, pip_n = step(u0, v0, w0, pi0, pip0, pip, rhs_u, rhs_v, rhs_w, trho, rho0, n, dts, grid)
-    thn, _ = update_scalar_euler(rho0, th0, th, u, v, w, fdiv, zeros2, 0.0, WEPS_THETA, dt_stage, grid)
+    thn, _ = update_scalar_euler(rho0, th0, th, u, v, w, fdiv, zeros2, 0.0 if heating is None else heating,
+                                 WEPS_THETA, dt_stage, grid)
     qvn, _ = update_scalar_euler(rho0, qv0_now, qv, u, v, w, fdiv, zeros2, 0.0, WEPS_QV, dt_stage, grid)
     pasn = tuple(update_scalar_euler(rho0, a0, a, u, v, w, fdiv, zeros2, 0.0, WEPS_QV, dt_stage, grid)[0]
                  for a0, a in zip(pas0, pas))
@@ -964,15 +1007,26 @@ def rk_sub_step(stage, xi0, xi, base, grid, opts, dt_stage, frozen=None):
         thn = thn + padding3_array(sgs[3] * dt_stage)
         qvn = qvn + padding3_array(sgs[4] * dt_stage)
         pasn = tuple(a + padding3_array(t * dt_stage) for a, t in zip(pasn, sgs[5:]))
+    if opts.damp_opt:  # one_step_integration.py:164-175
+        tauh, tauf = setup_damping_tau(grid, opts.z_damping)
+        tauh, tauf = tauh.astype(rho0.dtype), tauf.astype(rho0.dtype)
+        ut, vt, wt, tht = rayleigh_damping(tauh, tauf, un, vn, wn, thn, opts.rd_alpha)
+        un = un + padding3u_array(ut * dt_stage)
+        vn = vn + padding3v_array(vt * dt_stage)
+        wn = wn + padding3_array(wt * dt_stage)
+        thn = thn + padding3_array(tht * dt_stage)
     del dt_
     return (thn, un, vn, wn, pip_n, qvn) + pasn, frozen, info
 
 
 def ws_rk3_big_step(xi0, base, grid, opts):
-    """Wicker-Skamarock RK3: every stage restarts from xi0 (one_sprint_functions.py:10-30)."""
-    xi1, frozen, i0 = rk_sub_step(0, xi0, xi0, base, grid, opts, opts.dt / 3.0)
-    xi2, _, i1 = rk_sub_step(1, xi0, xi1, base, grid, opts, opts.dt / 2.0, frozen)
-    xi3, _, i2 = rk_sub_step(2, xi0, xi2, base, grid, opts, opts.dt, frozen)
+    """Wicker-Skamarock RK3: every stage restarts from xi0 (one_sprint_functions.py:10-30).
+    With rad_opt the heating is computed from the stage-0 theta and kept for the step
+    (one_step_integration.py:31-35, :208, :338)."""
+    kw = {"heating": newton_radiation(xi0[0], base[2])} if opts.rad_opt else {}
+    xi1, frozen, i0 = rk_sub_step(0, xi0, xi0, base, grid, opts, opts.dt / 3.0, **kw)
+    xi2, _, i1 = rk_sub_step(1, xi0, xi1, base, grid, opts, opts.dt / 2.0, frozen, **kw)
+    xi3, _, i2 = rk_sub_step(2, xi0, xi2, base, grid, opts, opts.dt, frozen, **kw)
     return xi3, (i0, i1, i2)
 
 

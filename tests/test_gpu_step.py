@@ -95,6 +95,39 @@ def test_implicit_solver_step_vs_oracle(nsteps):
             assert rel_l2(dx, dy) < 1e-5, n
 
 
+@pytest.mark.parametrize("impl,nsteps", [(0, 1), (0, 3), (1, 2)])
+def test_damping_and_heating_step_vs_oracle(impl, nsteps):
+    """damp_opt and rad_opt (off in every workload configuration): the Rayleigh layer
+    above 1500 m of a 2400 m column and the Newtonian heating (capped inside the 2 K
+    bubble) in the device step (fused kernels and one kernel per routine) vs the oracle."""
+    from lex_amd import _lib, namelist_n_constants as nl, one_sprint_functions as sp
+    from lex_amd._core import grid_from
+    phys, base, grids, mo, og, oo = _setup(1, 24, 20, 24)
+    old = nl.z_damping
+    nl.z_damping = 1500.0
+    mo = (mo[0], True, True) + tuple(mo[3:])
+    oo.damp_opt, oo.rad_opt, oo.z_damping = True, True, 1500.0
+    try:
+        init = tuple(from_dev(a) for a in phys)
+        st = tuple(a.clone() for a in phys)
+        g = grid_from(base[1], *grids[3:6])
+        for _ in range(nsteps):
+            sp.fused_step(st, base, g, mo, impl=impl)
+        got = tuple(from_dev(a) for a in st)
+    finally:
+        nl.z_damping = old
+    r64 = _oracle_run(phys, base, og, oo, nsteps, np.float64)
+    r32 = _oracle_run(phys, base, og, oo, nsteps, np.float32)
+    errs = _compare(got, r64, r32, init)
+    print({k: f"{v[0]:.2e}/{v[1]:.2e}" for k, v in errs.items()})
+    # the options change the result (the layer and the heating are active)
+    oo.damp_opt = oo.rad_opt = False
+    plain = _oracle_run(phys, base, og, oo, nsteps, np.float64)
+    assert rel_l2(r64[0] - init[0], plain[0] - init[0]) > 1e-4
+    assert rel_l2(r64[1] - init[1], plain[1] - init[1]) > 1e-6
+    del _lib
+
+
 def test_fused_equals_staged():
     """The one-call device step and the stage-by-stage composition agree to FP32 rounding."""
     from lex_amd import one_sprint_functions as sp
