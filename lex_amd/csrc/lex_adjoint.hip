@@ -1498,6 +1498,101 @@ inline void launch_fold(const Dims& d, int mode, float* a, hipStream_t st) {
   hipLaunchKernelGGL(k_fold_edges, dim3((unsigned)((e.nwork() + 255) / 256)), dim3(256), 0, st, d, mode, a);
 }
 
+// Several folds in one launch (blockIdx.y selects the field): the same work items as
+// k_fold_edges for each field.  Fields with mode < 0 are skipped.
+constexpr int kFoldBatch = 8;
+struct FoldBatch {
+  float* a[kFoldBatch];
+  int mode[kFoldBatch];
+};
+__global__ void k_fold_edges_batch(Dims d, FoldBatch fb) {
+  const int f = blockIdx.y;
+  const int mode = fb.mode[f];
+  if (mode < 0) return;
+  float* __restrict__ a = fb.a[f];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const FoldExt e(d, mode);
+  const long nb = (long)e.nband() * e.nk;
+  if (t < nb) {
+    const int b = (int)(t / e.nk), k = (int)(t - (long)b * e.nk);
+    int i, j;
+    if (b < 8 * e.nj) {
+      const int r = b / e.nj;
+      i = r < 4 ? r : e.ni - 8 + r;
+      j = b - r * e.nj;
+    } else {
+      const int q = b - 8 * e.nj, r = q >> 3, c = q & 7;
+      i = 4 + r;
+      j = c < 4 ? c : e.nj - 8 + c;
+    }
+    fold_point(d, mode, a, i, j, k, e.nk);
+    return;
+  }
+  const long q = t - nb;
+  if (q >= e.ninner()) return;
+  const int i = 4 + (int)(q / (e.nj - 8)), j = 4 + (int)(q % (e.nj - 8));
+  float* col = a + fold_idx(d, mode, i + 3, j + 3, 0);
+  const float g0 = col[0], s1 = col[1], sn = col[e.nk], gn = col[e.nk + 1];
+  col[0] = 0.f;
+  col[e.nk + 1] = 0.f;
+  if (e.nk == 1) {
+    col[1] = s1 + (g0 + gn);
+  } else {
+    col[1] = s1 + g0;
+    col[e.nk] = sn + gn;
+  }
+}
+
+// folds of n fields (modes[q], a[q]), batched by kFoldBatch (small grids: one by one)
+inline void launch_fold_many(const Dims& d, int n, const int* modes, float* const* a, hipStream_t st) {
+  if (d.nx + 1 < 9 || d.ny + 1 < 9 || d.nx < 9 || d.ny < 9) {
+    for (int q = 0; q < n; ++q) launch_fold(d, modes[q], a[q], st);
+    return;
+  }
+  for (int q0 = 0; q0 < n; q0 += kFoldBatch) {
+    FoldBatch fb;
+    long work = 0;
+    const int m = (n - q0 < kFoldBatch) ? n - q0 : kFoldBatch;
+    for (int q = 0; q < kFoldBatch; ++q) {
+      fb.a[q] = q < m ? a[q0 + q] : nullptr;
+      fb.mode[q] = q < m ? modes[q0 + q] : -1;
+      if (q < m) {
+        const long wq = FoldExt(d, modes[q0 + q]).nwork();
+        work = wq > work ? wq : work;
+      }
+    }
+    hipLaunchKernelGGL(k_fold_edges_batch, dim3((unsigned)((work + 255) / 256), (unsigned)m), dim3(256), 0, st, d, fb);
+  }
+}
+
+// zero several buffers in one launch (blockIdx.y selects the buffer)
+struct FillBatch {
+  float* a[kFoldBatch];
+  long n[kFoldBatch];
+};
+__global__ void k_fill_batch(FillBatch fb) {
+  const int f = blockIdx.y;
+  float* __restrict__ a = fb.a[f];
+  const long n = fb.n[f];
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) a[t] = 0.0f;
+}
+inline void launch_zero_many(int n, float* const* a, const size_t* counts, hipStream_t st) {
+  for (int q0 = 0; q0 < n; q0 += kFoldBatch) {
+    FillBatch fb;
+    const int m = (n - q0 < kFoldBatch) ? n - q0 : kFoldBatch;
+    long mx = 0;
+    for (int q = 0; q < kFoldBatch; ++q) {
+      fb.a[q] = q < m ? a[q0 + q] : nullptr;
+      fb.n[q] = q < m ? (long)counts[q0 + q] : 0;
+      mx = fb.n[q] > mx ? fb.n[q] : mx;
+    }
+    long blocks = (mx + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_fill_batch, dim3((unsigned)blocks, (unsigned)m), dim3(256), 0, st, fb);
+  }
+}
+
 // ------------------------------------------------------------- E1 scalar update
 // next = pad(s0 + dt T + dt sgs) (one_step_integration.py:450-465, :158-162):
 // ls0 += lnext, lT = dt lnext, lsgs += dt lnext (lnext folded; interior cells).

@@ -284,7 +284,13 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
   const double dtd = (double)o.dt;
   const double dts_small = dtd / (double)o.n_sound;
 
-  for (int q = 0; q < nf; ++q) cx.zero(lam_in[q], field_n(L, q));
+  {
+    size_t cnt[16];
+    for (int q = 0; q < nf; ++q) cnt[q] = field_n(L, q);
+    launch_zero_many(nf, lam_in, cnt, st);
+  }
+  int fmodes[16];
+  for (int q = 0; q < nf; ++q) fmodes[q] = field_mode(q);
   // the whole adjoint scratch region (stage adjoints, rhs, SGS and Smagorinsky adjoints)
   // is contiguous: one fill
   cx.zero((float*)A(L.off_lXA), (L.off_lcg2 - L.off_lXA) / sizeof(float));
@@ -299,7 +305,7 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
     const float* ru = (const float*)(ws + L.off_ru[s]);
     const float* rv = (const float*)(ws + L.off_rv[s]);
     const float* rw = (const float*)(ws + L.off_rw[s]);
-    for (int q = 0; q < nf; ++q) cx.fold(field_mode(q), ldst[q]);
+    launch_fold_many(d, nf, fmodes, ldst, st);
     cx.zero(lru, (L.off_lsu - L.off_lru) / sizeof(float));  // lru, lrv, lrw, ltr are contiguous
     // scalars: next = pad(s0 + dt_s adv(s; u, v, w) + dt_s sgs)
     for (int q = 0; q < nsc; ++q) {
@@ -381,7 +387,10 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
     hipLaunchKernelGGL(k_smag_k_vjp, dim3(nbk), dim3(256), 0, st, n, (const float*)sm[6], (const float*)sm[7],
                        (const float*)lsm[8], (const float*)lsm[9], cg * cg, o.prandtl, lsm[6], lsm[7], kpart);
     hipLaunchKernelGGL(k_smag_k_sum, dim3(1), dim3(256), 0, st, (const double*)kpart, (int)nbk, lcg2);
-    for (int q = 0; q < 8; ++q) cx.fold(0, lsm[q]);
+    {
+      const int zm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      launch_fold_many(d, 8, zm, lsm, st);
+    }
     hipLaunchKernelGGL(k_smag_nm_vjp, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, (const float*)X0[0],
                        (const float*)X0[5], (const float*)lsm[7], lam_in[0], lam_in[5]);
     hipLaunchKernelGGL(k_smag_deform_h, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, rc, (const float*)sm[0],
@@ -396,7 +405,7 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
       hipLaunchKernelGGL(k_cs_grad, dim3(1), dim3(64), 0, st, (const double*)lcg2, factor, lam_cs);
     }
   }
-  for (int q = 0; q < nf; ++q) cx.fold(field_mode(q), lam_in[q]);
+  launch_fold_many(d, nf, fmodes, lam_in, st);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return (int)e;
   return 0;
