@@ -164,18 +164,22 @@ def dl_sprint_timing(sprint_n=10):
         hi = (inner + spread).unsqueeze(0).unsqueeze(0).contiguous()
         sc += [lo, hi]
     scaling = tuple(sc) + (None,) * 4
-    net = dlm.simpleCNN(seed=7)
-    with torch.no_grad():
-        net.weights[-1].mul_(1e-3)
-    st = tuple(a.clone() for a in phys)
-    sp.ws_rk3_sprint_dl(st, base, grids, mo, net, scaling)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record()
-    out, _ = sp.ws_rk3_sprint_dl(st, base, grids, mo, net, scaling)
-    e1.record()
-    torch.cuda.synchronize()
-    t_dl = e0.elapsed_time(e1) / sprint_n
+    t_net = {}
+    for name in ("simpleCNN", "Zcnn"):
+        net = getattr(dlm, name)(seed=7)
+        with torch.no_grad():
+            net.weights[-1].mul_(1e-3)
+        st = tuple(a.clone() for a in phys)
+        sp.ws_rk3_sprint_dl(st, base, grids, mo, net, scaling)
+        torch.cuda.synchronize()
+        e0.record()
+        out, _ = sp.ws_rk3_sprint_dl(st, base, grids, mo, net, scaling)
+        e1.record()
+        torch.cuda.synchronize()
+        t_net[name] = (e0.elapsed_time(e1) / sprint_n, all(bool(torch.isfinite(a).all()) for a in out))
+        del net, out, st
+    t_dl = t_net["simpleCNN"][0]
     st2 = tuple(a.clone() for a in phys)
     e0.record()
     for _ in range(sprint_n):
@@ -185,9 +189,11 @@ def dl_sprint_timing(sprint_n=10):
     t_plain = e0.elapsed_time(e1) / sprint_n
     ncell = case.nx * case.ny * case.nz
     return {"workload": f"ws_rk3_sprint_dl, {case.nx}x{case.ny}x{case.nz}, n_sound 7, sprint_n {sprint_n}, "
-                        "simpleCNN post-step correction",
+                        "post-step correction by simpleCNN and by Zcnn",
             "ms_per_step": t_dl, "ms_per_step_without_correction": t_plain, "correction_ms": t_dl - t_plain,
-            "cell_steps_per_s": ncell / (t_dl / 1e3), "state_finite": all(bool(torch.isfinite(a).all()) for a in out)}
+            "cell_steps_per_s": ncell / (t_dl / 1e3), "state_finite": t_net["simpleCNN"][1],
+            "zcnn_ms_per_step": t_net["Zcnn"][0], "zcnn_correction_ms": t_net["Zcnn"][0] - t_plain,
+            "zcnn_state_finite": t_net["Zcnn"][1]}
 
 
 def physics_timing(steps=10):

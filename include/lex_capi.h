@@ -283,13 +283,19 @@ int lex_rk3_step_vjp_taped(const lex_grid_t* g, const lex_step_opts_t* o, const 
  * (x, y, z, c) BF16 with the channel count padded to 16 (c <= 16) or 32; weights FP32
  * W[kx*9 + ky*3 + kz][cin][cout] (flax's (3,3,3,in,out) kernel) packed to BF16 by
  * lex_conv3d_pack_weights (flip = 1 packs the transposed convolution of the data
- * gradient).  Accumulation is FP32.  Channels 1..32. */
+ * gradient).  Accumulation is FP32.  Channels 1..64 (padded to 16, 32 or 64); the weight
+ * gradient takes 1..32. */
 size_t lex_conv3d_packed_weight_bytes(int cin, int cout, int flip);
 int lex_conv3d_pack_weights(const float* w, int cin, int cout, int flip, void* wp, void* stream);
 /* y = gelu?(conv(x) + bias): BF16 y (padded cout) or FP32 yf (cout channels); pre
  * (optional) receives the FP32 pre-activation (padded cout channels). */
 int lex_conv3d_bf16_fwd(int nx, int ny, int nz, int cin, int cout, const void* x, const void* wp, const float* bias,
                         int gelu, int out_fp32, void* y, float* yf, float* pre, void* stream);
+/* The same with periodic x and y and zero z padding: Zcnn's layers (dl_models.py:71-122),
+ * flax padding 'CIRCULAR' on a column whose z ends are held at zero. */
+int lex_conv3d_bf16_fwd_circular(int nx, int ny, int nz, int cin, int cout, const void* x, const void* wp,
+                                 const float* bias, int gelu, int out_fp32, void* y, float* yf, float* pre,
+                                 void* stream);
 /* dx (FP32, cin channels) = transposed convolution of g_pre (BF16, padded cout
  * channels) with the flip-packed weights; zero_bias: padded-cin zeros. */
 int lex_conv3d_bf16_bwd_data(int nx, int ny, int nz, int cin, int cout, const void* g_pre, const void* wp_flip,

@@ -828,27 +828,28 @@ int lex_rk3_step_vjp_taped_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o
 // ------------------------------------------------------------------ X4 conv3d (BF16 MFMA)
 }  // extern "C"
 namespace {
-inline int padc(int c) { return c <= 16 ? 16 : 32; }
-inline bool conv_ok(int nx, int ny, int nz, int cin, int cout) {
-  return nx > 0 && ny > 0 && nz > 0 && cin >= 1 && cin <= 32 && cout >= 1 && cout <= 32;
+inline int padc(int c) { return c <= 16 ? 16 : (c <= 32 ? 32 : 64); }
+// forward and data-gradient convolutions take up to 64 channels; the weight gradient 32
+inline bool conv_ok(int nx, int ny, int nz, int cin, int cout, int cmax = 32) {
+  return nx > 0 && ny > 0 && nz > 0 && cin >= 1 && cin <= cmax && cout >= 1 && cout <= cmax;
 }
+inline int conv_nstep(int cinp) { return cinp <= 32 ? (27 + 32 / cinp - 1) / (32 / cinp) : 27 * (cinp / 32); }
 }  // namespace
 extern "C" {
 
 size_t lex_conv3d_packed_weight_bytes(int cin, int cout, int flip) {
   int ci = flip ? cout : cin, co = flip ? cin : cout;
-  if (ci < 1 || ci > 32 || co < 1 || co > 32) return 0;
+  if (ci < 1 || ci > 64 || co < 1 || co > 64) return 0;
   int cip = padc(ci), cop = padc(co);
-  int tps = 32 / cip, nstep = (27 + tps - 1) / tps;
-  return (size_t)cop * nstep * 32 * 2;
+  return (size_t)cop * conv_nstep(cip) * 32 * 2;
 }
 
 int lex_conv3d_pack_weights(const float* w, int cin, int cout, int flip, void* wp, void* stream) {
   REQ(w); REQ(wp);
-  if (cin < 1 || cin > 32 || cout < 1 || cout > 32) return fail(-1, "lex_conv3d_pack_weights: channels must be 1..32%s");
+  if (cin < 1 || cin > 64 || cout < 1 || cout > 64) return fail(-1, "lex_conv3d_pack_weights: channels must be 1..64%s");
   int ci = flip ? cout : cin, co = flip ? cin : cout;
   int cip = padc(ci), cop = padc(co);
-  int tps = 32 / cip, nstep = (27 + tps - 1) / tps;
+  int nstep = conv_nstep(cip);
   long n = (long)cop * nstep * 32;
   hipLaunchKernelGGL(k_conv3_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), w, cin, cout, cip, cop,
                      nstep, flip ? 1 : 0, (__bf16*)wp);
@@ -874,25 +875,40 @@ void launch_wgrad(int nx, int ny, int nz, const void* x, const void* g, float* d
 }  // namespace
 extern "C" {
 
+static int conv_fwd(const char* name, int nx, int ny, int nz, int cin, int cout, const void* x, const void* wp,
+                    const float* bias, int mode, void* y, float* yf, float* pre, hipStream_t st) {
+  if (!conv_ok(nx, ny, nz, cin, cout, 64)) return fail(-1, "%s: bad sizes (channels 1..64)", name);
+  if ((mode & 2) ? !yf : !y) return fail(-2, "%s: missing output", name);
+  if ((mode & 4) && (nx < 2 || ny < 2)) return fail(-1, "%s: periodic padding needs nx, ny >= 2", name);
+  const int cip = padc(cin), cop = padc(cout);
+#define LEX_FWD(CI, CO) \
+  if (cip == CI && cop == CO) { launch_fwd<CI, CO>(nx, ny, nz, x, wp, bias, mode, cout, y, yf, pre, st); return check_launch(name); }
+  LEX_FWD(16, 16) LEX_FWD(16, 32) LEX_FWD(16, 64)
+  LEX_FWD(32, 16) LEX_FWD(32, 32) LEX_FWD(32, 64)
+  LEX_FWD(64, 16) LEX_FWD(64, 32) LEX_FWD(64, 64)
+#undef LEX_FWD
+  return fail(-1, "%s: unsupported channel padding", name);
+}
+
 int lex_conv3d_bf16_fwd(int nx, int ny, int nz, int cin, int cout, const void* x, const void* wp, const float* bias,
                         int gelu, int out_fp32, void* y, float* yf, float* pre, void* stream) {
   REQ(x); REQ(wp); REQ(bias);
-  if (!conv_ok(nx, ny, nz, cin, cout)) return fail(-1, "lex_conv3d_bf16_fwd: bad sizes (channels 1..32)%s");
-  if (out_fp32 ? !yf : !y) return fail(-2, "lex_conv3d_bf16_fwd: missing output%s");
-  int cip = padc(cin), cop = padc(cout);
-  int mode = (gelu ? 1 : 0) | (out_fp32 ? 2 : 0);
-  hipStream_t st = S(stream);
-  if (cip == 16 && cop == 16) launch_fwd<16, 16>(nx, ny, nz, x, wp, bias, mode, cout, y, yf, pre, st);
-  else if (cip == 16 && cop == 32) launch_fwd<16, 32>(nx, ny, nz, x, wp, bias, mode, cout, y, yf, pre, st);
-  else if (cip == 32 && cop == 16) launch_fwd<32, 16>(nx, ny, nz, x, wp, bias, mode, cout, y, yf, pre, st);
-  else launch_fwd<32, 32>(nx, ny, nz, x, wp, bias, mode, cout, y, yf, pre, st);
-  return check_launch("lex_conv3d_bf16_fwd");
+  const int mode = (gelu ? 1 : 0) | (out_fp32 ? 2 : 0);
+  return conv_fwd("lex_conv3d_bf16_fwd", nx, ny, nz, cin, cout, x, wp, bias, mode, y, yf, pre, S(stream));
+}
+
+int lex_conv3d_bf16_fwd_circular(int nx, int ny, int nz, int cin, int cout, const void* x, const void* wp,
+                                 const float* bias, int gelu, int out_fp32, void* y, float* yf, float* pre,
+                                 void* stream) {
+  REQ(x); REQ(wp); REQ(bias);
+  const int mode = (gelu ? 1 : 0) | (out_fp32 ? 2 : 0) | 4;
+  return conv_fwd("lex_conv3d_bf16_fwd_circular", nx, ny, nz, cin, cout, x, wp, bias, mode, y, yf, pre, S(stream));
 }
 
 int lex_conv3d_bf16_bwd_data(int nx, int ny, int nz, int cin, int cout, const void* g_pre, const void* wp_flip,
                              const float* zero_bias, float* dx, void* stream) {
   REQ(g_pre); REQ(wp_flip); REQ(zero_bias); REQ(dx);
-  if (!conv_ok(nx, ny, nz, cin, cout)) return fail(-1, "lex_conv3d_bf16_bwd_data: bad sizes (channels 1..32)%s");
+  if (!conv_ok(nx, ny, nz, cin, cout, 64)) return fail(-1, "lex_conv3d_bf16_bwd_data: bad sizes (channels 1..64)%s");
   // transposed convolution: input channels = cout, output channels = cin
   return lex_conv3d_bf16_fwd(nx, ny, nz, cout, cin, g_pre, wp_flip, zero_bias, 0, 1, nullptr, dx, nullptr, stream);
 }

@@ -37,28 +37,46 @@ __device__ __forceinline__ float gelu_t_grad(float x) {
   return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * k0 * (1.0f + 3.0f * k1 * x * x);
 }
 
+// K-steps of 32 over (tap, channel): CIN <= 32 packs 32 / CIN taps per step; CIN = 64
+// spends two steps per tap (channels 0..31, then 32..63).
 template <int CIN>
 struct ConvSteps {
-  static constexpr int TPS = 32 / CIN;                  // taps per K-step
-  static constexpr int NSTEP = (27 + TPS - 1) / TPS;   // K-steps
+  static constexpr int TPS = CIN <= 32 ? 32 / CIN : 1;              // taps per K-step
+  static constexpr int SPT = CIN <= 32 ? 1 : CIN / 32;              // K-steps per tap
+  static constexpr int NSTEP = CIN <= 32 ? (27 + TPS - 1) / TPS : 27 * SPT;
 };
+// tap and first channel of a lane's 8 k values (kl = 8 * (lane / 16)) in step s
+__host__ __device__ __forceinline__ void conv_step_k(int cinp, int s, int kl, int& tap, int& ch) {
+  if (cinp <= 32) {
+    tap = s * (32 / cinp) + kl / cinp;
+    ch = kl % cinp;
+  } else {
+    tap = s / (cinp / 32);
+    ch = (s % (cinp / 32)) * 32 + kl;
+  }
+}
 
 // Forward convolution.  mode bit 0: GELU after the bias; bit 1: FP32 output Yf with
-// cout_real channels (else BF16 Y with COUT channels).  pre (optional): FP32
+// cout_real channels (else BF16 Y with COUT channels); bit 2: periodic x and y.  pre (optional): FP32
 // pre-activation, COUT channels.
 template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void k_conv3_fwd(int nx, int ny, int nz, const __bf16* __restrict__ X,
                                                    const __bf16* __restrict__ Wp, const float* __restrict__ bias,
                                                    int mode, int cout_real, __bf16* __restrict__ Y,
                                                    float* __restrict__ Yf, float* __restrict__ pre) {
-  constexpr int TPS = ConvSteps<CIN>::TPS, NSTEP = ConvSteps<CIN>::NSTEP, NT = COUT / 16;
+  constexpr int NSTEP = ConvSteps<CIN>::NSTEP, NT = COUT / 16;
   __shared__ __attribute__((aligned(16))) __bf16 tile[HX * HY * HZ * CIN];
   const int z0 = blockIdx.x * CTZ, y0 = blockIdx.y * CTY, x0 = blockIdx.z * CTX;
   constexpr int nchunk = HX * HY * HZ * CIN / 8;
+  const bool circ = (mode & 4) != 0;  // periodic x and y (flax padding 'CIRCULAR' there), zero z
   for (int c = threadIdx.x; c < nchunk; c += 256) {
     int cell = c / (CIN / 8), g = c % (CIN / 8);
     int hz = cell % HZ, hy = (cell / HZ) % HY, hx = cell / (HZ * HY);
     int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+    if (circ) {
+      gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+      gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+    }
     bf16x8 v;
     if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
       v = *(const bf16x8*)(X + (((long)gx * ny + gy) * nz + gz) * CIN + 8 * g);
@@ -78,8 +96,8 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(int nx, int ny, int nz, const
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) bw[nt] = *(const bf16x8*)(Wp + (long)(nt * 16 + m) * (NSTEP * 32) + kl);
   for (int s = 0; s < NSTEP; ++s) {
-    int tap = s * TPS + kl / CIN;
-    const int ch0 = kl % CIN;
+    int tap, ch0;
+    conv_step_k(CIN, s, kl, tap, ch0);
     if (tap > 26) tap = 26;              // padded K-step: weights are zero there
     const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
     const int sn = (s + 1 < NSTEP) ? s + 1 : s;
@@ -306,8 +324,8 @@ __global__ void k_conv3_pack(const float* __restrict__ W, int cin, int cout, int
   int n = (int)(t / (nstep * 32));
   int k = (int)(t % (nstep * 32));
   int s = k / 32, kl = k % 32;
-  int tps = 32 / cinp;
-  int tap = s * tps + kl / cinp, ch = kl % cinp;
+  int tap, ch;
+  conv_step_k(cinp, s, kl, tap, ch);
   float v = 0.f;
   if (tap < 27) {
     if (!flip) {

@@ -10,7 +10,8 @@ Mirrors dl_models.py of the reference for the pieces ws_rk3_sprint_dl
   ``stagger_uvw``): ``lex_dl_pack`` and ``lex_dl_stagger``.
 
 The flax parameters of the reference become the net object itself (``dl_params``).
-Zcnn and AutoEncoder (:33-122) are not built.
+``Zcnn`` (:71-122) runs on the same kernels with periodic x/y padding (inference);
+AutoEncoder (:33-68, strided and transposed convolutions, a dense layer) is not built.
 """
 from __future__ import annotations
 
@@ -23,6 +24,18 @@ from .sgs_net import SgsNet
 def simpleCNN(seed=2031, device="cuda"):  # noqa: N802 - the reference's class name
     """The reference's simpleCNN shape on the MFMA kernels: 5 -> 32 -> 32 -> 32 -> 5."""
     return SgsNet(channels=(5, 32, 32, 32, 5), seed=seed, device=device)
+
+
+ZCNN_CHANNELS = (5, 64, 32, 32, 32, 64, 32, 32, 32, 64, 32, 32, 32, 5)
+
+
+def Zcnn(seed=2031, device="cuda"):  # noqa: N802 - the reference's class name
+    """Zcnn (dl_models.py:71-122): thirteen 3x3x3 convolutions, 5 -> 64 -> 32 -> 32 -> 32 ->
+    64 -> 32 -> 32 -> 32 -> 64 -> 32 -> 32 -> 32 -> 5, GELU between them, periodic in x and y;
+    the zero planes the reference concatenates above and below the column and resets
+    after every layer are zero padding in z.  Inference only (the 64-channel weight
+    gradient is not built)."""
+    return SgsNet(channels=ZCNN_CHANNELS, seed=seed, device=device, circular=True)
 
 
 def level_norm(scaling_params):

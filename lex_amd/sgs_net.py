@@ -22,7 +22,7 @@ CHANNELS = (11, 32, 32, 32, 5)
 
 
 def _padc(c):
-    return 16 if c <= 16 else 32
+    return 16 if c <= 16 else (32 if c <= 32 else 64)
 
 
 def _pack(w, flip):
@@ -43,8 +43,10 @@ def to_bf16_channels_last(x, cp=None):
     return out
 
 
-def conv3d_fwd(x_bf, shape, w, b, gelu, out_fp32, want_pre=False):
-    """One convolution: x_bf (ncell, padded cin) BF16 -> BF16 (ncell, padded cout) or FP32 (ncell, cout)."""
+def conv3d_fwd(x_bf, shape, w, b, gelu, out_fp32, want_pre=False, circular=False):
+    """One convolution: x_bf (ncell, padded cin) BF16 -> BF16 (ncell, padded cout) or FP32 (ncell, cout).
+    circular: periodic x, y and zero z padding (else zero padding everywhere)."""
+    entry = "lex_conv3d_bf16_fwd_circular" if circular else "lex_conv3d_bf16_fwd"
     nx, ny, nz = shape
     cin, cout = int(w.shape[1]), int(w.shape[2])
     cop = _padc(cout)
@@ -55,11 +57,11 @@ def conv3d_fwd(x_bf, shape, w, b, gelu, out_fp32, want_pre=False):
     pre = torch.empty((ncell, cop), dtype=torch.float32, device=w.device) if want_pre else None
     if out_fp32:
         yf = torch.empty((ncell, cout), dtype=torch.float32, device=w.device)
-        _lib.call("lex_conv3d_bf16_fwd", nx, ny, nz, cin, cout, _lib.ptr(x_bf), _lib.ptr(wp), _lib.ptr(bp), int(gelu), 1,
+        _lib.call(entry, nx, ny, nz, cin, cout, _lib.ptr(x_bf), _lib.ptr(wp), _lib.ptr(bp), int(gelu), 1,
                   None, _lib.ptr(yf), _lib.ptr(pre), _lib.stream())
         return yf, pre
     y = torch.empty((ncell, cop), dtype=torch.bfloat16, device=w.device)
-    _lib.call("lex_conv3d_bf16_fwd", nx, ny, nz, cin, cout, _lib.ptr(x_bf), _lib.ptr(wp), _lib.ptr(bp), int(gelu), 0,
+    _lib.call(entry, nx, ny, nz, cin, cout, _lib.ptr(x_bf), _lib.ptr(wp), _lib.ptr(bp), int(gelu), 0,
               _lib.ptr(y), None, _lib.ptr(pre), _lib.stream())
     return y, pre
 
@@ -94,34 +96,42 @@ def gelu_bwd(g_post, gstride, pre, cp, c_real, gelu):
     return out
 
 
+def _net_forward(ctx, x, params, circular):
+    nx, ny, nz, cin = x.shape
+    shape = (nx, ny, nz)
+    a = to_bf16_channels_last(x)
+    acts, pres = [a], []
+    nl = len(params) // 2
+    out = None
+    for li in range(nl):
+        w, b = params[2 * li], params[2 * li + 1]
+        last = li == nl - 1
+        y, pre = conv3d_fwd(acts[-1], shape, w, b, gelu=not last, out_fp32=last, want_pre=not last,
+                            circular=circular)
+        if last:
+            out = y
+        else:
+            acts.append(y)
+            pres.append(pre)
+    ctx.shape = shape
+    ctx.save_for_backward(*params, *acts, *pres)
+    ctx.nl = nl
+    ctx.need_x = ctx.needs_input_grad[0]
+    ctx.circular = circular
+    return out.view(nx, ny, nz, -1)
+
+
 class SgsNetFn(torch.autograd.Function):
-    """x (nx, ny, nz, cin) FP32 -> (nx, ny, nz, cout) FP32 through the four layers."""
+    """x (nx, ny, nz, cin) FP32 -> (nx, ny, nz, cout) FP32 through the layers (zero padding)."""
 
     @staticmethod
     def forward(ctx, x, *params):
-        nx, ny, nz, cin = x.shape
-        shape = (nx, ny, nz)
-        a = to_bf16_channels_last(x)
-        acts, pres = [a], []
-        nl = len(params) // 2
-        out = None
-        for li in range(nl):
-            w, b = params[2 * li], params[2 * li + 1]
-            last = li == nl - 1
-            y, pre = conv3d_fwd(acts[-1], shape, w, b, gelu=not last, out_fp32=last, want_pre=not last)
-            if last:
-                out = y
-            else:
-                acts.append(y)
-                pres.append(pre)
-        ctx.shape = shape
-        ctx.save_for_backward(*params, *acts, *pres)
-        ctx.nl = nl
-        ctx.need_x = ctx.needs_input_grad[0]
-        return out.view(nx, ny, nz, -1)
+        return _net_forward(ctx, x, params, False)
 
     @staticmethod
     def backward(ctx, gy):
+        if ctx.circular:
+            raise NotImplementedError("the backward of the periodic-padding (Zcnn) layers is not built")
         nl = ctx.nl
         saved = ctx.saved_tensors
         params, acts, pres = saved[:2 * nl], saved[2 * nl:3 * nl], saved[3 * nl:]
@@ -143,11 +153,20 @@ class SgsNetFn(torch.autograd.Function):
         return (dx, *grads)
 
 
+class CircularNetFn(SgsNetFn):
+    """The same layers with periodic x, y and zero z padding (inference only)."""
+
+    @staticmethod
+    def forward(ctx, x, *params):
+        return _net_forward(ctx, x, params, True)
+
+
 class SgsNet(torch.nn.Module):
     """simpleCNN-shaped net with FP32 master weights W[27, cin, cout] and biases."""
 
-    def __init__(self, channels=CHANNELS, seed=2031, device="cuda"):
+    def __init__(self, channels=CHANNELS, seed=2031, device="cuda", circular=False):
         super().__init__()
+        self.circular = bool(circular)
         gen = torch.Generator().manual_seed(seed)
         self.channels = tuple(channels)
         ws, bs = [], []
@@ -166,4 +185,5 @@ class SgsNet(torch.nn.Module):
         return out
 
     def forward(self, x):
-        return SgsNetFn.apply(x, *self.params())
+        fn = CircularNetFn if self.circular else SgsNetFn
+        return fn.apply(x, *self.params())

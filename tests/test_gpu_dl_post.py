@@ -72,7 +72,8 @@ def test_pack_stagger_and_apply_match_the_oracle():
         assert err <= 2.0 ** -23, f"{name}: {err:.2e}"
 
 
-def test_sprint_dl_matches_the_oracle_loop():
+@pytest.mark.parametrize("model", ["simpleCNN", "Zcnn"])
+def test_sprint_dl_matches_the_oracle_loop(model):
     """Two steps of ws_rk3_sprint_dl (simpleCNN, random weights, correction about a tenth
     of the step's increment) vs the oracle's loop (FP64 steps) with the same network as
     net_fn: increments over the sprint agree to the FP32/BF16 budget."""
@@ -88,7 +89,7 @@ def test_sprint_dl_matches_the_oracle_loop():
     st0 = tuple(from_dev(a).astype(np.float32) for a in phys)
     spr = _scaling(rng, case.nx, case.ny, case.nz)
     spd = tuple(None if a is None else to_dev(a) for a in spr)
-    net = dlm.simpleCNN(seed=7)
+    net = getattr(dlm, model)(seed=7)
     with torch.no_grad():
         net.weights[-1].mul_(1e-3)
         net.biases[-1].fill_(1e-4)

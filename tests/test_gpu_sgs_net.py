@@ -110,3 +110,50 @@ def test_net_forward_backward_vs_fp32_autograd():
     for a, r in zip(net.biases, bs):
         assert _rel(a.grad.cpu(), r.grad) < 3e-2, _rel(a.grad.cpu(), r.grad)
     assert _rel(xd.grad.cpu(), xr.grad) < 3e-2
+
+
+def _ref_conv(xb, Wb, circular):
+    import torch
+    xin = _ncdhw(xb)
+    if circular:  # periodic x and y, zero z (Zcnn's padding)
+        xin = torch.nn.functional.pad(xin, (0, 0, 1, 1, 1, 1), mode="circular")
+        return torch.nn.functional.conv3d(xin, _torch_w(Wb), padding=(0, 0, 1))[0].permute(1, 2, 3, 0)
+    return torch.nn.functional.conv3d(xin, _torch_w(Wb), padding=1)[0].permute(1, 2, 3, 0)
+
+
+@pytest.mark.parametrize("cin,cout,circular", [(32, 64, False), (64, 32, False), (64, 64, False), (5, 64, True),
+                                               (64, 32, True), (32, 32, True), (32, 5, True)])
+def test_conv_forward_64_channels_and_periodic_padding(cin, cout, circular):
+    """64-channel layers (two K-steps per tap) and the periodic x/y padding of Zcnn's
+    layers against torch conv3d in float64 on the same BF16-rounded operands."""
+    import torch
+    from lex_amd import sgs_net as sn
+    g = torch.Generator().manual_seed(cin * 1000 + cout * 10 + int(circular))
+    x = torch.randn(*SHAPE, cin, generator=g)
+    W = torch.randn(27, cin, cout, generator=g) / np.sqrt(27 * cin)
+    b = torch.randn(cout, generator=g) * 0.1
+    ref = _gelu(_ref_conv(x.bfloat16().double(), W.bfloat16().double(), circular) + b.double())
+    xd = sn.to_bf16_channels_last(x.cuda())
+    yf, _ = sn.conv3d_fwd(xd, SHAPE, W.cuda(), b.cuda(), gelu=1, out_fp32=True, circular=circular)
+    got = yf.cpu().double().reshape(*SHAPE, cout)
+    assert _rel(got, ref) < 1e-5, _rel(got, ref)
+
+
+def test_zcnn_forward_vs_fp32():
+    """The reference's Zcnn shape (13 layers, 64/32 channels, periodic x/y) against FP32
+    torch with the same weights: the BF16 budget."""
+    import torch
+    from lex_amd import dl_models as dlm
+    net = dlm.Zcnn(seed=3)
+    g = torch.Generator().manual_seed(4)
+    x = torch.randn(*SHAPE, 5, generator=g)
+    with torch.no_grad():
+        y = net(x.cuda()).cpu().double()
+        h = x.double()
+        nl = len(net.weights)
+        for i, (w, b) in enumerate(zip(net.weights, net.biases)):
+            h = _ref_conv(h, w.detach().cpu().double(), True) + b.detach().cpu().double()
+            if i < nl - 1:
+                h = _gelu(h)
+    assert y.shape == h.shape
+    assert _rel(y, h) < 3e-2, _rel(y, h)
