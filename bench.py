@@ -198,16 +198,29 @@ def dl_sprint_timing(sprint_n=10):
 
 def physics_timing(steps=10):
     """The 256x256x128 step with the upper-level Rayleigh damping (layer above 9.6 km of
-    the 12.8 km column) and the Newtonian heating switched on, against the plain step."""
+    the 12.8 km column), the Newtonian heating and the bulk surface fluxes (sea surface
+    at the set-up's 300 K) switched on, against the plain step; and the surface-flux
+    kernel alone on the 256x256 columns."""
     import torch
-    from lex_amd import _lib, namelist_n_constants as nl, one_sprint_functions as sp
+    from lex_amd import _lib, boundary_conditions as bcm, namelist_n_constants as nl, one_sprint_functions as sp
     from lex_amd.setup_lex import build_case, config1
     case, opts = config1()
     phys, base, grids, _ = build_case(case)
     nl.dt, nl.n_sound, nl.solver_opt, nl.turb_opt = opts["dt"], 7, 2, 1
     g = _lib.grid(case.nx, case.ny, case.nz, case.dx, case.dy, case.dz)
     out = {}
-    for name, mo in (("plain", (2, False, False, 0, False, False, 1)), ("damp_rad", (2, True, True, 0, False, False, 1))):
+    reps = 200
+    bcm.surface_fluxes(phys, base, g)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        bcm.surface_fluxes(phys, base, g)
+    e1.record()
+    torch.cuda.synchronize()
+    out["surface_fluxes_us_per_call"] = e0.elapsed_time(e1) / reps * 1e3
+    for name, mo in (("plain", (2, False, False, 0, False, False, 1)), ("damp_rad", (2, True, True, 0, False, False, 1)),
+                     ("sfc", (2, False, False, 0, True, False, 1)), ("damp_rad_sfc", (2, True, True, 0, True, False, 1))):
         st = tuple(a.clone() for a in phys)
         sp.fused_step(st, base, g, mo)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

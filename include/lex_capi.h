@@ -176,6 +176,10 @@ typedef struct lex_step_opts {
                         the stage-0 theta and kept for the step (one_step_integration.py:31-35) */
   float z_damping;   /* height (m) above which the damping acts (namelist z_damping, 9600) */
   float rd_alpha;    /* inverse e-folding time (1/s) of the damping (namelist rd_alpha, 1/120) */
+  int sfc;           /* 1 = bulk atmosphere-ocean surface fluxes (sfc_opt, boundary_conditions.py:7-75)
+                        from the stage-0 state, kept for the step (one_step_integration.py:37-52):
+                        stress on the bottom u / v faces, sen/Cp and evaporation through the
+                        bottom theta / qv faces; needs surface_t in the base state */
 } lex_step_opts_t;
 
 typedef struct lex_base_cols {
@@ -184,11 +188,12 @@ typedef struct lex_base_cols {
   const float* qv0;
   const float* pi0;
   const float* rho0_theta0;
+  const float* surface_t;  /* (nx, ny) sea-surface temperature (K), base_state[5]; read only with sfc */
 } lex_base_cols_t;
 
 /* ws_rk3_big_step (one_sprint_functions.py:10-30) for solver_opt 2 and 3: advances the
  * state in place.  state[] = theta, u, v, w, pip, qv, then n_passive scalars (all
- * padded).  The adjoint entry points below reject damp and rad. */
+ * padded).  The adjoint entry points below reject damp, rad and sfc. */
 size_t lex_rk3_workspace_bytes(const lex_grid_t* g, int n_passive);
 int lex_rk3_step(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b, float* const* state,
                  void* workspace, size_t workspace_bytes, void* stream);
@@ -378,6 +383,20 @@ int lex_dl_stagger(const lex_grid_t* g, const float* c, float* theta_c, float* u
 int lex_dl_apply(const lex_grid_t* g, const float* c, const float* r_theta, const float* r_u, const float* r_v,
                  const float* r_w, const float* r_qv, float* theta, float* u, float* v, float* w, float* qv,
                  void* stream);
+
+/* ---- surface fluxes (sfc_opt) ----------------------------------------------------------
+ * atm_ocn_flux (boundary_conditions.py:7-75) on n columns of device FP32 inputs (the
+ * reference's arguments: z_bottom, u_bottom, v_bottom, theta_bottom, q_bottom,
+ * rho_bottom, surface_t) -> out[0..6] = tau_x, tau_y, sen, evap, t_ref, q_ref, u10n
+ * (n floats each, device), with the reference's signs (upward fluxes positive). */
+int lex_atm_ocn_flux(long n, const float* z_bottom, const float* u_bottom, const float* v_bottom,
+                     const float* theta_bottom, const float* q_bottom, const float* rho_bottom,
+                     const float* surface_t, float* const* out, void* stream);
+/* The same on the lowest level of a padded state as rk_sub_step0 slices it
+ * (one_step_integration.py:37-47): u, v averaged to the cell centre, theta, qv and
+ * rho0 (column, nz+2) of level 1, z_bottom = dz/2; out[0..6] (nx, ny) as above. */
+int lex_surface_fluxes(const lex_grid_t* g, const float* rho0, const float* theta, const float* u, const float* v,
+                       const float* qv, const float* surface_t, float* const* out, void* stream);
 
 #ifdef __cplusplus
 }

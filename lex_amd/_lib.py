@@ -30,12 +30,13 @@ class Grid(C.Structure):
 class StepOpts(C.Structure):
     _fields_ = [("dt", C.c_float), ("n_sound", C.c_int), ("turb", C.c_int), ("cs", C.c_float),
                 ("prandtl", C.c_float), ("n_passive", C.c_int), ("impl", C.c_int), ("solver", C.c_int),
-                ("damp", C.c_int), ("rad", C.c_int), ("z_damping", C.c_float), ("rd_alpha", C.c_float)]
+                ("damp", C.c_int), ("rad", C.c_int), ("z_damping", C.c_float), ("rd_alpha", C.c_float),
+                ("sfc", C.c_int)]
 
 
 class BaseCols(C.Structure):
     _fields_ = [("rho0", C.c_void_p), ("theta0", C.c_void_p), ("qv0", C.c_void_p), ("pi0", C.c_void_p),
-                ("rho0_theta0", C.c_void_p)]
+                ("rho0_theta0", C.c_void_p), ("surface_t", C.c_void_p)]
 
 
 def build(verbose: bool = False) -> str:
@@ -87,6 +88,8 @@ _SIGS = {
     "lex_dl_pack": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "lex_dl_stagger": [_G, _P, _P, _P, _P, _P, _P, _P],
     "lex_dl_apply": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lex_atm_ocn_flux": [C.c_long, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lex_surface_fluxes": [_G, _P, _P, _P, _P, _P, _P, _P, _P],
     "lex_scalar_update": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _F, _P, _P, _P],
     "lex_momentum_update": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P, _P, _P],
     "lex_smag_deformation": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],

@@ -175,9 +175,9 @@ void rk3_forward_tape(const Dims& d, const O& o, const B& b, float* const* X0, c
     if (fused) {
       dim3 fb(nzl, kFTY, 1), fg((d.nx + kFTX - 1) / kFTX, (d.ny + kFTY - 1) / kFTY, 1);
       hipLaunchKernelGGL(k_mom_u, fg, fb, mom_uv_lds(d), st, d, rc, src[1], src[2], src[3],
-                         turb ? (const float*)su : nullptr, ru);
+                         turb ? (const float*)su : nullptr, (const float*)nullptr, ru);
       hipLaunchKernelGGL(k_mom_v, fg, fb, mom_uv_lds(d), st, d, rc, src[1], src[2], src[3],
-                         turb ? (const float*)sv : nullptr, rv);
+                         turb ? (const float*)sv : nullptr, (const float*)nullptr, rv);
       hipLaunchKernelGGL(k_mom_w, fg, fb, mom_w_lds(d), st, d, rc, b.theta0, b.qv0, src[1], src[2], src[3], src[0],
                          src[5], turb ? (const float*)sw : nullptr, rw);
       ScalarSet S;
@@ -204,7 +204,8 @@ void rk3_forward_tape(const Dims& d, const O& o, const B& b, float* const* X0, c
       hipLaunchKernelGGL(k_div_rho, cx.G(d.nz + 2, d.ny + 6, d.nx + 6), cx.bk, 0, st, d, rc, src[1], src[2], src[3],
                          fd);
       hipLaunchKernelGGL(k_rhs_uv, cx.G(d.nz, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, src[1], src[2], src[3], fd,
-                         turb ? (const float*)su : nullptr, turb ? (const float*)sv : nullptr, ru, rv);
+                         turb ? (const float*)su : nullptr, turb ? (const float*)sv : nullptr, (const float*)nullptr,
+                         (const float*)nullptr, ru, rv);
       hipLaunchKernelGGL(k_rhs_w, cx.G(d.nz + 1, d.ny, d.nx), cx.bk, 0, st, d, rc, b.theta0, b.qv0, src[1], src[2],
                          src[3], src[0], src[5], fd, turb ? (const float*)sw : nullptr, rw);
       long n = (long)L.nc;

@@ -493,6 +493,7 @@ int lex_rk3_step(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_c
   if (o->solver != 0 && o->solver != 2 && o->solver != 3) return fail(-1, "lex_rk3_step: solver must be 0, 2 or 3%s");
   if (o->solver == 3 && (g->nz < 2 || g->nz > 256)) return fail(-1, "lex_rk3_step: solver 3 needs nz 2..256%s");
   if (o->damp && g->nz > 256) return fail(-1, "lex_rk3_step: damping needs nz <= 256%s");
+  if (o->sfc && !b->surface_t) return fail(-2, "lex_rk3_step: sfc needs the surface temperature (surface_t)%s");
   if (o->impl != 1 && (64 * ((g->nz + 63) / 64)) * kFTY > 1024)
     return fail(-1, "lex_rk3_step: the fused kernels need nz <= 256 (use impl 1)%s");
   int rc = rk3_step(d, *o, *b, state, (char*)ws, L, S(stream));
@@ -682,7 +683,7 @@ int lex_rk3_step_vjp(const lex_grid_t* g, const lex_step_opts_t* o, const lex_ba
       return fail(-1, "lex_rk3_step_vjp: lam_in must not alias lam_out or the state%s");
   }
   if (o->solver == 3) return fail(-1, "lex_rk3_step_vjp: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
-  if (o->damp || o->rad) return fail(-1, "lex_rk3_step_vjp: the adjoint covers damp_opt and rad_opt off%s");
+  if (o->damp || o->rad || o->sfc) return fail(-1, "lex_rk3_step_vjp: the adjoint covers damp_opt, rad_opt and sfc_opt off%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (ws_bytes < L.total) return fail(-3, "lex_rk3_step_vjp: workspace too small%s");
@@ -716,7 +717,7 @@ int lex_rk3_step_tape(const lex_grid_t* g, const lex_step_opts_t* o, const lex_b
     if (state[q] == state_out[q]) return fail(-1, "lex_rk3_step_tape: state_out must not alias state%s");
   }
   if (o->solver == 3) return fail(-1, "lex_rk3_step_tape: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
-  if (o->damp || o->rad) return fail(-1, "lex_rk3_step_tape: the adjoint covers damp_opt and rad_opt off%s");
+  if (o->damp || o->rad || o->sfc) return fail(-1, "lex_rk3_step_tape: the adjoint covers damp_opt, rad_opt and sfc_opt off%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_tape: tape too small%s");
@@ -738,7 +739,7 @@ int lex_rk3_step_vjp_taped(const lex_grid_t* g, const lex_step_opts_t* o, const 
       return fail(-1, "lex_rk3_step_vjp_taped: lam_in must not alias lam_out or the state%s");
   }
   if (o->solver == 3) return fail(-1, "lex_rk3_step_vjp_taped: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
-  if (o->damp || o->rad) return fail(-1, "lex_rk3_step_vjp_taped: the adjoint covers damp_opt and rad_opt off%s");
+  if (o->damp || o->rad || o->sfc) return fail(-1, "lex_rk3_step_vjp_taped: the adjoint covers damp_opt, rad_opt and sfc_opt off%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_vjp_taped: tape too small%s");
@@ -765,7 +766,7 @@ int lex_rk3_step_tape_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o, con
   for (int q = 0; q < 2 + o->n_passive; ++q)
     if (!ss[q]) return fail(-2, "lex_rk3_step_tape_ext_sgs: null scalar SGS pointer%s");
   if (o->solver == 3) return fail(-1, "lex_rk3_step_tape_ext_sgs: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
-  if (o->damp || o->rad) return fail(-1, "lex_rk3_step_tape_ext_sgs: the adjoint covers damp_opt and rad_opt off%s");
+  if (o->damp || o->rad || o->sfc) return fail(-1, "lex_rk3_step_tape_ext_sgs: the adjoint covers damp_opt, rad_opt and sfc_opt off%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_tape_ext_sgs: tape too small%s");
@@ -802,7 +803,7 @@ int lex_rk3_step_vjp_taped_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o
   for (int q = 0; q < 2 + o->n_passive; ++q)
     if (!lss[q]) return fail(-2, "lex_rk3_step_vjp_taped_ext_sgs: null scalar SGS adjoint pointer%s");
   if (o->solver == 3) return fail(-1, "lex_rk3_step_vjp_taped_ext_sgs: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
-  if (o->damp || o->rad) return fail(-1, "lex_rk3_step_vjp_taped_ext_sgs: the adjoint covers damp_opt and rad_opt off%s");
+  if (o->damp || o->rad || o->sfc) return fail(-1, "lex_rk3_step_vjp_taped_ext_sgs: the adjoint covers damp_opt, rad_opt and sfc_opt off%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_vjp_taped_ext_sgs: tape too small%s");
@@ -1006,6 +1007,31 @@ int lex_dl_apply(const lex_grid_t* g, const float* c, const float* r_th, const f
   rg.qv = r_qv;
   hipLaunchKernelGGL(k_dl_apply, grd(d.nz, d.ny, d.nx), blk(), 0, S(stream), d, c, rg, th, u, v, w, qv);
   return check_launch("lex_dl_apply");
+}
+
+int lex_atm_ocn_flux(long n, const float* z, const float* ub, const float* vb, const float* th, const float* q,
+                     const float* rho, const float* ts, float* const* out, void* stream) {
+  REQ(z); REQ(ub); REQ(vb); REQ(th); REQ(q); REQ(rho); REQ(ts); REQ(out);
+  if (n < 0) return fail(-1, "lex_atm_ocn_flux: n must be >= 0%s");
+  for (int c = 0; c < 7; ++c)
+    if (!out[c]) return fail(-2, "lex_atm_ocn_flux: null output pointer%s");
+  if (n == 0) return 0;
+  SfcOut o{out[0], out[1], out[2], out[3], out[4], out[5], out[6], nullptr};
+  hipLaunchKernelGGL(k_atm_ocn_flux_cols, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), n, z, ub, vb,
+                     th, q, rho, ts, o);
+  return check_launch("lex_atm_ocn_flux");
+}
+
+int lex_surface_fluxes(const lex_grid_t* g, const float* rho0, const float* th, const float* u, const float* v,
+                       const float* qv, const float* ts, float* const* out, void* stream) {
+  REQGRID(g);
+  REQ(rho0); REQ(th); REQ(u); REQ(v); REQ(qv); REQ(ts); REQ(out);
+  for (int c = 0; c < 7; ++c)
+    if (!out[c]) return fail(-2, "lex_surface_fluxes: null output pointer%s");
+  Dims d = mkdims(g);
+  SfcOut o{out[0], out[1], out[2], out[3], out[4], out[5], out[6], nullptr};
+  launch_atm_ocn_flux(d, rho0, th, u, v, qv, ts, o, S(stream));
+  return check_launch("lex_surface_fluxes");
 }
 
 int lex_sgs_apply(const lex_grid_t* g, const float* y, const float* out_scale, float* su, float* sv, float* sw,

@@ -73,8 +73,17 @@ struct ScalarSet {
   float* out[kMaxScalars];        // next-stage fields (padded, halos filled)
   float weps[kMaxScalars];
   const float* heat = nullptr;    // diabatic heating added to scalar 0's tendency (interior) or null
+  // bottom-face fluxes (nx, ny) of scalar 0 (sen/Cp) and 1 (evaporation) or null: the
+  // lower boundary condition of the vertical flux (advection.py:15)
+  const float* sfc[2] = {nullptr, nullptr};
   int n;
 };
+
+// scalar q's flux through the surface at column p (zero without surface fluxes)
+__device__ __forceinline__ float bottom_flux(const ScalarSet& S, int q, long p) {
+  const float* f = (q == 0) ? S.sfc[0] : (q == 1 ? S.sfc[1] : nullptr);
+  return f ? f[p] : 0.0f;
+}
 
 template <int NS>
 __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __restrict__ rc,
@@ -167,7 +176,7 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
       if (q == 0) thc = qc;
       if (q == 1) qvc = qc;
       if (act) {
-        float fz = (k == 0) ? 0.0f : mz_lo * vrecon(Pc, Lz, K - 1, pz, weps);
+        float fz = (k == 0) ? bottom_flux(S, q, (long)i * d.ny + j) : mz_lo * vrecon(Pc, Lz, K - 1, pz, weps);
         FZ[ty * nzl + k] = fz;
         float fy = my_lo * hrecon(Pc[Kc - 3 * Lz], Pc[Kc - 2 * Lz], Pc[Kc - Lz], Pc[Kc], Pc[Kc + Lz], Pc[Kc + 2 * Lz],
                                   py0, weps);
@@ -221,7 +230,8 @@ __device__ __forceinline__ float fd_cell(const Dims& d, const float* __restrict_
 // ---------------------------------------------------------------------------------
 __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
                                                  const float* __restrict__ v, const float* __restrict__ w,
-                                                 const float* __restrict__ sgs, float* __restrict__ ru) {
+                                                 const float* __restrict__ sgs, const float* __restrict__ tau_x,
+                                                 float* __restrict__ ru) {
   extern __shared__ float lds[];
   const float weps = 1.0e-16f;
   const int nzl = blockDim.x, tz = threadIdx.x, ty = threadIdx.y, nt = nzl * kFTY, tid = ty * nzl + tz;
@@ -270,11 +280,15 @@ __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict_
     __syncthreads();
     const float* Pc = P + (ty + 3) * Lz;
     if (act) {
-      // z flux at w level K (face K-1): mass averaged over the two columns of the u point
+      // z flux at w level K (face K-1): mass averaged over the two columns of the u point;
+      // at the surface the stress averaged onto the face (periodic in x, advection.py:43-48)
       float fz = 0.0f;
       if (k > 0) {
         float m = 0.5f * (0.5f * (rc[K - 1] + rc[K]) * wA + 0.5f * (rc[K - 1] + rc[K]) * wB);
         fz = m * vrecon(Pc, Lz, K - 1, upwind_pos(m), weps);
+      } else if (tau_x) {
+        const int il = (i == 0) ? d.nx - 1 : i - 1;
+        fz = 0.5f * (tau_x[(long)il * d.ny + j] + tau_x[(long)i * d.ny + j]);
       }
       FZ[ty * nzl + k] = fz;
       float my = 0.5f * (r * vA + r * vB);
@@ -312,7 +326,8 @@ __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict_
 // ---------------------------------------------------------------------------------
 __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
                                                  const float* __restrict__ v, const float* __restrict__ w,
-                                                 const float* __restrict__ sgs, float* __restrict__ rv) {
+                                                 const float* __restrict__ sgs, const float* __restrict__ tau_y,
+                                                 float* __restrict__ rv) {
   extern __shared__ float lds[];
   const float weps = 1.0e-16f;
   const int nzl = blockDim.x, tz = threadIdx.x, ty = threadIdx.y, nt = nzl * kFTY, tid = ty * nzl + tz;
@@ -363,6 +378,9 @@ __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict_
       if (k > 0) {
         float m = 0.5f * (0.5f * (rc[K - 1] + rc[K]) * wA + 0.5f * (rc[K - 1] + rc[K]) * wB);
         fz = m * vrecon(Pc, Lz, K - 1, upwind_pos(m), weps);
+      } else if (tau_y) {   // advection.py:83-88
+        const int jl = (j == 0) ? d.ny - 1 : j - 1;
+        fz = 0.5f * (tau_y[(long)i * d.ny + jl] + tau_y[(long)i * d.ny + j]);
       }
       FZ[ty * nzl + k] = fz;
       float my = 0.5f * (Pc[K - Lz] + Pc[K]) * r;   // y-flux at cell J-1
@@ -616,7 +634,8 @@ __global__ void __launch_bounds__(512) k_stage_scalars_db(Dims d, const float* _
       const float* Pc = Pb + (b * NS + q) * PL + (ty + 3) * Lz;
       qcs[q] = Pc[Kc];
       if (act) {
-        FZb[(q * kFTY + ty) * nzl + k] = (k == 0) ? 0.0f : mz_lo * vrecon(Pc, Lz, K - 1, pz, weps);
+        FZb[(q * kFTY + ty) * nzl + k] =
+            (k == 0) ? bottom_flux(S, q, (long)i * d.ny + j) : mz_lo * vrecon(Pc, Lz, K - 1, pz, weps);
         FYb[(q * (kFTY + 1) + ty) * nzl + k] =
             my_lo * hrecon(Pc[Kc - 3 * Lz], Pc[Kc - 2 * Lz], Pc[Kc - Lz], Pc[Kc], Pc[Kc + Lz], Pc[Kc + 2 * Lz], py0,
                            weps);

@@ -10,15 +10,18 @@
 #include "lex_prof.hpp"
 #include "lex_fused.hip"
 #include "lex_acoustic_vi.hip"
+#include "lex_surface.hip"
 
 namespace lex {
 
 // momentum right-hand sides rhs_u = adv4u + fv + sgs_u, rhs_v = adv4v - fu + sgs_v,
 // rhs_w = adv4w + b8w + sgs_w (one_step_integration.py:126-133); Coriolis is off in
 // every configuration on this path (cor_opt 0), sgs may be NULL.
+// tau_x, tau_y: surface stresses (nx, ny) averaged onto the bottom u / v faces, or null.
 __global__ void k_rhs_uv(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
                          const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ fd,
-                         const float* __restrict__ su, const float* __restrict__ sv, float* __restrict__ ru,
+                         const float* __restrict__ su, const float* __restrict__ sv,
+                         const float* __restrict__ tau_x, const float* __restrict__ tau_y, float* __restrict__ ru,
                          float* __restrict__ rv) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
@@ -28,14 +31,18 @@ __global__ void k_rhs_uv(Dims d, const float* __restrict__ rc, const float* __re
   int I = i + 3, J = j + 3, K = k + 1;
   if (j < d.ny) {
     long o = ((long)i * d.ny + j) * d.nz + k;
-    float a = adv_u_point(d, rc, u, v, w, fd[ic(d, I - 1, J, K)], fd[ic(d, I, J, K)], 0.0f, i, j, k, weps);
+    float tf = 0.0f;
+    if (tau_x) tf = 0.5f * (tau_x[(long)((i - 1 + d.nx) % d.nx) * d.ny + j] + tau_x[(long)(i % d.nx) * d.ny + j]);
+    float a = adv_u_point(d, rc, u, v, w, fd[ic(d, I - 1, J, K)], fd[ic(d, I, J, K)], tf, i, j, k, weps);
     a = a + 0.0f;  // + fv (zero)
     if (su) a = a + su[o];
     ru[o] = a;
   }
   if (i < d.nx) {
     long o = ((long)i * (d.ny + 1) + j) * d.nz + k;
-    float a = adv_v_point(d, rc, u, v, w, fd[ic(d, I, J - 1, K)], fd[ic(d, I, J, K)], 0.0f, i, j, k, weps);
+    float tf = 0.0f;
+    if (tau_y) tf = 0.5f * (tau_y[(long)i * d.ny + (j - 1 + d.ny) % d.ny] + tau_y[(long)i * d.ny + j % d.ny]);
+    float a = adv_v_point(d, rc, u, v, w, fd[ic(d, I, J - 1, K)], fd[ic(d, I, J, K)], tf, i, j, k, weps);
     a = a - 0.0f;  // - fu (zero)
     if (sv) a = a + sv[o];
     rv[o] = a;
@@ -171,6 +178,7 @@ struct Rk3Layout {
   size_t off_smag;              // 10 padded scratch fields
   size_t off_heat;              // Newtonian heating of theta (interior), rad
   size_t off_lmean;             // per-level partial sums (nx x nz doubles) and means (nz floats), damp
+  size_t off_sfc;               // surface fluxes: 8 (nx, ny) fields (SfcOut order), sfc
   size_t total;
   int np;
 };
@@ -207,6 +215,7 @@ inline Rk3Layout rk3_layout(const Dims& d, int np) {
   L.off_smag = o; o += 10 * al256(L.nc * 4);
   L.off_heat = o; o += al256(L.ic_ * 4);
   L.off_lmean = o; o += al256((size_t)d.nx * d.nz * 8) + al256((size_t)d.nz * 4);
+  L.off_sfc = o; o += 8 * al256((size_t)d.nx * d.ny * 4);
   L.total = o;
   return L;
 }
@@ -245,7 +254,15 @@ struct StepOpts {
   int rad = 0;
   float z_damping = 9600.0f;
   float rd_alpha = 1.0f / 120.0f;
+  int sfc = 0;
 };
+
+// the surface-flux fields of the workspace
+inline SfcOut sfc_fields(char* ws, const Rk3Layout& L, const Dims& d) {
+  float* f[8];
+  for (int q = 0; q < 8; ++q) f[q] = (float*)(ws + L.off_sfc + q * al256((size_t)d.nx * d.ny * 4));
+  return SfcOut{f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7]};
+}
 
 template <class O, class B>
 int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* ws, const Rk3Layout& L,
@@ -275,6 +292,11 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
   float* heat = o.rad ? (float*)(ws + L.off_heat) : nullptr;
   double* lpart = (double*)(ws + L.off_lmean);
   float* lmean = (float*)(ws + L.off_lmean + al256((size_t)d.nx * d.nz * 8));
+  const SfcOut sf = sfc_fields(ws, L, d);
+  const float* tau_x = o.sfc ? sf.tau_x : nullptr;
+  const float* tau_y = o.sfc ? sf.tau_y : nullptr;
+  const float* sen_cp = o.sfc ? sf.sen_cp : nullptr;
+  const float* evap = o.sfc ? sf.evap : nullptr;
   const dim3 bk(64, 4, 1);
   auto G = [&](int a, int bb, int c) { return dim3((a + 63) / 64, (bb + 3) / 4, c); };
   const float* rc = b.rho0;
@@ -288,6 +310,10 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
     // diabatic heating from the stage-0 theta, kept for the step (one_step_integration.py:31-35)
     if (s == 0 && o.rad)
       LEX_LAUNCH(KID_HEATING, k_newton_heating, G(d.nz, d.ny, d.nx), bk, 0, st, d, X0[0], b.theta0, heat);
+    // surface fluxes from the stage-0 state, kept for the step (one_step_integration.py:37-47, :209)
+    if (s == 0 && o.sfc)
+      LEX_LAUNCH(KID_SFC, k_atm_ocn_flux, dim3((d.ny + 63) / 64, d.nx, 1), dim3(64, 1, 1), 0, st, d, rc, X0[0],
+                 X0[1], X0[2], X0[5], b.surface_t, sf);
     // turb 1: Smagorinsky computed here in stage 0; turb 2: SGS tendencies supplied by
     // the caller in the workspace slots (the learned SGS net); both frozen over the step
     const bool turb = o.turb != 0;
@@ -330,15 +356,15 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
       const int nzl = 64 * ((d.nz + 63) / 64);
       dim3 fb(nzl, kFTY, 1), fg((d.nx + kFTX - 1) / kFTX, (d.ny + kFTY - 1) / kFTY, 1);
       LEX_LAUNCH(KID_STAGE_MOMENTUM, k_mom_u, fg, fb, mom_uv_lds(d), st, d, rc, src[1], src[2], src[3],
-                 turb ? (const float*)su : nullptr, ru);
+                 turb ? (const float*)su : nullptr, tau_x, ru);
       LEX_LAUNCH(KID_MOM_V, k_mom_v, fg, fb, mom_uv_lds(d), st, d, rc, src[1], src[2], src[3],
-                 turb ? (const float*)sv : nullptr, rv);
+                 turb ? (const float*)sv : nullptr, tau_y, rv);
       LEX_LAUNCH(KID_MOM_W, k_mom_w, fg, fb, mom_w_lds(d), st, d, rc, b.theta0, b.qv0, src[1], src[2], src[3],
                  src[0], src[5], turb ? (const float*)sw : nullptr, rw);
     } else {
       LEX_LAUNCH(KID_DIV_RHO, k_div_rho, G(d.nz + 2, d.ny + 6, d.nx + 6), bk, 0, st, d, rc, src[1], src[2], src[3], fd);
       LEX_LAUNCH(KID_RHS_UV, k_rhs_uv, G(d.nz, d.ny + 1, d.nx + 1), bk, 0, st, d, rc, src[1], src[2], src[3], fd,
-                 turb ? su : nullptr, turb ? sv : nullptr, ru, rv);
+                 turb ? su : nullptr, turb ? sv : nullptr, tau_x, tau_y, ru, rv);
       LEX_LAUNCH(KID_RHS_W, k_rhs_w, G(d.nz + 1, d.ny, d.nx), bk, 0, st, d, rc, b.theta0, b.qv0, src[1], src[2], src[3],
                  src[0], src[5], fd, turb ? sw : nullptr, rw);
     }
@@ -355,6 +381,8 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
         S.weps[q] = (q == 0) ? 1.0e-17f : 1.0e-18f;
       }
       S.heat = heat;
+      S.sfc[0] = sen_cp;
+      S.sfc[1] = evap;
       const int nzl = 64 * ((d.nz + 63) / 64);
       dim3 fb(nzl, kFTY, 1), fg((d.nx + kFTX - 1) / kFTX, (d.ny + kFTY - 1) / kFTY, 1);
       size_t lds = stage_scalars_lds(d);
@@ -434,7 +462,8 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
         int fi = (q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2));
         float weps = (q == 0) ? 1.0e-17f : 1.0e-18f;
         LEX_LAUNCH(KID_SCALAR_UPDATE, k_scalar_update, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, X0[fi], src[fi], src[1],
-                   src[2], src[3], fd, (const float*)nullptr, (q == 0) ? (const float*)heat : nullptr,
+                   src[2], src[3], fd, (q == 0) ? sen_cp : (q == 1 ? evap : nullptr),
+                   (q == 0) ? (const float*)heat : nullptr,
                    turb ? ss[q] : nullptr, dst[fi], (float*)nullptr, (float)dt_s, weps);
       }
     }

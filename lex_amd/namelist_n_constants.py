@@ -39,6 +39,13 @@ damp_opt = False
 rad_opt = False
 pic_opt = False
 
+# Bulk surface-flux scheme (namelist_n_constants.py:43-47; fixed in the device kernel)
+atm_ocn_max_iter = 8
+atm_ocn_min_wind = 0.25
+sfc_z_ref = 10.0
+sfc_z_t_ref = 2.0
+Karman = 0.4
+
 # Rayleigh damping layer (namelist_n_constants.py:54-55)
 z_damping = 9600.0
 rd_alpha = 1.0 / 120.0

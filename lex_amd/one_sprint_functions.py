@@ -11,6 +11,7 @@ import ctypes
 import torch
 
 from . import _lib
+from . import boundary_conditions as bc
 from . import namelist_n_constants as nl
 from . import one_step_integration as one
 from ._core import column, grid_from
@@ -39,15 +40,22 @@ def fused_step(phys_state, base_state, grid, model_opt, dt=None, n_sound=None, c
         raise NotImplementedError("the fused step covers solver_opt 2 and 3 without Coriolis")
     rho0_theta0, rho0, theta0, pi0, qv0 = base_state[:5]
     n_passive = len(phys_state) - 6
+    sfc = bool(model_opt[4])
     opts = _lib.StepOpts(float(nl.dt if dt is None else dt), int(nl.n_sound if n_sound is None else n_sound),
                          int(turb_opt), float(nl.cs if cs is None else cs),
                          float(nl.prandtl if prandtl is None else prandtl), n_passive, int(impl), int(solver_opt),
-                         int(bool(model_opt[1])), int(bool(model_opt[2])), float(nl.z_damping), float(nl.rd_alpha))
+                         int(bool(model_opt[1])), int(bool(model_opt[2])), float(nl.z_damping), float(nl.rd_alpha),
+                         int(sfc))
     cols = [column(a) for a in (rho0, theta0, qv0, pi0, rho0_theta0)]
-    bc = _lib.BaseCols(*[c.data_ptr() for c in cols])
+    if sfc:
+        if len(base_state) < 6 or base_state[5] is None:
+            raise ValueError("sfc_opt needs the surface temperature base_state[5]")
+        ts = torch.as_tensor(base_state[5], dtype=torch.float32, device=phys_state[0].device).contiguous()
+        cols.append(ts)
+    bcols = _lib.BaseCols(*[c.data_ptr() for c in cols])
     ptrs = (ctypes.c_void_p * len(phys_state))(*[_lib.ptr(a) for a in phys_state])
     ws = _workspace(grid, n_passive, phys_state[0].device)
-    _lib.call("lex_rk3_step", grid, opts, bc, ptrs, _lib.ptr(ws), ws.numel(), _lib.stream())
+    _lib.call("lex_rk3_step", grid, opts, bcols, ptrs, _lib.ptr(ws), ws.numel(), _lib.stream())
     return phys_state
 
 
@@ -60,8 +68,9 @@ def ws_rk3_big_step(phys_state, base_state, grids, model_opt):
     if solver_opt in (2, 3) and model_opt[3] == 0:
         g = grid_from(base_state[1], *grids[3:6])
         new = tuple(a.clone() for a in phys_state)
+        sfc_others = _first_stage_sfc(phys_state, base_state, g, model_opt)
         fused_step(new, base_state, g, model_opt)
-        return new, one._sfc_etc(-999, new[0])
+        return new, sfc_others
     xi0 = phys_state
     xi1, _, sfc_others, heating, sgs = one.rk_sub_step0(xi0, xi0, base_state, grids, model_opt, nl.dt / 3.0)
     xi2, _ = one.rk_sub_step1(xi0, xi1, base_state, grids, heating, sfc_others, sgs, model_opt, nl.dt / 2.0)
@@ -100,8 +109,9 @@ def ws_rk3_sprint_dl(phys_state, base_state, grids, model_opt, dl_params, scalin
     with torch.no_grad():
         for _ in range(nl.sprint_n):
             if model_opt[0] in (2, 3) and model_opt[3] == 0:
-                fused_step(state, base_state, grid_from(base_state[1], *grids[3:6]), model_opt)
-                sfc_others = one._sfc_etc(-999, state[0])
+                g = grid_from(base_state[1], *grids[3:6])
+                sfc_others = _first_stage_sfc(state, base_state, g, model_opt)
+                fused_step(state, base_state, g, model_opt)
             else:
                 state, sfc_others = ws_rk3_big_step(state, base_state, grids, model_opt)
             x = dlm.phys_state_tuple2array(state, scaling_params, norm)
@@ -110,6 +120,15 @@ def ws_rk3_sprint_dl(phys_state, base_state, grids, model_opt, dl_params, scalin
     return state, sfc_others
 
 
+def _first_stage_sfc(phys_state, base_state, g, model_opt):
+    """sfc_others of a step: the surface fluxes of its input state (the first stage's,
+    one_step_integration.py:37-52, :186), zeros with sfc_opt off."""
+    if not model_opt[4]:
+        return one._sfc_etc(-999, phys_state[0])
+    return one._sfc_etc(-999, phys_state[0], bc.surface_fluxes(phys_state, base_state, g))
+
+
 def get_first_step_flux(phys_state, base_state, grids, model_opt):
-    """Stage-0 surface fluxes for output (one_sprint_functions.py:168-174); zero with sfc_opt off."""
-    return one._sfc_etc(-999, phys_state[0])
+    """Surface fluxes of the state for output (one_sprint_functions.py:168-174); zero with
+    sfc_opt off."""
+    return _first_stage_sfc(phys_state, base_state, grid_from(base_state[1], *grids[3:6]), model_opt)

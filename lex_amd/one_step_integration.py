@@ -218,18 +218,18 @@ def _theta_rho(theta, qv):
 
 # ------------------------------------------------------------------ R1 stages
 def _check_opts(model_opt, physics=False):
-    """Validates model_opt; physics=True (the device step driver) also accepts damp_opt
-    and rad_opt, which the per-routine stage functions here do not implement."""
+    """Validates model_opt; physics=True (the device step driver) also accepts damp_opt,
+    rad_opt and sfc_opt, which the per-routine stage functions here do not implement."""
     solver_opt, damp_opt, rad_opt, cor_opt, sfc_opt, pic_opt, turb_opt = model_opt
     if solver_opt not in (1, 2, 3):
         raise NotImplementedError(f"solver_opt {solver_opt} does not exist in the reference")
-    flags = (("sfc_opt", sfc_opt), ("pic_opt", pic_opt))
+    flags = (("pic_opt", pic_opt),)
     if not physics:
-        flags = (("damp_opt", damp_opt), ("rad_opt", rad_opt)) + flags
+        flags = (("damp_opt", damp_opt), ("rad_opt", rad_opt), ("sfc_opt", sfc_opt)) + flags
     for name, flag in flags:
         if flag:
             raise NotImplementedError(f"{name} is not built on this path"
-                                      + ("" if physics else " (damp_opt and rad_opt: use the fused step)"))
+                                      + ("" if physics else " (damp_opt, rad_opt, sfc_opt: use the fused step)"))
     if turb_opt not in (0, 1):
         raise NotImplementedError("turb_opt 2 (anisotropic Smagorinsky) is not on this path")
     return solver_opt, cor_opt, turb_opt
@@ -314,10 +314,14 @@ def _stage(stage, phys_state_now, phys_state, base_state, grids, model_opt, dt, 
     return new_state, tends, sgs_tend, info
 
 
-def _sfc_etc(info, like):
-    nx, ny = like.shape[0] - 6, like.shape[1] - 6
-    z = torch.zeros(nx, ny, dtype=torch.float32, device=like.device)
-    return (info, -999.9, z, z, z, z, z, z, z)
+def _sfc_etc(info, like, fluxes=None):
+    """The reference's sfc_etc tuple (one_step_integration.py:186): info, pip_const, then
+    tau_x, tau_y, sen, evap, t_ref, q_ref, u10n (zeros without surface fluxes)."""
+    if fluxes is None:
+        nx, ny = like.shape[0] - 6, like.shape[1] - 6
+        z = torch.zeros(nx, ny, dtype=torch.float32, device=like.device)
+        fluxes = (z,) * 7
+    return (info, -999.9) + tuple(fluxes)
 
 
 def rk_sub_step0(phys_state_now, phys_state, base_state, grids, model_opt, dt):

@@ -96,6 +96,7 @@ class Options:
     rad_opt: bool = False       # Newtonian heating (radiation.py:7-16)
     z_damping: float = 9600.0   # namelist_n_constants.py:54
     rd_alpha: float = 1.0 / 120.0  # namelist_n_constants.py:55
+    sfc_opt: bool = False       # bulk surface fluxes (boundary_conditions.py:7-75); base[5] = surface T
 
 
 # ---------------------------------------------------------------------------------
@@ -944,12 +945,103 @@ def newton_radiation(theta, theta0_ic):
     return t[NGX:-NGX, NGY:-NGY, NGZ:-NGZ]
 
 
-def rk_sub_step(stage, xi0, xi, base, grid, opts, dt_stage, frozen=None, heating=None):
+# ---------------------------------------------------------------------------------
+# Surface fluxes: the bulk atmosphere-ocean scheme of CAM (boundary_conditions.py:7-101)
+# ---------------------------------------------------------------------------------
+ATM_OCN_MAX_ITER = 8     # namelist_n_constants.py:43
+ATM_OCN_MIN_WIND = 0.25  # namelist_n_constants.py:44
+SFC_Z_REF = 10.0         # namelist_n_constants.py:45
+SFC_Z_T_REF = 2.0        # namelist_n_constants.py:46
+KARMAN = 0.4             # namelist_n_constants.py:47
+CPWV = 1810.0            # namelist_n_constants.py:56
+CPVIR = CPWV / Cp - 1.0  # namelist_n_constants.py:57
+LAT_VAP = 2.501e6        # namelist_n_constants.py:60
+
+
+def q_sat_sfc(tk):
+    """Saturation vapour density (kg m-3) over the sea surface (boundary_conditions.py:78-81)."""
+    return 640380.0 / np.exp(5107.4 / tk)
+
+
+def cdn(u_ref):
+    """Neutral 10 m drag coefficient (boundary_conditions.py:84-87)."""
+    return 0.0027 / u_ref + 0.000142 + 0.0000764 * u_ref
+
+
+def psi_m_s(hol):
+    """Integrated stability functions for momentum and scalars at z/L = hol
+    (boundary_conditions.py:90-103): -5 hol when stable, the unstable forms otherwise."""
+    stable = -5.0 * hol
+    xsq = np.maximum(np.sqrt(np.abs(1.0 - 16.0 * hol)), 1.0)
+    xqq = np.sqrt(xsq)
+    pm_u = np.log((1.0 + xqq * (2.0 + xqq)) * (1.0 + xqq * xqq) / 8.0) - 2.0 * np.arctan(xqq) + 1.571
+    ps_u = 2.0 * np.log((1.0 + xqq * xqq) / 2.0)
+    return np.where(hol > 0, stable, pm_u), np.where(hol > 0, stable, ps_u)
+
+
+def atm_ocn_flux(z_bottom, u_bottom, v_bottom, theta_bottom, q_bottom, rho_bottom, surface_t):
+    """Bulk fluxes at the lowest level (boundary_conditions.py:7-75): a fixed eight passes
+    of the Monin-Obukhov iteration.  Returns (tau_x, tau_y, sen, evap, t_ref, q_ref, u10n)
+    with the reference's final sign flip (upward fluxes positive)."""
+    v_mag = np.maximum(np.sqrt(u_bottom ** 2 + v_bottom ** 2), ATM_OCN_MIN_WIND)
+    ssq = 0.98 * q_sat_sfc(surface_t) / rho_bottom
+    delta_t = theta_bottom - surface_t
+    delta_q = q_bottom - ssq
+    alz = np.log(z_bottom / SFC_Z_REF)
+    cp = Cp * (1.0 + CPVIR * ssq)
+    ft = np.result_type(theta_bottom).type   # keep FP32 inputs in FP32
+    rdn = np.sqrt(cdn(v_mag))
+    rhn = np.where(delta_t > 0.0, ft(0.018), ft(0.0327))
+    ren = 0.0346
+    u_star, t_star, q_star = rdn * v_mag, rhn * delta_t, ren * delta_q
+    for _ in range(ATM_OCN_MAX_ITER):
+        hol = KARMAN * g * z_bottom * (t_star / theta_bottom + q_star / (1.0 / repsm1 + q_bottom)) / u_star ** 2
+        hol = np.clip(hol, -10.0, 10.0)
+        psi_mh, psi_xh = psi_m_s(hol)
+        rd = rdn / (1.0 + rdn / KARMAN * (alz - psi_mh))
+        u10n = v_mag * rd / rdn
+        rdn = np.sqrt(cdn(u10n))
+        rhn = np.where(hol > 0.0, ft(0.018), ft(0.0327))
+        rd = rdn / (1.0 + rdn / KARMAN * (alz - psi_mh))
+        rh = rhn / (1.0 + rhn / KARMAN * (alz - psi_xh))
+        re = ren / (1.0 + ren / KARMAN * (alz - psi_xh))
+        u_star, t_star, q_star = rd * v_mag, rh * delta_t, re * delta_q
+    tau = rho_bottom * u_star * u_star
+    tau_x = tau * u_bottom / v_mag
+    tau_y = tau * v_bottom / v_mag
+    sen = cp * tau * t_star / u_star
+    evap = LAT_VAP * tau * q_star / u_star / LAT_VAP
+    _, psi_x2 = psi_m_s(hol * SFC_Z_T_REF / z_bottom)
+    al2 = math.log(SFC_Z_REF / SFC_Z_T_REF)
+    fac = (rh / KARMAN) * (alz + al2 - psi_xh + psi_x2)
+    t_ref = theta_bottom - delta_t * fac - 0.01 * SFC_Z_T_REF
+    fac = (re / KARMAN) * (alz + al2 - psi_xh + psi_x2)
+    q_ref = q_bottom - delta_q * fac
+    return -tau_x, -tau_y, -sen, -evap, t_ref, q_ref, u10n
+
+
+def surface_fluxes(xi, base, grid):
+    """atm_ocn_flux on the lowest level of state xi as rk_sub_step0 feeds it
+    (one_step_integration.py:37-47): u, v averaged to the cell centre, rho0 of level 1,
+    the height of the first cell centre; base[5] is the surface temperature (nx, ny)."""
+    th, u, v, _, _, qv = xi[:6]
+    rho0, surface_t = base[1], base[5]
+    I, J, k = slice(NGX, -NGX), slice(NGY, -NGY), NGZ
+    z_bottom = np.full((grid.nx, grid.ny), 0.5 * grid.dz, dtype=rho0.dtype)
+    u_b = 0.5 * (u[NGX:-(NGX + 1), J, k] + u[NGX + 1:-NGX, J, k])
+    v_b = 0.5 * (v[I, NGY:-(NGY + 1), k] + v[I, NGY + 1:-NGY, k])
+    return atm_ocn_flux(z_bottom, u_b, v_b, th[I, J, k], qv[I, J, k], rho0[I, J, k],
+                        np.asarray(surface_t, dtype=rho0.dtype))
+
+
+def rk_sub_step(stage, xi0, xi, base, grid, opts, dt_stage, frozen=None, heating=None, sfc=None):
     """One WS-RK3 stage (rk_sub_step0/1/2, one_step_integration.py:16-188, :191-317, :320-447).
 
     State tuples are (theta, u, v, w, pip, qv, *passive) with optional passive scalars
     (qc, qr) advected and SGS-diffused like qv.  `frozen` carries the stage-0 SGS
-    tendencies (one_step_integration.py:57-76, :245) into stages 1 and 2.
+    tendencies (one_step_integration.py:57-76, :245) into stages 1 and 2; `heating` the
+    Newtonian heating and `sfc` the surface fluxes (tau_x, tau_y, sen, evap, ...) of the
+    step's first stage (one_step_integration.py:31-52, :208-209).
     Returns (xi_next, frozen, info) with info the BiCGSTAB exit code (solver_opt 1).
     """
     rho0_theta0, rho0, theta0, pi0, qv0 = base[:5]
@@ -966,8 +1058,9 @@ def rk_sub_step(stage, xi0, xi, base, grid, opts, dt_stage, frozen=None, heating
     sgs = frozen
     buoy, b8w = calculate_buoyancy(theta0, th, qv0, qv)
     fdiv = get_divergence(rho0, u, v, w, grid)
-    adv4u = advection_u(rho0, u, v, w, WEPS_MOM, fdiv, zeros2, grid)
-    adv4v = advection_v(rho0, u, v, w, WEPS_MOM, fdiv, zeros2, grid)
+    tau_x, tau_y, sen, evap = (zeros2,) * 4 if sfc is None else tuple(sfc[:4])
+    adv4u = advection_u(rho0, u, v, w, WEPS_MOM, fdiv, tau_x, grid)
+    adv4v = advection_v(rho0, u, v, w, WEPS_MOM, fdiv, tau_y, grid)
     adv4w = advection_w(rho0, u, v, w, WEPS_MOM, fdiv, grid)
     if opts.cor_opt == 1:
         fu, fv = calculate_coriolis_force(u, v)
@@ -998,9 +1091,9 @@ def rk_sub_step(stage, xi0, xi, base, grid, opts, dt_stage, frozen=None, heating
         n, dts = _n_acoustic(stage, dt_stage, opts)
         step = sound_implicit_rk_step if opts.solver_opt == 3 else sound_rk_step
         un, vn, wn, pip_n = step(u0, v0, w0, pi0, pip0, pip, rhs_u, rhs_v, rhs_w, trho, rho0, n, dts, grid)
-    thn, _ = update_scalar_euler(rho0, th0, th, u, v, w, fdiv, zeros2, 0.0 if heating is None else heating,
+    thn, _ = update_scalar_euler(rho0, th0, th, u, v, w, fdiv, sen / Cp, 0.0 if heating is None else heating,
                                  WEPS_THETA, dt_stage, grid)
-    qvn, _ = update_scalar_euler(rho0, qv0_now, qv, u, v, w, fdiv, zeros2, 0.0, WEPS_QV, dt_stage, grid)
+    qvn, _ = update_scalar_euler(rho0, qv0_now, qv, u, v, w, fdiv, evap, 0.0, WEPS_QV, dt_stage, grid)
     pasn = tuple(update_scalar_euler(rho0, a0, a, u, v, w, fdiv, zeros2, 0.0, WEPS_QV, dt_stage, grid)[0]
                  for a0, a in zip(pas0, pas))
     if sgs is not None:
@@ -1021,9 +1114,11 @@ def rk_sub_step(stage, xi0, xi, base, grid, opts, dt_stage, frozen=None, heating
 
 def ws_rk3_big_step(xi0, base, grid, opts):
     """Wicker-Skamarock RK3: every stage restarts from xi0 (one_sprint_functions.py:10-30).
-    With rad_opt the heating is computed from the stage-0 theta and kept for the step
-    (one_step_integration.py:31-35, :208, :338)."""
+    With rad_opt the heating, with sfc_opt the surface fluxes are computed from the stage-0
+    state and kept for the step (one_step_integration.py:31-52, :208-209, :338-339)."""
     kw = {"heating": newton_radiation(xi0[0], base[2])} if opts.rad_opt else {}
+    if opts.sfc_opt:
+        kw["sfc"] = surface_fluxes(xi0, base, grid)
     xi1, frozen, i0 = rk_sub_step(0, xi0, xi0, base, grid, opts, opts.dt / 3.0, **kw)
     xi2, _, i1 = rk_sub_step(1, xi0, xi1, base, grid, opts, opts.dt / 2.0, frozen, **kw)
     xi3, _, i2 = rk_sub_step(2, xi0, xi2, base, grid, opts, opts.dt, frozen, **kw)
