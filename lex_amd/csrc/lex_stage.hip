@@ -89,37 +89,34 @@ __global__ void k_newton_heating(Dims d, const float* __restrict__ th, const flo
 }
 
 // Horizontal mean of theta per level (rayleigh_damping's theta_avg,
-// boundary_conditions.py:136): block i sums its x-plane over y for every level (lanes
-// along z, 4 rows of threads taking every 4th y, FP64, combined in a fixed order), one
-// block then adds the nx partials per level in order.
+// boundary_conditions.py:136), deterministic in FP64: block (i, c) sums rows
+// c*cy .. (c+1)*cy-1 of x-plane i for every level (lanes along z, kMeanRows rows of
+// threads, combined in a fixed order); then one wave per level adds the nx*kMeanChunks
+// partials (lane l takes l, l+64, ... in order, then a fixed butterfly).
 constexpr int kMeanRows = 4;
+constexpr int kMeanChunks = 4;
 __global__ void __launch_bounds__(1024) k_level_partial(Dims d, const float* __restrict__ th,
                                                         double* __restrict__ part) {
   __shared__ double red[kMeanRows][256];
   const int k = threadIdx.x, r = threadIdx.y;
-  const int i = blockIdx.x;
+  const int i = blockIdx.x, c = blockIdx.y;
+  const int cy = (d.ny + kMeanChunks - 1) / kMeanChunks;
+  const int jend = min((c + 1) * cy, d.ny);
   const int kc = k < d.nz ? k : d.nz - 1;
   double s0 = 0.0, s1 = 0.0;
-  int j = r;
-  for (; j + kMeanRows < d.ny; j += 2 * kMeanRows) {
+  int j = c * cy + r;
+  for (; j + kMeanRows < jend; j += 2 * kMeanRows) {
     s0 += (double)th[ic(d, i + 3, j + 3, kc + 1)];
     s1 += (double)th[ic(d, i + 3, j + kMeanRows + 3, kc + 1)];
   }
-  if (j < d.ny) s0 += (double)th[ic(d, i + 3, j + 3, kc + 1)];
+  if (j < jend) s0 += (double)th[ic(d, i + 3, j + 3, kc + 1)];
   red[r][k] = s0 + s1;
   __syncthreads();
   if (r == 0 && k < d.nz) {
     double sum = red[0][k];
     for (int q = 1; q < kMeanRows; ++q) sum += red[q][k];
-    part[(long)i * d.nz + k] = sum;
+    part[((long)i * kMeanChunks + c) * d.nz + k] = sum;
   }
-}
-__global__ void k_level_mean(Dims d, const double* __restrict__ part, float* __restrict__ mean) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= d.nz) return;
-  double sum = 0.0;
-  for (int i = 0; i < d.nx; ++i) sum += part[(long)i * d.nz + k];
-  mean[k] = (float)(sum / ((double)d.nx * (double)d.ny));
 }
 
 // Ramp of the damping layer at height z (setup_damping_tau, setup_lex.py:149-156): zero
@@ -129,24 +126,46 @@ __device__ __forceinline__ float damp_tau(double z, double zd, double depth) {
   return (z <= zd) ? 0.0f : (float)(0.5 * (1.0 - cos(M_PI * (z - zd) / depth)));
 }
 
+// Per level k: the theta mean, the ramp at the cell centre (tauh) and at w level k (tauf;
+// the last wave also writes w level nz).  Blocks of 4 waves, one level per wave.
+__global__ void __launch_bounds__(256) k_level_mean(Dims d, double zd, const double* __restrict__ part,
+                                                    float* __restrict__ mean, float* __restrict__ tauh,
+                                                    float* __restrict__ tauf) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= d.nz) return;   // whole waves
+  const int np = d.nx * kMeanChunks;
+  double s = 0.0;
+  for (int p = lane; p < np; p += 64) s += part[(long)p * d.nz + k];
+  for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+  if (lane == 0) {
+    const double dz = (double)d.dz, depth = (d.nz + 1) * dz - zd;
+    mean[k] = (float)(s / ((double)d.nx * (double)d.ny));
+    tauh[k] = damp_tau((k + 0.5) * dz, zd, depth);
+    tauf[k] = damp_tau((double)k * dz, zd, depth);
+    if (k == d.nz - 1) tauf[d.nz] = damp_tau((double)d.nz * dz, zd, depth);
+  }
+}
+
 // Upper-level Rayleigh damping (rayleigh_damping, boundary_conditions.py:119-141) applied
 // to the stage result in place (one_step_integration.py:164-175): u, v, w relax to zero and
 // theta to its horizontal mean with rate alpha x ramp, ghosts refilled.
 // Levels below k0 have a zero ramp, where the update leaves every value unchanged, so the
-// launch covers levels k0..nz-1 only.
-__global__ void k_rayleigh(Dims d, int k0, double zd, float alpha, float dt, const float* __restrict__ mean,
-                           float* __restrict__ th, float* __restrict__ u, float* __restrict__ v,
-                           float* __restrict__ w) {
-  const TileId tb = xcd_tile();
-  const int k = k0 + tb.x * blockDim.x + threadIdx.x;
-  const int j = tb.y * blockDim.y + threadIdx.y;
-  const int i = tb.z;
-  if (k >= d.nz || j >= d.ny) return;
+// launch covers levels k0..nz-1 only: thread t of x-plane blockIdx.y takes level
+// k0 + t % nl of row t / nl (nl = nz - k0), so a wave runs along whole layer columns.
+__global__ void __launch_bounds__(256) k_rayleigh(Dims d, int k0, float alpha, float dt,
+                                                  const float* __restrict__ mean, const float* __restrict__ tauh,
+                                                  const float* __restrict__ tauf, float* __restrict__ th,
+                                                  float* __restrict__ u, float* __restrict__ v,
+                                                  float* __restrict__ w) {
+  const int nl = d.nz - k0;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (t >= nl * d.ny) return;
+  const int j = t / nl, k = k0 + t % nl;
   const int I = i + 3, J = j + 3, K = k + 1;
-  const double dz = (double)d.dz, depth = (d.nz + 1) * dz - zd;
-  const float th_k = damp_tau((K - 0.5) * dz, zd, depth);   // cell centre, padded level K
-  const float tf_k = damp_tau((double)k * dz, zd, depth);   // w level k
-  const float tf_top = damp_tau((double)d.nz * dz, zd, depth);
+  const float th_k = tauh[k];
+  const float tf_k = tauf[k];
   const float hu = 0.5f * (th_k + th_k);
   const float un = u[iu(d, I, J, K)], vn = v[iv(d, I, J, K)], wn = w[iw(d, I, J, K)], tn = th[ic(d, I, J, K)];
   store_u_fast(d, u, i, j, k, un + (-alpha * (hu * un)) * dt);
@@ -162,7 +181,7 @@ __global__ void k_rayleigh(Dims d, int k0, double zd, float alpha, float dt, con
   store_w_fast(d, w, i, j, k, wn + (-alpha * (tf_k * wn)) * dt);
   if (k == d.nz - 1) {
     const float wt = w[iw(d, I, J, d.nz + 1)];
-    store_w(d, w, i, j, d.nz, wt + (-alpha * (tf_top * wt)) * dt);
+    store_w(d, w, i, j, d.nz, wt + (-alpha * (tauf[d.nz] * wt)) * dt);
   }
   store_c_fast(d, th, i, j, k, tn + (-alpha * th_k * (tn - mean[k])) * dt);
 }
@@ -177,7 +196,8 @@ struct Rk3Layout {
   size_t off_su, off_sv, off_sw, off_ss;   // frozen SGS tendencies (ss: 2+np scalars)
   size_t off_smag;              // 10 padded scratch fields
   size_t off_heat;              // Newtonian heating of theta (interior), rad
-  size_t off_lmean;             // per-level partial sums (nx x nz doubles) and means (nz floats), damp
+  size_t off_lmean;             // damp: per-level partial sums (nx x kMeanChunks x nz doubles), then
+                                // the theta mean (nz floats) and the ramps tauh (nz), tauf (nz+1)
   size_t off_sfc;               // surface fluxes: 8 (nx, ny) fields (SfcOut order), sfc
   size_t total;
   int np;
@@ -214,7 +234,7 @@ inline Rk3Layout rk3_layout(const Dims& d, int np) {
   L.off_ss = o; o += (2 + np) * al256(L.ic_ * 4);
   L.off_smag = o; o += 10 * al256(L.nc * 4);
   L.off_heat = o; o += al256(L.ic_ * 4);
-  L.off_lmean = o; o += al256((size_t)d.nx * d.nz * 8) + al256((size_t)d.nz * 4);
+  L.off_lmean = o; o += al256((size_t)d.nx * kMeanChunks * d.nz * 8) + 3 * al256((size_t)(d.nz + 1) * 4);
   L.off_sfc = o; o += 8 * al256((size_t)d.nx * d.ny * 4);
   L.total = o;
   return L;
@@ -291,7 +311,9 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
   for (int q = 0; q < 10; ++q) sm[q] = (float*)(ws + L.off_smag + q * al256(L.nc * 4));
   float* heat = o.rad ? (float*)(ws + L.off_heat) : nullptr;
   double* lpart = (double*)(ws + L.off_lmean);
-  float* lmean = (float*)(ws + L.off_lmean + al256((size_t)d.nx * d.nz * 8));
+  float* lmean = (float*)(ws + L.off_lmean + al256((size_t)d.nx * kMeanChunks * d.nz * 8));
+  float* ltauh = lmean + al256((size_t)(d.nz + 1) * 4) / 4;
+  float* ltauf = ltauh + al256((size_t)(d.nz + 1) * 4) / 4;
   const SfcOut sf = sfc_fields(ws, L, d);
   const float* tau_x = o.sfc ? sf.tau_x : nullptr;
   const float* tau_y = o.sfc ? sf.tau_y : nullptr;
@@ -469,14 +491,17 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
     }
     // upper-level Rayleigh damping of the stage result (one_step_integration.py:164-175)
     if (o.damp) {
-      LEX_LAUNCH(KID_DAMPING, k_level_partial, dim3(d.nx), dim3(64 * ((d.nz + 63) / 64), kMeanRows), 0, st, d,
-                 (const float*)dst[0], lpart);
-      LEX_LAUNCH(KID_DAMPING, k_level_mean, dim3((d.nz + 63) / 64), dim3(64), 0, st, d, (const double*)lpart, lmean);
+      LEX_LAUNCH(KID_DAMPING, k_level_partial, dim3(d.nx, kMeanChunks), dim3(64 * ((d.nz + 63) / 64), kMeanRows), 0,
+                 st, d, (const float*)dst[0], lpart);
+      LEX_LAUNCH(KID_DAMPING, k_level_mean, dim3((d.nz + 3) / 4), dim3(256), 0, st, d, (double)o.z_damping,
+                 (const double*)lpart, lmean, ltauh, ltauf);
       // first level whose cell centre or lower w level can lie above z_damping
       int k0 = (int)floor((double)o.z_damping / (double)d.dz - 0.5);
       k0 = k0 < 0 ? 0 : (k0 > d.nz - 1 ? d.nz - 1 : k0);
-      LEX_LAUNCH(KID_DAMPING, k_rayleigh, G(d.nz - k0, d.ny, d.nx), bk, 0, st, d, k0, (double)o.z_damping,
-                 o.rd_alpha, (float)dt_s, (const float*)lmean, dst[0], dst[1], dst[2], dst[3]);
+      const int nl = d.nz - k0;
+      LEX_LAUNCH(KID_DAMPING, k_rayleigh, dim3((unsigned)((nl * d.ny + 255) / 256), d.nx), dim3(256), 0, st, d, k0,
+                 o.rd_alpha, (float)dt_s, (const float*)lmean, (const float*)ltauh, (const float*)ltauf, dst[0],
+                 dst[1], dst[2], dst[3]);
     }
   }
   hipError_t e = hipGetLastError();
