@@ -199,7 +199,8 @@ def dl_sprint_timing(sprint_n=10):
 def physics_timing(steps=10):
     """The 256x256x128 step with the upper-level Rayleigh damping (layer above 9.6 km of
     the 12.8 km column), the Newtonian heating and the bulk surface fluxes (sea surface
-    at the set-up's 300 K) switched on, against the plain step; and the surface-flux
+    at the set-up's 300 K) switched on, and with the horizontal / vertical Smagorinsky
+    closure (turb_opt 2), against the plain step (turb_opt 1); and the surface-flux
     kernel alone on the 256x256 columns."""
     import torch
     from lex_amd import _lib, boundary_conditions as bcm, namelist_n_constants as nl, one_sprint_functions as sp
@@ -220,7 +221,8 @@ def physics_timing(steps=10):
     torch.cuda.synchronize()
     out["surface_fluxes_us_per_call"] = e0.elapsed_time(e1) / reps * 1e3
     for name, mo in (("plain", (2, False, False, 0, False, False, 1)), ("damp_rad", (2, True, True, 0, False, False, 1)),
-                     ("sfc", (2, False, False, 0, True, False, 1)), ("damp_rad_sfc", (2, True, True, 0, True, False, 1))):
+                     ("sfc", (2, False, False, 0, True, False, 1)), ("damp_rad_sfc", (2, True, True, 0, True, False, 1)),
+                     ("turb2", (2, False, False, 0, False, False, 2))):
         st = tuple(a.clone() for a in phys)
         sp.fused_step(st, base, g, mo)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

@@ -151,6 +151,22 @@ int lex_smag_momentum(const lex_grid_t* g, const float* rho0_col, const float* k
 /* compute_smag (turbulence.py:80-109): one scalar's SGS tendency (interior) */
 int lex_smag_scalar(const lex_grid_t* g, const float* rho0_col, const float* kh, const float* s, float* out,
                     void* stream);
+/* compute_k2 (turbulence.py:449-471), turb_opt 2: horizontal Kmh = cs^2 dx dy Dh and vertical
+ * Kmv = cs^2 Lv^2 sqrt(max(Dv - n2, 0)) (prandtl 1, so Khh = Kmh, Khv = Kmv) from the
+ * deformation's s11..s23 and n2 (padded) -> padded kmh, kmv */
+int lex_smag_k2(const lex_grid_t* g, const float* rho0_col, const float* s11, const float* s22, const float* s33,
+                const float* s12, const float* s13, const float* s23, const float* n2, float cs, float* kmh,
+                float* kmv, void* stream);
+/* compute_smag2 (turbulence.py:114-295): momentum part with kmh on tau11, tau22, tau33,
+ * tau12 and kmv on tau13, tau23.  sgs_w at the surface is written as zero (compute_smag2
+ * leaves it at its stencil value; the step sets w to zero there after every substep). */
+int lex_smag2_momentum(const lex_grid_t* g, const float* rho0_col, const float* kmh, const float* kmv,
+                       const float* s11, const float* s22, const float* s33, const float* s12, const float* s13,
+                       const float* s23, float* sgs_u, float* sgs_v, float* sgs_w, void* stream);
+/* compute_smag2 (turbulence.py:207-234): one scalar's SGS tendency with khh on the x / y
+ * faces and khv on the z faces */
+int lex_smag2_scalar(const lex_grid_t* g, const float* rho0_col, const float* khh, const float* khv, const float* s,
+                     float* out, void* stream);
 
 /* ---- I1: hydrostatic base state (FP64 on the GPU) ---------------------------------- */
 /* setup_ic_option1 (setup_lex.py:103-137): rh_col (nz+2 doubles, device) ->
@@ -163,7 +179,8 @@ int lex_base_state(int nz, double dz, double theta0, const double* rh_col, doubl
 typedef struct lex_step_opts {
   float dt;          /* full step (namelist dt) */
   int n_sound;       /* acoustic substeps per full step (namelist n_sound) */
-  int turb;          /* 1 = Smagorinsky (turb_opt 1), 0 = off */
+  int turb;          /* 1 = Smagorinsky (turb_opt 1), 2 = horizontal / vertical Smagorinsky
+                        (turb_opt 2, turbulence.py:114-295, 449-471), 0 = off */
   float cs;          /* Smagorinsky constant (turbulence.py:433) */
   float prandtl;     /* turbulence.py:432 */
   int n_passive;     /* passive scalars advected and diffused like qv (qc, qr) */
@@ -336,7 +353,7 @@ int lex_sgs_apply(const lex_grid_t* g, const float* y, const float* out_scale, f
 int lex_coarsen(const lex_grid_t* g, int mode, int fx, int fy, int fz, const float* a, float* out, void* stream);
 /* lex_rk3_step with caller-supplied SGS tendencies (frozen over the step, as the
  * stage-0 Smagorinsky tendencies are, one_step_integration.py:57-76): su, sv, sw on the
- * u, v, w points, ss[] for theta, qv and the passive scalars. */
+ * u, v, w points, ss[] for theta, qv and the passive scalars.  o->turb is ignored. */
 int lex_rk3_step_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b, float* const* state,
                          const float* su, const float* sv, const float* sw, const float* const* ss, void* workspace,
                          size_t workspace_bytes, void* stream);

@@ -97,6 +97,9 @@ _SIGS = {
     "lex_smag_k": [_G, _P, _P, _F, _F, _P, _P, _P],
     "lex_smag_momentum": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "lex_smag_scalar": [_G, _P, _P, _P, _P, _P],
+    "lex_smag_k2": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P],
+    "lex_smag2_momentum": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lex_smag2_scalar": [_G, _P, _P, _P, _P, _P, _P],
     "lex_base_state": [_I, _D, _D, _P, _P, _I, _D, C.POINTER(_I), _P],
     "lex_rk3_workspace_bytes": [_G, _I],
     "lex_rk3_step": [_G, C.POINTER(StepOpts), C.POINTER(BaseCols), C.POINTER(_P), _P, C.c_size_t, _P],

@@ -150,8 +150,8 @@ void rk3_forward_tape(const Dims& d, const O& o, const B& b, float* const* X0, c
     float* ru = (float*)(ws + L.off_ru[s]);
     float* rv = (float*)(ws + L.off_rv[s]);
     float* rw = (float*)(ws + L.off_rw[s]);
-    // turb 1: Smagorinsky recorded here; turb 2: the caller's SGS tendencies are already in
-    // the tape's SGS slots
+    // turb 1: Smagorinsky recorded here; kTurbExternal: the caller's SGS tendencies are
+    // already in the tape's SGS slots
     if (s == 0 && o.turb == 1) {
       hipLaunchKernelGGL(k_smag_deform, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, rc, src[1], src[2], src[3], sm[0],
                          sm[1], sm[2], sm[3], sm[4], sm[5], sm[6]);
@@ -161,11 +161,11 @@ void rk3_forward_tape(const Dims& d, const O& o, const B& b, float* const* X0, c
       long n = (long)L.nc;
       hipLaunchKernelGGL(k_smag_k, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, sm[6], sm[7], sm[8], sm[9],
                          cg * cg, o.prandtl);
-      hipLaunchKernelGGL(k_smag_uvw, cx.G(d.nz + 1, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, sm[8], sm[0], sm[1],
-                         sm[2], sm[3], sm[4], sm[5], su, sv, sw);
+      hipLaunchKernelGGL(k_smag_uvw, cx.G(d.nz + 1, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, sm[8], sm[8], sm[0],
+                         sm[1], sm[2], sm[3], sm[4], sm[5], su, sv, sw);
       for (int q = 0; q < nsc; ++q) {
         int fi = (q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2));
-        hipLaunchKernelGGL(k_smag_scalar, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, rc, sm[9], src[fi], ss[q]);
+        hipLaunchKernelGGL(k_smag_scalar, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, rc, sm[9], sm[9], src[fi], ss[q]);
       }
     }
     // the fused stage kernels of the forward step where they apply (same trajectory as

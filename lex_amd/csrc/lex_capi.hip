@@ -392,8 +392,8 @@ int lex_smag_momentum(const lex_grid_t* g, const float* r, const float* km, cons
   REQGRID(g);
   REQ(r); REQ(km); REQ(s11); REQ(s22); REQ(s33); REQ(s12); REQ(s13); REQ(s23); REQ(su); REQ(sv); REQ(sw);
   Dims d = mkdims(g);
-  hipLaunchKernelGGL(k_smag_uvw, grd(d.nz + 1, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, r, km, s11, s22, s33, s12,
-                     s13, s23, su, sv, sw);
+  hipLaunchKernelGGL(k_smag_uvw, grd(d.nz + 1, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, r, km, km, s11, s22, s33,
+                     s12, s13, s23, su, sv, sw);
   return check_launch("lex_smag_momentum");
 }
 
@@ -401,8 +401,40 @@ int lex_smag_scalar(const lex_grid_t* g, const float* r, const float* kh, const 
   REQGRID(g);
   REQ(r); REQ(kh); REQ(s); REQ(out);
   Dims d = mkdims(g);
-  hipLaunchKernelGGL(k_smag_scalar, grd(d.nz, d.ny, d.nx), blk(), 0, S(stream), d, r, kh, s, out);
+  hipLaunchKernelGGL(k_smag_scalar, grd(d.nz, d.ny, d.nx), blk(), 0, S(stream), d, r, kh, kh, s, out);
   return check_launch("lex_smag_scalar");
+}
+
+int lex_smag_k2(const lex_grid_t* g, const float* r, const float* s11, const float* s22, const float* s33,
+                const float* s12, const float* s13, const float* s23, const float* n2, float cs, float* kmh, float* kmv,
+                void* stream) {
+  REQGRID(g);
+  REQ(r); REQ(s11); REQ(s22); REQ(s33); REQ(s12); REQ(s13); REQ(s23); REQ(n2); REQ(kmh); REQ(kmv);
+  Dims d = mkdims(g);
+  long n = d.nc();
+  hipLaunchKernelGGL(k_smag_k2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), d, r, s11, s22, s33, s12,
+                     s13, s23, n2, kmh, kmv, cs * cs);
+  return check_launch("lex_smag_k2");
+}
+
+int lex_smag2_momentum(const lex_grid_t* g, const float* r, const float* kmh, const float* kmv, const float* s11,
+                       const float* s22, const float* s33, const float* s12, const float* s13, const float* s23,
+                       float* su, float* sv, float* sw, void* stream) {
+  REQGRID(g);
+  REQ(r); REQ(kmh); REQ(kmv); REQ(s11); REQ(s22); REQ(s33); REQ(s12); REQ(s13); REQ(s23); REQ(su); REQ(sv); REQ(sw);
+  Dims d = mkdims(g);
+  hipLaunchKernelGGL(k_smag_uvw, grd(d.nz + 1, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, r, kmh, kmv, s11, s22,
+                     s33, s12, s13, s23, su, sv, sw);
+  return check_launch("lex_smag2_momentum");
+}
+
+int lex_smag2_scalar(const lex_grid_t* g, const float* r, const float* khh, const float* khv, const float* s,
+                     float* out, void* stream) {
+  REQGRID(g);
+  REQ(r); REQ(khh); REQ(khv); REQ(s); REQ(out);
+  Dims d = mkdims(g);
+  hipLaunchKernelGGL(k_smag_scalar, grd(d.nz, d.ny, d.nx), blk(), 0, S(stream), d, r, khh, khv, s, out);
+  return check_launch("lex_smag2_scalar");
 }
 
 // ---------------------------------- I1 -------------------------------------------
@@ -477,8 +509,18 @@ size_t lex_rk3_workspace_bytes(const lex_grid_t* g, int n_passive) {
   return rk3_layout(mkdims(g), n_passive).total;
 }
 
+static int rk3_step_entry(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b,
+                          float* const* state, void* ws, size_t ws_bytes, void* stream);
+
 int lex_rk3_step(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b, float* const* state,
                  void* ws, size_t ws_bytes, void* stream) {
+  REQ(o);
+  if (o->turb < 0 || o->turb > 2) return fail(-1, "lex_rk3_step: turb must be 0, 1 or 2%s");
+  return rk3_step_entry(g, o, b, state, ws, ws_bytes, stream);
+}
+
+static int rk3_step_entry(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b,
+                          float* const* state, void* ws, size_t ws_bytes, void* stream) {
   REQGRID(g);
   REQ(o); REQ(b); REQ(state); REQ(ws);
   REQ(b->rho0); REQ(b->theta0); REQ(b->qv0); REQ(b->pi0);
@@ -683,7 +725,7 @@ int lex_rk3_step_vjp(const lex_grid_t* g, const lex_step_opts_t* o, const lex_ba
       return fail(-1, "lex_rk3_step_vjp: lam_in must not alias lam_out or the state%s");
   }
   if (o->solver == 3) return fail(-1, "lex_rk3_step_vjp: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
-  if (o->damp || o->rad || o->sfc) return fail(-1, "lex_rk3_step_vjp: the adjoint covers damp_opt, rad_opt and sfc_opt off%s");
+  if (o->damp || o->rad || o->sfc || o->turb == 2) return fail(-1, "lex_rk3_step_vjp: the adjoint covers damp_opt, rad_opt, sfc_opt and turb_opt 2 off%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (ws_bytes < L.total) return fail(-3, "lex_rk3_step_vjp: workspace too small%s");
@@ -717,7 +759,7 @@ int lex_rk3_step_tape(const lex_grid_t* g, const lex_step_opts_t* o, const lex_b
     if (state[q] == state_out[q]) return fail(-1, "lex_rk3_step_tape: state_out must not alias state%s");
   }
   if (o->solver == 3) return fail(-1, "lex_rk3_step_tape: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
-  if (o->damp || o->rad || o->sfc) return fail(-1, "lex_rk3_step_tape: the adjoint covers damp_opt, rad_opt and sfc_opt off%s");
+  if (o->damp || o->rad || o->sfc || o->turb == 2) return fail(-1, "lex_rk3_step_tape: the adjoint covers damp_opt, rad_opt, sfc_opt and turb_opt 2 off%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_tape: tape too small%s");
@@ -739,7 +781,7 @@ int lex_rk3_step_vjp_taped(const lex_grid_t* g, const lex_step_opts_t* o, const 
       return fail(-1, "lex_rk3_step_vjp_taped: lam_in must not alias lam_out or the state%s");
   }
   if (o->solver == 3) return fail(-1, "lex_rk3_step_vjp_taped: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
-  if (o->damp || o->rad || o->sfc) return fail(-1, "lex_rk3_step_vjp_taped: the adjoint covers damp_opt, rad_opt and sfc_opt off%s");
+  if (o->damp || o->rad || o->sfc || o->turb == 2) return fail(-1, "lex_rk3_step_vjp_taped: the adjoint covers damp_opt, rad_opt, sfc_opt and turb_opt 2 off%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_vjp_taped: tape too small%s");
@@ -780,7 +822,7 @@ int lex_rk3_step_tape_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o, con
   cp(sw, L.off_sw, L.iw_);
   for (int q = 0; q < 2 + o->n_passive; ++q) cp(ss[q], L.off_ss + q * al256(L.ic_ * 4), L.ic_);
   lex_step_opts_t o2 = *o;
-  o2.turb = 2;
+  o2.turb = kTurbExternal;
   rk3_forward_tape(d, o2, *b, state, t, L, st, state_out);
   return check_launch("lex_rk3_step_tape_ext_sgs");
 }
@@ -809,7 +851,7 @@ int lex_rk3_step_vjp_taped_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o
   if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_vjp_taped_ext_sgs: tape too small%s");
   if (adj_bytes < L.total - L.off_lXA) return fail(-3, "lex_rk3_step_vjp_taped_ext_sgs: adjoint workspace too small%s");
   lex_step_opts_t o2 = *o;
-  o2.turb = 2;
+  o2.turb = kTurbExternal;
   hipStream_t st = S(stream);
   int rc = rk3_step_vjp(d, o2, *b, state, lam_out, lam_in, nullptr, (char*)tape, (char*)adj, L, st, false);
   if (rc > 0) return fail(rc, "lex_rk3_step_vjp_taped_ext_sgs: HIP error %s", hipGetErrorString((hipError_t)rc));
@@ -1102,8 +1144,8 @@ int lex_rk3_step_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o, const le
     cp(ss[q], L.off_ss + q * al256(L.ic_ * 4), L.ic_);
   }
   lex_step_opts_t o2 = *o;
-  o2.turb = 2;
-  return lex_rk3_step(g, &o2, b, state, ws, ws_bytes, stream);
+  o2.turb = kTurbExternal;
+  return rk3_step_entry(g, &o2, b, state, ws, ws_bytes, stream);
 }
 
 }  // extern "C"

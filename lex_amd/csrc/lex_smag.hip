@@ -1,8 +1,13 @@
-// G1: isotropic Smagorinsky-Lilly SGS with the N^2 correction (turbulence.py).
+// G1: Smagorinsky-Lilly SGS with the N^2 correction (turbulence.py).
 //   k_smag_deform : compute_deformation (turbulence.py:298-388)  -> rho*S_ij, deformation
 //   k_smag_nm     : compute_nm          (turbulence.py:391-427)  -> N^2
-//   k_smag_k      : compute_k           (turbulence.py:430-446)  -> Km, Kh
-//   k_smag_uvw / k_smag_scalar : compute_smag (turbulence.py:9-111)
+//   k_smag_k      : compute_k           (turbulence.py:430-446)  -> Km, Kh (turb_opt 1)
+//   k_smag_k2     : compute_k2          (turbulence.py:449-471)  -> Kmh, Kmv (turb_opt 2)
+//   k_smag_uvw / k_smag_scalar(s) : compute_smag (turbulence.py:9-111) and compute_smag2
+//                   (:114-295), which differ only in the coefficients: horizontal (kmh, khh)
+//                   for tau11, tau22, tau33, tau12 and the x / y scalar fluxes, vertical
+//                   (kmv, khv) for tau13, tau23 and the z scalar fluxes.  turb_opt 1 passes
+//                   the same array for both.
 #pragma once
 #include "lex_common.hpp"
 
@@ -139,13 +144,41 @@ __global__ void k_smag_k(long n, const float* __restrict__ s2, const float* __re
   kh[t] = m / prandtl;
 }
 
-// tau_ij = -2 km s_ij at padded scalar index c
-#define TAU(arr, c) (-2.0f * km[c] * arr[c])
+// compute_k2 (turbulence.py:449-471), elementwise over the padded arrays with prandtl 1:
+// Kmh = Cs^2 dx dy Dh with Dh = (2 (s11^2 + s22^2 + s33^2) + 4 s12^2) / rho^2 (no square
+// root, as the reference has it); Kmv = Cs^2 Lv^2 sqrt(max(Dv - N^2, 0)) with
+// Dv = 4 (s13^2 + s23^2) / rho^2 and 1/Lv^2 = (0.4 z)^-2 + dz^-2, z the cell-centre height
+// of the padded level ((K - 1/2) dz, negative in the ghost level below the surface).
+// Khh = Kmh and Khv = Kmv (prandtl 1).
+__global__ void k_smag_k2(Dims d, const float* __restrict__ rc, const float* __restrict__ s11,
+                          const float* __restrict__ s22, const float* __restrict__ s33, const float* __restrict__ s12,
+                          const float* __restrict__ s13, const float* __restrict__ s23, const float* __restrict__ n2,
+                          float* __restrict__ kmh, float* __restrict__ kmv, float csq) {
+  const long n = d.nc();
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int K = (int)(t % (d.nz + 2));
+  const float r = rc[K];
+  const float z = ((float)K - 0.5f) * d.dz;
+  const float zk = 0.4f * z;
+  const float lv2 = 1.0f / (1.0f / (zk * zk) + 1.0f / (d.dz * d.dz));
+  const float a11 = s11[t], a22 = s22[t], a33 = s33[t], a12 = s12[t], a13 = s13[t], a23 = s23[t];
+  const float dh = (2.0f * (a11 * a11 + a22 * a22 + a33 * a33) + 4.0f * (a12 * a12)) / (r * r);
+  const float dv = 4.0f * (a13 * a13 + a23 * a23) / (r * r);
+  kmh[t] = csq * (d.dx * d.dy) * dh;
+  kmv[t] = csq * lv2 * sqrtf(fmaxf(dv - n2[t] / 1.0f, 0.0f));
+}
 
-__global__ void k_smag_uvw(Dims d, const float* __restrict__ rc, const float* __restrict__ km,
-                           const float* __restrict__ s11, const float* __restrict__ s22, const float* __restrict__ s33,
-                           const float* __restrict__ s12, const float* __restrict__ s13, const float* __restrict__ s23,
-                           float* __restrict__ su, float* __restrict__ sv, float* __restrict__ sw) {
+// tau_ij = -2 km s_ij at padded scalar index c: horizontal coefficient for 11, 22, 33, 12,
+// vertical for 13, 23
+#define TAU(arr, c) (-2.0f * kmh[c] * arr[c])
+#define TAUV(arr, c) (-2.0f * kmv[c] * arr[c])
+
+__global__ void k_smag_uvw(Dims d, const float* __restrict__ rc, const float* __restrict__ kmh,
+                           const float* __restrict__ kmv, const float* __restrict__ s11, const float* __restrict__ s22,
+                           const float* __restrict__ s33, const float* __restrict__ s12, const float* __restrict__ s13,
+                           const float* __restrict__ s23, float* __restrict__ su, float* __restrict__ sv,
+                           float* __restrict__ sw) {
   const TileId tb = xcd_tile();
   int k = tb.x * blockDim.x + threadIdx.x;  // 0..nz
   int j = tb.y * blockDim.y + threadIdx.y;  // 0..ny
@@ -156,11 +189,11 @@ __global__ void k_smag_uvw(Dims d, const float* __restrict__ rc, const float* __
   // interpolate_scalar2w_0 of tau at w level l: zero at l=0, nz
   auto t13w = [&](long col, int l) -> float {
     if (l <= 0 || l >= nz) return 0.0f;
-    return 0.5f * (TAU(s13, col + l) + TAU(s13, col + l + 1));
+    return 0.5f * (TAUV(s13, col + l) + TAUV(s13, col + l + 1));
   };
   auto t23w = [&](long col, int l) -> float {
     if (l <= 0 || l >= nz) return 0.0f;
-    return 0.5f * (TAU(s23, col + l) + TAU(s23, col + l + 1));
+    return 0.5f * (TAUV(s23, col + l) + TAUV(s23, col + l + 1));
   };
   if (k < nz && j < d.ny) {  // sgs_u at u face i (0..nx)
     long c = ic(d, I, J, K);
@@ -194,8 +227,8 @@ __global__ void k_smag_uvw(Dims d, const float* __restrict__ rc, const float* __
       int l = k;
       long cu = ic(d, I, J, l + 1), cl = ic(d, I, J, l);  // cells above and below the level
       float d33 = (TAU(s33, cu) - TAU(s33, cl)) * d.idz;
-      auto d13x = [&](long cc) { return (TAU(s13, cc + sx) - TAU(s13, cc - sx)) / (2.0f * d.dx); };
-      auto d23y = [&](long cc) { return (TAU(s23, cc + sy) - TAU(s23, cc - sy)) / (2.0f * d.dy); };
+      auto d13x = [&](long cc) { return (TAUV(s13, cc + sx) - TAUV(s13, cc - sx)) / (2.0f * d.dx); };
+      auto d23y = [&](long cc) { return (TAUV(s23, cc + sy) - TAUV(s23, cc - sy)) / (2.0f * d.dy); };
       float a = 0.5f * (d13x(cl) + d13x(cu));
       float b = 0.5f * (d23y(cl) + d23y(cu));
       float r8w = 0.5f * (rc[l] + rc[l + 1]);
@@ -205,9 +238,10 @@ __global__ void k_smag_uvw(Dims d, const float* __restrict__ rc, const float* __
   }
 }
 
-// SGS diffusion of a scalar (turbulence.py:80-109)
+// SGS diffusion of a scalar (turbulence.py:80-109, :207-234): khh on the x / y faces,
+// khv on the z faces
 __device__ float smag_scalar_point(const Dims& d, const float* __restrict__ rc, const float* __restrict__ kh,
-                                   const float* __restrict__ s, int i, int j, int k) {
+                                   const float* __restrict__ khv, const float* __restrict__ s, int i, int j, int k) {
   const long sx = d.sxc(), sy = d.syc();
   int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
   long c = ic(d, I, J, K);
@@ -220,20 +254,20 @@ __device__ float smag_scalar_point(const Dims& d, const float* __restrict__ rc, 
     if (l <= 0 || l >= nz) return 0.0f;
     long cl = c - K + l;
     float r8w = 0.5f * (rc[l] + rc[l + 1]);
-    return -r8w * (0.5f * (kh[cl] + kh[cl + 1])) * ((s[cl + 1] - s[cl]) * d.idz);
+    return -r8w * (0.5f * (khv[cl] + khv[cl + 1])) * ((s[cl + 1] - s[cl]) * d.idz);
   };
   float div = (tx(c + sx) - tx(c)) * d.idx + (ty(c + sy) - ty(c)) * d.idy + (tz(k + 1) - tz(k)) * d.idz;
   return -div / r;
 }
 
 __global__ void k_smag_scalar(Dims d, const float* __restrict__ rc, const float* __restrict__ kh,
-                              const float* __restrict__ s, float* __restrict__ out) {
+                              const float* __restrict__ khv, const float* __restrict__ s, float* __restrict__ out) {
   const TileId tb = xcd_tile();
   int k = tb.x * blockDim.x + threadIdx.x;
   int j = tb.y * blockDim.y + threadIdx.y;
   int i = tb.z;
   if (k >= d.nz || j >= d.ny) return;
-  out[((long)i * d.ny + j) * d.nz + k] = smag_scalar_point(d, rc, kh, s, i, j, k);
+  out[((long)i * d.ny + j) * d.nz + k] = smag_scalar_point(d, rc, kh, khv, s, i, j, k);
 }
 
 // All scalars' SGS diffusion in one pass: the diffusivity faces (kh averaged to the six
@@ -244,7 +278,8 @@ struct SgsScalars {
   int n;
 };
 
-__global__ void k_smag_scalars(Dims d, const float* __restrict__ rc, const float* __restrict__ kh, SgsScalars S) {
+__global__ void k_smag_scalars(Dims d, const float* __restrict__ rc, const float* __restrict__ kh,
+                               const float* __restrict__ khv, SgsScalars S) {
   const TileId tb = xcd_tile();
   int k = tb.x * blockDim.x + threadIdx.x;
   int j = tb.y * blockDim.y + threadIdx.y;
@@ -261,8 +296,8 @@ __global__ void k_smag_scalars(Dims d, const float* __restrict__ rc, const float
   // vertical faces: zero flux at the surface and the lid (levels c-1 and c+1 exist in the
   // padded column, so the loads need no branch)
   const bool zlo = k > 0, zhi = k + 1 < nz;
-  const float kz0 = zlo ? -(0.5f * (rc[K - 1] + rc[K])) * (0.5f * (kh[c - 1] + kh[c])) : 0.0f;
-  const float kz1 = zhi ? -(0.5f * (rc[K] + rc[K + 1])) * (0.5f * (kh[c] + kh[c + 1])) : 0.0f;
+  const float kz0 = zlo ? -(0.5f * (rc[K - 1] + rc[K])) * (0.5f * (khv[c - 1] + khv[c])) : 0.0f;
+  const float kz1 = zhi ? -(0.5f * (rc[K] + rc[K + 1])) * (0.5f * (khv[c] + khv[c + 1])) : 0.0f;
   const long o = ((long)i * d.ny + j) * d.nz + k;
   for (int q = 0; q < S.n; ++q) {
     const float* sc = S.s[q];
@@ -278,4 +313,5 @@ __global__ void k_smag_scalars(Dims d, const float* __restrict__ rc, const float
 }
 
 #undef TAU
+#undef TAUV
 }  // namespace lex

@@ -263,6 +263,10 @@ inline void acou_ptrs(char* base, const Rk3Layout& L, float** f) {  // u, v, w, 
   f[3] = (float*)(base + o);
 }
 
+// turb code of a step whose SGS tendencies the caller has placed in the workspace (or
+// tape) slots, frozen over the step: the learned SGS net
+constexpr int kTurbExternal = 9;
+
 struct StepOpts {
   float dt;
   int n_sound;
@@ -336,21 +340,33 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
     if (s == 0 && o.sfc)
       LEX_LAUNCH(KID_SFC, k_atm_ocn_flux, dim3((d.ny + 63) / 64, d.nx, 1), dim3(64, 1, 1), 0, st, d, rc, X0[0],
                  X0[1], X0[2], X0[5], b.surface_t, sf);
-    // turb 1: Smagorinsky computed here in stage 0; turb 2: SGS tendencies supplied by
-    // the caller in the workspace slots (the learned SGS net); both frozen over the step
+    // turb 1 / 2: Smagorinsky (isotropic / horizontal-vertical) computed here in stage 0;
+    // kTurbExternal: SGS tendencies supplied by the caller in the workspace slots (the
+    // learned SGS net); all frozen over the step
     const bool turb = o.turb != 0;
-    if (s == 0 && o.turb == 1) {
-      // compute_deformation -> compute_nm -> compute_k -> compute_smag
+    if (s == 0 && (o.turb == 1 || o.turb == 2)) {
+      // compute_deformation -> compute_nm -> compute_k (compute_k2) -> compute_smag (compute_smag2)
       LEX_LAUNCH(KID_SMAG_DEFORM, k_smag_deform, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, src[1], src[2], src[3], sm[0], sm[1],
                          sm[2], sm[3], sm[4], sm[5], sm[6]);
       LEX_LAUNCH(KID_SMAG_NM, k_smag_nm, G(d.nz, d.ny, d.nx), bk, 0, st, d, src[0], src[5], sm[7]);
-      float gs = cbrtf(d.dx * d.dy * d.dz);
-      float cg = o.cs * gs;
       long n = (long)L.nc;
-      LEX_LAUNCH(KID_SMAG_K, k_smag_k, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, sm[6], sm[7], sm[8], sm[9],
-                         cg * cg, o.prandtl);
-      LEX_LAUNCH(KID_SMAG_UVW, k_smag_uvw, G(d.nz + 1, d.ny + 1, d.nx + 1), bk, 0, st, d, rc, sm[8], sm[0], sm[1], sm[2],
-                         sm[3], sm[4], sm[5], su, sv, sw);
+      if (o.turb == 1) {
+        float gs = cbrtf(d.dx * d.dy * d.dz);
+        float cg = o.cs * gs;
+        LEX_LAUNCH(KID_SMAG_K, k_smag_k, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, sm[6], sm[7], sm[8],
+                   sm[9], cg * cg, o.prandtl);
+      } else {  // sm[8] = Kmh = Khh, sm[9] = Kmv = Khv
+        LEX_LAUNCH(KID_SMAG_K, k_smag_k2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, rc, sm[0], sm[1],
+                   sm[2], sm[3], sm[4], sm[5], sm[7], sm[8], sm[9], o.cs * o.cs);
+      }
+      // coefficients: turb 1 Km (sm[8]) for every stress, Kh (sm[9]) for every scalar
+      // face; turb 2 Kmh / Kmv (sm[8] / sm[9]) for both
+      const float* kmh = sm[8];
+      const float* kmv = (o.turb == 1) ? sm[8] : sm[9];
+      const float* khh = (o.turb == 1) ? sm[9] : sm[8];
+      const float* khv = sm[9];
+      LEX_LAUNCH(KID_SMAG_UVW, k_smag_uvw, G(d.nz + 1, d.ny + 1, d.nx + 1), bk, 0, st, d, rc, kmh, kmv, sm[0], sm[1],
+                 sm[2], sm[3], sm[4], sm[5], su, sv, sw);
       if (o.impl != 1) {  // every scalar in one pass (impl 1: one call per scalar, as the reference)
         SgsScalars SS;
         SS.n = 2 + o.n_passive;
@@ -358,12 +374,12 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
           SS.s[q] = src[(q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2))];
           SS.out[q] = ss[q];
         }
-        LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalars, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, sm[9], SS);
+        LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalars, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, khh, khv, SS);
       } else {
-        LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, sm[9], src[0], ss[0]);
-        LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, sm[9], src[5], ss[1]);
+        LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, khh, khv, src[0], ss[0]);
+        LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, khh, khv, src[5], ss[1]);
         for (int q = 0; q < o.n_passive; ++q)
-          LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, sm[9], src[6 + q],
+          LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, khh, khv, src[6 + q],
                      ss[2 + q]);
       }
     }

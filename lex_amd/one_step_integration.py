@@ -219,7 +219,8 @@ def _theta_rho(theta, qv):
 # ------------------------------------------------------------------ R1 stages
 def _check_opts(model_opt, physics=False):
     """Validates model_opt; physics=True (the device step driver) also accepts damp_opt,
-    rad_opt and sfc_opt, which the per-routine stage functions here do not implement."""
+    rad_opt, sfc_opt and turb_opt 2, which the per-routine stage functions here do not
+    implement."""
     solver_opt, damp_opt, rad_opt, cor_opt, sfc_opt, pic_opt, turb_opt = model_opt
     if solver_opt not in (1, 2, 3):
         raise NotImplementedError(f"solver_opt {solver_opt} does not exist in the reference")
@@ -230,8 +231,8 @@ def _check_opts(model_opt, physics=False):
         if flag:
             raise NotImplementedError(f"{name} is not built on this path"
                                       + ("" if physics else " (damp_opt, rad_opt, sfc_opt: use the fused step)"))
-    if turb_opt not in (0, 1):
-        raise NotImplementedError("turb_opt 2 (anisotropic Smagorinsky) is not on this path")
+    if turb_opt not in (0, 1, 2) or (turb_opt == 2 and not physics):
+        raise NotImplementedError("turb_opt 2 (horizontal / vertical Smagorinsky) runs in the fused step")
     return solver_opt, cor_opt, turb_opt
 
 

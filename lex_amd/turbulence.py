@@ -1,4 +1,6 @@
-"""Smagorinsky SGS with the reference's signatures (turbulence.py:9-111, :298-446), on HIP."""
+"""Smagorinsky SGS with the reference's signatures (turbulence.py:9-471), on HIP: the
+isotropic closure (compute_k, compute_smag; turb_opt 1) and the horizontal / vertical one
+(compute_k2, compute_smag2; turb_opt 2)."""
 from __future__ import annotations
 
 import torch
@@ -58,3 +60,33 @@ def compute_smag(rho, km, kh, s11, s22, s33, s12, s13, s23, theta, qv, x3d, y3d,
               _lib.stream())
     scal = tuple(compute_scalar_sgs(rho, kh, s, x3d4u, y3d4v, z3d4w) for s in (theta, qv) + tuple(extra_scalars))
     return (su, sv, sw) + scal
+
+
+def compute_k2(rho, s11, s22, s33, s12, s13, s23, n2, x3d4u, y3d4v, z3d4w, z3d, cs=None):
+    """(kmh, kmv, khh, khv), padded (turbulence.py:449-471): prandtl 1, so khh is kmh and
+    khv is kmv; z3d gives the cell-centre heights of the uniform grid."""
+    g = grid_from(rho, x3d4u, y3d4v, z3d4w)
+    kmh = torch.empty_like(s11)
+    kmv = torch.empty_like(s11)
+    _lib.call("lex_smag_k2", g, _lib.ptr(column(rho)), *[_lib.ptr(a) for a in (s11, s22, s33, s12, s13, s23, n2)],
+              float(nl.cs if cs is None else cs), _lib.ptr(kmh), _lib.ptr(kmv), _lib.stream())
+    return kmh, kmv, kmh, kmv
+
+
+def compute_smag2(rho, kmh, kmv, khh, khv, s11, s22, s33, s12, s13, s23, theta, qv, x3d, y3d, z3d, x3d4u, y3d4v,
+                  z3d4w, extra_scalars=()):
+    """(sgs_u, sgs_v, sgs_w, sgs_theta, sgs_qv[, ...]) with horizontal and vertical
+    coefficients (turbulence.py:114-295); sgs_w is zero at the surface."""
+    g = grid_from(rho, x3d4u, y3d4v, z3d4w)
+    su = empty(g.nx + 1, g.ny, g.nz, like=rho)
+    sv = empty(g.nx, g.ny + 1, g.nz, like=rho)
+    sw = empty(g.nx, g.ny, g.nz + 1, like=rho)
+    _lib.call("lex_smag2_momentum", g, _lib.ptr(column(rho)), _lib.ptr(kmh), _lib.ptr(kmv),
+              *[_lib.ptr(a) for a in (s11, s22, s33, s12, s13, s23, su, sv, sw)], _lib.stream())
+    outs = []
+    for sc in (theta, qv) + tuple(extra_scalars):
+        out = empty(g.nx, g.ny, g.nz, like=rho)
+        _lib.call("lex_smag2_scalar", g, _lib.ptr(column(rho)), _lib.ptr(khh), _lib.ptr(khv), _lib.ptr(sc),
+                  _lib.ptr(out), _lib.stream())
+        outs.append(out)
+    return (su, sv, sw) + tuple(outs)

@@ -717,10 +717,13 @@ def compute_scalar_gradient(s, grid):
     return dx, dy, dz
 
 
-def _scalar_sgs(rho, kh, s, grid):
-    """SGS diffusion of one scalar (turbulence.py:80-109)."""
+def _scalar_sgs(rho, kh, s, grid, khv=None):
+    """SGS diffusion of one scalar (turbulence.py:80-109); with khv the vertical faces take
+    khv (compute_smag2, turbulence.py:207-234)."""
     r8u, r8v, r8w = stagger_scalar(rho)
     k8u, k8v, k8w = stagger_scalar(kh)
+    if khv is not None:
+        k8w = stagger_scalar(khv)[2]
     dx, dy, dz = compute_scalar_gradient(s, grid)
     tx = -r8u * k8u * dx
     ty = -r8v * k8v * dy
@@ -733,6 +736,17 @@ def _scalar_sgs(rho, kh, s, grid):
 
 def compute_smag(rho, km, kh, s11, s22, s33, s12, s13, s23, scalars, grid):
     """SGS tendencies for u, v, w and each scalar in `scalars` (turbulence.py:9-111)."""
+    return _smag_tendencies(rho, km, km, kh, kh, s11, s22, s33, s12, s13, s23, scalars, grid, True)
+
+
+def compute_smag2(rho, kmh, kmv, khh, khv, s11, s22, s33, s12, s13, s23, scalars, grid):
+    """compute_smag with horizontal (kmh, khh) and vertical (kmv, khv) coefficients
+    (turbulence.py:114-295); unlike compute_smag it leaves sgs_w at the surface at its
+    stencil value (turbulence.py:208 zeroes the lid only)."""
+    return _smag_tendencies(rho, kmh, kmv, khh, khv, s11, s22, s33, s12, s13, s23, scalars, grid, False)
+
+
+def _smag_tendencies(rho, km, kmv, kh, khv, s11, s22, s33, s12, s13, s23, scalars, grid, zero_surface_w):
     I = slice(NGX, -NGX)
     J = slice(NGY, -NGY)
     K = slice(NGZ, -NGZ)
@@ -740,8 +754,8 @@ def compute_smag(rho, km, kh, s11, s22, s33, s12, s13, s23, scalars, grid):
     t22 = -2.0 * km * s22
     t33 = -2.0 * km * s33
     t12 = -2.0 * km * s12
-    t13 = -2.0 * km * s13
-    t23 = -2.0 * km * s23
+    t13 = -2.0 * kmv * s13
+    t23 = -2.0 * kmv * s23
     d11 = (t11[NGX:-(NGX - 1), J, K] - t11[NGX - 1:-NGX, J, K]) / grid.dx
     d12y = (t12[NGX - 1:-(NGX - 1), NGY:-(NGY - 1), K] - t12[NGX - 1:-(NGX - 1), NGY - 1:-NGY, K]) / grid.dy
     d12 = 0.25 * (d12y[:-1, :-1] + d12y[1:, 1:] + d12y[:-1, 1:] + d12y[1:, :-1])
@@ -763,16 +777,37 @@ def compute_smag(rho, km, kh, s11, s22, s33, s12, s13, s23, scalars, grid):
     d23y = (t23[I, NGY + 1:-(NGY - 1), K] - t23[I, NGY - 1:-(NGY + 1), K]) / (2.0 * grid.dy)
     r8w = interpolate_scalar2w(rho[I, J, K])
     sgs_w = -(interpolate_scalar2w_0(d13x) + interpolate_scalar2w_0(d23y) + d33) / r8w
-    sgs_w[:, :, 0] = 0.0
+    if zero_surface_w:
+        sgs_w[:, :, 0] = 0.0
     sgs_w[:, :, -1] = 0.0
-    sgs_s = tuple(_scalar_sgs(rho, kh, s, grid) for s in scalars)
+    sgs_s = tuple(_scalar_sgs(rho, kh, s, grid, None if khv is kh else khv) for s in scalars)
     return (sgs_u, sgs_v, sgs_w) + sgs_s
 
 
-def smagorinsky(rho0, u, v, w, theta, qv, scalars, grid, cs=0.18, prandtl=1.0 / 3.0):
-    """compute_deformation -> compute_nm -> compute_k -> compute_smag (one_step_integration.py:58-74)."""
+def compute_k2(rho, s11, s22, s33, s12, s13, s23, n2, grid, cs=0.18):
+    """(kmh, kmv, khh, khv) of the horizontal / vertical closure (turbulence.py:449-471),
+    prandtl 1: kmh = cs^2 dx dy Dh (no square root), kmv = cs^2 Lv^2 sqrt(max(Dv - n2, 0))
+    with 1/Lv^2 = (0.4 z)^-2 + dz^-2 at the padded cell-centre heights z."""
+    prandtl = 1.0
+    ft = s11.dtype.type
+    z3d = np.linspace(-0.5 * grid.dz, (grid.nz + 0.5) * grid.dz, grid.nz + 2).astype(s11.dtype)
+    gh2 = ft(grid.dx) * ft(grid.dy)
+    gv2 = 1.0 / ((0.4 * z3d) ** (-2) + ft(grid.dz) ** (-2))
+    deform_h = (2.0 * (s11 ** 2 + s22 ** 2 + s33 ** 2) + 4.0 * s12 ** 2) / rho ** 2
+    deform_v = 4.0 * (s13 ** 2 + s23 ** 2) / rho ** 2
+    kmh = ft(cs) ** 2 * gh2 * deform_h
+    kmv = ft(cs) ** 2 * gv2 * np.sqrt(np.fmax(deform_v - n2 / prandtl, 0.0))
+    return kmh, kmv, kmh / prandtl, kmv / prandtl
+
+
+def smagorinsky(rho0, u, v, w, theta, qv, scalars, grid, cs=0.18, prandtl=1.0 / 3.0, turb_opt=1):
+    """compute_deformation -> compute_nm -> compute_k -> compute_smag, or with turb_opt 2
+    compute_k2 -> compute_smag2 (one_step_integration.py:58-74)."""
     s11, s22, s33, s12, s13, s23, deform = compute_deformation(rho0, u, v, w, grid)
     n2 = compute_nm(theta, qv, grid)
+    if turb_opt == 2:
+        kmh, kmv, khh, khv = compute_k2(rho0, s11, s22, s33, s12, s13, s23, n2, grid, cs)
+        return compute_smag2(rho0, kmh, kmv, khh, khv, s11, s22, s33, s12, s13, s23, scalars, grid)
     km, kh = compute_k(deform, n2, grid, cs, prandtl)
     return compute_smag(rho0, km, kh, s11, s22, s33, s12, s13, s23, scalars, grid)
 
@@ -1053,8 +1088,8 @@ def rk_sub_step(stage, xi0, xi, base, grid, opts, dt_stage, frozen=None, heating
     nx, ny, nz = grid.nx, grid.ny, grid.nz
     zeros2 = np.zeros((nx, ny), dtype=rho0.dtype)
     if stage == 0 and frozen is None:
-        if opts.turb_opt == 1:
-            frozen = smagorinsky(rho0, u, v, w, th, qv, (th, qv) + pas, grid, opts.cs, opts.prandtl)
+        if opts.turb_opt in (1, 2):
+            frozen = smagorinsky(rho0, u, v, w, th, qv, (th, qv) + pas, grid, opts.cs, opts.prandtl, opts.turb_opt)
     sgs = frozen
     buoy, b8w = calculate_buoyancy(theta0, th, qv0, qv)
     fdiv = get_divergence(rho0, u, v, w, grid)
