@@ -79,10 +79,17 @@ struct ScalarSet {
   int n;
 };
 
-// scalar q's flux through the surface at column p (zero without surface fluxes)
-__device__ __forceinline__ float bottom_flux(const ScalarSet& S, int q, long p) {
-  const float* f = (q == 0) ? S.sfc[0] : (q == 1 ? S.sfc[1] : nullptr);
-  return f ? f[p] : 0.0f;
+// the surface fluxes of scalars 0 and 1 at column p, read by the surface lanes only
+// (zero elsewhere and without surface fluxes), once per column before the scalar loop
+struct BottomFlux {
+  float f0, f1;
+  __device__ __forceinline__ float of(int q) const { return q == 0 ? f0 : (q == 1 ? f1 : 0.0f); }
+};
+__device__ __forceinline__ BottomFlux bottom_fluxes(const ScalarSet& S, bool surface, long p) {
+  BottomFlux b{0.0f, 0.0f};
+  if (surface && S.sfc[0]) b.f0 = S.sfc[0][p];
+  if (surface && S.sfc[1]) b.f1 = S.sfc[1][p];
+  return b;
 }
 
 template <int NS>
@@ -161,6 +168,7 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
       sgv[q] = S.sgs[q] ? S.sgs[q][oint] : 0.0f;
     }
     const float hv = S.heat ? S.heat[oint] : 0.0f;
+    const BottomFlux bf = bottom_fluxes(S, act && k == 0, (long)i * d.ny + j);
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
       const float* s = S.s[q];
@@ -176,7 +184,7 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
       if (q == 0) thc = qc;
       if (q == 1) qvc = qc;
       if (act) {
-        float fz = (k == 0) ? bottom_flux(S, q, (long)i * d.ny + j) : mz_lo * vrecon(Pc, Lz, K - 1, pz, weps);
+        float fz = (k == 0) ? bf.of(q) : mz_lo * vrecon(Pc, Lz, K - 1, pz, weps);
         FZ[ty * nzl + k] = fz;
         float fy = my_lo * hrecon(Pc[Kc - 3 * Lz], Pc[Kc - 2 * Lz], Pc[Kc - Lz], Pc[Kc], Pc[Kc + Lz], Pc[Kc + 2 * Lz],
                                   py0, weps);
@@ -628,6 +636,7 @@ __global__ void __launch_bounds__(512) k_stage_scalars_db(Dims d, const float* _
     const float mx = r * uhi, my_lo = r * vlo, my_hi = r * vhi, mz_lo = rz * wlo;
     const bool px = upwind_pos(mx), py0 = upwind_pos(my_lo), py1 = upwind_pos(my_hi), pz = upwind_pos(mz_lo);
     float qcs[NS], fxhi[NS];
+    const BottomFlux bf = bottom_fluxes(S, act && k == 0, (long)i * d.ny + j);
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
       const float weps = S.weps[q];
@@ -635,7 +644,7 @@ __global__ void __launch_bounds__(512) k_stage_scalars_db(Dims d, const float* _
       qcs[q] = Pc[Kc];
       if (act) {
         FZb[(q * kFTY + ty) * nzl + k] =
-            (k == 0) ? bottom_flux(S, q, (long)i * d.ny + j) : mz_lo * vrecon(Pc, Lz, K - 1, pz, weps);
+            (k == 0) ? bf.of(q) : mz_lo * vrecon(Pc, Lz, K - 1, pz, weps);
         FYb[(q * (kFTY + 1) + ty) * nzl + k] =
             my_lo * hrecon(Pc[Kc - 3 * Lz], Pc[Kc - 2 * Lz], Pc[Kc - Lz], Pc[Kc], Pc[Kc + Lz], Pc[Kc + 2 * Lz], py0,
                            weps);
