@@ -193,7 +193,44 @@ def dl_sprint_timing(sprint_n=10):
             "ms_per_step": t_dl, "ms_per_step_without_correction": t_plain, "correction_ms": t_dl - t_plain,
             "cell_steps_per_s": ncell / (t_dl / 1e3), "state_finite": t_net["simpleCNN"][1],
             "zcnn_ms_per_step": t_net["Zcnn"][0], "zcnn_correction_ms": t_net["Zcnn"][0] - t_plain,
-            "zcnn_state_finite": t_net["Zcnn"][1]}
+            "zcnn_state_finite": t_net["Zcnn"][1], "loss_terms": dl_loss_timing(case, grids, sprint_n)}
+
+
+def dl_loss_timing(case, grids, sprint_n):
+    """The loss terms of the correction's training (lex_dl_train.py:62-97) after a
+    sprint_n-step sprint of one sequence on the 256x256x128 grid: the two Laplacians of
+    the theta and qv stacks and the eight weighted FP64 sums (stacks synthetic)."""
+    import torch
+    from lex_amd import _lib, dl_train as dt
+    nx, ny, nz = case.nx, case.ny, case.nz
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    shp = (1, sprint_n, nx, ny, nz)
+    th = 300.0 + torch.randn(shp, device="cuda", generator=gen)
+    qv = 1e-2 + 1e-3 * torch.randn(shp, device="cuda", generator=gen)
+    wt = torch.rand((nx, ny, nz), device="cuda", generator=gen)
+    x3d, y3d, z3d, x3d4u, y3d4v, z3d4w = grids[:6]
+    ws = torch.empty(_lib.load().lex_weighted_sum_workspace_bytes(), dtype=torch.uint8, device="cuda")
+    out = torch.empty(8, dtype=torch.float64, device="cuda")
+
+    def once():
+        lt = dt.laplace_of_tensor(x3d, x3d4u, y3d, y3d4v, z3d, z3d4w, th)
+        lq = dt.laplace_of_tensor(x3d, x3d4u, y3d, y3d4v, z3d, z3d4w, qv)
+        for q, (a, b) in enumerate(((th, th), (th, lt), (qv, qv), (qv, lq), (th, lt), (qv, lq), (lt, lq), (qv, qv))):
+            _lib.call("lex_weighted_sum", a.numel(), 1 if q == 7 else 0, _lib.ptr(a), _lib.ptr(b), _lib.ptr(wt),
+                      wt.numel(), _lib.ptr(out[q:q + 1]), _lib.ptr(ws), ws.numel(), _lib.stream())
+
+    once()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(5):
+        once()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 5
+    nbytes = (2 * 2 + 7 * 2 + 1) * th.numel() * 4   # Laplacians: read + write; sums: 2 reads (penalty 1)
+    return {"ms_per_loss": ms, "fields": f"2 Laplacians + 8 sums over {sprint_n} x {nx}x{ny}x{nz}",
+            "GBs": nbytes / (ms / 1e3) / 1e9}
 
 
 def physics_timing(steps=10):

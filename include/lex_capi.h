@@ -401,6 +401,19 @@ int lex_dl_apply(const lex_grid_t* g, const float* c, const float* r_theta, cons
                  const float* r_w, const float* r_qv, float* theta, float* u, float* v, float* w, float* qv,
                  void* stream);
 
+/* ---- the loss of the post-step correction's training (lex_dl_train.py:49-99) ---------
+ * laplace_of_tensor (dataset_generation.py:271-320) of nfields stacked interior scalar
+ * fields (nx, ny, nz): periodic x, y; end levels copied in z; no vertical term in the first
+ * and last x planes (the reference's s_z[:, :, (0, -1)] = 0 on 5-D stacks indexes x). */
+int lex_laplace(const lex_grid_t* g, long nfields, const float* s, float* out, void* stream);
+/* Deterministic FP64 sum over n elements (*out, device) of op 0: w * 0.5 (a - b)^2
+ * (optax.l2_loss times a weight) or op 1: w * max(-a, 0) (the negative-qv penalty), with
+ * w repeating every `period` elements (b unused for op 1).  Workspace: device memory of
+ * lex_weighted_sum_workspace_bytes(). */
+size_t lex_weighted_sum_workspace_bytes(void);
+int lex_weighted_sum(long n, int op, const float* a, const float* b, const float* w, long period, double* out,
+                     void* workspace, size_t workspace_bytes, void* stream);
+
 /* ---- surface fluxes (sfc_opt) ----------------------------------------------------------
  * atm_ocn_flux (boundary_conditions.py:7-75) on n columns of device FP32 inputs (the
  * reference's arguments: z_bottom, u_bottom, v_bottom, theta_bottom, q_bottom,

@@ -88,6 +88,9 @@ _SIGS = {
     "lex_dl_pack": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "lex_dl_stagger": [_G, _P, _P, _P, _P, _P, _P, _P],
     "lex_dl_apply": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lex_laplace": [_G, C.c_long, _P, _P, _P],
+    "lex_weighted_sum_workspace_bytes": [],
+    "lex_weighted_sum": [C.c_long, _I, _P, _P, _P, C.c_long, _P, _P, C.c_size_t, _P],
     "lex_atm_ocn_flux": [C.c_long, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "lex_surface_fluxes": [_G, _P, _P, _P, _P, _P, _P, _P, _P],
     "lex_scalar_update": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _F, _P, _P, _P],
@@ -153,7 +156,7 @@ _SIGS = {
 }
 _RESTYPE = {"lex_last_error": C.c_char_p, "lex_bicgstab_workspace_bytes": C.c_size_t,
             "lex_rk3_workspace_bytes": C.c_size_t, "lex_profile_kernel_name": C.c_char_p,
-            "lex_smag_vjp_workspace_bytes": C.c_size_t, "lex_rk3_vjp_workspace_bytes": C.c_size_t,
+            "lex_smag_vjp_workspace_bytes": C.c_size_t, "lex_weighted_sum_workspace_bytes": C.c_size_t, "lex_rk3_vjp_workspace_bytes": C.c_size_t,
             "lex_conv3d_packed_weight_bytes": C.c_size_t, "lex_rk3_tape_bytes": C.c_size_t,
             "lex_rk3_adjoint_bytes": C.c_size_t}
 

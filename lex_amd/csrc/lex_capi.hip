@@ -16,6 +16,7 @@
 #include "lex_conv.hip"
 #include "lex_sgs_io.hip"
 #include "lex_dl_post.hip"
+#include "lex_dl_train.hip"
 #include "../../include/lex_capi.h"
 
 using namespace lex;
@@ -1049,6 +1050,33 @@ int lex_dl_apply(const lex_grid_t* g, const float* c, const float* r_th, const f
   rg.qv = r_qv;
   hipLaunchKernelGGL(k_dl_apply, grd(d.nz, d.ny, d.nx), blk(), 0, S(stream), d, c, rg, th, u, v, w, qv);
   return check_launch("lex_dl_apply");
+}
+
+int lex_laplace(const lex_grid_t* g, long nfields, const float* s, float* out, void* stream) {
+  REQGRID(g);
+  REQ(s); REQ(out);
+  if (nfields < 0) return fail(-1, "lex_laplace: nfields must be >= 0%s");
+  const long n = nfields * (long)g->nx * g->ny * g->nz;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_laplace, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), nfields, g->nx, g->ny,
+                     g->nz, g->dx, g->dy, g->dz, s, out);
+  return check_launch("lex_laplace");
+}
+
+size_t lex_weighted_sum_workspace_bytes(void) { return (size_t)kWsumBlocks * sizeof(double); }
+
+int lex_weighted_sum(long n, int op, const float* a, const float* b, const float* w, long period, double* out,
+                     void* workspace, size_t workspace_bytes, void* stream) {
+  REQ(a); REQ(w); REQ(out); REQ(workspace);
+  if (op != 0 && op != 1) return fail(-1, "lex_weighted_sum: op must be 0 or 1%s");
+  if (op == 0 && !b) return fail(-2, "lex_weighted_sum: op 0 needs b%s");
+  if (n < 0 || period < 1) return fail(-1, "lex_weighted_sum: n >= 0 and period >= 1 required%s");
+  if (workspace_bytes < (size_t)kWsumBlocks * sizeof(double))
+    return fail(-3, "lex_weighted_sum: workspace too small%s");
+  double* part = (double*)workspace;
+  hipLaunchKernelGGL(k_wsum_partial, dim3(kWsumBlocks), dim3(kWsumThreads), 0, S(stream), n, op, a, b, w, period, part);
+  hipLaunchKernelGGL(k_wsum_final, dim3(1), dim3(64), 0, S(stream), (const double*)part, out);
+  return check_launch("lex_weighted_sum");
 }
 
 int lex_atm_ocn_flux(long n, const float* z, const float* ub, const float* vb, const float* th, const float* q,

@@ -1452,6 +1452,71 @@ def ws_rk3_sprint_dl(xi, base, grid, opts, net_fn, scaling_params, sprint_n):
     return xi
 
 
+def ws_rk3_sprint_dl_train(xi, base, grid, opts, net_fn, scaling_params, sprint_n):
+    """ws_rk3_sprint_dl_train (one_sprint_functions.py:34-97): ws_rk3_sprint_dl keeping
+    each step's interior theta, u, v, w, pi', qv (after the correction) in stacks of
+    sprint_n."""
+    stacks = [[] for _ in range(6)]
+    for _ in range(sprint_n):
+        xi = ws_rk3_sprint_dl(xi, base, grid, opts, net_fn, scaling_params, 1)
+        for q in range(6):
+            stacks[q].append(xi[q][NGX:-NGX, NGY:-NGY, NGZ:-NGZ])
+    return tuple(np.stack(s) for s in stacks), xi
+
+
+def laplace_of_tensor(s, grid):
+    """laplace_of_tensor (dataset_generation.py:271-320) of interior scalar fields with any
+    leading axes (..., nx, ny, nz): periodic in x and y, the end levels copied in z.  The
+    reference applies it to 5-D stacks, where its s_z[:, :, (0, -1)] = 0 indexes the x
+    axis: the z derivative of the first and last x planes is zeroed (no vertical term
+    there), kept here on the x axis whatever the leading axes."""
+    sp = np.concatenate([s[..., -1:, :, :], s, s[..., :1, :, :]], axis=-3)
+    sp = np.concatenate([sp[..., :, -1:, :], sp, sp[..., :, :1, :]], axis=-2)
+    sp = np.concatenate([sp[..., :1], sp, sp[..., -1:]], axis=-1)
+    sx = (sp[..., 1:, 1:-1, 1:-1] - sp[..., :-1, 1:-1, 1:-1]) / grid.dx
+    sy = (sp[..., 1:-1, 1:, 1:-1] - sp[..., 1:-1, :-1, 1:-1]) / grid.dy
+    sz = (sp[..., 1:-1, 1:-1, 1:] - sp[..., 1:-1, 1:-1, :-1]) / grid.dz
+    sz[..., 0, :, :] = 0.0
+    sz[..., -1, :, :] = 0.0
+    return ((sx[..., 1:, :, :] - sx[..., :-1, :, :]) / grid.dx + (sy[..., :, 1:, :] - sy[..., :, :-1, :]) / grid.dy
+            + (sz[..., 1:] - sz[..., :-1]) / grid.dz)
+
+
+def dl_loss_weights(scaling_params):
+    """The per-point weights of the training loss (lex_dl_train.py:69-96): for each term
+    (theta, u, v, w, qv, the Laplacians of theta and qv) the scaling range divided by the
+    cube of the level range plus its epsilon (0.01, 1e-5 for qv, 1e-7 and 1e-10 for the
+    Laplacians), the level range being the max over x, y of the maximum field minus the
+    min over x, y of the minimum field; and the negative-qv weight with the square."""
+    eps = (0.01, 0.01, 0.01, 0.01, 1.0e-5, 1.0e-7, 1.0e-10)
+    w = []
+    for q in range(7):
+        mn = np.asarray(scaling_params[2 * q], dtype=np.float64)
+        mx = np.asarray(scaling_params[2 * q + 1], dtype=np.float64)
+        lev = (np.max(mx, axis=(-3, -2), keepdims=True) - np.min(mn, axis=(-3, -2), keepdims=True) + eps[q])
+        w.append((mx - mn) / lev ** 3)
+    mn, mx = np.asarray(scaling_params[8], np.float64), np.asarray(scaling_params[9], np.float64)
+    lev = np.max(mx, axis=(-3, -2), keepdims=True) - np.min(mn, axis=(-3, -2), keepdims=True) + 1.0e-5
+    w.append((mx - mn) / lev ** 2)
+    return w
+
+
+def dl_loss(stacks, truth, scaling_params, grid):
+    """_loss_and_aux (lex_dl_train.py:49-99) given the sprint's stacks (theta, u, v, w, pip,
+    qv; leading axes batch, step) and the truth (theta, u, v, w, pip, qv, del_th, del_qv):
+    weighted l2 terms of theta, u, v, w, qv and of the Laplacians of theta - theta_min and
+    qv - qv_min, plus the weighted negative-qv penalty.  FP64."""
+    th, u, v, w, _, qv = (np.asarray(a, np.float64) for a in stacks)
+    w8 = dl_loss_weights(scaling_params)
+    theta_min, qv_min = (np.asarray(scaling_params[i], np.float64) for i in (0, 8))
+    del_th = laplace_of_tensor(th - theta_min, grid)
+    del_qv = laplace_of_tensor(qv - qv_min, grid)
+    preds = (th, u, v, w, qv, del_th, del_qv)
+    trues = tuple(np.asarray(truth[i], np.float64) for i in (0, 1, 2, 3, 5, 6, 7))
+    loss = sum(float(np.sum(0.5 * (p - t) ** 2 * wq)) for p, t, wq in zip(preds, trues, w8[:7]))
+    return loss + float(np.sum(np.where(qv < 0.0, -qv, 0.0) * w8[7]))
+
+
 def n_acoustic_total(opts):
     """Acoustic substeps per full step: 1 + round(n/2) + n (solver_opt 2)."""
     return 1 + _n_acoustic(1, opts.dt / 2.0, opts)[0] + _n_acoustic(2, opts.dt, opts)[0]

@@ -1,0 +1,156 @@
+"""The loss side of the post-step correction's training on the device vs the oracle:
+laplace_of_tensor (lex_laplace), the deterministic weighted sums (lex_weighted_sum), the
+training sprint's stacks and the whole loss_and_aux with simpleCNN, and the dataset split
+with its device padding."""
+import numpy as np
+import pytest
+
+from oracle import lex_oracle as o
+from tests._helpers import check_fp32, from_dev, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _grids(gr):
+    from lex_amd.setup_lex import make_grids
+    return make_grids(gr.nx, gr.ny, gr.nz, gr.dx, gr.dy, gr.dz)
+
+
+def test_laplace_of_tensor_vs_oracle():
+    from lex_amd import dl_train as dt
+    gr = o.Grid(20, 12, 16, 100.0, 80.0, 50.0)
+    rng = np.random.default_rng(4)
+    s = (300.0 + rng.normal(0, 1, (2, 3, gr.nx, gr.ny, gr.nz))).astype(np.float32)
+    x3d, y3d, z3d, x3d4u, y3d4v, z3d4w = _grids(gr)[:6]
+    got = from_dev(dt.laplace_of_tensor(x3d, x3d4u, y3d, y3d4v, z3d, z3d4w, to_dev(s - np.float32(299.0))))
+    r64 = o.laplace_of_tensor(s.astype(np.float64) - 299.0, gr)
+    r32 = o.laplace_of_tensor(s - np.float32(299.0), gr)
+    check_fp32(got, r64, r32, "laplace", factor=8.0)
+    assert np.max(np.abs(got[..., [0, -1], :, :] - r64[..., [0, -1], :, :])) < 1e-4   # the x-plane rule
+
+
+@pytest.mark.parametrize("op", [0, 1])
+def test_weighted_sum_vs_numpy(op):
+    import torch
+    from lex_amd import _lib
+    rng = np.random.default_rng(9 + op)
+    n, period = 3 * 1_000_003, 1_000_003
+    a = rng.normal(0, 1, n).astype(np.float32)
+    b = rng.normal(0, 1, n).astype(np.float32)
+    w = rng.uniform(0, 2, period).astype(np.float32)
+    ws = torch.empty(_lib.load().lex_weighted_sum_workspace_bytes(), dtype=torch.uint8, device="cuda")
+    out = torch.empty(1, dtype=torch.float64, device="cuda")
+    da, db, dw = to_dev(a), to_dev(b), to_dev(w)
+    _lib.call("lex_weighted_sum", n, op, _lib.ptr(da), _lib.ptr(db), _lib.ptr(dw), period, _lib.ptr(out),
+              _lib.ptr(ws), ws.numel(), _lib.stream())
+    a64, b64, w64 = a.astype(np.float64), b.astype(np.float64), np.tile(w.astype(np.float64), 3)
+    exp = np.sum(w64 * 0.5 * (a64 - b64) ** 2) if op == 0 else np.sum(w64 * np.where(a64 < 0, -a64, 0.0))
+    got = float(out.item())
+    assert abs(got - exp) <= 1e-10 * abs(exp)
+    # deterministic: the same bits again
+    _lib.call("lex_weighted_sum", n, op, _lib.ptr(da), _lib.ptr(db), _lib.ptr(dw), period, _lib.ptr(out),
+              _lib.ptr(ws), ws.numel(), _lib.stream())
+    assert float(out.item()) == got
+
+
+def _scaling(rng, nx, ny, nz):
+    shapes = [(nx, ny, nz), (nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1), (nx, ny, nz), (nx, ny, nz),
+              (nx, ny, nz)]
+    scales = [1.0, 1.0, 1.0, 0.5, 1e-3, 1e-4, 1e-8]
+    offs = [300.0, 0.0, 0.0, 0.0, 5e-3, 0.0, 0.0]
+    out = []
+    for shp, sc, off in zip(shapes, scales, offs):
+        b = rng.normal(0, sc, shp) + off
+        out += [(b - rng.uniform(0.1, 0.5, shp) * sc)[None, None].astype(np.float32),
+                (b + rng.uniform(0.1, 0.5, shp) * sc)[None, None].astype(np.float32)]
+    return tuple(out)
+
+
+def test_loss_and_aux_vs_oracle():
+    import torch
+    from lex_amd import dl_models as dlm, dl_train as dt, namelist_n_constants as nl
+    from lex_amd.setup_lex import build_case, config1
+    case, opts = config1()
+    case = case.scaled(16, 12, 12)
+    phys, base, grids, _ = build_case(case)
+    nl.dt, nl.n_sound, nl.solver_opt, nl.turb_opt = opts["dt"], 7, 2, 1
+    model_opt = (2, False, False, 0, False, False, 1)
+    rng = np.random.default_rng(12)
+    spr = _scaling(rng, case.nx, case.ny, case.nz)
+    spd = tuple(to_dev(a) for a in spr)
+    net = dlm.simpleCNN(seed=9)
+    # a small correction, so that the steps dominate the stacks' increments (the BF16
+    # network sees inputs rounded from slightly different FP32 / FP64 states)
+    with torch.no_grad():
+        net.weights[-1].mul_(1e-5)
+        net.biases[-1].fill_(1e-6)
+    B, S = 2, 2
+    # two initial states: the bubble and the bubble with a small wind
+    ic = [torch.stack([a, a + (0.5 if q == 1 else 0.0)]) for q, a in enumerate(phys[:6])]
+    got_stacks = dt.ws_rk3_sprint_dl_train_vmap(*ic, base, grids, model_opt, net, spd, sprint_n=S)
+
+    def net_fn(x):
+        with torch.no_grad():
+            return from_dev(net(to_dev(x.astype(np.float32))))
+
+    og = o.Grid(case.nx, case.ny, case.nz, case.dx, case.dy, case.dz)
+    oo = o.Options(dt=opts["dt"], solver_opt=2, n_sound=7, turb_opt=1)
+    bs = tuple(from_dev(a).astype(np.float64) for a in base[:5]) + (None,)
+    sp64 = tuple(a.astype(np.float64) for a in spr)
+    bs32 = tuple(from_dev(a).astype(np.float32) for a in base[:5]) + (None,)
+
+    def oracle_stacks(dtype):
+        ref = []
+        for b in range(B):
+            x0 = tuple(from_dev(a[b]).astype(dtype) for a in ic)
+            st, _ = o.ws_rk3_sprint_dl_train(x0, bs if dtype == np.float64 else bs32, og, oo, net_fn,
+                                             tuple(a.astype(dtype) for a in spr), S)
+            ref.append(st)
+        return tuple(np.stack([r[q] for r in ref]) for q in range(6))
+
+    ref_stacks, ref32 = oracle_stacks(np.float64), oracle_stacks(np.float32)
+    # increments of the stacks over the initial state, vs the FP64 oracle, within 10x the
+    # FP32 oracle's own error (theta's increments over two steps are a few ulp of 300 K)
+    for q, name in enumerate(("theta", "u", "v", "w", "pip", "qv")):
+        i0 = np.stack([np.stack([from_dev(ic[q][b])[3:-3, 3:-3, 1:-1]] * S) for b in range(B)])
+        r = ref_stacks[q] - i0
+        nr = max(np.linalg.norm(r), 1e-30)
+        err = np.linalg.norm(from_dev(got_stacks[q]) - i0 - r) / nr
+        e32 = np.linalg.norm(ref32[q].astype(np.float64) - i0 - r) / nr
+        assert err <= max(10.0 * e32, 1e-3), f"{name}: rel-L2 of the increments {err:.2e} (FP32 oracle {e32:.2e})"
+    # the loss on the device's own stacks: truth = those stacks perturbed, and their
+    # Laplacians perturbed
+    own = tuple(from_dev(a) for a in got_stacks)
+    truth = [r + rng.normal(0, s, r.shape) for r, s in zip(own, (0.05, 0.05, 0.05, 0.05, 1e-4, 1e-4))]
+    truth += [o.laplace_of_tensor(own[0] - sp64[0][0, 0], og) + rng.normal(0, 1e-4, own[0].shape),
+              o.laplace_of_tensor(own[5] - sp64[8][0, 0], og) + rng.normal(0, 1e-9, own[5].shape)]
+    sp_o = tuple(a[0, 0] for a in sp64)
+    exp = o.dl_loss(own, truth, sp_o, og)
+    got = float(dt.loss_and_aux(net, tuple(ic), tuple(to_dev(t.astype(np.float32)) for t in truth), spd, base,
+                                grids, model_opt, sprint_n=S).item())
+    assert abs(got - exp) <= 1e-3 * exp, (got, exp)
+    ev = dt.eval_step(net, tuple(ic), tuple(to_dev(t.astype(np.float32)) for t in truth), spd, base, grids,
+                      model_opt, sprint_n=S)
+    assert float(ev["total_mse"].item()) == got
+
+
+def test_generate_training_data_split_and_padding():
+    from lex_amd import dl_train as dt
+    nx, ny, nz, n_seq, n_step = 8, 6, 5, 8, 3
+    rng = np.random.default_rng(1)
+    shapes = [(nx, ny, nz), (nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1), (nx, ny, nz), (nx, ny, nz),
+              (nx, ny, nz), (nx, ny, nz)]
+    data = tuple(rng.normal(size=(n_seq, n_step) + s).astype(np.float32) for s in shapes)
+    tr_ic, tr_true, te_ic, te_true, n_tr, n_te = dt.generate_training_data(data, 5)
+    heads, n_train = dt.split_indices(n_seq, 5)
+    assert (n_tr, n_te) == (n_train, n_seq - n_train) == (6, 2)
+    pads = (o.padding3_array, o.padding3u_array, o.padding3v_array, o.padding3_array, o.padding3_array,
+            o.padding3_array)
+    for q in range(6):
+        for b in range(n_tr):
+            exp = pads[q](data[q][heads[b], 0].astype(np.float64))
+            assert np.array_equal(from_dev(tr_ic[q][b]), exp.astype(np.float32).astype(np.float64)), (q, b)
+        assert np.array_equal(from_dev(te_ic[q][0]), pads[q](data[q][heads[n_train], 0].astype(np.float64)))
+    for q in range(8):
+        assert np.array_equal(from_dev(tr_true[q]), data[q][heads[:n_train], 1:].astype(np.float64))
+        assert np.array_equal(from_dev(te_true[q]), data[q][heads[n_train:], 1:].astype(np.float64))
