@@ -1056,10 +1056,13 @@ int lex_laplace(const lex_grid_t* g, long nfields, const float* s, float* out, v
   REQGRID(g);
   REQ(s); REQ(out);
   if (nfields < 0) return fail(-1, "lex_laplace: nfields must be >= 0%s");
-  const long n = nfields * (long)g->nx * g->ny * g->nz;
-  if (n == 0) return 0;
-  hipLaunchKernelGGL(k_laplace, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), nfields, g->nx, g->ny,
-                     g->nz, g->dx, g->dy, g->dz, s, out);
+  const long per = (long)g->nx * g->ny * g->nz;
+  const long chunk = 65535 / g->nx;   // fields per launch (grid z = fields x nx)
+  for (long f0 = 0; f0 < nfields; f0 += chunk) {
+    const long nf = (nfields - f0 < chunk) ? nfields - f0 : chunk;
+    hipLaunchKernelGGL(k_laplace, dim3((g->nz + 63) / 64, g->ny, (unsigned)(nf * g->nx)), dim3(64), 0, S(stream),
+                       g->nx, g->ny, g->nz, g->dx, g->dy, g->dz, s + f0 * per, out + f0 * per);
+  }
   return check_launch("lex_laplace");
 }
 

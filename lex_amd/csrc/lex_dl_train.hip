@@ -18,19 +18,15 @@
 
 namespace lex {
 
-// one thread per (field, i, j, k); fields of nx * ny * nz, z fastest
-__global__ void k_laplace(long nfields, int nx, int ny, int nz, float dx, float dy, float dz,
-                          const float* __restrict__ s, float* __restrict__ out) {
-  const long n = nfields * (long)nx * ny * nz;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int k = (int)(t % nz);
-  const long r = t / nz;
-  const int j = (int)(r % ny);
-  const long r2 = r / ny;
-  const int i = (int)(r2 % nx);
-  const long f = r2 / nx;
-  const float* a = s + f * (long)nx * ny * nz;
+// lanes along z, one block row per (j, x-plane of a field): blockIdx.y = j,
+// blockIdx.z = f * nx + i (the caller launches field chunks with nx * chunk <= 65535)
+__global__ void k_laplace(int nx, int ny, int nz, float dx, float dy, float dz, const float* __restrict__ s,
+                          float* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nz) return;
+  const int j = blockIdx.y;
+  const int f = blockIdx.z / nx, i = blockIdx.z - f * nx;
+  const float* a = s + (long)f * nx * ny * nz;
   auto at = [&](int ii, int jj, int kk) { return a[((long)ii * ny + jj) * nz + kk]; };
   const float c = at(i, j, k);
   const int im = (i == 0) ? nx - 1 : i - 1, ip = (i == nx - 1) ? 0 : i + 1;
@@ -41,7 +37,8 @@ __global__ void k_laplace(long nfields, int nx, int ny, int nz, float dx, float 
   const bool zt = (i != 0) && (i != nx - 1);
   const float sz_lo = (zt && k > 0) ? (c - at(i, j, k - 1)) / dz : 0.0f;
   const float sz_hi = (zt && k < nz - 1) ? (at(i, j, k + 1) - c) / dz : 0.0f;
-  out[t] = (sx_hi - sx_lo) / dx + (sy_hi - sy_lo) / dy + (sz_hi - sz_lo) / dz;
+  out[(long)f * nx * ny * nz + ((long)i * ny + j) * nz + k] =
+      (sx_hi - sx_lo) / dx + (sy_hi - sy_lo) / dy + (sz_hi - sz_lo) / dz;
 }
 
 constexpr int kWsumBlocks = 1024;
@@ -62,8 +59,11 @@ __global__ void __launch_bounds__(kWsumThreads) k_wsum_partial(long n, int op, c
   __shared__ double red[kWsumThreads];
   double acc = 0.0;
   const long stride = (long)kWsumBlocks * kWsumThreads;
-  for (long t = (long)blockIdx.x * kWsumThreads + threadIdx.x; t < n; t += stride)
-    acc += wsum_term(op, a[t], b ? b[t] : 0.0f, w[t % period]);
+  // thread p takes weight index p (grid-stride over the period) in every repetition
+  for (long p = (long)blockIdx.x * kWsumThreads + threadIdx.x; p < period; p += stride) {
+    const float wp = w[p];
+    for (long t = p; t < n; t += period) acc += wsum_term(op, a[t], b ? b[t] : 0.0f, wp);
+  }
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int m = kWsumThreads / 2; m > 0; m >>= 1) {
