@@ -410,6 +410,22 @@ int lex_laplace(const lex_grid_t* g, long nfields, const float* s, float* out, v
  * (optax.l2_loss times a weight) or op 1: w * max(-a, 0) (the negative-qv penalty), with
  * w repeating every `period` elements (b unused for op 1).  Workspace: device memory of
  * lex_weighted_sum_workspace_bytes(). */
+/* The gradient of those terms with respect to a, accumulated into out (n floats):
+ * op 0: w (a - b); op 1: -w where a < 0. */
+int lex_weighted_sum_grad(long n, int op, const float* a, const float* b, const float* w, long period, float* out,
+                          void* stream);
+/* Reverses for the training's gradient through the post-step correction
+ * (one_sprint_functions.py:54-83, differentiated by jax.value_and_grad in the reference,
+ * lex_dl_train.py:114): lex_dl_apply_vjp writes dc (nx, ny, nz, 5), the adjoint of the
+ * network's output, from the adjoint of the corrected state (padded, ghosts folded) and
+ * the ranges of lex_dl_apply; lex_dl_pack_vjp ADDS the adjoint of the state's interior
+ * values for the adjoint dp (nx, ny, nz, 5) of lex_dl_pack's output (lo unused: the map
+ * is affine). */
+int lex_dl_apply_vjp(const lex_grid_t* g, const float* l_theta, const float* l_u, const float* l_v, const float* l_w,
+                     const float* l_qv, const float* r_theta, const float* r_u, const float* r_v, const float* r_w,
+                     const float* r_qv, float* dc, void* stream);
+int lex_dl_pack_vjp(const lex_grid_t* g, const float* dp, const float* lo, const float* den, float* l_theta,
+                    float* l_u, float* l_v, float* l_w, float* l_qv, void* stream);
 size_t lex_weighted_sum_workspace_bytes(void);
 int lex_weighted_sum(long n, int op, const float* a, const float* b, const float* w, long period, double* out,
                      void* workspace, size_t workspace_bytes, void* stream);

@@ -90,6 +90,9 @@ _SIGS = {
     "lex_dl_apply": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "lex_laplace": [_G, C.c_long, _P, _P, _P],
     "lex_weighted_sum_workspace_bytes": [],
+    "lex_weighted_sum_grad": [C.c_long, _I, _P, _P, _P, C.c_long, _P, _P],
+    "lex_dl_apply_vjp": [_G] + [_P] * 12,
+    "lex_dl_pack_vjp": [_G] + [_P] * 9,
     "lex_weighted_sum": [C.c_long, _I, _P, _P, _P, C.c_long, _P, _P, C.c_size_t, _P],
     "lex_atm_ocn_flux": [C.c_long, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "lex_surface_fluxes": [_G, _P, _P, _P, _P, _P, _P, _P, _P],
@@ -201,10 +204,20 @@ def exported_symbols():
     return list(_SIGS)
 
 
+# tensors whose pointers were taken for the call being assembled: a temporary (say
+# `ptr(x.contiguous())`) stays alive until the call has been issued, so a later allocation
+# in the same argument list cannot take its memory first; after the call, reuse of the
+# memory is ordered on the stream behind the launched kernels
+_KEEP: list = []
+
+
 def call(name, *args):
     """Call an entry point; raise LexError with lex_last_error() on a non-zero status."""
     lib = load()
-    rc = getattr(lib, name)(*args)
+    try:
+        rc = getattr(lib, name)(*args)
+    finally:
+        _KEEP.clear()
     if rc != 0:
         msg = lib.lex_last_error().decode(errors="replace")
         raise LexError(f"{name} failed ({rc}): {msg}")
@@ -219,6 +232,7 @@ def ptr(t: torch.Tensor | None):
         raise LexError("liblexhip needs device tensors (got a CPU tensor)")
     if not t.is_contiguous():
         raise LexError("liblexhip needs contiguous tensors")
+    _KEEP.append(t)
     return t.data_ptr()
 
 

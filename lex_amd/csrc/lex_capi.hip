@@ -1066,6 +1066,45 @@ int lex_laplace(const lex_grid_t* g, long nfields, const float* s, float* out, v
   return check_launch("lex_laplace");
 }
 
+int lex_weighted_sum_grad(long n, int op, const float* a, const float* b, const float* w, long period, float* out,
+                          void* stream) {
+  REQ(a); REQ(w); REQ(out);
+  if (op != 0 && op != 1) return fail(-1, "lex_weighted_sum_grad: op must be 0 or 1%s");
+  if (op == 0 && !b) return fail(-2, "lex_weighted_sum_grad: op 0 needs b%s");
+  if (n < 0 || period < 1) return fail(-1, "lex_weighted_sum_grad: n >= 0 and period >= 1 required%s");
+  if (n == 0) return 0;
+  const long blocks = (period + 255) / 256 < 4096 ? (period + 255) / 256 : 4096;
+  hipLaunchKernelGGL(k_wgrad, dim3((unsigned)blocks), dim3(256), 0, S(stream), n, op, a, b, w, period, out);
+  return check_launch("lex_weighted_sum_grad");
+}
+
+int lex_dl_apply_vjp(const lex_grid_t* g, const float* l_th, const float* l_u, const float* l_v, const float* l_w,
+                     const float* l_qv, const float* r_th, const float* r_u, const float* r_v, const float* r_w,
+                     const float* r_qv, float* dc, void* stream) {
+  REQGRID(g);
+  REQ(l_th); REQ(l_u); REQ(l_v); REQ(l_w); REQ(l_qv); REQ(r_th); REQ(r_u); REQ(r_v); REQ(r_w); REQ(r_qv); REQ(dc);
+  Dims d = mkdims(g);
+  DlRanges rg;
+  rg.th = r_th;
+  rg.u = r_u;
+  rg.v = r_v;
+  rg.w = r_w;
+  rg.qv = r_qv;
+  hipLaunchKernelGGL(k_dl_apply_vjp, grd(d.nz, d.ny, d.nx), blk(), 0, S(stream), d, l_th, l_u, l_v, l_w, l_qv, rg, dc);
+  return check_launch("lex_dl_apply_vjp");
+}
+
+int lex_dl_pack_vjp(const lex_grid_t* g, const float* dp, const float* lo, const float* den, float* l_th, float* l_u,
+                    float* l_v, float* l_w, float* l_qv, void* stream) {
+  REQGRID(g);
+  REQ(dp); REQ(den); REQ(l_th); REQ(l_u); REQ(l_v); REQ(l_w); REQ(l_qv);
+  Dims d = mkdims(g);
+  DlNorm nrm{lo, den};
+  hipLaunchKernelGGL(k_dl_pack_vjp, dim3((d.nz + 1 + 63) / 64, (d.ny + 1 + 3) / 4, d.nx + 1), dim3(64, 4, 1), 0,
+                     S(stream), d, dp, nrm, l_th, l_u, l_v, l_w, l_qv);
+  return check_launch("lex_dl_pack_vjp");
+}
+
 size_t lex_weighted_sum_workspace_bytes(void) { return (size_t)kWsumBlocks * sizeof(double); }
 
 int lex_weighted_sum(long n, int op, const float* a, const float* b, const float* w, long period, double* out,

@@ -123,4 +123,80 @@ __global__ void k_dl_apply(Dims d, const float* __restrict__ c, DlRanges rg, flo
   if (k == nz - 1) store_w(d, w, i, j, nz, w[iw(d, I, J, nz + 1)] + dl_w_level(d, c, i, j, nz) * rg.w[ow + 1]);
 }
 
+// Reverse of k_dl_apply with respect to the correction (the training's gradient through
+// the post-step correction, one_sprint_functions.py:54-83): dc (nx, ny, nz, 5) = the
+// transposed stagger of the state adjoint (padded, ghosts folded) times the ranges.  The u
+// correction of cell i enters faces i and i+1 (those in 1..nx-1), face 0 when i is nx-2
+// or nx-1 and face nx when i is 0 or 1 (the end faces of stagger_uvw); v alike; w levels
+// k and k+1 (those in 1..nz-1).  One thread per cell, written once.
+__global__ void k_dl_apply_vjp(Dims d, const float* __restrict__ lth, const float* __restrict__ lu,
+                               const float* __restrict__ lv, const float* __restrict__ lw,
+                               const float* __restrict__ lq, DlRanges rg, float* __restrict__ dc) {
+  const TileId tb = xcd_tile();
+  const int k = tb.x * blockDim.x + threadIdx.x;
+  const int j = tb.y * blockDim.y + threadIdx.y;
+  const int i = tb.z;
+  if (k >= d.nz || j >= d.ny) return;
+  const int I = i + 3, J = j + 3, K = k + 1, nx = d.nx, ny = d.ny, nz = d.nz;
+  const long o = ((long)i * ny + j) * nz + k;
+  auto fu = [&](int f) { return lu[iu(d, f + 3, J, K)] * rg.u[((long)f * ny + j) * nz + k]; };
+  auto fv = [&](int f) { return lv[iv(d, I, f + 3, K)] * rg.v[((long)i * (ny + 1) + f) * nz + k]; };
+  auto fw = [&](int l) { return lw[iw(d, I, J, l + 1)] * rg.w[((long)i * ny + j) * (nz + 1) + l]; };
+  float su = 0.0f, sv = 0.0f, sw = 0.0f;
+  if (i >= 1) su += fu(i);
+  if (i + 1 <= nx - 1) su += fu(i + 1);
+  if (i >= nx - 2) su += fu(0);
+  if (i <= 1) su += fu(nx);
+  if (j >= 1) sv += fv(j);
+  if (j + 1 <= ny - 1) sv += fv(j + 1);
+  if (j >= ny - 2) sv += fv(0);
+  if (j <= 1) sv += fv(ny);
+  if (k >= 1) sw += fw(k);
+  if (k + 1 <= nz - 1) sw += fw(k + 1);
+  float* c = dc + o * 5;
+  c[0] = lth[ic(d, I, J, K)] * rg.th[o];
+  c[1] = 0.5f * su;
+  c[2] = 0.5f * sv;
+  c[3] = 0.5f * sw;
+  c[4] = lq[ic(d, I, J, K)] * rg.qv[o];
+}
+
+// Reverse of k_dl_pack: adds the adjoint of the state's interior values (theta, qv at
+// cells, u faces 0..nx, v faces 0..ny, w levels 0..nz) for the input adjoint dp
+// (nx, ny, nz, 5).  One thread per point of the (nx+1, ny+1, nz+1) box; every adjoint
+// element is written by one thread.
+__global__ void k_dl_pack_vjp(Dims d, const float* __restrict__ dp, DlNorm nrm, float* __restrict__ lth,
+                              float* __restrict__ lu, float* __restrict__ lv, float* __restrict__ lw,
+                              float* __restrict__ lq) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = blockIdx.z;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  if (k > nz || j > ny || i > nx) return;
+  auto g = [&](int ii, int jj, int kk, int ch) { return dp[(((long)ii * ny + jj) * nz + kk) * 5 + ch]; };
+  const float* den = nrm.den;
+  if (i < nx && j < ny && k < nz) {
+    lth[ic(d, i + 3, j + 3, k + 1)] += g(i, j, k, 0) / den[k];
+    lq[ic(d, i + 3, j + 3, k + 1)] += g(i, j, k, 4) / den[4 * nz + 1 + k];
+  }
+  if (j < ny && k < nz) {   // u face i
+    float a = 0.0f;
+    if (i >= 1) a += g(i - 1, j, k, 1);
+    if (i < nx) a += g(i, j, k, 1);
+    lu[iu(d, i + 3, j + 3, k + 1)] += 0.5f * a / den[nz + k];
+  }
+  if (i < nx && k < nz) {   // v face j
+    float a = 0.0f;
+    if (j >= 1) a += g(i, j - 1, k, 2);
+    if (j < ny) a += g(i, j, k, 2);
+    lv[iv(d, i + 3, j + 3, k + 1)] += 0.5f * a / den[2 * nz + k];
+  }
+  if (i < nx && j < ny) {   // w level k
+    float a = 0.0f;
+    if (k >= 1) a += g(i, j, k - 1, 3);
+    if (k < nz) a += g(i, j, k, 3);
+    lw[iw(d, i + 3, j + 3, k + 1)] += 0.5f * a / den[3 * nz + k];
+  }
+}
+
 }  // namespace lex

@@ -80,4 +80,18 @@ __global__ void __launch_bounds__(64) k_wsum_final(const double* __restrict__ pa
   if (threadIdx.x == 0) out[0] = s;
 }
 
+// The gradient of those terms with respect to a, accumulated: out += w (a - b) (op 0) or
+// out += -w where a < 0 (op 1), w repeating every `period` elements.
+__global__ void k_wgrad(long n, int op, const float* __restrict__ a, const float* __restrict__ b,
+                        const float* __restrict__ w, long period, float* __restrict__ out) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < period; p += stride) {
+    const float wp = w[p];
+    for (long t = p; t < n; t += period) {
+      const float g = (op == 0) ? wp * (a[t] - b[t]) : (a[t] < 0.0f ? -wp : 0.0f);
+      out[t] += g;
+    }
+  }
+}
+
 }  // namespace lex

@@ -11,11 +11,21 @@ Mirrors, with the reference's names and arguments:
   ``padding3v_batch`` (dataset_generation.py:115-268): the shuffled 75 / 25 split of a
   master dataset of sequences into padded initial states and truth.
 
+* ``loss_and_grad`` / ``train_step`` (lex_dl_train.py:112-132): the gradient of that loss
+  with respect to the network through the solver, which the reference takes with
+  jax.value_and_grad, assembled per step in reverse from hand-written pieces:
+    lex_weighted_sum_grad, lex_laplace   the loss's gradient with respect to the stacks
+    lex_dl_apply_vjp                     -> the adjoint of the network's output
+    the network's MFMA backward          -> weight gradients, the adjoint of its input
+    lex_dl_pack_vjp                      -> added to the stepped state's adjoint
+    lex_rk3_step_vjp_taped               -> the adjoint of the step's input state
+  with every step recorded (lex_rk3_step_tape) on the way forward.  The step's adjoint
+  covers the compressible core with Smagorinsky (solver_opt 2, turb_opt 1) and the
+  physics options off.
+
 The reference's ``dl_params`` (flax parameters) are the network object itself here (a
-module mapping the (nx, ny, nz, 5) input to the correction, e.g. dl_models.simpleCNN()).
-The gradient of the loss with respect to the network through the solver (train_step's
-jax.value_and_grad, lex_dl_train.py:112-132) is not built here: the through-solver
-training of this build is the SGS net's (lex_amd.sgs_unroll).
+module mapping the (nx, ny, nz, 5) input to the correction, e.g. dl_models.simpleCNN()),
+and its optax state a torch optimizer over the network's parameters.
 """
 from __future__ import annotations
 
@@ -134,6 +144,107 @@ def loss_and_aux(params, ic, truth, scaling_params, base_state, grids, model_opt
     return s.total()
 
 
+def _wgrad(op, a, b, w, out):
+    _lib.call("lex_weighted_sum_grad", a.numel(), op, _lib.ptr(a), _lib.ptr(b) if b is not None else None, _lib.ptr(w),
+              w.numel(), _lib.ptr(out), _lib.stream())
+
+
+def _check_differentiable(model_opt):
+    solver_opt, damp_opt, rad_opt, cor_opt, sfc_opt, pic_opt, turb_opt = model_opt
+    if solver_opt != 2 or turb_opt != 1 or damp_opt or rad_opt or cor_opt or sfc_opt or pic_opt:
+        raise NotImplementedError("the step's adjoint covers solver_opt 2 with turb_opt 1 and the physics options off")
+
+
+def _stack_grads(stacks, truth, wts, th_min, qv_min, lap, summer):
+    """Loss terms of one sequence's stacks (theta, u, v, w, pip, qv; leading axis step)
+    added to `summer`, and the loss's gradient with respect to each stack."""
+    th, u, v, w, pip, qv = stacks
+    grads = [torch.zeros_like(a) for a in stacks]
+    for q, (a, t, wq) in enumerate(zip((th, u, v, w, qv), (truth[0], truth[1], truth[2], truth[3], truth[5]),
+                                       wts[:5])):
+        summer.add(0, a, t, wq)
+        _wgrad(0, a.contiguous(), t.contiguous(), wq, grads[q if q < 4 else 5])
+    for a, amin, t, wq, gq in ((th, th_min, truth[6], wts[5], grads[0]), (qv, qv_min, truth[7], wts[6], grads[5])):
+        la = lap(a - amin)
+        summer.add(0, la, t, wq)
+        r = torch.zeros_like(la)
+        _wgrad(0, la, t.contiguous(), wq, r)
+        gq += lap(r)   # the Laplacian is symmetric
+    summer.add(1, qv, None, wts[7])
+    _wgrad(1, qv.contiguous(), None, wts[7], grads[5])
+    return grads
+
+
+def loss_and_grad(params, ic, truth, scaling_params, base_state, grids, model_opt, sprint_n=None):
+    """loss_and_aux and its gradient with respect to the network's parameters, accumulated
+    into their .grad (lex_dl_train.py:112-116).  Returns the loss (0-d FP64 device tensor)."""
+    from . import adjoint as adj
+    _check_differentiable(model_opt)
+    n = nl.sprint_n if sprint_n is None else int(sprint_n)
+    dev = ic[0].device
+    g = grid_from(base_state[1], *grids[3:6])
+    norm = dlm.level_norm(scaling_params)
+    lo, den = (a.to(dev).contiguous() for a in norm)
+    ranges = dlm.scaling_ranges(scaling_params, dev)
+    wts = loss_weights(scaling_params, dev)
+    x3d, y3d, z3d, x3d4u, y3d4v, z3d4w = grids[:6]
+    th_min = torch.as_tensor(scaling_params[0], dtype=torch.float32, device=dev).squeeze()
+    qv_min = torch.as_tensor(scaling_params[8], dtype=torch.float32, device=dev).squeeze()
+
+    def lap(a):
+        return laplace_of_tensor(x3d, x3d4u, y3d, y3d4v, z3d, z3d4w, a)
+
+    summer = _Summer(dev)
+    step_args = (base_state, g, nl.dt, nl.n_sound, nl.cs, nl.prandtl)
+    for b in range(ic[0].shape[0]):
+        state = tuple(a[b].clone() for a in ic[:6])
+        xs, tapes, recs = [], [], []
+        stacks = [torch.empty((n,) + tuple(_interior(a).shape), dtype=torch.float32, device=dev) for a in state]
+        for i in range(n):
+            xs.append(state)
+            tape, s_out = adj.step_tape(state, *step_args)
+            p = dlm.phys_state_tuple2array(s_out, scaling_params, (lo, den))[0].detach().clone().requires_grad_(True)
+            c = params(p)
+            nxt = tuple(a.clone() for a in s_out)
+            dlm.apply_correction(nxt, c.detach(), ranges)
+            for q in range(6):
+                stacks[q][i].copy_(_interior(nxt[q]))
+            tapes.append(tape)
+            recs.append((p, c))
+            state = nxt
+        grads = _stack_grads(stacks, tuple(torch.as_tensor(t[b], dtype=torch.float32, device=dev) for t in truth),
+                             wts, th_min, qv_min, lap, summer)
+        lam = tuple(torch.zeros_like(a) for a in state)
+        for i in reversed(range(n)):
+            for q in (0, 1, 2, 3, 5):
+                _interior(lam[q]).add_(grads[q][i])
+            p, c = recs[i]
+            dc = torch.empty((g.nx, g.ny, g.nz, 5), dtype=torch.float32, device=dev)
+            _lib.call("lex_dl_apply_vjp", g, *[_lib.ptr(lam[q]) for q in (0, 1, 2, 3, 5)],
+                      *[_lib.ptr(r) for r in ranges], _lib.ptr(dc), _lib.stream())
+            c.backward(dc.view_as(c))
+            lam_s = tuple(t.clone() for t in lam)
+            _lib.call("lex_dl_pack_vjp", g, _lib.ptr(p.grad.contiguous()), _lib.ptr(lo), _lib.ptr(den),
+                      *[_lib.ptr(lam_s[q]) for q in (0, 1, 2, 3, 5)], _lib.stream())
+            lam, _ = adj.step_vjp_taped(xs[i], tapes[i], lam_s, *step_args)
+            recs[i] = None
+    return summer.total()
+
+
+def train_step(dl_state, ic, truth, scaling_params, base_state, grids, model_opt, sprint_n=None):
+    """train_step (lex_dl_train.py:112-132): dl_state = (network, torch optimizer over its
+    parameters); one gradient step on the loss; returns {"total_mse": loss, "grad_norm": ...}."""
+    net, opt = dl_state
+    opt.zero_grad(set_to_none=False)
+    for prm in net.parameters():
+        if prm.grad is None:
+            prm.grad = torch.zeros_like(prm)
+    loss = loss_and_grad(net, ic, truth, scaling_params, base_state, grids, model_opt, sprint_n)
+    gn = torch.sqrt(sum((prm.grad.double() ** 2).sum() for prm in net.parameters()))
+    opt.step()
+    return {"total_mse": loss, "grad_norm": gn}
+
+
 def eval_step(dl_state, ic, truth, scaling_params, base_state, grids, model_opt, sprint_n=None):
     """eval_step (lex_dl_train.py:135-139): {"total_mse": loss} (dl_state: the network)."""
     return {"total_mse": loss_and_aux(dl_state, ic, truth, scaling_params, base_state, grids, model_opt, sprint_n)}
@@ -193,5 +304,5 @@ def generate_training_data(dataset, seed):
 
 
 __all__ = ["ws_rk3_sprint_dl_train", "ws_rk3_sprint_dl_train_vmap", "laplace_of_tensor", "loss_weights",
-           "loss_and_aux", "eval_step", "padding3_batch", "padding3u_batch", "padding3v_batch", "split_indices",
+           "loss_and_aux", "loss_and_grad", "train_step", "eval_step", "padding3_batch", "padding3u_batch", "padding3v_batch", "split_indices",
            "generate_training_data"]

@@ -154,3 +154,193 @@ def test_generate_training_data_split_and_padding():
     for q in range(8):
         assert np.array_equal(from_dev(tr_true[q]), data[q][heads[:n_train], 1:].astype(np.float64))
         assert np.array_equal(from_dev(te_true[q]), data[q][heads[n_train:], 1:].astype(np.float64))
+
+
+def _rand_padded(rng, gr):
+    return (o.padding3_array(rng.normal(0, 1, (gr.nx, gr.ny, gr.nz))).astype(np.float32),
+            o.padding3u_array(rng.normal(0, 1, (gr.nx + 1, gr.ny, gr.nz))).astype(np.float32),
+            o.padding3v_array(rng.normal(0, 1, (gr.nx, gr.ny + 1, gr.nz))).astype(np.float32),
+            o.padding3_array(rng.normal(0, 1, (gr.nx, gr.ny, gr.nz + 1))).astype(np.float32),
+            o.padding3_array(rng.normal(0, 1, (gr.nx, gr.ny, gr.nz))).astype(np.float32),
+            o.padding3_array(rng.normal(0, 1, (gr.nx, gr.ny, gr.nz))).astype(np.float32))
+
+
+def _interior_of(a, q):
+    return a[3:-3, 3:-3, 1:-1]
+
+
+def test_correction_reverses_are_transposes():
+    """<A c, lam> = <c, A^T lam> for the correction (lex_dl_apply on a zero state) and
+    <pack(x), dp> = <x, pack^T dp> for the packing with zero offsets, on the interior points
+    each map reads or writes (lam folded: zero ghosts)."""
+    import torch
+    from lex_amd import _lib
+    gr = o.Grid(12, 10, 8, 100.0, 100.0, 100.0)
+    g = _lib.grid(gr.nx, gr.ny, gr.nz, gr.dx, gr.dy, gr.dz)
+    rng = np.random.default_rng(21)
+    c = rng.normal(0, 1, (gr.nx, gr.ny, gr.nz, 5)).astype(np.float32)
+    rs = [rng.uniform(0.5, 2, s).astype(np.float32) for s in ((gr.nx, gr.ny, gr.nz), (gr.nx + 1, gr.ny, gr.nz),
+                                                              (gr.nx, gr.ny + 1, gr.nz), (gr.nx, gr.ny, gr.nz + 1),
+                                                              (gr.nx, gr.ny, gr.nz))]
+    # device copies held in names: a pointer taken from a temporary tensor would dangle
+    zero = [to_dev(np.zeros_like(a)) for a in _rand_padded(rng, gr)]
+    c_d = to_dev(c)
+    rs_d = [to_dev(r) for r in rs]
+    _lib.call("lex_dl_apply", g, _lib.ptr(c_d), *[_lib.ptr(r) for r in rs_d],
+              *[_lib.ptr(zero[q]) for q in (0, 1, 2, 3, 5)], _lib.stream())
+    lam = [o.padding3_array(rng.normal(0, 1, (gr.nx, gr.ny, gr.nz))), None, None, None, None, None]
+    lam[1] = np.zeros((gr.nx + 7, gr.ny + 6, gr.nz + 2))
+    lam[1][3:-3, 3:-3, 1:-1] = rng.normal(0, 1, (gr.nx + 1, gr.ny, gr.nz))
+    lam[2] = np.zeros((gr.nx + 6, gr.ny + 7, gr.nz + 2))
+    lam[2][3:-3, 3:-3, 1:-1] = rng.normal(0, 1, (gr.nx, gr.ny + 1, gr.nz))
+    lam[3] = np.zeros((gr.nx + 6, gr.ny + 6, gr.nz + 3))
+    lam[3][3:-3, 3:-3, 1:-1] = rng.normal(0, 1, (gr.nx, gr.ny, gr.nz + 1))
+    lam[5] = np.zeros((gr.nx + 6, gr.ny + 6, gr.nz + 2))
+    lam[5][3:-3, 3:-3, 1:-1] = rng.normal(0, 1, (gr.nx, gr.ny, gr.nz))
+    lam[0][:3] = lam[0][-3:] = 0.0
+    lam[0][:, :3] = lam[0][:, -3:] = 0.0
+    lam[0][:, :, 0] = lam[0][:, :, -1] = 0.0
+    lam32 = [None if a is None else a.astype(np.float32) for a in lam]
+    lhs = sum(float(np.sum(from_dev(zero[q]) * lam32[q].astype(np.float64))) for q in (0, 1, 2, 3, 5))
+    dc = torch.empty((gr.nx, gr.ny, gr.nz, 5), dtype=torch.float32, device="cuda")
+    lam_d = [None if a is None else to_dev(a) for a in lam32]
+    _lib.call("lex_dl_apply_vjp", g, *[_lib.ptr(lam_d[q]) for q in (0, 1, 2, 3, 5)],
+              *[_lib.ptr(r) for r in rs_d], _lib.ptr(dc), _lib.stream())
+    rhs = float(np.sum(from_dev(dc) * c.astype(np.float64)))
+    assert abs(lhs - rhs) <= 1e-5 * (abs(lhs) + 1.0), (lhs, rhs)
+    # the packing (lo = 0, so linear)
+    x = [to_dev(a) for a in _rand_padded(rng, gr)]
+    lo = torch.zeros(5 * gr.nz + 1, dtype=torch.float32, device="cuda")
+    den = to_dev(rng.uniform(0.5, 2.0, 5 * gr.nz + 1).astype(np.float32))
+    out = torch.empty((gr.nx, gr.ny, gr.nz, 5), dtype=torch.float32, device="cuda")
+    _lib.call("lex_dl_pack", g, *[_lib.ptr(x[q]) for q in (0, 1, 2, 3, 5)], _lib.ptr(lo), _lib.ptr(den), _lib.ptr(out),
+              _lib.stream())
+    dp = rng.normal(0, 1, (gr.nx, gr.ny, gr.nz, 5)).astype(np.float32)
+    lhs = float(np.sum(from_dev(out) * dp.astype(np.float64)))
+    adjs = [torch.zeros_like(x[q]) for q in range(6)]
+    dp_d = to_dev(dp)
+    _lib.call("lex_dl_pack_vjp", g, _lib.ptr(dp_d), _lib.ptr(lo), _lib.ptr(den),
+              *[_lib.ptr(adjs[q]) for q in (0, 1, 2, 3, 5)], _lib.stream())
+    rhs = sum(float(np.sum(from_dev(adjs[q]) * from_dev(x[q]))) for q in (0, 1, 2, 3, 5))
+    assert abs(lhs - rhs) <= 1e-5 * (abs(lhs) + 1.0), (lhs, rhs)
+    for q in (0, 1, 2, 3, 5):   # only interior points receive adjoint
+        a = from_dev(adjs[q])
+        assert not a[:3].any() and not a[-3:].any() and not a[:, :3].any() and not a[:, -3:].any()
+
+
+def test_loss_gradient_of_the_last_bias():
+    """One corrected step: a constant on the last layer's bias of channel q shifts field q
+    by the bias times its range on every point the correction reaches (stagger_uvw's end
+    faces average two equal values; w's surface and lid stay), so d loss / d bias_q is the
+    sum over those points of range x d loss / d field, with the loss's gradient in FP64
+    from the device's own stacks (l2 terms, the symmetric Laplacian's chain, the
+    negative-qv penalty).  Tolerance: the network's BF16 backward (2^-7 of the summed
+    magnitudes)."""
+    import torch
+    from lex_amd import dl_models as dlm, dl_train as dt, namelist_n_constants as nl
+    from lex_amd.setup_lex import build_case, config1
+    case, opts = config1()
+    case = case.scaled(16, 12, 12)
+    phys, base, grids, _ = build_case(case)
+    nl.dt, nl.n_sound, nl.solver_opt, nl.turb_opt = opts["dt"], 7, 2, 1
+    model_opt = (2, False, False, 0, False, False, 1)
+    rng = np.random.default_rng(31)
+    spd = tuple(to_dev(a) for a in _scaling(rng, case.nx, case.ny, case.nz))
+    net = dlm.simpleCNN(seed=11)
+    with torch.no_grad():
+        net.weights[-1].mul_(1e-3)
+        net.biases[-1].fill_(1e-4)
+    ic = tuple(a.unsqueeze(0).clone() for a in phys[:6])
+    stacks = dt.ws_rk3_sprint_dl_train_vmap(*ic, base, grids, model_opt, net, spd, sprint_n=1)
+    og = o.Grid(case.nx, case.ny, case.nz, case.dx, case.dy, case.dz)
+    own = tuple(from_dev(a) for a in stacks)
+    sp0 = tuple(from_dev(a)[0, 0] for a in spd)
+    truth = [r + rng.normal(0, s, r.shape) for r, s in zip(own, (0.05, 0.05, 0.05, 0.05, 1e-4, 1e-4))]
+    truth += [o.laplace_of_tensor(own[0] - sp0[0], og) + rng.normal(0, 1e-4, own[0].shape),
+              o.laplace_of_tensor(own[5] - sp0[8], og) + rng.normal(0, 1e-9, own[5].shape)]
+    truth_d = tuple(to_dev(t.astype(np.float32)) for t in truth)
+    for prm in net.parameters():
+        prm.grad = torch.zeros_like(prm)
+    loss = float(dt.loss_and_grad(net, ic, truth_d, spd, base, grids, model_opt, sprint_n=1).item())
+    assert abs(loss - o.dl_loss(own, truth, sp0, og)) <= 1e-5 * loss
+    gb = from_dev(net.biases[-1].grad)
+    w8 = o.dl_loss_weights(sp0)
+    tr = [np.asarray(t, np.float64) for t in truth]
+    g = [w8[0] * (own[0] - tr[0]), w8[1] * (own[1] - tr[1]), w8[2] * (own[2] - tr[2]), w8[3] * (own[3] - tr[3]),
+         w8[4] * (own[5] - tr[5])]
+    g[0] = g[0] + o.laplace_of_tensor(w8[5] * (o.laplace_of_tensor(own[0] - sp0[0], og) - tr[6]), og)
+    g[4] = g[4] + o.laplace_of_tensor(w8[6] * (o.laplace_of_tensor(own[5] - sp0[8], og) - tr[7]), og)
+    g[4] = g[4] - np.where(own[5] < 0, w8[7], 0.0)
+    for q in range(5):
+        r = sp0[2 * q + 1] - sp0[2 * q]
+        terms = g[q] * r
+        if q == 3:   # w: the surface and lid levels receive no correction
+            terms = terms[..., 1:-1]
+        exact, mag = float(terms.sum()), float(np.abs(terms).sum())
+        print(f"channel {q}: adjoint {gb[q]:.6e} exact {exact:.6e} (sum of magnitudes {mag:.3e})")
+        assert abs(gb[q] - exact) <= 2.0 ** -7 * mag + 1e-6 * abs(exact), q
+    assert all(np.isfinite(from_dev(prm.grad)).all() for prm in net.parameters())
+
+
+def test_two_step_gradient_is_finite_and_uses_the_solver():
+    """Two corrected steps: the gradient passes through the step's adjoint (the first
+    step's bias gradient differs from a one-step sprint's) and stays finite."""
+    import torch
+    from lex_amd import dl_models as dlm, dl_train as dt, namelist_n_constants as nl
+    from lex_amd.setup_lex import build_case, config1
+    case, opts = config1()
+    case = case.scaled(16, 12, 12)
+    phys, base, grids, _ = build_case(case)
+    nl.dt, nl.n_sound, nl.solver_opt, nl.turb_opt = opts["dt"], 7, 2, 1
+    model_opt = (2, False, False, 0, False, False, 1)
+    rng = np.random.default_rng(33)
+    spd = tuple(to_dev(a) for a in _scaling(rng, case.nx, case.ny, case.nz))
+    net = dlm.simpleCNN(seed=12)
+    with torch.no_grad():
+        net.weights[-1].mul_(1e-3)
+    ic = tuple(a.unsqueeze(0).clone() for a in phys[:6])
+    stacks = dt.ws_rk3_sprint_dl_train_vmap(*ic, base, grids, model_opt, net, spd, sprint_n=2)
+    og = o.Grid(case.nx, case.ny, case.nz, case.dx, case.dy, case.dz)
+    own = tuple(from_dev(a) for a in stacks)
+    sp0 = tuple(from_dev(a)[0, 0] for a in spd)
+    truth = [r + 0.01 for r in own] + [o.laplace_of_tensor(own[0] - sp0[0], og), o.laplace_of_tensor(own[5] - sp0[8], og)]
+    truth_d = tuple(to_dev(t.astype(np.float32)) for t in truth)
+    grads = []
+    for n in (1, 2):
+        for prm in net.parameters():
+            prm.grad = torch.zeros_like(prm)
+        tr = tuple(t[:, :n].contiguous() for t in truth_d)
+        dt.loss_and_grad(net, ic, tr, spd, base, grids, model_opt, sprint_n=n)
+        grads.append([from_dev(prm.grad) for prm in net.parameters()])
+    assert all(np.isfinite(a).all() for a in grads[1])
+    assert any(not np.allclose(a, b) for a, b in zip(grads[0], grads[1]))
+
+
+def test_train_step_lowers_the_loss():
+    import torch
+    from lex_amd import dl_models as dlm, dl_train as dt, namelist_n_constants as nl
+    from lex_amd.setup_lex import build_case, config1
+    case, opts = config1()
+    case = case.scaled(16, 12, 12)
+    phys, base, grids, _ = build_case(case)
+    nl.dt, nl.n_sound, nl.solver_opt, nl.turb_opt = opts["dt"], 7, 2, 1
+    model_opt = (2, False, False, 0, False, False, 1)
+    rng = np.random.default_rng(41)
+    spd = tuple(to_dev(a) for a in _scaling(rng, case.nx, case.ny, case.nz))
+    net = dlm.simpleCNN(seed=13)
+    with torch.no_grad():
+        net.weights[-1].mul_(1e-3)
+    ic = tuple(a.unsqueeze(0).clone() for a in phys[:6])
+    stacks = dt.ws_rk3_sprint_dl_train_vmap(*ic, base, grids, model_opt, net, spd, sprint_n=1)
+    og = o.Grid(case.nx, case.ny, case.nz, case.dx, case.dy, case.dz)
+    own = tuple(from_dev(a) for a in stacks)
+    sp0 = tuple(from_dev(a)[0, 0] for a in spd)
+    truth = [own[0] + 0.01, own[1] - 0.02, own[2], own[3], own[4], own[5]]
+    truth += [o.laplace_of_tensor(own[0] - sp0[0], og), o.laplace_of_tensor(own[5] - sp0[8], og)]
+    truth = tuple(to_dev(t.astype(np.float32)) for t in truth)
+    # Adam's first steps move every parameter by about lr along minus the gradient's sign:
+    # a small lr keeps them in the loss's first-order neighbourhood
+    opt = torch.optim.Adam(net.parameters(), lr=1e-6)
+    losses = [float(dt.train_step((net, opt), ic, truth, spd, base, grids, model_opt, sprint_n=1)["total_mse"].item())
+              for _ in range(3)]
+    assert losses[1] < losses[0] and losses[2] < losses[1], losses

@@ -27,7 +27,8 @@ def test_coarsen_matches_oracle():
              (3, p32[3], (nxc, nyc, nzc + 1), o.coarsen_face_mean(f64(p32[3][3:-3, 3:-3, 1:-1]), 2, *f))]
     for mode, a, shp, ref in cases:
         out = torch.empty(shp, dtype=torch.float32, device="cuda")
-        _lib.call("lex_coarsen", g, mode, *f, _lib.ptr(to_dev(a)), _lib.ptr(out), _lib.stream())
+        a_d = to_dev(a)   # held: a pointer to a temporary would dangle
+        _lib.call("lex_coarsen", g, mode, *f, _lib.ptr(a_d), _lib.ptr(out), _lib.stream())
         got = from_dev(out)
         np.testing.assert_allclose(got, ref, rtol=2e-6, atol=2e-6 * float(np.abs(ref).max()))
 

@@ -59,7 +59,8 @@ def test_sgs_apply_vjp_dot():
     lam = [rng.normal(size=s) for s in _sgs_shapes(nx, ny, nz)]
     osc = (ctypes.c_float * 5)(1.5, 0.5, 2.0, 0.25, 3.0)
     outs = [torch.empty(s, dtype=torch.float32, device="cuda") for s in _sgs_shapes(nx, ny, nz)]
-    _lib.call("lex_sgs_apply", g, _lib.ptr(to_dev(y)), osc, *[_lib.ptr(t) for t in outs], _lib.stream())
+    y_d = to_dev(y)   # held: a pointer to a temporary would dangle
+    _lib.call("lex_sgs_apply", g, _lib.ptr(y_d), osc, *[_lib.ptr(t) for t in outs], _lib.stream())
     ly = adj.sgs_apply_vjp(g, tuple(to_dev(a) for a in lam), osc)
     lhs = sum(_dot(from_dev(t), l) for t, l in zip(outs, lam))
     rhs = _dot(y, from_dev(ly))
