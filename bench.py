@@ -233,6 +233,57 @@ def dl_loss_timing(case, grids, sprint_n):
             "GBs": nbytes / (ms / 1e3) / 1e9}
 
 
+def dl_train_timing(sprint_n=5, nx=128, ny=128, nz=64):
+    """One train_step of the post-step correction (lex_dl_train.py:112-132) with simpleCNN
+    on a 128x128x64 slice of the 256x256x128 workload recipe: a sprint_n-step corrected
+    sprint recorded, the loss, and its gradient through the solver; against the corrected
+    sprint alone (loss_and_aux)."""
+    import numpy as np
+    import torch
+    from lex_amd import dl_models as dlm, dl_train as dt, namelist_n_constants as nl
+    from lex_amd.setup_lex import build_case, config1
+    case, opts = config1()
+    case = case.scaled(nx, ny, nz)
+    phys, base, grids, _ = build_case(case)
+    nl.dt, nl.n_sound, nl.solver_opt, nl.turb_opt = opts["dt"], 7, 2, 1
+    mo = (2, False, False, 0, False, False, 1)
+    sc = []
+    for a, q in zip((phys[0], phys[1], phys[2], phys[3], phys[5]), range(5)):
+        inner = a[3:-3, 3:-3, 1:-1]
+        spread = float(inner.std()) + 1e-3
+        sc += [(inner - spread)[None, None].contiguous(), (inner + spread)[None, None].contiguous()]
+    for a in (phys[0], phys[5]):
+        inner = a[3:-3, 3:-3, 1:-1]
+        sc += [(torch.zeros_like(inner) - 1e-4)[None, None], (torch.zeros_like(inner) + 1e-4)[None, None]]
+    scaling = tuple(sc)
+    net = dlm.simpleCNN(seed=7)
+    with torch.no_grad():
+        net.weights[-1].mul_(1e-3)
+    ic = tuple(a.unsqueeze(0).clone() for a in phys[:6])
+    st = dt.ws_rk3_sprint_dl_train_vmap(*ic, base, grids, mo, net, scaling, sprint_n=sprint_n)
+    lap = lambda a: dt.laplace_of_tensor(*(grids[i] for i in (0, 3, 1, 4, 2, 5)), a)  # noqa: E731
+    truth = tuple(a + 0.01 for a in st) + (lap(st[0] - scaling[0][0, 0]), lap(st[5] - scaling[8][0, 0]))
+    opt = torch.optim.Adam(net.parameters(), lr=1e-6)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    dt.train_step((net, opt), ic, truth, scaling, base, grids, mo, sprint_n=sprint_n)
+    torch.cuda.synchronize()
+    e0.record()
+    m = dt.train_step((net, opt), ic, truth, scaling, base, grids, mo, sprint_n=sprint_n)
+    e1.record()
+    torch.cuda.synchronize()
+    t_train = e0.elapsed_time(e1)
+    e0.record()
+    loss = dt.loss_and_aux(net, ic, truth, scaling, base, grids, mo, sprint_n=sprint_n)
+    e1.record()
+    torch.cuda.synchronize()
+    t_fwd = e0.elapsed_time(e1)
+    return {"workload": f"train_step of the post-step correction (simpleCNN), {nx}x{ny}x{nz}, n_sound 7, "
+                        f"sprint_n {sprint_n}, batch 1",
+            "train_step_ms": t_train, "loss_only_ms": t_fwd, "ratio": t_train / t_fwd,
+            "loss": float(loss.item()), "grad_norm": float(m["grad_norm"].item()),
+            "finite": bool(np.isfinite(float(m["grad_norm"].item())))}
+
+
 def physics_timing(steps=10):
     """The 256x256x128 step with the upper-level Rayleigh damping (layer above 9.6 km of
     the 12.8 km column), the Newtonian heating and the bulk surface fluxes (sea surface
@@ -400,9 +451,11 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, copy a 256 MiB buffer 3 times (a kernel of known HBM "
                          "traffic, used to scale rocprof FETCH_SIZE/WRITE_SIZE counters)")
-    ap.add_argument("--only", choices=("adjoint", "sgs-net", "output", "dl-sprint", "physics"), default=None,
+    ap.add_argument("--only", choices=("adjoint", "sgs-net", "output", "dl-sprint", "dl-train", "physics"),
+                    default=None,
                     help="run only the forward+adjoint workload, the SGS-net timing, the output-file "
-                         "timing, the post-step-correction sprint or the damping/heating step and print its "
+                         "timing, the post-step-correction sprint, its train_step or the physics-option steps and "
+                         "print its "
                          "JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -417,7 +470,7 @@ def main():
     _lib.load()
     if args.only is not None:
         fn = {"adjoint": fwd_adjoint, "sgs-net": sgs_net_timing, "output": output_timing,
-              "dl-sprint": dl_sprint_timing, "physics": physics_timing}[args.only]
+              "dl-sprint": dl_sprint_timing, "dl-train": dl_train_timing, "physics": physics_timing}[args.only]
         print(json.dumps(fn()))
         return
     case, opts = config1()
