@@ -39,7 +39,7 @@ inline const char* kernel_name(int id) {
   static const char* names[KID_COUNT] = {
       "k_smag_deform", "k_smag_nm",     "k_smag_k",    "k_smag_uvw",         "k_smag_scalar",
       "k_div_rho",     "k_rhs_uv",      "k_rhs_w",     "k_theta_rho",        "k_acoustic",
-      "k_scalar_update", "k_copy",      "k_stage_scalars", "k_mom_u", "k_smag_fused", "k_mom_v", "k_mom_w",
+      "k_scalar_update", "k_copy",      "k_stage_scalars", "k_mom_u", "k_smag_tau", "k_mom_v", "k_mom_w",
       "k_acoustic_vi", "k_acoustic2", "k_newton_heating", "k_rayleigh",
       "k_atm_ocn_flux"};
   return (id >= 0 && id < KID_COUNT) ? names[id] : "";

@@ -13,11 +13,15 @@
 
 namespace lex {
 
-__device__ __forceinline__ float ex3(float a0, float a1, float a2) { return 1.875f * a0 + (-1.25f) * a1 + 0.375f * a2; }
+__device__ __forceinline__ float ex3(float a0, float a1, float a2) {
+#pragma clang fp contract(off)
+  return 1.875f * a0 + (-1.25f) * a1 + 0.375f * a2;
+}
 
 // e12 at corner (u-face F, v-face G), scalar level K
 __device__ __forceinline__ float e12_at(const Dims& d, const float* __restrict__ u, const float* __restrict__ v,
                                         int F, int G, int K) {
+#pragma clang fp contract(off)
   float dudy = (u[iu(d, F, G, K)] - u[iu(d, F, G - 1, K)]) * d.idy;
   float dvdx = (v[iv(d, F, G, K)] - v[iv(d, F - 1, G, K)]) * d.idx;
   return 0.5f * (dudy + dvdx);
@@ -25,6 +29,7 @@ __device__ __forceinline__ float e12_at(const Dims& d, const float* __restrict__
 // e13 at (u-face F, row J, interior w level l in 1..nz-1)
 __device__ __forceinline__ float e13_at(const Dims& d, const float* __restrict__ u, const float* __restrict__ w,
                                         int F, int J, int l) {
+#pragma clang fp contract(off)
   float dudz = (u[iu(d, F, J, l + 1)] - u[iu(d, F, J, l)]) * d.idz;
   float dwdx = (w[iw(d, F, J, l + 1)] - w[iw(d, F - 1, J, l + 1)]) * d.idx;
   return 0.5f * (dudz + dwdx);
@@ -32,6 +37,7 @@ __device__ __forceinline__ float e13_at(const Dims& d, const float* __restrict__
 // e23 at (column I, v-face G, interior w level l)
 __device__ __forceinline__ float e23_at(const Dims& d, const float* __restrict__ v, const float* __restrict__ w,
                                         int I, int G, int l) {
+#pragma clang fp contract(off)
   float dvdz = (v[iv(d, I, G, l + 1)] - v[iv(d, I, G, l)]) * d.idz;
   float dwdy = (w[iw(d, I, G, l + 1)] - w[iw(d, I, G - 1, l + 1)]) * d.idy;
   return 0.5f * (dvdz + dwdy);
@@ -61,15 +67,16 @@ __device__ __forceinline__ ZStencil z_stencil(int k, int nz) {
   return z;
 }
 
-__global__ void k_smag_deform(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
-                              const float* __restrict__ v, const float* __restrict__ w, float* __restrict__ s11,
-                              float* __restrict__ s22, float* __restrict__ s33, float* __restrict__ s12,
-                              float* __restrict__ s13, float* __restrict__ s23, float* __restrict__ deform) {
-  const TileId tb = xcd_tile();
-  int k = tb.x * blockDim.x + threadIdx.x;
-  int j = tb.y * blockDim.y + threadIdx.y;
-  int i = tb.z;
-  if (k >= d.nz || j >= d.ny) return;
+struct SmagCell {
+  float a11, a22, a33, a12, a13, a23, df;
+};
+
+// rho S_ij and the deformation at interior cell (i, j, k) (compute_deformation,
+// turbulence.py:298-388)
+__device__ __forceinline__ SmagCell smag_deform_cell(const Dims& d, const float* __restrict__ rc,
+                                                     const float* __restrict__ u, const float* __restrict__ v,
+                                                     const float* __restrict__ w, int i, int j, int k) {
+#pragma clang fp contract(off)
   int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
   float r = rc[K];
   float a11 = r * (u[iu(d, I + 1, J, K)] - u[iu(d, I, J, K)]) * d.idx;
@@ -101,21 +108,47 @@ __global__ void k_smag_deform(Dims d, const float* __restrict__ rc, const float*
   a13 = r * a13;
   a23 = r * a23;
   float df = (2.0f * (a11 * a11 + a22 * a22 + a33 * a33) + 4.0f * (a12 * a12 + a13 * a13 + a23 * a23)) / (r * r);
-  store_c_fast(d, s11, i, j, k, a11);
-  store_c_fast(d, s22, i, j, k, a22);
-  store_c_fast(d, s33, i, j, k, a33);
-  store_c_fast(d, s12, i, j, k, a12);
-  store_c_fast(d, s13, i, j, k, a13);
-  store_c_fast(d, s23, i, j, k, a23);
-  store_c_fast(d, deform, i, j, k, df);
+  return SmagCell{a11, a22, a33, a12, a13, a23, df};
+}
+
+__global__ void k_smag_deform(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
+                              const float* __restrict__ v, const float* __restrict__ w, float* __restrict__ s11,
+                              float* __restrict__ s22, float* __restrict__ s33, float* __restrict__ s12,
+                              float* __restrict__ s13, float* __restrict__ s23, float* __restrict__ deform) {
+  const TileId tb = xcd_tile();
+  int k = tb.x * blockDim.x + threadIdx.x;
+  int j = tb.y * blockDim.y + threadIdx.y;
+  int i = tb.z;
+  if (k >= d.nz || j >= d.ny) return;
+  const SmagCell c = smag_deform_cell(d, rc, u, v, w, i, j, k);
+  store_c_fast(d, s11, i, j, k, c.a11);
+  store_c_fast(d, s22, i, j, k, c.a22);
+  store_c_fast(d, s33, i, j, k, c.a33);
+  store_c_fast(d, s12, i, j, k, c.a12);
+  store_c_fast(d, s13, i, j, k, c.a13);
+  store_c_fast(d, s23, i, j, k, c.a23);
+  store_c_fast(d, deform, i, j, k, c.df);
 }
 
 // N^2 at interior w level l (1..nz-1) of column (I,J)
 __device__ __forceinline__ float n28w_at(const Dims& d, const float* __restrict__ th, const float* __restrict__ qv,
                                          long c, int l) {
+#pragma clang fp contract(off)
   float a = th[c + l + 1] * (1.0f + (float)kRepsm1 * qv[c + l + 1]);
   float b = th[c + l] * (1.0f + (float)kRepsm1 * qv[c + l]);
   return logf(a / b) / d.dz * (float)kG;
+}
+
+// N^2 at interior cell (i, j, k) (compute_nm, turbulence.py:391-427)
+__device__ __forceinline__ float smag_n2_cell(const Dims& d, const float* __restrict__ th,
+                                              const float* __restrict__ qv, int i, int j, int k) {
+#pragma clang fp contract(off)
+  long c = ic(d, i + 3, j + 3, 0);
+  const ZStencil z = z_stencil(k, d.nz);
+  float val = 0.f;
+#pragma unroll
+  for (int t = 0; t < 3; ++t) val += z.c[t] * n28w_at(d, th, qv, c, z.l[t]);
+  return val;
 }
 
 __global__ void k_smag_nm(Dims d, const float* __restrict__ th, const float* __restrict__ qv, float* __restrict__ n2) {
@@ -124,12 +157,15 @@ __global__ void k_smag_nm(Dims d, const float* __restrict__ th, const float* __r
   int j = tb.y * blockDim.y + threadIdx.y;
   int i = tb.z;
   if (k >= d.nz || j >= d.ny) return;
-  long c = ic(d, i + 3, j + 3, 0);
-  const ZStencil z = z_stencil(k, d.nz);
-  float val = 0.f;
-#pragma unroll
-  for (int t = 0; t < 3; ++t) val += z.c[t] * n28w_at(d, th, qv, c, z.l[t]);
-  store_c_fast(d, n2, i, j, k, val);
+  store_c_fast(d, n2, i, j, k, smag_n2_cell(d, th, qv, i, j, k));
+}
+
+// compute_k's eddy viscosity from the deformation and N^2 (turbulence.py:430-446)
+__device__ __forceinline__ float smag_km(float s2, float n2, float csgs2, float prandtl) {
+#pragma clang fp contract(off)
+  float x = s2 - n2 / prandtl;
+  x = (x > 0.0f) ? x : 0.0f;
+  return csgs2 * sqrtf(x);
 }
 
 // elementwise over the padded array
@@ -137,9 +173,7 @@ __global__ void k_smag_k(long n, const float* __restrict__ s2, const float* __re
                          float* __restrict__ kh, float csgs2, float prandtl) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  float x = s2[t] - n2[t] / prandtl;
-  x = (x > 0.0f) ? x : 0.0f;
-  float m = csgs2 * sqrtf(x);
+  const float m = smag_km(s2[t], n2[t], csgs2, prandtl);
   km[t] = m;
   kh[t] = m / prandtl;
 }
@@ -150,6 +184,18 @@ __global__ void k_smag_k(long n, const float* __restrict__ s2, const float* __re
 // Dv = 4 (s13^2 + s23^2) / rho^2 and 1/Lv^2 = (0.4 z)^-2 + dz^-2, z the cell-centre height
 // of the padded level ((K - 1/2) dz, negative in the ghost level below the surface).
 // Khh = Kmh and Khv = Kmv (prandtl 1).
+// compute_k2's (Kmh, Kmv) at padded level K with density r (turbulence.py:449-471)
+__device__ __forceinline__ float2 smag_k2(const Dims& d, float r, int K, float a11, float a22, float a33, float a12,
+                                          float a13, float a23, float n2, float csq) {
+#pragma clang fp contract(off)
+  const float z = ((float)K - 0.5f) * d.dz;
+  const float zk = 0.4f * z;
+  const float lv2 = 1.0f / (1.0f / (zk * zk) + 1.0f / (d.dz * d.dz));
+  const float dh = (2.0f * (a11 * a11 + a22 * a22 + a33 * a33) + 4.0f * (a12 * a12)) / (r * r);
+  const float dv = 4.0f * (a13 * a13 + a23 * a23) / (r * r);
+  return make_float2(csq * (d.dx * d.dy) * dh, csq * lv2 * sqrtf(fmaxf(dv - n2 / 1.0f, 0.0f)));
+}
+
 __global__ void k_smag_k2(Dims d, const float* __restrict__ rc, const float* __restrict__ s11,
                           const float* __restrict__ s22, const float* __restrict__ s33, const float* __restrict__ s12,
                           const float* __restrict__ s13, const float* __restrict__ s23, const float* __restrict__ n2,
@@ -158,27 +204,67 @@ __global__ void k_smag_k2(Dims d, const float* __restrict__ rc, const float* __r
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int K = (int)(t % (d.nz + 2));
-  const float r = rc[K];
-  const float z = ((float)K - 0.5f) * d.dz;
-  const float zk = 0.4f * z;
-  const float lv2 = 1.0f / (1.0f / (zk * zk) + 1.0f / (d.dz * d.dz));
-  const float a11 = s11[t], a22 = s22[t], a33 = s33[t], a12 = s12[t], a13 = s13[t], a23 = s23[t];
-  const float dh = (2.0f * (a11 * a11 + a22 * a22 + a33 * a33) + 4.0f * (a12 * a12)) / (r * r);
-  const float dv = 4.0f * (a13 * a13 + a23 * a23) / (r * r);
-  kmh[t] = csq * (d.dx * d.dy) * dh;
-  kmv[t] = csq * lv2 * sqrtf(fmaxf(dv - n2[t] / 1.0f, 0.0f));
+  const float2 k2 = smag_k2(d, rc[K], K, s11[t], s22[t], s33[t], s12[t], s13[t], s23[t], n2[t], csq);
+  kmh[t] = k2.x;
+  kmv[t] = k2.y;
+}
+
+// compute_deformation -> compute_nm -> compute_k (aniso 0) or compute_k2 (aniso 1) in one
+// pass per interior cell, writing the stresses tau_ij = -2 K s_ij (horizontal K for 11,
+// 22, 33, 12, vertical for 13, 23) and the scalar diffusivities (khh; khv when aniso) with
+// their ghost images: the same values, operation for operation, as k_smag_deform,
+// k_smag_nm, k_smag_k / k_smag_k2 and the products inside k_smag_uvw.
+struct SmagTau {
+  float* t[6];
+  float* khh;
+  float* khv;
+};
+__global__ void k_smag_tau(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
+                           const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ th,
+                           const float* __restrict__ qv, float csgs2, float prandtl, int aniso, float csq, SmagTau o) {
+#pragma clang fp contract(off)
+  const TileId tb = xcd_tile();
+  int k = tb.x * blockDim.x + threadIdx.x;
+  int j = tb.y * blockDim.y + threadIdx.y;
+  int i = tb.z;
+  if (k >= d.nz || j >= d.ny) return;
+  const SmagCell c = smag_deform_cell(d, rc, u, v, w, i, j, k);
+  const float n2 = smag_n2_cell(d, th, qv, i, j, k);
+  float kmh, kmv, khh, khv;
+  if (aniso) {
+    const float2 k2 = smag_k2(d, rc[k + 1], k + 1, c.a11, c.a22, c.a33, c.a12, c.a13, c.a23, n2, csq);
+    kmh = k2.x;
+    kmv = k2.y;
+    khh = kmh;
+    khv = kmv;
+  } else {
+    kmh = smag_km(c.df, n2, csgs2, prandtl);
+    kmv = kmh;
+    khh = kmh / prandtl;
+    khv = khh;
+  }
+  store_c_fast(d, o.t[0], i, j, k, -2.0f * kmh * c.a11);
+  store_c_fast(d, o.t[1], i, j, k, -2.0f * kmh * c.a22);
+  store_c_fast(d, o.t[2], i, j, k, -2.0f * kmh * c.a33);
+  store_c_fast(d, o.t[3], i, j, k, -2.0f * kmh * c.a12);
+  store_c_fast(d, o.t[4], i, j, k, -2.0f * kmv * c.a13);
+  store_c_fast(d, o.t[5], i, j, k, -2.0f * kmv * c.a23);
+  store_c_fast(d, o.khh, i, j, k, khh);
+  if (aniso) store_c_fast(d, o.khv, i, j, k, khv);
 }
 
 // tau_ij = -2 km s_ij at padded scalar index c: horizontal coefficient for 11, 22, 33, 12,
-// vertical for 13, 23
-#define TAU(arr, c) (-2.0f * kmh[c] * arr[c])
-#define TAUV(arr, c) (-2.0f * kmv[c] * arr[c])
+// vertical for 13, 23; with PRE the arrays hold tau_ij already (k_smag_tau)
+#define TAU(arr, c) (PRE ? arr[c] : -2.0f * kmh[c] * arr[c])
+#define TAUV(arr, c) (PRE ? arr[c] : -2.0f * kmv[c] * arr[c])
 
+template <bool PRE>
 __global__ void k_smag_uvw(Dims d, const float* __restrict__ rc, const float* __restrict__ kmh,
                            const float* __restrict__ kmv, const float* __restrict__ s11, const float* __restrict__ s22,
                            const float* __restrict__ s33, const float* __restrict__ s12, const float* __restrict__ s13,
                            const float* __restrict__ s23, float* __restrict__ su, float* __restrict__ sv,
                            float* __restrict__ sw) {
+#pragma clang fp contract(off)   // the stresses are rounded products whether precomputed or not
   const TileId tb = xcd_tile();
   int k = tb.x * blockDim.x + threadIdx.x;  // 0..nz
   int j = tb.y * blockDim.y + threadIdx.y;  // 0..ny

@@ -345,29 +345,24 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
     // learned SGS net); all frozen over the step
     const bool turb = o.turb != 0;
     if (s == 0 && (o.turb == 1 || o.turb == 2)) {
-      // compute_deformation -> compute_nm -> compute_k (compute_k2) -> compute_smag (compute_smag2)
-      LEX_LAUNCH(KID_SMAG_DEFORM, k_smag_deform, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, src[1], src[2], src[3], sm[0], sm[1],
-                         sm[2], sm[3], sm[4], sm[5], sm[6]);
-      LEX_LAUNCH(KID_SMAG_NM, k_smag_nm, G(d.nz, d.ny, d.nx), bk, 0, st, d, src[0], src[5], sm[7]);
       long n = (long)L.nc;
-      if (o.turb == 1) {
-        float gs = cbrtf(d.dx * d.dy * d.dz);
-        float cg = o.cs * gs;
-        LEX_LAUNCH(KID_SMAG_K, k_smag_k, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, sm[6], sm[7], sm[8],
-                   sm[9], cg * cg, o.prandtl);
-      } else {  // sm[8] = Kmh = Khh, sm[9] = Kmv = Khv
-        LEX_LAUNCH(KID_SMAG_K, k_smag_k2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, rc, sm[0], sm[1],
-                   sm[2], sm[3], sm[4], sm[5], sm[7], sm[8], sm[9], o.cs * o.cs);
-      }
-      // coefficients: turb 1 Km (sm[8]) for every stress, Kh (sm[9]) for every scalar
-      // face; turb 2 Kmh / Kmv (sm[8] / sm[9]) for both
-      const float* kmh = sm[8];
-      const float* kmv = (o.turb == 1) ? sm[8] : sm[9];
-      const float* khh = (o.turb == 1) ? sm[9] : sm[8];
-      const float* khv = sm[9];
-      LEX_LAUNCH(KID_SMAG_UVW, k_smag_uvw, G(d.nz + 1, d.ny + 1, d.nx + 1), bk, 0, st, d, rc, kmh, kmv, sm[0], sm[1],
-                 sm[2], sm[3], sm[4], sm[5], su, sv, sw);
-      if (o.impl != 1) {  // every scalar in one pass (impl 1: one call per scalar, as the reference)
+      const float gs = cbrtf(d.dx * d.dy * d.dz);
+      const float cg = o.cs * gs;
+      const float* khh;
+      const float* khv;
+      if (o.impl != 1) {
+        // the fused paths: deformation, N^2 and the coefficients in one pass writing the
+        // stresses and diffusivities (k_smag_tau), then the stress divergence
+        SmagTau T;
+        for (int q = 0; q < 6; ++q) T.t[q] = sm[q];
+        T.khh = sm[6];
+        T.khv = (o.turb == 2) ? sm[7] : nullptr;
+        LEX_LAUNCH(KID_SMAG_FUSED, k_smag_tau, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, src[1], src[2], src[3], src[0],
+                   src[5], cg * cg, o.prandtl, (int)(o.turb == 2), o.cs * o.cs, T);
+        LEX_LAUNCH(KID_SMAG_UVW, k_smag_uvw<true>, G(d.nz + 1, d.ny + 1, d.nx + 1), bk, 0, st, d, rc,
+                   (const float*)nullptr, (const float*)nullptr, sm[0], sm[1], sm[2], sm[3], sm[4], sm[5], su, sv, sw);
+        khh = sm[6];
+        khv = (o.turb == 2) ? sm[7] : sm[6];
         SgsScalars SS;
         SS.n = 2 + o.n_passive;
         for (int q = 0; q < SS.n; ++q) {
@@ -376,6 +371,26 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
         }
         LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalars, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, khh, khv, SS);
       } else {
+        // one kernel per reference routine: compute_deformation -> compute_nm -> compute_k
+        // (compute_k2) -> compute_smag (compute_smag2), one call per scalar
+        LEX_LAUNCH(KID_SMAG_DEFORM, k_smag_deform, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, src[1], src[2], src[3], sm[0],
+                   sm[1], sm[2], sm[3], sm[4], sm[5], sm[6]);
+        LEX_LAUNCH(KID_SMAG_NM, k_smag_nm, G(d.nz, d.ny, d.nx), bk, 0, st, d, src[0], src[5], sm[7]);
+        if (o.turb == 1) {
+          LEX_LAUNCH(KID_SMAG_K, k_smag_k, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, sm[6], sm[7], sm[8],
+                     sm[9], cg * cg, o.prandtl);
+        } else {  // sm[8] = Kmh = Khh, sm[9] = Kmv = Khv
+          LEX_LAUNCH(KID_SMAG_K, k_smag_k2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, rc, sm[0], sm[1],
+                     sm[2], sm[3], sm[4], sm[5], sm[7], sm[8], sm[9], o.cs * o.cs);
+        }
+        // coefficients: turb 1 Km (sm[8]) for every stress, Kh (sm[9]) for every scalar
+        // face; turb 2 Kmh / Kmv (sm[8] / sm[9]) for both
+        const float* kmh = sm[8];
+        const float* kmv = (o.turb == 1) ? sm[8] : sm[9];
+        khh = (o.turb == 1) ? sm[9] : sm[8];
+        khv = sm[9];
+        LEX_LAUNCH(KID_SMAG_UVW, k_smag_uvw<false>, G(d.nz + 1, d.ny + 1, d.nx + 1), bk, 0, st, d, rc, kmh, kmv, sm[0],
+                   sm[1], sm[2], sm[3], sm[4], sm[5], su, sv, sw);
         LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, khh, khv, src[0], ss[0]);
         LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, khh, khv, src[5], ss[1]);
         for (int q = 0; q < o.n_passive; ++q)
