@@ -7,6 +7,8 @@ every LEX grid (setup_lex.py:12-24).  The base state is horizontally uniform
 """
 from __future__ import annotations
 
+import weakref
+
 import numpy as np
 import torch
 
@@ -42,6 +44,10 @@ def grid_from(shape_src, x=None, y=None, z=None, dx=None, dy=None, dz=None) -> _
     return _lib.grid(nx, ny, nz, dx, dy, dz)
 
 
+# id(tensor) -> (weakref to the tensor, its _version when checked, storage pointer, column).
+# A hit needs the weakref to resolve to the very same live tensor object: a new tensor that
+# the caching allocator places at a freed tensor's address (and that starts again at
+# _version 0) never matches, so a freed base state's column is never handed out for a new one.
 _COL_CACHE: dict = {}
 
 
@@ -49,17 +55,23 @@ def column(t: torch.Tensor) -> torch.Tensor:
     """z-profile (nz+2,) of a horizontally uniform padded scalar field (checked once per tensor version)."""
     if t.dim() == 1:
         return t.contiguous()
-    key = (t.data_ptr(), tuple(t.shape), t._version)
+    key = id(t)
     hit = _COL_CACHE.get(key)
     if hit is not None:
-        return hit
+        ref, ver, ptr, col = hit
+        if ref() is t and ver == t._version and ptr == t.data_ptr():
+            return col
+        del _COL_CACHE[key]
     col = t[NGX, NGY, :].contiguous()
     if not bool(torch.equal(t, col.expand_as(t))):
         raise ValueError("base-state field is not horizontally uniform (the reference assumes it is: "
                          "pressure_gradient_coriolis.py:12)")
     if len(_COL_CACHE) > 64:
-        _COL_CACHE.clear()
-    _COL_CACHE[key] = col
+        for k in [k for k, v in _COL_CACHE.items() if v[0]() is None]:
+            del _COL_CACHE[k]
+        if len(_COL_CACHE) > 64:
+            _COL_CACHE.clear()
+    _COL_CACHE[key] = (weakref.ref(t), t._version, t.data_ptr(), col)
     return col
 
 
