@@ -96,6 +96,18 @@ int lex_p_matvec(const lex_grid_t* g, const float* rtt, const float* x, float* o
 size_t lex_bicgstab_workspace_bytes(const lex_grid_t* g);
 int lex_bicgstab(const lex_grid_t* g, const float* rtt, const float* rhs, const float* x0, float* x, double tol,
                  double atol, int maxiter, void* workspace, size_t workspace_bytes, int* k_out, void* stream);
+/* lex_bicgstab without a workspace argument and without any device allocation: no
+ * counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * loop state, partial sums and Krylov vectors live in a static device array of the
+ * library (one per GPU, about 32 MiB, held until the process ends), so grids are limited
+ * to nx*ny*nz <= LEX_BICGSTAB_NOALLOC_MAX_CELLS (larger ones return -3).  Results equal
+ * lex_bicgstab's bit for bit (same kernels, same launch shape).  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (two streams, two host threads)
+ * race and must be serialised by the caller. */
+#define LEX_BICGSTAB_NOALLOC_MAX_CELLS 1048576
+int lex_bicgstab_noalloc(const lex_grid_t* g, const float* rtt, const float* rhs, const float* x0, float* x,
+                         double tol, double atol, int maxiter, int* k_out, void* stream);
 
 /* ---- S1/M1/E1: acoustic substep and Euler updates --------------------------------- */
 /* one pass of the loop in sound_rk_step0/1/2 (one_step_integration.py:528-553):

@@ -83,6 +83,7 @@ _SIGS = {
     "lex_p_matvec": [_G, _P, _P, _P, _P],
     "lex_bicgstab_workspace_bytes": [_G],
     "lex_bicgstab": [_G, _P, _P, _P, _P, _D, _D, _I, _P, C.c_size_t, C.POINTER(_I), _P],
+    "lex_bicgstab_noalloc": [_G, _P, _P, _P, _P, _D, _D, _I, C.POINTER(_I), _P],
     "lex_acoustic_substep": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P],
     "lex_acoustic_substep_pair": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P],
     "lex_dl_pack": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P],

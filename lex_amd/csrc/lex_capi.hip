@@ -6,6 +6,7 @@
 #include <string.h>
 #include <string>
 #include <cmath>
+#include <mutex>
 
 #include "lex_common.hpp"
 #include "lex_ops.hip"
@@ -227,17 +228,26 @@ size_t lex_bicgstab_workspace_bytes(const lex_grid_t* g) {
   return 256 + (size_t)kBicgBlocks * 4 * sizeof(double) + 8 * vec;
 }
 
-int lex_bicgstab(const lex_grid_t* g, const float* rtt, const float* rhs, const float* x0, float* x, double tol,
-                 double atol, int maxiter, void* ws, size_t ws_bytes, int* k_out, void* stream) {
-  REQGRID(g);
-  REQ(rtt); REQ(rhs); REQ(x0); REQ(x); REQ(ws);
-  if (ws_bytes < lex_bicgstab_workspace_bytes(g)) return fail(-3, "lex_bicgstab: workspace too small%s");
-  if (maxiter < 0) return fail(-1, "lex_bicgstab: maxiter < 0%s");
-  if (x == x0) return fail(-1, "lex_bicgstab: x must not alias x0%s");
+}  // extern "C"
+
+// Scratch of lex_bicgstab_noalloc: one copy per GPU, held until the process ends.
+// Sized for LEX_BICGSTAB_NOALLOC_MAX_CELLS cells in the BiCGSTAB workspace layout
+// (loop state, partial sums, eight vectors).
+constexpr size_t kNoAllocVec = (size_t)LEX_BICGSTAB_NOALLOC_MAX_CELLS * sizeof(float);
+constexpr size_t kNoAllocBytes = 256 + (size_t)kBicgBlocks * 4 * sizeof(double) + 8 * kNoAllocVec;
+static_assert(kNoAllocVec % 256 == 0, "vectors stay 256-byte aligned");
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+
+namespace {
+// The BiCGSTAB iteration over a workspace laid out as: loop state (256 B), the
+// per-block FP64 partial sums, then eight 256-byte-aligned vectors of n floats
+// (r, r-hat, p x2, q x2, s, t).  Shared by the caller-workspace and the
+// allocation-free entry points.
+int bicgstab_run(const lex_grid_t* g, const float* rtt, const float* rhs, const float* x0, float* x, double tol,
+                 double atol, int maxiter, char* base, int* k_out, hipStream_t hs, const char* who) {
   Dims d = mkdims(g);
   long n = (long)g->nx * g->ny * g->nz;
   size_t vec = (((size_t)n * sizeof(float) + 255) / 256) * 256;
-  char* base = (char*)ws;
   BicgState* st = (BicgState*)base;
   double* partials = (double*)(base + 256);
   char* v0 = base + 256 + (size_t)kBicgBlocks * 4 * sizeof(double);
@@ -247,12 +257,12 @@ int lex_bicgstab(const lex_grid_t* g, const float* rtt, const float* rhs, const 
   float* q[2] = {(float*)(v0 + 4 * vec), (float*)(v0 + 5 * vec)};
   float* s = (float*)(v0 + 6 * vec);
   float* t = (float*)(v0 + 7 * vec);
-  hipStream_t hs = S(stream);
   int nb = (int)std::min<long>(kBicgBlocks, (n + kBicgThreads - 1) / kBicgThreads);
+  // the loop state (tickets, flags, counters) is cleared by a kernel, not a memset
   hipLaunchKernelGGL(k_bicg_reset, dim3(1), dim3(64), 0, hs, st, maxiter);
   hipLaunchKernelGGL(k_bicg_init, dim3(nb), dim3(kBicgThreads), 0, hs, d, rtt, rhs, x0, x, r, rhat, p[0], q[0],
                      partials, st, tol, atol);
-  int rc = check_launch("lex_bicgstab(init)");
+  int rc = check_launch(who);
   if (rc) return rc;
   int cur = 0;
   const int poll = 8;
@@ -262,7 +272,7 @@ int lex_bicgstab(const lex_grid_t* g, const float* rtt, const float* rhs, const 
     hipLaunchKernelGGL(k_bicg_st, dim3(nb), dim3(kBicgThreads), 0, hs, d, rtt, r, q[1 - cur], s, t, partials, st);
     hipLaunchKernelGGL(k_bicg_xr, dim3(nb), dim3(kBicgThreads), 0, hs, n, p[1 - cur], s, t, rhat, x, r, partials, st);
     cur = 1 - cur;
-    rc = check_launch("lex_bicgstab(iter)");
+    rc = check_launch(who);
     if (rc) return rc;
     if ((it % poll) == poll - 1) {
       int done = 0;
@@ -280,6 +290,59 @@ int lex_bicgstab(const lex_grid_t* g, const float* rtt, const float* rhs, const 
     *k_out = kk;
   }
   return 0;
+}
+
+// Device address of `scratch` per device, looked up once per device (with that device
+// current) on the first call there.  In host code `scratch` is only a host-side stand-in
+// for the symbol; kernels and copies get the looked-up address.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(char** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail((int)e, "lex_bicgstab_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(-1, "lex_bicgstab_noalloc: device index out of range%s");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(e != hipSuccess ? (int)e : -4, "lex_bicgstab_noalloc: hipGetSymbolAddress(scratch): %s",
+                  e != hipSuccess ? hipGetErrorString(e) : "null address");
+    g_scratch_ptr[dev] = (char*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int lex_bicgstab(const lex_grid_t* g, const float* rtt, const float* rhs, const float* x0, float* x, double tol,
+                 double atol, int maxiter, void* ws, size_t ws_bytes, int* k_out, void* stream) {
+  REQGRID(g);
+  REQ(rtt); REQ(rhs); REQ(x0); REQ(x); REQ(ws);
+  if (ws_bytes < lex_bicgstab_workspace_bytes(g)) return fail(-3, "lex_bicgstab: workspace too small%s");
+  if (maxiter < 0) return fail(-1, "lex_bicgstab: maxiter < 0%s");
+  if (x == x0) return fail(-1, "lex_bicgstab: x must not alias x0%s");
+  return bicgstab_run(g, rtt, rhs, x0, x, tol, atol, maxiter, (char*)ws, k_out, S(stream), "lex_bicgstab");
+}
+
+int lex_bicgstab_noalloc(const lex_grid_t* g, const float* rtt, const float* rhs, const float* x0, float* x,
+                         double tol, double atol, int maxiter, int* k_out, void* stream) {
+  REQGRID(g);
+  REQ(rtt); REQ(rhs); REQ(x0); REQ(x);
+  if (maxiter < 0) return fail(-1, "lex_bicgstab_noalloc: maxiter < 0%s");
+  if (x == x0) return fail(-1, "lex_bicgstab_noalloc: x must not alias x0%s");
+  if ((long long)g->nx * g->ny * g->nz > (long long)LEX_BICGSTAB_NOALLOC_MAX_CELLS)
+    return fail(-3, "lex_bicgstab_noalloc: nx*ny*nz exceeds LEX_BICGSTAB_NOALLOC_MAX_CELLS (the static "
+                    "scratch's capacity)%s");
+  char* base = nullptr;
+  int rc = scratch_for_current_device(&base);
+  if (rc) return rc;
+  return bicgstab_run(g, rtt, rhs, x0, x, tol, atol, maxiter, base, k_out, S(stream), "lex_bicgstab_noalloc");
 }
 
 // ------------------------------- S1/M1/E1 ----------------------------------------
