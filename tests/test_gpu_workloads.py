@@ -1,0 +1,312 @@
+"""Parity on the benchmarked workloads themselves, through the C ABI.
+
+- The 256x256x128 compressible recipe as the bench runs it (n_sound 7, four scalars
+  with nonzero qc and qr) on a 64x64x128 sub-grid and a ragged 37x23x70 grid, ten
+  steps, through the fused stage kernels (impl 0) and the double-buffered scalar kernel
+  (impl 2), against the FP64 and FP32 oracle.
+- The pseudo-incompressible recipe at its full 64x64x40 for several steps, with every
+  BiCGSTAB solve's exit code compared; and a consistent pressure system on which
+  BiCGSTAB converges, with the iteration count compared.
+- Two cases of one grid shape with different base states in one process.
+- The one-pass Smagorinsky kernel of the fused paths against the routine-by-routine
+  chain: the frozen SGS tendencies must be bit-identical.
+
+Fields are compared as perturbations from the initial state (theta', qv', qc', qr',
+u, v, w, pi' with its mean removed for the pseudo-incompressible core).  Tolerance: the
+GPU's rel-L2 error against the FP64 oracle within a small multiple of the FP32 oracle's
+own drift from FP64 (the reference's intrinsic FP32 error).
+"""
+from __future__ import annotations
+
+import functools
+
+import numpy as np
+import pytest
+
+from oracle import lex_oracle as o
+from tests._helpers import from_dev, rel_l2, to_dev
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ["theta", "u", "v", "w", "pip", "qv", "qc", "qr"]
+
+
+def _seed_passive(phys, nx, ny, nz, seed):
+    """qc, qr: nonnegative fields with a smooth blob, sharp edges and noise, so WENO sees
+    smooth data, jumps and zero regions; periodic halos as padding3_array writes them."""
+    import torch
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    z = (np.arange(nz) + 0.5) / nz
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    r = np.sqrt(((X - 0.5) / 0.3) ** 2 + ((Y - 0.5) / 0.3) ** 2 + ((Z - 0.4) / 0.25) ** 2)
+    out = list(phys)
+    for q, amp in ((6, 1e-3), (7, 4e-4)):
+        blob = amp * np.where(r < 1.0, np.cos(0.5 * np.pi * r) ** 2, 0.0)
+        step = np.where((X > 0.2) & (X < 0.35) & (Z < 0.6), 0.5 * amp, 0.0)
+        noise = np.abs(rng.normal(0.0, 0.05 * amp, (nx, ny, nz)))
+        a = o.padding3_array(blob + step + noise)
+        out[q] = torch.from_numpy(a.astype(np.float32)).to(phys[0].device)
+    return tuple(out)
+
+
+def _case1(nx, ny, nz, seed=5):
+    from lex_amd import namelist_n_constants as nl
+    from lex_amd.setup_lex import build_case, config1
+    case, opts = config1()
+    case = case.scaled(nx, ny, nz)
+    phys, base, grids, _ = build_case(case)
+    phys = _seed_passive(phys, nx, ny, nz, seed)
+    nl.dt, nl.n_sound, nl.solver_opt, nl.turb_opt = opts["dt"], 7, 2, 1
+    og = o.Grid(nx, ny, nz, case.dx, case.dy, case.dz)
+    oo = o.Options(dt=opts["dt"], solver_opt=2, n_sound=7, turb_opt=1)
+    return phys, base, grids, og, oo
+
+
+def _oracle_run(init, base, og, oo, nsteps, dtype):
+    xi = tuple(a.astype(dtype) for a in init)
+    bs = tuple(b.astype(dtype) for b in base) + (None,)
+    for _ in range(nsteps):
+        xi, _ = o.ws_rk3_big_step(xi, bs, og, oo)
+    return xi
+
+
+@functools.lru_cache(maxsize=4)
+def _config1_reference(shape, nsteps):
+    """Initial state and FP64 / FP32 oracle runs of the n_sound 7 recipe (cached per shape)."""
+    phys, base, grids, og, oo = _case1(*shape)
+    init = tuple(from_dev(a) for a in phys)
+    b = tuple(from_dev(a) for a in base[:5])
+    r64 = _oracle_run(init, b, og, oo, nsteps, np.float64)
+    r32 = _oracle_run(init, b, og, oo, nsteps, np.float32)
+    return init, r64, r32
+
+
+def _errors(got, r64, r32, init, demean_pip=False):
+    out = {}
+    for n, a, b, c, a0 in zip(NAMES, got, r64, r32, init):
+        da, db, dc = a - a0, np.asarray(b, np.float64) - a0, np.asarray(c, np.float64) - a0
+        if n == "pip" and demean_pip:
+            da, db, dc = da - da.mean(), db - db.mean(), dc - dc.mean()
+        out[n] = (rel_l2(da, db), rel_l2(dc, db), float(np.linalg.norm(db)))
+    return out
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("impl", [0, 2])
+@pytest.mark.parametrize("shape", [(64, 64, 128), (37, 23, 70)], ids=lambda s: "x".join(map(str, s)))
+def test_config1_nsound7_moist_passive_vs_oracle(shape, impl):
+    """The benchmarked step (n_sound 7, theta, qv and nonzero qc, qr) for 10 steps."""
+    from lex_amd import one_sprint_functions as sp
+    from lex_amd._core import grid_from
+    nsteps = 10
+    print(f"config-1 recipe n_sound 7 {shape} impl {impl}: {nsteps} steps", flush=True)
+    init, r64, r32 = _config1_reference(shape, nsteps)
+    phys, base, grids, og, oo = _case1(*shape)
+    g = grid_from(base[1], *grids[3:6])
+    st = tuple(t.clone() for t in phys)
+    mo = (2, False, False, 0, False, False, 1)
+    for _ in range(nsteps):
+        sp.fused_step(st, base, g, mo, impl=impl)
+    got = tuple(from_dev(a) for a in st)
+    errs = _errors(got, r64, r32, init)
+    print({k: f"{v[0]:.2e}/{v[1]:.2e}" for k, v in errs.items()})
+    for n in NAMES:
+        e_g, e_32, norm = errs[n]
+        assert norm > 0.0, f"{n} did not change: the comparison would be vacuous"
+        assert e_g <= max(3.0 * e_32, 1e-6), f"{n}: GPU {e_g:.3e} vs FP32 oracle {e_32:.3e}"
+    # the passive scalars were really advected and mixed, not just carried
+    for q in (6, 7):
+        assert rel_l2(got[q], init[q]) > 1e-4
+
+
+@pytest.mark.timeout(600)
+def test_config0_full_size_steps_vs_oracle(monkeypatch):
+    """The pseudo-incompressible recipe at its full 64x64x40 for 4 steps (12 BiCGSTAB
+    solves): fields vs the FP64/FP32 oracle and every solve's exit code vs the oracle's."""
+    from lex_amd import namelist_n_constants as nl, one_sprint_functions as sp
+    from lex_amd import one_step_integration as one
+    from lex_amd.setup_lex import build_case, config0
+    case, opts = config0()
+    phys, base, grids, _ = build_case(case)
+    nl.dt, nl.n_sound, nl.solver_opt, nl.turb_opt = opts["dt"], opts["n_sound"], 1, 1
+    mo = (1, False, False, 0, False, False, 1)
+    og = o.Grid(case.nx, case.ny, case.nz, case.dx, case.dy, case.dz)
+    oo = o.Options(dt=opts["dt"], solver_opt=1, n_sound=opts["n_sound"], turb_opt=1)
+    assert (case.nx, case.ny, case.nz) == (64, 64, 40)
+    nsteps = 4
+    k_gpu, k_o = [], []
+    solve = one.solve_pres_eqn
+
+    def rec(*a, **k):
+        x, info = solve(*a, **k)
+        k_gpu.append(info)
+        return x, info
+    monkeypatch.setattr(one, "solve_pres_eqn", rec)
+    print(f"config-0 recipe 64x64x40: {nsteps} steps", flush=True)
+    init = tuple(from_dev(a) for a in phys)
+    st = phys
+    for _ in range(nsteps):
+        st, _ = sp.ws_rk3_big_step(st, base, grids, mo)
+    got = tuple(from_dev(a) for a in st)
+    b = tuple(from_dev(a) for a in base[:5])
+    bic = o.bicgstab
+
+    def orec(*a, **k):
+        x, kk = bic(*a, **k)
+        k_o.append(kk)
+        return x, kk
+    monkeypatch.setattr(o, "bicgstab", orec)
+    r64 = _oracle_run(init, b, og, oo, nsteps, np.float64)
+    k64 = list(k_o)
+    r32 = _oracle_run(init, b, og, oo, nsteps, np.float32)
+    k32 = k_o[len(k64):]
+    errs = _errors(got, r64, r32, init, demean_pip=True)
+    print({k: f"{v[0]:.2e}/{v[1]:.2e}" for k, v in errs.items()}, "iters", k_gpu, k64, k32)
+    assert len(k_gpu) == len(k64) == 3 * nsteps
+    # the affine pressure system is not consistent to the solver's tolerance, so every
+    # solve runs to maxiter in both precisions of the oracle and on the GPU
+    assert k_gpu == k64 == k32
+    for n in NAMES[:6]:
+        e_g, e_32, norm = errs[n]
+        if n == "qv":
+            assert norm == 0.0 and np.max(np.abs(got[5] - init[5])) == 0.0   # dry recipe
+            continue
+        assert e_g <= max(3.0 * e_32, 1e-6), f"{n}: GPU {e_g:.3e} vs FP32 oracle {e_32:.3e}"
+
+
+@pytest.mark.parametrize("seed", [4, 5, 6])
+def test_bicgstab_converging_iteration_count(seed):
+    """Consistent pressure systems on which BiCGSTAB converges before maxiter: the
+    operator of a rest state (zero buoyancy, so rtt's ghost planes and the affine part
+    vanish) and rhs = A(x_true) of a random x_true.  Near the threshold the residual falls
+    only ~1.5x per iteration, so the FP64 and FP32 oracles themselves stop 0-2 iterations
+    apart; the GPU's exit code must be a converged count within 2 of the FP64 oracle's,
+    and its true residual must meet the stopping test to within that spread."""
+    import ctypes
+    import torch
+    from lex_amd import _lib
+    from tests._helpers import random_state
+    gr, phys, base = random_state(24, 20, 16, dx=600.0, dy=600.0, dz=300.0, seed=seed)
+    b32 = tuple(np.asarray(a, np.float32) for a in base[:5])
+    th, qv = b32[2], b32[4]
+    bz = np.asarray(o.calculate_buoyancy(b32[2], th, b32[4], qv)[0], np.float32)
+    assert not bz.any()
+    rtt = np.asarray(o.calculate_rtt(b32[0], th, qv, bz), np.float32)
+    A64 = lambda x: o.laplace_of_pressure(rtt.astype(np.float64), x, gr)  # noqa: E731
+    assert not A64(np.zeros((gr.nx, gr.ny, gr.nz))).any(), "the operator must be linear here"
+    rng = np.random.default_rng(seed + 10)
+    x_true = rng.normal(0, 1e-3, (gr.nx, gr.ny, gr.nz))
+    rhs = np.asarray(A64(x_true), np.float32)
+    x0 = np.zeros_like(rhs)
+    tol = 1e-4
+    _, k64 = o.bicgstab(A64, rhs.astype(np.float64), x0.astype(np.float64), tol, 1e-8, 100)
+    _, k32 = o.bicgstab(lambda x: o.laplace_of_pressure(rtt, x, gr), rhs, x0, tol, 1e-8, 100)
+    g = _lib.grid(gr.nx, gr.ny, gr.nz, gr.dx, gr.dy, gr.dz)
+    nbytes = _lib.load().lex_bicgstab_workspace_bytes(g)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    x = torch.empty(gr.nx, gr.ny, gr.nz, device="cuda")
+    k = ctypes.c_int(-99)
+    rt, rh, xz = to_dev(rtt), to_dev(rhs), to_dev(x0)
+    _lib.call("lex_bicgstab", g, _lib.ptr(rt), _lib.ptr(rh), _lib.ptr(xz), _lib.ptr(x), tol, 1e-8, 100,
+              _lib.ptr(ws), nbytes, ctypes.byref(k), _lib.stream())
+    torch.cuda.synchronize()
+    b = rhs.astype(np.float64)
+    r = b - A64(from_dev(x))
+    ratio = float(np.vdot(r, r) / (tol * tol * np.vdot(b, b)))
+    print(f"seed {seed}: iterations GPU {k.value}, oracle FP64 {k64}, FP32 {k32}; "
+          f"GPU |r|^2 / atol^2 = {ratio:.3f}")
+    assert 0 < k64 < 100 and 0 < k32 < 100, "the system must converge before maxiter"
+    assert 0 < k.value < 100
+    assert abs(k.value - k64) <= 2, (k.value, k64, k32)
+    assert ratio <= 2.0, ratio
+
+
+def test_two_cases_same_shape_different_base():
+    """A moist case, then a dry case of the same grid in one process: each step is
+    checked against the oracle on its own base state (base-state profiles must never be
+    taken from the earlier case)."""
+    import gc
+    from lex_amd import namelist_n_constants as nl, one_sprint_functions as sp
+    from lex_amd.setup_lex import build_case, config1
+    shape = (24, 20, 24)
+    for moist in (True, False, True):
+        case, opts = config1()
+        case = case.scaled(*shape)
+        if not moist:
+            case.rh_surface = case.rh_top = case.rh_bubble = 0.0
+        phys, base, grids, _ = build_case(case)
+        nl.dt, nl.n_sound, nl.solver_opt, nl.turb_opt = opts["dt"], 7, 2, 1
+        mo = (2, False, False, 0, False, False, 1)
+        og = o.Grid(*shape, case.dx, case.dy, case.dz)
+        oo = o.Options(dt=opts["dt"], solver_opt=2, n_sound=7, turb_opt=1)
+        init = tuple(from_dev(a) for a in phys)
+        out, _ = sp.ws_rk3_big_step(phys, base, grids, mo)
+        got = tuple(from_dev(a) for a in out)
+        b = tuple(from_dev(a) for a in base[:5])
+        r64 = _oracle_run(init, b, og, oo, 1, np.float64)
+        r32 = _oracle_run(init, b, og, oo, 1, np.float32)
+        errs = _errors(got, r64, r32, init)
+        for n in NAMES[:6]:
+            e_g, e_32, norm = errs[n]
+            if norm == 0.0:
+                continue
+            assert e_g <= max(3.0 * e_32, 1e-6), f"moist={moist} {n}: GPU {e_g:.3e} vs FP32 oracle {e_32:.3e}"
+        del phys, base, grids, out
+        gc.collect()
+
+
+def _al256(b):
+    return (b + 255) & ~255
+
+
+def _sgs_views(ws, nx, ny, nz, np_):
+    """The frozen SGS tendencies inside lex_rk3_step's workspace (lex_stage.hip rk3_layout):
+    su (nx+1, ny, nz), sv (nx, ny+1, nz), sw (nx, ny, nz+1), then 2 + n_passive scalars."""
+    nc = (nx + 6) * (ny + 6) * (nz + 2)
+    nu = (nx + 7) * (ny + 6) * (nz + 2)
+    nv = (nx + 6) * (ny + 7) * (nz + 2)
+    nw = (nx + 6) * (ny + 6) * (nz + 3)
+    iu, iv, iw, icc = (nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1), nx * ny * nz
+    state = _al256(nu * 4) + _al256(nv * 4) + _al256(nw * 4) + (3 + np_) * _al256(nc * 4)
+    acou = _al256(nu * 4) + _al256(nv * 4) + _al256(nw * 4) + _al256(nc * 4)
+    off = 2 * state + 2 * acou + 2 * _al256(nc * 4) + _al256(iu * 4) + _al256(iv * 4) + _al256(iw * 4)
+    import torch
+    views = []
+    for n in (iu, iv, iw):
+        views.append(ws[off:off + 4 * n].view(torch.float32))
+        off += _al256(n * 4)
+    for _ in range(2 + np_):
+        views.append(ws[off:off + 4 * icc].view(torch.float32))
+        off += _al256(icc * 4)
+    return views
+
+
+@pytest.mark.parametrize("turb", [1, 2])
+@pytest.mark.parametrize("shape", [(24, 20, 24), (37, 23, 70)], ids=lambda s: "x".join(map(str, s)))
+def test_fused_smagorinsky_bit_equal_to_chain(shape, turb):
+    """k_smag_tau + k_smag_uvw + k_smag_scalars (impl 0) and the routine-by-routine chain
+    (impl 1) freeze the same SGS tendencies bit for bit, for every momentum component and
+    every scalar (theta, qv, qc, qr), with turb_opt 1 and 2."""
+    import torch
+    from lex_amd import _lib, namelist_n_constants as nl, one_sprint_functions as sp
+    from lex_amd._core import grid_from
+    phys, base, grids, og, oo = _case1(*shape)
+    nl.turb_opt = turb
+    g = grid_from(base[1], *grids[3:6])
+    mo = (2, False, False, 0, False, False, turb)
+    np_ = len(phys) - 6
+    frozen = {}
+    for impl in (0, 1):
+        st = tuple(t.clone() for t in phys)
+        sp.fused_step(st, base, g, mo, impl=impl)
+        torch.cuda.synchronize()
+        ws = sp._WS[(g.nx, g.ny, g.nz, np_, str(st[0].device))]
+        assert ws.numel() == _lib.load().lex_rk3_workspace_bytes(g, np_)
+        frozen[impl] = [v.clone() for v in _sgs_views(ws, *shape, np_)]
+    names = ["su", "sv", "sw", "theta", "qv", "qc", "qr"]
+    for n, a, b in zip(names, frozen[0], frozen[1]):
+        assert float(b.abs().max()) > 0.0, f"{n}: SGS tendency is zero, the comparison would be vacuous"
+        assert torch.equal(a, b), (n, float((a - b).abs().max()))
+    nl.turb_opt = 1
