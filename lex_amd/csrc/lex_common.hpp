@@ -104,19 +104,32 @@ __device__ __forceinline__ float hrecon(float qm3, float qm2, float qm1, float q
 //   upward:   f=1 centred, f=2 WENO3, f=3..L-4 WENO5, f=L-3 WENO3
 //   downward: f=1 WENO3, f=2..L-5 WENO5, f=L-4 WENO3, f=L-3 centred
 // Faces 0 and L-2 carry zero (boundary-condition slots).  `p` has unit stride.
+//
+// Branch-free in the upwind side: the lanes of a wave run along z, and the sign of w
+// changes from level to level, so an if/else on the sign would make every wave execute
+// both WENO5 branches.  Instead the upwind-ordered stencil is selected from the six
+// values p[f-2..f+3] (index clamped so the loads stay in the column) and one WENO5 is
+// evaluated; the table's near-boundary rows (WENO3, centred, zero) are patched in a
+// branch that only waves holding a face with f <= 2 or f >= L-4 enter.  Every face gets
+// exactly the operands of the table above, so the result is the table's.  Needs L >= 8.
 __device__ __forceinline__ float vrecon(const float* __restrict__ p, int L, int f, bool pos, float weps) {
-  if (f <= 0 || f >= L - 2) return 0.0f;
-  if (pos) {
-    if (f == 1) return 0.5f * (p[1] + p[2]);
-    if (f == 2) return weno3(p[1], p[2], p[3], weps);
-    if (f == L - 3) return weno3(p[L - 4], p[L - 3], p[L - 2], weps);
-    return weno5(p[f - 2], p[f - 1], p[f], p[f + 1], p[f + 2], weps);
-  } else {
-    if (f == 1) return weno3(p[3], p[2], p[1], weps);
-    if (f == L - 3) return 0.5f * (p[L - 2] + p[L - 3]);
-    if (f == L - 4) return weno3(p[L - 2], p[L - 3], p[L - 4], weps);
-    return weno5(p[f + 3], p[f + 2], p[f + 1], p[f], p[f - 1], weps);
+  const int fc = min(max(f, 2), L - 4);
+  const float q0 = p[fc - 2], q1 = p[fc - 1], q2 = p[fc], q3 = p[fc + 1], q4 = p[fc + 2], q5 = p[fc + 3];
+  float r = weno5(pos ? q0 : q5, pos ? q1 : q4, pos ? q2 : q3, pos ? q3 : q2, pos ? q4 : q1, weps);
+  if (f <= 2 || f >= L - 4) {
+    // 1 <= f <= L-3 here unless the face is a zero slot: p[f-1..f+2] are in the column
+    const int fe = min(max(f, 1), L - 3);
+    const float a = p[fe - 1], b = p[fe], c = p[fe + 1], e = p[fe + 2];
+    // upward rows f = 2 and L-3 take WENO3 of (p[f-1], p[f], p[f+1]); downward rows
+    // f = 1 and L-4 take WENO3 of (p[f+2], p[f+1], p[f]); the centred rows (upward f = 1,
+    // downward f = L-3) take 0.5 (p[f] + p[f+1])
+    const float r3 = weno3(pos ? a : e, pos ? b : c, pos ? c : b, weps);
+    const bool w3 = pos ? (f == 2 || f == L - 3) : (f == 1 || f == L - 4);
+    const bool cen = pos ? (f == 1) : (f == L - 3);
+    r = w3 ? r3 : (cen ? 0.5f * (b + c) : r);
+    if (f <= 0 || f >= L - 2) r = 0.0f;
   }
+  return r;
 }
 
 // select(mass >= 0, mass*up, mass*dn) with only the upwind side evaluated.
