@@ -42,7 +42,11 @@ int check_launch(const char* what) {
 }
 
 bool grid_ok(const lex_grid_t* g) {
-  return g && g->nx >= 6 && g->ny >= 6 && g->nz >= 6 && g->dx > 0.f && g->dy > 0.f && g->dz > 0.f;
+  if (!(g && g->nx >= 6 && g->ny >= 6 && g->nz >= 6 && g->dx > 0.f && g->dy > 0.f && g->dz > 0.f)) return false;
+  // the kernels address fields with 32-bit element offsets: the largest padded field
+  // (u, v or w) must stay below 2^30 elements
+  const long px = g->nx + 7L, py = g->ny + 7L, pz = g->nz + 3L;
+  return px * py * pz < kMaxOffset32;
 }
 
 Dims mkdims(const lex_grid_t* g) {
@@ -68,7 +72,8 @@ inline dim3 grd(int nkz, int nj, int ni) { return dim3((nkz + 63) / 64, (nj + 3)
 #define REQ(ptr) \
   if (!(ptr)) return fail(-2, "null pointer argument: %s", #ptr)
 #define REQGRID(g) \
-  if (!grid_ok(g)) return fail(-1, "invalid grid (need nx,ny,nz >= 6 and positive spacings)%s")
+  if (!grid_ok(g)) return fail(-1, "invalid grid (need nx,ny,nz >= 6, positive spacings and fewer than 2^30 "   \
+                                  "elements per padded field)%s")
 }  // namespace
 
 extern "C" {

@@ -46,13 +46,29 @@ __device__ __forceinline__ long iu(const Dims& d, int I, int J, int K) { return 
 __device__ __forceinline__ long iv(const Dims& d, int I, int J, int K) { return I * d.sxv() + (long)J * d.syc() + K; }
 __device__ __forceinline__ long iw(const Dims& d, int I, int J, int K) { return I * d.sxw() + (long)J * d.syw() + K; }
 
+// Access at a 32-bit element offset.  The address is the (wave-uniform) base pointer
+// plus a zero-extended 32-bit byte offset, which the compiler emits as
+// `global_load/store ... v_off, s[base]` with no per-lane 64-bit address arithmetic.
+// Callers keep offsets below 2^30 elements (every field is checked at the C ABI).
+__device__ __forceinline__ float ld32(const float* __restrict__ base, uint32_t e) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (size_t)(e * 4u));
+}
+__device__ __forceinline__ void st32(float* __restrict__ base, uint32_t e, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (size_t)(e * 4u)) = v;
+}
+constexpr long kMaxOffset32 = 1L << 30;   // elements addressable by ld32 / st32
+
 // Fast reciprocal (v_rcp_f32, 1 ulp).  Used for the WENO weight ratios and the
 // normalisation; results stay within the FP32-vs-FP64 tolerance of the reference.
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // WENO5 face value from the upwind-ordered stencil (a, b, c | d, e), c upwind.
 // Jiang & Shu (1996) candidates/indicators, Borges et al. (2008) weights with the
-// squared ratio and the 1e30 cap, as advection.py:311-326.
+// squared ratio, as advection.py:311-326.  The reference caps the ratio at 1e30
+// (fmin) before squaring; in FP32 that cap cannot change a result: any ratio above
+// ~1.8e19 squares to +inf with or without it, and a ratio below 1e30 passes unchanged
+// (tau and beta + eps are finite and beta + eps > 0, so the ratio is never NaN for
+// finite data).  The FP32 kernels therefore omit it.
 __device__ __forceinline__ float weno5(float a, float b, float c, float d, float e, float weps) {
   const float c1312 = (float)(13.0 / 12.0);
   float t1 = a - 2.0f * b + c, t2 = a - 4.0f * b + 3.0f * c;
@@ -62,9 +78,9 @@ __device__ __forceinline__ float weno5(float a, float b, float c, float d, float
   float t5 = c - 2.0f * d + e, t6 = 3.0f * c - 4.0f * d + e;
   float b3 = c1312 * t5 * t5 + 0.25f * t6 * t6;
   float tau = fabsf(b1 - b3);
-  float r1 = fminf(1.0e30f, tau * frcp(b1 + weps));
-  float r2 = fminf(1.0e30f, tau * frcp(b2 + weps));
-  float r3 = fminf(1.0e30f, tau * frcp(b3 + weps));
+  float r1 = tau * frcp(b1 + weps);
+  float r2 = tau * frcp(b2 + weps);
+  float r3 = tau * frcp(b3 + weps);
   float w1 = 0.1f * (1.0f + r1 * r1);
   float w2 = 0.6f * (1.0f + r2 * r2);
   float w3 = 0.3f * (1.0f + r3 * r3);
@@ -75,13 +91,14 @@ __device__ __forceinline__ float weno5(float a, float b, float c, float d, float
   return (w1 * q1 + w2 * q2 + w3 * q3) * frcp(w1 + w2 + w3);
 }
 
-// WENO3 face value from (a, b | c), b upwind (advection.py:329-340).
+// WENO3 face value from (a, b | c), b upwind (advection.py:329-340); the 1e30 cap is
+// omitted for the reason given at weno5.
 __device__ __forceinline__ float weno3(float a, float b, float c, float weps) {
   float b1 = (a - b) * (a - b);
   float b2 = (b - c) * (b - c);
   float tau = fabsf(b1 - b2);
-  float r1 = fminf(1.0e30f, tau * frcp(b1 + weps));
-  float r2 = fminf(1.0e30f, tau * frcp(b2 + weps));
+  float r1 = tau * frcp(b1 + weps);
+  float r2 = tau * frcp(b2 + weps);
   float w1 = (float)(1.0 / 3.0) * (1.0f + r1 * r1);
   float w2 = (float)(2.0 / 3.0) * (1.0f + r2 * r2);
   return (w1 * (-0.5f * a + 1.5f * b) + w2 * (0.5f * b + 0.5f * c)) * frcp(w1 + w2);
@@ -130,6 +147,18 @@ __device__ __forceinline__ float vrecon(const float* __restrict__ p, int L, int 
     if (f <= 0 || f >= L - 2) r = 0.0f;
   }
   return r;
+}
+
+// z level of lane tz in a block of nzl = 64*ceil(nz/64) lanes along z.  When a column
+// spans two or more waves, wave 0 holds the 32 lowest and the 32 highest levels (the
+// rows of the vertical face table that need WENO3 / centred values live there) and the
+// later waves the levels in between, so only wave 0 enters vrecon's near-boundary
+// branch.  Lanes of later waves with no level left get a level >= nz (idle).
+__device__ __forceinline__ int lane_level(int tz, int nzl, int nz) {
+  if (nzl <= 64 || tz < 32) return tz;
+  if (tz < 64) return nz - 64 + tz;
+  const int k = tz - 32;
+  return k < nz - 32 ? k : nz + (tz - 64);
 }
 
 // select(mass >= 0, mass*up, mass*dn) with only the upwind side evaluated.
@@ -257,6 +286,17 @@ __device__ __forceinline__ void store_c_fast(const Dims& d, float* __restrict__ 
   a[b + k + 1] = val;
   if (k == 0) a[b] = val;
   if (k == d.nz - 1) a[b + d.nz + 1] = val;
+}
+// store_c_fast with the cell's 32-bit element offset oc = ic(d, i+3, j+3, k+1) known
+__device__ __forceinline__ void store_c_fast32(const Dims& d, float* __restrict__ a, int i, int j, int k, uint32_t oc,
+                                               float val) {
+  if (near_xy(d, i, j)) {
+    store_c(d, a, i, j, k, val);
+    return;
+  }
+  st32(a, oc, val);
+  if (k == 0) st32(a, oc - 1, val);
+  if (k == d.nz - 1) st32(a, oc + 1, val);
 }
 __device__ __forceinline__ void store_u_fast(const Dims& d, float* __restrict__ a, int f, int j, int k, float val) {
   if (f < 4 || f >= d.nx - 3 || j < 3 || j >= d.ny - 3) {

@@ -60,6 +60,11 @@ __device__ __forceinline__ void plane_fetch(PlaneRegs& r, const float* __restric
 #pragma unroll
   for (int q = 0; q < kSlots; ++q) r.v[q] = (q < ps.n) ? plane[ps.off[q]] : 0.0f;
 }
+__device__ __forceinline__ void plane_fetch32(PlaneRegs& r, const float* __restrict__ field, uint32_t plane_off,
+                                              const PlaneSlots& ps) {
+#pragma unroll
+  for (int q = 0; q < kSlots; ++q) r.v[q] = (q < ps.n) ? ld32(field, plane_off + (uint32_t)ps.off[q]) : 0.0f;
+}
 __device__ __forceinline__ void plane_store(float* __restrict__ P, const PlaneRegs& r, const PlaneSlots& ps) {
 #pragma unroll
   for (int q = 0; q < kSlots; ++q)
@@ -110,47 +115,55 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
   const TileId tb = xcd_tile();
   const int j0 = tb.y * kFTY;
   const int i0 = tb.x * kFTX;
-  const int k = tz, j = j0 + ty;
+  const int k = lane_level(tz, nzl, d.nz), j = j0 + ty;
   const bool act = (k < d.nz) && (j < d.ny);
   const int K = k + 1, J = j + 3;
   const int Kc = act ? K : 1, Jc = (j < d.ny) ? J : 3;   // clamped indices for idle lanes
   const bool last_row = (ty == kFTY - 1) || (j == d.ny - 1);
   const long sx = d.sxc(), sy = d.syc();
   const float r = rc[Kc];
+  const float ir = 1.0f / r;   // one division per thread instead of one per cell and field
   const float r8lo = (Kc <= 0 || Kc >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc - 1] + rc[Kc]);
   const float r8hi = (Kc + 1 >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc] + rc[Kc + 1]);
 
   const PlaneSlots psl = plane_slots(tid, nt, Lz, j0, d.ny + 5, (int)sy);
+  // 32-bit element offsets of this thread's cell at x = I, advanced by one x-plane per
+  // step of the march: oc indexes scalars and u (u has the scalar strides), ov v, ow w,
+  // oi the interior (nx, ny, nz) arrays, op the first row of the LDS plane's source.
+  const uint32_t usx = (uint32_t)sx, usy = (uint32_t)sy;
+  const uint32_t usxv = (uint32_t)d.sxv(), usxw = (uint32_t)d.sxw(), usyw = (uint32_t)d.syw();
+  const uint32_t usi = (uint32_t)d.ny * (uint32_t)d.nz;
+  uint32_t oc = (uint32_t)(i0 + 3) * usx + (uint32_t)Jc * usy + (uint32_t)Kc;
+  uint32_t ov = (uint32_t)(i0 + 3) * usxv + (uint32_t)Jc * usy + (uint32_t)Kc;
+  uint32_t ow = (uint32_t)(i0 + 3) * usxw + (uint32_t)Jc * usyw + (uint32_t)Kc;
+  uint32_t oi = (uint32_t)i0 * usi + (uint32_t)(Jc - 3) * (uint32_t)d.nz + (uint32_t)(Kc - 1);
+  uint32_t op = (uint32_t)(i0 + 3) * usx;
   // x windows q[I-2..I+3] per scalar and the lower x-face flux
   float win[NS][6];
   float fxlo[NS];
   const int iend = min(i0 + kFTX, d.nx);
   {
-    const int I = i0 + 3;
-    const float mlo = r * u[iu(d, I, Jc, Kc)];
+    const float mlo = r * ld32(u, oc);
     const bool pos = upwind_pos(mlo);
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
       const float* s = S.s[q];
-      long c = ic(d, I, Jc, Kc);
-      float m3 = s[c - 3 * sx];
+      float m3 = ld32(s, oc - 3 * usx);
 #pragma unroll
-      for (int t = 0; t < 6; ++t) win[q][t] = s[c + (t - 2) * sx];
+      for (int t = 0; t < 6; ++t) win[q][t] = ld32(s, oc + (uint32_t)(t - 2) * usx);
       fxlo[q] = mlo * hrecon(m3, win[q][0], win[q][1], win[q][2], win[q][3], win[q][4], pos, S.weps[q]);
     }
   }
-  float ulo = u[iu(d, i0 + 3, Jc, Kc)];
+  float ulo = ld32(u, oc);
   PlaneRegs pr;
-  plane_fetch(pr, S.s[0] + (long)(i0 + 3) * sx, psl);
+  plane_fetch32(pr, S.s[0], op, psl);
 
   for (int i = i0; i < iend; ++i) {
-    const int I = i + 3;
-    const long c = ic(d, I, Jc, Kc);
-    const float uhi = u[iu(d, I + 1, Jc, Kc)];
-    const float vlo = v[iv(d, I, Jc, Kc)];
-    const float vhi = v[iv(d, I, Jc + 1, Kc)];
-    const float wlo = w[iw(d, I, Jc, Kc)];
-    const float whi = w[iw(d, I, Jc, Kc + 1)];
+    const float uhi = ld32(u, oc + usx);
+    const float vlo = ld32(v, ov);
+    const float vhi = ld32(v, ov + usy);
+    const float wlo = ld32(w, ow);
+    const float whi = ld32(w, ow + 1);
     // flow divergence at the cell (get_divergence with uniform rho columns)
     const float fd = (r * uhi - r * ulo) * d.idx + (r * vhi - r * vlo) * d.idy + (r8hi * whi - r8lo * wlo) * d.idz;
     const float mx = r * uhi;                      // x-face I+1
@@ -161,13 +174,12 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
     float thc = 0.f, qvc = 0.f;
     // the update's xi0 values and frozen SGS tendencies, issued before the first barrier
     float s0v[NS], sgv[NS];
-    const long oint = ((long)i * d.ny + (Jc - 3)) * d.nz + (Kc - 1);
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
-      s0v[q] = S.s0[q][c];
-      sgv[q] = S.sgs[q] ? S.sgs[q][oint] : 0.0f;
+      s0v[q] = ld32(S.s0[q], oc);
+      sgv[q] = S.sgs[q] ? ld32(S.sgs[q], oi) : 0.0f;
     }
-    const float hv = S.heat ? S.heat[oint] : 0.0f;
+    const float hv = S.heat ? ld32(S.heat, oi) : 0.0f;
     const BottomFlux bf = bottom_fluxes(S, act && k == 0, (long)i * d.ny + j);
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
@@ -176,8 +188,8 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
       // plane of this scalar at x = I: rows J = j0 .. j0+kFTY+5 (padded), all levels,
       // fetched one plane ahead; then fetch the next (scalar q+1 here, or scalar 0 at I+1)
       plane_store(P, pr, psl);
-      if (q + 1 < NS) plane_fetch(pr, S.s[(q + 1) % NS] + (long)I * sx, psl);
-      else if (i + 1 < iend) plane_fetch(pr, S.s[0] + (long)(I + 1) * sx, psl);
+      if (q + 1 < NS) plane_fetch32(pr, S.s[(q + 1) % NS], op, psl);
+      else if (i + 1 < iend) plane_fetch32(pr, S.s[0], op + usx, psl);
       __syncthreads();
       const float* Pc = P + (ty + 3) * Lz;   // this row's column
       const float qc = Pc[Kc];
@@ -202,20 +214,25 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
         float fzhi = (k == d.nz - 1) ? 0.0f : FZ[ty * nzl + k + 1];
         float conv = -((fxhi - fxlo[q]) * d.idx + (FY[(ty + 1) * nzl + k] - FY[ty * nzl + k]) * d.idy +
                        (fzhi - FZ[ty * nzl + k]) * d.idz);
-        float tend = (conv + qc * fd) / r;
+        float tend = (conv + qc * fd) * ir;
         if (q == 0 && S.heat) tend = tend + hv;
         float nxt = s0v[q] + tend * dt;
         if (S.sgs[q]) nxt = nxt + sgv[q] * dt;
-        store_c_fast(d, S.out[q], i, j, k, nxt);
+        store_c_fast32(d, S.out[q], i, j, k, oc, nxt);
       }
       fxlo[q] = fxhi;
       // advance the window: q[I-1..I+4]
 #pragma unroll
       for (int t = 0; t < 5; ++t) win[q][t] = win[q][t + 1];
-      win[q][5] = (i + 1 < iend) ? s[c + 4 * sx] : 0.0f;
+      win[q][5] = (i + 1 < iend) ? ld32(s, oc + 4 * usx) : 0.0f;
     }
-    if (act && trho) store_c_fast(d, trho, i, j, k, theta_rho(thc, qvc));
+    if (act && trho) store_c_fast32(d, trho, i, j, k, oc, theta_rho(thc, qvc));
     ulo = uhi;
+    oc += usx;
+    ov += usxv;
+    ow += usxw;
+    oi += usi;
+    op += usx;
   }
 }
 
@@ -249,13 +266,14 @@ __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict_
   float* FZ = FY + (kFTY + 1) * nzl;
   const TileId tb = xcd_tile();
   const int j0 = tb.y * kFTY, i0 = tb.x * kFTX;
-  const int k = tz, j = j0 + ty;
+  const int k = lane_level(tz, nzl, d.nz), j = j0 + ty;
   const bool act = (k < d.nz) && (j < d.ny);
   const int K = k + 1;
   const int Kc = act ? K : 1, Jc = (j < d.ny) ? j + 3 : 3;
   const bool last_row = (ty == kFTY - 1) || (j == d.ny - 1);
   const long sx = d.sxu();
   const float r = rc[Kc];
+  const float ir = 1.0f / r;   // one division per thread instead of one per cell and field
   const int iend = min(i0 + kFTX, d.nx);
   const PlaneSlots psl = plane_slots(tid, nt, Lz, j0, d.ny + 5, (int)d.syc());
   float win[6];
@@ -314,7 +332,7 @@ __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict_
       float fzhi = (k == d.nz - 1) ? 0.0f : FZ[ty * nzl + k + 1];
       float conv = -((fxhi - fxlo) * d.idx + (FY[(ty + 1) * nzl + k] - FY[ty * nzl + k]) * d.idy +
                      (fzhi - FZ[ty * nzl + k]) * d.idz);
-      float a = (conv + uc * (0.5f * (fdlo + fdhi))) / (0.5f * (r + r));
+      float a = (conv + uc * (0.5f * (fdlo + fdhi))) * ir;   // rho8u = 0.5 (r + r) = r
       if (sgs) a = a + sgv;
       ru[((long)i * d.ny + j) * d.nz + k] = a;
       if (i == 0) ru[((long)d.nx * d.ny + j) * d.nz + k] = a;
@@ -345,13 +363,14 @@ __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict_
   float* FZ = FY + (kFTY + 1) * nzl;
   const TileId tb = xcd_tile();
   const int j0 = tb.y * kFTY, i0 = tb.x * kFTX;
-  const int k = tz, j = j0 + ty;
+  const int k = lane_level(tz, nzl, d.nz), j = j0 + ty;
   const bool act = (k < d.nz) && (j < d.ny);
   const int K = k + 1;
   const int Kc = act ? K : 1, Jc = (j < d.ny) ? j + 3 : 3;
   const bool last_row = (ty == kFTY - 1) || (j == d.ny - 1);
   const long sx = d.sxv();
   const float r = rc[Kc];
+  const float ir = 1.0f / r;   // one division per thread instead of one per cell and field
   const int iend = min(i0 + kFTX, d.nx);
   const PlaneSlots psl = plane_slots(tid, nt, Lz, j0, d.ny + 6, (int)d.syc());
   float win[6];
@@ -406,7 +425,7 @@ __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict_
       float fzhi = (k == d.nz - 1) ? 0.0f : FZ[ty * nzl + k + 1];
       float conv = -((fxhi - fxlo) * d.idx + (FY[(ty + 1) * nzl + k] - FY[ty * nzl + k]) * d.idy +
                      (fzhi - FZ[ty * nzl + k]) * d.idz);
-      float a = (conv + vc * (0.5f * (fd0 + fd1))) / (0.5f * (r + r));
+      float a = (conv + vc * (0.5f * (fd0 + fd1))) * ir;     // rho8v = 0.5 (r + r) = r
       if (sgs) a = a + sgv;
       rv[((long)i * (d.ny + 1) + j) * d.nz + k] = a;
       if (j == 0) rv[((long)i * (d.ny + 1) + d.ny) * d.nz + k] = a;
@@ -441,7 +460,7 @@ __global__ void __launch_bounds__(1024) k_mom_w(Dims d, const float* __restrict_
   float* BB = FD + kFTY * nzl;                      // kFTY x nzl: buoyancy at cell l
   const TileId tb = xcd_tile();
   const int j0 = tb.y * kFTY, i0 = tb.x * kFTX;
-  const int l = tz, j = j0 + ty;
+  const int l = lane_level(tz, nzl, d.nz), j = j0 + ty;
   const bool row_ok = j < d.ny;
   const bool lane_ok = l < d.nz;                   // owns level l and cell l
   const bool act = row_ok && lane_ok;
@@ -449,6 +468,7 @@ __global__ void __launch_bounds__(1024) k_mom_w(Dims d, const float* __restrict_
   const int Kc = act ? K : 1, Jc = row_ok ? j + 3 : 3;
   const bool last_row = (ty == kFTY - 1) || (j == d.ny - 1);
   const bool interior = act && l >= 1;              // levels 1..nz-1 carry advection
+  const float irho8w = 1.0f / (0.5f * (rc[Kc - 1] + rc[Kc]));
   const long sx = d.sxw();
   const int iend = min(i0 + kFTX, d.nx);
   const PlaneSlots psl = plane_slots(tid, nt, Lw, j0, d.ny + 5, (int)d.syw());
@@ -515,8 +535,7 @@ __global__ void __launch_bounds__(1024) k_mom_w(Dims d, const float* __restrict_
         float conv = -((fxhi - fxlo) * d.idx + (FY[(ty + 1) * nzl + l] - FY[ty * nzl + l]) * d.idy +
                        (FZ[ty * nzl + l] - FZ[ty * nzl + l - 1]) * d.idz);
         float div8w = 0.5f * (FD[ty * nzl + l - 1] + FD[ty * nzl + l]);
-        float rho8w = 0.5f * (rc[K - 1] + rc[K]);
-        a = (conv + wc * div8w) / rho8w;
+        a = (conv + wc * div8w) * irho8w;
       }
       a = a + b8;
       if (sgs) a = a + sgl;
@@ -582,13 +601,14 @@ __global__ void __launch_bounds__(512) k_stage_scalars_db(Dims d, const float* _
   const TileId tb = xcd_tile();
   const int j0 = tb.y * kFTY;
   const int i0 = tb.x * kFTX;
-  const int k = tz, j = j0 + ty;
+  const int k = lane_level(tz, nzl, d.nz), j = j0 + ty;
   const bool act = (k < d.nz) && (j < d.ny);
   const int K = k + 1;
   const int Kc = act ? K : 1, Jc = (j < d.ny) ? j + 3 : 3;
   const bool last_row = (ty == kFTY - 1) || (j == d.ny - 1);
   const long sx = d.sxc(), sy = d.syc();
   const float r = rc[Kc];
+  const float ir = 1.0f / r;   // one division per thread instead of one per cell and field
   const float r8lo = (Kc <= 0 || Kc >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc - 1] + rc[Kc]);
   const float r8hi = (Kc + 1 >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc] + rc[Kc + 1]);
   const float rz = 0.5f * (rc[Kc - 1] + rc[Kc]);
@@ -674,7 +694,7 @@ __global__ void __launch_bounds__(512) k_stage_scalars_db(Dims d, const float* _
         float fzhi = (k == d.nz - 1) ? 0.0f : FZ[ty * nzl + k + 1];
         float conv = -((fxhi[q] - fxlo[q]) * d.idx + (FY[(ty + 1) * nzl + k] - FY[ty * nzl + k]) * d.idy +
                        (fzhi - FZ[ty * nzl + k]) * d.idz);
-        float tend = (conv + qcs[q] * fd) / r;
+        float tend = (conv + qcs[q] * fd) * ir;
         if (q == 0 && S.heat) tend = tend + S.heat[((long)i * d.ny + j) * d.nz + k];
         float nxt = S.s0[q][c] + tend * dt;
         if (S.sgs[q]) nxt = nxt + S.sgs[q][((long)i * d.ny + j) * d.nz + k] * dt;
