@@ -11,10 +11,13 @@ vertically implicit acoustic substeps (solver_opt 3) at the specified n_sound = 
 stable, and the forward and forward+adjoint step times of the differentiable 128x128x64
 workload (20 steps, checkpoint every 5).
 
-Prints one JSON line: cell-steps/s, the dominant kernel's roofline (algorithmic bytes per
-launch / its average launch time from HIP events over the timed region, against the
-8 TB/s HBM peak) and a CPU baseline (the NumPy FP32 oracle on a bounded sample of the
-same workload, one core).
+Prints one JSON line: cell-steps/s from an uninstrumented timed loop; the dominant
+kernel's roofline (algorithmic bytes per launch / its average launch time from HIP
+events around each launch in a separate profiled loop, against the 8 TB/s HBM peak); the
+per-variable error of a GPU run against the FP64 and FP32 NumPy oracle on a sub-grid of
+the same recipe, the FP32 oracle's time on that sub-grid being the CPU baseline (one
+core); and the pseudo-incompressible 64x64x40 recipe (BiCGSTAB iterations per solve and
+the BiCGSTAB iteration's roofline).
 """
 from __future__ import annotations
 
@@ -69,22 +72,136 @@ def measured_traffic(kernel, nx, ny, nz, n_sound):
     return t["kernels"][kernel]["traffic_bytes"]
 
 
-def cpu_baseline(sample_steps=3, n_sound=7, nx=64, ny=64, nz=128):
-    """NumPy FP32 oracle on a sub-grid of the same workload (one core)."""
+ERR_NAMES = ["theta", "u", "v", "w", "pip", "qv", "qc", "qr"]
+
+
+def _field_errors(got, ref, init, demean_pip=False):
+    """Per variable rel-L2 and L-inf of the perturbation (field minus the initial state;
+    pi' with its mean removed for the pseudo-incompressible core) against `ref`."""
     import numpy as np
+    out = {}
+    for n, a, b, a0 in zip(ERR_NAMES, got, ref, init):
+        da = np.asarray(a, np.float64) - a0
+        db = np.asarray(b, np.float64) - a0
+        if n == "pip" and demean_pip:
+            da, db = da - da.mean(), db - db.mean()
+        nb = float(np.linalg.norm(db))
+        if nb == 0.0:
+            out[n] = {"rel_l2": 0.0 if float(np.abs(da).max()) == 0.0 else float("inf"), "linf": float(np.abs(da).max()),
+                      "note": "field stays zero"}
+            continue
+        out[n] = {"rel_l2": float(np.linalg.norm(da - db) / nb), "linf": float(np.abs(da - db).max())}
+    return out
+
+
+def errors_and_cpu_baseline(steps=3, n_sound=7, nx=64, ny=64, nz=128, time_cpu=True):
+    """The same recipe on a nx x ny x nz sub-grid: `steps` fused GPU steps against the
+    FP64 and the FP32 NumPy oracle from the same initial state.  The FP32 oracle run is
+    also the CPU baseline (one core, NumPy elementwise is single-threaded)."""
+    import numpy as np
+    import torch
+    from lex_amd import one_sprint_functions as sp
+    from lex_amd._core import grid_from
+    from lex_amd.setup_lex import build_case, config1
     from oracle import lex_oracle as o
-    case, opts = o.case_config1()
-    opts.n_sound = n_sound
-    c = o.scaled_case(case, nx, ny, nz)
-    phys, base, _ = o.setup_warm_bubble(c, dtype=np.float32)
-    t0 = time.perf_counter()
-    xi = phys
-    for _ in range(sample_steps):
-        xi, _ = o.ws_rk3_big_step(xi, base, c.grid, opts)
-    dt = time.perf_counter() - t0
-    return {"value": nx * ny * nz * sample_steps / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{sample_steps} WS-RK3 steps of the same recipe on a {nx}x{ny}x{nz} sub-grid "
-                      f"(NumPy FP32 oracle, single-threaded), {dt:.1f} s"}
+    case, opts = config1()
+    case = case.scaled(nx, ny, nz)
+    phys, base, grids, _ = build_case(case)
+    g = grid_from(base[1], *grids[3:6])
+    init = tuple(a.double().cpu().numpy() for a in phys)
+    b = tuple(a.double().cpu().numpy() for a in base[:5])
+    st = tuple(a.clone() for a in phys)
+    mo = (2, False, False, 0, False, False, 1)
+    for _ in range(steps):
+        sp.fused_step(st, base, g, mo, n_sound=n_sound)
+    torch.cuda.synchronize()
+    got = tuple(a.double().cpu().numpy() for a in st)
+    og = o.Grid(nx, ny, nz, case.dx, case.dy, case.dz)
+    oo = o.Options(dt=opts["dt"], solver_opt=2, n_sound=n_sound, turb_opt=1)
+    runs, secs = {}, {}
+    for name, dt in (("fp32", np.float32), ("fp64", np.float64)):
+        xi = tuple(a.astype(dt) for a in init)
+        bs = tuple(c.astype(dt) for c in b) + (None,)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            xi, _ = o.ws_rk3_big_step(xi, bs, og, oo)
+        secs[name] = time.perf_counter() - t0
+        runs[name] = xi
+    errs = {"workload": f"{steps} WS-RK3 steps of the same recipe (n_sound {n_sound}) on a {nx}x{ny}x{nz} sub-grid",
+            "fields": "perturbation from the initial state; rel_l2 = |d - d_ref| / |d_ref|",
+            "gpu_vs_fp64_oracle": _field_errors(got, runs["fp64"], init),
+            "gpu_vs_fp32_oracle": _field_errors(got, runs["fp32"], init),
+            "fp32_oracle_vs_fp64_oracle": _field_errors(runs["fp32"], runs["fp64"], init)}
+    cpu = None
+    if time_cpu:
+        cpu = {"value": nx * ny * nz * steps / secs["fp32"], "unit": "cell-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{steps} WS-RK3 steps of the same recipe on a {nx}x{ny}x{nz} sub-grid "
+                         f"(NumPy FP32 oracle, single-threaded), {secs['fp32']:.1f} s"}
+    return errs, cpu
+
+
+def c0_timing(steps=10, warmup=2):
+    """The pseudo-incompressible recipe at its full 64x64x40 (dx = dy = 600 m, dz = 300 m,
+    dt = 30 s, BiCGSTAB pressure solve, tol 1e-4, maxiter 100): step time, iterations of
+    every solve, and the BiCGSTAB iteration's roofline (88 B/cell per iteration: two
+    matvecs plus the s, x, r, p updates, dots fused) from the per-kernel HIP events."""
+    import torch
+    from lex_amd import _lib, namelist_n_constants as nl, one_sprint_functions as sp
+    from lex_amd import one_step_integration as one
+    from lex_amd.setup_lex import build_case, config0
+    case, opts = config0()
+    phys, base, grids, _ = build_case(case)
+    nl.dt, nl.n_sound, nl.solver_opt, nl.turb_opt = opts["dt"], opts["n_sound"], 1, 1
+    mo = (1, False, False, 0, False, False, 1)
+    ks = []
+    solve = one.solve_pres_eqn
+
+    def rec(*a, **k):
+        x, info = solve(*a, **k)
+        ks.append(info)
+        return x, info
+    one.solve_pres_eqn = rec
+    try:
+        st = phys
+        for _ in range(warmup):
+            st, _ = sp.ws_rk3_big_step(st, base, grids, mo)
+        torch.cuda.synchronize()
+        ks.clear()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            st, _ = sp.ws_rk3_big_step(st, base, grids, mo)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        iters = list(ks)
+        # a profiled repetition for the per-kernel times of the solver
+        ks.clear()
+        _lib.profile_enable(True, 16384)
+        for _ in range(2):
+            st, _ = sp.ws_rk3_big_step(st, base, grids, mo)
+        prof = _lib.profile_read()
+        _lib.profile_enable(False)
+        piters = sum(max(k, 0) for k in ks)
+    finally:
+        one.solve_pres_eqn = solve
+    ncell = case.nx * case.ny * case.nz
+    it_ms = sum(prof.get(k, (0.0, 0))[0] for k in ("k_bicg_pq", "k_bicg_st", "k_bicg_xr"))
+    launches = prof.get("k_bicg_pq", (0.0, 0))[1]
+    gbs = 88.0 * ncell * piters / (it_ms / 1e3) / 1e9 if it_ms > 0 else None
+    return {"workload": f"pseudo-incompressible dry warm bubble {case.nx}x{case.ny}x{case.nz}, dx {case.dx:.0f} m, "
+                        f"dz {case.dz:.0f} m, dt {opts['dt']:.0f} s, WS-RK3, BiCGSTAB (tol 1e-4, maxiter 100)",
+            "steps": steps, "warmup": warmup, "ms_per_step": ms / steps,
+            "value": ncell * steps / (ms / 1e3), "unit": "cell-steps/s",
+            "bicgstab_iterations": iters, "solves": len(iters),
+            "bicgstab_exit_codes_note": "k = iterations done; every solve stops at maxiter 100 because the affine "
+                                        "pressure system is not consistent to the tolerance (as in the reference)",
+            "bicgstab": {"bound": "hbm", "bytes_per_cell_per_iteration": 88, "iterations_profiled": piters,
+                         "iteration_launches_profiled": launches, "iteration_ms": it_ms / piters if piters else None,
+                         "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS if gbs else None},
+            "kernels_profiled": {k: {"avg_ms": round(v[0] / v[1], 5), "launches": v[1]} for k, v in prof.items()},
+            "state_finite": all(bool(torch.isfinite(a).all()) for a in st)}
 
 
 def implicit_solver(init, base, g, ncell, n_sound, steps=20, warmup=2):
@@ -451,7 +568,12 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, copy a 256 MiB buffer 3 times (a kernel of known HBM "
                          "traffic, used to scale rocprof FETCH_SIZE/WRITE_SIZE counters)")
-    ap.add_argument("--only", choices=("adjoint", "sgs-net", "output", "dl-sprint", "dl-train", "physics"),
+    ap.add_argument("--no-errors", action="store_true",
+                    help="skip the error report against the FP64/FP32 oracle (and with it the CPU baseline)")
+    ap.add_argument("--no-c0", action="store_true", help="skip the pseudo-incompressible 64x64x40 timing")
+    ap.add_argument("--profile-steps", type=int, default=10,
+                    help="steps of the separate profiled loop that feeds the per-kernel table and the roofline")
+    ap.add_argument("--only", choices=("adjoint", "sgs-net", "output", "dl-sprint", "dl-train", "physics", "c0"),
                     default=None,
                     help="run only the forward+adjoint workload, the SGS-net timing, the output-file "
                          "timing, the post-step-correction sprint, its train_step or the physics-option steps and "
@@ -470,7 +592,8 @@ def main():
     _lib.load()
     if args.only is not None:
         fn = {"adjoint": fwd_adjoint, "sgs-net": sgs_net_timing, "output": output_timing,
-              "dl-sprint": dl_sprint_timing, "dl-train": dl_train_timing, "physics": physics_timing}[args.only]
+              "dl-sprint": dl_sprint_timing, "dl-train": dl_train_timing, "physics": physics_timing,
+              "c0": c0_timing}[args.only]
         print(json.dumps(fn()))
         return
     case, opts = config1()
@@ -515,7 +638,7 @@ def main():
         sp.fused_step(state, base, g, model_opt)
     torch.cuda.synchronize()
 
-    _lib.profile_enable(True, 16384)
+    # headline: the timed steps with no per-kernel instrumentation
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -525,20 +648,33 @@ def main():
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1)
-    prof = _lib.profile_read()
-    _lib.profile_enable(False)
     finite = all(bool(torch.isfinite(a).all()) for a in state)
-
     ms_step = ms / args.steps
     value = ncell * args.steps / (ms / 1e3)
-    # dominant kernel = largest total time in the timed region
+
+    # a separate profiled loop (HIP events around every launch, on the launch stream)
+    # feeds the per-kernel table and the dominant kernel's roofline
+    psteps = max(1, args.profile_steps)
+    _lib.profile_enable(True, 16384)
+    p0 = torch.cuda.Event(enable_timing=True)
+    p1 = torch.cuda.Event(enable_timing=True)
+    p0.record()
+    for _ in range(psteps):
+        sp.fused_step(state, base, g, model_opt)
+    p1.record()
+    torch.cuda.synchronize()
+    pms = p0.elapsed_time(p1)
+    prof = _lib.profile_read()
+    _lib.profile_enable(False)
+
+    # dominant kernel = largest total time in the profiled loop
     dom, (dom_ms, dom_n) = max(prof.items(), key=lambda kv: kv[1][0])
     avg_ms = dom_ms / dom_n
     bpc = BYTES_PER_CELL.get(dom, 0)
     traffic_alg = bpc * ncell
     achieved = traffic_alg / (avg_ms / 1e3) / 1e9
-    kernels = {k: {"avg_ms": round(v[0] / v[1], 5), "launches_per_step": v[1] / args.steps,
-                   "share": round(v[0] / ms, 4)} for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
+    kernels = {k: {"avg_ms": round(v[0] / v[1], 5), "launches_per_step": v[1] / psteps,
+                   "share": round(v[0] / pms, 4)} for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
     if args.profile:
         print(json.dumps(kernels, indent=1), file=sys.stderr)
 
@@ -555,7 +691,9 @@ def main():
     cb = copy_bandwidth()
     fa = None if args.no_adjoint else fwd_adjoint()
     sn = None if args.no_adjoint else sgs_net_timing()
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample_steps, n_sound)
+    errs, cpu = (None, None) if args.no_errors else errors_and_cpu_baseline(args.cpu_sample_steps, n_sound,
+                                                                            time_cpu=not args.no_cpu_baseline)
+    c0 = None if args.no_c0 else c0_timing()
     line = {
         "metric": "cell-steps/s (fwd, fwd+adjoint) vs plain at stated error; % HBM roofline",
         "value": value,
@@ -582,13 +720,18 @@ def main():
                      "traffic_source": "profiles/rk3_step_256x256x128_hbm_traffic.md (rocprofv3 --pmc FETCH_SIZE, "
                                        "WRITE_SIZE; copy-kernel scaled)",
                      "bytes_per_cell": bpc, "avg_launch_ms": avg_ms,
+                     "avg_launch_source": f"HIP events around each launch on its stream, separate loop of {psteps} "
+                                          "steps",
                      "copy_GBs": cb["GBs"], "frac_of_copy": achieved / cb["GBs"]},
         "cpu_baseline": cpu,
+        "errors": errs,
+        "c0": c0,
         "as_specified": as_specified,
         "implicit_as_specified": impl,
         "fwd_adjoint": fa,
         "sgs_net": sn,
         "kernels": kernels,
+        "profiled_ms_per_step": pms / psteps,
         "state_finite": finite,
     }
     print(json.dumps(line))

@@ -265,17 +265,18 @@ int bicgstab_run(const lex_grid_t* g, const float* rtt, const float* rhs, const 
   int nb = (int)std::min<long>(kBicgBlocks, (n + kBicgThreads - 1) / kBicgThreads);
   // the loop state (tickets, flags, counters) is cleared by a kernel, not a memset
   hipLaunchKernelGGL(k_bicg_reset, dim3(1), dim3(64), 0, hs, st, maxiter);
-  hipLaunchKernelGGL(k_bicg_init, dim3(nb), dim3(kBicgThreads), 0, hs, d, rtt, rhs, x0, x, r, rhat, p[0], q[0],
-                     partials, st, tol, atol);
+  LEX_LAUNCH(KID_BICG_INIT, k_bicg_init, dim3(nb), dim3(kBicgThreads), 0, hs, d, rtt, rhs, x0, x, r, rhat, p[0],
+             q[0], partials, st, tol, atol);
   int rc = check_launch(who);
   if (rc) return rc;
   int cur = 0;
   const int poll = 8;
   for (int it = 0; it < maxiter; ++it) {
-    hipLaunchKernelGGL(k_bicg_pq, dim3(nb), dim3(kBicgThreads), 0, hs, d, rtt, r, rhat, p[cur], q[cur], p[1 - cur],
-                       q[1 - cur], partials, st);
-    hipLaunchKernelGGL(k_bicg_st, dim3(nb), dim3(kBicgThreads), 0, hs, d, rtt, r, q[1 - cur], s, t, partials, st);
-    hipLaunchKernelGGL(k_bicg_xr, dim3(nb), dim3(kBicgThreads), 0, hs, n, p[1 - cur], s, t, rhat, x, r, partials, st);
+    LEX_LAUNCH(KID_BICG_PQ, k_bicg_pq, dim3(nb), dim3(kBicgThreads), 0, hs, d, rtt, r, rhat, p[cur], q[cur],
+               p[1 - cur], q[1 - cur], partials, st);
+    LEX_LAUNCH(KID_BICG_ST, k_bicg_st, dim3(nb), dim3(kBicgThreads), 0, hs, d, rtt, r, q[1 - cur], s, t, partials, st);
+    LEX_LAUNCH(KID_BICG_XR, k_bicg_xr, dim3(nb), dim3(kBicgThreads), 0, hs, n, p[1 - cur], s, t, rhat, x, r, partials,
+               st);
     cur = 1 - cur;
     rc = check_launch(who);
     if (rc) return rc;
