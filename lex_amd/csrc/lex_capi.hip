@@ -230,7 +230,7 @@ size_t lex_bicgstab_workspace_bytes(const lex_grid_t* g) {
   if (!grid_ok(g)) return 0;
   size_t n = (size_t)g->nx * g->ny * g->nz;
   size_t vec = ((n * sizeof(float) + 255) / 256) * 256;
-  return 256 + (size_t)kBicgBlocks * 4 * sizeof(double) + 8 * vec;
+  return 256 + kBicgPartialBytes + 8 * vec;
 }
 
 }  // extern "C"
@@ -239,7 +239,7 @@ size_t lex_bicgstab_workspace_bytes(const lex_grid_t* g) {
 // Sized for LEX_BICGSTAB_NOALLOC_MAX_CELLS cells in the BiCGSTAB workspace layout
 // (loop state, partial sums, eight vectors).
 constexpr size_t kNoAllocVec = (size_t)LEX_BICGSTAB_NOALLOC_MAX_CELLS * sizeof(float);
-constexpr size_t kNoAllocBytes = 256 + (size_t)kBicgBlocks * 4 * sizeof(double) + 8 * kNoAllocVec;
+constexpr size_t kNoAllocBytes = 256 + kBicgPartialBytes + 8 * kNoAllocVec;
 static_assert(kNoAllocVec % 256 == 0, "vectors stay 256-byte aligned");
 static __device__ __align__(256) char scratch[kNoAllocBytes];
 
@@ -255,28 +255,32 @@ int bicgstab_run(const lex_grid_t* g, const float* rtt, const float* rhs, const 
   size_t vec = (((size_t)n * sizeof(float) + 255) / 256) * 256;
   BicgState* st = (BicgState*)base;
   double* partials = (double*)(base + 256);
-  char* v0 = base + 256 + (size_t)kBicgBlocks * 4 * sizeof(double);
+  char* v0 = base + 256 + kBicgPartialBytes;
   float* r = (float*)(v0 + 0 * vec);
   float* rhat = (float*)(v0 + 1 * vec);
   float* p[2] = {(float*)(v0 + 2 * vec), (float*)(v0 + 3 * vec)};
   float* q[2] = {(float*)(v0 + 4 * vec), (float*)(v0 + 5 * vec)};
   float* s = (float*)(v0 + 6 * vec);
   float* t = (float*)(v0 + 7 * vec);
-  int nb = (int)std::min<long>(kBicgBlocks, (n + kBicgThreads - 1) / kBicgThreads);
-  // the loop state (tickets, flags, counters) is cleared by a kernel, not a memset
+  // kBicgCellsPerThread cells per thread, at most kBicgBlocks blocks (the partial-sum capacity)
+  const long cpb = (long)kBicgCellsPerThread * kBicgThreads;
+  const int nb = (int)std::min<long>(kBicgBlocks, std::max<long>(1, (n + cpb - 1) / cpb));
+  // the loop state (flags, counters, scalars) is cleared by a kernel, not a memset
   hipLaunchKernelGGL(k_bicg_reset, dim3(1), dim3(64), 0, hs, st, maxiter);
   LEX_LAUNCH(KID_BICG_INIT, k_bicg_init, dim3(nb), dim3(kBicgThreads), 0, hs, d, rtt, rhs, x0, x, r, rhat, p[0],
-             q[0], partials, st, tol, atol);
+             q[0], partials);
   int rc = check_launch(who);
   if (rc) return rc;
   int cur = 0;
   const int poll = 8;
-  for (int it = 0; it < maxiter; ++it) {
+  int it = 0;
+  for (; it < maxiter; ++it) {
     LEX_LAUNCH(KID_BICG_PQ, k_bicg_pq, dim3(nb), dim3(kBicgThreads), 0, hs, d, rtt, r, rhat, p[cur], q[cur],
-               p[1 - cur], q[1 - cur], partials, st);
-    LEX_LAUNCH(KID_BICG_ST, k_bicg_st, dim3(nb), dim3(kBicgThreads), 0, hs, d, rtt, r, q[1 - cur], s, t, partials, st);
+               p[1 - cur], q[1 - cur], partials, st, it, nb, tol, atol);
+    LEX_LAUNCH(KID_BICG_ST, k_bicg_st, dim3(nb), dim3(kBicgThreads), 0, hs, d, rtt, r, q[1 - cur], s, t, partials, st,
+               it, nb, tol, atol);
     LEX_LAUNCH(KID_BICG_XR, k_bicg_xr, dim3(nb), dim3(kBicgThreads), 0, hs, n, p[1 - cur], s, t, rhat, x, r, partials,
-               st);
+               st, it, nb, tol, atol);
     cur = 1 - cur;
     rc = check_launch(who);
     if (rc) return rc;
@@ -285,12 +289,19 @@ int bicgstab_run(const lex_grid_t* g, const float* rtt, const float* rhs, const 
       hipError_t e = hipMemcpyAsync(&done, &st->done, sizeof(int), hipMemcpyDeviceToHost, hs);
       if (e == hipSuccess) e = hipStreamSynchronize(hs);
       if (e != hipSuccess) return fail((int)e, "lex_bicgstab: %s", hipGetErrorString(e));
-      if (done) break;
+      if (done) {
+        ++it;
+        break;
+      }
     }
   }
+  // maxiter reached without the loop test failing: the exit code is the last count
+  hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(64), 0, hs, st, it);
+  rc = check_launch(who);
+  if (rc) return rc;
   if (k_out) {
     int kk = 0;
-    hipError_t e = hipMemcpyAsync(&kk, &st->k, sizeof(int), hipMemcpyDeviceToHost, hs);
+    hipError_t e = hipMemcpyAsync(&kk, &st->k_final, sizeof(int), hipMemcpyDeviceToHost, hs);
     if (e == hipSuccess) e = hipStreamSynchronize(hs);
     if (e != hipSuccess) return fail((int)e, "lex_bicgstab: %s", hipGetErrorString(e));
     *k_out = kk;

@@ -3,10 +3,10 @@
 // calls (one_step_integration.py:468-487).  Vectors are FP32 (nx, ny, nz); every
 // reduction (dot product) and the scalar recurrence run in FP64.
 //
-// One iteration = 3 kernels, each a grid-stride pass with a deterministic two-level
-// reduction (per-block FP64 partials, the last block to finish sums them in a fixed
-// order and advances the scalar state).  All kernels check a device-side `done` flag,
-// so the host can enqueue iterations ahead and poll the flag only every few iterations.
+// One iteration = 3 kernels, each a grid-stride pass whose dot products leave FP64
+// per-block partials that the following kernels sum in a fixed order (deterministic).
+// All kernels check a device-side `done` flag, so the host can enqueue iterations ahead
+// and poll the flag only every few iterations.
 #pragma once
 #include "lex_common.hpp"
 
@@ -96,22 +96,51 @@ __global__ void k_p_matvec(Dims d, const float* __restrict__ rtt, const float* _
 }
 
 // ------------------------------- BiCGSTAB ----------------------------------------
+// Every dot product is reduced in two phases across a kernel boundary: a kernel writes
+// one FP64 partial sum per block, and each block of the next kernel sums all partials
+// itself, in the same fixed order, so every block derives bit-identical scalars.  No
+// kernel needs a grid-wide fence or a last-block ticket (on a multi-XCD part a
+// device-scope release writes back the XCD's L2, which made those tails dominate).
+//
+// Scalars entering iteration `it` (rho, alpha, omega, k) are double-buffered by parity:
+// iteration it reads s[it & 1] and its last kernel's block 0 writes s[(it + 1) & 1].
+// The loop test (JAX: |r|^2 > atol^2 and 0 <= k < maxiter) runs at the top of each
+// iteration; the first kernel that finds it false sets the sticky `done` and `k_final`.
+struct BicgScalars {
+  double rho, alpha, omega;
+  int k, pad;
+};
 struct BicgState {
-  double bs, atol2;
-  double rho, alpha, omega;   // recurrence scalars (rho = previous rho_)
-  double rho_new;             // rhat.r of the current residual
-  double rr;                  // r.r of the current residual
-  double alpha_new, omega_new, ss;
-  int k;                      // iteration counter / exit code
-  int done;
-  int early;
+  BicgScalars s[2];
+  int done;      // sticky: the loop has ended
+  int k_final;   // exit code (iterations, or -10 / -11 on breakdown)
   int maxiter;
-  unsigned int counter[4];    // last-block tickets, one per kernel
   int pad;
 };
+static_assert(sizeof(BicgState) <= 256, "BiCGSTAB loop state fits its 256-byte slot");
 
 constexpr int kBicgThreads = 256;
-constexpr int kBicgBlocks = 1024;  // grid-stride; partials buffer holds kBicgBlocks*4 doubles
+constexpr int kBicgBlocks = 1024;   // upper bound on blocks (partials per sum)
+constexpr int kBicgCellsPerThread = 2;   // grid size (1-16 measured within 15 % on 64x64x40)
+// partial-sum slots (each kBicgBlocks doubles): b.b; (r.r, rhat.r) x 2 parities;
+// rhat.q; (t.s, t.t, s.s)
+constexpr int kBicgSlots = 1 + 4 + 1 + 3;
+constexpr size_t kBicgPartialBytes = (size_t)kBicgSlots * kBicgBlocks * sizeof(double);
+struct BicgPartials {
+  double* bb;      // [nb]
+  double* rr[2];   // [2][nb] by parity: r.r, then rhat.r
+  double* rq;      // [nb]
+  double* ts;      // [3][nb]: t.s, t.t, s.s
+};
+__host__ __device__ inline BicgPartials bicg_partials(double* base) {
+  BicgPartials P;
+  P.bb = base;
+  P.rr[0] = base + kBicgBlocks;
+  P.rr[1] = base + 3 * kBicgBlocks;
+  P.rq = base + 5 * kBicgBlocks;
+  P.ts = base + 6 * kBicgBlocks;
+  return P;
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -119,84 +148,126 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// block-sum NV values per thread into lane 0 of the block; returns in `out` on thread 0
+// Block-level sum of NV per-thread values; the result is returned to every thread.
 template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double (&out)[NV]) {
+__device__ __forceinline__ void block_sum_all(const double (&v)[NV], double (&out)[NV]) {
   __shared__ double sh[NV][kBicgThreads / 64];
-  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
-    double s = wave_sum(v[q]);
+    const double s = wave_sum(v[q]);
     if (lane == 0) sh[q][wv] = s;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-      double s = 0.0;
-      for (int t = 0; t < kBicgThreads / 64; ++t) s += sh[q][t];
-      out[q] = s;
-    }
-  }
-}
-
-// Returns true on the one block that finished last; partial sums of this block are in
-// `part` (NV values), written to partials[q*gridDim.x + blockIdx.x].
-template <int NV>
-__device__ bool publish_partials(double (&part)[NV], double* __restrict__ partials, unsigned int* ticket) {
-  __shared__ int last;
-  if (threadIdx.x == 0) {
-    for (int q = 0; q < NV; ++q) partials[q * gridDim.x + blockIdx.x] = part[q];
-    __threadfence();
-    unsigned int t = atomicAdd(ticket, 1u);
-    last = (t == gridDim.x - 1);
+  for (int q = 0; q < NV; ++q) {
+    double t = 0.0;
+    for (int w = 0; w < kBicgThreads / 64; ++w) t += sh[q][w];
+    out[q] = t;
   }
   __syncthreads();
-  return last != 0;
 }
 
+// Sum of NV partial arrays (stride kBicgBlocks, nb entries each) in a fixed order,
+// returned to every thread of the block.
 template <int NV>
-__device__ void final_sums(const double* __restrict__ partials, double (&tot)[NV]) {
-  // executed by the whole last block; fixed summation order -> deterministic
-  __shared__ double sh[NV][kBicgThreads];
-  __threadfence();
+__device__ __forceinline__ void sum_partials(const double* __restrict__ P, int nb, double (&tot)[NV]) {
+  double v[NV];
+#pragma unroll
   for (int q = 0; q < NV; ++q) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) s += ((volatile const double*)partials)[q * gridDim.x + b];
-    sh[q][threadIdx.x] = s;
+    for (int b = threadIdx.x; b < nb; b += kBicgThreads) s += P[q * kBicgBlocks + b];
+    v[q] = s;
   }
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-      for (int q = 0; q < NV; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
-    __syncthreads();
-  }
-  for (int q = 0; q < NV; ++q) tot[q] = sh[q][0];
+  block_sum_all<NV>(v, tot);
 }
 
-// zero the tickets and set the iteration cap (runs before k_bicg_init)
+// Write this block's sums of NV per-thread values to P[q * kBicgBlocks + blockIdx.x].
+template <int NV>
+__device__ __forceinline__ void put_partials(const double (&acc)[NV], double* __restrict__ P) {
+  double tot[NV];
+  block_sum_all<NV>(acc, tot);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) P[q * kBicgBlocks + blockIdx.x] = tot[q];
+  }
+}
+
+// Interior cell (i, j, k) of the unpadded linear index t (x slowest, z fastest).
+__device__ __forceinline__ void cell_of(const Dims& d, long t, int& i, int& j, int& k) {
+  const long jt = t / d.nz;
+  k = (int)(t - jt * d.nz);
+  i = (int)(jt / d.ny);
+  j = (int)(jt - (long)i * d.ny);
+}
+
+// laplace_of_pressure (pressure_equations.py:8-48) at (i, j, k) of a vector given by
+// F(offset) on the unpadded (nx, ny, nz) layout: periodic in x and y, z clamped to the
+// end levels; arithmetic in the order of matvec_point.
+template <class F>
+__device__ __forceinline__ float matvec_at(const Dims& d, const float* __restrict__ rtt, int i, int j, int k, long t,
+                                           const F& X, float& xc) {
+  const long syz = (long)d.ny * d.nz;
+  const long oxm = (i == 0) ? t + (long)(d.nx - 1) * syz : t - syz;
+  const long oxp = (i == d.nx - 1) ? t - (long)(d.nx - 1) * syz : t + syz;
+  const long oym = (j == 0) ? t + (long)(d.ny - 1) * d.nz : t - d.nz;
+  const long oyp = (j == d.ny - 1) ? t - (long)(d.ny - 1) * d.nz : t + d.nz;
+  const long ozm = (k == 0) ? t : t - 1;
+  const long ozp = (k == d.nz - 1) ? t : t + 1;
+  const long cc = ic(d, i + 3, j + 3, k + 1);
+  const long sx = d.sxc(), sy = d.syc();
+  xc = X(t);
+  float px0 = 0.5f * (rtt[cc] + rtt[cc - sx]) * ((xc - X(oxm)) * d.idx);
+  float px1 = 0.5f * (rtt[cc + sx] + rtt[cc]) * ((X(oxp) - xc) * d.idx);
+  float py0 = 0.5f * (rtt[cc] + rtt[cc - sy]) * ((xc - X(oym)) * d.idy);
+  float py1 = 0.5f * (rtt[cc + sy] + rtt[cc]) * ((X(oyp) - xc) * d.idy);
+  float pz0 = (k == 0) ? rtt[cc - 1] : 0.5f * (rtt[cc] + rtt[cc - 1]) * ((xc - X(ozm)) * d.idz);
+  float pz1 = (k == d.nz - 1) ? rtt[cc + 1] : 0.5f * (rtt[cc + 1] + rtt[cc]) * ((X(ozp) - xc) * d.idz);
+  return (float)kCp * ((px1 - px0) * d.idx + (py1 - py0) * d.idy + (pz1 - pz0) * d.idz);
+}
+
+// Scalars every kernel of iteration `it` derives from the partials: |r|^2 and rhat.r
+// of the iteration's starting residual, and atol^2.
+struct BicgTop {
+  double rr, rho_new, atol2;
+};
+__device__ __forceinline__ BicgTop bicg_top(const BicgPartials& P, int nb, int it, double tol, double atol) {
+  double a[2], bb[1];
+  sum_partials<2>(P.rr[(it + 1) & 1], nb, a);
+  sum_partials<1>(P.bb, nb, bb);
+  BicgTop t;
+  t.rr = a[0];
+  t.rho_new = a[1];
+  t.atol2 = fmax(tol * tol * bb[0], atol * atol);
+  return t;
+}
+
+// clear the loop state (a kernel, not a memset, so the solve can be graph-captured)
 __global__ void k_bicg_reset(BicgState* st, int maxiter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    for (int c = 0; c < 4; ++c) st->counter[c] = 0;
+    for (int q = 0; q < 2; ++q) {
+      st->s[q].rho = st->s[q].alpha = st->s[q].omega = 1.0;
+      st->s[q].k = 0;
+    }
     st->maxiter = maxiter;
     st->done = 0;
-    st->k = 0;
+    st->k_final = 0;
   }
 }
 
-// init: r = b - A(x0), rhat = p = q = r, x = x0; sums: b.b, r.r (rhat.r = r.r)
+// init: r = b - A(x0), rhat = p = q = r, x = x0; partials b.b and (r.r, rhat.r = r.r)
+// in the parity slot iteration 0 reads
 __global__ void __launch_bounds__(kBicgThreads) k_bicg_init(Dims d, const float* __restrict__ rtt,
                                                            const float* __restrict__ b, const float* __restrict__ x0,
                                                            float* __restrict__ x, float* __restrict__ r,
                                                            float* __restrict__ rhat, float* __restrict__ p,
-                                                           float* __restrict__ q, double* __restrict__ partials,
-                                                           BicgState* st, double tol, double atol) {
-  long n = (long)d.nx * d.ny * d.nz;
+                                                           float* __restrict__ q, double* __restrict__ partials) {
+  const BicgPartials P = bicg_partials(partials);
+  const long n = (long)d.nx * d.ny * d.nz;
   double acc[2] = {0.0, 0.0};
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    int k = t % d.nz;
-    long jt = t / d.nz;
-    int j = jt % d.ny, i = jt / d.ny;
+    int i, j, k;
+    cell_of(d, t, i, j, k);
     float ri = b[t] - matvec_point(d, rtt, x0, i, j, k);
     x[t] = x0[t];
     r[t] = ri;
@@ -206,134 +277,95 @@ __global__ void __launch_bounds__(kBicgThreads) k_bicg_init(Dims d, const float*
     acc[0] += (double)b[t] * b[t];
     acc[1] += (double)ri * ri;
   }
-  double part[2];
-  block_sum<2>(acc, part);
-  if (publish_partials<2>(part, partials, &st->counter[0])) {
-    double tot[2];
-    final_sums<2>(partials, tot);
-    if (threadIdx.x == 0) {
-      st->bs = tot[0];
-      st->atol2 = fmax(tol * tol * tot[0], atol * atol);
-      st->rr = tot[1];
-      st->rho_new = tot[1];
-      st->rho = st->alpha = st->omega = 1.0;
-      st->k = 0;
-      st->done = !(st->rr > st->atol2 && 0 < st->maxiter);
-      for (int c = 0; c < 4; ++c) st->counter[c] = 0;
-    }
-  }
+  double bb[1] = {acc[0]};
+  double rr[2] = {acc[1], acc[1]};
+  put_partials<1>(bb, P.bb);
+  put_partials<2>(rr, P.rr[1]);
 }
 
-// K1: (loop test) beta; p' = r + beta (p - omega q) recomputed on the 7-point stencil;
-// q' = A(p'); sum rhat.q'  -> alpha
+// K1 of iteration it: the loop test; beta; p' = r + beta (p - omega q) (recomputed on
+// the stencil's points); q' = A(p'); partials rhat.q'
 __global__ void __launch_bounds__(kBicgThreads) k_bicg_pq(Dims d, const float* __restrict__ rtt,
                                                          const float* __restrict__ r, const float* __restrict__ rhat,
                                                          const float* __restrict__ p, const float* __restrict__ q,
                                                          float* __restrict__ pn, float* __restrict__ qn,
-                                                         double* __restrict__ partials, BicgState* st) {
+                                                         double* __restrict__ partials, BicgState* st, int it, int nb,
+                                                         double tol, double atol) {
   if (st->done) return;
-  long n = (long)d.nx * d.ny * d.nz;
-  double beta_d = st->rho_new / st->rho * st->alpha / st->omega;
-  float beta = (float)beta_d, om = (float)st->omega;
-  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const BicgPartials P = bicg_partials(partials);
+  const BicgTop top = bicg_top(P, nb, it, tol, atol);
+  const BicgScalars sc = st->s[it & 1];
+  if (!(top.rr > top.atol2 && sc.k < st->maxiter && sc.k >= 0)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done = 1;
+      st->k_final = sc.k;
+    }
+    return;
+  }
+  const double beta_d = top.rho_new / sc.rho * sc.alpha / sc.omega;
+  const float beta = (float)beta_d, om = (float)sc.omega;
+  const long n = (long)d.nx * d.ny * d.nz;
   double acc[1] = {0.0};
+  auto Pv = [&](long o) { return r[o] + beta * (p[o] - om * q[o]); };
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    int k = t % nz;
-    long jt = t / nz;
-    int j = jt % ny, i = jt / ny;
-    auto P = [&](int a, int b2, int c) {
-      a = (a + nx) % nx;
-      b2 = (b2 + ny) % ny;
-      c = c < 0 ? 0 : (c >= nz ? nz - 1 : c);
-      long o = ((long)a * ny + b2) * nz + c;
-      return r[o] + beta * (p[o] - om * q[o]);
-    };
-    long cc = ic(d, i + 3, j + 3, k + 1);
-    const long sx = d.sxc(), sy = d.syc();
-    float xc = P(i, j, k);
-    float px0 = 0.5f * (rtt[cc] + rtt[cc - sx]) * ((xc - P(i - 1, j, k)) * d.idx);
-    float px1 = 0.5f * (rtt[cc + sx] + rtt[cc]) * ((P(i + 1, j, k) - xc) * d.idx);
-    float py0 = 0.5f * (rtt[cc] + rtt[cc - sy]) * ((xc - P(i, j - 1, k)) * d.idy);
-    float py1 = 0.5f * (rtt[cc + sy] + rtt[cc]) * ((P(i, j + 1, k) - xc) * d.idy);
-    float pz0 = (k == 0) ? rtt[cc - 1] : 0.5f * (rtt[cc] + rtt[cc - 1]) * ((xc - P(i, j, k - 1)) * d.idz);
-    float pz1 = (k == nz - 1) ? rtt[cc + 1] : 0.5f * (rtt[cc + 1] + rtt[cc]) * ((P(i, j, k + 1) - xc) * d.idz);
-    float qv = (float)kCp * ((px1 - px0) * d.idx + (py1 - py0) * d.idy + (pz1 - pz0) * d.idz);
+    int i, j, k;
+    cell_of(d, t, i, j, k);
+    float xc;
+    const float qv = matvec_at(d, rtt, i, j, k, t, Pv, xc);
     pn[t] = xc;
     qn[t] = qv;
     acc[0] += (double)rhat[t] * qv;
   }
-  double part[1];
-  block_sum<1>(acc, part);
-  if (publish_partials<1>(part, partials, &st->counter[1])) {
-    double tot[1];
-    final_sums<1>(partials, tot);
-    if (threadIdx.x == 0) {
-      st->alpha_new = st->rho_new / tot[0];
-      st->counter[1] = 0;
-    }
-  }
+  put_partials<1>(acc, P.rq);
 }
 
-// K2: s = r - alpha q (recomputed on the stencil), t = A(s); sums t.s, t.t, s.s
+// K2: alpha = rho_new / rhat.q; s = r - alpha q (recomputed on the stencil), t = A(s);
+// partials t.s, t.t, s.s
 __global__ void __launch_bounds__(kBicgThreads) k_bicg_st(Dims d, const float* __restrict__ rtt,
                                                          const float* __restrict__ r, const float* __restrict__ q,
                                                          float* __restrict__ s, float* __restrict__ tv,
-                                                         double* __restrict__ partials, BicgState* st) {
+                                                         double* __restrict__ partials, BicgState* st, int it, int nb,
+                                                         double tol, double atol) {
   if (st->done) return;
-  long n = (long)d.nx * d.ny * d.nz;
-  float al = (float)st->alpha_new;
-  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const BicgPartials P = bicg_partials(partials);
+  const BicgTop top = bicg_top(P, nb, it, tol, atol);
+  double rq[1];
+  sum_partials<1>(P.rq, nb, rq);
+  const float al = (float)(top.rho_new / rq[0]);
+  const long n = (long)d.nx * d.ny * d.nz;
   double acc[3] = {0.0, 0.0, 0.0};
+  auto Sv = [&](long o) { return r[o] - al * q[o]; };
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    int k = t % nz;
-    long jt = t / nz;
-    int j = jt % ny, i = jt / ny;
-    auto S = [&](int a, int b2, int c) {
-      a = (a + nx) % nx;
-      b2 = (b2 + ny) % ny;
-      c = c < 0 ? 0 : (c >= nz ? nz - 1 : c);
-      long o = ((long)a * ny + b2) * nz + c;
-      return r[o] - al * q[o];
-    };
-    long cc = ic(d, i + 3, j + 3, k + 1);
-    const long sx = d.sxc(), sy = d.syc();
-    float xc = S(i, j, k);
-    float px0 = 0.5f * (rtt[cc] + rtt[cc - sx]) * ((xc - S(i - 1, j, k)) * d.idx);
-    float px1 = 0.5f * (rtt[cc + sx] + rtt[cc]) * ((S(i + 1, j, k) - xc) * d.idx);
-    float py0 = 0.5f * (rtt[cc] + rtt[cc - sy]) * ((xc - S(i, j - 1, k)) * d.idy);
-    float py1 = 0.5f * (rtt[cc + sy] + rtt[cc]) * ((S(i, j + 1, k) - xc) * d.idy);
-    float pz0 = (k == 0) ? rtt[cc - 1] : 0.5f * (rtt[cc] + rtt[cc - 1]) * ((xc - S(i, j, k - 1)) * d.idz);
-    float pz1 = (k == nz - 1) ? rtt[cc + 1] : 0.5f * (rtt[cc + 1] + rtt[cc]) * ((S(i, j, k + 1) - xc) * d.idz);
-    float tt = (float)kCp * ((px1 - px0) * d.idx + (py1 - py0) * d.idy + (pz1 - pz0) * d.idz);
+    int i, j, k;
+    cell_of(d, t, i, j, k);
+    float xc;
+    const float tt = matvec_at(d, rtt, i, j, k, t, Sv, xc);
     s[t] = xc;
     tv[t] = tt;
     acc[0] += (double)tt * xc;
     acc[1] += (double)tt * tt;
     acc[2] += (double)xc * xc;
   }
-  double part[3];
-  block_sum<3>(acc, part);
-  if (publish_partials<3>(part, partials, &st->counter[2])) {
-    double tot[3];
-    final_sums<3>(partials, tot);
-    if (threadIdx.x == 0) {
-      st->omega_new = tot[0] / tot[1];
-      st->ss = tot[2];
-      st->early = tot[2] < st->atol2;
-      st->counter[2] = 0;
-    }
-  }
+  put_partials<3>(acc, P.ts);
 }
 
-// K3: x, r updates; sums r.r, rhat.r; scalar recurrence and the loop test
+// K3: omega = t.s / t.t; x, r updates (early exit when |s|^2 < atol^2); partials r.r,
+// rhat.r for the next iteration; block 0 advances the recurrence scalars
 __global__ void __launch_bounds__(kBicgThreads) k_bicg_xr(long n, const float* __restrict__ p,
                                                          const float* __restrict__ s, const float* __restrict__ tv,
                                                          const float* __restrict__ rhat, float* __restrict__ x,
                                                          float* __restrict__ r, double* __restrict__ partials,
-                                                         BicgState* st) {
+                                                         BicgState* st, int it, int nb, double tol, double atol) {
   if (st->done) return;
-  float al = (float)st->alpha_new, om = (float)st->omega_new;
-  int early = st->early;
+  const BicgPartials P = bicg_partials(partials);
+  const BicgTop top = bicg_top(P, nb, it, tol, atol);
+  double rq[1], ts[3];
+  sum_partials<1>(P.rq, nb, rq);
+  sum_partials<3>(P.ts, nb, ts);
+  const double alpha_new = top.rho_new / rq[0];
+  const double omega_new = ts[0] / ts[1];
+  const bool early = ts[2] < top.atol2;
+  const float al = (float)alpha_new, om = (float)omega_new;
   double acc[2] = {0.0, 0.0};
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
     float rn;
@@ -348,24 +380,26 @@ __global__ void __launch_bounds__(kBicgThreads) k_bicg_xr(long n, const float* _
     acc[0] += (double)rn * rn;
     acc[1] += (double)rhat[t] * rn;
   }
-  double part[2];
-  block_sum<2>(acc, part);
-  if (publish_partials<2>(part, partials, &st->counter[3])) {
-    double tot[2];
-    final_sums<2>(partials, tot);
-    if (threadIdx.x == 0) {
-      double a = st->alpha_new, o = st->omega_new, rn = st->rho_new;
-      int kn = (o == 0.0 || a == 0.0) ? -11 : st->k + 1;
-      if (rn == 0.0) kn = -10;
-      st->k = kn;
-      st->rho = rn;
-      st->alpha = a;
-      st->omega = o;
-      st->rr = tot[0];
-      st->rho_new = tot[1];
-      st->done = !(st->rr > st->atol2 && kn < st->maxiter && kn >= 0);
-      st->counter[3] = 0;
-    }
+  put_partials<2>(acc, P.rr[it & 1]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const BicgScalars sc = st->s[it & 1];
+    int kn = (omega_new == 0.0 || alpha_new == 0.0) ? -11 : sc.k + 1;
+    if (top.rho_new == 0.0) kn = -10;
+    BicgScalars nx;
+    nx.rho = top.rho_new;
+    nx.alpha = alpha_new;
+    nx.omega = omega_new;
+    nx.k = kn;
+    nx.pad = 0;
+    st->s[(it + 1) & 1] = nx;
+  }
+}
+
+// after the host's loop: the exit code when no kernel ended the loop (maxiter reached)
+__global__ void k_bicg_finish(BicgState* st, int iters) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && !st->done) {
+    st->done = 1;
+    st->k_final = st->s[iters & 1].k;
   }
 }
 
