@@ -97,103 +97,67 @@ __device__ float weno3_grad(float a, float b, float c, float weps, float* g, flo
   return R;
 }
 
-// hrecon (lex_common.hpp) minus ref, with its gradient w.r.t. q[m-3+t], t = 0..5
+// hrecon (lex_common.hpp) minus ref, with its gradient w.r.t. q[m-3+t], t = 0..5.
+// Branch-free in the upwind side (lanes run along z, where the mass-flux sign varies, so a
+// branch would run both WENO5 gradients): the operands are selected, one weno5_grad is
+// evaluated, and its derivatives are mapped back by the same selection.
 __device__ float hrecon_grad(const float* q6, bool pos, float weps, float* g6, float ref = 0.f) {
   float g[5];
-  float R;
-  for (int t = 0; t < 6; ++t) g6[t] = 0.f;
-  if (pos) {
-    R = weno5_grad(q6[0], q6[1], q6[2], q6[3], q6[4], weps, g, ref);
-    for (int t = 0; t < 5; ++t) g6[t] = g[t];
-  } else {
-    R = weno5_grad(q6[5], q6[4], q6[3], q6[2], q6[1], weps, g, ref);
-    for (int t = 0; t < 5; ++t) g6[5 - t] = g[t];
+  const float R = weno5_grad(pos ? q6[0] : q6[5], pos ? q6[1] : q6[4], pos ? q6[2] : q6[3], pos ? q6[3] : q6[2],
+                             pos ? q6[4] : q6[1], weps, g, ref);
+  g6[0] = pos ? g[0] : 0.f;
+  g6[1] = pos ? g[1] : g[4];
+  g6[2] = pos ? g[2] : g[3];
+  g6[3] = pos ? g[3] : g[2];
+  g6[4] = pos ? g[4] : g[1];
+  g6[5] = pos ? 0.f : g[0];
+  return R;
+}
+
+// vrecon (lex_common.hpp) minus ref, with its gradient w.r.t. p[f-2+t], t = 0..5, from
+// the six values q[t] = p[f-2+t] (indexes clamped to the column: the rows that reach
+// past an end never use those entries).  Branch-free in the upwind side like
+// hrecon_grad; the table's near-boundary rows are patched in a branch that only waves
+// holding a face with f <= 2 or f >= L-4 enter.
+__device__ float vrecon_grad6(const float* q, int L, int f, bool pos, float weps, float* g6, float ref = 0.f) {
+  float R = hrecon_grad(q, pos, weps, g6, ref);
+  if (f <= 2 || f >= L - 4) {
+    // upward rows f = 2, L-3: weno3(p[f-1], p[f], p[f+1]) = (q1, q2, q3); downward rows
+    // f = 1, L-4: weno3(p[f+2], p[f+1], p[f]) = (q4, q3, q2); centred rows (upward f = 1,
+    // downward f = L-3): 0.5 (p[f] + p[f+1])
+    float h[3];
+    const float R3 = weno3_grad(pos ? q[1] : q[4], pos ? q[2] : q[3], pos ? q[3] : q[2], weps, h, ref);
+    const bool w3 = pos ? (f == 2 || f == L - 3) : (f == 1 || f == L - 4);
+    const bool cen = pos ? (f == 1) : (f == L - 3);
+    const bool zero = f <= 0 || f >= L - 2;
+    if (w3) {
+      R = R3;
+      g6[0] = 0.f;
+      g6[1] = pos ? h[0] : 0.f;
+      g6[2] = pos ? h[1] : h[2];
+      g6[3] = pos ? h[2] : h[1];
+      g6[4] = pos ? 0.f : h[0];
+      g6[5] = 0.f;
+    } else if (cen) {
+      R = 0.5f * ((q[2] - ref) + (q[3] - ref));
+      g6[0] = g6[1] = g6[4] = g6[5] = 0.f;
+      g6[2] = g6[3] = 0.5f;
+    }
+    if (zero) {
+      R = -ref;
+      for (int t = 0; t < 6; ++t) g6[t] = 0.f;
+    }
   }
   return R;
 }
 
-// vrecon (lex_common.hpp) minus ref, with its gradient w.r.t. p[f-2+t], t = 0..5
+// vrecon_grad6 on a unit-stride column p[0..L-1]
 __device__ float vrecon_grad(const float* __restrict__ p, int L, int f, bool pos, float weps, float* g6,
                             float ref = 0.f) {
-  for (int t = 0; t < 6; ++t) g6[t] = 0.f;
-  if (f <= 0 || f >= L - 2) return -ref;
-  float g[5];
-  float R;
-  if (pos) {
-    if (f == 1) {  // 0.5 (p1 + p2)
-      g6[2] = 0.5f;
-      g6[3] = 0.5f;
-      return 0.5f * ((p[1] - ref) + (p[2] - ref));
-    }
-    if (f == 2 || f == L - 3) {  // weno3(p[f-1], p[f], p[f+1])
-      R = weno3_grad(p[f - 1], p[f], p[f + 1], weps, g, ref);
-      g6[1] = g[0]; g6[2] = g[1]; g6[3] = g[2];
-      return R;
-    }
-    R = weno5_grad(p[f - 2], p[f - 1], p[f], p[f + 1], p[f + 2], weps, g, ref);
-    for (int t = 0; t < 5; ++t) g6[t] = g[t];
-    return R;
-  }
-  if (f == 1) {  // weno3(p3, p2, p1)
-    R = weno3_grad(p[3], p[2], p[1], weps, g, ref);
-    g6[4] = g[0]; g6[3] = g[1]; g6[2] = g[2];
-    return R;
-  }
-  if (f == L - 3) {  // 0.5 (p[f+1] + p[f])
-    g6[3] = 0.5f;
-    g6[2] = 0.5f;
-    return 0.5f * ((p[L - 2] - ref) + (p[L - 3] - ref));
-  }
-  if (f == L - 4) {  // weno3(p[f+2], p[f+1], p[f])
-    R = weno3_grad(p[f + 2], p[f + 1], p[f], weps, g, ref);
-    g6[4] = g[0]; g6[3] = g[1]; g6[2] = g[2];
-    return R;
-  }
-  R = weno5_grad(p[f + 3], p[f + 2], p[f + 1], p[f], p[f - 1], weps, g, ref);
-  g6[5] = g[0]; g6[4] = g[1]; g6[3] = g[2]; g6[2] = g[3]; g6[1] = g[4];
-  return R;
-}
-
-// vrecon_grad on preloaded values p6[t] = p[f-2+t] (entries outside the column unused), so
-// the caller can issue the column loads together with its other loads
-__device__ float vrecon_grad6(const float* p6, int L, int f, bool pos, float weps, float* g6, float ref = 0.f) {
-  for (int t = 0; t < 6; ++t) g6[t] = 0.f;
-  if (f <= 0 || f >= L - 2) return -ref;
-  float g[5];
-  float R;
-  if (pos) {
-    if (f == 1) {  // 0.5 (p1 + p2)
-      g6[2] = 0.5f;
-      g6[3] = 0.5f;
-      return 0.5f * ((p6[2] - ref) + (p6[3] - ref));
-    }
-    if (f == 2 || f == L - 3) {  // weno3(p[f-1], p[f], p[f+1])
-      R = weno3_grad(p6[1], p6[2], p6[3], weps, g, ref);
-      g6[1] = g[0]; g6[2] = g[1]; g6[3] = g[2];
-      return R;
-    }
-    R = weno5_grad(p6[0], p6[1], p6[2], p6[3], p6[4], weps, g, ref);
-    for (int t = 0; t < 5; ++t) g6[t] = g[t];
-    return R;
-  }
-  if (f == 1) {  // weno3(p3, p2, p1)
-    R = weno3_grad(p6[4], p6[3], p6[2], weps, g, ref);
-    g6[4] = g[0]; g6[3] = g[1]; g6[2] = g[2];
-    return R;
-  }
-  if (f == L - 3) {  // 0.5 (p[f+1] + p[f])
-    g6[3] = 0.5f;
-    g6[2] = 0.5f;
-    return 0.5f * ((p6[3] - ref) + (p6[2] - ref));
-  }
-  if (f == L - 4) {  // weno3(p[f+2], p[f+1], p[f])
-    R = weno3_grad(p6[4], p6[3], p6[2], weps, g, ref);
-    g6[4] = g[0]; g6[3] = g[1]; g6[2] = g[2];
-    return R;
-  }
-  R = weno5_grad(p6[5], p6[4], p6[3], p6[2], p6[1], weps, g, ref);
-  g6[5] = g[0]; g6[4] = g[1]; g6[3] = g[2]; g6[2] = g[3]; g6[1] = g[4];
-  return R;
+  float q[6];
+#pragma unroll
+  for (int t = 0; t < 6; ++t) q[t] = p[min(max(f - 2 + t, 0), L - 1)];
+  return vrecon_grad6(q, L, f, pos, weps, g6, ref);
 }
 
 __device__ __forceinline__ void scatter6(float* __restrict__ lq, long base, long stride, const float* g6, float coef) {
