@@ -79,6 +79,17 @@ int lex_pgf(const lex_grid_t* g, const float* pip, const float* theta_rho, float
             void* stream);
 /* calculate_coriolis_force (pressure_gradient_coriolis.py:80-95) -> fu (nx,ny+1,nz), fv (nx+1,ny,nz) */
 int lex_coriolis(const lex_grid_t* g, const float* u, const float* v, float fcor, float* fu, float* fv, void* stream);
+/* calculate_coriolis_force_with_lspgrad (pressure_gradient_coriolis.py:98-113):
+ * fu = f (mean of u - u_geo), fv = f (mean of v - v_geo); the reference fixes
+ * (u_geo, v_geo) = (1, 0) m/s */
+int lex_coriolis_lspgrad(const lex_grid_t* g, const float* u, const float* v, float fcor, float u_geo, float v_geo,
+                         float* fu, float* fv, void* stream);
+/* the momentum right-hand sides of a stage (one_step_integration.py:126-133):
+ * rhs_u = adv4u + fv + sgs_u (u points), rhs_v = adv4v - fu + sgs_v (v points),
+ * rhs_w = adv4w + b8w + sgs_w (w points); fu/fv, b8w and the sgs terms may be NULL */
+int lex_momentum_rhs(const lex_grid_t* g, const float* adv4u, const float* adv4v, const float* adv4w,
+                     const float* fu, const float* fv, const float* b8w, const float* sgs_u, const float* sgs_v,
+                     const float* sgs_w, float* rhs_u, float* rhs_v, float* rhs_w, void* stream);
 
 /* ---- P3/P4: pressure equation ---------------------------------------------------- */
 /* calculate_rtt (pressure_equations.py:175-195); buoyancy interior -> padded rtt */
@@ -209,6 +220,11 @@ typedef struct lex_step_opts {
                         from the stage-0 state, kept for the step (one_step_integration.py:37-52):
                         stress on the bottom u / v faces, sen/Cp and evaporation through the
                         bottom theta / qv faces; needs surface_t in the base state */
+  int cor;           /* Coriolis (cor_opt, one_step_integration.py:83-89): 0 off, 1 f-plane
+                        (pressure_gradient_coriolis.py:80-95), 2 f-plane with a large-scale
+                        pressure gradient balancing the geostrophic wind (:98-113) */
+  float fcor;        /* Coriolis parameter (namelist fCor, 1e-4 / s) */
+  float u_geo, v_geo;/* geostrophic wind for cor 2 (the reference's constants: 1, 0 m/s) */
 } lex_step_opts_t;
 
 typedef struct lex_base_cols {

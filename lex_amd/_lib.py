@@ -31,7 +31,7 @@ class StepOpts(C.Structure):
     _fields_ = [("dt", C.c_float), ("n_sound", C.c_int), ("turb", C.c_int), ("cs", C.c_float),
                 ("prandtl", C.c_float), ("n_passive", C.c_int), ("impl", C.c_int), ("solver", C.c_int),
                 ("damp", C.c_int), ("rad", C.c_int), ("z_damping", C.c_float), ("rd_alpha", C.c_float),
-                ("sfc", C.c_int)]
+                ("sfc", C.c_int), ("cor", C.c_int), ("fcor", C.c_float), ("u_geo", C.c_float), ("v_geo", C.c_float)]
 
 
 class BaseCols(C.Structure):
@@ -78,6 +78,8 @@ _SIGS = {
     "lex_theta_rho": [C.c_long, _P, _P, _P, _P],
     "lex_pgf": [_G, _P, _P, _P, _P, _P, _P],
     "lex_coriolis": [_G, _P, _P, _F, _P, _P, _P],
+    "lex_coriolis_lspgrad": [_G, _P, _P, _F, _F, _F, _P, _P, _P],
+    "lex_momentum_rhs": [_G] + [_P] * 12 + [_P],
     "lex_rtt": [_G, _P, _P, _P, _P, _P, _P],
     "lex_p_rhs": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "lex_p_matvec": [_G, _P, _P, _P, _P],

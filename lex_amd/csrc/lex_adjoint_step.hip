@@ -174,10 +174,13 @@ void rk3_forward_tape(const Dims& d, const O& o, const B& b, float* const* X0, c
     const bool fused = nzl * kFTY <= 1024 && nsc >= 2 && nsc <= 6;
     if (fused) {
       dim3 fb(nzl, kFTY, 1), fg((d.nx + kFTX - 1) / kFTX, (d.ny + kFTY - 1) / kFTY, 1);
+      // the taped forward covers cor_opt 0 (its reverse has no Coriolis term; the entry
+      // points reject cor != 0)
+      const CorOpt nocor{0, 0.0f, 0.0f, 0.0f};
       hipLaunchKernelGGL(k_mom_u, fg, fb, mom_uv_lds(d), st, d, rc, src[1], src[2], src[3],
-                         turb ? (const float*)su : nullptr, (const float*)nullptr, ru);
+                         turb ? (const float*)su : nullptr, (const float*)nullptr, nocor, ru);
       hipLaunchKernelGGL(k_mom_v, fg, fb, mom_uv_lds(d), st, d, rc, src[1], src[2], src[3],
-                         turb ? (const float*)sv : nullptr, (const float*)nullptr, rv);
+                         turb ? (const float*)sv : nullptr, (const float*)nullptr, nocor, rv);
       hipLaunchKernelGGL(k_mom_w, fg, fb, mom_w_lds(d), st, d, rc, b.theta0, b.qv0, src[1], src[2], src[3], src[0],
                          src[5], turb ? (const float*)sw : nullptr, rw);
       ScalarSet S;
@@ -205,7 +208,7 @@ void rk3_forward_tape(const Dims& d, const O& o, const B& b, float* const* X0, c
                          fd);
       hipLaunchKernelGGL(k_rhs_uv, cx.G(d.nz, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, src[1], src[2], src[3], fd,
                          turb ? (const float*)su : nullptr, turb ? (const float*)sv : nullptr, (const float*)nullptr,
-                         (const float*)nullptr, ru, rv);
+                         (const float*)nullptr, CorOpt{0, 0.0f, 0.0f, 0.0f}, ru, rv);
       hipLaunchKernelGGL(k_rhs_w, cx.G(d.nz + 1, d.ny, d.nx), cx.bk, 0, st, d, rc, b.theta0, b.qv0, src[1], src[2],
                          src[3], src[0], src[5], fd, turb ? (const float*)sw : nullptr, rw);
       long n = (long)L.nc;

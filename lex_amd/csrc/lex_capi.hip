@@ -79,7 +79,7 @@ inline dim3 grd(int nkz, int nj, int ni) { return dim3((nkz + 63) / 64, (nj + 3)
 extern "C" {
 
 const char* lex_last_error(void) { return g_err.c_str(); }
-int lex_abi_version(void) { return 1; }
+int lex_abi_version(void) { return 2; }
 
 // ---------------------------------- H1 -------------------------------------------
 int lex_pad(int mode, int ni, int nj, int nk, int pad, const float* in, float* out, void* stream) {
@@ -190,12 +190,36 @@ int lex_pgf(const lex_grid_t* g, const float* p, const float* tr, float* pu, flo
   return check_launch("lex_pgf");
 }
 
-int lex_coriolis(const lex_grid_t* g, const float* u, const float* v, float fcor, float* fu, float* fv, void* stream) {
+int lex_coriolis(const lex_grid_t* g, const float* u, const float* v, float fcor, float* fu, float* fv,
+                 void* stream) {
   REQGRID(g);
   REQ(u); REQ(v); REQ(fu); REQ(fv);
   Dims d = mkdims(g);
-  hipLaunchKernelGGL(k_coriolis, grd(d.nz, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, u, v, fu, fv, fcor);
+  const CorOpt c{1, fcor, 0.0f, 0.0f};
+  hipLaunchKernelGGL(k_coriolis, grd(d.nz, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, u, v, fu, fv, c);
   return check_launch("lex_coriolis");
+}
+
+int lex_coriolis_lspgrad(const lex_grid_t* g, const float* u, const float* v, float fcor, float u_geo, float v_geo,
+                         float* fu, float* fv, void* stream) {
+  REQGRID(g);
+  REQ(u); REQ(v); REQ(fu); REQ(fv);
+  Dims d = mkdims(g);
+  const CorOpt c{2, fcor, u_geo, v_geo};
+  hipLaunchKernelGGL(k_coriolis, grd(d.nz, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, u, v, fu, fv, c);
+  return check_launch("lex_coriolis_lspgrad");
+}
+
+int lex_momentum_rhs(const lex_grid_t* g, const float* au, const float* av, const float* aw, const float* fu,
+                     const float* fv, const float* b8w, const float* su, const float* sv, const float* sw,
+                     float* ru, float* rv, float* rw, void* stream) {
+  REQGRID(g);
+  REQ(au); REQ(av); REQ(aw); REQ(ru); REQ(rv); REQ(rw);
+  if ((fu == nullptr) != (fv == nullptr)) return fail(-2, "lex_momentum_rhs: fu and fv must both be given or both NULL%s");
+  Dims d = mkdims(g);
+  hipLaunchKernelGGL(k_momentum_rhs, grd(d.nz + 1, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, au, av, aw, fu, fv,
+                     b8w, su, sv, sw, ru, rv, rw);
+  return check_launch("lex_momentum_rhs");
 }
 
 // ------------------------------- P3/P4 -------------------------------------------
@@ -597,6 +621,7 @@ int lex_rk3_step(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_c
                  void* ws, size_t ws_bytes, void* stream) {
   REQ(o);
   if (o->turb < 0 || o->turb > 2) return fail(-1, "lex_rk3_step: turb must be 0, 1 or 2%s");
+  if (o->cor < 0 || o->cor > 2) return fail(-1, "lex_rk3_step: cor must be 0, 1 or 2%s");
   return rk3_step_entry(g, o, b, state, ws, ws_bytes, stream);
 }
 
@@ -806,7 +831,7 @@ int lex_rk3_step_vjp(const lex_grid_t* g, const lex_step_opts_t* o, const lex_ba
       return fail(-1, "lex_rk3_step_vjp: lam_in must not alias lam_out or the state%s");
   }
   if (o->solver == 3) return fail(-1, "lex_rk3_step_vjp: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
-  if (o->damp || o->rad || o->sfc || o->turb == 2) return fail(-1, "lex_rk3_step_vjp: the adjoint covers damp_opt, rad_opt, sfc_opt and turb_opt 2 off%s");
+  if (o->damp || o->rad || o->sfc || o->turb == 2 || o->cor) return fail(-1, "lex_rk3_step_vjp: the adjoint covers damp_opt, rad_opt, sfc_opt, cor_opt and turb_opt 2 off%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (ws_bytes < L.total) return fail(-3, "lex_rk3_step_vjp: workspace too small%s");
@@ -840,7 +865,7 @@ int lex_rk3_step_tape(const lex_grid_t* g, const lex_step_opts_t* o, const lex_b
     if (state[q] == state_out[q]) return fail(-1, "lex_rk3_step_tape: state_out must not alias state%s");
   }
   if (o->solver == 3) return fail(-1, "lex_rk3_step_tape: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
-  if (o->damp || o->rad || o->sfc || o->turb == 2) return fail(-1, "lex_rk3_step_tape: the adjoint covers damp_opt, rad_opt, sfc_opt and turb_opt 2 off%s");
+  if (o->damp || o->rad || o->sfc || o->turb == 2 || o->cor) return fail(-1, "lex_rk3_step_tape: the adjoint covers damp_opt, rad_opt, sfc_opt, cor_opt and turb_opt 2 off%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_tape: tape too small%s");
@@ -862,7 +887,7 @@ int lex_rk3_step_vjp_taped(const lex_grid_t* g, const lex_step_opts_t* o, const 
       return fail(-1, "lex_rk3_step_vjp_taped: lam_in must not alias lam_out or the state%s");
   }
   if (o->solver == 3) return fail(-1, "lex_rk3_step_vjp_taped: the adjoint covers solver_opt 2 (explicit acoustic substeps)%s");
-  if (o->damp || o->rad || o->sfc || o->turb == 2) return fail(-1, "lex_rk3_step_vjp_taped: the adjoint covers damp_opt, rad_opt, sfc_opt and turb_opt 2 off%s");
+  if (o->damp || o->rad || o->sfc || o->turb == 2 || o->cor) return fail(-1, "lex_rk3_step_vjp_taped: the adjoint covers damp_opt, rad_opt, sfc_opt, cor_opt and turb_opt 2 off%s");
   Dims d = mkdims(g);
   VjpLayout L = vjp_layout(d, (double)o->dt, o->n_sound, o->n_passive);
   if (tape_bytes < L.off_lXA) return fail(-3, "lex_rk3_step_vjp_taped: tape too small%s");

@@ -161,6 +161,19 @@ __device__ __forceinline__ int lane_level(int tz, int nzl, int nz) {
   return k < nz - 32 ? k : nz + (tz - 64);
 }
 
+// Coriolis force of a stage (pressure_gradient_coriolis.py:80-113): mode 0 off, 1
+// calculate_coriolis_force (f times the four-point mean), 2 ..._with_lspgrad (a
+// large-scale pressure gradient in balance with the geostrophic wind (ug, vg) is
+// subtracted: f (mean - geostrophic component)).  `s4` is the four-point sum in the
+// reference's order; the arithmetic follows the reference's expressions.
+struct CorOpt {
+  int mode;
+  float f, ug, vg;
+  __device__ __forceinline__ float of(float s4, float geo) const {
+    return mode == 2 ? f * (0.25f * s4 - geo) : (f * 0.25f) * s4;
+  }
+};
+
 // select(mass >= 0, mass*up, mass*dn) with only the upwind side evaluated.
 __device__ __forceinline__ bool upwind_pos(float mass) { return mass >= 0.0f; }
 

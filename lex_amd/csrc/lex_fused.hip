@@ -256,7 +256,7 @@ __device__ __forceinline__ float fd_cell(const Dims& d, const float* __restrict_
 __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
                                                  const float* __restrict__ v, const float* __restrict__ w,
                                                  const float* __restrict__ sgs, const float* __restrict__ tau_x,
-                                                 float* __restrict__ ru) {
+                                                 CorOpt cor, float* __restrict__ ru) {
   extern __shared__ float lds[];
   const float weps = 1.0e-16f;
   const int nzl = blockDim.x, tz = threadIdx.x, ty = threadIdx.y, nt = nzl * kFTY, tid = ty * nzl + tz;
@@ -333,6 +333,8 @@ __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict_
       float conv = -((fxhi - fxlo) * d.idx + (FY[(ty + 1) * nzl + k] - FY[ty * nzl + k]) * d.idy +
                      (fzhi - FZ[ty * nzl + k]) * d.idz);
       float a = (conv + uc * (0.5f * (fdlo + fdhi))) * ir;   // rho8u = 0.5 (r + r) = r
+      // + fv: f times the mean of the four v around the u point (v at I-1, I and J, J+1)
+      if (cor.mode) a = a + cor.of(((vA + vC) + vB) + vD, cor.vg);
       if (sgs) a = a + sgv;
       ru[((long)i * d.ny + j) * d.nz + k] = a;
       if (i == 0) ru[((long)d.nx * d.ny + j) * d.nz + k] = a;
@@ -353,7 +355,7 @@ __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict_
 __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
                                                  const float* __restrict__ v, const float* __restrict__ w,
                                                  const float* __restrict__ sgs, const float* __restrict__ tau_y,
-                                                 float* __restrict__ rv) {
+                                                 CorOpt cor, float* __restrict__ rv) {
   extern __shared__ float lds[];
   const float weps = 1.0e-16f;
   const int nzl = blockDim.x, tz = threadIdx.x, ty = threadIdx.y, nt = nzl * kFTY, tid = ty * nzl + tz;
@@ -375,13 +377,16 @@ __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict_
   const PlaneSlots psl = plane_slots(tid, nt, Lz, j0, d.ny + 6, (int)d.syc());
   float win[6];
   float fxlo;
+  float uw0, uw1;   // u at (I, J-1) and (I, J): the west corners of the v point
   {
     const int I = i0 + 3;
     long b = iv(d, I, Jc, Kc);
     float m3 = v[b - 3 * sx];
 #pragma unroll
     for (int t = 0; t < 6; ++t) win[t] = v[b + (t - 2) * sx];
-    float mass = 0.5f * (r * u[iu(d, I, Jc - 1, Kc)] + r * u[iu(d, I, Jc, Kc)]);
+    uw0 = u[iu(d, I, Jc - 1, Kc)];
+    uw1 = u[iu(d, I, Jc, Kc)];
+    float mass = 0.5f * (r * uw0 + r * uw1);
     fxlo = mass * hrecon(m3, win[0], win[1], win[2], win[3], win[4], upwind_pos(mass), weps);
   }
   PlaneRegs pr;
@@ -389,7 +394,8 @@ __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict_
   for (int i = i0; i < iend; ++i) {
     const int I = i + 3;
     const float vc = win[2];
-    const float mx = 0.5f * (r * u[iu(d, I + 1, Jc - 1, Kc)] + r * u[iu(d, I + 1, Jc, Kc)]);
+    const float ue0 = u[iu(d, I + 1, Jc - 1, Kc)], ue1 = u[iu(d, I + 1, Jc, Kc)];
+    const float mx = 0.5f * (r * ue0 + r * ue1);
     const float fxhi = mx * hrecon(win[0], win[1], win[2], win[3], win[4], win[5], upwind_pos(mx), weps);
     const float fd0 = fd_cell(d, rc, u, v, w, I, Jc - 1, Kc);
     const float fd1 = fd_cell(d, rc, u, v, w, I, Jc, Kc);
@@ -426,11 +432,15 @@ __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict_
       float conv = -((fxhi - fxlo) * d.idx + (FY[(ty + 1) * nzl + k] - FY[ty * nzl + k]) * d.idy +
                      (fzhi - FZ[ty * nzl + k]) * d.idz);
       float a = (conv + vc * (0.5f * (fd0 + fd1))) * ir;     // rho8v = 0.5 (r + r) = r
+      // - fu: f times the mean of the four u around the v point (u at I, I+1 and J-1, J)
+      if (cor.mode) a = a - cor.of(((uw0 + uw1) + ue0) + ue1, cor.ug);
       if (sgs) a = a + sgv;
       rv[((long)i * (d.ny + 1) + j) * d.nz + k] = a;
       if (j == 0) rv[((long)i * (d.ny + 1) + d.ny) * d.nz + k] = a;
     }
     fxlo = fxhi;
+    uw0 = ue0;
+    uw1 = ue1;
 #pragma unroll
     for (int t = 0; t < 5; ++t) win[t] = win[t + 1];
     win[5] = (i + 1 < iend) ? v[iv(d, I + 4, Jc, Kc)] : 0.0f;

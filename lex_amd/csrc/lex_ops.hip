@@ -386,21 +386,60 @@ __global__ void k_pgf(Dims d, const float* __restrict__ p, const float* __restri
 // ----------------------------------------------------------------------------------
 // P2: calculate_coriolis_force (pressure_gradient_coriolis.py:80-95)
 // ----------------------------------------------------------------------------------
-__device__ __forceinline__ float cor_fv(const Dims& d, const float* __restrict__ v, int I, int J, int K, float f) {
-  return (f * 0.25f) * (v[iv(d, I - 1, J, K)] + v[iv(d, I - 1, J + 1, K)] + v[iv(d, I, J, K)] + v[iv(d, I, J + 1, K)]);
+// fv at u point (I, J, K) and fu at v point (I, J, K) (pressure_gradient_coriolis.py:80-113)
+__device__ __forceinline__ float cor_fv(const Dims& d, const float* __restrict__ v, int I, int J, int K,
+                                        const CorOpt& c) {
+  return c.of(v[iv(d, I - 1, J, K)] + v[iv(d, I - 1, J + 1, K)] + v[iv(d, I, J, K)] + v[iv(d, I, J + 1, K)], c.vg);
 }
-__device__ __forceinline__ float cor_fu(const Dims& d, const float* __restrict__ u, int I, int J, int K, float f) {
-  return (f * 0.25f) * (u[iu(d, I, J - 1, K)] + u[iu(d, I, J, K)] + u[iu(d, I + 1, J - 1, K)] + u[iu(d, I + 1, J, K)]);
+__device__ __forceinline__ float cor_fu(const Dims& d, const float* __restrict__ u, int I, int J, int K,
+                                        const CorOpt& c) {
+  return c.of(u[iu(d, I, J - 1, K)] + u[iu(d, I, J, K)] + u[iu(d, I + 1, J - 1, K)] + u[iu(d, I + 1, J, K)], c.ug);
 }
 
 __global__ void k_coriolis(Dims d, const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ fu,
-                           float* __restrict__ fv, float f) {
+                           float* __restrict__ fv, CorOpt c) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
   int i = blockIdx.z;                             // 0..nx
   if (k >= d.nz || j > d.ny) return;
-  if (j < d.ny) fv[((long)i * d.ny + j) * d.nz + k] = cor_fv(d, v, i + 3, j + 3, k + 1, f);
-  if (i < d.nx) fu[((long)i * (d.ny + 1) + j) * d.nz + k] = cor_fu(d, u, i + 3, j + 3, k + 1, f);
+  if (j < d.ny) fv[((long)i * d.ny + j) * d.nz + k] = cor_fv(d, v, i + 3, j + 3, k + 1, c);
+  if (i < d.nx) fu[((long)i * (d.ny + 1) + j) * d.nz + k] = cor_fu(d, u, i + 3, j + 3, k + 1, c);
+}
+
+// momentum right-hand sides of the staged stage (one_step_integration.py:126-133):
+// rhs_u = adv4u + fv + sgs_u, rhs_v = adv4v - fu + sgs_v, rhs_w = adv4w + b8w + sgs_w,
+// each optional term NULL when absent.  Thread (k, j, i) over (nz+1, ny+1, nx+1).
+__global__ void k_momentum_rhs(Dims d, const float* __restrict__ au, const float* __restrict__ av,
+                               const float* __restrict__ aw, const float* __restrict__ fu,
+                               const float* __restrict__ fv, const float* __restrict__ b8w,
+                               const float* __restrict__ su, const float* __restrict__ sv,
+                               const float* __restrict__ sw, float* __restrict__ ru, float* __restrict__ rv,
+                               float* __restrict__ rw) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nz
+  int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
+  int i = blockIdx.z;                             // 0..nx
+  if (k > d.nz || j > d.ny) return;
+  if (k < d.nz && j < d.ny) {   // u points (nx+1, ny, nz)
+    const long o = ((long)i * d.ny + j) * d.nz + k;
+    float a = au[o];
+    if (fv) a = a + fv[o];
+    if (su) a = a + su[o];
+    ru[o] = a;
+  }
+  if (k < d.nz && i < d.nx) {   // v points (nx, ny+1, nz)
+    const long o = ((long)i * (d.ny + 1) + j) * d.nz + k;
+    float a = av[o];
+    if (fu) a = a - fu[o];
+    if (sv) a = a + sv[o];
+    rv[o] = a;
+  }
+  if (i < d.nx && j < d.ny) {   // w points (nx, ny, nz+1)
+    const long o = ((long)i * d.ny + j) * (d.nz + 1) + k;
+    float a = aw[o];
+    if (b8w) a = a + b8w[o];
+    if (sw) a = a + sw[o];
+    rw[o] = a;
+  }
 }
 
 // ----------------------------------------------------------------------------------

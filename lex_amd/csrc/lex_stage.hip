@@ -15,14 +15,14 @@
 namespace lex {
 
 // momentum right-hand sides rhs_u = adv4u + fv + sgs_u, rhs_v = adv4v - fu + sgs_v,
-// rhs_w = adv4w + b8w + sgs_w (one_step_integration.py:126-133); Coriolis is off in
-// every configuration on this path (cor_opt 0), sgs may be NULL.
+// rhs_w = adv4w + b8w + sgs_w (one_step_integration.py:126-133); fu, fv from the stage's
+// u, v (cor_opt 1 or 2, off with cor.mode 0), sgs may be NULL.
 // tau_x, tau_y: surface stresses (nx, ny) averaged onto the bottom u / v faces, or null.
 __global__ void k_rhs_uv(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
                          const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ fd,
                          const float* __restrict__ su, const float* __restrict__ sv,
-                         const float* __restrict__ tau_x, const float* __restrict__ tau_y, float* __restrict__ ru,
-                         float* __restrict__ rv) {
+                         const float* __restrict__ tau_x, const float* __restrict__ tau_y, CorOpt cor,
+                         float* __restrict__ ru, float* __restrict__ rv) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
   int i = blockIdx.z;                             // 0..nx
@@ -34,7 +34,7 @@ __global__ void k_rhs_uv(Dims d, const float* __restrict__ rc, const float* __re
     float tf = 0.0f;
     if (tau_x) tf = 0.5f * (tau_x[(long)((i - 1 + d.nx) % d.nx) * d.ny + j] + tau_x[(long)(i % d.nx) * d.ny + j]);
     float a = adv_u_point(d, rc, u, v, w, fd[ic(d, I - 1, J, K)], fd[ic(d, I, J, K)], tf, i, j, k, weps);
-    a = a + 0.0f;  // + fv (zero)
+    if (cor.mode) a = a + cor_fv(d, v, I, J, K, cor);
     if (su) a = a + su[o];
     ru[o] = a;
   }
@@ -43,7 +43,7 @@ __global__ void k_rhs_uv(Dims d, const float* __restrict__ rc, const float* __re
     float tf = 0.0f;
     if (tau_y) tf = 0.5f * (tau_y[(long)i * d.ny + (j - 1 + d.ny) % d.ny] + tau_y[(long)i * d.ny + j % d.ny]);
     float a = adv_v_point(d, rc, u, v, w, fd[ic(d, I, J - 1, K)], fd[ic(d, I, J, K)], tf, i, j, k, weps);
-    a = a - 0.0f;  // - fu (zero)
+    if (cor.mode) a = a - cor_fu(d, u, I, J, K, cor);
     if (sv) a = a + sv[o];
     rv[o] = a;
   }
@@ -292,6 +292,7 @@ template <class O, class B>
 int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* ws, const Rk3Layout& L,
              hipStream_t st) {
   const int nf = 6 + o.n_passive;
+  const CorOpt cor{o.cor, o.fcor, o.u_geo, o.v_geo};
   float* X0[16];
   float* XA[16];
   float* XB[16];
@@ -409,15 +410,15 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
       const int nzl = 64 * ((d.nz + 63) / 64);
       dim3 fb(nzl, kFTY, 1), fg((d.nx + kFTX - 1) / kFTX, (d.ny + kFTY - 1) / kFTY, 1);
       LEX_LAUNCH(KID_STAGE_MOMENTUM, k_mom_u, fg, fb, mom_uv_lds(d), st, d, rc, src[1], src[2], src[3],
-                 turb ? (const float*)su : nullptr, tau_x, ru);
+                 turb ? (const float*)su : nullptr, tau_x, cor, ru);
       LEX_LAUNCH(KID_MOM_V, k_mom_v, fg, fb, mom_uv_lds(d), st, d, rc, src[1], src[2], src[3],
-                 turb ? (const float*)sv : nullptr, tau_y, rv);
+                 turb ? (const float*)sv : nullptr, tau_y, cor, rv);
       LEX_LAUNCH(KID_MOM_W, k_mom_w, fg, fb, mom_w_lds(d), st, d, rc, b.theta0, b.qv0, src[1], src[2], src[3],
                  src[0], src[5], turb ? (const float*)sw : nullptr, rw);
     } else {
       LEX_LAUNCH(KID_DIV_RHO, k_div_rho, G(d.nz + 2, d.ny + 6, d.nx + 6), bk, 0, st, d, rc, src[1], src[2], src[3], fd);
       LEX_LAUNCH(KID_RHS_UV, k_rhs_uv, G(d.nz, d.ny + 1, d.nx + 1), bk, 0, st, d, rc, src[1], src[2], src[3], fd,
-                 turb ? su : nullptr, turb ? sv : nullptr, tau_x, tau_y, ru, rv);
+                 turb ? su : nullptr, turb ? sv : nullptr, tau_x, tau_y, cor, ru, rv);
       LEX_LAUNCH(KID_RHS_W, k_rhs_w, G(d.nz + 1, d.ny, d.nx), bk, 0, st, d, rc, b.theta0, b.qv0, src[1], src[2], src[3],
                  src[0], src[5], fd, turb ? sw : nullptr, rw);
     }

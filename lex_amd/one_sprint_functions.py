@@ -14,6 +14,7 @@ from . import _lib
 from . import boundary_conditions as bc
 from . import namelist_n_constants as nl
 from . import one_step_integration as one
+from . import pressure_gradient_coriolis as pgc
 from ._core import column, grid_from
 
 _WS: dict = {}
@@ -36,8 +37,10 @@ def fused_step(phys_state, base_state, grid, model_opt, dt=None, n_sound=None, c
     solver_opt 3 takes the vertically implicit acoustic substeps.
     """
     solver_opt, cor_opt, turb_opt = one._check_opts(model_opt, physics=True)
-    if solver_opt not in (2, 3) or cor_opt != 0:
-        raise NotImplementedError("the fused step covers solver_opt 2 and 3 without Coriolis")
+    if solver_opt not in (2, 3):
+        raise NotImplementedError("the fused step covers solver_opt 2 and 3")
+    if cor_opt not in (0, 1, 2):
+        raise NotImplementedError(f"cor_opt {cor_opt} does not exist in the reference")
     rho0_theta0, rho0, theta0, pi0, qv0 = base_state[:5]
     n_passive = len(phys_state) - 6
     sfc = bool(model_opt[4])
@@ -45,7 +48,7 @@ def fused_step(phys_state, base_state, grid, model_opt, dt=None, n_sound=None, c
                          int(turb_opt), float(nl.cs if cs is None else cs),
                          float(nl.prandtl if prandtl is None else prandtl), n_passive, int(impl), int(solver_opt),
                          int(bool(model_opt[1])), int(bool(model_opt[2])), float(nl.z_damping), float(nl.rd_alpha),
-                         int(sfc))
+                         int(sfc), int(cor_opt), float(nl.fCor), float(pgc.U_GEO), float(pgc.V_GEO))
     cols = [column(a) for a in (rho0, theta0, qv0, pi0, rho0_theta0)]
     if sfc:
         if len(base_state) < 6 or base_state[5] is None:
@@ -65,7 +68,7 @@ def ws_rk3_big_step(phys_state, base_state, grids, model_opt):
     Returns (phys_state, sfc_others) with new tensors, as the reference does.
     """
     solver_opt = model_opt[0]
-    if solver_opt in (2, 3) and model_opt[3] == 0:
+    if solver_opt in (2, 3):
         g = grid_from(base_state[1], *grids[3:6])
         new = tuple(a.clone() for a in phys_state)
         sfc_others = _first_stage_sfc(phys_state, base_state, g, model_opt)
@@ -108,7 +111,7 @@ def ws_rk3_sprint_dl(phys_state, base_state, grids, model_opt, dl_params, scalin
     sfc_others = None
     with torch.no_grad():
         for _ in range(nl.sprint_n):
-            if model_opt[0] in (2, 3) and model_opt[3] == 0:
+            if model_opt[0] in (2, 3):
                 g = grid_from(base_state[1], *grids[3:6])
                 sfc_others = _first_stage_sfc(state, base_state, g, model_opt)
                 fused_step(state, base_state, g, model_opt)

@@ -16,7 +16,7 @@ from . import namelist_n_constants as nl
 from ._core import column, empty, grid_from
 from .advection import get_divergence, advection_u, advection_v, advection_w
 from .pressure_equations import calculate_buoyancy, calculate_rtt
-from .pressure_gradient_coriolis import calculate_coriolis_force
+from .pressure_gradient_coriolis import calculate_coriolis_force, calculate_coriolis_force_with_lspgrad
 from . import turbulence as turb
 
 WEPS_MOM = 1.0e-16    # one_step_integration.py:896
@@ -244,6 +244,17 @@ def _n_sound(stage, dt):
     return round(dt / dt_sound), dt_sound
 
 
+def momentum_rhs(g, adv4u, adv4v, adv4w, fu, fv, b8w, sgs_tend=None):
+    """The stage's momentum right-hand sides (one_step_integration.py:126-133): adv4u + fv
+    + sgs_u, adv4v - fu + sgs_v, adv4w + b8w + sgs_w; absent terms are None."""
+    ru, rv, rw = torch.empty_like(adv4u), torch.empty_like(adv4v), torch.empty_like(adv4w)
+    su, sv, sw = sgs_tend[:3] if sgs_tend is not None else (None, None, None)
+    _lib.call("lex_momentum_rhs", g, _lib.ptr(adv4u), _lib.ptr(adv4v), _lib.ptr(adv4w), _lib.ptr(fu), _lib.ptr(fv),
+              _lib.ptr(b8w), _lib.ptr(su), _lib.ptr(sv), _lib.ptr(sw), _lib.ptr(ru), _lib.ptr(rv), _lib.ptr(rw),
+              _lib.stream())
+    return ru, rv, rw
+
+
 def _stage(stage, phys_state_now, phys_state, base_state, grids, model_opt, dt, sgs_tend, want_tends=True):
     solver_opt, cor_opt, turb_opt = _check_opts(model_opt)
     rho0_theta0, rho0, theta0, pi0, qv0 = base_state[:5]
@@ -267,37 +278,28 @@ def _stage(stage, phys_state_now, phys_state, base_state, grids, model_opt, dt, 
     adv4u, adv4v, adv4w = prep_momentum_eqn(rho0, u, v, w, fd, None, None, x3d, y3d, z3d, x3d4u, y3d4v, z3d4w)
     if cor_opt == 1:
         fu, fv = calculate_coriolis_force(u, v)
-    elif cor_opt == 0:
-        fu = fv = None
+    elif cor_opt == 2:
+        fu, fv = calculate_coriolis_force_with_lspgrad(u, v)
     else:
-        raise NotImplementedError("cor_opt 2 is not on this path")
+        fu = fv = None   # the reference's zero arrays (one_step_integration.py:88-89) as NULL
     tends = None
     info = -999
     trho = _theta_rho(th, qv)
     if solver_opt == 1:
         rtt = calculate_rtt(rho0_theta0, th, qv, buoyancy)
-        zu = fu if fu is not None else torch.zeros(g.nx, g.ny + 1, g.nz, device=th.device)
-        zv = fv if fv is not None else torch.zeros(g.nx + 1, g.ny, g.nz, device=th.device)
-        pint, info = solve_pres_eqn(pip0, rho0_theta0, pi0, rtt, adv4u, adv4v, adv4w, zu, zv, buoyancy,
+        pint, info = solve_pres_eqn(pip0, rho0_theta0, pi0, rtt, adv4u, adv4v, adv4w, fu, fv, buoyancy,
                                     x3d, x3d4u, y3d, y3d4v, z3d, z3d4w)
         pip_n = padding3_array(pint)
         sg = sgs_tend[:3] if sgs_tend is not None else None
         un, vn, wn, du, dv, dw = update_momentum_eqn_euler(u0, v0, w0, pi0, pip_n, trho, adv4u, adv4v, adv4w,
                                                            fu, fv, b8w, x3d, y3d, z3d, dt, sgs=sg)
     else:
-        # rhs_u = adv4u + fv + sgs_u, ... (one_step_integration.py:126-133): folded into
-        # the acoustic kernel inputs
-        rhs_u, rhs_v, rhs_w = adv4u, adv4v, adv4w + b8w
-        if fv is not None:
-            rhs_u = rhs_u + fv
-            rhs_v = rhs_v - fu
-        if sgs_tend is not None:
-            rhs_u = rhs_u + sgs_tend[0]
-            rhs_v = rhs_v + sgs_tend[1]
-            rhs_w = rhs_w + sgs_tend[2]
+        # rhs_u = adv4u + fv + sgs_u, rhs_v = adv4v - fu + sgs_v, rhs_w = adv4w + b8w + sgs_w
+        # (one_step_integration.py:126-133), assembled on the device
+        rhs_u, rhs_v, rhs_w = momentum_rhs(g, adv4u, adv4v, adv4w, fu, fv, b8w, sgs_tend)
         n, dts = _n_sound(stage, dt)
-        un, vn, wn, pip_n = _sound(u0, v0, w0, pi0, pip0, pip, rhs_u.contiguous(), rhs_v.contiguous(),
-                                   rhs_w.contiguous(), 0.0, trho, rho0, n, dts, g, implicit=solver_opt == 3)
+        un, vn, wn, pip_n = _sound(u0, v0, w0, pi0, pip0, pip, rhs_u, rhs_v, rhs_w, 0.0, trho, rho0, n, dts, g,
+                                   implicit=solver_opt == 3)
     sg_th = sgs_tend[3] if sgs_tend is not None else None
     sg_qv = sgs_tend[4] if sgs_tend is not None else None
     thn, dth = _scalar_update(g, rho0, th0, th, u, v, w, fd, None, None, sg_th, dt, WEPS_THETA)

@@ -24,3 +24,21 @@ def calculate_coriolis_force(u, v):
     fv = empty(nx + 1, ny, nz, like=u)
     _lib.call("lex_coriolis", g, _lib.ptr(u), _lib.ptr(v), float(nl.fCor), _lib.ptr(fu), _lib.ptr(fv), _lib.stream())
     return fu, fv
+
+
+# the geostrophic wind the reference balances with its large-scale pressure gradient
+# (pressure_gradient_coriolis.py:102-111 writes the constants inline)
+U_GEO, V_GEO = 1.0, 0.0
+
+
+def calculate_coriolis_force_with_lspgrad(u, v):
+    """f-plane Coriolis minus the large-scale pressure gradient that balances the
+    geostrophic wind (1, 0) m/s: fu = f (u_bar - 1) at v points, fv = f (v_bar - 0) at u
+    points (pressure_gradient_coriolis.py:98-113)."""
+    nx, ny, nz = u.shape[0] - 7, u.shape[1] - 6, u.shape[2] - 2
+    g = _lib.grid(nx, ny, nz, 1.0, 1.0, 1.0)
+    fu = empty(nx, ny + 1, nz, like=u)
+    fv = empty(nx + 1, ny, nz, like=u)
+    _lib.call("lex_coriolis_lspgrad", g, _lib.ptr(u), _lib.ptr(v), float(nl.fCor), U_GEO, V_GEO, _lib.ptr(fu),
+              _lib.ptr(fv), _lib.stream())
+    return fu, fv

@@ -628,14 +628,29 @@ def pressure_gradient_force(pip, theta_rho, grid):
     return pu, pv, pw
 
 
+def _corner_means(u, v):
+    """0.25 x the sums of the four v around each u point and the four u around each v
+    point, in the reference's summation order (pressure_gradient_coriolis.py:82-91)."""
+    K = slice(NGZ, -NGZ)
+    sv = (v[NGX - 1:-NGX, NGY:-(NGY + 1), K] + v[NGX - 1:-NGX, NGY + 1:-NGY, K]
+          + v[NGX:-(NGX - 1), NGY:-(NGY + 1), K] + v[NGX:-(NGX - 1), NGY + 1:-NGY, K])
+    su = (u[NGX:-(NGX + 1), NGY - 1:-NGY, K] + u[NGX:-(NGX + 1), NGY:-(NGY - 1), K]
+          + u[NGX + 1:-NGX, NGY - 1:-NGY, K] + u[NGX + 1:-NGX, NGY:-(NGY - 1), K])
+    return su, sv
+
+
 def calculate_coriolis_force(u, v, f=fCor):
     """fu at v points, fv at u points (pressure_gradient_coriolis.py:80-95)."""
-    K = slice(NGZ, -NGZ)
-    fv = f * 0.25 * (v[NGX - 1:-NGX, NGY:-(NGY + 1), K] + v[NGX - 1:-NGX, NGY + 1:-NGY, K]
-                     + v[NGX:-(NGX - 1), NGY:-(NGY + 1), K] + v[NGX:-(NGX - 1), NGY + 1:-NGY, K])
-    fu = f * 0.25 * (u[NGX:-(NGX + 1), NGY - 1:-NGY, K] + u[NGX:-(NGX + 1), NGY:-(NGY - 1), K]
-                     + u[NGX + 1:-NGX, NGY - 1:-NGY, K] + u[NGX + 1:-NGX, NGY:-(NGY - 1), K])
-    return fu, fv
+    su, sv = _corner_means(u, v)
+    return f * 0.25 * su, f * 0.25 * sv
+
+
+def calculate_coriolis_force_with_lspgrad(u, v, f=fCor, u_geo=1.0, v_geo=0.0):
+    """The Coriolis force with a large-scale pressure gradient in geostrophic balance with
+    the wind (u_geo, v_geo) = (1, 0) m/s: fu = f (u_bar - u_geo), fv = f (v_bar - v_geo)
+    (pressure_gradient_coriolis.py:98-113; the reference writes the constants inline)."""
+    su, sv = _corner_means(u, v)
+    return f * (0.25 * su - u_geo), f * (0.25 * sv - v_geo)
 
 
 # ---------------------------------------------------------------------------------
@@ -1099,6 +1114,8 @@ def rk_sub_step(stage, xi0, xi, base, grid, opts, dt_stage, frozen=None, heating
     adv4w = advection_w(rho0, u, v, w, WEPS_MOM, fdiv, grid)
     if opts.cor_opt == 1:
         fu, fv = calculate_coriolis_force(u, v)
+    elif opts.cor_opt == 2:
+        fu, fv = calculate_coriolis_force_with_lspgrad(u, v)
     else:
         fu = np.zeros((nx, ny + 1, nz), rho0.dtype)
         fv = np.zeros((nx + 1, ny, nz), rho0.dtype)

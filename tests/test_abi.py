@@ -49,7 +49,7 @@ def test_abi_version_and_error_reporting():
     if not os.path.exists(_lib.LIB_PATH):
         pytest.skip("liblexhip.so not built")
     lib = _lib.load()
-    assert lib.lex_abi_version() == 1
+    assert lib.lex_abi_version() == 2
     # an argument error is reported without touching the GPU
     g = _lib.grid(2, 2, 2, 1.0, 1.0, 1.0)
     rc = lib.lex_div(g, None, None, None, None, None)
