@@ -354,6 +354,16 @@ int lex_conv3d_bf16_bwd_data(int nx, int ny, int nz, int cin, int cout, const vo
  * be NULL).  x: BF16, padded cin; g_pre: BF16, padded cout. */
 int lex_conv3d_bf16_bwd_weight(int nx, int ny, int nz, int cin, int cout, const void* x, const void* g_pre, float* dw,
                                float* db, void* stream);
+/* AutoEncoder (dl_models.py:33-68) layers in FP32, channels last: one correlation over
+ * the input dilated by lhs_dilation and zero-padded (pad_lo per axis; the high side is
+ * implied by the output extent), window stride `stride`, kernel k^3 (k = 3, or 1 for the
+ * dense layer) laid out [tap][cin][cout] like flax's kernels; bias may be NULL; relu 1
+ * applies max(x, 0).  flax Conv(stride s, 'SAME') = (stride s, dilation 1, pad_lo =
+ * total // 2); flax ConvTranspose(stride s) = (stride 1, dilation s, lax.conv_transpose's
+ * padding); nn.Dense = k 1 on the flattened input. */
+int lex_conv3d_f32(const float* x, int nix, int niy, int niz, int cin, const float* w, const float* bias, int k,
+                   int stride, int lhs_dilation, int pad_lo_x, int pad_lo_y, int pad_lo_z, int relu, float* y, int nox,
+                   int noy, int noz, int cout, void* stream);
 /* g_pre = g_post * gelu'(pre) (tanh form) or g_post when gelu == 0, as BF16 with cp
  * channels (channels >= c_real zero). */
 int lex_gelu_bwd(long ncell, int cp, int c_real, int cpre, const float* gpost, int gstride, const float* pre, int gelu,

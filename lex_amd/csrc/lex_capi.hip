@@ -15,6 +15,7 @@
 #include "lex_stage.hip"
 #include "lex_adjoint_step.hip"
 #include "lex_conv.hip"
+#include "lex_conv_f32.hip"
 #include "lex_sgs_io.hip"
 #include "lex_dl_post.hip"
 #include "lex_dl_train.hip"
@@ -1078,6 +1079,22 @@ int lex_conv3d_bf16_bwd_weight(int nx, int ny, int nz, int cin, int cout, const 
                        db);
   }
   return check_launch("lex_conv3d_bf16_bwd_weight");
+}
+
+int lex_conv3d_f32(const float* x, int nix, int niy, int niz, int cin, const float* w, const float* bias, int k,
+                   int stride, int lhs_dilation, int pad_lo_x, int pad_lo_y, int pad_lo_z, int relu, float* y, int nox,
+                   int noy, int noz, int cout, void* stream) {
+  REQ(x); REQ(w); REQ(y);
+  if (nix < 1 || niy < 1 || niz < 1 || nox < 1 || noy < 1 || noz < 1 || cin < 1 || cout < 1)
+    return fail(-1, "lex_conv3d_f32: extents must be positive%s");
+  if ((k != 1 && k != 3) || stride < 1 || lhs_dilation < 1)
+    return fail(-1, "lex_conv3d_f32: k must be 1 or 3, stride and dilation >= 1%s");
+  const long nout = (long)nox * noy * noz * cout;
+  if (nout >= (1L << 31) || (long)nix * niy * niz * cin >= (1L << 31))
+    return fail(-1, "lex_conv3d_f32: tensor too large%s");
+  ConvGeom g{nix, niy, niz, cin, nox, noy, noz, cout, k, stride, lhs_dilation, pad_lo_x, pad_lo_y, pad_lo_z};
+  hipLaunchKernelGGL(k_conv3_f32, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, S(stream), g, x, w, bias, relu, y);
+  return check_launch("lex_conv3d_f32");
 }
 
 int lex_gelu_bwd(long ncell, int cp, int c_real, int cpre, const float* gpost, int gstride, const float* pre, int gelu,

@@ -10,8 +10,10 @@ Mirrors dl_models.py of the reference for the pieces ws_rk3_sprint_dl
   ``stagger_uvw``): ``lex_dl_pack`` and ``lex_dl_stagger``.
 
 The flax parameters of the reference become the net object itself (``dl_params``).
-``Zcnn`` (:71-122) runs on the same kernels with periodic x/y padding (inference);
-AutoEncoder (:33-68, strided and transposed convolutions, a dense layer) is not built.
+``Zcnn`` (:71-122) runs on the same kernels with periodic x/y padding (inference).
+``AutoEncoder`` (:33-68, the reference's default model, namelist_n_constants.py:78) runs
+in FP32 on ``lex_conv3d_f32``: strided 'SAME' convolutions, the dense layer and the
+transposed convolutions (inference).
 """
 from __future__ import annotations
 
@@ -36,6 +38,86 @@ def Zcnn(seed=2031, device="cuda"):  # noqa: N802 - the reference's class name
     after every layer are zero padding in z.  Inference only (the 64-channel weight
     gradient is not built)."""
     return SgsNet(channels=ZCNN_CHANNELS, seed=seed, device=device, circular=True)
+
+
+# (kind, features, stride or padding) of dl_models.py:36-66; ReLU after all but the last
+AUTOENCODER_LAYERS = (("conv", 64, 1), ("conv", 32, 2), ("conv", 32, 2), ("conv", 64, 2), ("conv", 128, 2),
+                      ("conv", 256, 2), ("dense", 512, None), ("convT", 256, "SAME"), ("convT", 128, "VALID"),
+                      ("convT", 64, "SAME"), ("convT", 32, "SAME"), ("convT", 32, "SAME"), ("conv", 64, 1),
+                      ("conv", 5, 1))
+
+
+def _same_pads(n, k, s):
+    """'SAME' padding (lo, hi) of a strided window: ceil(n / s) outputs, the smaller half
+    of the total padding low (lax.padtype_to_pads)."""
+    out = -(-n // s)
+    total = max((out - 1) * s + k - n, 0)
+    return total // 2, total - total // 2
+
+
+def _transpose_pads(k, s, padding):
+    """Padding of a transposed convolution's input-dilated correlation (lax.conv_transpose)."""
+    pad_len = k + s - 2 + (max(k - s, 0) if padding == "VALID" else 0)
+    if padding == "VALID" or s > k - 1:
+        pad_a = k - 1
+    else:
+        pad_a = -(-pad_len // 2)
+    return pad_a, pad_len - pad_a
+
+
+class AutoEncoder(torch.nn.Module):  # noqa: N801 - the reference's class name
+    """AutoEncoder (dl_models.py:33-68) on one (40, 40, 40, 5) sample: a stride-1 and five
+    stride-2 'SAME' convolutions (5 -> 64 -> 32 -> 32 -> 64 -> 128 -> 256, 40^3 -> 2^3),
+    flatten, Dense 2048 -> 512, five stride-2 transposed convolutions (512 -> 256 -> 128
+    -> 64 -> 32 -> 32, 1 -> 2 -> 5 -> 10 -> 20 -> 40; the second 'VALID'), two stride-1
+    convolutions (32 -> 64 -> 5); ReLU after every layer but the last.  FP32 on
+    lex_conv3d_f32, inference only.  Kernels in flax's layout (3, 3, 3, ci, co), the
+    dense kernel (2048, 512); random normal(0, 1/sqrt(fan_in)) weights and zero biases
+    stand in for trained parameters (flax's initialisers are not restated)."""
+
+    SIZE = 40   # the dense layer fixes the sample to 40^3 (dl_models.py:49-52)
+
+    def __init__(self, seed=2031, device="cuda"):
+        super().__init__()
+        gen = torch.Generator(device="cpu").manual_seed(seed)
+        ws, bs = [], []
+        ci = 5
+        for kind, f, _ in AUTOENCODER_LAYERS:
+            shape = (2 * 2 * 2 * ci, f) if kind == "dense" else (3, 3, 3, ci, f)
+            fan_in = shape[0] if kind == "dense" else 27 * ci
+            w = torch.randn(shape, generator=gen) * fan_in ** -0.5
+            ws.append(torch.nn.Parameter(w.to(device).contiguous(), requires_grad=False))
+            bs.append(torch.nn.Parameter(torch.zeros(f, device=device), requires_grad=False))
+            ci = f
+        self.weights = torch.nn.ParameterList(ws)
+        self.biases = torch.nn.ParameterList(bs)
+
+    def forward(self, x):
+        n = self.SIZE
+        h = x.reshape(x.shape[-4:]).contiguous().float()
+        if tuple(h.shape) != (n, n, n, 5):
+            raise ValueError(f"AutoEncoder takes one (40, 40, 40, 5) sample, got {tuple(x.shape)}")
+        last = len(AUTOENCODER_LAYERS) - 1
+        for q, ((kind, f, arg), w, b) in enumerate(zip(AUTOENCODER_LAYERS, self.weights, self.biases)):
+            ni = tuple(h.shape[:3])
+            ci = h.shape[3]
+            if kind == "conv":
+                pads = [_same_pads(m, 3, arg) for m in ni]
+                no = [(m + lo + hi - 3) // arg + 1 for m, (lo, hi) in zip(ni, pads)]
+                k, stride, dil = 3, arg, 1
+            elif kind == "convT":
+                pads = [_transpose_pads(3, 2, arg)] * 3
+                no = [(m - 1) * 2 + 1 + lo + hi - 2 for m, (lo, hi) in zip(ni, pads)]
+                k, stride, dil = 3, 1, 2
+            else:   # flatten (channels last, as the reference's reshape) and Dense
+                ci = h.numel()
+                ni, pads, no, k, stride, dil = (1, 1, 1), [(0, 0)] * 3, [1, 1, 1], 1, 1, 1
+            y = torch.empty((*no, f), dtype=torch.float32, device=h.device)
+            _lib.call("lex_conv3d_f32", _lib.ptr(h), ni[0], ni[1], ni[2], ci, _lib.ptr(w), _lib.ptr(b), k, stride,
+                      dil, pads[0][0], pads[1][0], pads[2][0], int(q != last), _lib.ptr(y), no[0], no[1], no[2], f,
+                      _lib.stream())
+            h = y
+        return h
 
 
 def level_norm(scaling_params):

@@ -1440,6 +1440,84 @@ def correct_array2tuple(correct):
     return tc, uc, vc, wc, qc
 
 
+# ---------------------------------------------------------------------------------
+# AutoEncoder (dl_models.py:33-68): the reference's default post-step correction net
+# (namelist_n_constants.py:78).  flax's Conv / ConvTranspose / Dense are third-party;
+# their semantics are restated from JAX's public lax (conv_general_dilated with 'SAME'
+# strides, conv_transpose with transpose_kernel=False).  Pinned by per-output loops and
+# by torch's conv_transpose3d on flipped kernels (tests/test_oracle_autoencoder.py);
+# flax's initialisers are not restated (parity unpinned).
+# ---------------------------------------------------------------------------------
+def same_pads(n, k, s):
+    """lax 'SAME' padding (lo, hi) of a window k, stride s on n points: the output has
+    ceil(n / s) points and the total padding is split with the smaller half low."""
+    out = -(-n // s)
+    total = max((out - 1) * s + k - n, 0)
+    return total // 2, total - total // 2
+
+
+def transpose_pads(k, s, padding):
+    """The padding lax.conv_transpose gives the input-dilated correlation for 'SAME' and
+    'VALID' (JAX's _conv_transpose_padding)."""
+    if padding == "SAME":
+        pad_len = k + s - 2
+        pad_a = k - 1 if s > k - 1 else int(math.ceil(pad_len / 2))
+    else:
+        pad_len = k + s - 2 + max(k - s, 0)
+        pad_a = k - 1
+    return pad_a, pad_len - pad_a
+
+
+def conv3d_general(x, W, b=None, stride=1, pads=((1, 1),) * 3, dil=1, relu=False):
+    """Correlation of x (X, Y, Z, ci) with W (k, k, k, ci, co) over x dilated by `dil`
+    (dil - 1 zeros between points) and zero-padded by `pads`, window stride `stride`;
+    + b; optional ReLU."""
+    if dil > 1:
+        xd = np.zeros(tuple((n - 1) * dil + 1 for n in x.shape[:3]) + x.shape[3:], x.dtype)
+        xd[::dil, ::dil, ::dil] = x
+        x = xd
+    xp = np.pad(x, tuple(pads) + ((0, 0),))
+    k = W.shape[0]
+    no = tuple((xp.shape[a] - k) // stride + 1 for a in range(3))
+    out = np.zeros(no + (W.shape[4],), np.result_type(x, W))
+    for tx in range(k):
+        for ty in range(k):
+            for tz in range(k):
+                sl = xp[tx:tx + stride * (no[0] - 1) + 1:stride, ty:ty + stride * (no[1] - 1) + 1:stride,
+                        tz:tz + stride * (no[2] - 1) + 1:stride]
+                out = out + sl @ W[tx, ty, tz]
+    if b is not None:
+        out = out + b
+    return np.maximum(out, 0) if relu else out
+
+
+# (kind, features, stride or padding) in the order of dl_models.py:36-66; ReLU after all
+# but the last layer
+AUTOENCODER_LAYERS = (("conv", 64, 1), ("conv", 32, 2), ("conv", 32, 2), ("conv", 64, 2), ("conv", 128, 2),
+                      ("conv", 256, 2), ("dense", 512, None), ("convT", 256, "SAME"), ("convT", 128, "VALID"),
+                      ("convT", 64, "SAME"), ("convT", 32, "SAME"), ("convT", 32, "SAME"), ("conv", 64, 1),
+                      ("conv", 5, 1))
+
+
+def autoencoder_forward(params, x):
+    """AutoEncoder (dl_models.py:33-68) on one sample x (X, Y, Z, 5); params: (W, b) per
+    layer, conv kernels (3, 3, 3, ci, co), the dense kernel (ci, co)."""
+    h = x
+    last = len(AUTOENCODER_LAYERS) - 1
+    for q, ((kind, _, arg), (W, b)) in enumerate(zip(AUTOENCODER_LAYERS, params)):
+        relu = q != last
+        if kind == "conv":
+            pads = tuple(same_pads(n, 3, arg) for n in h.shape[:3])
+            h = conv3d_general(h, W, b, stride=arg, pads=pads, relu=relu)
+        elif kind == "convT":
+            pads = (transpose_pads(3, 2, arg),) * 3
+            h = conv3d_general(h, W, b, stride=1, pads=pads, dil=2, relu=relu)
+        else:   # flatten, Dense, reshape to (1, 1, 1, features) (dl_models.py:49-52)
+            h = h.reshape(1, 1, 1, -1) @ W + b
+            h = np.maximum(h, 0) if relu else h
+    return h
+
+
 def dl_apply_correction(phys_state, correct, scaling_params):
     """The correction step of ws_rk3_sprint_dl (one_sprint_functions.py:137-160): each
     field's interior plus its correction times the scaling range, padded again; pi'
