@@ -389,6 +389,18 @@ int lex_sgs_apply(const lex_grid_t* g, const float* y, const float* out_scale, f
  * of a padded fine field into an interior coarse field: mode 0 cell mean of a scalar,
  * 1/2/3 face-area means of u/v/w on the coarse faces. */
 int lex_coarsen(const lex_grid_t* g, int mode, int fx, int fy, int fz, const float* a, float* out, void* stream);
+/* The step with the SGS tendencies recomputed at the start of every WS-RK3 stage from
+ * that stage's input state (the learned SGS net run inside each stage); no counterpart
+ * in the reference, whose SGS is computed in stage 0 and frozen
+ * (one_step_integration.py:57-76).  fn(stage, stage_state, su, sv, sw, ss, user) is called
+ * on the host thread for stage = 0, 1, 2 with the stage's 6 + n_passive padded state
+ * fields and the library's SGS slots (u, v, w points; scalars interior); it fills them by
+ * work it enqueues on `stream` and returns 0 (non-zero aborts the step with -5). */
+typedef int (*lex_stage_sgs_fn)(int stage, float* const* stage_state, float* sgs_u, float* sgs_v, float* sgs_w,
+                                float* const* sgs_scalars, void* user);
+int lex_rk3_step_stage_sgs(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b,
+                           float* const* state, lex_stage_sgs_fn fn, void* user, void* workspace,
+                           size_t workspace_bytes, void* stream);
 /* lex_rk3_step with caller-supplied SGS tendencies (frozen over the step, as the
  * stage-0 Smagorinsky tendencies are, one_step_integration.py:57-76): su, sv, sw on the
  * u, v, w points, ss[] for theta, qv and the passive scalars.  o->turb is ignored. */

@@ -34,6 +34,13 @@ class StepOpts(C.Structure):
                 ("sfc", C.c_int), ("cor", C.c_int), ("fcor", C.c_float), ("u_geo", C.c_float), ("v_geo", C.c_float)]
 
 
+# int fn(int stage, float* const* stage_state, float* sgs_u, float* sgs_v, float* sgs_w,
+#        float* const* sgs_scalars, void* user): the per-stage SGS function of
+# lex_rk3_step_stage_sgs
+STAGE_SGS_FN = C.CFUNCTYPE(C.c_int, C.c_int, C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_void_p,
+                           C.POINTER(C.c_void_p), C.c_void_p)
+
+
 class BaseCols(C.Structure):
     _fields_ = [("rho0", C.c_void_p), ("theta0", C.c_void_p), ("qv0", C.c_void_p), ("pi0", C.c_void_p),
                 ("rho0_theta0", C.c_void_p), ("surface_t", C.c_void_p)]
@@ -151,6 +158,8 @@ _SIGS = {
     "lex_coarsen": [_G, _I, _I, _I, _I, _P, _P, _P],
     "lex_rk3_step_ext_sgs": [_G, C.POINTER(StepOpts), C.POINTER(BaseCols), C.POINTER(_P), _P, _P, _P,
                              C.POINTER(_P), _P, C.c_size_t, _P],
+    "lex_rk3_step_stage_sgs": [_G, C.POINTER(StepOpts), C.POINTER(BaseCols), C.POINTER(_P), STAGE_SGS_FN, _P, _P,
+                               C.c_size_t, _P],
     "lex_acoustic_substep_implicit": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _F, _P, _P, _P, _P, _P],
     # training the net through the solver
     "lex_sgs_apply_vjp": [_G, _P, _P, _P, _P, _P, _P, _P, _P],
@@ -213,15 +222,19 @@ def exported_symbols():
 # in the same argument list cannot take its memory first; after the call, reuse of the
 # memory is ordered on the stream behind the launched kernels
 _KEEP: list = []
+_DEPTH = [0]
 
 
 def call(name, *args):
     """Call an entry point; raise LexError with lex_last_error() on a non-zero status."""
     lib = load()
+    _DEPTH[0] += 1
     try:
         rc = getattr(lib, name)(*args)
     finally:
-        _KEEP.clear()
+        _DEPTH[0] -= 1
+        if _DEPTH[0] == 0:   # calls made from inside a callback keep the outer call's tensors
+            _KEEP.clear()
     if rc != 0:
         msg = lib.lex_last_error().decode(errors="replace")
         raise LexError(f"{name} failed ({rc}): {msg}")

@@ -616,7 +616,8 @@ size_t lex_rk3_workspace_bytes(const lex_grid_t* g, int n_passive) {
 }
 
 static int rk3_step_entry(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b,
-                          float* const* state, void* ws, size_t ws_bytes, void* stream);
+                          float* const* state, void* ws, size_t ws_bytes, void* stream,
+                          StageSgsFn stage_sgs = nullptr, void* stage_user = nullptr);
 
 int lex_rk3_step(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b, float* const* state,
                  void* ws, size_t ws_bytes, void* stream) {
@@ -627,7 +628,8 @@ int lex_rk3_step(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_c
 }
 
 static int rk3_step_entry(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b,
-                          float* const* state, void* ws, size_t ws_bytes, void* stream) {
+                          float* const* state, void* ws, size_t ws_bytes, void* stream, StageSgsFn stage_sgs,
+                          void* stage_user) {
   REQGRID(g);
   REQ(o); REQ(b); REQ(state); REQ(ws);
   REQ(b->rho0); REQ(b->theta0); REQ(b->qv0); REQ(b->pi0);
@@ -645,7 +647,8 @@ static int rk3_step_entry(const lex_grid_t* g, const lex_step_opts_t* o, const l
   if (o->sfc && !b->surface_t) return fail(-2, "lex_rk3_step: sfc needs the surface temperature (surface_t)%s");
   if (o->impl != 1 && (64 * ((g->nz + 63) / 64)) * kFTY > 1024)
     return fail(-1, "lex_rk3_step: the fused kernels need nz <= 256 (use impl 1)%s");
-  int rc = rk3_step(d, *o, *b, state, (char*)ws, L, S(stream));
+  int rc = rk3_step(d, *o, *b, state, (char*)ws, L, S(stream), stage_sgs, stage_user);
+  if (rc == -5 && stage_sgs) return fail(-5, "lex_rk3_step_stage_sgs: the stage SGS function failed%s");
   if (rc < 0) return fail(rc, "lex_rk3_step: unsupported configuration%s");
   if (rc > 0) return fail(rc, "lex_rk3_step: HIP error %s", hipGetErrorString((hipError_t)rc));
   return 0;
@@ -1312,6 +1315,16 @@ int lex_coarsen(const lex_grid_t* g, int mode, int fx, int fy, int fz, const flo
   int ni = d.nx / fx + (mode == 1), nj = d.ny / fy + (mode == 2), nk = d.nz / fz + (mode == 3);
   hipLaunchKernelGGL(k_coarsen, grd(nk, nj, ni), blk(), 0, S(stream), d, mode, fx, fy, fz, a, out);
   return check_launch("lex_coarsen");
+}
+
+int lex_rk3_step_stage_sgs(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b,
+                           float* const* state, lex_stage_sgs_fn fn, void* user, void* ws, size_t ws_bytes,
+                           void* stream) {
+  REQ(o); REQ(fn);
+  if (o->cor < 0 || o->cor > 2) return fail(-1, "lex_rk3_step_stage_sgs: cor must be 0, 1 or 2%s");
+  lex_step_opts_t o2 = *o;
+  o2.turb = kTurbExternal;
+  return rk3_step_entry(g, &o2, b, state, ws, ws_bytes, stream, (StageSgsFn)fn, user);
 }
 
 int lex_rk3_step_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b, float* const* state,

@@ -267,6 +267,13 @@ inline void acou_ptrs(char* base, const Rk3Layout& L, float** f) {  // u, v, w, 
 // tape) slots, frozen over the step: the learned SGS net
 constexpr int kTurbExternal = 9;
 
+// Host function called at the start of every stage with the stage's input state (6 +
+// n_passive padded fields in state order) and the SGS slots of the workspace; it fills
+// the slots for that stage by work enqueued on the step's stream (the learned SGS net
+// run inside each stage).  Returns 0 or an error code.
+typedef int (*StageSgsFn)(int stage, float* const* stage_state, float* su, float* sv, float* sw, float* const* ss,
+                          void* user);
+
 struct StepOpts {
   float dt;
   int n_sound;
@@ -290,7 +297,7 @@ inline SfcOut sfc_fields(char* ws, const Rk3Layout& L, const Dims& d) {
 
 template <class O, class B>
 int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* ws, const Rk3Layout& L,
-             hipStream_t st) {
+             hipStream_t st, StageSgsFn stage_sgs = nullptr, void* stage_user = nullptr) {
   const int nf = 6 + o.n_passive;
   const CorOpt cor{o.cor, o.fcor, o.u_geo, o.v_geo};
   float* X0[16];
@@ -345,6 +352,10 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
     // kTurbExternal: SGS tendencies supplied by the caller in the workspace slots (the
     // learned SGS net); all frozen over the step
     const bool turb = o.turb != 0;
+    if (stage_sgs) {   // SGS tendencies of this stage from its input state (caller's function)
+      const int rc = stage_sgs(s, src, su, sv, sw, ss, stage_user);
+      if (rc) return -5;
+    }
     if (s == 0 && (o.turb == 1 || o.turb == 2)) {
       long n = (long)L.nc;
       const float gs = cbrtf(d.dx * d.dy * d.dz);

@@ -11,9 +11,11 @@ coarse model's missing tendency
 with M_0 one coarse WS-RK3 step without SGS, brought to cell centres.  The net
 (simpleCNN shape, dl_models.py:19-30; 11 inputs, 5 outputs) is trained on BF16 MFMA
 kernels; at inference its tendencies replace the stage-0 Smagorinsky tendencies and are
-frozen over the step (lex_rk3_step_ext_sgs).  Coarse runs from C(truth(0)) of a held-out
-member with (a) no SGS, (b) Smagorinsky and (c) the net are scored by the RMS of theta'
-against the coarsened truth; Smagorinsky stays the fallback when the net is worse.
+frozen over the step (lex_rk3_step_ext_sgs), or the net runs at the start of every
+RK3 stage on that stage's state (lex_rk3_step_stage_sgs).  Coarse runs from C(truth(0))
+of a held-out member with (a) no SGS, (b) Smagorinsky, (c) the frozen net and (d) the
+per-stage net are scored by the RMS of theta' against the coarsened truth; Smagorinsky
+stays the fallback when the net is worse.
 
 Every array operation on fields runs in liblexhip; torch holds tensors, runs the
 optimiser and computes scalar statistics.
@@ -91,6 +93,32 @@ def coarsen_state(phys, gf, f, gc):
     return tuple(out)
 
 
+def stage_sgs_step(g, opts, base_cols, state, fn, ws):
+    """One step through lex_rk3_step_stage_sgs: fn(stage, stage_ptrs, su, sv, sw, ss_ptrs)
+    enqueues the SGS tendencies of each stage on the current stream from the stage's
+    input state (device pointers in state order) into the library's slots.  An exception
+    raised by fn aborts the step and is re-raised here."""
+    err = []
+
+    def tramp(stage, sp, su, sv, sw, ss, _user):
+        try:
+            fn(stage, sp, su, sv, sw, ss)
+            return 0
+        except BaseException as e:   # noqa: BLE001 - handed back to the caller below
+            err.append(e)
+            return 1
+
+    cb = _lib.STAGE_SGS_FN(tramp)
+    ptrs = (ctypes.c_void_p * len(state))(*[_lib.ptr(a) for a in state])
+    try:
+        _lib.call("lex_rk3_step_stage_sgs", g, opts, base_cols, ptrs, cb, None, _lib.ptr(ws), ws.numel(),
+                  _lib.stream())
+    except _lib.LexError:
+        if err:
+            raise err[0]
+        raise
+
+
 class CoarseModel:
     """Coarse 96 x 96 x 48 runs with no SGS, Smagorinsky or the net."""
 
@@ -106,19 +134,29 @@ class CoarseModel:
         self.scale = (ctypes.c_float * 11)(*([1.0] * 11))
         self.out_scale = (ctypes.c_float * 5)(*([1.0] * 5))
         self._ws = None
+        self._feat = None
 
     def step(self, st, mode, net=None):
+        """One coarse step: "none", "smag" (stage-0 Smagorinsky, frozen), "net" (the net's
+        tendencies from the step's initial state, frozen) or "net_stage" (the net run at the
+        start of every stage on that stage's input state)."""
         if mode in ("none", "smag"):
             fused_step(st, self.base, self.g, MO_NONE if mode == "none" else MO_SMAG, dt=self.dtc,
                        n_sound=self.n_sound)
             return st
-        sgs = self.net_sgs(st, net)
         cols = [column(a) for a in (self.base[1], self.base[2], self.base[4], self.base[3], self.base[0])]
         bc = _lib.BaseCols(*[c.data_ptr() for c in cols])
         o = _lib.StepOpts(float(self.dtc), int(self.n_sound), 2, 0.18, 1.0 / 3.0, 0, 0)
         if self._ws is None:
             nb = _lib.load().lex_rk3_workspace_bytes(self.g, 0)
             self._ws = torch.empty(nb, dtype=torch.uint8, device=st[0].device)
+        if mode == "net_stage":
+            stage_sgs_step(self.g, o, bc, st, lambda s, sp, su, sv, sw, ss: self.net_sgs_into(sp, net, su, sv, sw, ss),
+                           self._ws)
+            return st
+        if mode != "net":
+            raise ValueError(f"unknown SGS mode {mode!r}")
+        sgs = self.net_sgs(st, net)
         ptrs = (ctypes.c_void_p * 6)(*[_lib.ptr(a) for a in st])
         ss = (ctypes.c_void_p * 2)(_lib.ptr(sgs[3]), _lib.ptr(sgs[4]))
         _lib.call("lex_rk3_step_ext_sgs", self.g, o, bc, ptrs, _lib.ptr(sgs[0]), _lib.ptr(sgs[1]), _lib.ptr(sgs[2]),
@@ -132,6 +170,18 @@ class CoarseModel:
                   _lib.ptr(st[3]), _lib.ptr(st[0]), _lib.ptr(st[5]), self.shift, self.scale, _lib.ptr(out),
                   _lib.stream())
         return out
+
+    def net_sgs_into(self, sp, net, su, sv, sw, ss):
+        """The net's SGS tendencies from the state at device pointers sp (state order) into
+        the device buffers su, sv, sw, ss[0] (theta), ss[1] (qv); work on the current stream."""
+        g = self.g
+        if self._feat is None:
+            self._feat = torch.empty((g.nx, g.ny, g.nz, 11), dtype=torch.float32, device=self.base[0].device)
+        _lib.call("lex_sgs_features", g, _lib.ptr(self.th0c), _lib.ptr(self.qv0c), sp[1], sp[2], sp[3], sp[0], sp[5],
+                  self.shift, self.scale, _lib.ptr(self._feat), _lib.stream())
+        with torch.no_grad():
+            y = net(self._feat).contiguous()
+        _lib.call("lex_sgs_apply", g, _lib.ptr(y), self.out_scale, su, sv, sw, ss[0], ss[1], _lib.stream())
 
     def net_sgs(self, st, net):
         g = self.g
@@ -244,7 +294,7 @@ def train(net, model, pairs, targets, epochs, lr, log, seed=2031):
 
 def evaluate(model, net, truth):
     out = {}
-    for mode in ("none", "smag", "net"):
+    for mode in ("none", "smag", "net", "net_stage"):
         st = tuple(a.clone() for a in truth[0])
         errs = []
         for n in range(1, len(truth)):
@@ -274,7 +324,8 @@ def run(cfg, log=print, device="cuda"):
     scores = evaluate(model, net, truth)
     fin = {k: v[-1] for k, v in scores.items()}
     mean = {k: float(np.mean(v)) for k, v in scores.items()}
-    chosen = "net" if (math.isfinite(mean["net"]) and mean["net"] <= mean["smag"]) else "smagorinsky"
+    nets = [m for m in ("net", "net_stage") if math.isfinite(mean[m]) and mean[m] <= mean["smag"]]
+    chosen = min(nets, key=lambda m: mean[m]) if nets else "smagorinsky"
     return dict(fine=[cfg["nx"], cfg["nx"], cfg["nz"]], coarse=[nxc, nxc, nzc], pairs=len(pairs),
                 target_std=tsd, train_mse=hist, theta_rmse_final=fin, theta_rmse_mean=mean, chosen=chosen,
                 seconds=dict(generate=t_gen, train=t_train, total=time.time() - t_start),

@@ -1187,6 +1187,50 @@ def ws_rk3_big_step_ext_sgs(xi0, base, grid, opts, sgs):
     return xi3, (i0, i1, i2)
 
 
+def ws_rk3_big_step_stage_sgs(xi0, base, grid, opts, sgs_fn):
+    """ws_rk3_big_step with the SGS tendencies recomputed at the start of every stage:
+    sgs_fn(stage, xi) returns (sgs_u, sgs_v, sgs_w, then one per scalar) from the stage's
+    input state xi (xi0, then the first and second stage results).  No counterpart in the
+    reference, which computes its SGS in stage 0 and freezes it (one_step_integration.py:57-76);
+    this is the learned SGS net run inside each stage.  A tendency that is a linear
+    function of the state then advances by the WS-RK3 polynomial
+    1 + L dt + (L dt)^2 / 2 + (L dt)^3 / 6 instead of the frozen 1 + L dt."""
+    xi1, _, i0 = rk_sub_step(0, xi0, xi0, base, grid, opts, opts.dt / 3.0, tuple(sgs_fn(0, xi0)))
+    xi2, _, i1 = rk_sub_step(1, xi0, xi1, base, grid, opts, opts.dt / 2.0, tuple(sgs_fn(1, xi1)))
+    xi3, _, i2 = rk_sub_step(2, xi0, xi2, base, grid, opts, opts.dt, tuple(sgs_fn(2, xi2)))
+    return xi3, (i0, i1, i2)
+
+
+def sgs_features(phys, base, grid):
+    """(nx, ny, nz, 11) inputs of the learned SGS net from a padded state: u, v, w at cell
+    centres, theta' and qv' against the base columns, and the six deformation components
+    S11, S22, S33, S12, S13, S23 of compute_deformation (turbulence.py:298-388) with rho = 1."""
+    th, u, v, w, _, qv = phys[:6]
+    I = (slice(NGX, -NGX), slice(NGY, -NGY), slice(NGZ, -NGZ))
+    uc, vc, wc = unstagger_uvw(u[I], v[I], w[I])
+    s = compute_deformation(np.ones_like(th), u, v, w, grid)
+    chans = [uc, vc, wc, th[I] - base[2][I], qv[I] - base[4][I]] + [a[I] for a in s[:6]]
+    return np.stack(chans, axis=3)
+
+
+def sgs_apply(y, out_scale):
+    """(nx, ny, nz, 5) cell-centred net outputs times out_scale -> the SGS tendencies on
+    compute_smag's points (turbulence.py:9-111): sgs_u on the nx+1 x faces and sgs_v on
+    the ny+1 y faces as the mean of the two adjacent cells (periodic), sgs_w on the nz+1
+    levels as the mean of the cells below and above (zero at the surface and lid),
+    sgs_theta and sgs_qv at the cells."""
+    t = [y[..., c] * out_scale[c] for c in range(5)]
+    ext_x = np.concatenate([t[0][-1:], t[0]], 0)
+    ext_x = np.concatenate([ext_x, ext_x[1:2]], 0)
+    su = 0.5 * (ext_x[:-1] + ext_x[1:])
+    ext_y = np.concatenate([t[1][:, -1:], t[1]], 1)
+    ext_y = np.concatenate([ext_y, ext_y[:, 1:2]], 1)
+    sv = 0.5 * (ext_y[:, :-1] + ext_y[:, 1:])
+    z = np.zeros(y.shape[:2] + (1,), y.dtype)
+    sw = np.concatenate([z, 0.5 * (t[2][:, :, :-1] + t[2][:, :, 1:]), z], 2)
+    return su, sv, sw, t[3], t[4]
+
+
 def ws_rk3_sprint(xi, base, grid, opts, nsteps):
     """nsteps WS-RK3 steps (one_sprint_functions.py:109-115 with sprint_n = nsteps)."""
     info = None

@@ -126,6 +126,7 @@ def test_small_pipeline_runs():
     res = sgs_train.run(cfg, log=lambda s: print(s, flush=True))
     assert res["pairs"] == cfg["members"] * cfg["snaps"]
     assert all(np.isfinite(res["train_mse"]))
-    for k in ("none", "smag", "net"):
+    for k in ("none", "smag", "net", "net_stage"):
         assert np.isfinite(res["theta_rmse_final"][k]), k
+    assert res["chosen"] in ("net", "net_stage", "smagorinsky")
     print(res["theta_rmse_mean"], res["chosen"])
