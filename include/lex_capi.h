@@ -242,6 +242,12 @@ typedef struct lex_base_cols {
 size_t lex_rk3_workspace_bytes(const lex_grid_t* g, int n_passive);
 int lex_rk3_step(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b, float* const* state,
                  void* workspace, size_t workspace_bytes, void* stream);
+/* Byte offsets inside the step's workspace of the SGS tendencies the last step used
+ * (frozen over its stages, one_step_integration.py:57-76, :245): offsets[0..2] = sgs_u,
+ * sgs_v, sgs_w (interior, (nx+1, ny, nz), (nx, ny+1, nz), (nx, ny, nz+1)), offsets[3..]
+ * = sgs_theta, sgs_qv, then one per passive scalar ((nx, ny, nz)); 3 + 2 + n_passive
+ * entries.  They hold the last step's values until the next step on the workspace. */
+int lex_rk3_workspace_sgs_offsets(const lex_grid_t* g, int n_passive, size_t* offsets);
 
 /* ---- measurement (no reference counterpart: replaces the wall-clock table of
  * lex_relay.py:65-75 with per-kernel device timing) ------------------------------ */

@@ -158,6 +158,7 @@ _SIGS = {
     "lex_coarsen": [_G, _I, _I, _I, _I, _P, _P, _P],
     "lex_rk3_step_ext_sgs": [_G, C.POINTER(StepOpts), C.POINTER(BaseCols), C.POINTER(_P), _P, _P, _P,
                              C.POINTER(_P), _P, C.c_size_t, _P],
+    "lex_rk3_workspace_sgs_offsets": [_G, _I, C.POINTER(C.c_size_t)],
     "lex_rk3_step_stage_sgs": [_G, C.POINTER(StepOpts), C.POINTER(BaseCols), C.POINTER(_P), STAGE_SGS_FN, _P, _P,
                                C.c_size_t, _P],
     "lex_acoustic_substep_implicit": [_G, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _F, _P, _P, _P, _P, _P],

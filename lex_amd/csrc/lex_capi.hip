@@ -615,6 +615,18 @@ size_t lex_rk3_workspace_bytes(const lex_grid_t* g, int n_passive) {
   return rk3_layout(mkdims(g), n_passive).total;
 }
 
+int lex_rk3_workspace_sgs_offsets(const lex_grid_t* g, int n_passive, size_t* offsets) {
+  REQGRID(g);
+  REQ(offsets);
+  if (n_passive < 0 || n_passive > 8) return fail(-1, "lex_rk3_workspace_sgs_offsets: n_passive must be 0..8%s");
+  const Rk3Layout L = rk3_layout(mkdims(g), n_passive);
+  offsets[0] = L.off_su;
+  offsets[1] = L.off_sv;
+  offsets[2] = L.off_sw;
+  for (int q = 0; q < 2 + n_passive; ++q) offsets[3 + q] = L.off_ss + q * al256(L.ic_ * 4);
+  return 0;
+}
+
 static int rk3_step_entry(const lex_grid_t* g, const lex_step_opts_t* o, const lex_base_cols_t* b,
                           float* const* state, void* ws, size_t ws_bytes, void* stream,
                           StageSgsFn stage_sgs = nullptr, void* stage_user = nullptr);

@@ -316,6 +316,84 @@ def test_two_step_gradient_is_finite_and_uses_the_solver():
     assert any(not np.allclose(a, b) for a, b in zip(grads[0], grads[1]))
 
 
+def test_two_step_bias_gradient_vs_central_differences():
+    """Two corrected steps: d loss / d (last-layer bias) from loss_and_grad (the chain of
+    k_dl_apply_vjp, the network's BF16 backward, k_dl_pack_vjp and the step's adjoint
+    lex_rk3_step_vjp_taped across the first step) against central differences of the
+    FP64 oracle sprint's loss, with the network's forward restated in FP64 (torch conv3d
+    on the CPU, the same weights).  The bias of step 1 reaches the loss through the
+    carried state and through the second step's network input, so a dropped or
+    mis-signed term of the chain moves the result by a large fraction.  The residuals
+    are one-signed (no cancellation in the sum over cells), so the tolerance is 2 % of
+    the finite difference: BF16 rounding of the per-cell gradients (2^-8) plus the
+    FP32 / BF16 trajectory against the FP64 one."""
+    import torch
+    from lex_amd import dl_models as dlm, dl_train as dt, namelist_n_constants as nl
+    from lex_amd.setup_lex import build_case, config1
+    case, opts = config1()
+    case = case.scaled(16, 12, 12)
+    phys, base, grids, _ = build_case(case)
+    nl.dt, nl.n_sound, nl.solver_opt, nl.turb_opt = opts["dt"], 7, 2, 1
+    model_opt = (2, False, False, 0, False, False, 1)
+    rng = np.random.default_rng(35)
+    spr = _scaling(rng, case.nx, case.ny, case.nz)
+    spd = tuple(to_dev(a) for a in spr)
+    net = dlm.simpleCNN(seed=13)
+    with torch.no_grad():
+        net.weights[-1].mul_(1e-2)
+        net.biases[-1].fill_(1e-4)
+    ic = tuple(a.unsqueeze(0).clone() for a in phys[:6])
+    S = 2
+    stacks = dt.ws_rk3_sprint_dl_train_vmap(*ic, base, grids, model_opt, net, spd, sprint_n=S)
+    og = o.Grid(case.nx, case.ny, case.nz, case.dx, case.dy, case.dz)
+    own = tuple(from_dev(a) for a in stacks)
+    sp0 = tuple(from_dev(a)[0, 0] for a in spd)
+    # one-signed residuals: every cell's loss gradient has the same sign per field, so the
+    # bias gradient is a sum without cancellation and the BF16 rounding of the per-cell
+    # gradients stays a relative error of the result
+    truth = [r - sc * (1.0 + 0.5 * rng.random(r.shape)) for r, sc in zip(own, (0.05, 0.05, 0.05, 0.05, 1e-4, 1e-4))]
+    truth += [o.laplace_of_tensor(own[0] - sp0[0], og) - 1e-4, o.laplace_of_tensor(own[5] - sp0[8], og) - 1e-9]
+    truth_d = tuple(to_dev(t.astype(np.float32)) for t in truth)
+    for prm in net.parameters():
+        prm.grad = torch.zeros_like(prm)
+    dt.loss_and_grad(net, ic, truth_d, spd, base, grids, model_opt, sprint_n=S)
+    gb = from_dev(net.biases[-1].grad)
+
+    # FP64 restatement of the network's forward (the same weights; the last bias varies)
+    ws = [w.detach().double().cpu() for w in net.weights]
+    bs = [b.detach().double().cpu() for b in net.biases]
+
+    def tw(W):
+        cin, cout = W.shape[1], W.shape[2]
+        return W.reshape(3, 3, 3, cin, cout).permute(4, 3, 0, 1, 2).contiguous()
+
+    def net64(x, b_last):
+        h = torch.from_numpy(np.asarray(x, np.float64)).permute(3, 0, 1, 2).unsqueeze(0)
+        for li, (w, b) in enumerate(zip(ws, bs)):
+            h = torch.nn.functional.conv3d(h, tw(w), b_last if li == len(ws) - 1 else b, padding=1)
+            if li < len(ws) - 1:
+                h = torch.nn.functional.gelu(h, approximate="tanh")
+        return h[0].permute(1, 2, 3, 0).numpy()
+
+    oo = o.Options(dt=opts["dt"], solver_opt=2, n_sound=7, turb_opt=1)
+    bs64 = tuple(from_dev(a).astype(np.float64) for a in base[:5]) + (None,)
+    sp64 = tuple(a.astype(np.float64) for a in spr)
+    x0 = tuple(from_dev(a[0]) for a in ic)
+
+    def loss64(b_last):
+        st, _ = o.ws_rk3_sprint_dl_train(x0, bs64, og, oo, lambda x: net64(x, b_last), sp64, S)
+        return o.dl_loss(tuple(a[None] for a in st), truth, sp0, og)
+
+    h = 2e-5
+    for q in range(5):
+        bp, bm = bs[-1].clone(), bs[-1].clone()
+        bp[q] += h
+        bm[q] -= h
+        fd = (loss64(bp) - loss64(bm)) / (2 * h)
+        print(f"channel {q}: adjoint {gb[q]:.6e} central difference {fd:.6e}")
+        assert abs(gb[q] - fd) <= 2e-2 * abs(fd), (q, gb[q], fd)
+
+
 def test_train_step_lowers_the_loss():
     import torch
     from lex_amd import dl_models as dlm, dl_train as dt, namelist_n_constants as nl

@@ -167,6 +167,42 @@ def test_fused_kernels_equal_per_operator_kernels(shape, impl):
         assert rel_l2(dx, dy) < 1e-3, (n, rel_l2(dx, dy))
 
 
+@pytest.mark.parametrize("turb", [1, 2])
+@pytest.mark.parametrize("shape", [(24, 20, 24), (37, 23, 70)], ids=lambda s: "x".join(map(str, s)))
+def test_fused_smagorinsky_equals_routine_chain(shape, turb):
+    """The frozen SGS tendencies of one step (sgs_u, sgs_v, sgs_w and every scalar's, read
+    from the workspace through lex_rk3_workspace_sgs_offsets) from the fused passes
+    (k_smag_tau + k_smag_uvw<true> + k_smag_scalars, impl 0 and 2) equal the
+    routine-by-routine chain (impl 1) bit for bit; qc and qr seeded non-zero."""
+    import ctypes
+
+    import torch
+    from lex_amd import one_sprint_functions as sp, _lib
+    from lex_amd._core import grid_from
+    phys, base, grids, mo, og, oo = _setup(1, *shape)
+    g = grid_from(base[1], *grids[3:6])
+    phys[6].copy_(phys[5] * 0.01 + 1e-5)
+    phys[7].copy_(phys[5] * phys[5] * 3.0)
+    mo = mo[:6] + (turb,)
+    npas = len(phys) - 6
+    offs = (ctypes.c_size_t * (5 + npas))()
+    _lib.call("lex_rk3_workspace_sgs_offsets", g, npas, offs)
+    nx, ny, nz = shape
+    sizes = [(nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)] + [nx * ny * nz] * (2 + npas)
+    out = {}
+    for impl in (0, 2, 1):
+        st = tuple(t.clone() for t in phys)
+        sp.fused_step(st, base, g, mo, impl=impl)
+        ws = sp._workspace(g, npas, st[0].device)
+        torch.cuda.synchronize()
+        out[impl] = [ws[offs[q]:offs[q] + 4 * n].view(torch.float32).clone() for q, n in enumerate(sizes)]
+    names = ["sgs_u", "sgs_v", "sgs_w", "sgs_theta", "sgs_qv", "sgs_qc", "sgs_qr"]
+    for impl in (0, 2):
+        for name, a, b in zip(names, out[impl], out[1]):
+            assert float(b.abs().max()) > 0, name
+            assert torch.equal(a, b), (impl, name, float((a - b).abs().max()))
+
+
 def test_pseudo_incompressible_step_vs_oracle():
     """solver_opt 1 (BiCGSTAB) core, config-0 recipe on a reduced grid."""
     from lex_amd import one_sprint_functions as sp
