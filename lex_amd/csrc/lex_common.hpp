@@ -41,10 +41,22 @@ struct Dims {
   __host__ __device__ long nw() const { return (long)(nx + 6) * (ny + 6) * (nz + 3); }
 };
 
-__device__ __forceinline__ long ic(const Dims& d, int I, int J, int K) { return I * d.sxc() + (long)J * d.syc() + K; }
-__device__ __forceinline__ long iu(const Dims& d, int I, int J, int K) { return I * d.sxu() + (long)J * d.syc() + K; }
-__device__ __forceinline__ long iv(const Dims& d, int I, int J, int K) { return I * d.sxv() + (long)J * d.syc() + K; }
-__device__ __forceinline__ long iw(const Dims& d, int I, int J, int K) { return I * d.sxw() + (long)J * d.syw() + K; }
+// Element index of padded (I, J, K) in each staggering.  The arithmetic is 32-bit: every
+// grid accepted at the C ABI has fewer than 2^30 elements in each padded field (see
+// kMaxOffset32), so the products and sums fit an int, and an index that a caller takes
+// into the halo and offsets back stays exact when negative (sign extension).
+__device__ __forceinline__ long ic(const Dims& d, int I, int J, int K) {
+  return (long)(I * ((d.ny + 6) * (d.nz + 2)) + J * (d.nz + 2) + K);
+}
+__device__ __forceinline__ long iu(const Dims& d, int I, int J, int K) {
+  return (long)(I * ((d.ny + 6) * (d.nz + 2)) + J * (d.nz + 2) + K);
+}
+__device__ __forceinline__ long iv(const Dims& d, int I, int J, int K) {
+  return (long)(I * ((d.ny + 7) * (d.nz + 2)) + J * (d.nz + 2) + K);
+}
+__device__ __forceinline__ long iw(const Dims& d, int I, int J, int K) {
+  return (long)(I * ((d.ny + 6) * (d.nz + 3)) + J * (d.nz + 3) + K);
+}
 
 // Access at a 32-bit element offset.  The address is the (wave-uniform) base pointer
 // plus a zero-extended 32-bit byte offset, which the compiler emits as

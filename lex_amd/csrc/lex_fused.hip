@@ -237,17 +237,20 @@ __global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __r
 }
 
 
-// Flow divergence div(rho0 u) at padded scalar cell (I,J,K) (get_divergence,
-// advection.py:200-226) for a horizontally uniform rho column.
-__device__ __forceinline__ float fd_cell(const Dims& d, const float* __restrict__ rc, const float* __restrict__ u,
-                                         const float* __restrict__ v, const float* __restrict__ w, int I, int J, int K) {
-  float r = rc[K];
-  float r8lo = (K <= 0 || K >= d.nz + 2) ? 0.0f : 0.5f * (rc[K - 1] + rc[K]);
-  float r8hi = (K + 1 >= d.nz + 2) ? 0.0f : 0.5f * (rc[K] + rc[K + 1]);
-  return (r * u[iu(d, I + 1, J, K)] - r * u[iu(d, I, J, K)]) * d.idx +
-         (r * v[iv(d, I, J + 1, K)] - r * v[iv(d, I, J, K)]) * d.idy +
-         (r8hi * w[iw(d, I, J, K + 1)] - r8lo * w[iw(d, I, J, K)]) * d.idz;
+
+// Flow divergence div(rho0 u) of a cell (get_divergence, advection.py:200-226) for a
+// horizontally uniform rho column, from the cell's face velocities already in registers:
+// r = rho at the cell, r8lo / r8hi at its lower / upper w level (zero at the ghost levels).
+__device__ __forceinline__ float fd_ops(const Dims& d, float r, float r8lo, float r8hi, float u1, float u0, float v1,
+                                        float v0, float w1, float w0) {
+  return (r * u1 - r * u0) * d.idx + (r * v1 - r * v0) * d.idy + (r8hi * w1 - r8lo * w0) * d.idz;
 }
+
+// The momentum kernels address every field through 32-bit element offsets of the
+// thread's point, advanced by one x-plane per step of the march (one integer add per
+// field instead of a 64-bit index product per load); an operand shared by a mass flux
+// and the flow divergence is loaded once, and the x-1 column's operands are carried
+// from the previous step.
 
 // ---------------------------------------------------------------------------------
 // k_mom_u: rhs_u = advection_u + sgs_u at u faces 0..nx-1 (face nx is the periodic
@@ -271,38 +274,51 @@ __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict_
   const int K = k + 1;
   const int Kc = act ? K : 1, Jc = (j < d.ny) ? j + 3 : 3;
   const bool last_row = (ty == kFTY - 1) || (j == d.ny - 1);
-  const long sx = d.sxu();
   const float r = rc[Kc];
   const float ir = 1.0f / r;   // one division per thread instead of one per cell and field
+  const float r8lo = (Kc <= 0 || Kc >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc - 1] + rc[Kc]);
+  const float r8hi = (Kc + 1 >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc] + rc[Kc + 1]);
+  const float r8w = 0.5f * (rc[Kc - 1] + rc[Kc]);   // rho at the w level below (z mass flux)
   const int iend = min(i0 + kFTX, d.nx);
   const PlaneSlots psl = plane_slots(tid, nt, Lz, j0, d.ny + 5, (int)d.syc());
+  const uint32_t usx = (uint32_t)d.sxu(), usy = (uint32_t)d.syc();
+  const uint32_t usxv = (uint32_t)d.sxv(), usxw = (uint32_t)d.sxw(), usyw = (uint32_t)d.syw();
+  const uint32_t usi = (uint32_t)d.ny * (uint32_t)d.nz;
+  // offsets of the point at x = I: ou (u), ov (v), ow (w), oi the interior (nx+1, ny, nz)
+  // arrays (sgs_u, ru), op the first element of u's x-plane
+  uint32_t ou = (uint32_t)(i0 + 3) * usx + (uint32_t)Jc * usy + (uint32_t)Kc;
+  uint32_t ov = (uint32_t)(i0 + 3) * usxv + (uint32_t)Jc * usy + (uint32_t)Kc;
+  uint32_t ow = (uint32_t)(i0 + 3) * usxw + (uint32_t)Jc * usyw + (uint32_t)Kc;
+  uint32_t oi = (uint32_t)i0 * usi + (uint32_t)(Jc - 3) * (uint32_t)d.nz + (uint32_t)(Kc - 1);
+  uint32_t op = (uint32_t)(i0 + 3) * usx;
   float win[6];
   float fxlo, fdlo;
+  float vA, vC, wA;   // v at (I-1, J), (I-1, J+1) and w at (I-1, J): carried from the previous step
   {
-    const int I = i0 + 3;
-    long b = iu(d, I, Jc, Kc);
-    float m3 = u[b - 3 * sx];
+    float m3 = ld32(u, ou - 3 * usx);
 #pragma unroll
-    for (int t = 0; t < 6; ++t) win[t] = u[b + (t - 2) * sx];
+    for (int t = 0; t < 6; ++t) win[t] = ld32(u, ou + (uint32_t)(t - 2) * usx);
     float mass = 0.5f * (win[1] + win[2]) * r;   // cell I-1
     fxlo = mass * hrecon(m3, win[0], win[1], win[2], win[3], win[4], upwind_pos(mass), weps);
-    fdlo = fd_cell(d, rc, u, v, w, I - 1, Jc, Kc);
+    vA = ld32(v, ov - usxv);
+    vC = ld32(v, ov - usxv + usy);
+    wA = ld32(w, ow - usxw);
+    const float wA1 = ld32(w, ow - usxw + 1);
+    fdlo = fd_ops(d, r, r8lo, r8hi, win[2], win[1], vC, vA, wA1, wA);   // fd_cell at I-1
   }
   PlaneRegs pr;
-  plane_fetch(pr, u + (long)(i0 + 3) * sx, psl);
+  plane_fetch32(pr, u, op, psl);
   for (int i = i0; i < iend; ++i) {
-    const int I = i + 3;
     const float uc = win[2];
     const float mx = 0.5f * (win[2] + win[3]) * r;   // x-flux at cell I
     const float fxhi = mx * hrecon(win[0], win[1], win[2], win[3], win[4], win[5], upwind_pos(mx), weps);
-    const float fdhi = fd_cell(d, rc, u, v, w, I, Jc, Kc);
-    // operands of the y and z mass fluxes and the SGS add-in, issued before the barrier
-    const float wA = w[iw(d, I - 1, Jc, Kc)], wB = w[iw(d, I, Jc, Kc)];
-    const float vA = v[iv(d, I - 1, Jc, Kc)], vB = v[iv(d, I, Jc, Kc)];
-    const float vC = v[iv(d, I - 1, Jc + 1, Kc)], vD = v[iv(d, I, Jc + 1, Kc)];
-    const float sgv = sgs ? sgs[((long)i * d.ny + (Jc - 3)) * d.nz + (Kc - 1)] : 0.0f;
+    // operands at column I, issued before the barrier
+    const float vB = ld32(v, ov), vD = ld32(v, ov + usy);
+    const float wB = ld32(w, ow), wB1 = ld32(w, ow + 1);
+    const float sgv = sgs ? ld32(sgs, oi) : 0.0f;
+    const float fdhi = fd_ops(d, r, r8lo, r8hi, win[3], win[2], vD, vB, wB1, wB);   // fd_cell at I
     plane_store(P, pr, psl);  // fetched one plane ahead
-    if (i + 1 < iend) plane_fetch(pr, u + (long)(I + 1) * sx, psl);
+    if (i + 1 < iend) plane_fetch32(pr, u, op + usx, psl);
     __syncthreads();
     const float* Pc = P + (ty + 3) * Lz;
     if (act) {
@@ -310,7 +326,7 @@ __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict_
       // at the surface the stress averaged onto the face (periodic in x, advection.py:43-48)
       float fz = 0.0f;
       if (k > 0) {
-        float m = 0.5f * (0.5f * (rc[K - 1] + rc[K]) * wA + 0.5f * (rc[K - 1] + rc[K]) * wB);
+        float m = 0.5f * (r8w * wA + r8w * wB);
         fz = m * vrecon(Pc, Lz, K - 1, upwind_pos(m), weps);
       } else if (tau_x) {
         const int il = (i == 0) ? d.nx - 1 : i - 1;
@@ -336,14 +352,22 @@ __global__ void __launch_bounds__(1024) k_mom_u(Dims d, const float* __restrict_
       // + fv: f times the mean of the four v around the u point (v at I-1, I and J, J+1)
       if (cor.mode) a = a + cor.of(((vA + vC) + vB) + vD, cor.vg);
       if (sgs) a = a + sgv;
-      ru[((long)i * d.ny + j) * d.nz + k] = a;
-      if (i == 0) ru[((long)d.nx * d.ny + j) * d.nz + k] = a;
+      st32(ru, oi, a);                                  // ((i ny + j) nz + k)
+      if (i == 0) st32(ru, oi + (uint32_t)d.nx * usi, a);
     }
     fxlo = fxhi;
     fdlo = fdhi;
+    vA = vB;
+    vC = vD;
+    wA = wB;
 #pragma unroll
     for (int t = 0; t < 5; ++t) win[t] = win[t + 1];
-    win[5] = (i + 1 < iend) ? u[iu(d, I + 4, Jc, Kc)] : 0.0f;
+    win[5] = (i + 1 < iend) ? ld32(u, ou + 4 * usx) : 0.0f;
+    ou += usx;
+    ov += usxv;
+    ow += usxw;
+    oi += usi;
+    op += usx;
     __syncthreads();
   }
 }
@@ -370,46 +394,58 @@ __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict_
   const int K = k + 1;
   const int Kc = act ? K : 1, Jc = (j < d.ny) ? j + 3 : 3;
   const bool last_row = (ty == kFTY - 1) || (j == d.ny - 1);
-  const long sx = d.sxv();
   const float r = rc[Kc];
   const float ir = 1.0f / r;   // one division per thread instead of one per cell and field
+  const float r8lo = (Kc <= 0 || Kc >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc - 1] + rc[Kc]);
+  const float r8hi = (Kc + 1 >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc] + rc[Kc + 1]);
+  const float r8w = 0.5f * (rc[Kc - 1] + rc[Kc]);
   const int iend = min(i0 + kFTX, d.nx);
   const PlaneSlots psl = plane_slots(tid, nt, Lz, j0, d.ny + 6, (int)d.syc());
+  const uint32_t usx = (uint32_t)d.sxv(), usy = (uint32_t)d.syc();
+  const uint32_t usxu = (uint32_t)d.sxu(), usxw = (uint32_t)d.sxw(), usyw = (uint32_t)d.syw();
+  const uint32_t usi = (uint32_t)(d.ny + 1) * (uint32_t)d.nz;
+  // offsets of the point at x = I: ov (v), ou (u), ow (w), oi the interior (nx, ny+1, nz)
+  // arrays (sgs_v, rv), op the first element of v's x-plane
+  uint32_t ov = (uint32_t)(i0 + 3) * usx + (uint32_t)Jc * usy + (uint32_t)Kc;
+  uint32_t ou = (uint32_t)(i0 + 3) * usxu + (uint32_t)Jc * usy + (uint32_t)Kc;
+  uint32_t ow = (uint32_t)(i0 + 3) * usxw + (uint32_t)Jc * usyw + (uint32_t)Kc;
+  uint32_t oi = (uint32_t)i0 * usi + (uint32_t)(Jc - 3) * (uint32_t)d.nz + (uint32_t)(Kc - 1);
+  uint32_t op = (uint32_t)(i0 + 3) * usx;
   float win[6];
   float fxlo;
   float uw0, uw1;   // u at (I, J-1) and (I, J): the west corners of the v point
   {
-    const int I = i0 + 3;
-    long b = iv(d, I, Jc, Kc);
-    float m3 = v[b - 3 * sx];
+    float m3 = ld32(v, ov - 3 * usx);
 #pragma unroll
-    for (int t = 0; t < 6; ++t) win[t] = v[b + (t - 2) * sx];
-    uw0 = u[iu(d, I, Jc - 1, Kc)];
-    uw1 = u[iu(d, I, Jc, Kc)];
+    for (int t = 0; t < 6; ++t) win[t] = ld32(v, ov + (uint32_t)(t - 2) * usx);
+    uw0 = ld32(u, ou - usy);
+    uw1 = ld32(u, ou);
     float mass = 0.5f * (r * uw0 + r * uw1);
     fxlo = mass * hrecon(m3, win[0], win[1], win[2], win[3], win[4], upwind_pos(mass), weps);
   }
   PlaneRegs pr;
-  plane_fetch(pr, v + (long)(i0 + 3) * sx, psl);
+  plane_fetch32(pr, v, op, psl);
   for (int i = i0; i < iend; ++i) {
-    const int I = i + 3;
     const float vc = win[2];
-    const float ue0 = u[iu(d, I + 1, Jc - 1, Kc)], ue1 = u[iu(d, I + 1, Jc, Kc)];
+    const float ue0 = ld32(u, ou + usxu - usy), ue1 = ld32(u, ou + usxu);
     const float mx = 0.5f * (r * ue0 + r * ue1);
     const float fxhi = mx * hrecon(win[0], win[1], win[2], win[3], win[4], win[5], upwind_pos(mx), weps);
-    const float fd0 = fd_cell(d, rc, u, v, w, I, Jc - 1, Kc);
-    const float fd1 = fd_cell(d, rc, u, v, w, I, Jc, Kc);
-    // operands of the z mass flux and the SGS add-in, issued before the barrier
-    const float wA = w[iw(d, I, Jc - 1, Kc)], wB = w[iw(d, I, Jc, Kc)];
-    const float sgv = sgs ? sgs[((long)i * (d.ny + 1) + (Jc - 3)) * d.nz + (Kc - 1)] : 0.0f;
+    // operands of the flow divergence at rows J-1 and J, the z mass flux and the SGS
+    // add-in, issued before the barrier
+    const float vS = ld32(v, ov - usy), vN = ld32(v, ov + usy);
+    const float wA = ld32(w, ow - usyw), wA1 = ld32(w, ow - usyw + 1);
+    const float wB = ld32(w, ow), wB1 = ld32(w, ow + 1);
+    const float sgv = sgs ? ld32(sgs, oi) : 0.0f;
+    const float fd0 = fd_ops(d, r, r8lo, r8hi, ue0, uw0, vc, vS, wA1, wA);   // fd_cell at (I, J-1)
+    const float fd1 = fd_ops(d, r, r8lo, r8hi, ue1, uw1, vN, vc, wB1, wB);   // fd_cell at (I, J)
     plane_store(P, pr, psl);  // fetched one plane ahead
-    if (i + 1 < iend) plane_fetch(pr, v + (long)(I + 1) * sx, psl);
+    if (i + 1 < iend) plane_fetch32(pr, v, op + usx, psl);
     __syncthreads();
     const float* Pc = P + (ty + 3) * Lz;
     if (act) {
       float fz = 0.0f;
       if (k > 0) {
-        float m = 0.5f * (0.5f * (rc[K - 1] + rc[K]) * wA + 0.5f * (rc[K - 1] + rc[K]) * wB);
+        float m = 0.5f * (r8w * wA + r8w * wB);
         fz = m * vrecon(Pc, Lz, K - 1, upwind_pos(m), weps);
       } else if (tau_y) {   // advection.py:83-88
         const int jl = (j == 0) ? d.ny - 1 : j - 1;
@@ -435,15 +471,20 @@ __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict_
       // - fu: f times the mean of the four u around the v point (u at I, I+1 and J-1, J)
       if (cor.mode) a = a - cor.of(((uw0 + uw1) + ue0) + ue1, cor.ug);
       if (sgs) a = a + sgv;
-      rv[((long)i * (d.ny + 1) + j) * d.nz + k] = a;
-      if (j == 0) rv[((long)i * (d.ny + 1) + d.ny) * d.nz + k] = a;
+      st32(rv, oi, a);                                  // ((i (ny+1) + j) nz + k)
+      if (j == 0) st32(rv, oi + (uint32_t)d.ny * (uint32_t)d.nz, a);
     }
     fxlo = fxhi;
     uw0 = ue0;
     uw1 = ue1;
 #pragma unroll
     for (int t = 0; t < 5; ++t) win[t] = win[t + 1];
-    win[5] = (i + 1 < iend) ? v[iv(d, I + 4, Jc, Kc)] : 0.0f;
+    win[5] = (i + 1 < iend) ? ld32(v, ov + 4 * usx) : 0.0f;
+    ov += usx;
+    ou += usxu;
+    ow += usxw;
+    oi += usi;
+    op += usx;
     __syncthreads();
   }
 }
@@ -454,7 +495,9 @@ __global__ void __launch_bounds__(1024) k_mom_v(Dims d, const float* __restrict_
 // vertical flux at scalar level l+1 (its upper face); the lid level is written by the
 // top lane.
 // ---------------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_mom_w(Dims d, const float* __restrict__ rc, const float* __restrict__ th0c,
+// 80 VGPRs at most: six waves per SIMD, three 512-thread workgroups per CU (at 93 the
+// compiler's choice, two)
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) k_mom_w(Dims d, const float* __restrict__ rc, const float* __restrict__ th0c,
                                                  const float* __restrict__ qv0c, const float* __restrict__ u,
                                                  const float* __restrict__ v, const float* __restrict__ w,
                                                  const float* __restrict__ th, const float* __restrict__ qv,
@@ -479,38 +522,54 @@ __global__ void __launch_bounds__(1024) k_mom_w(Dims d, const float* __restrict_
   const bool last_row = (ty == kFTY - 1) || (j == d.ny - 1);
   const bool interior = act && l >= 1;              // levels 1..nz-1 carry advection
   const float irho8w = 1.0f / (0.5f * (rc[Kc - 1] + rc[Kc]));
-  const long sx = d.sxw();
+  // fd_cell's weights at cell Kc
+  const float rK = rc[Kc];
+  const float r8lo = (Kc <= 0 || Kc >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc - 1] + rc[Kc]);
+  const float r8hi = (Kc + 1 >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc] + rc[Kc + 1]);
   const int iend = min(i0 + kFTX, d.nx);
   const PlaneSlots psl = plane_slots(tid, nt, Lw, j0, d.ny + 5, (int)d.syw());
+  const uint32_t usx = (uint32_t)d.sxw(), usyw = (uint32_t)d.syw();
+  const uint32_t usxu = (uint32_t)d.sxu(), usy = (uint32_t)d.syc(), usxv = (uint32_t)d.sxv();
+  const uint32_t usi = (uint32_t)d.ny * (uint32_t)(d.nz + 1);
+  // offsets of the point at x = I: ow (w), ou (u and the scalars, which share u's
+  // strides), ov (v), oi the row of the interior (nx, ny, nz+1) arrays (sgs_w, rw), op the
+  // first element of w's x-plane
+  uint32_t ow = (uint32_t)(i0 + 3) * usx + (uint32_t)Jc * usyw + (uint32_t)Kc;
+  uint32_t ou = (uint32_t)(i0 + 3) * usxu + (uint32_t)Jc * usy + (uint32_t)Kc;
+  uint32_t ov = (uint32_t)(i0 + 3) * usxv + (uint32_t)Jc * usy + (uint32_t)Kc;
+  uint32_t oi = (uint32_t)i0 * usi + (uint32_t)(Jc - 3) * (uint32_t)(d.nz + 1);
+  uint32_t op = (uint32_t)(i0 + 3) * usx;
   float win[6];
   float fxlo;
+  float uI;   // u at (I, J, K): the west face of the cell, carried from the previous step
   {
-    const int I = i0 + 3;
-    long b = iw(d, I, Jc, Kc);
-    float m3 = w[b - 3 * sx];
+    float m3 = ld32(w, ow - 3 * usx);
 #pragma unroll
-    for (int t = 0; t < 6; ++t) win[t] = w[b + (t - 2) * sx];
-    float mass = 0.5f * (rc[Kc - 1] * u[iu(d, I, Jc, Kc - 1)] + rc[Kc] * u[iu(d, I, Jc, Kc)]);
+    for (int t = 0; t < 6; ++t) win[t] = ld32(w, ow + (uint32_t)(t - 2) * usx);
+    uI = ld32(u, ou);
+    float mass = 0.5f * (rc[Kc - 1] * ld32(u, ou - 1) + rc[Kc] * uI);
     fxlo = mass * hrecon(m3, win[0], win[1], win[2], win[3], win[4], upwind_pos(mass), weps);
   }
   PlaneRegs pr;
-  plane_fetch(pr, w + (long)(i0 + 3) * sx, psl);
+  plane_fetch32(pr, w, op, psl);
   for (int i = i0; i < iend; ++i) {
-    const int I = i + 3;
     const float wc = win[2];
-    const float mx = 0.5f * (rc[Kc - 1] * u[iu(d, I + 1, Jc, Kc - 1)] + rc[Kc] * u[iu(d, I + 1, Jc, Kc)]);
+    const float uE0 = ld32(u, ou + usxu - 1), uE = ld32(u, ou + usxu);   // u at (I+1, J, K-1), (I+1, J, K)
+    const float mx = 0.5f * (rc[Kc - 1] * uE0 + rc[Kc] * uE);
     const float fxhi = mx * hrecon(win[0], win[1], win[2], win[3], win[4], win[5], upwind_pos(mx), weps);
-    // operands of the y mass fluxes and the SGS add-in, issued before the barrier
-    const float vA = v[iv(d, I, Jc, Kc - 1)], vB = v[iv(d, I, Jc, Kc)];
-    const float vC = v[iv(d, I, Jc + 1, Kc - 1)], vD = v[iv(d, I, Jc + 1, Kc)];
-    const long ow = ((long)i * d.ny + (Jc - 3)) * (d.nz + 1);
-    const float sgl = sgs ? sgs[ow + (Kc - 1)] : 0.0f, sgt = sgs ? sgs[ow + d.nz] : 0.0f;
+    // operands of the y mass fluxes, the flow divergence, the buoyancy and the SGS add-in,
+    // issued before the barrier
+    const float vA = ld32(v, ov - 1), vB = ld32(v, ov);
+    const float vC = ld32(v, ov + usy - 1), vD = ld32(v, ov + usy);
+    const float wT = ld32(w, ow + 1);
+    const float thc = ld32(th, ou), qvc = ld32(qv, ou);
+    const float sgl = sgs ? ld32(sgs, oi + (uint32_t)(Kc - 1)) : 0.0f;
+    const float sgt = sgs ? ld32(sgs, oi + (uint32_t)d.nz) : 0.0f;
     plane_store(P, pr, psl);  // fetched one plane ahead
-    if (i + 1 < iend) plane_fetch(pr, w + (long)(I + 1) * sx, psl);
+    if (i + 1 < iend) plane_fetch32(pr, w, op + usx, psl);
     if (act) {
-      FD[ty * nzl + l] = fd_cell(d, rc, u, v, w, I, Jc, K);
-      long c = ic(d, I, Jc, K);
-      BB[ty * nzl + l] = buoy_at(th0c, qv0c, th[c], qv[c], K);
+      FD[ty * nzl + l] = fd_ops(d, rK, r8lo, r8hi, uE, uI, vD, vB, wT, wc);   // fd_cell at cell l
+      BB[ty * nzl + l] = buoy_at(th0c, qv0c, thc, qvc, K);
     }
     __syncthreads();
     const float* Pc = P + (ty + 3) * Lw;
@@ -535,7 +594,6 @@ __global__ void __launch_bounds__(1024) k_mom_w(Dims d, const float* __restrict_
     if (act) {
       const float* Bb = BB + ty * nzl;
       const int nz = d.nz;
-      long o = ((long)i * d.ny + j) * (nz + 1);
       float b8;
       float a = 0.0f;
       if (l == 0) {
@@ -549,18 +607,24 @@ __global__ void __launch_bounds__(1024) k_mom_w(Dims d, const float* __restrict_
       }
       a = a + b8;
       if (sgs) a = a + sgl;
-      rw[o + l] = a;
+      st32(rw, oi + (uint32_t)l, a);                    // ((i ny + j) (nz+1) + l)
       if (l == nz - 1) {   // lid level: zero advection, extrapolated buoyancy
         float bt = 1.875f * Bb[nz - 1] + (-1.25f) * Bb[nz - 2] + 0.375f * Bb[nz - 3];
         float at = 0.0f + bt;
         if (sgs) at = at + sgt;
-        rw[o + nz] = at;
+        st32(rw, oi + (uint32_t)nz, at);
       }
     }
     fxlo = fxhi;
+    uI = uE;
 #pragma unroll
     for (int t = 0; t < 5; ++t) win[t] = win[t + 1];
-    win[5] = (i + 1 < iend) ? w[iw(d, I + 4, Jc, Kc)] : 0.0f;
+    win[5] = (i + 1 < iend) ? ld32(w, ow + 4 * usx) : 0.0f;
+    ow += usx;
+    ou += usxu;
+    ov += usxv;
+    oi += usi;
+    op += usx;
     __syncthreads();
   }
 }
