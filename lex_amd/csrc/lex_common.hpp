@@ -81,40 +81,59 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 // ~1.8e19 squares to +inf with or without it, and a ratio below 1e30 passes unchanged
 // (tau and beta + eps are finite and beta + eps > 0, so the ratio is never NaN for
 // finite data).  The FP32 kernels therefore omit it.
+//
+// Evaluated as c + sum_k a_k (q_k - c) / sum_k a_k, the same function as
+// sum_k a_k q_k / sum_k a_k, in the first differences E = c - b, D = d - c and the
+// second differences t1, t3, t5 centred at b, c, d, which every indicator and
+// candidate shares:
+//   q1 - c = t1/3 + E/2,  q2 - c = (D - (b - d))/6,  q3 - c = D/2 - t5/6,
+//   a - 4b + 3c = t1 + 2E,  3c - 4d + e = t5 - 2D,
+// with the indicators scaled by 4 (B_k = 4 beta_k, eps' = 4 eps, so tau/(beta_k + eps)
+// is unchanged) and the linear weights folded into the candidates.  47 operations
+// instead of 56, and the correction to c is formed without the cancellation of
+// sum_k a_k q_k for a field far from zero (theta ~ 300 K).
 __device__ __forceinline__ float weno5(float a, float b, float c, float d, float e, float weps) {
-  const float c1312 = (float)(13.0 / 12.0);
-  float t1 = a - 2.0f * b + c, t2 = a - 4.0f * b + 3.0f * c;
-  float b1 = c1312 * t1 * t1 + 0.25f * t2 * t2;
-  float t3 = b - 2.0f * c + d, t4 = b - d;
-  float b2 = c1312 * t3 * t3 + 0.25f * t4 * t4;
-  float t5 = c - 2.0f * d + e, t6 = 3.0f * c - 4.0f * d + e;
-  float b3 = c1312 * t5 * t5 + 0.25f * t6 * t6;
-  float tau = fabsf(b1 - b3);
-  float r1 = tau * frcp(b1 + weps);
-  float r2 = tau * frcp(b2 + weps);
-  float r3 = tau * frcp(b3 + weps);
-  float w1 = 0.1f * (1.0f + r1 * r1);
-  float w2 = 0.6f * (1.0f + r2 * r2);
-  float w3 = 0.3f * (1.0f + r3 * r3);
-  const float s6 = (float)(1.0 / 6.0);
-  float q1 = (float)(2.0 / 6.0) * a + (float)(-7.0 / 6.0) * b + (float)(11.0 / 6.0) * c;
-  float q2 = -s6 * b + (float)(5.0 / 6.0) * c + (float)(2.0 / 6.0) * d;
-  float q3 = (float)(2.0 / 6.0) * c + (float)(5.0 / 6.0) * d - s6 * e;
-  return (w1 * q1 + w2 * q2 + w3 * q3) * frcp(w1 + w2 + w3);
+  const float k13 = (float)(13.0 / 3.0);   // 4 x 13/12
+  const float E = c - b, D = d - c;
+  const float t1 = a - 2.0f * b + c;
+  const float t3 = b - 2.0f * c + d;
+  const float t5 = c - 2.0f * d + e;
+  const float t2 = t1 + 2.0f * E;          // a - 4b + 3c
+  const float t4 = b - d;
+  const float t6 = t5 - 2.0f * D;          // 3c - 4d + e
+  const float B1 = (k13 * t1) * t1 + t2 * t2;
+  const float B2 = (k13 * t3) * t3 + t4 * t4;
+  const float B3 = (k13 * t5) * t5 + t6 * t6;
+  const float eps4 = 4.0f * weps;
+  const float tau = fabsf(B1 - B3);
+  const float r1 = tau * frcp(B1 + eps4);
+  const float r2 = tau * frcp(B2 + eps4);
+  const float r3 = tau * frcp(B3 + eps4);
+  const float e1 = 1.0f + r1 * r1, e2 = 1.0f + r2 * r2, e3 = 1.0f + r3 * r3;
+  // d_k (q_k - c) with d = (0.1, 0.6, 0.3)
+  const float g1 = (float)(0.1 / 3.0) * t1 + 0.05f * E;
+  const float g2 = 0.1f * (D - t4);
+  const float g3 = 0.15f * D - 0.05f * t5;
+  const float num = e1 * g1 + e2 * g2 + e3 * g3;
+  const float den = 0.1f * e1 + 0.6f * e2 + 0.3f * e3;
+  return c + num * frcp(den);
 }
 
 // WENO3 face value from (a, b | c), b upwind (advection.py:329-340); the 1e30 cap is
-// omitted for the reason given at weno5.
+// omitted for the reason given at weno5.  Weights (1/3, 2/3) (1 + (tau/(beta_k+eps))^2)
+// on the candidates -a/2 + 3b/2 and (b + c)/2, evaluated as b + (correction) in the
+// differences P = b - a, Q = c - b like weno5.
 __device__ __forceinline__ float weno3(float a, float b, float c, float weps) {
-  float b1 = (a - b) * (a - b);
-  float b2 = (b - c) * (b - c);
-  float tau = fabsf(b1 - b2);
-  float r1 = tau * frcp(b1 + weps);
-  float r2 = tau * frcp(b2 + weps);
-  float w1 = (float)(1.0 / 3.0) * (1.0f + r1 * r1);
-  float w2 = (float)(2.0 / 3.0) * (1.0f + r2 * r2);
-  return (w1 * (-0.5f * a + 1.5f * b) + w2 * (0.5f * b + 0.5f * c)) * frcp(w1 + w2);
+  const float P = b - a, Q = c - b;
+  const float b1 = P * P, b2 = Q * Q;
+  const float tau = fabsf(b1 - b2);
+  const float r1 = tau * frcp(b1 + weps);
+  const float r2 = tau * frcp(b2 + weps);
+  const float e1 = 1.0f + r1 * r1, e2 = 1.0f + r2 * r2;
+  // (1/3) e1 (q1 - b) + (2/3) e2 (q2 - b) over (1/3) e1 + (2/3) e2, times 3 above and below
+  return b + (e1 * (0.5f * P) + e2 * Q) * frcp(e1 + 2.0f * e2);
 }
+
 
 // Horizontal reconstruction at the flux point between q[m-1] and q[m] from the six
 // values q[m-3..m+2] (advection.py:425-470): upwind side chosen by the mass-flux sign.

@@ -124,7 +124,9 @@ def test_config1_nsound7_moist_passive_vs_oracle(shape, impl):
 @pytest.mark.timeout(600)
 def test_config0_full_size_steps_vs_oracle(monkeypatch):
     """The pseudo-incompressible recipe at its full 64x64x40 for 4 steps (12 BiCGSTAB
-    solves): fields vs the FP64/FP32 oracle and every solve's exit code vs the oracle's."""
+    solves): fields vs the FP64 oracle within 3x the spread of two FP32 oracle runs
+    (the second from theta moved by one ulp at random), and every solve's exit code vs
+    the oracle's."""
     from lex_amd import namelist_n_constants as nl, one_sprint_functions as sp
     from lex_amd import one_step_integration as one
     from lex_amd.setup_lex import build_case, config0
@@ -162,18 +164,33 @@ def test_config0_full_size_steps_vs_oracle(monkeypatch):
     k64 = list(k_o)
     r32 = _oracle_run(init, b, og, oo, nsteps, np.float32)
     k32 = k_o[len(k64):]
+    # a second FP32 sample of the rounding noise: the same run from theta moved by one
+    # ulp up or down at random in every cell (zero-mean, like rounding).  The solves
+    # stagnate at maxiter, where rounding differences are amplified, so any FP32
+    # evaluation order (the GPU's is another one) lands within the spread of such samples
+    # around the FP64 result, not at the FP32 oracle's own point.  (A uniform one-ulp
+    # shift of theta is a bias, not noise: it moves u by ~45 % here.)
+    init_p = list(init)
+    th32 = np.asarray(init[0], np.float32)
+    up = np.random.default_rng(3).random(th32.shape) < 0.5
+    init_p[0] = np.where(up, np.nextafter(th32, np.float32(np.inf)),
+                         np.nextafter(th32, np.float32(-np.inf))).astype(np.float64)
+    r32p = _oracle_run(tuple(init_p), b, og, oo, nsteps, np.float32)
+    k32p = k_o[len(k64) + len(k32):]
     errs = _errors(got, r64, r32, init, demean_pip=True)
-    print({k: f"{v[0]:.2e}/{v[1]:.2e}" for k, v in errs.items()}, "iters", k_gpu, k64, k32)
+    errs_p = _errors(r32p, r64, r32, init, demean_pip=True)
+    print({k: f"{v[0]:.2e}/{v[1]:.2e}/{errs_p[k][0]:.2e}" for k, v in errs.items()}, "iters", k_gpu, k64, k32)
     assert len(k_gpu) == len(k64) == 3 * nsteps
     # the affine pressure system is not consistent to the solver's tolerance, so every
     # solve runs to maxiter in both precisions of the oracle and on the GPU
-    assert k_gpu == k64 == k32
+    assert k_gpu == k64 == k32 == k32p
     for n in NAMES[:6]:
         e_g, e_32, norm = errs[n]
         if n == "qv":
             assert norm == 0.0 and np.max(np.abs(got[5] - init[5])) == 0.0   # dry recipe
             continue
-        assert e_g <= max(3.0 * e_32, 1e-6), f"{n}: GPU {e_g:.3e} vs FP32 oracle {e_32:.3e}"
+        spread = max(e_32, errs_p[n][0])
+        assert e_g <= max(3.0 * spread, 1e-6), f"{n}: GPU {e_g:.3e} vs FP32 oracle spread {spread:.3e}"
 
 
 @pytest.mark.parametrize("seed", [4, 5, 6])
