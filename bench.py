@@ -15,15 +15,18 @@ Prints one JSON line: cell-steps/s from an uninstrumented timed loop; the domina
 kernel's roofline (algorithmic bytes per launch / its average launch time from HIP
 events around each launch in a separate profiled loop, against the 8 TB/s HBM peak); the
 per-variable error of a GPU run against the FP64 and FP32 NumPy oracle on a sub-grid of
-the same recipe, the FP32 oracle's time on that sub-grid being the CPU baseline (one
-core); and the pseudo-incompressible 64x64x40 recipe (BiCGSTAB iterations per solve and
-the BiCGSTAB iteration's roofline).
+the same recipe; the CPU baseline: a few steps of the full workload in the C++/OpenMP
+FP32 restatement of the step on every host core the OpenMP runtime is given (with the
+GPU-vs-CPU error of those steps at full size, and the 1-core NumPy oracle's time on the
+sub-grid beside it); and the pseudo-incompressible 64x64x40 recipe (BiCGSTAB iterations
+per solve and the BiCGSTAB iteration's roofline).
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -138,6 +141,75 @@ def errors_and_cpu_baseline(steps=3, n_sound=7, nx=64, ny=64, nz=128, time_cpu=T
                "sample": f"{steps} WS-RK3 steps of the same recipe on a {nx}x{ny}x{nz} sub-grid "
                          f"(NumPy FP32 oracle, single-threaded), {secs['fp32']:.1f} s"}
     return errs, cpu
+
+
+def _cpu_model():
+    try:
+        r = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10)
+        for ln in r.stdout.splitlines():
+            if ln.startswith("Model name:"):
+                return ln.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _merge_cpu(port, numpy_1core):
+    """The C++ all-core port is the CPU baseline; the 1-core NumPy oracle time rides along."""
+    if port is None:
+        return numpy_1core
+    out = dict(port)
+    if numpy_1core is not None:
+        out["numpy_oracle_1_core"] = numpy_1core
+    return out
+
+
+def _merge_errors(errs, full):
+    if errs is None:
+        return full
+    if full is not None:
+        errs = dict(errs)
+        errs["full_size"] = full
+    return errs
+
+
+def cpu_port_baseline(init, base, case, opts, n_sound, steps):
+    """The CPU baseline: `steps` WS-RK3 steps of the full workload from its initial state
+    on every host core the OpenMP runtime is given, in the C++ FP32 restatement of the
+    step (oracle/lex_cpu_step.cpp, pinned to the NumPy oracle by the CPU tests); the same
+    steps on the GPU from the same state give a full-size GPU-vs-CPU comparison."""
+    import numpy as np
+    import torch
+    from lex_amd import _lib, one_sprint_functions as sp
+    from oracle import lex_cpu, lex_oracle as o
+    host0 = tuple(a.cpu().numpy() for a in init)
+    cols = [np.ascontiguousarray(base[i][3, 3, :].cpu().numpy()) for i in (1, 2, 4, 3)]
+    og = o.Grid(case.nx, case.ny, case.nz, case.dx, case.dy, case.dz)
+    t0 = time.perf_counter()
+    cpu = lex_cpu.rk3_steps(host0, cols, og, opts["dt"], n_sound, steps, np.float32)
+    secs = time.perf_counter() - t0
+    gg = _lib.grid(case.nx, case.ny, case.nz, case.dx, case.dy, case.dz)
+    st = tuple(a.clone() for a in init)
+    for _ in range(steps):
+        sp.fused_step(st, base, gg, (2, False, False, 0, False, False, 1), n_sound=n_sound)
+    torch.cuda.synchronize()
+    gpu = tuple(a.double().cpu().numpy() for a in st)
+    ncell = case.nx * case.ny * case.nz
+    base_line = {"value": ncell * steps / secs, "unit": "cell-steps/s", "cores": lex_cpu.threads(), "kind": "port",
+                 "sample": f"{steps} WS-RK3 steps of the full {case.nx}x{case.ny}x{case.nz} workload (n_sound "
+                           f"{n_sound}, theta, qv, qc, qr, Smagorinsky), C++/OpenMP FP32 restatement of the step "
+                           f"(oracle/lex_cpu_step.cpp), {secs:.1f} s",
+                 "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
+    errs = _field_errors(gpu, tuple(np.asarray(a, np.float64) for a in cpu), tuple(np.asarray(a, np.float64) for a in host0))
+    return base_line, {"workload": f"{steps} WS-RK3 steps of the full {case.nx}x{case.ny}x{case.nz} workload",
+                       "gpu_vs_cpu_port_fp32": errs}
 
 
 def c0_timing(steps=10, warmup=2):
@@ -682,6 +754,9 @@ def main():
     if args.profile:
         print(json.dumps(kernels, indent=1), file=sys.stderr)
 
+    cpu_port, err_full = None, None
+    if not args.no_cpu_baseline:
+        cpu_port, err_full = cpu_port_baseline(init, base, case, opts, n_sound, args.cpu_sample_steps)
     if args.copy_probe:
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -727,8 +802,8 @@ def main():
                      "avg_launch_source": f"HIP events around each launch on its stream, separate loop of {psteps} "
                                           "steps",
                      "copy_GBs": cb["GBs"], "frac_of_copy": achieved / cb["GBs"]},
-        "cpu_baseline": cpu,
-        "errors": errs,
+        "cpu_baseline": _merge_cpu(cpu_port, cpu),
+        "errors": _merge_errors(errs, err_full),
         "c0": c0,
         "as_specified": as_specified,
         "implicit_as_specified": impl,

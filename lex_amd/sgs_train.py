@@ -15,7 +15,8 @@ frozen over the step (lex_rk3_step_ext_sgs), or the net runs at the start of eve
 RK3 stage on that stage's state (lex_rk3_step_stage_sgs).  Coarse runs from C(truth(0))
 of a held-out member with (a) no SGS, (b) Smagorinsky, (c) the frozen net and (d) the
 per-stage net are scored by the RMS of theta' against the coarsened truth; Smagorinsky
-stays the fallback when the net is worse.
+stays the fallback when the net is worse.  The full workload scores two held-out members
+over 450 coarse steps (1800 s) each.
 
 Every array operation on fields runs in liblexhip; torch holds tensors, runs the
 optimiser and computes scalar statistics.
@@ -39,9 +40,9 @@ from .setup_lex import BubbleCase, build_case
 from .sgs_net import SgsNet
 
 FULL = dict(nx=576, ny=576, nz=144, dx=100.0, dz=100.0, dt=1.0, n_sound=7, f=(6, 6, 3), dtc=4.0, n_sound_c=10,
-            members=8, snaps=40, gap=11, eval_steps=150, epochs=12)
+            members=8, snaps=40, gap=11, eval_steps=450, eval_members=2, epochs=12)
 SMALL = dict(nx=72, ny=72, nz=36, dx=100.0, dz=100.0, dt=1.0, n_sound=7, f=(6, 6, 3), dtc=4.0, n_sound_c=10,
-             members=2, snaps=4, gap=3, eval_steps=8, epochs=2)
+             members=2, snaps=4, gap=3, eval_steps=8, eval_members=1, epochs=2)
 MO_NONE = (2, False, False, 0, False, False, 0)
 MO_SMAG = (2, False, False, 0, False, False, 1)
 
@@ -320,16 +321,29 @@ def run(cfg, log=print, device="cuda"):
     hist = train(net, model, pairs, targets, cfg["epochs"], 1e-3, log)
     torch.cuda.synchronize()
     t_train = time.time() - t0
-    truth = truth_trajectory(cfg, 2031 + cfg["members"], model, log)
-    scores = evaluate(model, net, truth)
-    fin = {k: v[-1] for k, v in scores.items()}
-    mean = {k: float(np.mean(v)) for k, v in scores.items()}
+    # held-out members (seeds after the training ones), each scored from its coarsened
+    # initial state over eval_steps coarse steps
+    per_member, curves = [], {}
+    for e in range(cfg.get("eval_members", 1)):
+        seed = 2031 + cfg["members"] + e
+        truth = truth_trajectory(cfg, seed, model, log)
+        scores = evaluate(model, net, truth)
+        del truth
+        per_member.append(dict(seed=seed, theta_rmse_final={k: v[-1] for k, v in scores.items()},
+                               theta_rmse_mean={k: float(np.mean(v)) for k, v in scores.items()}))
+        curves[seed] = {k: v[:: max(1, len(v) // 20)] for k, v in scores.items()}
+        log(f"held-out seed {seed}: mean theta' RMSE " +
+            ", ".join(f"{k} {v:.5f}" for k, v in per_member[-1]["theta_rmse_mean"].items()))
+    modes = list(per_member[0]["theta_rmse_mean"])
+    fin = {k: float(np.mean([p["theta_rmse_final"][k] for p in per_member])) for k in modes}
+    mean = {k: float(np.mean([p["theta_rmse_mean"][k] for p in per_member])) for k in modes}
     nets = [m for m in ("net", "net_stage") if math.isfinite(mean[m]) and mean[m] <= mean["smag"]]
     chosen = min(nets, key=lambda m: mean[m]) if nets else "smagorinsky"
     return dict(fine=[cfg["nx"], cfg["nx"], cfg["nz"]], coarse=[nxc, nxc, nzc], pairs=len(pairs),
-                target_std=tsd, train_mse=hist, theta_rmse_final=fin, theta_rmse_mean=mean, chosen=chosen,
+                target_std=tsd, train_mse=hist, eval_seconds=cfg["eval_steps"] * cfg["dtc"],
+                theta_rmse_final=fin, theta_rmse_mean=mean, per_member=per_member, chosen=chosen,
                 seconds=dict(generate=t_gen, train=t_train, total=time.time() - t_start),
-                rmse_curves={k: v[:: max(1, len(v) // 20)] for k, v in scores.items()})
+                rmse_curves=curves)
 
 
 def main(argv=None):
@@ -339,6 +353,7 @@ def main(argv=None):
     ap.add_argument("--snaps", type=int)
     ap.add_argument("--epochs", type=int)
     ap.add_argument("--eval-steps", type=int)
+    ap.add_argument("--eval-members", type=int)
     a = ap.parse_args(argv)
     cfg = dict(SMALL if a.small else FULL)
     for k in ("members", "snaps", "epochs"):
@@ -346,6 +361,8 @@ def main(argv=None):
             cfg[k] = getattr(a, k)
     if a.eval_steps is not None:
         cfg["eval_steps"] = a.eval_steps
+    if a.eval_members is not None:
+        cfg["eval_members"] = a.eval_members
     res = run(cfg, log=lambda s: print(s, file=sys.stderr, flush=True))
     print(json.dumps(res))
 
