@@ -161,7 +161,7 @@ void rk3_forward_tape(const Dims& d, const O& o, const B& b, float* const* X0, c
       long n = (long)L.nc;
       hipLaunchKernelGGL(k_smag_k, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, sm[6], sm[7], sm[8], sm[9],
                          cg * cg, o.prandtl);
-      hipLaunchKernelGGL(k_smag_uvw<false>, cx.G(d.nz + 1, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, sm[8], sm[8], sm[0],
+      hipLaunchKernelGGL(k_smag_uvw<false>, cx.G(d.nz, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, sm[8], sm[8], sm[0],
                          sm[1], sm[2], sm[3], sm[4], sm[5], su, sv, sw);
       for (int q = 0; q < nsc; ++q) {
         int fi = (q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2));

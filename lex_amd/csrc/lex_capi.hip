@@ -498,7 +498,7 @@ int lex_smag_momentum(const lex_grid_t* g, const float* r, const float* km, cons
   REQGRID(g);
   REQ(r); REQ(km); REQ(s11); REQ(s22); REQ(s33); REQ(s12); REQ(s13); REQ(s23); REQ(su); REQ(sv); REQ(sw);
   Dims d = mkdims(g);
-  hipLaunchKernelGGL(k_smag_uvw<false>, grd(d.nz + 1, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, r, km, km, s11, s22, s33,
+  hipLaunchKernelGGL(k_smag_uvw<false>, grd(d.nz, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, r, km, km, s11, s22, s33,
                      s12, s13, s23, su, sv, sw);
   return check_launch("lex_smag_momentum");
 }
@@ -529,7 +529,7 @@ int lex_smag2_momentum(const lex_grid_t* g, const float* r, const float* kmh, co
   REQGRID(g);
   REQ(r); REQ(kmh); REQ(kmv); REQ(s11); REQ(s22); REQ(s33); REQ(s12); REQ(s13); REQ(s23); REQ(su); REQ(sv); REQ(sw);
   Dims d = mkdims(g);
-  hipLaunchKernelGGL(k_smag_uvw<false>, grd(d.nz + 1, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, r, kmh, kmv, s11, s22,
+  hipLaunchKernelGGL(k_smag_uvw<false>, grd(d.nz, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, r, kmh, kmv, s11, s22,
                      s33, s12, s13, s23, su, sv, sw);
   return check_launch("lex_smag2_momentum");
 }

@@ -266,10 +266,13 @@ __global__ void k_smag_uvw(Dims d, const float* __restrict__ rc, const float* __
                            float* __restrict__ sw) {
 #pragma clang fp contract(off)   // the stresses are rounded products whether precomputed or not
   const TileId tb = xcd_tile();
-  int k = tb.x * blockDim.x + threadIdx.x;  // 0..nz
+  // level k of the three outputs; the lid level nz of sgs_w (zero) is written by the
+  // thread of k = nz - 1, so the grid's z extent is nz, not nz + 1 (a workgroup row of
+  // 64 lanes for the one extra level would be idle but for one lane)
+  int k = tb.x * blockDim.x + threadIdx.x;  // 0..nz-1
   int j = tb.y * blockDim.y + threadIdx.y;  // 0..ny
   int i = tb.z;                             // 0..nx
-  if (k > d.nz || j > d.ny) return;
+  if (k >= d.nz || j > d.ny) return;
   const long sx = d.sxc(), sy = d.syc();
   int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
   // interpolate_scalar2w_0 of tau at w level l: zero at l=0, nz
@@ -321,6 +324,7 @@ __global__ void k_smag_uvw(Dims d, const float* __restrict__ rc, const float* __
       val = -(a + b + d33) / r8w;
     }
     sw[((long)i * d.ny + j) * (nz + 1) + k] = val;
+    if (k == nz - 1) sw[((long)i * d.ny + j) * (nz + 1) + nz] = 0.0f;   // the lid level
   }
 }
 

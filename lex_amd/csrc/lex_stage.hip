@@ -371,7 +371,7 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
         T.khv = (o.turb == 2) ? sm[7] : nullptr;
         LEX_LAUNCH(KID_SMAG_FUSED, k_smag_tau, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, src[1], src[2], src[3], src[0],
                    src[5], cg * cg, o.prandtl, (int)(o.turb == 2), o.cs * o.cs, T);
-        LEX_LAUNCH(KID_SMAG_UVW, k_smag_uvw<true>, G(d.nz + 1, d.ny + 1, d.nx + 1), bk, 0, st, d, rc,
+        LEX_LAUNCH(KID_SMAG_UVW, k_smag_uvw<true>, G(d.nz, d.ny + 1, d.nx + 1), bk, 0, st, d, rc,
                    (const float*)nullptr, (const float*)nullptr, sm[0], sm[1], sm[2], sm[3], sm[4], sm[5], su, sv, sw);
         khh = sm[6];
         khv = (o.turb == 2) ? sm[7] : sm[6];
@@ -401,7 +401,7 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
         const float* kmv = (o.turb == 1) ? sm[8] : sm[9];
         khh = (o.turb == 1) ? sm[9] : sm[8];
         khv = sm[9];
-        LEX_LAUNCH(KID_SMAG_UVW, k_smag_uvw<false>, G(d.nz + 1, d.ny + 1, d.nx + 1), bk, 0, st, d, rc, kmh, kmv, sm[0],
+        LEX_LAUNCH(KID_SMAG_UVW, k_smag_uvw<false>, G(d.nz, d.ny + 1, d.nx + 1), bk, 0, st, d, rc, kmh, kmv, sm[0],
                    sm[1], sm[2], sm[3], sm[4], sm[5], su, sv, sw);
         LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, khh, khv, src[0], ss[0]);
         LEX_LAUNCH(KID_SMAG_SCALAR, k_smag_scalar, G(d.nz, d.ny, d.nx), bk, 0, st, d, rc, khh, khv, src[5], ss[1]);
