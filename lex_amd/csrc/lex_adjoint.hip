@@ -852,10 +852,12 @@ __global__ void k_acoustic_vjp_faces3(Dims d, const float* __restrict__ rc, cons
                                       float* __restrict__ lv, float* __restrict__ lw, float* __restrict__ lru,
                                       float* __restrict__ lrv, float* __restrict__ lrw, float* __restrict__ gu,
                                       float* __restrict__ gv, float* __restrict__ gw) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nz
+  // levels 0..nz-1: the lid level nz carries no face of this pass (w levels 1..nz-1
+  // only), so the grid's z extent is nz
+  int k = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nz-1
   int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
   int i = blockIdx.z;                             // 0..nx
-  if (k > d.nz || j > d.ny) return;
+  if (k >= d.nz || j > d.ny) return;
   const float cr = (float)(-kRd / kCv);
   const float g11 = 1.1f * dts;
   const bool du = k < d.nz && j < d.ny;                      // u face (i, j, k)

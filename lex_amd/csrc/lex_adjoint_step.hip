@@ -336,7 +336,7 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
         for (int c = 0; c < 4; ++c) li[c] = lP[t & 1][c];
         cx.zero(li[3], L.nc);
       }
-      hipLaunchKernelGGL(k_acoustic_vjp_faces3, cx.G(d.nz + 1, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, b.pi0,
+      hipLaunchKernelGGL(k_acoustic_vjp_faces3, cx.G(d.nz, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, b.pi0,
                          (const float*)src[4], (const float*)lo[3], (const float*)lo[0], (const float*)lo[1],
                          (const float*)lo[2], dts, acc, li[0], li[1], li[2], lru, lrv, lrw, gU, gV, gW);
       hipLaunchKernelGGL(k_acoustic_vjp_cells3, cx.G(d.nz, d.ny + 2, d.nx + 2), cx.bk, 0, st, d, rc,
