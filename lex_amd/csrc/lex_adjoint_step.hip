@@ -196,13 +196,8 @@ void rk3_forward_tape(const Dims& d, const O& o, const B& b, float* const* X0, c
       }
       size_t lds = stage_scalars_lds(d);
       float dts_f = (float)dt_s;
-      switch (nsc) {
-        case 2: hipLaunchKernelGGL(k_stage_scalars<2>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
-        case 3: hipLaunchKernelGGL(k_stage_scalars<3>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
-        case 4: hipLaunchKernelGGL(k_stage_scalars<4>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
-        case 5: hipLaunchKernelGGL(k_stage_scalars<5>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
-        default: hipLaunchKernelGGL(k_stage_scalars<6>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
-      }
+      hipLaunchKernelGGL(k_stage_scalars, dim3(nsc, fg.x, fg.y), fb, lds, st, d, rc, src[1], src[2], src[3], S, tr,
+                         dts_f);
     } else {
       hipLaunchKernelGGL(k_div_rho, cx.G(d.nz + 2, d.ny + 6, d.nx + 6), cx.bk, 0, st, d, rc, src[1], src[2], src[3],
                          fd);

@@ -1,12 +1,13 @@
 // Fused stage kernels for the device WS-RK3 driver.
 //
-// k_stage_scalars: every scalar's WENO advection + Euler update + frozen-SGS add-in of
+// k_stage_scalars: the scalars' WENO advection + Euler update + frozen-SGS add-in of
 // one stage (update_theta_euler / update_qv_euler, one_step_integration.py:450-465,
 // :158-162), plus the flow divergence (advection.py:200-226) and theta_rho
-// (one_step_integration.py:135) inline, in one launch.
+// (one_step_integration.py:135) inline, in one launch (one scalar per workgroup).
 //
 // Work decomposition (each face flux computed once):
-//   block  = full z column (64*ceil(nz/64) lanes) x TY rows of y; marches TX cells in x
+//   block  = full z column (64*ceil(nz/64) lanes) x TY rows of y of one scalar; marches
+//            TX cells in x
 //   x-face : from a 6-value register window along x; the lower face is the previous
 //            step's upper face
 //   y-face : each row computes its lower face from an LDS plane (rows j0-3..j0+TY+2);
@@ -96,147 +97,6 @@ __device__ __forceinline__ BottomFlux bottom_fluxes(const ScalarSet& S, bool sur
   if (surface && S.sfc[1]) b.f1 = S.sfc[1][p];
   return b;
 }
-
-template <int NS>
-__global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __restrict__ rc,
-                                                         const float* __restrict__ u, const float* __restrict__ v,
-                                                         const float* __restrict__ w, ScalarSet S,
-                                                         float* __restrict__ trho, float dt) {
-  extern __shared__ float lds[];
-  const int nzl = blockDim.x;              // z lanes (multiple of 64, >= nz)
-  const int tz = threadIdx.x, ty = threadIdx.y;
-  const int nt = nzl * kFTY;
-  const int tid = ty * nzl + tz;
-  const int Lz = d.nz + 2;                 // padded scalar column
-  float* P = lds;                          // (kFTY+6) x Lz plane of the current scalar
-  float* FY = P + (kFTY + 6) * Lz;         // (kFTY+1) x nzl lower y-faces (+ tile top)
-  float* FZ = FY + (kFTY + 1) * nzl;       // kFTY x nzl lower z-faces
-
-  const TileId tb = xcd_tile();
-  const int j0 = tb.y * kFTY;
-  const int i0 = tb.x * kFTX;
-  const int k = lane_level(tz, nzl, d.nz), j = j0 + ty;
-  const bool act = (k < d.nz) && (j < d.ny);
-  const int K = k + 1, J = j + 3;
-  const int Kc = act ? K : 1, Jc = (j < d.ny) ? J : 3;   // clamped indices for idle lanes
-  const bool last_row = (ty == kFTY - 1) || (j == d.ny - 1);
-  const long sx = d.sxc(), sy = d.syc();
-  const float r = rc[Kc];
-  const float ir = 1.0f / r;   // one division per thread instead of one per cell and field
-  const float r8lo = (Kc <= 0 || Kc >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc - 1] + rc[Kc]);
-  const float r8hi = (Kc + 1 >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc] + rc[Kc + 1]);
-
-  const PlaneSlots psl = plane_slots(tid, nt, Lz, j0, d.ny + 5, (int)sy);
-  // 32-bit element offsets of this thread's cell at x = I, advanced by one x-plane per
-  // step of the march: oc indexes scalars and u (u has the scalar strides), ov v, ow w,
-  // oi the interior (nx, ny, nz) arrays, op the first row of the LDS plane's source.
-  const uint32_t usx = (uint32_t)sx, usy = (uint32_t)sy;
-  const uint32_t usxv = (uint32_t)d.sxv(), usxw = (uint32_t)d.sxw(), usyw = (uint32_t)d.syw();
-  const uint32_t usi = (uint32_t)d.ny * (uint32_t)d.nz;
-  uint32_t oc = (uint32_t)(i0 + 3) * usx + (uint32_t)Jc * usy + (uint32_t)Kc;
-  uint32_t ov = (uint32_t)(i0 + 3) * usxv + (uint32_t)Jc * usy + (uint32_t)Kc;
-  uint32_t ow = (uint32_t)(i0 + 3) * usxw + (uint32_t)Jc * usyw + (uint32_t)Kc;
-  uint32_t oi = (uint32_t)i0 * usi + (uint32_t)(Jc - 3) * (uint32_t)d.nz + (uint32_t)(Kc - 1);
-  uint32_t op = (uint32_t)(i0 + 3) * usx;
-  // x windows q[I-2..I+3] per scalar and the lower x-face flux
-  float win[NS][6];
-  float fxlo[NS];
-  const int iend = min(i0 + kFTX, d.nx);
-  {
-    const float mlo = r * ld32(u, oc);
-    const bool pos = upwind_pos(mlo);
-#pragma unroll
-    for (int q = 0; q < NS; ++q) {
-      const float* s = S.s[q];
-      float m3 = ld32(s, oc - 3 * usx);
-#pragma unroll
-      for (int t = 0; t < 6; ++t) win[q][t] = ld32(s, oc + (uint32_t)(t - 2) * usx);
-      fxlo[q] = mlo * hrecon(m3, win[q][0], win[q][1], win[q][2], win[q][3], win[q][4], pos, S.weps[q]);
-    }
-  }
-  float ulo = ld32(u, oc);
-  PlaneRegs pr;
-  plane_fetch32(pr, S.s[0], op, psl);
-
-  for (int i = i0; i < iend; ++i) {
-    const float uhi = ld32(u, oc + usx);
-    const float vlo = ld32(v, ov);
-    const float vhi = ld32(v, ov + usy);
-    const float wlo = ld32(w, ow);
-    const float whi = ld32(w, ow + 1);
-    // flow divergence at the cell (get_divergence with uniform rho columns)
-    const float fd = (r * uhi - r * ulo) * d.idx + (r * vhi - r * vlo) * d.idy + (r8hi * whi - r8lo * wlo) * d.idz;
-    const float mx = r * uhi;                      // x-face I+1
-    const float my_lo = r * vlo;                   // y-face J
-    const float my_hi = r * vhi;                   // y-face J+1 (last row only)
-    const float mz_lo = 0.5f * (rc[Kc - 1] + rc[Kc]) * wlo;   // z-face between K-1 and K
-    const bool px = upwind_pos(mx), py0 = upwind_pos(my_lo), py1 = upwind_pos(my_hi), pz = upwind_pos(mz_lo);
-    float thc = 0.f, qvc = 0.f;
-    // the update's xi0 values and frozen SGS tendencies, issued before the first barrier
-    float s0v[NS], sgv[NS];
-#pragma unroll
-    for (int q = 0; q < NS; ++q) {
-      s0v[q] = ld32(S.s0[q], oc);
-      sgv[q] = S.sgs[q] ? ld32(S.sgs[q], oi) : 0.0f;
-    }
-    const float hv = S.heat ? ld32(S.heat, oi) : 0.0f;
-    const BottomFlux bf = bottom_fluxes(S, act && k == 0, (long)i * d.ny + j);
-#pragma unroll
-    for (int q = 0; q < NS; ++q) {
-      const float* s = S.s[q];
-      const float weps = S.weps[q];
-      // plane of this scalar at x = I: rows J = j0 .. j0+kFTY+5 (padded), all levels,
-      // fetched one plane ahead; then fetch the next (scalar q+1 here, or scalar 0 at I+1)
-      plane_store(P, pr, psl);
-      if (q + 1 < NS) plane_fetch32(pr, S.s[(q + 1) % NS], op, psl);
-      else if (i + 1 < iend) plane_fetch32(pr, S.s[0], op + usx, psl);
-      __syncthreads();
-      const float* Pc = P + (ty + 3) * Lz;   // this row's column
-      const float qc = Pc[Kc];
-      if (q == 0) thc = qc;
-      if (q == 1) qvc = qc;
-      if (act) {
-        float fz = (k == 0) ? bf.of(q) : mz_lo * vrecon(Pc, Lz, K - 1, pz, weps);
-        FZ[ty * nzl + k] = fz;
-        float fy = my_lo * hrecon(Pc[Kc - 3 * Lz], Pc[Kc - 2 * Lz], Pc[Kc - Lz], Pc[Kc], Pc[Kc + Lz], Pc[Kc + 2 * Lz],
-                                  py0, weps);
-        FY[ty * nzl + k] = fy;
-        if (last_row) {
-          const float* Pn = Pc + Lz;
-          FY[(ty + 1) * nzl + k] =
-              my_hi * hrecon(Pn[Kc - 3 * Lz], Pn[Kc - 2 * Lz], Pn[Kc - Lz], Pn[Kc], Pn[Kc + Lz], Pn[Kc + 2 * Lz], py1, weps);
-        }
-      }
-      // x-face I+1 from the window q[I-2..I+3]
-      float fxhi = mx * hrecon(win[q][0], win[q][1], win[q][2], win[q][3], win[q][4], win[q][5], px, weps);
-      __syncthreads();
-      if (act) {
-        float fzhi = (k == d.nz - 1) ? 0.0f : FZ[ty * nzl + k + 1];
-        float conv = -((fxhi - fxlo[q]) * d.idx + (FY[(ty + 1) * nzl + k] - FY[ty * nzl + k]) * d.idy +
-                       (fzhi - FZ[ty * nzl + k]) * d.idz);
-        float tend = (conv + qc * fd) * ir;
-        if (q == 0 && S.heat) tend = tend + hv;
-        float nxt = s0v[q] + tend * dt;
-        if (S.sgs[q]) nxt = nxt + sgv[q] * dt;
-        store_c_fast32(d, S.out[q], i, j, k, oc, nxt);
-      }
-      fxlo[q] = fxhi;
-      // advance the window: q[I-1..I+4]
-#pragma unroll
-      for (int t = 0; t < 5; ++t) win[q][t] = win[q][t + 1];
-      win[q][5] = (i + 1 < iend) ? ld32(s, oc + 4 * usx) : 0.0f;
-    }
-    if (act && trho) store_c_fast32(d, trho, i, j, k, oc, theta_rho(thc, qvc));
-    ulo = uhi;
-    oc += usx;
-    ov += usxv;
-    ow += usxw;
-    oi += usi;
-    op += usx;
-  }
-}
-
-
 
 // Flow divergence div(rho0 u) of a cell (get_divergence, advection.py:200-226) for a
 // horizontally uniform rho column, from the cell's face velocities already in registers:
@@ -638,13 +498,154 @@ inline size_t mom_w_lds(const Dims& d) {
   return sizeof(float) * ((size_t)(kFTY + 6) * (d.nz + 3) + (size_t)(4 * kFTY + 1) * nzl);
 }
 
+// The entry of a per-scalar array of ScalarSet for a wave-uniform scalar index q
+// (a select chain on scalar registers, not an indexed copy of the argument struct).
+template <class T>
+__device__ __forceinline__ T pick_q(const T (&a)[kMaxScalars], int q) {
+  T r = a[0];
+#pragma unroll
+  for (int t = 1; t < kMaxScalars; ++t) r = (q == t) ? a[t] : r;
+  return r;
+}
+
+// ---------------------------------------------------------------------------------
+// k_stage_scalars: one scalar's WENO advection + Euler update + frozen-SGS add-in per
+// workgroup.  The grid's fastest dimension is the scalar, so the workgroups of the S.n
+// scalars of one tile are consecutive, run on the same XCD and share the tile's u, v, w
+// lines in its L2; each recomputes the tile's mass fluxes and flow divergence.  Holding
+// one scalar's x window (not all S.n) keeps the kernel at 80 VGPRs: three workgroups per
+// CU, whose barriers and loads overlap (the all-scalars-per-workgroup form needed 119 and
+// ran one fewer, 12 % slower).  theta_rho is written by the workgroups of scalar 0
+// (theta), which read qv at their cells.
+// ---------------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_stage_scalars(Dims d, const float* __restrict__ rc,
+                                                        const float* __restrict__ u, const float* __restrict__ v,
+                                                        const float* __restrict__ w, ScalarSet S,
+                                                        float* __restrict__ trho, float dt) {
+  extern __shared__ float lds[];
+  const int nzl = blockDim.x;
+  const int tz = threadIdx.x, ty = threadIdx.y;
+  const int nt = nzl * kFTY;
+  const int tid = ty * nzl + tz;
+  const int Lz = d.nz + 2;
+  float* P = lds;
+  float* FY = P + (kFTY + 6) * Lz;
+  float* FZ = FY + (kFTY + 1) * nzl;
+
+  const TileId tb = xcd_tile();
+  const int q = tb.x;
+  const int i0 = tb.y * kFTX;
+  const int j0 = tb.z * kFTY;
+  const float* __restrict__ s = pick_q(S.s, q);
+  const float* __restrict__ s0 = pick_q(S.s0, q);
+  const float* __restrict__ sgs = pick_q(S.sgs, q);
+  float* __restrict__ out = pick_q(S.out, q);
+  const float weps = pick_q(S.weps, q);
+  const bool heat = (q == 0) && S.heat;
+  const bool wtr = (q == 0) && trho;
+  const float* __restrict__ sfc = (q == 0) ? S.sfc[0] : ((q == 1) ? S.sfc[1] : nullptr);
+  const int k = lane_level(tz, nzl, d.nz), j = j0 + ty;
+  const bool act = (k < d.nz) && (j < d.ny);
+  const int K = k + 1, J = j + 3;
+  const int Kc = act ? K : 1, Jc = (j < d.ny) ? J : 3;
+  const bool last_row = (ty == kFTY - 1) || (j == d.ny - 1);
+  const long sx = d.sxc(), sy = d.syc();
+  const float r = rc[Kc];
+  const float ir = 1.0f / r;
+  const float r8lo = (Kc <= 0 || Kc >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc - 1] + rc[Kc]);
+  const float r8hi = (Kc + 1 >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kc] + rc[Kc + 1]);
+
+  const PlaneSlots psl = plane_slots(tid, nt, Lz, j0, d.ny + 5, (int)sy);
+  const uint32_t usx = (uint32_t)sx, usy = (uint32_t)sy;
+  const uint32_t usxv = (uint32_t)d.sxv(), usxw = (uint32_t)d.sxw(), usyw = (uint32_t)d.syw();
+  const uint32_t usi = (uint32_t)d.ny * (uint32_t)d.nz;
+  uint32_t oc = (uint32_t)(i0 + 3) * usx + (uint32_t)Jc * usy + (uint32_t)Kc;
+  uint32_t ov = (uint32_t)(i0 + 3) * usxv + (uint32_t)Jc * usy + (uint32_t)Kc;
+  uint32_t ow = (uint32_t)(i0 + 3) * usxw + (uint32_t)Jc * usyw + (uint32_t)Kc;
+  uint32_t oi = (uint32_t)i0 * usi + (uint32_t)(Jc - 3) * (uint32_t)d.nz + (uint32_t)(Kc - 1);
+  uint32_t op = (uint32_t)(i0 + 3) * usx;
+  float win[6];
+  float fxlo;
+  const int iend = min(i0 + kFTX, d.nx);
+  {
+    const float mlo = r * ld32(u, oc);
+    const bool pos = upwind_pos(mlo);
+    const float m3 = ld32(s, oc - 3 * usx);
+#pragma unroll
+    for (int t = 0; t < 6; ++t) win[t] = ld32(s, oc + (uint32_t)(t - 2) * usx);
+    fxlo = mlo * hrecon(m3, win[0], win[1], win[2], win[3], win[4], pos, weps);
+  }
+  float ulo = ld32(u, oc);
+  PlaneRegs pr;
+  plane_fetch32(pr, s, op, psl);
+
+  for (int i = i0; i < iend; ++i) {
+    const float uhi = ld32(u, oc + usx);
+    const float vlo = ld32(v, ov);
+    const float vhi = ld32(v, ov + usy);
+    const float wlo = ld32(w, ow);
+    const float whi = ld32(w, ow + 1);
+    const float fd = (r * uhi - r * ulo) * d.idx + (r * vhi - r * vlo) * d.idy + (r8hi * whi - r8lo * wlo) * d.idz;
+    const float mx = r * uhi;
+    const float my_lo = r * vlo;
+    const float my_hi = r * vhi;
+    const float mz_lo = 0.5f * (rc[Kc - 1] + rc[Kc]) * wlo;
+    const bool px = upwind_pos(mx), py0 = upwind_pos(my_lo), py1 = upwind_pos(my_hi), pz = upwind_pos(mz_lo);
+    const float s0v = ld32(s0, oc);
+    const float sgv = sgs ? ld32(sgs, oi) : 0.0f;
+    const float hv = heat ? ld32(S.heat, oi) : 0.0f;
+    const float qvc = wtr ? ld32(S.s[1], oc) : 0.0f;
+    const float bfv = (sfc && act && k == 0) ? sfc[(long)i * d.ny + j] : 0.0f;
+    plane_store(P, pr, psl);
+    if (i + 1 < iend) plane_fetch32(pr, s, op + usx, psl);
+    __syncthreads();
+    const float* Pc = P + (ty + 3) * Lz;
+    const float qc = Pc[Kc];
+    if (act) {
+      float fz = (k == 0) ? bfv : mz_lo * vrecon(Pc, Lz, K - 1, pz, weps);
+      FZ[ty * nzl + k] = fz;
+      float fy = my_lo * hrecon(Pc[Kc - 3 * Lz], Pc[Kc - 2 * Lz], Pc[Kc - Lz], Pc[Kc], Pc[Kc + Lz], Pc[Kc + 2 * Lz],
+                                py0, weps);
+      FY[ty * nzl + k] = fy;
+      if (last_row) {
+        const float* Pn = Pc + Lz;
+        FY[(ty + 1) * nzl + k] =
+            my_hi * hrecon(Pn[Kc - 3 * Lz], Pn[Kc - 2 * Lz], Pn[Kc - Lz], Pn[Kc], Pn[Kc + Lz], Pn[Kc + 2 * Lz], py1, weps);
+      }
+    }
+    float fxhi = mx * hrecon(win[0], win[1], win[2], win[3], win[4], win[5], px, weps);
+    __syncthreads();
+    if (act) {
+      float fzhi = (k == d.nz - 1) ? 0.0f : FZ[ty * nzl + k + 1];
+      float conv = -((fxhi - fxlo) * d.idx + (FY[(ty + 1) * nzl + k] - FY[ty * nzl + k]) * d.idy +
+                     (fzhi - FZ[ty * nzl + k]) * d.idz);
+      float tend = (conv + qc * fd) * ir;
+      if (heat) tend = tend + hv;
+      float nxt = s0v + tend * dt;
+      if (sgs) nxt = nxt + sgv * dt;
+      store_c_fast32(d, out, i, j, k, oc, nxt);
+      if (wtr) store_c_fast32(d, trho, i, j, k, oc, theta_rho(qc, qvc));
+    }
+    fxlo = fxhi;
+#pragma unroll
+    for (int t = 0; t < 5; ++t) win[t] = win[t + 1];
+    win[5] = (i + 1 < iend) ? ld32(s, oc + 4 * usx) : 0.0f;
+    ulo = uhi;
+    oc += usx;
+    ov += usxv;
+    ow += usxw;
+    oi += usi;
+    op += usx;
+  }
+}
+
 inline size_t stage_scalars_lds(const Dims& d) {
   int nzl = 64 * ((d.nz + 63) / 64);
   return sizeof(float) * ((size_t)(kFTY + 6) * (d.nz + 2) + (size_t)(kFTY + 1) * nzl + (size_t)kFTY * nzl);
 }
 
 // ---------------------------------------------------------------------------------
-// k_stage_scalars_db: the same arithmetic as k_stage_scalars, reorganised for latency:
+// k_stage_scalars_db: the same arithmetic as k_stage_scalars, all scalars per workgroup:
 // the planes of all NS scalars at x = I+1 are fetched into registers at the start of
 // step I and written to the second of two LDS plane buffers after the step's face
 // fluxes, so one x-step takes two barriers for all scalars (k_stage_scalars: two per

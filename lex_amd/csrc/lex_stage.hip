@@ -460,13 +460,9 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
           case 4: LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars_db<4>, fg, fb, ldb, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
           default: return -4;
         }
-      } else switch (nsc) {
-        case 2: LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars<2>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
-        case 3: LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars<3>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
-        case 4: LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars<4>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
-        case 5: LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars<5>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
-        case 6: LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars<6>, fg, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f); break;
-        default: return -1;
+      } else {
+        dim3 g1(nsc, (d.nx + kFTX - 1) / kFTX, (d.ny + kFTY - 1) / kFTY);   // scalar fastest
+        LEX_LAUNCH(KID_STAGE_SCALARS, k_stage_scalars, g1, fb, lds, st, d, rc, src[1], src[2], src[3], S, tr, dts_f);
       }
     } else {
       long n = (long)L.nc;
