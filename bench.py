@@ -56,6 +56,7 @@ BYTES_PER_CELL = {
     "k_mom_u": 20,           # u, v, w, sgs_u in; rhs_u out
     "k_mom_v": 20,           # u, v, w, sgs_v in; rhs_v out
     "k_mom_w": 28,           # u, v, w, theta, qv, sgs_w in; rhs_w out
+    "k_smag_fused": 56,      # u, v, w, theta, qv, qc, qr in; sgs_u, sgs_v, sgs_w and 4 scalar tendencies out
 }
 
 

@@ -69,6 +69,15 @@ __device__ __forceinline__ void st32(float* __restrict__ base, uint32_t e, float
   *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (size_t)(e * 4u)) = v;
 }
 constexpr long kMaxOffset32 = 1L << 30;   // elements addressable by ld32 / st32
+// An opaque copy of a loop-invariant 32-bit lane offset, taken inside the loop: without
+// it the compiler hoists the zero-extended byte offset out of the loop as a 64-bit
+// register pair and then adds it to each uniform base pointer per access (one 64-bit
+// vector add per load); with it every access is `global_load/store v_off, s[base]`.
+// The empty asm statement emits no instruction.
+__device__ __forceinline__ uint32_t lane_off(uint32_t x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
 
 // Fast reciprocal (v_rcp_f32, 1 ulp).  Used for the WENO weight ratios and the
 // normalisation; results stay within the FP32-vs-FP64 tolerance of the reference.
@@ -285,6 +294,9 @@ __device__ __forceinline__ void store_w(const Dims& d, float* __restrict__ a, in
     for (int b1 = 0; b1 < nys; ++b1)
       for (int c1 = 0; c1 < nzs; ++c1) a[iw(d, xs[a1], ys[b1], zs[c1])] = val;
 }
+
+// Interior index of a cell one period away at most (periodic halo cells and planes).
+__device__ __forceinline__ int wrap_cell(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // a[idx] when ok, else 0, without a branch: the load always issues (at index 0 when
 // !ok), so it is batched with the thread's other loads instead of waiting behind a branch.

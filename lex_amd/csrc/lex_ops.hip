@@ -635,7 +635,6 @@ inline dim3 acoustic2_grid(const Dims& d) {
   return dim3((d.ny + ty - 1) / ty, (d.nx + ax - 1) / ax, 1);
 }
 
-__device__ __forceinline__ int wrap_cell(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 __global__ void __launch_bounds__(1024) k_acoustic2(Dims d, const float* __restrict__ rc,
                                                     const float* __restrict__ pi0c, const float* __restrict__ u,

@@ -36,6 +36,7 @@ enum KernelId {
   KID_BICG_PQ,
   KID_BICG_ST,
   KID_BICG_XR,
+  KID_SMAG_ONE,
   KID_COUNT
 };
 
@@ -45,7 +46,7 @@ inline const char* kernel_name(int id) {
       "k_div_rho",     "k_rhs_uv",      "k_rhs_w",     "k_theta_rho",        "k_acoustic",
       "k_scalar_update", "k_copy",      "k_stage_scalars", "k_mom_u", "k_smag_tau", "k_mom_v", "k_mom_w",
       "k_acoustic_vi", "k_acoustic2", "k_newton_heating", "k_rayleigh",
-      "k_atm_ocn_flux", "k_bicg_init", "k_bicg_pq", "k_bicg_st", "k_bicg_xr"};
+      "k_atm_ocn_flux", "k_bicg_init", "k_bicg_pq", "k_bicg_st", "k_bicg_xr", "k_smag_fused"};
   return (id >= 0 && id < KID_COUNT) ? names[id] : "";
 }
 

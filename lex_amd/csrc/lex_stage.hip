@@ -362,9 +362,26 @@ int rk3_step(const Dims& d, const O& o, const B& b, float* const* state, char* w
       const float cg = o.cs * gs;
       const float* khh;
       const float* khv;
-      if (o.impl != 1) {
-        // the fused paths: deformation, N^2 and the coefficients in one pass writing the
-        // stresses and diffusivities (k_smag_tau), then the stress divergence
+      if (o.impl != 1 && smag_fused_ok(d)) {
+        // the fused paths: the whole evaluation in one pass, stresses on chip
+        SgsScalars SS;
+        SS.n = 2 + o.n_passive;
+        for (int q = 0; q < SS.n; ++q) {
+          SS.s[q] = src[(q == 0) ? 0 : (q == 1 ? 5 : 6 + (q - 2))];
+          SS.out[q] = ss[q];
+        }
+        const bool an = (o.turb == 2);
+        const dim3 fbk(64 * ((d.nz + 63) / 64), smag_fused_rows(d), 1);
+        if (an)
+          LEX_LAUNCH(KID_SMAG_ONE, k_smag_fused<true>, smag_fused_grid(d), fbk, smag_fused_lds(d, true), st, d, rc,
+                     src[1], src[2], src[3], src[0], src[5], cg * cg, o.prandtl, o.cs * o.cs, SS, su, sv, sw);
+        else
+          LEX_LAUNCH(KID_SMAG_ONE, k_smag_fused<false>, smag_fused_grid(d), fbk, smag_fused_lds(d, false), st, d, rc,
+                     src[1], src[2], src[3], src[0], src[5], cg * cg, o.prandtl, o.cs * o.cs, SS, su, sv, sw);
+      } else if (o.impl != 1) {
+        // the fused paths when the one-pass kernel does not apply (nz mod 64 of 1 or 2):
+        // deformation, N^2 and the coefficients in one pass writing the stresses and
+        // diffusivities (k_smag_tau), then the stress divergence
         SmagTau T;
         for (int q = 0; q < 6; ++q) T.t[q] = sm[q];
         T.khh = sm[6];
