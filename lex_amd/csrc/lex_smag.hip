@@ -438,10 +438,10 @@ __global__ void k_smag_scalars(Dims d, const float* __restrict__ rc, const float
 //   - N^2 at the cell's lower and upper w levels (two logarithms instead of three); the
 //     surface and lid cells take the extrapolation levels from the lanes above / below;
 //   - t11 and the diffusivity stay in registers for the x differences; t12 and t13 go to
-//     a three-plane LDS ring (x, y and z neighbours), t22, t23, t33 and the diffusivities
-//     to a two-plane ring.
+//     a four-plane LDS ring (x, y and z neighbours), t22, t23, t33 and the diffusivities
+//     to a three-plane ring.
 // Then (phase 2) its TYT-2 inner rows form the stress divergence on the u, v, w points
-// of plane p-1 and the scalars' SGS diffusion there.  Two barriers per plane.  The lanes
+// of plane p-1 and the scalars' SGS diffusion there.  One barrier per plane.  The lanes
 // above / below the surface and lid cells take values from within their wave, so nz mod
 // 64 is not 1 or 2 (the host checks, and otherwise launches the three-kernel form).
 // ---------------------------------------------------------------------------------
@@ -456,7 +456,7 @@ inline bool smag_fused_ok(const Dims& d) {
 }
 inline size_t smag_fused_lds(const Dims& d, bool aniso) {
   const int nzl = 64 * ((d.nz + 63) / 64);
-  return sizeof(float) * (size_t)smag_fused_rows(d) * nzl * (aniso ? 16 : 14);
+  return sizeof(float) * (size_t)smag_fused_rows(d) * nzl * (aniso ? 23 : 20);
 }
 constexpr int kSmagAx = 16;   // x cells per workgroup
 inline dim3 smag_fused_grid(const Dims& d) {
@@ -475,13 +475,16 @@ __global__ void __launch_bounds__(1024) k_smag_fused(Dims d, const float* __rest
   extern __shared__ float lds[];
   const int nzl = blockDim.x, TYT = blockDim.y;
   const int PL = TYT * nzl;
-  float* L12 = lds;             // [3][TYT][nzl]
-  float* L13 = L12 + 3 * PL;    // [3]
-  float* L22 = L13 + 3 * PL;    // [2]
-  float* L23 = L22 + 2 * PL;    // [2]
-  float* L33 = L23 + 2 * PL;    // [2]
-  float* LKH = L33 + 2 * PL;    // [2] khh
-  float* LKV = LKH + 2 * PL;    // [2] khv (ANISO)
+  // rings deep enough for one barrier per plane: while the slowest thread forms the
+  // divergences of plane i (t12, t13 of planes i-1..i+1; the rest of plane i), the
+  // fastest may already store plane i+2
+  float* L12 = lds;             // [4][TYT][nzl]
+  float* L13 = L12 + 4 * PL;    // [4]
+  float* L22 = L13 + 4 * PL;    // [3]
+  float* L23 = L22 + 3 * PL;    // [3]
+  float* L33 = L23 + 3 * PL;    // [3]
+  float* LKH = L33 + 3 * PL;    // [3] khh
+  float* LKV = LKH + 3 * PL;    // [3] khv (ANISO)
   const int lane = threadIdx.x, row = threadIdx.y;
   const TileId tb = xcd_tile();
   const int j0 = tb.x * (TYT - 2);
@@ -610,14 +613,14 @@ __global__ void __launch_bounds__(1024) k_smag_fused(Dims d, const float* __rest
       khh = kmh * ipr;
       khv = khh;
     }
-    const int s3 = (p + 3) % 3, s2 = (p + 2) % 2;   // p >= -1
-    L12[s3 * PL + me] = -2.0f * kmh * a12;
-    L13[s3 * PL + me] = -2.0f * kmv * a13;
-    L22[s2 * PL + me] = -2.0f * kmh * a22;
-    L23[s2 * PL + me] = -2.0f * kmv * a23;
-    L33[s2 * PL + me] = -2.0f * kmh * a33;
-    LKH[s2 * PL + me] = khh;
-    if (ANISO) LKV[s2 * PL + me] = khv;
+    const int s4 = (p + 4) % 4, s3 = (p + 3) % 3;   // p >= -1
+    L12[s4 * PL + me] = -2.0f * kmh * a12;
+    L13[s4 * PL + me] = -2.0f * kmv * a13;
+    L22[s3 * PL + me] = -2.0f * kmh * a22;
+    L23[s3 * PL + me] = -2.0f * kmv * a23;
+    L33[s3 * PL + me] = -2.0f * kmh * a33;
+    LKH[s3 * PL + me] = khh;
+    if (ANISO) LKV[s3 * PL + me] = khv;
     t11_n = -2.0f * kmh * a11;
     kh_n = khh;
     // the upper face becomes the next plane's lower face
@@ -644,7 +647,7 @@ __global__ void __launch_bounds__(1024) k_smag_fused(Dims d, const float* __rest
     phase1(i + 1, i + 1 == d.nx);
     __syncthreads();
     if (out_row) {
-      const int sm = (i + 2) % 3, sc = (i + 3) % 3, sp = (i + 4) % 3, c2 = (i + 2) % 2;
+      const int sm = (i + 3) % 4, sc = (i + 4) % 4, sp = (i + 5) % 4, c2 = (i + 3) % 3;
       const float* T12m = L12 + sm * PL;
       const float* T12c = L12 + sc * PL;
       const float* T12p = L12 + sp * PL;
@@ -724,7 +727,6 @@ __global__ void __launch_bounds__(1024) k_smag_fused(Dims d, const float* __rest
         }
       }
     }
-    __syncthreads();
     t11_a = t11_b;
     t11_b = t11_n;
     kh_a = kh_b;
