@@ -664,22 +664,51 @@ __global__ void __launch_bounds__(1024) k_acoustic2(Dims d, const float* __restr
   float bru0 = 0.f, bru1 = 0.f, brv0 = 0.f, brv1 = 0.f, brw0 = 0.f, brw1 = 0.f;
   float btc = 0.f, btxm = 0.f, btxp = 0.f, btym = 0.f, btyp = 0.f, btzm = 0.f, btzp = 0.f;
   float bsc = 0.f, bsxm = 0.f, bsxp = 0.f, bsym = 0.f, bsyp = 0.f, bszm = 0.f, bszp = 0.f;
+  // x-neighbour values carried along the march: p and u at the previous position (the
+  // next cell's pxm / pc and u0), valid when the previous position is the x-neighbour in
+  // memory (not across the wrap to a periodic image); theta_rho, pi* and ru from the
+  // substep-B operands (btc, btxp, bsc, bsxp, bru1)
+  float cpc = 0.f, cpxp = 0.f, cu1 = 0.f;
+  const bool zlo_ld = (k & 63) == 0 || kc == 0;               // lanes whose z neighbours are
+  const bool zhi_ld = (k & 63) == 63 || kc >= d.nz - 1;       // not held by an adjacent lane
   for (int s = x0 - 1; s <= x1; ++s) {
     // ---- substep A at cell (s, j, k), taken at its periodic image (sw, jw, kc)
     const int sw = wrap_cell(s, d.nx), I = sw + 3;
+    const bool carry = (s > x0 - 1) && sw == wrap_cell(s - 1, d.nx) + 1;
     const long c = ic(d, I, J, K);
     const long cu = iu(d, I, J, K), cv = iv(d, I, J, K), cw = iw(d, I, J, K);
     const long ou = ((long)sw * d.ny + jw) * d.nz + kc, ov = ((long)sw * (d.ny + 1) + jw) * d.nz + kc;
     const long ow = ((long)sw * d.ny + jw) * (d.nz + 1) + kc;
-    const float u0 = u[cu], u1 = u[cu + d.sxu()], v0 = v[cv], v1 = v[cv + d.syc()], w0 = w[cw], w1 = w[cw + 1];
-    const float ru0 = ru[ou], ru1 = ru[ou + (long)d.ny * d.nz], rv0 = rv[ov], rv1 = rv[ov + d.nz];
+    const float u1 = u[cu + d.sxu()], v0 = v[cv], v1 = v[cv + d.syc()], w0 = w[cw], w1 = w[cw + 1];
+    const float ru1 = ru[ou + (long)d.ny * d.nz], rv0 = rv[ov], rv1 = rv[ov + d.nz];
     const float rw0 = rw[ow], rw1 = rw[ow + 1];
-    const float pc = p[c], pxm = p[c - sx], pxp = p[c + sx], pym = p[c - sy], pyp = p[c + sy], pzm = p[c - 1],
-                pzp = p[c + 1];
-    const float tc = tr[c], txm = tr[c - sx], txp = tr[c + sx], tym = tr[c - sy], typ = tr[c + sy],
-                tzm = tr[c - 1], tzp = tr[c + 1];
-    const float sc = ps[c], sxm = ps[c - sx], sxp = ps[c + sx], sym = ps[c - sy], syp = ps[c + sy],
-                szm = ps[c - 1], szp = ps[c + 1];
+    const float pxp = p[c + sx], pym = p[c - sy], pyp = p[c + sy];
+    const float txp = tr[c + sx], tym = tr[c - sy], typ = tr[c + sy];
+    const float sxp = ps[c + sx], sym = ps[c - sy], syp = ps[c + sy];
+    float u0, ru0, pc, pxm, tc, txm, sc, sxm;
+    if (carry) {
+      u0 = cu1; ru0 = bru1; pc = cpxp; pxm = cpc; tc = btxp; txm = btc; sc = bsxp; sxm = bsc;
+    } else {
+      u0 = u[cu]; ru0 = ru[ou]; pc = p[c]; pxm = p[c - sx]; tc = tr[c]; txm = tr[c - sx]; sc = ps[c]; sxm = ps[c - sx];
+    }
+    // z neighbours from the adjacent lanes (level k-1 is lane k-1); the surface, lid,
+    // idle and wave-edge lanes load theirs
+    float pzm = __shfl_up(pc, 1, 64), pzp = __shfl_down(pc, 1, 64);
+    float tzm = __shfl_up(tc, 1, 64), tzp = __shfl_down(tc, 1, 64);
+    float szm = __shfl_up(sc, 1, 64), szp = __shfl_down(sc, 1, 64);
+    if (zlo_ld) {
+      pzm = p[c - 1];
+      tzm = tr[c - 1];
+      szm = ps[c - 1];
+    }
+    if (zhi_ld) {
+      pzp = p[c + 1];
+      tzp = tr[c + 1];
+      szp = ps[c + 1];
+    }
+    cpc = pc;
+    cpxp = pxp;
+    cu1 = u1;
     const AcousticOut a = acoustic_cell(d, kc, u0, u1, v0, v1, w0, w1, ru0, ru1, rv0, rv1, rw0, rw1, pc, pxm, pxp,
                                         pym, pyp, pzm, pzp, tc, txm, txp, tym, typ, tzm, tzp, sc, sxm, sxp, sym, syp,
                                         szm, szp, rcm, rcc, rcp, pim, pic, pip, dts);
