@@ -1026,6 +1026,12 @@ namespace {
 template <int CI, int CO>
 void launch_fwd(int nx, int ny, int nz, const void* x, const void* wp, const float* bias, int mode, int cout_real,
                 void* y, float* yf, float* pre, hipStream_t st) {
+  if (CO % 32 == 0) {   // 32 x 32 MFMA tiles: one A fragment per 32 output channels
+    dim3 grid((nz + CT2Z - 1) / CT2Z, (ny + CT2Y - 1) / CT2Y, (nx + CT2X - 1) / CT2X);
+    hipLaunchKernelGGL((k_conv3_fwd32<CI, (CO % 32 == 0 ? CO : 32)>), grid, dim3(256), 0, st, nx, ny, nz,
+                       (const __bf16*)x, (const __bf16*)wp, bias, mode, cout_real, (__bf16*)y, yf, pre);
+    return;
+  }
   dim3 grid((nz + CTZ - 1) / CTZ, (ny + CTY - 1) / CTY, (nx + CTX - 1) / CTX);
   hipLaunchKernelGGL((k_conv3_fwd<CI, CO>), grid, dim3(256), 0, st, nx, ny, nz, (const __bf16*)x, (const __bf16*)wp,
                      bias, mode, cout_real, (__bf16*)y, yf, pre);

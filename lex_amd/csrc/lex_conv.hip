@@ -26,14 +26,22 @@ __device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
+// tanh for the GELU of BF16 activations: 1 - 2 / (1 + e^(2y)) from the hardware
+// exponential and reciprocal (a few instructions instead of the library's tens).  It
+// saturates to +-1 with e^(2y) at inf / 0; near 0 its absolute error is ~1e-7, which the
+// GELU's factor x makes a relative error of the same size, far below BF16's 4e-3.
+__device__ __forceinline__ float tanh_fast(float y) {
+  const float e = __expf(2.0f * y);
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + e);
+}
 // tanh-approximate GELU and its derivative
 __device__ __forceinline__ float gelu_t(float x) {
   const float k0 = 0.7978845608028654f, k1 = 0.044715f;
-  return 0.5f * x * (1.0f + tanhf(k0 * (x + k1 * x * x * x)));
+  return 0.5f * x * (1.0f + tanh_fast(k0 * (x + k1 * x * x * x)));
 }
 __device__ __forceinline__ float gelu_t_grad(float x) {
   const float k0 = 0.7978845608028654f, k1 = 0.044715f;
-  float t = tanhf(k0 * (x + k1 * x * x * x));
+  float t = tanh_fast(k0 * (x + k1 * x * x * x));
   return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * k0 * (1.0f + 3.0f * k1 * x * x);
 }
 
@@ -128,6 +136,105 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(int nx, int ny, int nz, const
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int zc = z0 + 4 * g + i;
+        if (xc < nx && yc < ny && zc < nz) {
+          const long cell = ((long)xc * ny + yc) * nz + zc;
+          float v = acc[mt][nt][i] + bn;
+          if (pre) pre[cell * COUT + n] = v;
+          if (mode & 1) v = gelu_t(v);
+          if (mode & 2) {
+            if (n < cout_real) Yf[cell * cout_real + n] = v;
+          } else {
+            Y[cell * COUT + n] = (__bf16)v;
+          }
+        }
+      }
+    }
+  }
+}
+
+// The forward convolution for COUT a multiple of 32 on v_mfma_f32_32x32x16_bf16: one
+// MFMA covers 32 cells x 32 output channels, so each A fragment read from LDS (32 cells x
+// 16 k) feeds 32 (or 64) output channels, half the LDS bytes per flop of the 16x16x32
+// form, whose A fragments feed 16.  Output tile 4 x 4 x 16 (x, y, z) per 4-wave
+// workgroup, as k_conv3_fwd (the same 41 KB halo tile for 32 channels): wave w owns
+// x = x0 + w and two accumulator blocks, each 2 y rows x 16 z cells as the 32 MFMA rows
+// (row m: y offset m / 16, z offset m % 16).  Operand layout: lane l supplies A[row l%32][k 8(l/32)..+7]
+// and B[k 8(l/32)..+7][col l%32] and receives D[row 8(i/4) + 4(l/32) + i%4][col l%32]
+// in accumulator i = 0..15.  Same arguments and results as k_conv3_fwd.
+constexpr int CT2X = 4, CT2Y = 4, CT2Z = 16;
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_conv3_fwd32(int nx, int ny, int nz, const __bf16* __restrict__ X,
+                                                     const __bf16* __restrict__ Wp, const float* __restrict__ bias,
+                                                     int mode, int cout_real, __bf16* __restrict__ Y,
+                                                     float* __restrict__ Yf, float* __restrict__ pre) {
+  constexpr int NSTEP = ConvSteps<CIN>::NSTEP, NT = COUT / 32, NK = 2 * NSTEP;   // 16-wide K steps
+  constexpr int GX = CT2X + 2, GY = CT2Y + 2, GZ = CT2Z + 2;
+  __shared__ __attribute__((aligned(16))) __bf16 tile[GX * GY * GZ * CIN];
+  const int z0 = blockIdx.x * CT2Z, y0 = blockIdx.y * CT2Y, x0 = blockIdx.z * CT2X;
+  constexpr int nchunk = GX * GY * GZ * CIN / 8;
+  const bool circ = (mode & 4) != 0;  // periodic x and y (flax padding 'CIRCULAR' there), zero z
+  for (int c = threadIdx.x; c < nchunk; c += 256) {
+    int cell = c / (CIN / 8), g = c % (CIN / 8);
+    int hz = cell % GZ, hy = (cell / GZ) % GY, hx = cell / (GZ * GY);
+    int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+    if (circ) {
+      gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+      gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+    }
+    bf16x8 v;
+    if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
+      v = *(const bf16x8*)(X + (((long)gx * ny + gy) * nz + gz) * CIN + 8 * g);
+    } else {
+      for (int j = 0; j < 8; ++j) v[j] = (__bf16)0.0f;
+    }
+    *(bf16x8*)(tile + cell * CIN + 8 * g) = v;
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, m = l & 31, h = l >> 5;
+  f32x16 acc[CT2Y / 2][NT];
+#pragma unroll
+  for (int a = 0; a < CT2Y / 2; ++a)
+#pragma unroll
+    for (int b = 0; b < NT; ++b)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.f;
+  // fully unrolled: the tap of a K step does not depend on the lane (a step of 16 k is
+  // one tap's channels 16 s' .. 16 s' + 15), so every tap offset is a constant and only
+  // the channel offset 8 (l / 32) varies by lane
+  const __bf16* tw = tile + ((w + 1) * GY + 1) * GZ * CIN + (((m >> 4) * GZ + (m & 15) + 1) * CIN) + 8 * h;
+  const __bf16* wl = Wp + (long)m * (NSTEP * 32) + 8 * h;
+#pragma unroll
+  for (int s = 0; s < NK; ++s) {
+    const int kk0 = s * 16;          // the step's first k (lane half 0)
+    int tap, ch0;
+    conv_step_k(CIN, kk0 / 32, kk0 % 32, tap, ch0);
+    if (tap > 26) tap = 26;          // padded K: weights are zero there
+    const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+    bf16x8 b[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) b[nt] = *(const bf16x8*)(wl + (long)nt * 32 * (NSTEP * 32) + kk0);
+#pragma unroll
+    for (int mt = 0; mt < CT2Y / 2; ++mt) {
+      // cell ((w + 1 + dx), (2 mt + m/16 + 1 + dy), (m%16 + 1 + dz)), channel ch0 + 8 h
+      const int off = ((dx * GY + 2 * mt + dy) * GZ + dz) * CIN + ch0;
+      const bf16x8 a = *(const bf16x8*)(tw + off);
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[nt], acc[mt][nt], 0, 0, 0);
+    }
+  }
+  const int xc = x0 + w;
+#pragma unroll
+  for (int mt = 0; mt < CT2Y / 2; ++mt) {
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int n = nt * 32 + m;
+      const float bn = bias[n];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int row = 8 * (i / 4) + 4 * h + (i % 4);   // MFMA row of accumulator i
+        const int yc = y0 + 2 * mt + (row >> 4), zc = z0 + (row & 15);
         if (xc < nx && yc < ny && zc < nz) {
           const long cell = ((long)xc * ny + yc) * nz + zc;
           float v = acc[mt][nt][i] + bn;
