@@ -750,8 +750,14 @@ def main():
     bpc = BYTES_PER_CELL.get(dom, 0)
     traffic_alg = bpc * ncell
     achieved = traffic_alg / (avg_ms / 1e3) / 1e9
-    kernels = {k: {"avg_ms": round(v[0] / v[1], 5), "launches_per_step": v[1] / psteps,
-                   "share": round(v[0] / pms, 4)} for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
+    kernels = {}
+    for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0]):
+        e = {"avg_ms": round(v[0] / v[1], 5), "launches_per_step": v[1] / psteps, "share": round(v[0] / pms, 4)}
+        if k in BYTES_PER_CELL:   # algorithmic bytes of every kernel, not only the dominant one's
+            gbs = BYTES_PER_CELL[k] * ncell / (v[0] / v[1] / 1e3) / 1e9
+            e.update({"bytes_per_cell": BYTES_PER_CELL[k], "achieved_GBs": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4)})
+        kernels[k] = e
     if args.profile:
         print(json.dumps(kernels, indent=1), file=sys.stderr)
 
