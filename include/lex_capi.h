@@ -272,7 +272,12 @@ const char* lex_profile_kernel_name(int id);
 int lex_fold(const lex_grid_t* g, int mode, float* adj, void* stream);
 
 /* advection_scalar (advection.py:9-34) with flow_divergence = get_divergence(u, v, w)
- * (:200-226) and a prescribed surface flux: lam (nx,ny,nz) -> ls, lu, lv, lw (padded). */
+ * (:200-226) and a prescribed surface flux: lam (nx,ny,nz) -> ls, lu, lv, lw (padded).
+ * The advection reverses use no atomics: contributions several threads share go through
+ * slot planes (18 for the scalar, 24 per momentum component, each the size of the launch
+ * box) that a gather adds in a fixed order, so their results are bit-reproducible.  The
+ * planes are taken from the stream's pool (hipMallocAsync) and returned on the stream;
+ * -4 when the pool has no room. */
 int lex_adv_scalar_vjp(const lex_grid_t* g, const float* rho0, const float* s, const float* u, const float* v,
                        const float* w, const float* lam, float weps, float* ls, float* lu, float* lv, float* lw,
                        void* stream);

@@ -18,6 +18,45 @@ namespace lex {
 
 __device__ __forceinline__ void aadd(float* p, float v) { unsafeAtomicAdd(p, v); }
 
+// Slot planes: the deterministic form of a scatter.  A reverse kernel whose threads share
+// targets writes each shared contribution into the plane of its slot (slot = which of
+// the thread's targets it is), at the thread's own index in the launch box, and writes
+// every slot of every thread in the box (zero where it has nothing), so no plane holds
+// data of an earlier launch.  A gather kernel then adds, for every target point, the
+// slots that reach it in one fixed order: no atomics, bit-reproducible results.
+struct SlotPlanes {
+  float* p;          // plane s starts at p + s * plane; null: this kernel contributes nothing
+  long plane;        // elements per plane
+  int gx, gy, gz;    // launch box: a in [0, gx), b in [0, gy), k in [0, gz); k fastest
+  __device__ __forceinline__ long at(int a, int b, int k) const { return ((long)a * gy + b) * gz + k; }
+  __device__ __forceinline__ void put(int s, long i, float v) const { p[s * plane + i] = v; }
+  // slot s of box point (a, b, k), zero outside the box
+  __device__ __forceinline__ float get(int s, int a, int b, int k) const {
+    const bool ok = (unsigned)a < (unsigned)gx && (unsigned)b < (unsigned)gy && (unsigned)k < (unsigned)gz;
+    return (ok && p) ? p[s * plane + at(a, b, k)] : 0.f;
+  }
+};
+
+inline SlotPlanes slot_planes(float* p, int gx, int gy, int gz) {
+  SlotPlanes sp;
+  sp.p = p;
+  sp.gx = gx; sp.gy = gy; sp.gz = gz;
+  sp.plane = (((long)gx * gy * gz) + 63) / 64 * 64;
+  return sp;
+}
+
+// launch boxes and slot counts of the advection reverses
+constexpr int kSlotsScalar = 18, kSlotsMom = 24;
+inline SlotPlanes slots_scalar(const Dims& d, float* p) { return slot_planes(p, d.nx + 1, d.ny + 1, d.nz); }
+inline SlotPlanes slots_u(const Dims& d, float* p) { return slot_planes(p, d.nx + 2, d.ny + 1, d.nz); }
+inline SlotPlanes slots_v(const Dims& d, float* p) { return slot_planes(p, d.nx + 1, d.ny + 2, d.nz); }
+inline SlotPlanes slots_w(const Dims& d, float* p) { return slot_planes(p, d.nx + 1, d.ny + 1, d.nz + 1); }
+// bytes of the scalar reverse's planes, and of the three momentum reverses' planes together
+inline size_t slot_bytes_scalar(const Dims& d) { return (size_t)kSlotsScalar * slots_scalar(d, nullptr).plane * 4; }
+inline size_t slot_bytes_mom(const Dims& d) {
+  return (size_t)kSlotsMom * (slots_u(d, nullptr).plane + slots_v(d, nullptr).plane + slots_w(d, nullptr).plane) * 4;
+}
+
 // ---------------------------------------------------------------- WENO gradients
 // R = weno5(a..e) and g[0..4] = dR/d(a, b, c, d, e).  With r_k = tau/(beta_k + eps),
 // w_k = d_k (1 + r_k^2), R = sum w_k q_k / W:
@@ -160,21 +199,6 @@ __device__ float vrecon_grad(const float* __restrict__ p, int L, int f, bool pos
   return vrecon_grad6(q, L, f, pos, weps, g6, ref);
 }
 
-__device__ __forceinline__ void scatter6(float* __restrict__ lq, long base, long stride, const float* g6, float coef) {
-#pragma unroll
-  for (int t = 0; t < 6; ++t)
-    if (g6[t] != 0.f) aadd(lq + base + t * stride, coef * g6[t]);
-}
-
-// vertical variant: unit-stride column, stencil start f-2, column length L
-__device__ __forceinline__ void scatter6z(float* __restrict__ lcol, int f, int L, const float* g6, float coef) {
-#pragma unroll
-  for (int t = 0; t < 6; ++t) {
-    int idx = f - 2 + t;
-    if (g6[t] != 0.f && idx >= 0 && idx < L) aadd(lcol + idx, coef * g6[t]);
-  }
-}
-
 __device__ __forceinline__ float r8w_col(const Dims& d, const float* rc, int Kw) {
   return (Kw <= 0 || Kw >= d.nz + 2) ? 0.0f : 0.5f * (rc[Kw - 1] + rc[Kw]);
 }
@@ -188,18 +212,21 @@ __device__ __forceinline__ float fd_at(const Dims& d, const float* __restrict__ 
          (r8w_col(d, rc, K + 1) * w[iw(d, I, J, K + 1)] - r8w_col(d, rc, K) * w[iw(d, I, J, K)]) * d.idz;
 }
 
-// adjoint of fd_at with weight g
-__device__ __forceinline__ void fd_vjp(const Dims& d, const float* __restrict__ rc, int I, int J, int K, float g,
-                                       float* lu, float* lv, float* lw) {
-  if (g == 0.f) return;
-  float r = rc[K];
-  aadd(lu + iu(d, I + 1, J, K), g * r * d.idx);
-  aadd(lu + iu(d, I, J, K), -g * r * d.idx);
-  aadd(lv + iv(d, I, J + 1, K), g * r * d.idy);
-  aadd(lv + iv(d, I, J, K), -g * r * d.idy);
-  float hi = r8w_col(d, rc, K + 1), lo = r8w_col(d, rc, K);
-  if (hi != 0.f) aadd(lw + iw(d, I, J, K + 1), g * hi * d.idz);
-  if (lo != 0.f) aadd(lw + iw(d, I, J, K), -g * lo * d.idz);
+// adjoint of fd_at with weight g, as the six face-velocity weights (upper and lower face
+// of each direction)
+struct FdGrad {
+  float u_hi, u_lo, v_hi, v_lo, w_hi, w_lo;
+};
+__device__ __forceinline__ FdGrad fd_grad(const Dims& d, const float* __restrict__ rc, int K, float g) {
+  const float r = rc[K];
+  FdGrad f;
+  f.u_hi = g * r * d.idx;
+  f.u_lo = -g * r * d.idx;
+  f.v_hi = g * r * d.idy;
+  f.v_lo = -g * r * d.idy;
+  f.w_hi = g * r8w_col(d, rc, K + 1) * d.idz;
+  f.w_lo = -g * r8w_col(d, rc, K) * d.idz;
+  return f;
 }
 
 // ------------------------------------------------------------- A2 scalar advection
@@ -221,7 +248,7 @@ __device__ __forceinline__ float mu_cell(const Dims& d, const float* __restrict_
 
 __global__ void k_adv_scalar_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ s,
                                  const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
-                                 const float* __restrict__ lam, float weps, float* __restrict__ ls,
+                                 const float* __restrict__ lam, float weps, SlotPlanes sp,
                                  float* __restrict__ lu, float* __restrict__ lv, float* __restrict__ lw) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -257,35 +284,70 @@ __global__ void k_adv_scalar_vjp(Dims d, const float* __restrict__ rc, const flo
   const float Rcx = hrecon_grad(qx, upwind_pos(massx), weps, gx6, sc);
   const float Rcy = hrecon_grad(qy, upwind_pos(massy), weps, gy6, sc);
   const float Rcz = vrecon_grad6(qz, d.nz + 2, k, upwind_pos(massz), weps, gz6, sc);
+  // slots: 0..5 the x stencil (cells I-3..I+2 of row J), 6..11 the y stencil, 12..17 the
+  // column levels k-2..k+3; the cell's own div(rho0 u) term goes with slot 3 (cell I)
+  float ox[6], oy[6], oz[6];
+#pragma unroll
+  for (int t = 0; t < 6; ++t) ox[t] = oy[t] = oz[t] = 0.f;
   if (fx) {  // x face I: lower face of cell I, upper face of cell I-1
     const float gF = (mc - mlx) * d.idx;
     // d(flux)/du collects mc (R - s_I) - ml (R - s_{I-1}) = gF (R - s_I) - ml (s_I - s_{I-1}) / dx:
     // every factor is a small difference, not a difference of theta-sized products
     const float g = -mlx * d.idx * (sc - qx[2]) + gF * Rcx;
-    scatter6(ls, c - 3 * sx, sx, gx6, gF * massx);
+    const float cf = gF * massx;
+#pragma unroll
+    for (int t = 0; t < 6; ++t) ox[t] = cf * gx6[t];
     if (g != 0.f) lu[au] += r * g;
   }
   if (fy) {  // y face J
     const float gF = (mc - mly) * d.idy;
     const float g = -mly * d.idy * (sc - qy[2]) + gF * Rcy;
-    scatter6(ls, c - 3 * sy, sy, gy6, gF * massy);
+    const float cf = gF * massy;
+#pragma unroll
+    for (int t = 0; t < 6; ++t) oy[t] = cf * gy6[t];
     if (g != 0.f) lv[av] += r * g;
   }
-  if (!cell) return;
-  float g;
-  if (k > 0) {  // flux face between levels k and k+1 (the surface face k = 0 is prescribed)
-    const float gF = (mc - mb) * d.idz;
-    g = -mb * d.idz * (sc - qz[2]) + gF * Rcz;
-    scatter6z(ls + (c - K), k, d.nz + 2, gz6, gF * massz);
-  } else {
-    g = -mc * sc * d.idz;
+  if (cell) {
+    float g;
+    if (k > 0) {  // flux face between levels k and k+1 (the surface face k = 0 is prescribed)
+      const float gF = (mc - mb) * d.idz;
+      g = -mb * d.idz * (sc - qz[2]) + gF * Rcz;
+      const float cf = gF * massz;
+#pragma unroll
+      for (int t = 0; t < 6; ++t) oz[t] = cf * gz6[t];
+    } else {
+      g = -mc * sc * d.idz;
+    }
+    if (g != 0.f) lw[wb + K] += r8 * g;
+    if (k == d.nz - 1) {  // lid level: div(rho0 u) of the top cell only
+      const float gt = mc * sc * (0.5f * (r + rcp)) * d.idz;
+      if (gt != 0.f) lw[wb + K + 1] += gt;
+    }
+    ox[3] += mc * fd;
   }
-  if (g != 0.f) lw[wb + K] += r8 * g;
-  if (k == d.nz - 1) {  // lid level: div(rho0 u) of the top cell only
-    const float gt = mc * sc * (0.5f * (r + rcp)) * d.idz;
-    if (gt != 0.f) lw[wb + K + 1] += gt;
+  const long p = sp.at(i, j, k);
+#pragma unroll
+  for (int t = 0; t < 6; ++t) {
+    sp.put(t, p, ox[t]);
+    sp.put(6 + t, p, oy[t]);
+    sp.put(12 + t, p, oz[t]);
   }
-  aadd(ls + c, mc * fd);
+}
+
+// ls += the slots of k_adv_scalar_vjp that reach padded cell (I, J, K), in slot order
+__global__ void k_gather_scalar(Dims d, SlotPlanes sp, float* __restrict__ ls) {
+  const int K = blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = blockIdx.y * blockDim.y + threadIdx.y;
+  const int I = blockIdx.z;
+  if (K >= d.nz + 2 || J >= d.ny + 6) return;
+  float a = 0.f;
+#pragma unroll
+  for (int t = 0; t < 6; ++t) a += sp.get(t, I - t, J - 3, K - 1);
+#pragma unroll
+  for (int t = 0; t < 6; ++t) a += sp.get(6 + t, I - 3, J - t, K - 1);
+#pragma unroll
+  for (int t = 0; t < 6; ++t) a += sp.get(12 + t, I - 3, J - 3, K + 2 - t);
+  ls[ic(d, I, J, K)] += a;
 }
 
 // ------------------------------------------------------------ A3 momentum advection
@@ -314,7 +376,7 @@ __device__ __forceinline__ float mu_wlev(const Dims& d, const float* __restrict_
 // point's own term (a <= nx, b < ny), div(rho0 u) of cell (a+2, b+3, k+1) (b < ny).
 __global__ void k_adv_u_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
                             const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ lam,
-                            float* __restrict__ lu, float* __restrict__ lv, float* __restrict__ lw) {
+                            SlotPlanes sp) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   int b = blockIdx.y * blockDim.y + threadIdx.y;
   int a = blockIdx.z;
@@ -328,6 +390,12 @@ __global__ void k_adv_u_vjp(Dims d, const float* __restrict__ rc, const float* _
   const float mly = mu_uface(d, lam, rc, a, b - 1, k, b > 0 && a <= d.nx);
   const float mlz = mu_uface(d, lam, rc, a, b, k - 1, upt && k > 0);
   const long sx = d.sxu(), sy = d.syc();
+  // slots (padded targets): 0..5 lu (a+t, J, K), 6..11 lu (a+3, b+t, K), 12..17 lu
+  // (a+3, J, k-2+t); 18 lv (a+2, J+1, K), 19 lv (a+2, J, K), 20 lv (a+3, J, K);
+  // 21 lw (a+2, J, K+1), 22 lw (a+2, J, K), 23 lw (a+3, J, K)
+  float o[kSlotsMom];
+#pragma unroll
+  for (int q = 0; q < kSlotsMom; ++q) o[q] = 0.f;
   float q6[6], g6[6];
   if (b < d.ny) {
     // x flux at scalar cell a+3: lower flux of u face a, upper flux of u face a-1
@@ -335,55 +403,69 @@ __global__ void k_adv_u_vjp(Dims d, const float* __restrict__ rc, const float* _
     const float gF = (mc - ml) * d.idx;
     const long base = iu(d, a, J, K);
     if (gF != 0.f) {
+#pragma unroll
       for (int t = 0; t < 6; ++t) q6[t] = u[base + t * sx];
       const float mass = 0.5f * (q6[2] + q6[3]) * r;
-      float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
-      scatter6(lu, base, sx, g6, gF * mass);
+      const float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+      const float cf = gF * mass;
+#pragma unroll
+      for (int t = 0; t < 6; ++t) o[t] = cf * g6[t];
       const float gm = gF * R * 0.5f * r;
-      aadd(lu + base + 2 * sx, gm);
-      aadd(lu + base + 3 * sx, gm);
+      o[2] += gm;
+      o[3] += gm;
     }
     // div(rho0 u) of cell a+2, averaged by u faces a-1 (its lower) and a (its upper)
     const float gl = ml * u[base + 2 * sx], gr = mc * u[base + 3 * sx];
-    fd_vjp(d, rc, a + 2, J, K, 0.5f * (gl + gr), lu, lv, lw);
+    const FdGrad f = fd_grad(d, rc, K, 0.5f * (gl + gr));
+    o[3] += f.u_hi;
+    o[2] += f.u_lo;
+    o[18] = f.v_hi;
+    o[19] = f.v_lo;
+    o[21] = f.w_hi;
+    o[22] = f.w_lo;
   }
-  if (a > d.nx) return;
-  const int I = a + 3;
-  {  // y flux at corner (I, J): lower flux of u (a, b), upper flux of u (a, b-1)
-    const float ml = mly;
-    const float gF = (mc - ml) * d.idy;
-    if (gF != 0.f) {
-      const long va = iv(d, I - 1, J, K), vb = iv(d, I, J, K);
-      const float mass = 0.5f * (r * v[va] + r * v[vb]);
-      const long base = iu(d, I, b, K);
-      for (int t = 0; t < 6; ++t) q6[t] = u[base + t * sy];
-      float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
-      scatter6(lu, base, sy, g6, gF * mass);
-      const float gm = gF * R * 0.5f * r;
-      aadd(lv + va, gm);
-      aadd(lv + vb, gm);
-    }
-  }
-  if (!upt) return;
-  const long ub = iu(d, I, J, 0);
-  const float* col = u + ub;
-  float* lcol = lu + ub;
-  if (k > 0) {  // z flux at w level K between u levels K-1 and K
-    const float gF = (mc - mlz) * d.idz;
-    if (gF != 0.f) {
-      const float rf = 0.5f * (rc[k] + rc[k + 1]);
-      const long wa = iw(d, I - 1, J, K), wb = iw(d, I, J, K);
-      const float mass = 0.5f * (rf * w[wa] + rf * w[wb]);
-      float R = vrecon_grad(col, d.nz + 2, k, upwind_pos(mass), weps, g6);
-      scatter6z(lcol, k, d.nz + 2, g6, gF * mass);
-      if (R != 0.f) {
-        const float gm = gF * R * 0.5f * rf;
-        aadd(lw + wa, gm);
-        aadd(lw + wb, gm);
+  if (a <= d.nx) {
+    const int I = a + 3;
+    {  // y flux at corner (I, J): lower flux of u (a, b), upper flux of u (a, b-1)
+      const float gF = (mc - mly) * d.idy;
+      if (gF != 0.f) {
+        const long va = iv(d, I - 1, J, K), vb = iv(d, I, J, K);
+        const float mass = 0.5f * (r * v[va] + r * v[vb]);
+        const long base = iu(d, I, b, K);
+#pragma unroll
+        for (int t = 0; t < 6; ++t) q6[t] = u[base + t * sy];
+        const float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+        const float cf = gF * mass;
+#pragma unroll
+        for (int t = 0; t < 6; ++t) o[6 + t] = cf * g6[t];
+        const float gm = gF * R * 0.5f * r;
+        o[19] += gm;
+        o[20] = gm;
       }
     }
+    if (upt) {
+      const float* col = u + iu(d, I, J, 0);
+      if (k > 0) {  // z flux at w level K between u levels K-1 and K
+        const float gF = (mc - mlz) * d.idz;
+        if (gF != 0.f) {
+          const float rf = 0.5f * (rc[k] + rc[k + 1]);
+          const long wa = iw(d, I - 1, J, K), wb = iw(d, I, J, K);
+          const float mass = 0.5f * (rf * w[wa] + rf * w[wb]);
+          const float R = vrecon_grad(col, d.nz + 2, k, upwind_pos(mass), weps, g6);
+          const float cf = gF * mass;
+#pragma unroll
+          for (int t = 0; t < 6; ++t) o[12 + t] = cf * g6[t];
+          const float gm = gF * R * 0.5f * rf;
+          o[22] += gm;
+          o[23] = gm;
+        }
+      }
+      o[15] += mc * 0.5f * (fd_at(d, rc, u, v, w, I - 1, J, K) + fd_at(d, rc, u, v, w, I, J, K));
+    }
   }
-  aadd(lcol + K, mc * 0.5f * (fd_at(d, rc, u, v, w, I - 1, J, K) + fd_at(d, rc, u, v, w, I, J, K)));
+  const long p = sp.at(a, b, k);
+#pragma unroll
+  for (int q = 0; q < kSlotsMom; ++q) sp.put(q, p, o[q]);
 }
 
 // reverse of advection_v (advection.py:77-114), lam: (nx, ny+1, nz).  Thread (a, b, k),
@@ -392,7 +474,7 @@ __global__ void k_adv_u_vjp(Dims d, const float* __restrict__ rc, const float* _
 // div(rho0 u) of cell (a+3, b+2, k+1) (a < nx).
 __global__ void k_adv_v_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
                             const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ lam,
-                            float* __restrict__ lu, float* __restrict__ lv, float* __restrict__ lw) {
+                            SlotPlanes sp) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   int b = blockIdx.y * blockDim.y + threadIdx.y;
   int a = blockIdx.z;
@@ -406,63 +488,83 @@ __global__ void k_adv_v_vjp(Dims d, const float* __restrict__ rc, const float* _
   const float mly = mu_vface(d, lam, rc, a, b - 1, k, b > 0 && a < d.nx);
   const float mlz = mu_vface(d, lam, rc, a, b, k - 1, vpt && k > 0);
   const long sx = d.sxv(), sy = d.syc();
+  // slots (padded targets): 0..5 lv (a+t, b+3, K), 6..11 lv (I, b+t, K), 12..17 lv
+  // (I, b+3, k-2+t); 18 lu (I+1, b+2, K), 19 lu (I, b+2, K), 20 lu (I, b+3, K);
+  // 21 lw (I, b+2, K+1), 22 lw (I, b+2, K), 23 lw (I, b+3, K)
+  float o[kSlotsMom];
+#pragma unroll
+  for (int q = 0; q < kSlotsMom; ++q) o[q] = 0.f;
   float q6[6], g6[6];
   if (b <= d.ny) {  // x flux at corner (I, J): lower flux of v (a, b), upper flux of v (a-1, b)
     const int J = b + 3;
-    const float ml = mlx;
-    const float gF = (mc - ml) * d.idx;
+    const float gF = (mc - mlx) * d.idx;
     if (gF != 0.f) {
       const long ua = iu(d, I, J - 1, K), ub = iu(d, I, J, K);
       const float mass = 0.5f * (r * u[ua] + r * u[ub]);
       const long base = iv(d, a, J, K);
+#pragma unroll
       for (int t = 0; t < 6; ++t) q6[t] = v[base + t * sx];
-      float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
-      scatter6(lv, base, sx, g6, gF * mass);
+      const float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+      const float cf = gF * mass;
+#pragma unroll
+      for (int t = 0; t < 6; ++t) o[t] = cf * g6[t];
       const float gm = gF * R * 0.5f * r;
-      aadd(lu + ua, gm);
-      aadd(lu + ub, gm);
+      o[19] = gm;
+      o[20] = gm;
     }
   }
-  if (a >= d.nx) return;
-  {
-    // y flux at scalar cell b+3: lower flux of v face b, upper flux of v face b-1
-    const float ml = mly;
-    const float gF = (mc - ml) * d.idy;
-    const long base = iv(d, I, b, K);
-    if (gF != 0.f) {
-      for (int t = 0; t < 6; ++t) q6[t] = v[base + t * sy];
-      const float mass = 0.5f * (q6[2] + q6[3]) * r;
-      float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
-      scatter6(lv, base, sy, g6, gF * mass);
-      const float gm = gF * R * 0.5f * r;
-      aadd(lv + base + 2 * sy, gm);
-      aadd(lv + base + 3 * sy, gm);
-    }
-    // div(rho0 u) of cell (I, b+2), averaged by v faces b-1 and b
-    const float gl = ml * v[base + 2 * sy], gr = mc * v[base + 3 * sy];
-    fd_vjp(d, rc, I, b + 2, K, 0.5f * (gl + gr), lu, lv, lw);
-  }
-  if (!vpt) return;
-  const int J = b + 3;
-  const long vb = iv(d, I, J, 0);
-  const float* col = v + vb;
-  float* lcol = lv + vb;
-  if (k > 0) {
-    const float gF = (mc - mlz) * d.idz;
-    if (gF != 0.f) {
-      const float rf = 0.5f * (rc[k] + rc[k + 1]);
-      const long wa = iw(d, I, J - 1, K), wb = iw(d, I, J, K);
-      const float mass = 0.5f * (rf * w[wa] + rf * w[wb]);
-      float R = vrecon_grad(col, d.nz + 2, k, upwind_pos(mass), weps, g6);
-      scatter6z(lcol, k, d.nz + 2, g6, gF * mass);
-      if (R != 0.f) {
-        const float gm = gF * R * 0.5f * rf;
-        aadd(lw + wa, gm);
-        aadd(lw + wb, gm);
+  if (a < d.nx) {
+    {
+      // y flux at scalar cell b+3: lower flux of v face b, upper flux of v face b-1
+      const float ml = mly;
+      const float gF = (mc - ml) * d.idy;
+      const long base = iv(d, I, b, K);
+      if (gF != 0.f) {
+#pragma unroll
+        for (int t = 0; t < 6; ++t) q6[t] = v[base + t * sy];
+        const float mass = 0.5f * (q6[2] + q6[3]) * r;
+        const float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+        const float cf = gF * mass;
+#pragma unroll
+        for (int t = 0; t < 6; ++t) o[6 + t] = cf * g6[t];
+        const float gm = gF * R * 0.5f * r;
+        o[8] += gm;
+        o[9] += gm;
       }
+      // div(rho0 u) of cell (I, b+2), averaged by v faces b-1 and b
+      const float gl = ml * v[base + 2 * sy], gr = mc * v[base + 3 * sy];
+      const FdGrad f = fd_grad(d, rc, K, 0.5f * (gl + gr));
+      o[18] = f.u_hi;
+      o[19] += f.u_lo;
+      o[9] += f.v_hi;
+      o[8] += f.v_lo;
+      o[21] = f.w_hi;
+      o[22] = f.w_lo;
+    }
+    if (vpt) {
+      const int J = b + 3;
+      const float* col = v + iv(d, I, J, 0);
+      if (k > 0) {
+        const float gF = (mc - mlz) * d.idz;
+        if (gF != 0.f) {
+          const float rf = 0.5f * (rc[k] + rc[k + 1]);
+          const long wa = iw(d, I, J - 1, K), wb = iw(d, I, J, K);
+          const float mass = 0.5f * (rf * w[wa] + rf * w[wb]);
+          const float R = vrecon_grad(col, d.nz + 2, k, upwind_pos(mass), weps, g6);
+          const float cf = gF * mass;
+#pragma unroll
+          for (int t = 0; t < 6; ++t) o[12 + t] = cf * g6[t];
+          const float gm = gF * R * 0.5f * rf;
+          o[22] += gm;
+          o[23] = gm;
+        }
+      }
+      o[15] += mc * 0.5f * (fd_at(d, rc, u, v, w, I, J - 1, K) + fd_at(d, rc, u, v, w, I, J, K));
     }
   }
-  aadd(lcol + K, mc * 0.5f * (fd_at(d, rc, u, v, w, I, J - 1, K) + fd_at(d, rc, u, v, w, I, J, K)));
+  const long p = sp.at(a, b, k);
+#pragma unroll
+  for (int q = 0; q < kSlotsMom; ++q) sp.put(q, p, o[q]);
 }
 
 // reverse of advection_w (advection.py:117-149), lam: (nx, ny, nz+1); levels 0 and nz
@@ -471,7 +573,7 @@ __global__ void k_adv_v_vjp(Dims d, const float* __restrict__ rc, const float* _
 // own term of w level t and div(rho0 u) of cell level t+1 (a < nx, b < ny).
 __global__ void k_adv_w_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
                             const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ lam,
-                            float* __restrict__ lu, float* __restrict__ lv, float* __restrict__ lw) {
+                            SlotPlanes sp) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   int b = blockIdx.y * blockDim.y + threadIdx.y;
   int a = blockIdx.z;
@@ -487,59 +589,143 @@ __global__ void k_adv_w_vjp(Dims d, const float* __restrict__ rc, const float* _
   const float mlo = mu_wlev(d, lam, rc, a, b, t - 1, col_in && t >= 2 && t - 1 <= d.nz - 1);
   const float mup = mu_wlev(d, lam, rc, a, b, t + 1, col_in && t + 1 <= d.nz - 1);
   const long sx = d.sxw(), sy = d.syw();
+  // slots (padded targets): 0..5 lw (a+q, J, K), 6..11 lw (I, b+q, K), 12..17 lw
+  // (I, J, t-2+q); 18 lu (I, J, K-1), 19 lu (I, J, K), 20 lu (I+1, J, K);
+  // 21 lv (I, J, K-1), 22 lv (I, J, K), 23 lv (I, J+1, K)
+  float o[kSlotsMom];
+#pragma unroll
+  for (int q = 0; q < kSlotsMom; ++q) o[q] = 0.f;
   float q6[6], g6[6];
   if (lev && b < d.ny) {  // x flux: lower flux of w column a, upper flux of column a-1
-    const float ml = mlx;
-    const float gF = (mc - ml) * d.idx;
+    const float gF = (mc - mlx) * d.idx;
     if (gF != 0.f) {
       const long u0 = iu(d, I, J, K - 1), u1 = iu(d, I, J, K);
       const float mass = 0.5f * (rc[K - 1] * u[u0] + rc[K] * u[u1]);
       const long base = iw(d, a, J, K);
+#pragma unroll
       for (int q = 0; q < 6; ++q) q6[q] = w[base + q * sx];
-      float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
-      scatter6(lw, base, sx, g6, gF * mass);
-      aadd(lu + u0, gF * R * 0.5f * rc[K - 1]);
-      aadd(lu + u1, gF * R * 0.5f * rc[K]);
+      const float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+      const float cf = gF * mass;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) o[q] = cf * g6[q];
+      o[18] = gF * R * 0.5f * rc[K - 1];
+      o[19] = gF * R * 0.5f * rc[K];
     }
   }
   if (lev && a < d.nx) {  // y flux: lower flux of w row b, upper flux of row b-1
-    const float ml = mly;
-    const float gF = (mc - ml) * d.idy;
+    const float gF = (mc - mly) * d.idy;
     if (gF != 0.f) {
       const long v0 = iv(d, I, J, K - 1), v1 = iv(d, I, J, K);
       const float mass = 0.5f * (rc[K - 1] * v[v0] + rc[K] * v[v1]);
       const long base = iw(d, I, b, K);
+#pragma unroll
       for (int q = 0; q < 6; ++q) q6[q] = w[base + q * sy];
-      float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
-      scatter6(lw, base, sy, g6, gF * mass);
-      aadd(lv + v0, gF * R * 0.5f * rc[K - 1]);
-      aadd(lv + v1, gF * R * 0.5f * rc[K]);
+      const float R = hrecon_grad(q6, upwind_pos(mass), weps, g6);
+      const float cf = gF * mass;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) o[6 + q] = cf * g6[q];
+      o[21] = gF * R * 0.5f * rc[K - 1];
+      o[22] = gF * R * 0.5f * rc[K];
     }
   }
-  if (!col_in) return;
-  const long cb = iw(d, I, J, 0);
-  const float* col = w + cb;
-  float* lcol = lw + cb;
-  if (t >= 1) {  // z flux at scalar level t: lower flux of w level t, upper flux of level t-1
-    const float gF = (mc - mlo) * d.idz;
-    if (gF != 0.f) {
-      const int cz = t;
-      const float mass = 0.5f * (col[cz] + col[cz + 1]) * rc[cz];
-      float R = vrecon_grad(col, d.nz + 3, cz, upwind_pos(mass), weps, g6);
-      scatter6z(lcol, cz, d.nz + 3, g6, gF * mass);
-      if (R != 0.f) {
+  if (col_in) {
+    const float* col = w + iw(d, I, J, 0);
+    if (t >= 1) {  // z flux at scalar level t: lower flux of w level t, upper flux of level t-1
+      const float gF = (mc - mlo) * d.idz;
+      if (gF != 0.f) {
+        const int cz = t;
+        const float mass = 0.5f * (col[cz] + col[cz + 1]) * rc[cz];
+        const float R = vrecon_grad(col, d.nz + 3, cz, upwind_pos(mass), weps, g6);
+        const float cf = gF * mass;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) o[12 + q] = cf * g6[q];
         const float gm = gF * R * 0.5f * rc[cz];
-        aadd(lcol + cz, gm);
-        aadd(lcol + cz + 1, gm);
+        o[14] += gm;
+        o[15] += gm;
       }
     }
+    if (lev) o[15] += mc * 0.5f * (fd_at(d, rc, u, v, w, I, J, K - 1) + fd_at(d, rc, u, v, w, I, J, K));
+    if (t < d.nz) {
+      // div(rho0 u) of cell level K = t+1, averaged by w levels t (its lower) and t+1 (its upper)
+      const float gsum = mc * col[K] + mup * col[K + 1];
+      const FdGrad f = fd_grad(d, rc, K, 0.5f * gsum);
+      o[20] = f.u_hi;
+      o[19] += f.u_lo;
+      o[23] = f.v_hi;
+      o[22] += f.v_lo;
+      o[16] += f.w_hi;
+      o[15] += f.w_lo;
+    }
   }
-  if (lev) aadd(lcol + K, mc * 0.5f * (fd_at(d, rc, u, v, w, I, J, K - 1) + fd_at(d, rc, u, v, w, I, J, K)));
-  if (t < d.nz) {
-    // div(rho0 u) of cell level K = t+1, averaged by w levels t (its lower) and t+1 (its upper)
-    const float gsum = mc * col[K] + mup * col[K + 1];
-    fd_vjp(d, rc, I, J, K, 0.5f * gsum, lu, lv, lw);
-  }
+  const long p = sp.at(a, b, t);
+#pragma unroll
+  for (int q = 0; q < kSlotsMom; ++q) sp.put(q, p, o[q]);
+}
+
+// The momentum reverses' slots that reach each padded point, added in slot order
+// (U, V, W: the slot planes of k_adv_u_vjp, k_adv_v_vjp, k_adv_w_vjp; a null plane set
+// contributes nothing).
+__global__ void k_gather_lu(Dims d, SlotPlanes U, SlotPlanes V, SlotPlanes W, float* __restrict__ lu) {
+  const int K = blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = blockIdx.y * blockDim.y + threadIdx.y;
+  const int I = blockIdx.z;
+  if (K >= d.nz + 2 || J >= d.ny + 6) return;
+  float a = 0.f;
+#pragma unroll
+  for (int t = 0; t < 6; ++t) a += U.get(t, I - t, J - 3, K - 1);
+#pragma unroll
+  for (int t = 0; t < 6; ++t) a += U.get(6 + t, I - 3, J - t, K - 1);
+#pragma unroll
+  for (int t = 0; t < 6; ++t) a += U.get(12 + t, I - 3, J - 3, K + 2 - t);
+  a += V.get(18, I - 4, J - 2, K - 1);
+  a += V.get(19, I - 3, J - 2, K - 1);
+  a += V.get(20, I - 3, J - 3, K - 1);
+  a += W.get(18, I - 3, J - 3, K);
+  a += W.get(19, I - 3, J - 3, K - 1);
+  a += W.get(20, I - 4, J - 3, K - 1);
+  lu[iu(d, I, J, K)] += a;
+}
+
+__global__ void k_gather_lv(Dims d, SlotPlanes U, SlotPlanes V, SlotPlanes W, float* __restrict__ lv) {
+  const int K = blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = blockIdx.y * blockDim.y + threadIdx.y;
+  const int I = blockIdx.z;
+  if (K >= d.nz + 2 || J >= d.ny + 7) return;
+  float a = 0.f;
+#pragma unroll
+  for (int t = 0; t < 6; ++t) a += V.get(t, I - t, J - 3, K - 1);
+#pragma unroll
+  for (int t = 0; t < 6; ++t) a += V.get(6 + t, I - 3, J - t, K - 1);
+#pragma unroll
+  for (int t = 0; t < 6; ++t) a += V.get(12 + t, I - 3, J - 3, K + 2 - t);
+  a += U.get(18, I - 2, J - 4, K - 1);
+  a += U.get(19, I - 2, J - 3, K - 1);
+  a += U.get(20, I - 3, J - 3, K - 1);
+  a += W.get(21, I - 3, J - 3, K);
+  a += W.get(22, I - 3, J - 3, K - 1);
+  a += W.get(23, I - 3, J - 4, K - 1);
+  lv[iv(d, I, J, K)] += a;
+}
+
+__global__ void k_gather_lw(Dims d, SlotPlanes U, SlotPlanes V, SlotPlanes W, float* __restrict__ lw) {
+  const int K = blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = blockIdx.y * blockDim.y + threadIdx.y;
+  const int I = blockIdx.z;
+  if (K >= d.nz + 3 || J >= d.ny + 6) return;
+  float a = 0.f;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) a += W.get(q, I - q, J - 3, K - 1);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) a += W.get(6 + q, I - 3, J - q, K - 1);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) a += W.get(12 + q, I - 3, J - 3, K + 2 - q);
+  a += U.get(21, I - 2, J - 3, K - 2);
+  a += U.get(22, I - 2, J - 3, K - 1);
+  a += U.get(23, I - 3, J - 3, K - 1);
+  a += V.get(21, I - 3, J - 2, K - 2);
+  a += V.get(22, I - 3, J - 2, K - 1);
+  a += V.get(23, I - 3, J - 3, K - 1);
+  lw[iw(d, I, J, K)] += a;
 }
 
 // ------------------------------------------------------------------ B1 buoyancy
@@ -563,36 +749,33 @@ __global__ void k_theta_rho_vjp(long n, const float* __restrict__ th, const floa
 }
 
 // b8w (interpolate_scalar2w + extrapolate_bottom_top of calculate_buoyancy,
-// pressure_equations.py:123-172), lam on (nx, ny, nz+1) w levels -> theta, qv adjoints
+// pressure_equations.py:123-172), lam on (nx, ny, nz+1) w levels -> theta, qv adjoints.
+// Gather form, one thread per cell: w level l in 1..nz-1 averages cells l-1 and l, level 0
+// extrapolates from cells 0, 1, 2 and level nz from nz-1, nz-2, nz-3 (1.875, -1.25, 0.375);
+// each cell sums the weights of the levels that read it in one fixed order.
 __global__ void k_b8w_vjp(Dims d, const float* __restrict__ th0c, const float* __restrict__ qv0c,
                           const float* __restrict__ th, const float* __restrict__ qv, const float* __restrict__ lam,
                           float* __restrict__ lth, float* __restrict__ lqv) {
-  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   int i = blockIdx.z;
-  if (l > d.nz || j >= d.ny) return;
-  float L = lam[((long)i * d.ny + j) * (d.nz + 1) + l];
-  if (L == 0.f) return;
-  long c = ic(d, i + 3, j + 3, 0);
-  int ks[3];
-  float cf[3];
-  int n;
-  if (l == 0) {
-    ks[0] = 0; ks[1] = 1; ks[2] = 2; cf[0] = 1.875f; cf[1] = -1.25f; cf[2] = 0.375f; n = 3;
-  } else if (l == d.nz) {
-    ks[0] = d.nz - 1; ks[1] = d.nz - 2; ks[2] = d.nz - 3; cf[0] = 1.875f; cf[1] = -1.25f; cf[2] = 0.375f; n = 3;
-  } else {
-    ks[0] = l - 1; ks[1] = l; cf[0] = 0.5f; cf[1] = 0.5f; cf[2] = 0.f; n = 2;
-  }
-  for (int q = 0; q < n; ++q) {
-    int K = ks[q] + 1;
-    float tr0 = th0c[K] * (1.0f + (float)kReps * qv0c[K]) / (1.0f + qv0c[K]);
-    float a, b;
-    theta_rho_grad(th[c + K], qv[c + K], a, b);
-    float gB = L * cf[q] * (float)kG / tr0;
-    aadd(lth + c + K, gB * a);
-    aadd(lqv + c + K, gB * b);
-  }
+  if (k >= d.nz || j >= d.ny) return;
+  const int nz = d.nz;
+  const float* lc = lam + ((long)i * d.ny + j) * (nz + 1);
+  float W = 0.f;
+  if (k <= 2) W += lc[0] * (k == 0 ? 1.875f : (k == 1 ? -1.25f : 0.375f));
+  if (k >= 1) W += lc[k] * 0.5f;           // level k: upper cell of its average
+  if (k <= nz - 2) W += lc[k + 1] * 0.5f;  // level k+1: lower cell of its average
+  if (k >= nz - 3) W += lc[nz] * (k == nz - 1 ? 1.875f : (k == nz - 2 ? -1.25f : 0.375f));
+  if (W == 0.f) return;
+  const int K = k + 1;
+  const long c = ic(d, i + 3, j + 3, K);
+  const float tr0 = th0c[K] * (1.0f + (float)kReps * qv0c[K]) / (1.0f + qv0c[K]);
+  float a, b;
+  theta_rho_grad(th[c], qv[c], a, b);
+  const float gB = W * (float)kG / tr0;
+  lth[c] += gB * a;
+  lqv[c] += gB * b;
 }
 
 // ------------------------------------------------------------- S1 acoustic substep
@@ -727,116 +910,6 @@ __global__ void k_acoustic_vjp_faces(Dims d, const float* __restrict__ p, const 
       lrw[o] += g * dts;
       pgf_vjp(lp, ltr, p, tr, ic(d, I, J, K), 1, d.idz, g * dts);
     }
-  }
-}
-
-// Gather form of the same reverse, used by the step driver.
-// k_acoustic_vjp_cells2: the new pi' (folded adjoint lpo) of each cell -> pi' (identity)
-// and pi* (its seven advection_pi points); the face-velocity terms are gathered by
-// k_acoustic_vjp_faces2 instead of scattered.
-__global__ void k_acoustic_vjp_cells2(Dims d, const float* __restrict__ rc, const float* __restrict__ pi0c,
-                                      const float* __restrict__ u, const float* __restrict__ v,
-                                      const float* __restrict__ w, const float* __restrict__ p,
-                                      const float* __restrict__ ps, const float* __restrict__ ru,
-                                      const float* __restrict__ rv, const float* __restrict__ rw,
-                                      const float* __restrict__ tr, const float* __restrict__ lpo, float dts,
-                                      float* __restrict__ lp, float* __restrict__ lps) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int i = blockIdx.z;
-  if (k >= d.nz || j >= d.ny) return;
-  int I = i + 3, J = j + 3, K = k + 1;
-  long c = ic(d, I, J, K);
-  float Lp = lpo[c];
-  if (Lp == 0.f) return;
-  const long sx = d.sxc(), sy = d.syc();
-  float un0 = u[iu(d, I, J, K)] + (pgf_x(d, p, tr, I, J, K) + ru[((long)i * d.ny + j) * d.nz + k]) * dts;
-  float un1 = u[iu(d, I + 1, J, K)] + (pgf_x(d, p, tr, I + 1, J, K) + ru[((long)(i + 1) * d.ny + j) * d.nz + k]) * dts;
-  float vn0 = v[iv(d, I, J, K)] + (pgf_y(d, p, tr, I, J, K) + rv[((long)i * (d.ny + 1) + j) * d.nz + k]) * dts;
-  float vn1 = v[iv(d, I, J + 1, K)] + (pgf_y(d, p, tr, I, J + 1, K) + rv[((long)i * (d.ny + 1) + j + 1) * d.nz + k]) * dts;
-  float wn0 = (k == 0) ? 0.0f : w[iw(d, I, J, K)] + (pgf_z(d, p, tr, I, J, K) + rw[((long)i * d.ny + j) * (d.nz + 1) + k]) * dts;
-  float wn1 = (k == d.nz - 1)
-                  ? 0.0f
-                  : w[iw(d, I, J, K + 1)] + (pgf_z(d, p, tr, I, J, K + 1) + rw[((long)i * d.ny + j) * (d.nz + 1) + k + 1]) * dts;
-  float div = (un1 - un0) * d.idx + (vn1 - vn0) * d.idy + (wn1 - wn0) * d.idz;
-  const float cr = (float)(-kRd / kCv);
-  aadd(lp + c, Lp);
-  float G = 1.1f * dts * Lp;
-  float lpc = cr * div * G;
-  float gx = -0.5f * G * d.idx, gy = -0.5f * G * d.idy;
-  lpc += gx * un0 - gx * un1 + gy * vn0 - gy * vn1;
-  aadd(lps + c - sx, -gx * un0);
-  aadd(lps + c + sx, gx * un1);
-  aadd(lps + c - sy, -gy * vn0);
-  aadd(lps + c + sy, gy * vn1);
-  if (k > 0) {
-    float gz = -0.5f * (0.5f * (rc[K - 1] + rc[K])) / rc[K] * G;
-    lpc += gz * wn0 * d.idz;
-    aadd(lps + c - 1, -gz * wn0 * d.idz);
-  }
-  if (k < d.nz - 1) {
-    float gz = -0.5f * (0.5f * (rc[K] + rc[K + 1])) / rc[K] * G;
-    lpc -= gz * wn1 * d.idz;
-    aadd(lps + c + 1, gz * wn1 * d.idz);
-  }
-  aadd(lps + c, lpc);
-}
-
-// k_acoustic_vjp_faces2: per face, the total adjoint of the new velocity = its folded
-// output adjoint + the terms of the (one or two) cells whose pressure tendency reads it,
-// gathered here; then old velocity (+= when acc, else =), rhs (+=, one owner per face) and
-// the PGF inputs (atomics).
-__global__ void k_acoustic_vjp_faces2(Dims d, const float* __restrict__ rc, const float* __restrict__ pi0c,
-                                      const float* __restrict__ p, const float* __restrict__ ps,
-                                      const float* __restrict__ tr, const float* __restrict__ lpo,
-                                      const float* __restrict__ luo, const float* __restrict__ lvo,
-                                      const float* __restrict__ lwo, float dts, int acc, float* __restrict__ lu,
-                                      float* __restrict__ lv, float* __restrict__ lw, float* __restrict__ lp,
-                                      float* __restrict__ ltr, float* __restrict__ lru, float* __restrict__ lrv,
-                                      float* __restrict__ lrw) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nz
-  int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
-  int i = blockIdx.z;                             // 0..nx
-  if (k > d.nz || j > d.ny) return;
-  const float cr = (float)(-kRd / kCv);
-  const float g11 = 1.1f * dts;
-  int I = i + 3, J = j + 3, K = k + 1;
-  if (k < d.nz && j < d.ny) {  // u face i
-    long o = ((long)i * d.ny + j) * d.nz + k;
-    long a = iu(d, I, J, K);
-    long cR = ic(d, I, J, K), cL = cR - d.sxc();
-    float dps = (ps[cR] - ps[cL]) * d.idx;
-    float g = luo[a];
-    if (i < d.nx) g += g11 * lpo[cR] * (-cr * (pi0c[K] + ps[cR]) * d.idx - 0.5f * dps);
-    if (i > 0) g += g11 * lpo[cL] * (cr * (pi0c[K] + ps[cL]) * d.idx - 0.5f * dps);
-    lu[a] = acc ? lu[a] + g : g;
-    lru[o] += g * dts;
-    if (g != 0.f) pgf_vjp(lp, ltr, p, tr, cR, d.sxc(), d.idx, g * dts);
-  }
-  if (k < d.nz && i < d.nx) {  // v face j
-    long o = ((long)i * (d.ny + 1) + j) * d.nz + k;
-    long a = iv(d, I, J, K);
-    long cR = ic(d, I, J, K), cL = cR - d.syc();
-    float dps = (ps[cR] - ps[cL]) * d.idy;
-    float g = lvo[a];
-    if (j < d.ny) g += g11 * lpo[cR] * (-cr * (pi0c[K] + ps[cR]) * d.idy - 0.5f * dps);
-    if (j > 0) g += g11 * lpo[cL] * (cr * (pi0c[K] + ps[cL]) * d.idy - 0.5f * dps);
-    lv[a] = acc ? lv[a] + g : g;
-    lrv[o] += g * dts;
-    if (g != 0.f) pgf_vjp(lp, ltr, p, tr, cR, d.syc(), d.idy, g * dts);
-  }
-  if (i < d.nx && j < d.ny && k > 0 && k < d.nz) {  // w level k: cell above (K) and below (K-1)
-    long o = ((long)i * d.ny + j) * (d.nz + 1) + k;
-    long a = iw(d, I, J, K);
-    long ca = ic(d, I, J, K), cb = ca - 1;
-    float r8 = 0.5f * (rc[K - 1] + rc[K]);
-    float dzz = (pi0c[K] - pi0c[K - 1]) * d.idz + (ps[ca] - ps[cb]) * d.idz;
-    float g = lwo[a];
-    g += g11 * lpo[ca] * (-cr * (pi0c[K] + ps[ca]) * d.idz + (-0.5f * r8 / rc[K]) * dzz);
-    g += g11 * lpo[cb] * (cr * (pi0c[K - 1] + ps[cb]) * d.idz + (-0.5f * r8 / rc[K - 1]) * dzz);
-    lw[a] = acc ? lw[a] + g : g;
-    lrw[o] += g * dts;
-    if (g != 0.f) pgf_vjp(lp, ltr, p, tr, ca, 1, d.idz, g * dts);
   }
 }
 
@@ -1222,40 +1295,46 @@ __global__ void __launch_bounds__(256) k_smag_k_sum(const double* __restrict__ p
   if (threadIdx.x == 0) lcg2[0] += tot;
 }
 
-// compute_nm (turbulence.py:391-427): folded adjoint at interior cells -> theta, qv
+// sum over the cells whose N^2 reads w level l of (adjoint of that N^2) x its weight:
+// cells 1..nz-2 average levels k and k+1, cell 0 extrapolates levels 1, 2, 3 and cell nz-1
+// levels nz-1, nz-2, nz-3 (1.875, -1.25, 0.375).  Lc[k+1] = adjoint of cell k's N^2.
+__device__ __forceinline__ float nm_level_weight(const float* __restrict__ Lc, int nz, int l) {
+  float G = 0.f;
+  if (l >= 1 && l <= 3) G += Lc[1] * (l == 1 ? 1.875f : (l == 2 ? -1.25f : 0.375f));
+  if (l >= 1 && l <= nz - 2) G += 0.5f * Lc[l + 1];
+  if (l >= 2 && l <= nz - 1) G += 0.5f * Lc[l];
+  if (l >= nz - 3 && l <= nz - 1) G += Lc[nz] * (l == nz - 1 ? 1.875f : (l == nz - 2 ? -1.25f : 0.375f));
+  return G;
+}
+
+// compute_nm (turbulence.py:391-427), reverse: folded adjoint at interior cells -> theta,
+// qv.  N^2 at w level l = g/dz log(a/b), a = theta (1 + (Rv/Rd - 1) qv) at padded level
+// l+1, b the same at level l.  Gather form, one thread per cell: the cell is the upper
+// operand of level P-1 and the lower one of level P.
 __global__ void k_smag_nm_vjp(Dims d, const float* __restrict__ th, const float* __restrict__ qv,
                               const float* __restrict__ ln2, float* __restrict__ lth, float* __restrict__ lqv) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   int i = blockIdx.z;
   if (k >= d.nz || j >= d.ny) return;
-  long c = ic(d, i + 3, j + 3, 0);
-  float L = ln2[c + k + 1];
-  if (L == 0.f) return;
-  int nz = d.nz;
-  int ls[3];
-  float cf[3];
-  int n;
-  if (k == 0) {
-    ls[0] = 1; ls[1] = 2; ls[2] = 3; cf[0] = 1.875f; cf[1] = -1.25f; cf[2] = 0.375f; n = 3;
-  } else if (k == nz - 1) {
-    ls[0] = nz - 1; ls[1] = nz - 2; ls[2] = nz - 3; cf[0] = 1.875f; cf[1] = -1.25f; cf[2] = 0.375f; n = 3;
-  } else {
-    ls[0] = k; ls[1] = k + 1; cf[0] = 0.5f; cf[1] = 0.5f; cf[2] = 0.f; n = 2;
+  const long c = ic(d, i + 3, j + 3, 0);
+  const float* Lc = ln2 + c;
+  const int nz = d.nz, P = k + 1;
+  const float em = (float)kRepsm1, gs = (float)kG / d.dz;
+  const float t = th[c + P], f = 1.0f + em * qv[c + P];
+  float dth = 0.f, dqv = 0.f;
+  if (P - 1 >= 1) {  // upper operand of w level P-1
+    const float ga = nm_level_weight(Lc, nz, P - 1) * gs / (t * f);
+    dth += ga * f;
+    dqv += ga * t * em;
   }
-  const float em = (float)kRepsm1;
-  for (int q = 0; q < n; ++q) {
-    int l = ls[q];
-    // N^2 at w level l = g/dz * log(a/b), a at padded level l+1, b at level l
-    float g = L * cf[q] * (float)kG / d.dz;
-    long ca = c + l + 1, cb = c + l;
-    float fa = 1.0f + em * qv[ca], fb = 1.0f + em * qv[cb];
-    float ga = g / (th[ca] * fa), gb = -g / (th[cb] * fb);
-    aadd(lth + ca, ga * fa);
-    aadd(lqv + ca, ga * th[ca] * em);
-    aadd(lth + cb, gb * fb);
-    aadd(lqv + cb, gb * th[cb] * em);
+  if (P <= nz - 1) {  // lower operand of w level P
+    const float gb = -(nm_level_weight(Lc, nz, P) * gs) / (t * f);
+    dth += gb * f;
+    dqv += gb * t * em;
   }
+  lth[c + P] += dth;
+  lqv[c + P] += dqv;
 }
 
 // compute_deformation (turbulence.py:298-388), reverse in two passes.  Pass 1 (interior

@@ -7,6 +7,7 @@
 // Forward structure: rk_sub_step0/1/2 (one_step_integration.py:16-447) and
 // ws_rk3_big_step (one_sprint_functions.py:10-30), as in rk3_step (lex_stage.hip).
 #pragma once
+#include <algorithm>
 #include "lex_adjoint.hip"
 #include "lex_stage.hip"
 
@@ -23,6 +24,7 @@ struct VjpLayout {
   size_t off_ac;                      // acoustic substep inputs t >= 1, stage by stage
   size_t off_lXA, off_lXB, off_lP[2], off_lU, off_lV, off_lW, off_lru, off_lrv, off_lrw, off_ltr;
   size_t off_lsu, off_lsv, off_lsw, off_lss, off_lT, off_lsm, off_lcg2;
+  size_t off_slots;                   // slot planes of the advection reverses (never cleared)
   size_t total;
 };
 
@@ -81,6 +83,9 @@ inline VjpLayout vjp_layout(const Dims& d, double dt, int n_sound, int np) {
   L.off_lT = o; o += S;
   L.off_lsm = o; o += 10 * C;
   L.off_lcg2 = o; o += 256 + al256((size_t)kSmagKBlocks * sizeof(double));  // + reduction partials
+  // the scalar reverse and the momentum reverses use the planes in turn; every launch
+  // rewrites all of its planes, so they need no clearing
+  L.off_slots = o; o += al256(std::max(slot_bytes_scalar(d), slot_bytes_mom(d)));
   L.total = o;
   return L;
 }
@@ -278,6 +283,11 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
   for (int q = 0; q < nsc; ++q) lss[q] = (float*)(A(L.off_lss) + q * al256(L.ic_ * 4));
   float* lT = (float*)(A(L.off_lT));
   double* lcg2 = (double*)(A(L.off_lcg2));
+  float* slots = (float*)(A(L.off_slots));
+  const SlotPlanes spS = slots_scalar(d, slots);
+  const SlotPlanes spU = slots_u(d, slots);
+  const SlotPlanes spV = slots_v(d, slots + kSlotsMom * spU.plane);
+  const SlotPlanes spW = slots_w(d, slots + kSlotsMom * (spU.plane + spV.plane));
   const float* rc = b.rho0;
   const bool turb = o.turb != 0;
   const double dtd = (double)o.dt;
@@ -314,7 +324,8 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
                          (float)dt_s, lam_in[fi], lT, turb ? lss[q] : nullptr);
       hipLaunchKernelGGL(k_adv_scalar_vjp, cx.G(d.nz, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, (const float*)src[fi],
                          (const float*)src[1], (const float*)src[2], (const float*)src[3], (const float*)lT, weps,
-                         lsrc[fi], lsrc[1], lsrc[2], lsrc[3]);
+                         spS, lsrc[1], lsrc[2], lsrc[3]);
+      hipLaunchKernelGGL(k_gather_scalar, cx.G(d.nz + 2, d.ny + 6, d.nx + 6), cx.bk, 0, st, d, spS, lsrc[fi]);
     }
     // acoustic substeps, last to first
     const int nsub = L.nsub[s];
@@ -353,12 +364,15 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
                          (const float*)lrw, lsw);
     }
     hipLaunchKernelGGL(k_adv_u_vjp, cx.G(d.nz, d.ny + 1, d.nx + 2), cx.bk, 0, st, d, rc, (const float*)src[1],
-                       (const float*)src[2], (const float*)src[3], (const float*)lru, lsrc[1], lsrc[2], lsrc[3]);
+                       (const float*)src[2], (const float*)src[3], (const float*)lru, spU);
     hipLaunchKernelGGL(k_adv_v_vjp, cx.G(d.nz, d.ny + 2, d.nx + 1), cx.bk, 0, st, d, rc, (const float*)src[1],
-                       (const float*)src[2], (const float*)src[3], (const float*)lrv, lsrc[1], lsrc[2], lsrc[3]);
+                       (const float*)src[2], (const float*)src[3], (const float*)lrv, spV);
     hipLaunchKernelGGL(k_adv_w_vjp, cx.G(d.nz + 1, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, (const float*)src[1],
-                       (const float*)src[2], (const float*)src[3], (const float*)lrw, lsrc[1], lsrc[2], lsrc[3]);
-    hipLaunchKernelGGL(k_b8w_vjp, cx.G(d.nz + 1, d.ny, d.nx), cx.bk, 0, st, d, b.theta0, b.qv0,
+                       (const float*)src[2], (const float*)src[3], (const float*)lrw, spW);
+    hipLaunchKernelGGL(k_gather_lu, cx.G(d.nz + 2, d.ny + 6, d.nx + 7), cx.bk, 0, st, d, spU, spV, spW, lsrc[1]);
+    hipLaunchKernelGGL(k_gather_lv, cx.G(d.nz + 2, d.ny + 7, d.nx + 6), cx.bk, 0, st, d, spU, spV, spW, lsrc[2]);
+    hipLaunchKernelGGL(k_gather_lw, cx.G(d.nz + 3, d.ny + 6, d.nx + 6), cx.bk, 0, st, d, spU, spV, spW, lsrc[3]);
+    hipLaunchKernelGGL(k_b8w_vjp, cx.G(d.nz, d.ny, d.nx), cx.bk, 0, st, d, b.theta0, b.qv0,
                        (const float*)src[0], (const float*)src[5], (const float*)lrw, lsrc[0], lsrc[5]);
     long n = (long)L.nc;
     hipLaunchKernelGGL(k_theta_rho_vjp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n,

@@ -676,42 +676,82 @@ int lex_fold(const lex_grid_t* g, int mode, float* a, void* stream) {
   return check_launch("lex_fold");
 }
 
+// Stream-ordered scratch for the slot planes of a stand-alone advection reverse: taken
+// from the stream's memory pool before the kernels and returned after them on the same
+// stream.
+struct SlotScratch {
+  float* p = nullptr;
+  hipStream_t st = nullptr;
+  hipError_t take(size_t bytes, hipStream_t s) {
+    st = s;
+    return hipMallocAsync((void**)&p, bytes, s);
+  }
+  ~SlotScratch() {
+    if (p) (void)hipFreeAsync(p, st);
+  }
+};
+
 int lex_adv_scalar_vjp(const lex_grid_t* g, const float* r, const float* s, const float* u, const float* v,
                        const float* w, const float* lam, float weps, float* ls, float* lu, float* lv, float* lw,
                        void* stream) {
   REQGRID(g);
   REQ(r); REQ(s); REQ(u); REQ(v); REQ(w); REQ(lam); REQ(ls); REQ(lu); REQ(lv); REQ(lw);
   Dims d = mkdims(g);
-  hipLaunchKernelGGL(k_adv_scalar_vjp, grd(d.nz, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, r, s, u, v, w, lam, weps, ls,
+  SlotScratch sc;
+  if (sc.take(slot_bytes_scalar(d), S(stream)) != hipSuccess)
+    return fail(-4, "%s: no device memory for the slot planes", "lex_adv_scalar_vjp");
+  const SlotPlanes sp = slots_scalar(d, sc.p);
+  hipLaunchKernelGGL(k_adv_scalar_vjp, grd(d.nz, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, r, s, u, v, w, lam, weps, sp,
                      lu, lv, lw);
+  hipLaunchKernelGGL(k_gather_scalar, grd(d.nz + 2, d.ny + 6, d.nx + 6), blk(), 0, S(stream), d, sp, ls);
   return check_launch("lex_adv_scalar_vjp");
+}
+
+// one momentum reverse (which = 0, 1, 2 for u, v, w) and the three gathers of its slots
+static int adv_mom_vjp(const char* name, int which, const Dims& d, const float* r, const float* u, const float* v,
+                       const float* w, const float* lam, float* lu, float* lv, float* lw, hipStream_t st) {
+  SlotScratch sc;
+  const size_t bytes = (size_t)kSlotsMom * 4 *
+                       (which == 0 ? slots_u(d, nullptr).plane : (which == 1 ? slots_v(d, nullptr).plane
+                                                                               : slots_w(d, nullptr).plane));
+  if (sc.take(bytes, st) != hipSuccess) return fail(-4, "%s: no device memory for the slot planes", name);
+  SlotPlanes none = slot_planes(nullptr, 0, 0, 0);
+  SlotPlanes U = none, V = none, W = none;
+  if (which == 0) {
+    U = slots_u(d, sc.p);
+    hipLaunchKernelGGL(k_adv_u_vjp, grd(d.nz, d.ny + 1, d.nx + 2), blk(), 0, st, d, r, u, v, w, lam, U);
+  } else if (which == 1) {
+    V = slots_v(d, sc.p);
+    hipLaunchKernelGGL(k_adv_v_vjp, grd(d.nz, d.ny + 2, d.nx + 1), blk(), 0, st, d, r, u, v, w, lam, V);
+  } else {
+    W = slots_w(d, sc.p);
+    hipLaunchKernelGGL(k_adv_w_vjp, grd(d.nz + 1, d.ny + 1, d.nx + 1), blk(), 0, st, d, r, u, v, w, lam, W);
+  }
+  hipLaunchKernelGGL(k_gather_lu, grd(d.nz + 2, d.ny + 6, d.nx + 7), blk(), 0, st, d, U, V, W, lu);
+  hipLaunchKernelGGL(k_gather_lv, grd(d.nz + 2, d.ny + 7, d.nx + 6), blk(), 0, st, d, U, V, W, lv);
+  hipLaunchKernelGGL(k_gather_lw, grd(d.nz + 3, d.ny + 6, d.nx + 6), blk(), 0, st, d, U, V, W, lw);
+  return check_launch(name);
 }
 
 int lex_adv_u_vjp(const lex_grid_t* g, const float* r, const float* u, const float* v, const float* w,
                   const float* lam, float* lu, float* lv, float* lw, void* stream) {
   REQGRID(g);
   REQ(r); REQ(u); REQ(v); REQ(w); REQ(lam); REQ(lu); REQ(lv); REQ(lw);
-  Dims d = mkdims(g);
-  hipLaunchKernelGGL(k_adv_u_vjp, grd(d.nz, d.ny + 1, d.nx + 2), blk(), 0, S(stream), d, r, u, v, w, lam, lu, lv, lw);
-  return check_launch("lex_adv_u_vjp");
+  return adv_mom_vjp("lex_adv_u_vjp", 0, mkdims(g), r, u, v, w, lam, lu, lv, lw, S(stream));
 }
 
 int lex_adv_v_vjp(const lex_grid_t* g, const float* r, const float* u, const float* v, const float* w,
                   const float* lam, float* lu, float* lv, float* lw, void* stream) {
   REQGRID(g);
   REQ(r); REQ(u); REQ(v); REQ(w); REQ(lam); REQ(lu); REQ(lv); REQ(lw);
-  Dims d = mkdims(g);
-  hipLaunchKernelGGL(k_adv_v_vjp, grd(d.nz, d.ny + 2, d.nx + 1), blk(), 0, S(stream), d, r, u, v, w, lam, lu, lv, lw);
-  return check_launch("lex_adv_v_vjp");
+  return adv_mom_vjp("lex_adv_v_vjp", 1, mkdims(g), r, u, v, w, lam, lu, lv, lw, S(stream));
 }
 
 int lex_adv_w_vjp(const lex_grid_t* g, const float* r, const float* u, const float* v, const float* w,
                   const float* lam, float* lu, float* lv, float* lw, void* stream) {
   REQGRID(g);
   REQ(r); REQ(u); REQ(v); REQ(w); REQ(lam); REQ(lu); REQ(lv); REQ(lw);
-  Dims d = mkdims(g);
-  hipLaunchKernelGGL(k_adv_w_vjp, grd(d.nz + 1, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, r, u, v, w, lam, lu, lv, lw);
-  return check_launch("lex_adv_w_vjp");
+  return adv_mom_vjp("lex_adv_w_vjp", 2, mkdims(g), r, u, v, w, lam, lu, lv, lw, S(stream));
 }
 
 int lex_buoyancy_vjp(const lex_grid_t* g, const float* th0, const float* qv0, const float* th, const float* qv,
@@ -719,7 +759,7 @@ int lex_buoyancy_vjp(const lex_grid_t* g, const float* th0, const float* qv0, co
   REQGRID(g);
   REQ(th0); REQ(qv0); REQ(th); REQ(qv); REQ(lam_b8w); REQ(lth); REQ(lqv);
   Dims d = mkdims(g);
-  hipLaunchKernelGGL(k_b8w_vjp, grd(d.nz + 1, d.ny, d.nx), blk(), 0, S(stream), d, th0, qv0, th, qv, lam_b8w, lth, lqv);
+  hipLaunchKernelGGL(k_b8w_vjp, grd(d.nz, d.ny, d.nx), blk(), 0, S(stream), d, th0, qv0, th, qv, lam_b8w, lth, lqv);
   return check_launch("lex_buoyancy_vjp");
 }
 

@@ -266,6 +266,30 @@ def test_step_vjp_vs_fd():
     assert nf == len(grads)
 
 
+def test_step_vjp_is_bit_reproducible():
+    """The step's reverse uses no atomics (the advection reverses write slot planes that
+    fixed-order gathers add; buoyancy and N^2 reverses gather per cell), so two calls on
+    the same inputs give bit-identical adjoints and dL/dCs, on a grid large enough that
+    float atomics would reorder their additions from run to run."""
+    import torch
+    from lex_amd import adjoint as adj
+    from lex_amd._core import grid_from
+    case, opts, og, oo, st, base, grids = _step_setup(nx=40, ny=32, nz=24, seed=12)
+    rng = np.random.default_rng(13)
+    g = grid_from(base[1], *grids[3:6])
+    dev = tuple(to_dev(a) for a in st)
+    lam = tuple(to_dev(rng.normal(size=a.shape)) for a in st)
+    runs = [adj.step_vjp(dev, lam, base, g, opts["dt"], opts["n_sound"], 0.18, 1.0 / 3.0) for _ in range(3)]
+    torch.cuda.synchronize()
+    (li0, dcs0) = runs[0]
+    assert all(bool(torch.isfinite(t).all()) for t in li0)
+    assert any(float(t.abs().max()) > 0 for t in li0)
+    for li, dcs in runs[1:]:
+        for q, (a, b) in enumerate(zip(li0, li)):
+            assert torch.equal(a, b), f"adjoint of state field {q} differs between runs"
+        assert torch.equal(dcs0, dcs)
+
+
 def test_sprint_gradients_with_checkpoints():
     """Differentiable 4-step run (checkpoint every 2): d loss/d theta'_0 along random
     directions and d loss/d Cs vs FP64 central differences of the oracle; the gradient
