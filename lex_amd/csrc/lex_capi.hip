@@ -1066,6 +1066,17 @@ namespace {
 template <int CI, int CO>
 void launch_fwd(int nx, int ny, int nz, const void* x, const void* wp, const float* bias, int mode, int cout_real,
                 void* y, float* yf, float* pre, hipStream_t st) {
+  if (CI <= 32 && CO == 32) {   // persistent, weights in registers
+    constexpr int CIP = CI <= 32 ? CI : 32;
+#define LEX_FWD32P(MD)                                                                                  \
+  if ((mode & 7) == MD) {                                                                               \
+    hipLaunchKernelGGL((k_conv3_fwd32p<CIP, MD>), dim3(conv_fwd32p_grid<CIP, MD>(nx, ny, nz)), dim3(256), 0, st, \
+                       nx, ny, nz, (const __bf16*)x, (const __bf16*)wp, bias, cout_real, (__bf16*)y, yf, pre); \
+    return;                                                                                             \
+  }
+    LEX_FWD32P(0) LEX_FWD32P(1) LEX_FWD32P(2) LEX_FWD32P(3) LEX_FWD32P(4) LEX_FWD32P(5) LEX_FWD32P(6) LEX_FWD32P(7)
+#undef LEX_FWD32P
+  }
   if (CO % 32 == 0) {   // 32 x 32 MFMA tiles: one A fragment per 32 output channels
     dim3 grid((nz + CT2Z - 1) / CT2Z, (ny + CT2Y - 1) / CT2Y, (nx + CT2X - 1) / CT2X);
     hipLaunchKernelGGL((k_conv3_fwd32<CI, (CO % 32 == 0 ? CO : 32)>), grid, dim3(256), 0, st, nx, ny, nz,

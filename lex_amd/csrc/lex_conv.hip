@@ -13,6 +13,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "lex_common.hpp"
 
 namespace lex {
 
@@ -34,16 +35,27 @@ __device__ __forceinline__ float tanh_fast(float y) {
   const float e = __expf(2.0f * y);
   return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + e);
 }
-// tanh-approximate GELU and its derivative
+// tanh-approximate GELU and its derivative.  0.5 x (1 + tanh(y)) = x / (1 + e^(-2y)) with
+// y = k0 (x + k1 x^3): seven instructions, e^(-2y) = 2^(x (c0 + c1 x^2)) with
+// c0 = -2 k0 log2(e), c1 = -2 k0 k1 log2(e); it saturates to x and to -0 like the tanh form.
 __device__ __forceinline__ float gelu_t(float x) {
-  const float k0 = 0.7978845608028654f, k1 = 0.044715f;
-  return 0.5f * x * (1.0f + tanh_fast(k0 * (x + k1 * x * x * x)));
+  const float c0 = (float)(-2.0 * 0.7978845608028654 * 1.4426950408889634);
+  const float c1 = (float)(-2.0 * 0.7978845608028654 * 0.044715 * 1.4426950408889634);
+  const float e = __builtin_amdgcn_exp2f(x * fmaf(c1, x * x, c0));
+  return x * __builtin_amdgcn_rcpf(1.0f + e);
 }
 __device__ __forceinline__ float gelu_t_grad(float x) {
   const float k0 = 0.7978845608028654f, k1 = 0.044715f;
   float t = tanh_fast(k0 * (x + k1 * x * x * x));
   return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * k0 * (1.0f + 3.0f * k1 * x * x);
 }
+
+// Element stride of one cell in an LDS halo tile: the CIN channels plus 8 (16 bytes) of
+// padding.  An MFMA operand read takes 16 bytes from each of 16 consecutive cells (one z
+// run); with a stride of CIN elements (32 or 64 bytes for 16 or 32 channels) those reads
+// fall on 2 or 4 distinct bank groups of the 256-byte bank row, with CIN + 8 (48 or 80
+// bytes, an odd multiple of 16) on 16 distinct ones.
+__host__ __device__ constexpr int conv_cell_stride(int cin) { return cin + 8; }
 
 // K-steps of 32 over (tap, channel): CIN <= 32 packs 32 / CIN taps per step; CIN = 64
 // spends two steps per tap (channels 0..31, then 32..63).
@@ -72,9 +84,11 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(int nx, int ny, int nz, const
                                                    const __bf16* __restrict__ Wp, const float* __restrict__ bias,
                                                    int mode, int cout_real, __bf16* __restrict__ Y,
                                                    float* __restrict__ Yf, float* __restrict__ pre) {
-  constexpr int NSTEP = ConvSteps<CIN>::NSTEP, NT = COUT / 16;
-  __shared__ __attribute__((aligned(16))) __bf16 tile[HX * HY * HZ * CIN];
-  const int z0 = blockIdx.x * CTZ, y0 = blockIdx.y * CTY, x0 = blockIdx.z * CTX;
+  constexpr int NSTEP = ConvSteps<CIN>::NSTEP, NT = COUT / 16, CS = conv_cell_stride(CIN);
+  __shared__ __attribute__((aligned(16))) __bf16 tile[HX * HY * HZ * CS];
+  // neighbouring tiles (which share halo cells) on one XCD: z tiles fastest, then y, x
+  const TileId tb = xcd_tile();
+  const int z0 = tb.x * CTZ, y0 = tb.y * CTY, x0 = tb.z * CTX;
   constexpr int nchunk = HX * HY * HZ * CIN / 8;
   const bool circ = (mode & 4) != 0;  // periodic x and y (flax padding 'CIRCULAR' there), zero z
   for (int c = threadIdx.x; c < nchunk; c += 256) {
@@ -91,7 +105,7 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(int nx, int ny, int nz, const
     } else {
       for (int j = 0; j < 8; ++j) v[j] = (__bf16)0.0f;
     }
-    *(bf16x8*)(tile + cell * CIN + 8 * g) = v;
+    *(bf16x8*)(tile + cell * CS + 8 * g) = v;
   }
   __syncthreads();
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, m = l & 15, g = l >> 4;
@@ -116,7 +130,7 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(int nx, int ny, int nz, const
 #pragma unroll
     for (int mt = 0; mt < CTY; ++mt) {
       const int cell = ((w + 1 + dx) * HY + (mt + 1 + dy)) * HZ + (m + 1 + dz);
-      a[mt] = *(const bf16x8*)(tile + cell * CIN + ch0);
+      a[mt] = *(const bf16x8*)(tile + cell * CS + ch0);
     }
 #pragma unroll
     for (int mt = 0; mt < CTY; ++mt)
@@ -156,7 +170,7 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(int nx, int ny, int nz, const
 // MFMA covers 32 cells x 32 output channels, so each A fragment read from LDS (32 cells x
 // 16 k) feeds 32 (or 64) output channels, half the LDS bytes per flop of the 16x16x32
 // form, whose A fragments feed 16.  Output tile 4 x 4 x 16 (x, y, z) per 4-wave
-// workgroup, as k_conv3_fwd (the same 41 KB halo tile for 32 channels): wave w owns
+// workgroup, as k_conv3_fwd (the same 52 KB halo tile for 32 channels): wave w owns
 // x = x0 + w and two accumulator blocks, each 2 y rows x 16 z cells as the 32 MFMA rows
 // (row m: y offset m / 16, z offset m % 16).  Operand layout: lane l supplies A[row l%32][k 8(l/32)..+7]
 // and B[k 8(l/32)..+7][col l%32] and receives D[row 8(i/4) + 4(l/32) + i%4][col l%32]
@@ -170,9 +184,10 @@ __global__ __launch_bounds__(256) void k_conv3_fwd32(int nx, int ny, int nz, con
                                                      int mode, int cout_real, __bf16* __restrict__ Y,
                                                      float* __restrict__ Yf, float* __restrict__ pre) {
   constexpr int NSTEP = ConvSteps<CIN>::NSTEP, NT = COUT / 32, NK = 2 * NSTEP;   // 16-wide K steps
-  constexpr int GX = CT2X + 2, GY = CT2Y + 2, GZ = CT2Z + 2;
-  __shared__ __attribute__((aligned(16))) __bf16 tile[GX * GY * GZ * CIN];
-  const int z0 = blockIdx.x * CT2Z, y0 = blockIdx.y * CT2Y, x0 = blockIdx.z * CT2X;
+  constexpr int GX = CT2X + 2, GY = CT2Y + 2, GZ = CT2Z + 2, CS = conv_cell_stride(CIN);
+  __shared__ __attribute__((aligned(16))) __bf16 tile[GX * GY * GZ * CS];
+  const TileId tb = xcd_tile();  // neighbouring tiles on one XCD (shared halo cells in its L2)
+  const int z0 = tb.x * CT2Z, y0 = tb.y * CT2Y, x0 = tb.z * CT2X;
   constexpr int nchunk = GX * GY * GZ * CIN / 8;
   const bool circ = (mode & 4) != 0;  // periodic x and y (flax padding 'CIRCULAR' there), zero z
   for (int c = threadIdx.x; c < nchunk; c += 256) {
@@ -189,7 +204,7 @@ __global__ __launch_bounds__(256) void k_conv3_fwd32(int nx, int ny, int nz, con
     } else {
       for (int j = 0; j < 8; ++j) v[j] = (__bf16)0.0f;
     }
-    *(bf16x8*)(tile + cell * CIN + 8 * g) = v;
+    *(bf16x8*)(tile + cell * CS + 8 * g) = v;
   }
   __syncthreads();
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, m = l & 31, h = l >> 5;
@@ -203,7 +218,7 @@ __global__ __launch_bounds__(256) void k_conv3_fwd32(int nx, int ny, int nz, con
   // fully unrolled: the tap of a K step does not depend on the lane (a step of 16 k is
   // one tap's channels 16 s' .. 16 s' + 15), so every tap offset is a constant and only
   // the channel offset 8 (l / 32) varies by lane
-  const __bf16* tw = tile + ((w + 1) * GY + 1) * GZ * CIN + (((m >> 4) * GZ + (m & 15) + 1) * CIN) + 8 * h;
+  const __bf16* tw = tile + ((w + 1) * GY + 1) * GZ * CS + (((m >> 4) * GZ + (m & 15) + 1) * CS) + 8 * h;
   const __bf16* wl = Wp + (long)m * (NSTEP * 32) + 8 * h;
 #pragma unroll
   for (int s = 0; s < NK; ++s) {
@@ -218,7 +233,7 @@ __global__ __launch_bounds__(256) void k_conv3_fwd32(int nx, int ny, int nz, con
 #pragma unroll
     for (int mt = 0; mt < CT2Y / 2; ++mt) {
       // cell ((w + 1 + dx), (2 mt + m/16 + 1 + dy), (m%16 + 1 + dz)), channel ch0 + 8 h
-      const int off = ((dx * GY + 2 * mt + dy) * GZ + dz) * CIN + ch0;
+      const int off = ((dx * GY + 2 * mt + dy) * GZ + dz) * CS + ch0;
       const bf16x8 a = *(const bf16x8*)(tw + off);
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[nt], acc[mt][nt], 0, 0, 0);
@@ -249,6 +264,184 @@ __global__ __launch_bounds__(256) void k_conv3_fwd32(int nx, int ny, int nz, con
       }
     }
   }
+}
+
+// Persistent form for CIN <= 32 and 32 output channels (the SGS net's 32-channel layers).
+// Workgroup b owns the contiguous tile range [b T / G, (b+1) T / G) (z tiles fastest, then
+// y, then x): consecutive tiles of one workgroup share halo cells in its L2.  The weight
+// fragments of every K step stay in registers for the whole launch (NK x 4 VGPRs: 216 for
+// 32 input channels, so one wave per SIMD), the halo of tile t+1 is fetched into registers
+// while tile t's MFMAs and epilogue run, and the K loop reads only LDS.  MODE: bit 0 GELU
+// after the bias, bit 1 FP32 output Yf with cout_real channels (else BF16 Y), bit 2
+// periodic x and y.  Same results as k_conv3_fwd32.
+template <int CIN, int MODE>
+__global__ __launch_bounds__(256) void k_conv3_fwd32p(int nx, int ny, int nz, const __bf16* __restrict__ X,
+                                                      const __bf16* __restrict__ Wp, const float* __restrict__ bias,
+                                                      int cout_real, __bf16* __restrict__ Y, float* __restrict__ Yf,
+                                                      float* __restrict__ pre) {
+  constexpr int COUT = 32;
+  static_assert(CIN <= 32, "persistent form: the weights of CIN <= 32 fit the register file");
+  constexpr int NSTEP = ConvSteps<CIN>::NSTEP, NK = 2 * NSTEP;   // 16-wide K steps
+  constexpr int GX = CT2X + 2, GY = CT2Y + 2, GZ = CT2Z + 2, CS = conv_cell_stride(CIN);
+  constexpr int NCH = CIN / 8, NTOT = GX * GY * GZ * NCH, NPF = (NTOT + 255) / 256;  // 16-byte chunks
+  constexpr bool GELU = (MODE & 1) != 0, F32 = (MODE & 2) != 0, CIRC = (MODE & 4) != 0;
+  // LDS row (one z run of GZ cells) stride RS elements, padded to 256 bytes modulo 512: an
+  // operand read takes two z runs of 16 cells in adjacent rows (lanes 0-15, 16-31), whose
+  // 16-byte slots then fall on the two complementary halves of the 512-byte bank row
+  constexpr int RS = (GZ * CS * 2 + (((256 - GZ * CS * 2) % 512) + 512) % 512) / 2;
+  static_assert((RS * 2) % 512 == 256, "row stride");
+  __shared__ __attribute__((aligned(16))) __bf16 tile[GX * GY * RS];
+  const int ntz = (nz + CT2Z - 1) / CT2Z, nty = (ny + CT2Y - 1) / CT2Y, ntx = (nx + CT2X - 1) / CT2X;
+  const int ntiles = ntz * nty * ntx;
+  const int t_begin = (int)((long)blockIdx.x * ntiles / gridDim.x);
+  const int t_end = (int)((long)(blockIdx.x + 1) * ntiles / gridDim.x);
+  if (t_begin >= t_end) return;  // uniform over the workgroup
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, m = l & 31, h = l >> 5;
+  const float bn = bias[m];
+  bf16x8 wr[NK];
+  {
+    const __bf16* wl = Wp + (long)m * (NSTEP * 32) + 8 * h;
+#pragma unroll
+    for (int s = 0; s < NK; ++s) wr[s] = *(const bf16x8*)(wl + s * 16);
+  }
+  bf16x8 pf[NPF];
+  // thread t loads chunk t % NCH of halo cells t / NCH + STEP j; the cell coordinates
+  // advance by a constant per j (no divisions per tile)
+  constexpr int STEP = 256 / NCH, SZ = STEP % GZ, SY = (STEP / GZ) % GY, SX = STEP / (GZ * GY);
+  const int g = threadIdx.x % NCH, q0 = threadIdx.x / NCH;
+  const int hz0 = q0 % GZ, hy0 = (q0 / GZ) % GY, hx0 = q0 / (GZ * GY);
+  auto fetch = [&](int t) {
+    const int z0 = (t % ntz) * CT2Z, y0 = ((t / ntz) % nty) * CT2Y, x0 = (t / (ntz * nty)) * CT2X;
+    int hz = hz0, hy = hy0, hx = hx0;
+#pragma unroll
+    for (int j = 0; j < NPF; ++j) {
+      const int c = threadIdx.x + 256 * j;
+      bf16x8 v;
+      for (int e = 0; e < 8; ++e) v[e] = (__bf16)0.0f;
+      if (c < NTOT) {
+        int gx = x0 + hx - 1, gy = y0 + hy - 1;
+        const int gz = z0 + hz - 1;
+        if (CIRC) {
+          gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+          gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+        }
+        if ((unsigned)gx < (unsigned)nx && (unsigned)gy < (unsigned)ny && (unsigned)gz < (unsigned)nz)
+          v = *(const bf16x8*)(X + (((long)gx * ny + gy) * nz + gz) * CIN + 8 * g);
+      }
+      pf[j] = v;
+      hz += SZ;
+      const int cz = hz >= GZ;
+      hz -= cz ? GZ : 0;
+      hy += SY + cz;
+      const int cy = hy >= GY;
+      hy -= cy ? GY : 0;
+      hx += SX + cy;
+    }
+  };
+  // LDS element offset of this thread's chunk j: cell (hx, hy, hz) at (hx GY + hy) RS + hz CS
+  auto lds_off = [&](int j) {
+    const int cell = q0 + STEP * j;
+    return ((cell / GZ) * RS + (cell % GZ) * CS) + 8 * g;   // constant-folded per j
+  };
+  fetch(t_begin);
+  // LDS operand of K step s for accumulator block mt: cell ((w + 1 + dx), (2 mt + m/16 +
+  // 1 + dy), (m%16 + 1 + dz)), channel ch0 + 8 h; the tap of a step is lane-independent
+  const __bf16* tw = tile + ((w + 1) * GY + 1 + (m >> 4)) * RS + ((m & 15) + 1) * CS + 8 * h;
+  auto a_at = [&](int s, int mt) {
+    int tap, ch0;
+    conv_step_k(CIN, (s * 16) / 32, (s * 16) % 32, tap, ch0);
+    if (tap > 26) tap = 26;          // padded K: weights are zero there
+    const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+    return *(const bf16x8*)(tw + (dx * GY + 2 * mt + dy) * RS + dz * CS + ch0);
+  };
+  constexpr int AD = 2;   // LDS operands fetched AD steps ahead
+  for (int t = t_begin; t < t_end; ++t) {
+    __syncthreads();  // every wave has finished reading the previous tile
+#pragma unroll
+    for (int j = 0; j < NPF; ++j) {
+      const int c = threadIdx.x + 256 * j;
+      if (c < NTOT) *(bf16x8*)(tile + lds_off(j)) = pf[j];
+    }
+    __syncthreads();
+    if (t + 1 < t_end) fetch(t + 1);  // in flight during this tile's MFMAs and epilogue
+    f32x16 acc[CT2Y / 2];
+#pragma unroll
+    for (int a = 0; a < CT2Y / 2; ++a)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[a][i] = 0.f;
+    bf16x8 aq[AD][CT2Y / 2];
+#pragma unroll
+    for (int s = 0; s < AD; ++s)
+#pragma unroll
+      for (int mt = 0; mt < CT2Y / 2; ++mt) aq[s][mt] = a_at(s < NK ? s : NK - 1, mt);
+#pragma unroll
+    for (int s = 0; s < NK; ++s) {
+      bf16x8 a[CT2Y / 2];
+#pragma unroll
+      for (int mt = 0; mt < CT2Y / 2; ++mt) a[mt] = aq[s % AD][mt];
+      if (s + AD < NK) {
+#pragma unroll
+        for (int mt = 0; mt < CT2Y / 2; ++mt) aq[s % AD][mt] = a_at(s + AD, mt);
+      }
+#pragma unroll
+      for (int mt = 0; mt < CT2Y / 2; ++mt)
+        acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[mt], wr[s], acc[mt], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);  // keep the LDS prefetch distance written here
+    }
+    // epilogue: accumulator i of block mt is cell (x0 + w, y0 + 2 mt + i / 8, z0 + 4 h +
+    // zo), zo = 8 ((i / 4) % 2) + i % 4, output channel m; each (x, y) row is addressed
+    // from one base with constant offsets
+    const int z0 = (t % ntz) * CT2Z, y0 = ((t / ntz) % nty) * CT2Y, x0 = (t / (ntz * nty)) * CT2X;
+    const int xc = x0 + w;
+    const bool zfull = z0 + CT2Z <= nz;
+    if (xc < nx) {
+#pragma unroll
+      for (int mt = 0; mt < CT2Y / 2; ++mt) {
+#pragma unroll
+        for (int yq = 0; yq < 2; ++yq) {
+          const int yc = y0 + 2 * mt + yq;
+          if (yc >= ny) continue;  // uniform over the wave
+          const long c0 = ((long)xc * ny + yc) * nz + z0 + 4 * h;
+          float* pr = pre ? pre + c0 * COUT + m : nullptr;
+          auto out = [&](int j) {
+            const int zo = 8 * (j / 4) + (j % 4);
+            float v = acc[mt][8 * yq + j] + bn;
+            if (pr) pr[zo * COUT] = v;
+            if (GELU) v = gelu_t(v);
+            if (F32) {
+              if (m < cout_real) Yf[(c0 + zo) * cout_real + m] = v;
+            } else {
+              Y[(c0 + zo) * COUT + m] = (__bf16)v;
+            }
+          };
+          if (zfull) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) out(j);
+          } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+              if (z0 + 4 * h + 8 * (j / 4) + (j % 4) < nz) out(j);
+          }
+        }
+      }
+    }
+  }
+}
+
+// persistent grid of k_conv3_fwd32p: as many workgroups as the device holds at once
+// (occupancy from the runtime), at most one per tile
+template <int CIN, int MODE>
+inline unsigned conv_fwd32p_grid(int nx, int ny, int nz) {
+  static int resident = 0;
+  if (resident == 0) {
+    int dev = 0, ncu = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_conv3_fwd32p<CIN, MODE>, 256, 0);
+    resident = (ncu > 0 ? ncu : 256) * (per > 0 ? per : 1);
+  }
+  const long ntiles = (long)((nz + CT2Z - 1) / CT2Z) * ((ny + CT2Y - 1) / CT2Y) * ((nx + CT2X - 1) / CT2X);
+  return (unsigned)(ntiles < resident ? ntiles : resident);
 }
 
 // Weight gradient: dW[tap][c][n] += sum_p X[p + o(tap)][c] * G[p][n] over the cells of
