@@ -1083,6 +1083,17 @@ void launch_fwd(int nx, int ny, int nz, const void* x, const void* wp, const flo
                        (const __bf16*)x, (const __bf16*)wp, bias, mode, cout_real, (__bf16*)y, yf, pre);
     return;
   }
+  if (CI <= 32 && CO == 16) {   // persistent, weights in registers
+    constexpr int CIP = CI <= 32 ? CI : 32;
+#define LEX_FWDP(MD)                                                                                   \
+  if ((mode & 7) == MD) {                                                                              \
+    hipLaunchKernelGGL((k_conv3_fwdp<CIP, MD>), dim3(conv_fwdp_grid<CIP, MD>(nx, ny, nz)), dim3(256), 0, st, \
+                       nx, ny, nz, (const __bf16*)x, (const __bf16*)wp, bias, cout_real, (__bf16*)y, yf, pre); \
+    return;                                                                                            \
+  }
+    LEX_FWDP(0) LEX_FWDP(1) LEX_FWDP(2) LEX_FWDP(3) LEX_FWDP(4) LEX_FWDP(5) LEX_FWDP(6) LEX_FWDP(7)
+#undef LEX_FWDP
+  }
   dim3 grid((nz + CTZ - 1) / CTZ, (ny + CTY - 1) / CTY, (nx + CTX - 1) / CTX);
   hipLaunchKernelGGL((k_conv3_fwd<CI, CO>), grid, dim3(256), 0, st, nx, ny, nz, (const __bf16*)x, (const __bf16*)wp,
                      bias, mode, cout_real, (__bf16*)y, yf, pre);

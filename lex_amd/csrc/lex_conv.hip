@@ -275,7 +275,7 @@ __global__ __launch_bounds__(256) void k_conv3_fwd32(int nx, int ny, int nz, con
 // after the bias, bit 1 FP32 output Yf with cout_real channels (else BF16 Y), bit 2
 // periodic x and y.  Same results as k_conv3_fwd32.
 template <int CIN, int MODE>
-__global__ __launch_bounds__(256) void k_conv3_fwd32p(int nx, int ny, int nz, const __bf16* __restrict__ X,
+__global__ __launch_bounds__(256, CIN <= 16 ? 2 : 1) void k_conv3_fwd32p(int nx, int ny, int nz, const __bf16* __restrict__ X,
                                                       const __bf16* __restrict__ Wp, const float* __restrict__ bias,
                                                       int cout_real, __bf16* __restrict__ Y, float* __restrict__ Yf,
                                                       float* __restrict__ pre) {
@@ -346,13 +346,15 @@ __global__ __launch_bounds__(256) void k_conv3_fwd32p(int nx, int ny, int nz, co
   fetch(t_begin);
   // LDS operand of K step s for accumulator block mt: cell ((w + 1 + dx), (2 mt + m/16 +
   // 1 + dy), (m%16 + 1 + dz)), channel ch0 + 8 h; the tap of a step is lane-independent
-  const __bf16* tw = tile + ((w + 1) * GY + 1 + (m >> 4)) * RS + ((m & 15) + 1) * CS + 8 * h;
+  // base at the tap (-1, -1, -1) corner, so every offset below is a non-negative constant
+  // (an LDS instruction's immediate offset is unsigned)
+  const __bf16* tw = tile + (w * GY + (m >> 4)) * RS + (m & 15) * CS + 8 * h;
   auto a_at = [&](int s, int mt) {
     int tap, ch0;
     conv_step_k(CIN, (s * 16) / 32, (s * 16) % 32, tap, ch0);
     if (tap > 26) tap = 26;          // padded K: weights are zero there
-    const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
-    return *(const bf16x8*)(tw + (dx * GY + 2 * mt + dy) * RS + dz * CS + ch0);
+    const int ox = tap / 9, oy = (tap / 3) % 3, oz = tap % 3;
+    return *(const bf16x8*)(tw + (ox * GY + 2 * mt + oy) * RS + oz * CS + ch0);
   };
   constexpr int AD = 2;   // LDS operands fetched AD steps ahead
   for (int t = t_begin; t < t_end; ++t) {
@@ -426,6 +428,147 @@ __global__ __launch_bounds__(256) void k_conv3_fwd32p(int nx, int ny, int nz, co
       }
     }
   }
+}
+
+// Persistent form of k_conv3_fwd for CIN <= 32 and 16 output channels (the net's output
+// layer and the data gradient of its input layer), organised as k_conv3_fwd32p: a
+// contiguous tile range per workgroup, every K step's weight fragment in registers
+// (NSTEP x 4 VGPRs), the next tile's halo fetched during the current tile's MFMAs and
+// epilogue.  Same MFMA layout and results as k_conv3_fwd.
+template <int CIN, int MODE>
+__global__ __launch_bounds__(256, 2) void k_conv3_fwdp(int nx, int ny, int nz, const __bf16* __restrict__ X,
+                                                    const __bf16* __restrict__ Wp, const float* __restrict__ bias,
+                                                    int cout_real, __bf16* __restrict__ Y, float* __restrict__ Yf,
+                                                    float* __restrict__ pre) {
+  constexpr int COUT = 16;
+  static_assert(CIN <= 32, "persistent form: the weights of CIN <= 32 fit the register file");
+  constexpr int NSTEP = ConvSteps<CIN>::NSTEP, CS = conv_cell_stride(CIN);
+  constexpr int NCH = CIN / 8, NTOT = HX * HY * HZ * NCH, NPF = (NTOT + 255) / 256;  // 16-byte chunks
+  constexpr bool GELU = (MODE & 1) != 0, F32 = (MODE & 2) != 0, CIRC = (MODE & 4) != 0;
+  __shared__ __attribute__((aligned(16))) __bf16 tile[HX * HY * HZ * CS];
+  const int ntz = (nz + CTZ - 1) / CTZ, nty = (ny + CTY - 1) / CTY, ntx = (nx + CTX - 1) / CTX;
+  const int ntiles = ntz * nty * ntx;
+  const int t_begin = (int)((long)blockIdx.x * ntiles / gridDim.x);
+  const int t_end = (int)((long)(blockIdx.x + 1) * ntiles / gridDim.x);
+  if (t_begin >= t_end) return;  // uniform over the workgroup
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, m = l & 15, g = l >> 4;
+  const int kl = 8 * g;  // this lane's first k within a step
+  const float bn = bias[m];
+  bf16x8 wr[NSTEP];
+#pragma unroll
+  for (int s = 0; s < NSTEP; ++s) wr[s] = *(const bf16x8*)(Wp + (long)m * (NSTEP * 32) + s * 32 + kl);
+  constexpr int STEP = 256 / NCH, SZ = STEP % HZ, SY = (STEP / HZ) % HY, SX = STEP / (HZ * HY);
+  const int gc = threadIdx.x % NCH, q0 = threadIdx.x / NCH;
+  const int hz0 = q0 % HZ, hy0 = (q0 / HZ) % HY, hx0 = q0 / (HZ * HY);
+  bf16x8 pf[NPF];
+  auto fetch = [&](int t) {
+    const int z0 = (t % ntz) * CTZ, y0 = ((t / ntz) % nty) * CTY, x0 = (t / (ntz * nty)) * CTX;
+    int hz = hz0, hy = hy0, hx = hx0;
+#pragma unroll
+    for (int j = 0; j < NPF; ++j) {
+      const int c = threadIdx.x + 256 * j;
+      bf16x8 v;
+      for (int e = 0; e < 8; ++e) v[e] = (__bf16)0.0f;
+      if (c < NTOT) {
+        int gx = x0 + hx - 1, gy = y0 + hy - 1;
+        const int gz = z0 + hz - 1;
+        if (CIRC) {
+          gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+          gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+        }
+        if ((unsigned)gx < (unsigned)nx && (unsigned)gy < (unsigned)ny && (unsigned)gz < (unsigned)nz)
+          v = *(const bf16x8*)(X + (((long)gx * ny + gy) * nz + gz) * CIN + 8 * gc);
+      }
+      pf[j] = v;
+      hz += SZ;
+      const int cz = hz >= HZ;
+      hz -= cz ? HZ : 0;
+      hy += SY + cz;
+      const int cy = hy >= HY;
+      hy -= cy ? HY : 0;
+      hx += SX + cy;
+    }
+  };
+  fetch(t_begin);
+  // LDS operand of step s for y row mt: cell (w + 1 + dx, mt + 1 + dy, m + 1 + dz), channel
+  // ch0 (tap and channel of the lane's k group, conv_step_k)
+  // base at the tap (-1, -1, -1) corner: non-negative constant offsets when the tap of a
+  // step is lane-independent (CIN = 32)
+  const __bf16* tc = tile + ((w * HY) * HZ + m) * CS;
+  auto a_at = [&](int s, int mt) {
+    int tap, ch0;
+    conv_step_k(CIN, s, kl, tap, ch0);
+    if (tap > 26) tap = 26;              // padded K step: weights are zero there
+    const int ox = tap / 9, oy = (tap / 3) % 3, oz = tap % 3;
+    return *(const bf16x8*)(tc + ((ox * HY + mt + oy) * HZ + oz) * CS + ch0);
+  };
+  for (int t = t_begin; t < t_end; ++t) {
+    __syncthreads();  // every wave has finished reading the previous tile
+#pragma unroll
+    for (int j = 0; j < NPF; ++j) {
+      const int c = threadIdx.x + 256 * j;
+      if (c < NTOT) *(bf16x8*)(tile + (q0 + STEP * j) * CS + 8 * gc) = pf[j];
+    }
+    __syncthreads();
+    if (t + 1 < t_end) fetch(t + 1);  // in flight during this tile's MFMAs and epilogue
+    f32x4 acc[CTY];
+#pragma unroll
+    for (int a = 0; a < CTY; ++a) acc[a] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    bf16x8 an[CTY];
+#pragma unroll
+    for (int mt = 0; mt < CTY; ++mt) an[mt] = a_at(0, mt);
+#pragma unroll
+    for (int s = 0; s < NSTEP; ++s) {
+      bf16x8 a[CTY];
+#pragma unroll
+      for (int mt = 0; mt < CTY; ++mt) a[mt] = an[mt];
+      if (s + 1 < NSTEP) {
+#pragma unroll
+        for (int mt = 0; mt < CTY; ++mt) an[mt] = a_at(s + 1, mt);
+      }
+#pragma unroll
+      for (int mt = 0; mt < CTY; ++mt) acc[mt] = mfma16(a[mt], wr[s], acc[mt]);
+      __builtin_amdgcn_sched_barrier(0);  // keep the LDS prefetch distance written here
+    }
+    // epilogue: accumulator i of row mt is cell (x0 + w, y0 + mt, z0 + 4 g + i), channel m
+    const int z0 = (t % ntz) * CTZ, y0 = ((t / ntz) % nty) * CTY, x0 = (t / (ntz * nty)) * CTX;
+    const int xc = x0 + w;
+    if (xc < nx) {
+#pragma unroll
+      for (int mt = 0; mt < CTY; ++mt) {
+        const int yc = y0 + mt;
+        if (yc >= ny) continue;  // uniform over the wave
+        const long c0 = ((long)xc * ny + yc) * nz + z0 + 4 * g;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (z0 + 4 * g + i < nz) {
+            float v = acc[mt][i] + bn;
+            if (pre) pre[(c0 + i) * COUT + m] = v;
+            if (GELU) v = gelu_t(v);
+            if (F32) {
+              if (m < cout_real) Yf[(c0 + i) * cout_real + m] = v;
+            } else {
+              Y[(c0 + i) * COUT + m] = (__bf16)v;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int CIN, int MODE>
+inline unsigned conv_fwdp_grid(int nx, int ny, int nz) {
+  static int resident = 0;
+  if (resident == 0) {
+    int dev = 0, ncu = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_conv3_fwdp<CIN, MODE>, 256, 0);
+    resident = (ncu > 0 ? ncu : 256) * (per > 0 ? per : 1);
+  }
+  const long ntiles = (long)((nz + CTZ - 1) / CTZ) * ((ny + CTY - 1) / CTY) * ((nx + CTX - 1) / CTX);
+  return (unsigned)(ntiles < resident ? ntiles : resident);
 }
 
 // persistent grid of k_conv3_fwd32p: as many workgroups as the device holds at once
