@@ -28,6 +28,11 @@ struct SlotPlanes {
   float* p;          // plane s starts at p + s * plane; null: this kernel contributes nothing
   long plane;        // elements per plane
   int gx, gy, gz;    // launch box: a in [0, gx), b in [0, gy), k in [0, gz); k fastest
+  int zw;
+  // zw = 1: the column stencil's contributions (slots 12..17 of the scalar, u and v reverses)
+  // stay in the wave: the column is one 64-lane wave (gz <= 64, blocks of 64 lanes), whose
+  // lanes add the slots that reach their own level with shuffles in slot order and update
+  // the adjoint directly (one writer per level); the gathers skip those slots
   __device__ __forceinline__ long at(int a, int b, int k) const { return ((long)a * gy + b) * gz + k; }
   __device__ __forceinline__ void put(int s, long i, float v) const { p[s * plane + i] = v; }
   // slot s of box point (a, b, k), zero outside the box
@@ -37,19 +42,37 @@ struct SlotPlanes {
   }
 };
 
+// sum over t of slot t (o[t]) of the lane k + 3 - t, for the lanes (levels) of [0, nz):
+// the column-stencil contributions that reach this lane's level, in slot order
+__device__ __forceinline__ float zw_sum(const float* o, int k, int nz) {
+  float zs = 0.f;
+#pragma unroll
+  for (int t = 0; t < 6; ++t) {
+    const int src = k + 3 - t;
+    const float v = __shfl(o[t], src & 63, 64);
+    if (src >= 0 && src < nz) zs += v;
+  }
+  return zs;
+}
+
 inline SlotPlanes slot_planes(float* p, int gx, int gy, int gz) {
   SlotPlanes sp;
   sp.p = p;
   sp.gx = gx; sp.gy = gy; sp.gz = gz;
   sp.plane = (((long)gx * gy * gz) + 63) / 64 * 64;
+  sp.zw = 0;
   return sp;
 }
 
 // launch boxes and slot counts of the advection reverses
 constexpr int kSlotsScalar = 18, kSlotsMom = 24;
-inline SlotPlanes slots_scalar(const Dims& d, float* p) { return slot_planes(p, d.nx + 1, d.ny + 1, d.nz); }
-inline SlotPlanes slots_u(const Dims& d, float* p) { return slot_planes(p, d.nx + 2, d.ny + 1, d.nz); }
-inline SlotPlanes slots_v(const Dims& d, float* p) { return slot_planes(p, d.nx + 1, d.ny + 2, d.nz); }
+inline SlotPlanes with_zw(SlotPlanes sp) {
+  sp.zw = sp.gz <= 64 ? 1 : 0;
+  return sp;
+}
+inline SlotPlanes slots_scalar(const Dims& d, float* p) { return with_zw(slot_planes(p, d.nx + 1, d.ny + 1, d.nz)); }
+inline SlotPlanes slots_u(const Dims& d, float* p) { return with_zw(slot_planes(p, d.nx + 2, d.ny + 1, d.nz)); }
+inline SlotPlanes slots_v(const Dims& d, float* p) { return with_zw(slot_planes(p, d.nx + 1, d.ny + 2, d.nz)); }
 inline SlotPlanes slots_w(const Dims& d, float* p) { return slot_planes(p, d.nx + 1, d.ny + 1, d.nz + 1); }
 // bytes of the scalar reverse's planes, and of the three momentum reverses' planes together
 inline size_t slot_bytes_scalar(const Dims& d) { return (size_t)kSlotsScalar * slots_scalar(d, nullptr).plane * 4; }
@@ -248,12 +271,15 @@ __device__ __forceinline__ float mu_cell(const Dims& d, const float* __restrict_
 
 __global__ void k_adv_scalar_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ s,
                                  const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
-                                 const float* __restrict__ lam, float weps, SlotPlanes sp,
+                                 const float* __restrict__ lam, float weps, SlotPlanes sp, float* __restrict__ ls,
                                  float* __restrict__ lu, float* __restrict__ lv, float* __restrict__ lw) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   int i = blockIdx.z;
-  if (k >= d.nz || j > d.ny) return;
+  if (j > d.ny) return;  // whole wave
+  const bool live = k < d.nz;
+  if (!live && !sp.zw) return;
+  if (!live) k = d.nz - 1;  // a lane past the column only takes part in the shuffles
   const int I = i + 3, J = j + 3, K = k + 1;
   const bool cell = i < d.nx && j < d.ny;
   const bool fx = j < d.ny, fy = i < d.nx, fz = cell && k > 0;
@@ -297,7 +323,7 @@ __global__ void k_adv_scalar_vjp(Dims d, const float* __restrict__ rc, const flo
     const float cf = gF * massx;
 #pragma unroll
     for (int t = 0; t < 6; ++t) ox[t] = cf * gx6[t];
-    if (g != 0.f) lu[au] += r * g;
+    if (live && g != 0.f) lu[au] += r * g;
   }
   if (fy) {  // y face J
     const float gF = (mc - mly) * d.idy;
@@ -305,7 +331,7 @@ __global__ void k_adv_scalar_vjp(Dims d, const float* __restrict__ rc, const flo
     const float cf = gF * massy;
 #pragma unroll
     for (int t = 0; t < 6; ++t) oy[t] = cf * gy6[t];
-    if (g != 0.f) lv[av] += r * g;
+    if (live && g != 0.f) lv[av] += r * g;
   }
   if (cell) {
     float g;
@@ -318,19 +344,24 @@ __global__ void k_adv_scalar_vjp(Dims d, const float* __restrict__ rc, const flo
     } else {
       g = -mc * sc * d.idz;
     }
-    if (g != 0.f) lw[wb + K] += r8 * g;
-    if (k == d.nz - 1) {  // lid level: div(rho0 u) of the top cell only
+    if (live && g != 0.f) lw[wb + K] += r8 * g;
+    if (live && k == d.nz - 1) {  // lid level: div(rho0 u) of the top cell only
       const float gt = mc * sc * (0.5f * (r + rcp)) * d.idz;
       if (gt != 0.f) lw[wb + K + 1] += gt;
     }
     ox[3] += mc * fd;
   }
+  if (sp.zw) {
+    const float zs = zw_sum(oz, k, d.nz);   // every lane of the wave takes part
+    if (live && cell) ls[c] += zs;
+  }
+  if (!live) return;
   const long p = sp.at(i, j, k);
 #pragma unroll
   for (int t = 0; t < 6; ++t) {
     sp.put(t, p, ox[t]);
     sp.put(6 + t, p, oy[t]);
-    sp.put(12 + t, p, oz[t]);
+    if (!sp.zw) sp.put(12 + t, p, oz[t]);
   }
 }
 
@@ -345,8 +376,10 @@ __global__ void k_gather_scalar(Dims d, SlotPlanes sp, float* __restrict__ ls) {
   for (int t = 0; t < 6; ++t) a += sp.get(t, I - t, J - 3, K - 1);
 #pragma unroll
   for (int t = 0; t < 6; ++t) a += sp.get(6 + t, I - 3, J - t, K - 1);
+  if (!sp.zw) {
 #pragma unroll
-  for (int t = 0; t < 6; ++t) a += sp.get(12 + t, I - 3, J - 3, K + 2 - t);
+    for (int t = 0; t < 6; ++t) a += sp.get(12 + t, I - 3, J - 3, K + 2 - t);
+  }
   ls[ic(d, I, J, K)] += a;
 }
 
@@ -376,11 +409,14 @@ __device__ __forceinline__ float mu_wlev(const Dims& d, const float* __restrict_
 // point's own term (a <= nx, b < ny), div(rho0 u) of cell (a+2, b+3, k+1) (b < ny).
 __global__ void k_adv_u_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
                             const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ lam,
-                            SlotPlanes sp) {
+                            SlotPlanes sp, float* __restrict__ lu) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   int b = blockIdx.y * blockDim.y + threadIdx.y;
   int a = blockIdx.z;
-  if (k >= d.nz || b > d.ny) return;
+  if (b > d.ny) return;  // whole wave
+  const bool live = k < d.nz;
+  if (!live && !sp.zw) return;
+  if (!live) k = d.nz - 1;  // a lane past the column only takes part in the shuffles
   const float weps = 1.0e-16f;
   const int J = b + 3, K = k + 1;
   const float r = rc[K];
@@ -463,9 +499,15 @@ __global__ void k_adv_u_vjp(Dims d, const float* __restrict__ rc, const float* _
       o[15] += mc * 0.5f * (fd_at(d, rc, u, v, w, I - 1, J, K) + fd_at(d, rc, u, v, w, I, J, K));
     }
   }
+  if (sp.zw) {
+    const float zs = zw_sum(o + 12, k, d.nz);   // every lane of the wave takes part
+    if (live && upt) lu[iu(d, a + 3, J, K)] += zs;
+  }
+  if (!live) return;
   const long p = sp.at(a, b, k);
 #pragma unroll
-  for (int q = 0; q < kSlotsMom; ++q) sp.put(q, p, o[q]);
+  for (int q = 0; q < kSlotsMom; ++q)
+    if (q < 12 || q >= 18 || !sp.zw) sp.put(q, p, o[q]);
 }
 
 // reverse of advection_v (advection.py:77-114), lam: (nx, ny+1, nz).  Thread (a, b, k),
@@ -474,11 +516,14 @@ __global__ void k_adv_u_vjp(Dims d, const float* __restrict__ rc, const float* _
 // div(rho0 u) of cell (a+3, b+2, k+1) (a < nx).
 __global__ void k_adv_v_vjp(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
                             const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ lam,
-                            SlotPlanes sp) {
+                            SlotPlanes sp, float* __restrict__ lv) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   int b = blockIdx.y * blockDim.y + threadIdx.y;
   int a = blockIdx.z;
-  if (k >= d.nz || b > d.ny + 1) return;
+  if (b > d.ny + 1) return;  // whole wave
+  const bool live = k < d.nz;
+  if (!live && !sp.zw) return;
+  if (!live) k = d.nz - 1;  // a lane past the column only takes part in the shuffles
   const float weps = 1.0e-16f;
   const int I = a + 3, K = k + 1;
   const float r = rc[K];
@@ -562,9 +607,15 @@ __global__ void k_adv_v_vjp(Dims d, const float* __restrict__ rc, const float* _
       o[15] += mc * 0.5f * (fd_at(d, rc, u, v, w, I, J - 1, K) + fd_at(d, rc, u, v, w, I, J, K));
     }
   }
+  if (sp.zw) {
+    const float zs = zw_sum(o + 12, k, d.nz);   // every lane of the wave takes part
+    if (live && vpt) lv[iv(d, I, b + 3, K)] += zs;
+  }
+  if (!live) return;
   const long p = sp.at(a, b, k);
 #pragma unroll
-  for (int q = 0; q < kSlotsMom; ++q) sp.put(q, p, o[q]);
+  for (int q = 0; q < kSlotsMom; ++q)
+    if (q < 12 || q >= 18 || !sp.zw) sp.put(q, p, o[q]);
 }
 
 // reverse of advection_w (advection.py:117-149), lam: (nx, ny, nz+1); levels 0 and nz
@@ -675,8 +726,10 @@ __global__ void k_gather_lu(Dims d, SlotPlanes U, SlotPlanes V, SlotPlanes W, fl
   for (int t = 0; t < 6; ++t) a += U.get(t, I - t, J - 3, K - 1);
 #pragma unroll
   for (int t = 0; t < 6; ++t) a += U.get(6 + t, I - 3, J - t, K - 1);
+  if (!U.zw) {
 #pragma unroll
-  for (int t = 0; t < 6; ++t) a += U.get(12 + t, I - 3, J - 3, K + 2 - t);
+    for (int t = 0; t < 6; ++t) a += U.get(12 + t, I - 3, J - 3, K + 2 - t);
+  }
   a += V.get(18, I - 4, J - 2, K - 1);
   a += V.get(19, I - 3, J - 2, K - 1);
   a += V.get(20, I - 3, J - 3, K - 1);
@@ -696,8 +749,10 @@ __global__ void k_gather_lv(Dims d, SlotPlanes U, SlotPlanes V, SlotPlanes W, fl
   for (int t = 0; t < 6; ++t) a += V.get(t, I - t, J - 3, K - 1);
 #pragma unroll
   for (int t = 0; t < 6; ++t) a += V.get(6 + t, I - 3, J - t, K - 1);
+  if (!V.zw) {
 #pragma unroll
-  for (int t = 0; t < 6; ++t) a += V.get(12 + t, I - 3, J - 3, K + 2 - t);
+    for (int t = 0; t < 6; ++t) a += V.get(12 + t, I - 3, J - 3, K + 2 - t);
+  }
   a += U.get(18, I - 2, J - 4, K - 1);
   a += U.get(19, I - 2, J - 3, K - 1);
   a += U.get(20, I - 3, J - 3, K - 1);

@@ -324,7 +324,7 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
                          (float)dt_s, lam_in[fi], lT, turb ? lss[q] : nullptr);
       hipLaunchKernelGGL(k_adv_scalar_vjp, cx.G(d.nz, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, (const float*)src[fi],
                          (const float*)src[1], (const float*)src[2], (const float*)src[3], (const float*)lT, weps,
-                         spS, lsrc[1], lsrc[2], lsrc[3]);
+                         spS, lsrc[fi], lsrc[1], lsrc[2], lsrc[3]);
       hipLaunchKernelGGL(k_gather_scalar, cx.G(d.nz + 2, d.ny + 6, d.nx + 6), cx.bk, 0, st, d, spS, lsrc[fi]);
     }
     // acoustic substeps, last to first
@@ -364,9 +364,9 @@ int rk3_step_vjp(const Dims& d, const O& o, const B& b, float* const* X0, float*
                          (const float*)lrw, lsw);
     }
     hipLaunchKernelGGL(k_adv_u_vjp, cx.G(d.nz, d.ny + 1, d.nx + 2), cx.bk, 0, st, d, rc, (const float*)src[1],
-                       (const float*)src[2], (const float*)src[3], (const float*)lru, spU);
+                       (const float*)src[2], (const float*)src[3], (const float*)lru, spU, lsrc[1]);
     hipLaunchKernelGGL(k_adv_v_vjp, cx.G(d.nz, d.ny + 2, d.nx + 1), cx.bk, 0, st, d, rc, (const float*)src[1],
-                       (const float*)src[2], (const float*)src[3], (const float*)lrv, spV);
+                       (const float*)src[2], (const float*)src[3], (const float*)lrv, spV, lsrc[2]);
     hipLaunchKernelGGL(k_adv_w_vjp, cx.G(d.nz + 1, d.ny + 1, d.nx + 1), cx.bk, 0, st, d, rc, (const float*)src[1],
                        (const float*)src[2], (const float*)src[3], (const float*)lrw, spW);
     hipLaunchKernelGGL(k_gather_lu, cx.G(d.nz + 2, d.ny + 6, d.nx + 7), cx.bk, 0, st, d, spU, spV, spW, lsrc[1]);

@@ -702,7 +702,7 @@ int lex_adv_scalar_vjp(const lex_grid_t* g, const float* r, const float* s, cons
     return fail(-4, "%s: no device memory for the slot planes", "lex_adv_scalar_vjp");
   const SlotPlanes sp = slots_scalar(d, sc.p);
   hipLaunchKernelGGL(k_adv_scalar_vjp, grd(d.nz, d.ny + 1, d.nx + 1), blk(), 0, S(stream), d, r, s, u, v, w, lam, weps, sp,
-                     lu, lv, lw);
+                     ls, lu, lv, lw);
   hipLaunchKernelGGL(k_gather_scalar, grd(d.nz + 2, d.ny + 6, d.nx + 6), blk(), 0, S(stream), d, sp, ls);
   return check_launch("lex_adv_scalar_vjp");
 }
@@ -719,10 +719,10 @@ static int adv_mom_vjp(const char* name, int which, const Dims& d, const float* 
   SlotPlanes U = none, V = none, W = none;
   if (which == 0) {
     U = slots_u(d, sc.p);
-    hipLaunchKernelGGL(k_adv_u_vjp, grd(d.nz, d.ny + 1, d.nx + 2), blk(), 0, st, d, r, u, v, w, lam, U);
+    hipLaunchKernelGGL(k_adv_u_vjp, grd(d.nz, d.ny + 1, d.nx + 2), blk(), 0, st, d, r, u, v, w, lam, U, lu);
   } else if (which == 1) {
     V = slots_v(d, sc.p);
-    hipLaunchKernelGGL(k_adv_v_vjp, grd(d.nz, d.ny + 2, d.nx + 1), blk(), 0, st, d, r, u, v, w, lam, V);
+    hipLaunchKernelGGL(k_adv_v_vjp, grd(d.nz, d.ny + 2, d.nx + 1), blk(), 0, st, d, r, u, v, w, lam, V, lv);
   } else {
     W = slots_w(d, sc.p);
     hipLaunchKernelGGL(k_adv_w_vjp, grd(d.nz + 1, d.ny + 1, d.nx + 1), blk(), 0, st, d, r, u, v, w, lam, W);

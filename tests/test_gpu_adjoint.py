@@ -39,6 +39,19 @@ def _check(gpu, fd, tol, what):
     return err
 
 
+def _make_state(nx, ny, nz, seed):
+    import torch  # noqa: F401
+    gr, phys, base = random_state(nx, ny, nz, seed=seed)
+    p32 = tuple(f32(a) for a in phys)
+    b32 = tuple(f32(a) for a in base[:5])
+    from lex_amd import _lib
+    g = _lib.grid(nx, ny, nz, gr.dx, gr.dy, gr.dz)
+    cols = {n: to_dev(np.ascontiguousarray(b32[i][3, 3, :])) for n, i in
+            (("rho0", 1), ("theta0", 2), ("pi0", 3), ("qv0", 4))}
+    return dict(gr=gr, g=g, p=tuple(f64(a) for a in p32), b=tuple(f64(a) for a in b32), dev=tuple(to_dev(a) for a in p32),
+                cols=cols, rng=np.random.default_rng(11), shape=(nx, ny, nz))
+
+
 @pytest.fixture(scope="module")
 def st():
     import torch  # noqa: F401
@@ -105,6 +118,51 @@ def test_adv_momentum_vjp(st, which):
         vq = [np.zeros_like(a) for a in x]
         vq[q] = rng.normal(size=x[q].shape) * _scale(x[q])
         _check(_dot(grads[q], vq[q]), _fd(F, x, vq, (lam,)), 1e-4, f"adv_{which} d/d{name}")
+
+
+def test_adv_vjp_deep_column():
+    """The advection reverses on a 70-level column: deeper than one 64-lane wave, so the
+    column-stencil contributions go through slot planes and the gathers (a column of at
+    most 64 levels sums them inside its wave with shuffles, which the 12-level tests above
+    cover)."""
+    import torch
+    from lex_amd import adjoint as adj
+    sd = _make_state(8, 6, 70, seed=21)
+    nx, ny, nz = sd["shape"]
+    gr, p, b, dv, rng = sd["gr"], sd["p"], sd["b"], sd["dev"], sd["rng"]
+    rho0 = b[1]
+    z2 = np.zeros((nx, ny))
+
+    def Fs(s_, u, v, w):
+        fd = o.get_divergence(rho0, u, v, w, gr)
+        return (o.advection_scalar(rho0, s_, u, v, w, 1e-17, fd, z2, gr),)
+
+    lam = rng.normal(size=(nx, ny, nz))
+    x = (p[0], p[1], p[2], p[3])
+    outs = [torch.zeros_like(t) for t in (dv[0], dv[1], dv[2], dv[3])]
+    adj.adv_scalar_vjp(sd["g"], sd["cols"]["rho0"], dv[0], dv[1], dv[2], dv[3], to_dev(lam), 1e-17, *outs)
+    grads = [from_dev(t) for t in outs]
+    for q, name in enumerate(["s", "u", "v", "w"]):
+        vq = [np.zeros_like(a) for a in x]
+        vq[q] = rng.normal(size=x[q].shape) * _scale(x[q][3:-3, 3:-3, 1:-1] if q == 0 else x[q])
+        _check(_dot(grads[q], vq[q]), _fd(Fs, x, vq, (lam,)), 1e-4, f"deep adv_scalar d/d{name}")
+    for which in ("u", "v", "w"):
+        def F(u, v, w):
+            fd = o.get_divergence(rho0, u, v, w, gr)
+            if which == "u":
+                return (o.advection_u(rho0, u, v, w, 1e-16, fd, z2, gr),)
+            if which == "v":
+                return (o.advection_v(rho0, u, v, w, 1e-16, fd, z2, gr),)
+            return (o.advection_w(rho0, u, v, w, 1e-16, fd, gr),)
+        xm = (p[1], p[2], p[3])
+        lam = rng.normal(size=F(*xm)[0].shape)
+        outs = [torch.zeros_like(t) for t in (dv[1], dv[2], dv[3])]
+        adj.adv_momentum_vjp(which, sd["g"], sd["cols"]["rho0"], dv[1], dv[2], dv[3], to_dev(lam), *outs)
+        grads = [from_dev(t) for t in outs]
+        for q, name in enumerate(["u", "v", "w"]):
+            vq = [np.zeros_like(a) for a in xm]
+            vq[q] = rng.normal(size=xm[q].shape) * _scale(xm[q])
+            _check(_dot(grads[q], vq[q]), _fd(F, xm, vq, (lam,)), 1e-4, f"deep adv_{which} d/d{name}")
 
 
 def test_buoyancy_and_theta_rho_vjp(st):
