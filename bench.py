@@ -622,9 +622,33 @@ def sgs_net_timing(nx=96, ny=96, nz=48, reps=10):
     fb = e0.elapsed_time(e1) / reps
     flop = 2.0 * 27 * (11 * 32 + 32 * 32 + 32 * 32 + 32 * 5) * nx * ny * nz
     tf = flop / (fwd / 1e3) / 1e12
+    # the 32 -> 32 hidden layer (conv + bias + GELU, BF16 in and out) alone
+    from lex_amd import sgs_net as sn, _lib as lib
+    ncell = nx * ny * nz
+    xl = torch.randn(ncell, 32, device="cuda", generator=g).to(torch.bfloat16)
+    wl = torch.randn(27, 32, 32, device="cuda", generator=g) * 0.1
+    wp = sn._pack(wl, 0)
+    bl = torch.zeros(32, device="cuda")
+    yl = torch.empty(ncell, 32, dtype=torch.bfloat16, device="cuda")
+
+    def layer():
+        lib.call("lex_conv3d_bf16_fwd", nx, ny, nz, 32, 32, lib.ptr(xl), lib.ptr(wp), lib.ptr(bl), 1, 0,
+                 lib.ptr(yl), None, None, lib.stream())
+    for _ in range(5):
+        layer()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(50):
+        layer()
+    e1.record()
+    torch.cuda.synchronize()
+    lus = e0.elapsed_time(e1) / 50 * 1e3
+    ltf = 2.0 * 27 * 32 * 32 * ncell / (lus / 1e6) / 1e12
     return {"grid": [nx, ny, nz], "fwd_ms": fwd, "fwd_bwd_ms": fb,
             "roofline": {"bound": "mfma", "achieved": tf, "peak": 2500.0, "unit": "TFLOP/s", "frac": tf / 2500.0,
-                         "note": "useful flops of the four convolutions (unpadded channels), BF16 dense peak"}}
+                         "note": "useful flops of the four convolutions (unpadded channels), BF16 dense peak"},
+            "layer_32_32": {"us": lus, "achieved_TFLOPs": ltf, "frac": ltf / 2500.0,
+                            "note": "one 3x3x3 convolution 32 -> 32 channels + bias + GELU, 50 calls"}}
 
 
 def main():
