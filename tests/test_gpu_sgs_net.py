@@ -139,6 +139,34 @@ def test_conv_forward_64_channels_and_periodic_padding(cin, cout, circular):
     assert _rel(got, ref) < 1e-5, _rel(got, ref)
 
 
+@pytest.mark.parametrize("cin,cout,circular,out_fp32", [(32, 32, False, False), (11, 32, False, True),
+                                                        (32, 5, True, True), (32, 32, True, True)])
+def test_conv_forward_several_tiles_per_workgroup(cin, cout, circular, out_fp32):
+    """The persistent forward kernels (16 or 32 output channels, at most 32 input
+    channels) on a 70x36x40 grid: 486 output tiles of 4x4x16, more than the device's
+    resident workgroups, so workgroups run several tiles each and fetch the next tile's
+    halo while finishing the current one; the grid is ragged in x and z.  Against torch
+    conv3d in float64 on the same BF16-rounded operands (BF16 output: within its
+    rounding)."""
+    import torch
+    from lex_amd import sgs_net as sn
+    shape = (70, 36, 40)
+    g = torch.Generator().manual_seed(cin * 100 + cout + 7 * int(circular))
+    x = torch.randn(*shape, cin, generator=g)
+    W = torch.randn(27, cin, cout, generator=g) / np.sqrt(27 * cin)
+    b = torch.randn(cout, generator=g) * 0.1
+    ref = _gelu(_ref_conv(x.bfloat16().double(), W.bfloat16().double(), circular) + b.double())
+    xd = sn.to_bf16_channels_last(x.cuda())
+    if out_fp32:
+        yf, _ = sn.conv3d_fwd(xd, shape, W.cuda(), b.cuda(), gelu=1, out_fp32=True, circular=circular)
+        got = yf.cpu().double().reshape(*shape, cout)
+        assert _rel(got, ref) < 1e-5, _rel(got, ref)
+    else:
+        yb, _ = sn.conv3d_fwd(xd, shape, W.cuda(), b.cuda(), gelu=1, out_fp32=False, circular=circular)
+        got = yb.cpu().double().reshape(*shape, -1)[..., :cout]
+        assert _rel(got, ref) < 4e-3, _rel(got, ref)
+
+
 def test_zcnn_forward_vs_fp32():
     """The reference's Zcnn shape (13 layers, 64/32 channels, periodic x/y) against FP32
     torch with the same weights: the BF16 budget."""
