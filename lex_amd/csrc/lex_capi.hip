@@ -1098,12 +1098,18 @@ void launch_fwd(int nx, int ny, int nz, const void* x, const void* wp, const flo
   hipLaunchKernelGGL((k_conv3_fwd<CI, CO>), grid, dim3(256), 0, st, nx, ny, nz, (const __bf16*)x, (const __bf16*)wp,
                      bias, mode, cout_real, (__bf16*)y, yf, pre);
 }
-template <int CI, int CO>
-void launch_wgrad(int nx, int ny, int nz, const void* x, const void* g, float* dw, hipStream_t st) {
+// workgroups of k_conv3_wgrad: one per CU (the accumulators fill the register file)
+inline unsigned wgrad_blocks(int nx, int ny, int nz) {
   long nt = (long)((nx + CTX - 1) / CTX) * ((ny + CTY - 1) / CTY) * ((nz + CTZ - 1) / CTZ);
-  unsigned nb = (unsigned)(nt < 256 ? nt : 256);  // one workgroup per CU (the accumulators fill the register file)
+  return (unsigned)(nt < 256 ? nt : 256);
+}
+template <int CI, int CO>
+void launch_wgrad(int nx, int ny, int nz, const void* x, const void* g, float* dw, float* part, hipStream_t st) {
+  const unsigned nb = wgrad_blocks(nx, ny, nz);
   hipLaunchKernelGGL((k_conv3_wgrad<CI, CO>), dim3(nb), dim3(256), 0, st, nx, ny, nz, (const __bf16*)x,
-                     (const __bf16*)g, dw);
+                     (const __bf16*)g, part);
+  const int n = 27 * CI * CO;
+  hipLaunchKernelGGL(k_partial_sum, dim3((n + 255) / 256), dim3(256), 0, st, (int)nb, n, n, (const float*)part, dw);
 }
 }  // namespace
 extern "C" {
@@ -1152,14 +1158,22 @@ int lex_conv3d_bf16_bwd_weight(int nx, int ny, int nz, int cin, int cout, const 
   if (!conv_ok(nx, ny, nz, cin, cout)) return fail(-1, "lex_conv3d_bf16_bwd_weight: bad sizes (channels 1..32)%s");
   int cip = padc(cin), cop = padc(cout);
   hipStream_t st = S(stream);
-  if (cip == 16 && cop == 16) launch_wgrad<16, 16>(nx, ny, nz, x, g_pre, dw, st);
-  else if (cip == 16 && cop == 32) launch_wgrad<16, 32>(nx, ny, nz, x, g_pre, dw, st);
-  else if (cip == 32 && cop == 16) launch_wgrad<32, 16>(nx, ny, nz, x, g_pre, dw, st);
-  else launch_wgrad<32, 32>(nx, ny, nz, x, g_pre, dw, st);
+  // per-workgroup partial sums, added in workgroup order (bit-reproducible gradients):
+  // stream-ordered scratch, returned on the stream
+  const size_t nw = (size_t)wgrad_blocks(nx, ny, nz) * 27 * cip * cop, nbg = (size_t)kBgradBlocks * cop;
+  SlotScratch sc;
+  if (sc.take((nw + nbg) * sizeof(float), st) != hipSuccess)
+    return fail(-4, "%s: no device memory for the partial sums", "lex_conv3d_bf16_bwd_weight");
+  if (cip == 16 && cop == 16) launch_wgrad<16, 16>(nx, ny, nz, x, g_pre, dw, sc.p, st);
+  else if (cip == 16 && cop == 32) launch_wgrad<16, 32>(nx, ny, nz, x, g_pre, dw, sc.p, st);
+  else if (cip == 32 && cop == 16) launch_wgrad<32, 16>(nx, ny, nz, x, g_pre, dw, sc.p, st);
+  else launch_wgrad<32, 32>(nx, ny, nz, x, g_pre, dw, sc.p, st);
   if (db) {
     long ncell = (long)nx * ny * nz;
-    hipLaunchKernelGGL(k_conv3_bgrad, dim3(kBgradBlocks), dim3(256), 0, st, ncell, cop, cout, (const __bf16*)g_pre,
-                       db);
+    hipLaunchKernelGGL(k_conv3_bgrad, dim3(kBgradBlocks), dim3(256), 0, st, ncell, cop, (const __bf16*)g_pre,
+                       sc.p + nw);
+    hipLaunchKernelGGL(k_partial_sum, dim3(1), dim3(256), 0, st, (int)kBgradBlocks, cout, cop,
+                       (const float*)(sc.p + nw), db);
   }
   return check_launch("lex_conv3d_bf16_bwd_weight");
 }
@@ -1381,8 +1395,11 @@ int lex_sgs_features_vjp(const lex_grid_t* g, const float* lfeat, const float* s
     nrm.shift[c] = 0.f;
     nrm.scale[c] = scale[c];
   }
-  hipLaunchKernelGGL(k_sgs_features_vjp, grd(d.nz, d.ny, d.nx), blk(), 0, S(stream), d, lfeat, nrm, lu, lv, lw, lth,
-                     lqv);
+  const SgsFeatAdj A{d, lfeat, nrm};
+  hipStream_t st = S(stream);
+  hipLaunchKernelGGL(k_sgs_features_vjp_u, grd(d.nz + 2, d.ny + 6, d.nx + 7), blk(), 0, st, A, lu, lth, lqv);
+  hipLaunchKernelGGL(k_sgs_features_vjp_v, grd(d.nz + 2, d.ny + 7, d.nx + 6), blk(), 0, st, A, lv);
+  hipLaunchKernelGGL(k_sgs_features_vjp_w, grd(d.nz + 3, d.ny + 6, d.nx + 6), blk(), 0, st, A, lw);
   return check_launch("lex_sgs_features_vjp");
 }
 

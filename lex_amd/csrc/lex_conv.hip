@@ -589,7 +589,9 @@ inline unsigned conv_fwd32p_grid(int nx, int ny, int nz) {
 
 // Weight gradient: dW[tap][c][n] += sum_p X[p + o(tap)][c] * G[p][n] over the cells of
 // the grid (G: BF16 gradient of the pre-activation, COUT channels).  Workgroups loop
-// over output tiles and flush their FP32 partial sums with one atomic add per element.
+// over output tiles and store their FP32 partial sums (all 27 CIN COUT of them) in the
+// slot of part that belongs to the workgroup; k_wgrad_reduce then adds the slots in
+// workgroup order, so the result does not depend on the order the workgroups ran in.
 //
 // The reduction runs over cells, so an MFMA operand lane needs 8 consecutive cells of
 // one channel: both tiles are transposed into channel rows in LDS, the X rows as z-runs
@@ -623,7 +625,7 @@ __device__ __forceinline__ bf16x8 xrun_window(const __bf16* __restrict__ row, in
 
 template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, const __bf16* __restrict__ X,
-                                                     const __bf16* __restrict__ G, float* __restrict__ dW) {
+                                                     const __bf16* __restrict__ G, float* __restrict__ part) {
   static_assert(CTZ == 16 && XR == 24, "z-run layout assumes a 16-cell z tile");
   constexpr int MT = COUT / 16, NT = CIN / 16, TPW = 7;  // taps per wave: 7, 7, 7, 6
   constexpr int NCELL = CTX * CTY * CTZ;
@@ -693,7 +695,9 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, con
       }
     }
   }
-  // D[row = n][col = c]: lane holds rows 4g+i of the M-tile, column m of the N-tile
+  // D[row = n][col = c]: lane holds rows 4g+i of the M-tile, column m of the N-tile; the
+  // four waves' taps (w, w+4, ..) cover all 27, so every element of the slot is stored
+  float* dW = part + (long)blockIdx.x * (27 * CIN * COUT);
 #pragma unroll
   for (int q = 0; q < TPW; ++q) {
     const int tap = w + 4 * q;
@@ -703,19 +707,28 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, con
       for (int nt = 0; nt < NT; ++nt)
         for (int i = 0; i < 4; ++i) {
           const int n = mt * 16 + 4 * g + i, c = nt * 16 + m;
-          unsafeAtomicAdd(dW + ((long)tap * CIN + c) * COUT + n, acc[q][mt][nt][i]);
+          dW[((long)tap * CIN + c) * COUT + n] = acc[q][mt][nt][i];
         }
   }
 }
 
-// bias gradient: db[n] += sum_p G[p][n] for n < cout_real.  A thread reads 8 channels of
-// one cell per 16-byte load; partial sums are reduced across the wave, then across the
-// workgroup's 4 waves in LDS, and flushed with one atomic add per channel per workgroup
-// (all workgroups add into the same cp addresses, so their number is kept small).
+// out[e] += sum over b < nb of part[b * stride + e], in order of b (e < n)
+__global__ void k_partial_sum(int nb, int n, int stride, const float* __restrict__ part, float* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  float s = 0.f;
+  for (int b = 0; b < nb; ++b) s += part[(long)b * stride + e];
+  out[e] += s;
+}
+
+// bias gradient: the workgroup's partial sums of G[p][n] over its cells, part[b][n] for
+// n < cp (k_partial_sum adds them into db in workgroup order).  A thread reads 8 channels
+// of one cell per 16-byte load; partial sums are reduced across the wave, then across the
+// workgroup's 4 waves in LDS.
 constexpr int kBgradBlocks = 128;
 
-__global__ __launch_bounds__(256) void k_conv3_bgrad(long ncell, int cp, int cout_real,
-                                                     const __bf16* __restrict__ G, float* __restrict__ db) {
+__global__ __launch_bounds__(256) void k_conv3_bgrad(long ncell, int cp, const __bf16* __restrict__ G,
+                                                     float* __restrict__ part) {
   __shared__ float red[4][32];
   const int nch = cp / 8;  // 16-byte chunks per cell (2 or 4; divides 64)
   const int cg = threadIdx.x % nch;
@@ -738,7 +751,7 @@ __global__ __launch_bounds__(256) void k_conv3_bgrad(long ncell, int cp, int cou
   }
   __syncthreads();
   const int n = threadIdx.x;
-  if (n < cp && n < cout_real) unsafeAtomicAdd(db + n, red[0][n] + red[1][n] + red[2][n] + red[3][n]);
+  if (n < cp) part[(long)blockIdx.x * cp + n] = red[0][n] + red[1][n] + red[2][n] + red[3][n];
 }
 
 // g_pre = g_post * gelu'(pre) (or g_post when gelu is 0) -> BF16, cp channels (the

@@ -124,74 +124,90 @@ __global__ void k_sgs_apply_vjp(Dims d, const float* __restrict__ lsu, const flo
   out[4] = os4 * lsqv[o];
 }
 
-// reverses of the strain-rate stencils of k_sgs_features (scatter with atomics)
-__device__ __forceinline__ void sgs_e12_vjp(const Dims& d, float* lu, float* lv, int F, int G, int K, float g) {
-  unsafeAtomicAdd(lu + iu(d, F, G, K), 0.5f * g * d.idy);
-  unsafeAtomicAdd(lu + iu(d, F, G - 1, K), -0.5f * g * d.idy);
-  unsafeAtomicAdd(lv + iv(d, F, G, K), 0.5f * g * d.idx);
-  unsafeAtomicAdd(lv + iv(d, F - 1, G, K), -0.5f * g * d.idx);
-}
-__device__ __forceinline__ void sgs_e13_vjp(const Dims& d, float* lu, float* lw, int F, int J, int l, float g) {
-  unsafeAtomicAdd(lu + iu(d, F, J, l + 1), 0.5f * g * d.idz);
-  unsafeAtomicAdd(lu + iu(d, F, J, l), -0.5f * g * d.idz);
-  unsafeAtomicAdd(lw + iw(d, F, J, l + 1), 0.5f * g * d.idx);
-  unsafeAtomicAdd(lw + iw(d, F - 1, J, l + 1), -0.5f * g * d.idx);
-}
-__device__ __forceinline__ void sgs_e23_vjp(const Dims& d, float* lv, float* lw, int I, int G, int l, float g) {
-  unsafeAtomicAdd(lv + iv(d, I, G, l + 1), 0.5f * g * d.idz);
-  unsafeAtomicAdd(lv + iv(d, I, G, l), -0.5f * g * d.idz);
-  unsafeAtomicAdd(lw + iw(d, I, G, l + 1), 0.5f * g * d.idy);
-  unsafeAtomicAdd(lw + iw(d, I, G - 1, l + 1), -0.5f * g * d.idy);
+// Reverse of k_sgs_features in gather form (no atomics, bit-reproducible): one thread per
+// padded u, v or w point adds, in a fixed order, every term of the cells whose features
+// read that point.  The features of cell (i, j, k) read: u at its faces I, I+1 (the centre
+// mean and du/dx), v and w likewise; e12 as the mean of its four (x, y) corners, each a
+// difference of u across y and of v across x; e13 (e23) as the mean of its four x-z (y-z)
+// edges at its lower and upper w levels, or, in the surface and top cells, the
+// 1.875 / -1.25 / 0.375 extrapolation of the edges at levels 1..3 (nz-1..nz-3).
+struct SgsFeatAdj {
+  Dims d;
+  const float* lf;
+  FeatNorm nrm;
+  // scaled adjoint of feature c of interior cell (i, j, k), zero outside the interior
+  __device__ __forceinline__ float g(int c, int i, int j, int k) const {
+    const bool ok = (unsigned)i < (unsigned)d.nx && (unsigned)j < (unsigned)d.ny && (unsigned)k < (unsigned)d.nz;
+    return ok ? lf[(((long)i * d.ny + j) * d.nz + k) * 11 + c] * nrm.scale[c] : 0.f;
+  }
+  // weight of e12 corner (F, G) (padded) at cell level k: 1/4 of the e12 adjoint of each
+  // of the four cells around it
+  __device__ __forceinline__ float h12(int F, int G, int k) const {
+    return 0.25f * g(8, F - 4, G - 4, k) + 0.25f * g(8, F - 3, G - 4, k) + 0.25f * g(8, F - 4, G - 3, k) +
+           0.25f * g(8, F - 3, G - 3, k);
+  }
+  // weight of the e13 (c = 9) or e23 (c = 10) edge of cell (i, j) at padded w level l + 1
+  __device__ __forceinline__ float hedge(int c, int i, int j, int l) const {
+    const int nz = d.nz;
+    float h = 0.f;
+    if (l - 1 >= 1 && l - 1 <= nz - 2) h += 0.25f * g(c, i, j, l - 1);   // interior cell k = l-1 (its upper edge)
+    if (l >= 1 && l <= nz - 2) h += 0.25f * g(c, i, j, l);               // interior cell k = l (its lower edge)
+    if (l >= 1 && l <= 3) h += 0.5f * (l == 1 ? 1.875f : (l == 2 ? -1.25f : 0.375f)) * g(c, i, j, 0);
+    if (nz - 1 > 0 && l >= nz - 3 && l <= nz - 1)
+      h += 0.5f * (l == nz - 1 ? 1.875f : (l == nz - 2 ? -1.25f : 0.375f)) * g(c, i, j, nz - 1);
+    return h;
+  }
+  // e13 edge (F, J, l): cells i = F-4, F-3 of row j; e23 edge (I, G, l): rows j = G-4, G-3
+  __device__ __forceinline__ float h13(int F, int j, int l) const { return hedge(9, F - 4, j, l) + hedge(9, F - 3, j, l); }
+  __device__ __forceinline__ float h23(int i, int G, int l) const { return hedge(10, i, G - 4, l) + hedge(10, i, G - 3, l); }
+};
+
+// lu at padded (F, G, L), and lth, lqv of cell (F-3, G-3, L-1) when it is interior
+__global__ void k_sgs_features_vjp_u(SgsFeatAdj A, float* __restrict__ lu, float* __restrict__ lth,
+                                     float* __restrict__ lqv) {
+  const Dims& d = A.d;
+  const int L = blockIdx.x * blockDim.x + threadIdx.x;
+  const int G = blockIdx.y * blockDim.y + threadIdx.y;
+  const int F = blockIdx.z;
+  if (L >= d.nz + 2 || G >= d.ny + 6) return;
+  const int i = F - 3, j = G - 3, k = L - 1;
+  float a = 0.5f * A.g(0, i, j, k) - A.g(5, i, j, k) * d.idx;           // face I of cell i
+  a += 0.5f * A.g(0, i - 1, j, k) + A.g(5, i - 1, j, k) * d.idx;        // face I+1 of cell i-1
+  a += 0.5f * d.idy * (A.h12(F, G, k) - A.h12(F, G + 1, k));
+  a += 0.5f * d.idz * (A.h13(F, j, L - 1) - A.h13(F, j, L));
+  lu[iu(d, F, G, L)] += a;
+  if ((unsigned)i < (unsigned)d.nx && (unsigned)j < (unsigned)d.ny && (unsigned)k < (unsigned)d.nz) {
+    lth[ic(d, F, G, L)] += A.g(3, i, j, k);
+    lqv[ic(d, F, G, L)] += A.g(4, i, j, k);
+  }
 }
 
-// k_sgs_features_vjp: adjoint of the normalised features lf (nx, ny, nz, 11) -> padded
-// adjoints of u, v, w, theta, qv (accumulated, ghost lines included: fold afterwards).
-__global__ void k_sgs_features_vjp(Dims d, const float* __restrict__ lf, FeatNorm nrm, float* __restrict__ lu,
-                                   float* __restrict__ lv, float* __restrict__ lw, float* __restrict__ lth,
-                                   float* __restrict__ lqv) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = blockIdx.z;
-  if (k >= d.nz || j >= d.ny) return;
-  const int I = i + 3, J = j + 3, K = k + 1, nz = d.nz;
-  const long o = (((long)i * d.ny + j) * d.nz + k) * 11;
-  float g[11];
-#pragma unroll
-  for (int c = 0; c < 11; ++c) g[c] = lf[o + c] * nrm.scale[c];
-  unsafeAtomicAdd(lu + iu(d, I, J, K), 0.5f * g[0] - g[5] * d.idx);
-  unsafeAtomicAdd(lu + iu(d, I + 1, J, K), 0.5f * g[0] + g[5] * d.idx);
-  unsafeAtomicAdd(lv + iv(d, I, J, K), 0.5f * g[1] - g[6] * d.idy);
-  unsafeAtomicAdd(lv + iv(d, I, J + 1, K), 0.5f * g[1] + g[6] * d.idy);
-  unsafeAtomicAdd(lw + iw(d, I, J, K), 0.5f * g[2] - g[7] * d.idz);
-  unsafeAtomicAdd(lw + iw(d, I, J, K + 1), 0.5f * g[2] + g[7] * d.idz);
-  unsafeAtomicAdd(lth + ic(d, I, J, K), g[3]);
-  unsafeAtomicAdd(lqv + ic(d, I, J, K), g[4]);
-  const float h12 = 0.25f * g[8];
-  sgs_e12_vjp(d, lu, lv, I, J, K, h12);
-  sgs_e12_vjp(d, lu, lv, I + 1, J + 1, K, h12);
-  sgs_e12_vjp(d, lu, lv, I, J + 1, K, h12);
-  sgs_e12_vjp(d, lu, lv, I + 1, J, K, h12);
-  if (k == 0 || k == nz - 1) {
-    const int l0 = (k == 0) ? 1 : nz - 1, dl = (k == 0) ? 1 : -1;
-    const float cf[3] = {1.875f, -1.25f, 0.375f};
-    for (int t = 0; t < 3; ++t) {
-      const int l = l0 + t * dl;
-      sgs_e13_vjp(d, lu, lw, I + 1, J, l, 0.5f * cf[t] * g[9]);
-      sgs_e13_vjp(d, lu, lw, I, J, l, 0.5f * cf[t] * g[9]);
-      sgs_e23_vjp(d, lv, lw, I, J + 1, l, 0.5f * cf[t] * g[10]);
-      sgs_e23_vjp(d, lv, lw, I, J, l, 0.5f * cf[t] * g[10]);
-    }
-  } else {
-    const float a = 0.25f * g[9], b = 0.25f * g[10];
-    sgs_e13_vjp(d, lu, lw, I, J, k, a);
-    sgs_e13_vjp(d, lu, lw, I + 1, J, k + 1, a);
-    sgs_e13_vjp(d, lu, lw, I, J, k + 1, a);
-    sgs_e13_vjp(d, lu, lw, I + 1, J, k, a);
-    sgs_e23_vjp(d, lv, lw, I, J, k, b);
-    sgs_e23_vjp(d, lv, lw, I, J + 1, k + 1, b);
-    sgs_e23_vjp(d, lv, lw, I, J, k + 1, b);
-    sgs_e23_vjp(d, lv, lw, I, J + 1, k, b);
-  }
+__global__ void k_sgs_features_vjp_v(SgsFeatAdj A, float* __restrict__ lv) {
+  const Dims& d = A.d;
+  const int L = blockIdx.x * blockDim.x + threadIdx.x;
+  const int G = blockIdx.y * blockDim.y + threadIdx.y;
+  const int F = blockIdx.z;
+  if (L >= d.nz + 2 || G >= d.ny + 7) return;
+  const int i = F - 3, j = G - 3, k = L - 1;
+  float a = 0.5f * A.g(1, i, j, k) - A.g(6, i, j, k) * d.idy;           // face J of cell j
+  a += 0.5f * A.g(1, i, j - 1, k) + A.g(6, i, j - 1, k) * d.idy;        // face J+1 of cell j-1
+  a += 0.5f * d.idx * (A.h12(F, G, k) - A.h12(F + 1, G, k));
+  a += 0.5f * d.idz * (A.h23(i, G, L - 1) - A.h23(i, G, L));
+  lv[iv(d, F, G, L)] += a;
+}
+
+__global__ void k_sgs_features_vjp_w(SgsFeatAdj A, float* __restrict__ lw) {
+  const Dims& d = A.d;
+  const int L = blockIdx.x * blockDim.x + threadIdx.x;
+  const int G = blockIdx.y * blockDim.y + threadIdx.y;
+  const int F = blockIdx.z;
+  if (L >= d.nz + 3 || G >= d.ny + 6) return;
+  const int i = F - 3, j = G - 3;
+  float a = 0.5f * A.g(2, i, j, L - 1) - A.g(7, i, j, L - 1) * d.idz;   // lower level of cell L-1
+  a += 0.5f * A.g(2, i, j, L - 2) + A.g(7, i, j, L - 2) * d.idz;        // upper level of cell L-2
+  a += 0.5f * d.idx * (A.h13(F, j, L - 1) - A.h13(F + 1, j, L - 1));
+  a += 0.5f * d.idy * (A.h23(i, G, L - 1) - A.h23(i, G + 1, L - 1));
+  lw[iw(d, F, G, L)] += a;
 }
 
 // Coarsening of a padded fine field by (fx, fy, fz) into an interior coarse field.

@@ -105,6 +105,35 @@ def test_sgs_features_vjp_dot():
     assert _rel(rhs, lhs) < 2e-4, (lhs, rhs)
 
 
+def test_network_and_feature_gradients_are_bit_reproducible():
+    """The learned-SGS gradients use no atomics: the weight and bias gradients of a
+    convolution are per-workgroup partial sums added in workgroup order, and the feature
+    reverse gathers per u, v, w point.  Two calls on the same inputs agree bit for bit
+    (on a grid large enough that float atomics would reorder their additions)."""
+    import torch
+    from lex_amd import _lib, adjoint as adj, sgs_net as sn
+    nx, ny, nz = 48, 40, 32
+    gen = torch.Generator(device="cuda").manual_seed(17)
+    x = torch.randn(nx * ny * nz, 32, device="cuda", generator=gen).to(torch.bfloat16)
+    gp = torch.randn(nx * ny * nz, 32, device="cuda", generator=gen).to(torch.bfloat16)
+    r1 = sn.conv3d_bwd_weight(x, gp, (nx, ny, nz), 32, 32)
+    r2 = sn.conv3d_bwd_weight(x, gp, (nx, ny, nz), 32, 32)
+    assert torch.equal(r1[0], r2[0]) and torch.equal(r1[1], r2[1])
+    assert float(r1[0].abs().max()) > 0 and float(r1[1].abs().max()) > 0
+    g = _lib.grid(nx, ny, nz, 200.0, 250.0, 150.0)
+    scale = (ctypes.c_float * 11)(*[0.5 + 0.1 * c for c in range(11)])
+    lf = torch.randn(nx, ny, nz, 11, device="cuda", generator=gen)
+    shapes = [(nx + 6, ny + 6, nz + 2), (nx + 7, ny + 6, nz + 2), (nx + 6, ny + 7, nz + 2), (nx + 6, ny + 6, nz + 3),
+              (nx + 6, ny + 6, nz + 2), (nx + 6, ny + 6, nz + 2)]
+    runs = []
+    for _ in range(2):
+        lam = tuple(torch.zeros(sh, dtype=torch.float32, device="cuda") for sh in shapes)
+        runs.append(adj.sgs_features_vjp(g, lf, scale, lam))
+    for q in (0, 1, 2, 3, 5):
+        assert torch.equal(runs[0][q], runs[1][q]), q
+        assert float(runs[0][q].abs().max()) > 0, q
+
+
 def test_ext_sgs_step_vjp_vs_fd():
     from lex_amd import adjoint as adj
     from lex_amd._core import grid_from
