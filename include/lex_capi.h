@@ -362,7 +362,9 @@ int lex_conv3d_bf16_fwd_circular(int nx, int ny, int nz, int cin, int cout, cons
 int lex_conv3d_bf16_bwd_data(int nx, int ny, int nz, int cin, int cout, const void* g_pre, const void* wp_flip,
                              const float* zero_bias, float* dx, void* stream);
 /* dw[27][cin_p][cout_p] += sum_p x[p + o] g_pre[p]; db[cout] += sum_p g_pre[p] (db may
- * be NULL).  x: BF16, padded cin; g_pre: BF16, padded cout. */
+ * be NULL).  x: BF16, padded cin; g_pre: BF16, padded cout.  Bit-reproducible: per-
+ * workgroup partial sums (stream-ordered scratch from the stream's pool, returned on the
+ * stream; -4 when the pool has no room) are added in workgroup order. */
 int lex_conv3d_bf16_bwd_weight(int nx, int ny, int nz, int cin, int cout, const void* x, const void* g_pre, float* dw,
                                float* db, void* stream);
 /* AutoEncoder (dl_models.py:33-68) layers in FP32, channels last: one correlation over
@@ -424,7 +426,8 @@ int lex_rk3_step_ext_sgs(const lex_grid_t* g, const lex_step_opts_t* o, const le
  * (lex_dl_train.py:111-131; the sprint is one_sprint_functions.py:118-165) for the
  * build's in-stage SGS net.  Reverses of lex_sgs_apply (ly written) and of
  * lex_sgs_features (padded adjoints of u, v, w, theta, qv accumulated, ghost lines
- * included: fold with lex_fold afterwards; scale as passed to lex_sgs_features). */
+ * included: fold with lex_fold afterwards; scale as passed to lex_sgs_features; gathered
+ * per point, no atomics). */
 int lex_sgs_apply_vjp(const lex_grid_t* g, const float* lsu, const float* lsv, const float* lsw, const float* lsth,
                       const float* lsqv, const float* out_scale, float* ly, void* stream);
 int lex_sgs_features_vjp(const lex_grid_t* g, const float* lfeat, const float* scale, float* lu, float* lv, float* lw,
