@@ -1,11 +1,12 @@
 // Hand-written adjoint (vector-Jacobian products) of the compressible WS-RK3 step.
 //
 // Every forward kernel of the per-operator path (lex_ops.hip, lex_smag.hip) has a reverse
-// kernel here: one thread per forward output point recomputes the local forward
-// quantities and scatters lambda * d(out)/d(in) into the adjoints of the padded inputs
-// with hardware float atomics (global_atomic_add_f32).  All reverse kernels ACCUMULATE
-// into their adjoint outputs.  Ghost images written by the forward stores
-// (padding3*_array) are reversed by k_fold: every ghost's adjoint is added to its
+// kernel here, written without atomics so that every result is bit-reproducible: a
+// thread either owns the adjoint it updates (gather form: it adds, in a fixed order, the
+// terms of every forward point that read its input) or hands the contributions other
+// threads share to slot planes that a gather kernel adds in a fixed order.  All reverse
+// kernels ACCUMULATE into their adjoint outputs.  Ghost images written by the forward
+// stores (padding3*_array) are reversed by k_fold: every ghost's adjoint is added to its
 // interior source and the ghost cleared.
 //
 // Differentiation conventions are JAX's on the reference code: lax.select passes the
@@ -15,8 +16,6 @@
 #include "lex_common.hpp"
 
 namespace lex {
-
-__device__ __forceinline__ void aadd(float* p, float v) { unsafeAtomicAdd(p, v); }
 
 // Slot planes: the deterministic form of a scatter.  A reverse kernel whose threads share
 // targets writes each shared contribution into the plane of its slot (slot = which of
@@ -834,141 +833,8 @@ __global__ void k_b8w_vjp(Dims d, const float* __restrict__ th0c, const float* _
 }
 
 // ------------------------------------------------------------- S1 acoustic substep
-// Reverse of k_acoustic in two passes.  Pass 1 (per cell) takes the folded adjoint of
-// the new pi' through the pressure tendency (one_step_integration.py:543-546) and
-// scatters it to the six new face velocities (lU, lV, lW, interior face arrays), to pi*
-// and to pi'.
-__global__ void k_acoustic_vjp_cells(Dims d, const float* __restrict__ rc, const float* __restrict__ pi0c,
-                                     const float* __restrict__ u, const float* __restrict__ v,
-                                     const float* __restrict__ w, const float* __restrict__ p,
-                                     const float* __restrict__ ps, const float* __restrict__ ru,
-                                     const float* __restrict__ rv, const float* __restrict__ rw,
-                                     const float* __restrict__ tr, const float* __restrict__ lpo, float dts,
-                                     float* __restrict__ lU, float* __restrict__ lV, float* __restrict__ lW,
-                                     float* __restrict__ lp, float* __restrict__ lps) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int i = blockIdx.z;
-  if (k >= d.nz || j >= d.ny) return;
-  int I = i + 3, J = j + 3, K = k + 1;
-  long c = ic(d, I, J, K);
-  float Lp = lpo[c];
-  if (Lp == 0.f) return;
-  const long sx = d.sxc(), sy = d.syc();
-  float un0 = u[iu(d, I, J, K)] + (pgf_x(d, p, tr, I, J, K) + ru[((long)i * d.ny + j) * d.nz + k]) * dts;
-  float un1 = u[iu(d, I + 1, J, K)] + (pgf_x(d, p, tr, I + 1, J, K) + ru[((long)(i + 1) * d.ny + j) * d.nz + k]) * dts;
-  float vn0 = v[iv(d, I, J, K)] + (pgf_y(d, p, tr, I, J, K) + rv[((long)i * (d.ny + 1) + j) * d.nz + k]) * dts;
-  float vn1 = v[iv(d, I, J + 1, K)] + (pgf_y(d, p, tr, I, J + 1, K) + rv[((long)i * (d.ny + 1) + j + 1) * d.nz + k]) * dts;
-  float wn0 = (k == 0) ? 0.0f : w[iw(d, I, J, K)] + (pgf_z(d, p, tr, I, J, K) + rw[((long)i * d.ny + j) * (d.nz + 1) + k]) * dts;
-  float wn1 = (k == d.nz - 1)
-                  ? 0.0f
-                  : w[iw(d, I, J, K + 1)] + (pgf_z(d, p, tr, I, J, K + 1) + rw[((long)i * d.ny + j) * (d.nz + 1) + k + 1]) * dts;
-  float div = (un1 - un0) * d.idx + (vn1 - vn0) * d.idy + (wn1 - wn0) * d.idz;
-  float pc = ps[c];
-  float piadv = pi0c[K] + pc;
-  const float cr = (float)(-kRd / kCv);
-  aadd(lp + c, Lp);
-  float G = 1.1f * dts * Lp;  // adjoint of the pressure tendency
-  float ldiv = cr * piadv * G;
-  float lpc = cr * div * G;
-  // advp = -(0.5 (udx1 + udx0) + 0.5 (vdy1 + vdy0) + (0.5 wdz0 r8w0 + 0.5 wdz1 r8w1) / rho0)
-  float gx = -0.5f * G * d.idx, gy = -0.5f * G * d.idy;
-  float lun0 = -ldiv * d.idx + gx * (pc - ps[c - sx]);
-  float lun1 = ldiv * d.idx + gx * (ps[c + sx] - pc);
-  lpc += gx * un0 - gx * un1;
-  aadd(lps + c - sx, -gx * un0);
-  aadd(lps + c + sx, gx * un1);
-  float lvn0 = -ldiv * d.idy + gy * (pc - ps[c - sy]);
-  float lvn1 = ldiv * d.idy + gy * (ps[c + sy] - pc);
-  lpc += gy * vn0 - gy * vn1;
-  aadd(lps + c - sy, -gy * vn0);
-  aadd(lps + c + sy, gy * vn1);
-  float lwn0 = 0.f, lwn1 = 0.f;
-  if (k > 0) {
-    float r8w0 = 0.5f * (rc[K - 1] + rc[K]);
-    float gz = -0.5f * r8w0 / rc[K] * G;
-    float dz0 = (pi0c[K] - pi0c[K - 1]) * d.idz + (pc - ps[c - 1]) * d.idz;
-    lwn0 = -ldiv * d.idz + gz * dz0;
-    lpc += gz * wn0 * d.idz;
-    aadd(lps + c - 1, -gz * wn0 * d.idz);
-  }
-  if (k < d.nz - 1) {
-    float r8w1 = 0.5f * (rc[K] + rc[K + 1]);
-    float gz = -0.5f * r8w1 / rc[K] * G;
-    float dz1 = (pi0c[K + 1] - pi0c[K]) * d.idz + (ps[c + 1] - pc) * d.idz;
-    lwn1 = ldiv * d.idz + gz * dz1;
-    lpc -= gz * wn1 * d.idz;
-    aadd(lps + c + 1, gz * wn1 * d.idz);
-  }
-  aadd(lps + c, lpc);
-  aadd(lU + ((long)i * d.ny + j) * d.nz + k, lun0);
-  aadd(lU + ((long)(i + 1) * d.ny + j) * d.nz + k, lun1);
-  aadd(lV + ((long)i * (d.ny + 1) + j) * d.nz + k, lvn0);
-  aadd(lV + ((long)i * (d.ny + 1) + j + 1) * d.nz + k, lvn1);
-  if (k > 0) aadd(lW + ((long)i * d.ny + j) * (d.nz + 1) + k, lwn0);
-  if (k < d.nz - 1) aadd(lW + ((long)i * d.ny + j) * (d.nz + 1) + k + 1, lwn1);
-}
-
-// pgf = -cp/2 (tr[c] + tr[c-sd]) (p[c] - p[c-sd]) id, weight g
-__device__ __forceinline__ void pgf_vjp(float* __restrict__ lp, float* __restrict__ ltr, const float* __restrict__ p,
-                                        const float* __restrict__ tr, long c, long sd, float id, float g) {
-  const float h = (float)(-kCp * 0.5);
-  float dp = (p[c] - p[c - sd]) * id;
-  float ts = tr[c] + tr[c - sd];
-  aadd(ltr + c, g * h * dp);
-  aadd(ltr + c - sd, g * h * dp);
-  aadd(lp + c, g * h * ts * id);
-  aadd(lp + c - sd, -g * h * ts * id);
-}
-
-// Pass 2 (per face): the total adjoint of each new face velocity (its folded output
-// adjoint + pass 1's contribution) goes to the old velocity, the rhs and the PGF inputs.
-// Levels 0 and nz of w are constant zero after set_w_bc and take no adjoint.
-__global__ void k_acoustic_vjp_faces(Dims d, const float* __restrict__ p, const float* __restrict__ tr,
-                                     const float* __restrict__ luo, const float* __restrict__ lvo,
-                                     const float* __restrict__ lwo, const float* __restrict__ lU,
-                                     const float* __restrict__ lV, const float* __restrict__ lW, float dts,
-                                     float* __restrict__ lu, float* __restrict__ lv, float* __restrict__ lw,
-                                     float* __restrict__ lp, float* __restrict__ ltr, float* __restrict__ lru,
-                                     float* __restrict__ lrv, float* __restrict__ lrw) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nz
-  int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny
-  int i = blockIdx.z;                             // 0..nx
-  if (k > d.nz || j > d.ny) return;
-  int I = i + 3, J = j + 3, K = k + 1;
-  if (k < d.nz && j < d.ny) {
-    long o = ((long)i * d.ny + j) * d.nz + k;
-    long a = iu(d, I, J, K);
-    float g = lU[o] + luo[a];
-    if (g != 0.f) {
-      aadd(lu + a, g);
-      lru[o] += g * dts;
-      pgf_vjp(lp, ltr, p, tr, ic(d, I, J, K), d.sxc(), d.idx, g * dts);
-    }
-  }
-  if (k < d.nz && i < d.nx) {
-    long o = ((long)i * (d.ny + 1) + j) * d.nz + k;
-    long a = iv(d, I, J, K);
-    float g = lV[o] + lvo[a];
-    if (g != 0.f) {
-      aadd(lv + a, g);
-      lrv[o] += g * dts;
-      pgf_vjp(lp, ltr, p, tr, ic(d, I, J, K), d.syc(), d.idy, g * dts);
-    }
-  }
-  if (i < d.nx && j < d.ny && k > 0 && k < d.nz) {
-    long o = ((long)i * d.ny + j) * (d.nz + 1) + k;
-    long a = iw(d, I, J, K);
-    float g = lW[o] + lwo[a];
-    if (g != 0.f) {
-      aadd(lw + a, g);
-      lrw[o] += g * dts;
-      pgf_vjp(lp, ltr, p, tr, ic(d, I, J, K), 1, d.idz, g * dts);
-    }
-  }
-}
-
-// Atomic-free reverse used by the step driver, in two passes.
+// Reverse of k_acoustic in two passes, without atomics (the step driver and
+// lex_acoustic_substep_vjp).
 // k_acoustic_vjp_faces3: per face, the total adjoint g of the new velocity = its folded
 // output adjoint + the terms of the (one or two) cells whose pressure tendency reads it;
 // then old velocity (+= when acc, else =), rhs (+= g dts, one owner per face), and

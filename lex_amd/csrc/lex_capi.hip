@@ -794,11 +794,13 @@ int lex_acoustic_substep_vjp(const lex_grid_t* g, const float* r, const float* p
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((nU + 255) / 256)), dim3(256), 0, st, (long)nU, lU, 0.0f);
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((nV + 255) / 256)), dim3(256), 0, st, (long)nV, lV, 0.0f);
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((nW + 255) / 256)), dim3(256), 0, st, (long)nW, lW, 0.0f);
-  hipLaunchKernelGGL(k_acoustic_vjp_cells, grd(d.nz, d.ny, d.nx), blk(), 0, st, d, r, pi0, u, v, w, p, ps, ru, rv, rw,
-                     tr, (const float*)lpo, dts, lU, lV, lW, lp, lps);
-  hipLaunchKernelGGL(k_acoustic_vjp_faces, grd(d.nz + 1, d.ny + 1, d.nx + 1), blk(), 0, st, d, p, tr,
-                     (const float*)luo, (const float*)lvo, (const float*)lwo, (const float*)lU, (const float*)lV,
-                     (const float*)lW, dts, lu, lv, lw, lp, ltr, lru, lrv, lrw);
+  // the step's atomic-free two-pass reverse: faces (velocity adjoints accumulated, rhs,
+  // face weights g dts into lU, lV, lW), then cells (pi', pi*, theta_rho adjoints)
+  hipLaunchKernelGGL(k_acoustic_vjp_faces3, grd(d.nz, d.ny + 1, d.nx + 1), blk(), 0, st, d, r, pi0, ps,
+                     (const float*)lpo, (const float*)luo, (const float*)lvo, (const float*)lwo, dts, 1, lu, lv, lw, lru,
+                     lrv, lrw, lU, lV, lW);
+  hipLaunchKernelGGL(k_acoustic_vjp_cells3, grd(d.nz, d.ny + 2, d.nx + 2), blk(), 0, st, d, r, u, v, w, p, ru, rv, rw,
+                     tr, (const float*)lpo, (const float*)lU, (const float*)lV, (const float*)lW, dts, lp, ltr, lps);
   return check_launch("lex_acoustic_substep_vjp");
 }
 
