@@ -277,7 +277,8 @@ int lex_fold(const lex_grid_t* g, int mode, float* adj, void* stream);
  * slot planes (18 for the scalar, 24 per momentum component, each the size of the launch
  * box) that a gather adds in a fixed order, so their results are bit-reproducible.  The
  * planes are taken from the stream's pool (hipMallocAsync) and returned on the stream;
- * -4 when the pool has no room. */
+ * -4 when the pool has no room.  The library sets the device's default pool to keep
+ * freed memory (release threshold UINT64_MAX) so repeated calls do not map it afresh. */
 int lex_adv_scalar_vjp(const lex_grid_t* g, const float* rho0, const float* s, const float* u, const float* v,
                        const float* w, const float* lam, float weps, float* ls, float* lu, float* lv, float* lw,
                        void* stream);

@@ -684,7 +684,21 @@ struct SlotScratch {
   hipStream_t st = nullptr;
   hipError_t take(size_t bytes, hipStream_t s) {
     st = s;
+    keep_pool_memory();
     return hipMallocAsync((void**)&p, bytes, s);
+  }
+  // The default pool returns freed memory to the driver at every synchronisation
+  // (release threshold 0), so each call would map its scratch afresh; keep it pooled.
+  static void keep_pool_memory() {
+    static std::once_flag once;
+    std::call_once(once, [] {
+      int dev = 0;
+      hipMemPool_t pool;
+      if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+      }
+    });
   }
   ~SlotScratch() {
     if (p) (void)hipFreeAsync(p, st);
