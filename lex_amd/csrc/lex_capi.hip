@@ -1125,7 +1125,7 @@ void launch_wgrad(int nx, int ny, int nz, const void* x, const void* g, float* d
   hipLaunchKernelGGL((k_conv3_wgrad<CI, CO>), dim3(nb), dim3(256), 0, st, nx, ny, nz, (const __bf16*)x,
                      (const __bf16*)g, part);
   const int n = 27 * CI * CO;
-  hipLaunchKernelGGL(k_partial_sum, dim3((n + 255) / 256), dim3(256), 0, st, (int)nb, n, n, (const float*)part, dw);
+  hipLaunchKernelGGL(k_partial_sum, dim3((n + 63) / 64), dim3(64), 0, st, (int)nb, n, n, (const float*)part, dw);
 }
 }  // namespace
 extern "C" {
@@ -1188,7 +1188,7 @@ int lex_conv3d_bf16_bwd_weight(int nx, int ny, int nz, int cin, int cout, const 
     long ncell = (long)nx * ny * nz;
     hipLaunchKernelGGL(k_conv3_bgrad, dim3(kBgradBlocks), dim3(256), 0, st, ncell, cop, (const __bf16*)g_pre,
                        sc.p + nw);
-    hipLaunchKernelGGL(k_partial_sum, dim3(1), dim3(256), 0, st, (int)kBgradBlocks, cout, cop,
+    hipLaunchKernelGGL(k_partial_sum, dim3(1), dim3(64), 0, st, (int)kBgradBlocks, cout, cop,
                        (const float*)(sc.p + nw), db);
   }
   return check_launch("lex_conv3d_bf16_bwd_weight");

@@ -712,12 +712,22 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, con
   }
 }
 
-// out[e] += sum over b < nb of part[b * stride + e], in order of b (e < n)
+// out[e] += sum over b < nb of part[b * stride + e], in order of b (e < n); the loads are
+// issued 16 at a time ahead of their additions, so a thread waits for one memory round
+// trip per 16 partial sums, not per partial sum
 __global__ void k_partial_sum(int nb, int n, int stride, const float* __restrict__ part, float* __restrict__ out) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   float s = 0.f;
-  for (int b = 0; b < nb; ++b) s += part[(long)b * stride + e];
+  int b = 0;
+  for (; b + 16 <= nb; b += 16) {
+    float v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = part[(long)(b + q) * stride + e];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s += v[q];
+  }
+  for (; b < nb; ++b) s += part[(long)b * stride + e];
   out[e] += s;
 }
 
