@@ -629,8 +629,13 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, con
   static_assert(CTZ == 16 && XR == 24, "z-run layout assumes a 16-cell z tile");
   constexpr int MT = COUT / 16, NT = CIN / 16, TPW = 7;  // taps per wave: 7, 7, 7, 6
   constexpr int NCELL = CTX * CTY * CTZ;
-  __shared__ __attribute__((aligned(16))) __bf16 xt[CIN * HX * HY * XR];  // [c][hx][hy][z-run]
-  __shared__ __attribute__((aligned(16))) __bf16 gt[COUT * NCELL];        // [n][cell]
+  // channel rows padded to an odd multiple of 16 bytes (modulo 256): the 16 lanes of an
+  // operand read take 16 different channel rows at one offset, which then fall on 16
+  // distinct bank groups instead of 1 (G rows of 512 bytes) or 4 (X rows of 1728)
+  constexpr int XS = HX * HY * XR + 8, GS = NCELL + 8;   // row strides in elements
+  static_assert((XS * 2) % 32 == 16 && (GS * 2) % 32 == 16, "odd multiples of 16 bytes");
+  __shared__ __attribute__((aligned(16))) __bf16 xt[CIN * XS];  // [c][hx][hy][z-run]
+  __shared__ __attribute__((aligned(16))) __bf16 gt[COUT * GS];  // [n][cell]
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, m = l & 15, g = l >> 4;
   f32x4 acc[TPW][MT][NT];
 #pragma unroll
@@ -655,9 +660,9 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, con
       } else {
         for (int j = 0; j < 8; ++j) v[j] = (__bf16)0.0f;
       }
-      __bf16* dst = xt + ((8 * cg) * (HX * HY) + hx * HY + hy) * XR + (hz == HZ - 1 ? 0 : 7 + hz);
+      __bf16* dst = xt + (8 * cg) * XS + (hx * HY + hy) * XR + (hz == HZ - 1 ? 0 : 7 + hz);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) dst[j * (HX * HY * XR)] = v[j];
+      for (int j = 0; j < 8; ++j) dst[j * XS] = v[j];
     }
     // G tile: 8 channels of one cell per load, scattered to 8 channel rows
     for (int e = threadIdx.x; e < NCELL * (COUT / 8); e += 256) {
@@ -671,7 +676,7 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, con
         for (int j = 0; j < 8; ++j) v[j] = (__bf16)0.0f;
       }
 #pragma unroll
-      for (int j = 0; j < 8; ++j) gt[(8 * ng + j) * NCELL + cell] = v[j];
+      for (int j = 0; j < 8; ++j) gt[(8 * ng + j) * GS + cell] = v[j];
     }
     __syncthreads();
 #pragma unroll
@@ -684,10 +689,10 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, con
         const int hx = k0 / (CTY * CTZ), hy = (k0 / CTZ) % CTY, hz = k0 % CTZ;
         bf16x8 a[MT], b[NT];
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) a[mt] = *(const bf16x8*)(gt + (mt * 16 + m) * NCELL + k0);
+        for (int mt = 0; mt < MT; ++mt) a[mt] = *(const bf16x8*)(gt + (mt * 16 + m) * GS + k0);
         const int rb = (hx + 1 + dx) * HY + (hy + 1 + dy);
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) b[nt] = xrun_window(xt + ((nt * 16 + m) * (HX * HY) + rb) * XR, hz, dz);
+        for (int nt = 0; nt < NT; ++nt) b[nt] = xrun_window(xt + (nt * 16 + m) * XS + rb * XR, hz, dz);
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
