@@ -270,10 +270,10 @@ __global__ __launch_bounds__(256) void k_conv3_fwd32(int nx, int ny, int nz, con
 // Workgroup b owns the contiguous tile range [b T / G, (b+1) T / G) (z tiles fastest, then
 // y, then x): consecutive tiles of one workgroup share halo cells in its L2.  The weight
 // fragments of every K step stay in registers for the whole launch (NK x 4 VGPRs: 216 for
-// 32 input channels, so one wave per SIMD), the halo of tile t+1 is fetched into registers
-// while tile t's MFMAs and epilogue run, and the K loop reads only LDS.  MODE: bit 0 GELU
-// after the bias, bit 1 FP32 output Yf with cout_real channels (else BF16 Y), bit 2
-// periodic x and y.  Same results as k_conv3_fwd32.
+// 32 input channels, so one wave per SIMD; 108 for 16, which fit two), the halo of tile
+// t+1 is fetched into registers while tile t's MFMAs and epilogue run, and the K loop
+// reads only LDS.  MODE: bit 0 GELU after the bias, bit 1 FP32 output Yf with cout_real
+// channels (else BF16 Y), bit 2 periodic x and y.  Same results as k_conv3_fwd32.
 template <int CIN, int MODE>
 __global__ __launch_bounds__(256, CIN <= 16 ? 2 : 1) void k_conv3_fwd32p(int nx, int ny, int nz, const __bf16* __restrict__ X,
                                                       const __bf16* __restrict__ Wp, const float* __restrict__ bias,
