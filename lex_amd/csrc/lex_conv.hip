@@ -740,7 +740,7 @@ __global__ void k_partial_sum(int nb, int n, int stride, const float* __restrict
 // n < cp (k_partial_sum adds them into db in workgroup order).  A thread reads 8 channels
 // of one cell per 16-byte load; partial sums are reduced across the wave, then across the
 // workgroup's 4 waves in LDS.
-constexpr int kBgradBlocks = 128;
+constexpr int kBgradBlocks = 512;
 
 __global__ __launch_bounds__(256) void k_conv3_bgrad(long ncell, int cp, const __bf16* __restrict__ G,
                                                      float* __restrict__ part) {
