@@ -444,10 +444,15 @@ __global__ void k_smag_scalars(Dims d, const float* __restrict__ rc, const float
 // of plane p-1 and the scalars' SGS diffusion there.  One barrier per plane.  The lanes
 // above / below the surface and lid cells take values from within their wave, so nz mod
 // 64 is not 1 or 2 (the host checks, and otherwise launches the three-kernel form).
+//
+// Workgroups of at most 768 threads held to 80 VGPRs (five spilled, a few dwords of
+// private memory): two workgroups and 24 waves per CU, where 1024-thread workgroups at
+// 87 VGPRs ran one and 16; the pass waits on memory and the extra waves cover it (11 %
+// faster at 256x256x128 despite 12 % more recomputed halo rows).
 // ---------------------------------------------------------------------------------
 inline int smag_fused_rows(const Dims& d) {
   const int nzl = 64 * ((d.nz + 63) / 64);
-  const int r = 1024 / nzl;
+  const int r = 768 / nzl;
   return r > 8 ? 8 : r;
 }
 inline bool smag_fused_ok(const Dims& d) {
@@ -465,7 +470,7 @@ inline dim3 smag_fused_grid(const Dims& d) {
 }
 
 template <bool ANISO>
-__global__ void __launch_bounds__(1024) k_smag_fused(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
+__global__ void __launch_bounds__(768, 6) k_smag_fused(Dims d, const float* __restrict__ rc, const float* __restrict__ u,
                                                       const float* __restrict__ v, const float* __restrict__ w,
                                                       const float* __restrict__ th, const float* __restrict__ qv,
                                                       float csgs2, float prandtl, float csq, SgsScalars S,
