@@ -362,6 +362,15 @@ int lex_conv3d_bf16_fwd_circular(int nx, int ny, int nz, int cin, int cout, cons
  * channels) with the flip-packed weights; zero_bias: padded-cin zeros. */
 int lex_conv3d_bf16_bwd_data(int nx, int ny, int nz, int cin, int cout, const void* g_pre, const void* wp_flip,
                              const float* zero_bias, float* dx, void* stream);
+/* The data gradient of a layer whose input came out of a GELU, and the GELU's backward
+ * (jax.grad through the nn.gelu calls of simpleCNN, dl_models.py:24-28) in one pass:
+ * g_pre_in (BF16, padded cin channels) = BF16(dx * gelu'(pre_in)), dx as
+ * lex_conv3d_bf16_bwd_data computes it and pre_in the FP32 pre-activation of the layer
+ * below (padded cin channels, as lex_conv3d_bf16_fwd writes it).  Bit for bit the same
+ * as lex_conv3d_bf16_bwd_data followed by lex_gelu_bwd.  cout <= 32 and cin 17..32 (-1
+ * otherwise: run the two calls). */
+int lex_conv3d_bf16_bwd_data_gelu(int nx, int ny, int nz, int cin, int cout, const void* g_pre, const void* wp_flip,
+                                  const float* zero_bias, const float* pre_in, void* g_pre_in, void* stream);
 /* dw[27][cin_p][cout_p] += sum_p x[p + o] g_pre[p]; db[cout] += sum_p g_pre[p] (db may
  * be NULL).  x: BF16, padded cin; g_pre: BF16, padded cout.  Bit-reproducible: per-
  * workgroup partial sums (stream-ordered scratch from the stream's pool, returned on the

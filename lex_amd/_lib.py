@@ -149,6 +149,7 @@ _SIGS = {
     "lex_conv3d_bf16_fwd": [_I, _I, _I, _I, _I, _P, _P, _P, _I, _I, _P, _P, _P, _P],
     "lex_conv3d_bf16_fwd_circular": [_I, _I, _I, _I, _I, _P, _P, _P, _I, _I, _P, _P, _P, _P],
     "lex_conv3d_bf16_bwd_data": [_I, _I, _I, _I, _I, _P, _P, _P, _P, _P],
+    "lex_conv3d_bf16_bwd_data_gelu": [_I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P],
     "lex_conv3d_bf16_bwd_weight": [_I, _I, _I, _I, _I, _P, _P, _P, _P, _P],
     "lex_conv3d_f32": [_P, _I, _I, _I, _I, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _I, _I, _I, _I, _P],
     "lex_gelu_bwd": [C.c_long, _I, _I, _I, _P, _I, _P, _I, _P, _P],

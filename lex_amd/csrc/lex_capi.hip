@@ -1085,12 +1085,13 @@ void launch_fwd(int nx, int ny, int nz, const void* x, const void* wp, const flo
   if (CI <= 32 && CO == 32) {   // persistent, weights in registers
     constexpr int CIP = CI <= 32 ? CI : 32;
 #define LEX_FWD32P(MD)                                                                                  \
-  if ((mode & 7) == MD) {                                                                               \
+  if ((mode & 15) == MD) {                                                                               \
     hipLaunchKernelGGL((k_conv3_fwd32p<CIP, MD>), dim3(conv_fwd32p_grid<CIP, MD>(nx, ny, nz)), dim3(256), 0, st, \
                        nx, ny, nz, (const __bf16*)x, (const __bf16*)wp, bias, cout_real, (__bf16*)y, yf, pre); \
     return;                                                                                             \
   }
     LEX_FWD32P(0) LEX_FWD32P(1) LEX_FWD32P(2) LEX_FWD32P(3) LEX_FWD32P(4) LEX_FWD32P(5) LEX_FWD32P(6) LEX_FWD32P(7)
+    LEX_FWD32P(8)
 #undef LEX_FWD32P
   }
   if (CO % 32 == 0) {   // 32 x 32 MFMA tiles: one A fragment per 32 output channels
@@ -1136,6 +1137,9 @@ static int conv_fwd(const char* name, int nx, int ny, int nz, int cin, int cout,
   if ((mode & 2) ? !yf : !y) return fail(-2, "%s: missing output", name);
   if ((mode & 4) && (nx < 2 || ny < 2)) return fail(-1, "%s: periodic padding needs nx, ny >= 2", name);
   const int cip = padc(cin), cop = padc(cout);
+  // the GELU-gradient epilogue (mode 8) exists in the persistent 32-output-channel kernel only
+  if ((mode & 8) && (mode != 8 || cip > 32 || cop != 32 || !pre))
+    return fail(-1, "%s: the fused GELU gradient needs <= 32 input and 17..32 output channels", name);
 #define LEX_FWD(CI, CO) \
   if (cip == CI && cop == CO) { launch_fwd<CI, CO>(nx, ny, nz, x, wp, bias, mode, cout, y, yf, pre, st); return check_launch(name); }
   LEX_FWD(16, 16) LEX_FWD(16, 32) LEX_FWD(16, 64)
@@ -1166,6 +1170,17 @@ int lex_conv3d_bf16_bwd_data(int nx, int ny, int nz, int cin, int cout, const vo
   if (!conv_ok(nx, ny, nz, cin, cout, 64)) return fail(-1, "lex_conv3d_bf16_bwd_data: bad sizes (channels 1..64)%s");
   // transposed convolution: input channels = cout, output channels = cin
   return lex_conv3d_bf16_fwd(nx, ny, nz, cout, cin, g_pre, wp_flip, zero_bias, 0, 1, nullptr, dx, nullptr, stream);
+}
+
+int lex_conv3d_bf16_bwd_data_gelu(int nx, int ny, int nz, int cin, int cout, const void* g_pre, const void* wp_flip,
+                                  const float* zero_bias, const float* pre_in, void* g_pre_in, void* stream) {
+  REQ(g_pre); REQ(wp_flip); REQ(zero_bias); REQ(pre_in); REQ(g_pre_in);
+  if (!conv_ok(nx, ny, nz, cin, cout, 64))
+    return fail(-1, "lex_conv3d_bf16_bwd_data_gelu: bad sizes (channels 1..64)%s");
+  // transposed convolution (input channels = cout, output channels = cin), its epilogue
+  // multiplying by gelu'(pre_in) and writing BF16; pre_in is read, never written
+  return conv_fwd("lex_conv3d_bf16_bwd_data_gelu", nx, ny, nz, cout, cin, g_pre, wp_flip, zero_bias, 8, g_pre_in,
+                  nullptr, const_cast<float*>(pre_in), S(stream));
 }
 
 int lex_conv3d_bf16_bwd_weight(int nx, int ny, int nz, int cin, int cout, const void* x, const void* g_pre, float* dw,

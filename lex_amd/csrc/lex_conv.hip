@@ -273,7 +273,10 @@ __global__ __launch_bounds__(256) void k_conv3_fwd32(int nx, int ny, int nz, con
 // 32 input channels, so one wave per SIMD; 108 for 16, which fit two), the halo of tile
 // t+1 is fetched into registers while tile t's MFMAs and epilogue run, and the K loop
 // reads only LDS.  MODE: bit 0 GELU after the bias, bit 1 FP32 output Yf with cout_real
-// channels (else BF16 Y), bit 2 periodic x and y.  Same results as k_conv3_fwd32.
+// channels (else BF16 Y), bit 2 periodic x and y, bit 3 GELU gradient: `pre` is read (the
+// FP32 pre-activation of the layer below, 32 channels) and Y = BF16((acc + bias) *
+// gelu'(pre)), the data gradient and the k_gelu_bwd pass after it in one (bits 0-2
+// clear).  Same results as k_conv3_fwd32 (and, bit 3, as that pair).
 template <int CIN, int MODE>
 __global__ __launch_bounds__(256, CIN <= 16 ? 2 : 1) void k_conv3_fwd32p(int nx, int ny, int nz, const __bf16* __restrict__ X,
                                                       const __bf16* __restrict__ Wp, const float* __restrict__ bias,
@@ -284,7 +287,9 @@ __global__ __launch_bounds__(256, CIN <= 16 ? 2 : 1) void k_conv3_fwd32p(int nx,
   constexpr int NSTEP = ConvSteps<CIN>::NSTEP, NK = 2 * NSTEP;   // 16-wide K steps
   constexpr int GX = CT2X + 2, GY = CT2Y + 2, GZ = CT2Z + 2, CS = conv_cell_stride(CIN);
   constexpr int NCH = CIN / 8, NTOT = GX * GY * GZ * NCH, NPF = (NTOT + 255) / 256;  // 16-byte chunks
-  constexpr bool GELU = (MODE & 1) != 0, F32 = (MODE & 2) != 0, CIRC = (MODE & 4) != 0;
+  constexpr bool GELU = (MODE & 1) != 0, F32 = (MODE & 2) != 0, CIRC = (MODE & 4) != 0, GGRAD = (MODE & 8) != 0;
+  static_assert(!GGRAD || (MODE & 7) == 0, "the GELU-gradient epilogue writes BF16 without GELU");
+  constexpr bool PF = GGRAD && CIN > 16;   // pre-activations prefetched for the whole tile
   // LDS row (one z run of GZ cells) stride RS elements, padded to 256 bytes modulo 512: an
   // operand read takes two z runs of 16 cells in adjacent rows (lanes 0-15, 16-31), whose
   // 16-byte slots then fall on the two complementary halves of the 512-byte bank row
@@ -365,6 +370,23 @@ __global__ __launch_bounds__(256, CIN <= 16 ? 2 : 1) void k_conv3_fwd32p(int nx,
       if (c < NTOT) *(bf16x8*)(tile + lds_off(j)) = pf[j];
     }
     __syncthreads();
+    // GELU gradient: the tile's pre-activations, loaded ahead of the next tile's halo so
+    // that they arrive during the MFMAs and waiting for them does not wait for the halo
+    // (CIN <= 16, two waves per SIMD: no registers to spare, loaded per row below)
+    float pv[PF ? CT2Y / 2 : 1][2][8];
+    if constexpr (PF) {
+      const int z0 = (t % ntz) * CT2Z, y0 = ((t / ntz) % nty) * CT2Y, x0 = (t / (ntz * nty)) * CT2X;
+      const int xc = x0 + w;
+#pragma unroll
+      for (int mt = 0; mt < CT2Y / 2; ++mt)
+#pragma unroll
+        for (int yq = 0; yq < 2; ++yq)
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int yc = y0 + 2 * mt + yq, zc = z0 + 4 * h + 8 * (j / 4) + (j % 4);
+            pv[mt][yq][j] = (xc < nx && yc < ny && zc < nz) ? pre[(((long)xc * ny + yc) * nz + zc) * COUT + m] : 0.f;
+          }
+    }
     if (t + 1 < t_end) fetch(t + 1);  // in flight during this tile's MFMAs and epilogue
     f32x16 acc[CT2Y / 2];
 #pragma unroll
@@ -405,10 +427,21 @@ __global__ __launch_bounds__(256, CIN <= 16 ? 2 : 1) void k_conv3_fwd32p(int nx,
           if (yc >= ny) continue;  // uniform over the wave
           const long c0 = ((long)xc * ny + yc) * nz + z0 + 4 * h;
           float* pr = pre ? pre + c0 * COUT + m : nullptr;
+          float prow[8];   // GELU gradient without the tile prefetch: this row's 8 values first
+          if constexpr (GGRAD && !PF) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+              prow[j] = (z0 + 4 * h + 8 * (j / 4) + (j % 4) < nz) ? pr[(8 * (j / 4) + (j % 4)) * COUT] : 0.f;
+          }
           auto out = [&](int j) {
             const int zo = 8 * (j / 4) + (j % 4);
             float v = acc[mt][8 * yq + j] + bn;
-            if (pr) pr[zo * COUT] = v;
+            if constexpr (GGRAD) {
+              const float pa = PF ? pv[PF ? mt : 0][yq][j] : prow[j];
+              v = m < cout_real ? v * gelu_t_grad(pa) : 0.f;   // padded channels: +0
+            } else if (pr) {
+              pr[zo * COUT] = v;
+            }
             if (GELU) v = gelu_t(v);
             if (F32) {
               if (m < cout_real) Yf[(c0 + zo) * cout_real + m] = v;

@@ -78,6 +78,24 @@ def conv3d_bwd_data(g_pre_bf, shape, w):
     return dx
 
 
+def fused_gelu_grad_ok(cin, cout):
+    """Shapes the one-pass data gradient + GELU backward covers (17..32 output, <= 32 input channels)."""
+    return _padc(cin) == 32 and cout <= 32
+
+
+def conv3d_bwd_data_gelu(g_pre_bf, shape, w, pre_in):
+    """BF16 (ncell, padded cin) = conv3d_bwd_data(g_pre_bf, shape, w) * gelu'(pre_in), in one
+    pass; pre_in: the FP32 pre-activation (ncell, padded cin) of the layer below."""
+    nx, ny, nz = shape
+    cin, cout = int(w.shape[1]), int(w.shape[2])
+    wpf = _pack(w, 1)
+    zb = torch.zeros(_padc(cin), dtype=torch.float32, device=w.device)
+    out = torch.empty((nx * ny * nz, _padc(cin)), dtype=torch.bfloat16, device=w.device)
+    _lib.call("lex_conv3d_bf16_bwd_data_gelu", nx, ny, nz, cin, cout, _lib.ptr(g_pre_bf), _lib.ptr(wpf), _lib.ptr(zb),
+              _lib.ptr(pre_in.contiguous()), _lib.ptr(out), _lib.stream())
+    return out
+
+
 def conv3d_bwd_weight(x_bf, g_pre_bf, shape, cin, cout):
     """(dW [27, cin, cout], db [cout]) FP32."""
     nx, ny, nz = shape
@@ -139,15 +157,21 @@ class SgsNetFn(torch.autograd.Function):
         g = gy.reshape(-1, gy.shape[-1]).contiguous().float()
         grads = [None] * (2 * nl)
         dx = None
+        gp = None   # BF16 pre-activation gradient of layer li, when the layer above produced it
         for li in reversed(range(nl)):
             w = params[2 * li]
             cin, cout = int(w.shape[1]), int(w.shape[2])
             last = li == nl - 1
-            gp = gelu_bwd(g, cout, None if last else pres[li], _padc(cout), cout, gelu=not last)
+            if gp is None:
+                gp = gelu_bwd(g, cout, None if last else pres[li], _padc(cout), cout, gelu=not last)
             dw, db = conv3d_bwd_weight(acts[li], gp, shape, cin, cout)
             grads[2 * li], grads[2 * li + 1] = dw, db
-            if li > 0 or ctx.need_x:
+            if li > 0 and fused_gelu_grad_ok(cin, cout):
+                # the data gradient and the GELU backward of layer li - 1 in one pass
+                gp = conv3d_bwd_data_gelu(gp, shape, w, pres[li - 1])
+            elif li > 0 or ctx.need_x:
                 g = conv3d_bwd_data(gp, shape, w)
+                gp = None
                 if li == 0:
                     dx = g.view(*shape, cin)
         return (dx, *grads)

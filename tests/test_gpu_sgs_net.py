@@ -81,6 +81,32 @@ def test_conv_backward_kernels(cin, cout):
     assert _rel(db.cpu(), db_ref) < 1e-5, _rel(db.cpu(), db_ref)
 
 
+@pytest.mark.parametrize("cin,cout,shape", [(32, 32, (13, 9, 37)), (32, 5, (10, 6, 20)), (20, 32, (24, 24, 16)),
+                                            (32, 32, (40, 40, 48))], ids=["32-32", "32-5", "20-32", "32-32-big"])
+def test_fused_gelu_gradient_equals_two_pass(cin, cout, shape):
+    """conv3d_bwd_data_gelu (the data gradient with the GELU backward of the layer below in
+    its epilogue) equals conv3d_bwd_data followed by gelu_bwd bit for bit, signed zeros
+    and the zero padded channels (cin 20) included; 40x40x48 gives every workgroup of the
+    persistent grid several tiles."""
+    import torch
+    from lex_amd import sgs_net as sn
+    g = torch.Generator().manual_seed(11 * cin + cout)
+    ncell = int(np.prod(shape))
+    W = torch.randn(27, cin, cout, generator=g) / np.sqrt(27 * cin)
+    gy = torch.randn(*shape, cout, generator=g)
+    pre = torch.full((ncell, sn._padc(cin)), -1.5)   # gelu'(-1.5) < 0: an unmasked 0 * gelu' would be -0
+    pre[:, :cin] = torch.randn(ncell, cin, generator=g) * 2.0
+    gd = sn.to_bf16_channels_last(gy.cuda())
+    pre_d, Wd = pre.cuda(), W.cuda()
+    assert sn.fused_gelu_grad_ok(cin, cout)
+    two = sn.gelu_bwd(sn.conv3d_bwd_data(gd, shape, Wd), cin, pre_d, sn._padc(cin), cin, gelu=True)
+    one = sn.conv3d_bwd_data_gelu(gd, shape, Wd, pre_d)
+    torch.cuda.synchronize()
+    assert one.shape == two.shape == (ncell, sn._padc(cin))
+    assert torch.equal(one.view(torch.int16), two.view(torch.int16))
+    assert float(one.float().abs().max()) > 0
+
+
 def test_net_forward_backward_vs_fp32_autograd():
     import torch
     from lex_amd.sgs_net import SgsNet
