@@ -1121,12 +1121,15 @@ inline unsigned wgrad_blocks(int nx, int ny, int nz) {
   return (unsigned)(nt < 256 ? nt : 256);
 }
 template <int CI, int CO>
-void launch_wgrad(int nx, int ny, int nz, const void* x, const void* g, float* dw, float* part, hipStream_t st) {
+void launch_wgrad(int nx, int ny, int nz, const void* x, const void* g, float* dw, float* part, float* db,
+                  int cout, float* bpart, hipStream_t st) {
   const unsigned nb = wgrad_blocks(nx, ny, nz);
   hipLaunchKernelGGL((k_conv3_wgrad<CI, CO>), dim3(nb), dim3(256), 0, st, nx, ny, nz, (const __bf16*)x,
-                     (const __bf16*)g, part);
+                     (const __bf16*)g, part, db ? bpart : nullptr);
   const int n = 27 * CI * CO;
   hipLaunchKernelGGL(k_partial_sum, dim3((n + 63) / 64), dim3(64), 0, st, (int)nb, n, n, (const float*)part, dw);
+  if (db)
+    hipLaunchKernelGGL(k_partial_sum, dim3(1), dim3(64), 0, st, (int)nb, cout, CO, (const float*)bpart, db);
 }
 }  // namespace
 extern "C" {
@@ -1191,21 +1194,16 @@ int lex_conv3d_bf16_bwd_weight(int nx, int ny, int nz, int cin, int cout, const 
   hipStream_t st = S(stream);
   // per-workgroup partial sums, added in workgroup order (bit-reproducible gradients):
   // stream-ordered scratch, returned on the stream
-  const size_t nw = (size_t)wgrad_blocks(nx, ny, nz) * 27 * cip * cop, nbg = (size_t)kBgradBlocks * cop;
+  // (the bias gradient's partial sums come from the same workgroups, after the weights')
+  const size_t nb = wgrad_blocks(nx, ny, nz), nw = nb * 27 * cip * cop, nbg = nb * cop;
   SlotScratch sc;
   if (sc.take((nw + nbg) * sizeof(float), st) != hipSuccess)
     return fail(-4, "%s: no device memory for the partial sums", "lex_conv3d_bf16_bwd_weight");
-  if (cip == 16 && cop == 16) launch_wgrad<16, 16>(nx, ny, nz, x, g_pre, dw, sc.p, st);
-  else if (cip == 16 && cop == 32) launch_wgrad<16, 32>(nx, ny, nz, x, g_pre, dw, sc.p, st);
-  else if (cip == 32 && cop == 16) launch_wgrad<32, 16>(nx, ny, nz, x, g_pre, dw, sc.p, st);
-  else launch_wgrad<32, 32>(nx, ny, nz, x, g_pre, dw, sc.p, st);
-  if (db) {
-    long ncell = (long)nx * ny * nz;
-    hipLaunchKernelGGL(k_conv3_bgrad, dim3(kBgradBlocks), dim3(256), 0, st, ncell, cop, (const __bf16*)g_pre,
-                       sc.p + nw);
-    hipLaunchKernelGGL(k_partial_sum, dim3(1), dim3(64), 0, st, (int)kBgradBlocks, cout, cop,
-                       (const float*)(sc.p + nw), db);
-  }
+  float* bp = sc.p + nw;
+  if (cip == 16 && cop == 16) launch_wgrad<16, 16>(nx, ny, nz, x, g_pre, dw, sc.p, db, cout, bp, st);
+  else if (cip == 16 && cop == 32) launch_wgrad<16, 32>(nx, ny, nz, x, g_pre, dw, sc.p, db, cout, bp, st);
+  else if (cip == 32 && cop == 16) launch_wgrad<32, 16>(nx, ny, nz, x, g_pre, dw, sc.p, db, cout, bp, st);
+  else launch_wgrad<32, 32>(nx, ny, nz, x, g_pre, dw, sc.p, db, cout, bp, st);
   return check_launch("lex_conv3d_bf16_bwd_weight");
 }
 

@@ -656,12 +656,20 @@ __device__ __forceinline__ bf16x8 xrun_window(const __bf16* __restrict__ row, in
   return __builtin_bit_cast(bf16x8, r);
 }
 
+// The bias gradient (bpart, when not null) rides along: once a G tile is in LDS, the
+// 256 / COUT threads of each channel row add COUT consecutive cells of it, in order, to one
+// running FP32 sum each; at the end the sums of a channel are added in thread order and
+// stored as this workgroup's partial (bpart + blockIdx.x * COUT).
 template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, const __bf16* __restrict__ X,
-                                                     const __bf16* __restrict__ G, float* __restrict__ part) {
+                                                     const __bf16* __restrict__ G, float* __restrict__ part,
+                                                     float* __restrict__ bpart) {
   static_assert(CTZ == 16 && XR == 24, "z-run layout assumes a 16-cell z tile");
   constexpr int MT = COUT / 16, NT = CIN / 16, TPW = 7;  // taps per wave: 7, 7, 7, 6
   constexpr int NCELL = CTX * CTY * CTZ;
+  constexpr int TPC = 256 / COUT;   // bias: threads per channel row, COUT cells each
+  static_assert(NCELL == 256 && NCELL % TPC == 0, "bias sums: one 256-cell tile per 256 threads");
+  float bsum = 0.f;
   // channel rows padded to an odd multiple of 16 bytes (modulo 256): the 16 lanes of an
   // operand read take 16 different channel rows at one offset, which then fall on 16
   // distinct bank groups instead of 1 (G rows of 512 bytes) or 4 (X rows of 1728)
@@ -712,6 +720,15 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, con
       for (int j = 0; j < 8; ++j) gt[(8 * ng + j) * GS + cell] = v[j];
     }
     __syncthreads();
+    if (bpart) {   // uniform
+      const __bf16* row = gt + (threadIdx.x / TPC) * GS + (threadIdx.x % TPC) * COUT;
+#pragma unroll
+      for (int r = 0; r < COUT / 8; ++r) {
+        const bf16x8 v = *(const bf16x8*)(row + 8 * r);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bsum += (float)v[j];
+      }
+    }
 #pragma unroll
     for (int q = 0; q < TPW; ++q) {
       const int tap = w + 4 * q;
@@ -731,6 +748,17 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, con
 #pragma unroll
           for (int nt = 0; nt < NT; ++nt) acc[q][mt][nt] = mfma16(a[mt], b[nt], acc[q][mt][nt]);
       }
+    }
+  }
+  if (bpart) {
+    float* red = reinterpret_cast<float*>(xt);   // the X tile is no longer read
+    __syncthreads();
+    red[threadIdx.x] = bsum;
+    __syncthreads();
+    if (threadIdx.x < COUT) {
+      float s = 0.f;
+      for (int i = 0; i < TPC; ++i) s += red[threadIdx.x * TPC + i];
+      bpart[(long)blockIdx.x * COUT + threadIdx.x] = s;
     }
   }
   // D[row = n][col = c]: lane holds rows 4g+i of the M-tile, column m of the N-tile; the
@@ -767,39 +795,6 @@ __global__ void k_partial_sum(int nb, int n, int stride, const float* __restrict
   }
   for (; b < nb; ++b) s += part[(long)b * stride + e];
   out[e] += s;
-}
-
-// bias gradient: the workgroup's partial sums of G[p][n] over its cells, part[b][n] for
-// n < cp (k_partial_sum adds them into db in workgroup order).  A thread reads 8 channels
-// of one cell per 16-byte load; partial sums are reduced across the wave, then across the
-// workgroup's 4 waves in LDS.
-constexpr int kBgradBlocks = 512;
-
-__global__ __launch_bounds__(256) void k_conv3_bgrad(long ncell, int cp, const __bf16* __restrict__ G,
-                                                     float* __restrict__ part) {
-  __shared__ float red[4][32];
-  const int nch = cp / 8;  // 16-byte chunks per cell (2 or 4; divides 64)
-  const int cg = threadIdx.x % nch;
-  float s[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) s[j] = 0.f;
-  const long step = (long)gridDim.x * (blockDim.x / nch);
-  for (long p = ((long)blockIdx.x * blockDim.x + threadIdx.x) / nch; p < ncell; p += step) {
-    const bf16x8 v = *(const bf16x8*)(G + p * cp + 8 * cg);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) s[j] += (float)v[j];
-  }
-#pragma unroll
-  for (int j = 0; j < 8; ++j)
-    for (int off = 32; off >= nch; off >>= 1) s[j] += __shfl_down(s[j], off, 64);
-  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-  if (l < nch) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) red[wv][8 * l + j] = s[j];
-  }
-  __syncthreads();
-  const int n = threadIdx.x;
-  if (n < cp) part[(long)blockIdx.x * cp + n] = red[0][n] + red[1][n] + red[2][n] + red[3][n];
 }
 
 // g_pre = g_post * gelu'(pre) (or g_post when gelu is 0) -> BF16, cp channels (the
