@@ -1223,14 +1223,29 @@ int lex_conv3d_f32(const float* x, int nix, int niy, int niz, int cin, const flo
   return check_launch("lex_conv3d_f32");
 }
 
+// the 8-channel-chunk forms of the BF16 channels-last writers: whole 16-byte chunks,
+// aligned output, 32-bit chunk indices
+static bool chunk8_ok(long ncell, int cp, const void* out) {
+  return cp % 8 == 0 && ((uintptr_t)out & 15) == 0 && ncell * (cp / 8) < (1L << 31);
+}
+// every row of an FP32 array with this base and row stride starts on 16 bytes
+static bool rows16(const float* base, int stride) { return ((uintptr_t)base & 15) == 0 && stride % 4 == 0; }
+
 int lex_gelu_bwd(long ncell, int cp, int c_real, int cpre, const float* gpost, int gstride, const float* pre, int gelu,
                  void* gpre, void* stream) {
   REQ(gpost); REQ(gpre);
   if (gelu) REQ(pre);
   if (ncell <= 0 || cp < c_real || c_real < 1) return fail(-1, "lex_gelu_bwd: bad sizes%s");
   long n = ncell * cp;
-  hipLaunchKernelGGL(k_gelu_bwd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), ncell, cp, c_real, cpre,
-                     gpost, gstride, pre, gelu, (__bf16*)gpre);
+  if (chunk8_ok(ncell, cp, gpre)) {
+    const unsigned nch = (unsigned)(cp / 8), nchunk = (unsigned)(ncell * nch);
+    const int vec = (rows16(gpost, gstride) ? 1 : 0) | (gelu && rows16(pre, cpre) ? 2 : 0);
+    hipLaunchKernelGGL(k_gelu_bwd8, dim3((nchunk + 255) / 256), dim3(256), 0, S(stream), nchunk, nch, c_real, cpre,
+                       gpost, gstride, pre, gelu, vec, (__bf16*)gpre);
+  } else {
+    hipLaunchKernelGGL(k_gelu_bwd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), ncell, cp, c_real, cpre,
+                       gpost, gstride, pre, gelu, (__bf16*)gpre);
+  }
   return check_launch("lex_gelu_bwd");
 }
 
@@ -1238,8 +1253,14 @@ int lex_to_bf16_channels_last(long ncell, int c_real, int cstride, int cp, const
   REQ(in); REQ(out);
   if (ncell <= 0 || cp < c_real || c_real < 1) return fail(-1, "lex_to_bf16_channels_last: bad sizes%s");
   long n = ncell * cp;
-  hipLaunchKernelGGL(k_to_bf16_cl, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), ncell, c_real, cstride,
-                     cp, in, (__bf16*)out);
+  if (chunk8_ok(ncell, cp, out)) {
+    const unsigned nch = (unsigned)(cp / 8), nchunk = (unsigned)(ncell * nch);
+    hipLaunchKernelGGL(k_to_bf16_cl8, dim3((nchunk + 255) / 256), dim3(256), 0, S(stream), nchunk, nch, c_real,
+                       cstride, rows16(in, cstride) ? 1 : 0, in, (__bf16*)out);
+  } else {
+    hipLaunchKernelGGL(k_to_bf16_cl, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), ncell, c_real,
+                       cstride, cp, in, (__bf16*)out);
+  }
   return check_launch("lex_to_bf16_channels_last");
 }
 

@@ -797,6 +797,43 @@ __global__ void k_partial_sum(int nb, int n, int stride, const float* __restrict
   out[e] += s;
 }
 
+// channels n0 .. n0+7 of one FP32 row (zero from c_real on); vec: the row is 16-byte
+// aligned (two 16-byte loads when all 8 channels are real)
+__device__ __forceinline__ void load_row8(const float* __restrict__ src, int n0, int c_real, bool vec, float (&v)[8]) {
+  if (vec && n0 + 8 <= c_real) {
+    const float4 a = *(const float4*)(src + n0), b = *(const float4*)(src + n0 + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = n0 + j < c_real ? src[n0 + j] : 0.f;
+  }
+}
+
+// k_gelu_bwd with one thread per 8 output channels of a cell (one 16-byte store), as
+// k_to_bf16_cl8: nch = cp / 8, nchunk = ncell * nch < 2^31, gpre 16-byte aligned
+// (vec bit 0: gpost rows 16-byte aligned, bit 1: pre rows)
+__global__ void k_gelu_bwd8(unsigned nchunk, unsigned nch, int c_real, int cpre, const float* __restrict__ gpost,
+                            int gstride, const float* __restrict__ pre, int gelu, int vec, __bf16* __restrict__ gpre) {
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nchunk) return;
+  const unsigned p = t / nch;
+  const int n0 = 8 * (int)(t - p * nch);
+  float g[8], x[8];
+  load_row8(gpost + (long)p * gstride, n0, c_real, (vec & 1) != 0, g);
+  if (gelu) load_row8(pre + (long)p * cpre, n0, c_real, (vec & 2) != 0, x);
+  bf16x8 o;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    float v = 0.f;
+    if (n0 + j < c_real) {
+      v = g[j];
+      if (gelu) v *= gelu_t_grad(x[j]);
+    }
+    o[j] = (__bf16)v;
+  }
+  *(bf16x8*)(gpre + 8L * t) = o;
+}
+
 // g_pre = g_post * gelu'(pre) (or g_post when gelu is 0) -> BF16, cp channels (the
 // channels >= c_real are zero)
 __global__ void k_gelu_bwd(long ncell, int cp, int c_real, int cpre, const float* __restrict__ gpost, int gstride,
@@ -834,6 +871,23 @@ __global__ void k_conv3_pack(const float* __restrict__ W, int cin, int cout, int
     }
   }
   Wp[t] = (__bf16)v;
+}
+
+// The same with one thread per 8 output channels of a cell (one 16-byte store) and 32-bit
+// indexing: nch = cp / 8 chunks per cell, nchunk = ncell * nch < 2^31, out 16-byte aligned;
+// vec: in and cs keep every row 16-byte aligned.
+__global__ void k_to_bf16_cl8(unsigned nchunk, unsigned nch, int c_real, int cs, int vec, const float* __restrict__ in,
+                              __bf16* __restrict__ out) {
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nchunk) return;
+  const unsigned p = t / nch;
+  const int n0 = 8 * (int)(t - p * nch);
+  float x[8];
+  load_row8(in + (long)p * cs, n0, c_real, vec != 0, x);
+  bf16x8 v;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] = (__bf16)x[j];
+  *(bf16x8*)(out + 8L * t) = v;
 }
 
 // FP32 channels-last (c_real channels, stride cs) -> BF16 padded to cp channels

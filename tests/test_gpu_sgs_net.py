@@ -107,6 +107,44 @@ def test_fused_gelu_gradient_equals_two_pass(cin, cout, shape):
     assert float(one.float().abs().max()) > 0
 
 
+@pytest.mark.parametrize("c,cs,cp", [(11, 11, 16), (32, 32, 32), (5, 7, 16), (3, 3, 12), (20, 20, 64)])
+def test_bf16_channels_last_conversion(c, cs, cp):
+    """lex_to_bf16_channels_last equals torch's round-to-nearest-even BF16 conversion bit for
+    bit, with zero padding channels; cp 12 takes the element-wise form, the others the
+    8-channel-chunk form."""
+    import torch
+    from lex_amd import _lib
+    g = torch.Generator().manual_seed(c * 7 + cp)
+    ncell = 1237
+    x = torch.randn(ncell, cs, generator=g) * 3.0
+    out = torch.empty((ncell, cp), dtype=torch.bfloat16, device="cuda")
+    _lib.call("lex_to_bf16_channels_last", ncell, c, cs, cp, _lib.ptr(x.cuda()), _lib.ptr(out), _lib.stream())
+    ref = torch.zeros((ncell, cp), dtype=torch.bfloat16)
+    ref[:, :c] = x[:, :c].bfloat16()
+    assert torch.equal(out.cpu().view(torch.int16), ref.view(torch.int16))
+
+
+@pytest.mark.parametrize("gelu", [0, 1])
+def test_gelu_bwd_chunked_equals_elementwise(gelu):
+    """lex_gelu_bwd's 8-channel-chunk form (cp 16) and its element-wise form (cp 12) give the
+    same bits in the real channels and zeros in the padding."""
+    import torch
+    from lex_amd import _lib
+    g = torch.Generator().manual_seed(5 + gelu)
+    ncell, c = 1237, 11
+    gpost = torch.randn(ncell, 13, generator=g).cuda()
+    pre = (torch.randn(ncell, 16, generator=g) * 2.0).cuda()
+    outs = {}
+    for cp in (16, 12):
+        out = torch.empty((ncell, cp), dtype=torch.bfloat16, device="cuda")
+        _lib.call("lex_gelu_bwd", ncell, cp, c, 16, _lib.ptr(gpost), 13, _lib.ptr(pre) if gelu else None, gelu,
+                  _lib.ptr(out), _lib.stream())
+        outs[cp] = out.cpu()
+    assert torch.equal(outs[16][:, :c].view(torch.int16), outs[12][:, :c].view(torch.int16))
+    assert not bool(outs[16][:, c:].float().abs().max() > 0) and not bool(outs[12][:, c:].float().abs().max() > 0)
+    assert float(outs[16].float().abs().max()) > 0
+
+
 def test_net_forward_backward_vs_fp32_autograd():
     import torch
     from lex_amd.sgs_net import SgsNet
