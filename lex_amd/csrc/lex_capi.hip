@@ -1121,15 +1121,16 @@ inline unsigned wgrad_blocks(int nx, int ny, int nz) {
   return (unsigned)(nt < 256 ? nt : 256);
 }
 template <int CI, int CO>
+// part: wgrad_blocks() x (27 CI CO + CO) floats; one k_partial_sum adds the weight and
+// (db not null) the bias partial sums
 void launch_wgrad(int nx, int ny, int nz, const void* x, const void* g, float* dw, float* part, float* db,
-                  int cout, float* bpart, hipStream_t st) {
+                  int cout, hipStream_t st) {
   const unsigned nb = wgrad_blocks(nx, ny, nz);
   hipLaunchKernelGGL((k_conv3_wgrad<CI, CO>), dim3(nb), dim3(256), 0, st, nx, ny, nz, (const __bf16*)x,
-                     (const __bf16*)g, part, db ? bpart : nullptr);
-  const int n = 27 * CI * CO;
-  hipLaunchKernelGGL(k_partial_sum, dim3((n + 63) / 64), dim3(64), 0, st, (int)nb, n, n, (const float*)part, dw);
-  if (db)
-    hipLaunchKernelGGL(k_partial_sum, dim3(1), dim3(64), 0, st, (int)nb, cout, CO, (const float*)bpart, db);
+                     (const __bf16*)g, part, db ? 1 : 0);
+  const int n = 27 * CI * CO, n2 = db ? cout : 0;
+  hipLaunchKernelGGL(k_partial_sum, dim3((n + n2 + 63) / 64), dim3(64), 0, st, (int)nb, n, n + CO,
+                     (const float*)part, dw, n2, db);
 }
 }  // namespace
 extern "C" {
@@ -1195,15 +1196,14 @@ int lex_conv3d_bf16_bwd_weight(int nx, int ny, int nz, int cin, int cout, const 
   // per-workgroup partial sums, added in workgroup order (bit-reproducible gradients):
   // stream-ordered scratch, returned on the stream
   // (the bias gradient's partial sums come from the same workgroups, after the weights')
-  const size_t nb = wgrad_blocks(nx, ny, nz), nw = nb * 27 * cip * cop, nbg = nb * cop;
+  const size_t nb = wgrad_blocks(nx, ny, nz), npart = nb * (27 * cip * cop + cop);
   SlotScratch sc;
-  if (sc.take((nw + nbg) * sizeof(float), st) != hipSuccess)
+  if (sc.take(npart * sizeof(float), st) != hipSuccess)
     return fail(-4, "%s: no device memory for the partial sums", "lex_conv3d_bf16_bwd_weight");
-  float* bp = sc.p + nw;
-  if (cip == 16 && cop == 16) launch_wgrad<16, 16>(nx, ny, nz, x, g_pre, dw, sc.p, db, cout, bp, st);
-  else if (cip == 16 && cop == 32) launch_wgrad<16, 32>(nx, ny, nz, x, g_pre, dw, sc.p, db, cout, bp, st);
-  else if (cip == 32 && cop == 16) launch_wgrad<32, 16>(nx, ny, nz, x, g_pre, dw, sc.p, db, cout, bp, st);
-  else launch_wgrad<32, 32>(nx, ny, nz, x, g_pre, dw, sc.p, db, cout, bp, st);
+  if (cip == 16 && cop == 16) launch_wgrad<16, 16>(nx, ny, nz, x, g_pre, dw, sc.p, db, cout, st);
+  else if (cip == 16 && cop == 32) launch_wgrad<16, 32>(nx, ny, nz, x, g_pre, dw, sc.p, db, cout, st);
+  else if (cip == 32 && cop == 16) launch_wgrad<32, 16>(nx, ny, nz, x, g_pre, dw, sc.p, db, cout, st);
+  else launch_wgrad<32, 32>(nx, ny, nz, x, g_pre, dw, sc.p, db, cout, st);
   return check_launch("lex_conv3d_bf16_bwd_weight");
 }
 

@@ -656,14 +656,14 @@ __device__ __forceinline__ bf16x8 xrun_window(const __bf16* __restrict__ row, in
   return __builtin_bit_cast(bf16x8, r);
 }
 
-// The bias gradient (bpart, when not null) rides along: once a G tile is in LDS, the
-// 256 / COUT threads of each channel row add COUT consecutive cells of it, in order, to one
-// running FP32 sum each; at the end the sums of a channel are added in thread order and
-// stored as this workgroup's partial (bpart + blockIdx.x * COUT).
+// Workgroup b stores its partial sums at part + b * (27 CIN COUT + COUT): the weights', then
+// (with_bias) the bias gradient's, which rides along: once a G tile is in LDS, the 256 /
+// COUT threads of each channel row add COUT consecutive cells of it, in order, to one
+// running FP32 sum each; at the end the sums of a channel are added in thread order.
 template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, const __bf16* __restrict__ X,
                                                      const __bf16* __restrict__ G, float* __restrict__ part,
-                                                     float* __restrict__ bpart) {
+                                                     int with_bias) {
   static_assert(CTZ == 16 && XR == 24, "z-run layout assumes a 16-cell z tile");
   constexpr int MT = COUT / 16, NT = CIN / 16, TPW = 7;  // taps per wave: 7, 7, 7, 6
   constexpr int NCELL = CTX * CTY * CTZ;
@@ -720,7 +720,7 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, con
       for (int j = 0; j < 8; ++j) gt[(8 * ng + j) * GS + cell] = v[j];
     }
     __syncthreads();
-    if (bpart) {   // uniform
+    if (with_bias) {   // uniform
       const __bf16* row = gt + (threadIdx.x / TPC) * GS + (threadIdx.x % TPC) * COUT;
 #pragma unroll
       for (int r = 0; r < COUT / 8; ++r) {
@@ -750,7 +750,8 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, con
       }
     }
   }
-  if (bpart) {
+  float* dW = part + (long)blockIdx.x * (27 * CIN * COUT + COUT);
+  if (with_bias) {
     float* red = reinterpret_cast<float*>(xt);   // the X tile is no longer read
     __syncthreads();
     red[threadIdx.x] = bsum;
@@ -758,12 +759,11 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, con
     if (threadIdx.x < COUT) {
       float s = 0.f;
       for (int i = 0; i < TPC; ++i) s += red[threadIdx.x * TPC + i];
-      bpart[(long)blockIdx.x * COUT + threadIdx.x] = s;
+      dW[27 * CIN * COUT + threadIdx.x] = s;
     }
   }
   // D[row = n][col = c]: lane holds rows 4g+i of the M-tile, column m of the N-tile; the
   // four waves' taps (w, w+4, ..) cover all 27, so every element of the slot is stored
-  float* dW = part + (long)blockIdx.x * (27 * CIN * COUT);
 #pragma unroll
   for (int q = 0; q < TPW; ++q) {
     const int tap = w + 4 * q;
@@ -781,9 +781,10 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(int nx, int ny, int nz, con
 // out[e] += sum over b < nb of part[b * stride + e], in order of b (e < n); the loads are
 // issued 16 at a time ahead of their additions, so a thread waits for one memory round
 // trip per 16 partial sums, not per partial sum
-__global__ void k_partial_sum(int nb, int n, int stride, const float* __restrict__ part, float* __restrict__ out) {
+__global__ void k_partial_sum(int nb, int n, int stride, const float* __restrict__ part, float* __restrict__ out,
+                              int n2, float* __restrict__ out2) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
+  if (e >= n + n2) return;
   float s = 0.f;
   int b = 0;
   for (; b + 16 <= nb; b += 16) {
@@ -794,7 +795,11 @@ __global__ void k_partial_sum(int nb, int n, int stride, const float* __restrict
     for (int q = 0; q < 16; ++q) s += v[q];
   }
   for (; b < nb; ++b) s += part[(long)b * stride + e];
-  out[e] += s;
+  if (e < n) {
+    out[e] += s;
+  } else {
+    out2[e - n] += s;
+  }
 }
 
 // channels n0 .. n0+7 of one FP32 row (zero from c_real on); vec: the row is 16-byte
